@@ -1,0 +1,269 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/libglorc.so).
+
+Test infrastructure only: the oracle is the checker, never the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "libglorc.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle/libglorc.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.orc_matep.argtypes = [C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_point_hessian_literal.argtypes = [_dp, _dp, _dp]
+        L.orc_point_rho_literal.argtypes = [_dp, _dp, _dp]
+        L.orc_point_hessian.argtypes = [_dp, _dp, _dp]
+        L.orc_point_rho.argtypes = [_dp, _dp, _dp]
+        L.orc_point_energy.argtypes = [_dp, _dp, _dp]
+        L.orc_point_energy.restype = C.c_double
+        L.orc_lit_jac_point.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_lit_jac_point.restype = C.c_double
+        L.orc_lit_res_point.argtypes = [_dp, _dp, _dp, _dp, _dp]
+        L.orc_lit_res_point.restype = C.c_double
+        L.orc_refel.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_cell_faithful.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_cell_factorized.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_face_faithful.argtypes = [C.c_int, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_assemble.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_int, _ip, C.c_int, _ip, _ip, _ip, _dp,
+                                   _dp, _dp, C.c_int, _ip, _ip, _dp, _dp]
+        L.orc_free_energy.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_int, _ip, _dp, _dp]
+        L.orc_free_energy.restype = C.c_double
+        L.orc_bsr_spmv.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp]
+        L.orc_block_jacobi.argtypes = [C.c_int, _ip, _ip, _dp, _dp]
+        L.orc_gmres.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int,
+                                C.POINTER(C.c_double)]
+        L.orc_distribute.argtypes = [C.c_int, _ip, _ip, _ip, _dp, _dp]
+        L.orc_mt64.argtypes = [C.c_uint64, C.c_int, _up]
+        L.orc_initial_guess.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_uint64, _dp]
+        L.orc_newton.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_int, _ip, C.c_int, _ip, _ip, _ip, _dp, _dp,
+                                 _dp, _ip, _ip, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int,
+                                 C.c_double, C.c_int, _dp]
+        _lib = L
+    return _lib
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def matep(p, t, scc=True):
+    out = np.zeros(12)
+    rc = lib().orc_matep(float(p), float(t), int(scc), out)
+    keys = ["alpha", "beta1", "beta2", "beta3", "beta4", "beta5", "beta_A", "beta_B", "gap_A", "gap_B",
+            "f_A", "f_B"]
+    d = dict(zip(keys, out.tolist()))
+    d["rc"] = rc
+    return d
+
+
+K_CONST = 0.42072  # femgl/inc/femgl.h:320-322
+
+
+def coef(p, t, scc=True, bt=1.0e10, K=(K_CONST, K_CONST, K_CONST)):
+    m = matep(p, t, scc)
+    return f64([m["alpha"], m["beta1"], m["beta2"], m["beta3"], m["beta4"], m["beta5"], K[0], K[1], K[2], bt])
+
+
+def point_hessian(co, x, literal=False):
+    H = np.zeros(324)
+    (lib().orc_point_hessian_literal if literal else lib().orc_point_hessian)(f64(co), f64(x), H)
+    return H.reshape(18, 18)
+
+
+def point_rho(co, x, literal=False):
+    r = np.zeros(18)
+    (lib().orc_point_rho_literal if literal else lib().orc_point_rho)(f64(co), f64(x), r)
+    return r
+
+
+def point_energy(co, x, gx):
+    return lib().orc_point_energy(f64(co), f64(x), f64(gx))
+
+
+def refel(deg):
+    n = (deg + 1) ** 3
+    N = np.zeros(n * n)
+    dN = np.zeros(n * n * 3)
+    w = np.zeros(n)
+    lib().orc_refel(deg, N, dN, w)
+    return N.reshape(n, n), dN.reshape(n, n, 3), w
+
+
+def cell(deg, h, co, Uloc, faithful=False, face=None):
+    ns = (deg + 1) ** 3
+    dpc = 18 * ns
+    J = np.zeros(dpc * dpc)
+    R = np.zeros(dpc)
+    fn = lib().orc_cell_faithful if faithful else lib().orc_cell_factorized
+    rc = fn(deg, f64(h), f64(co), f64(Uloc), J, R)
+    assert rc == 0
+    if face is not None:
+        for (lf, bid) in face:
+            lib().orc_face_faithful(deg, f64(h), lf, bid, f64(co), f64(Uloc), J, R)
+    return J.reshape(dpc, dpc), R
+
+
+class Mesh:
+    """Plain description consumed by the oracle (and produced by the product's
+    host mesh layer): lexicographic cell->node map, cell sizes, boundary
+    faces (cell, local face, id), dof constraints and a BSR node pattern."""
+
+    def __init__(self, deg, cell_nodes, cell_h, faces, n_nodes, con_of, con_ptr, con_master, con_weight,
+                 row_ptr, col, node_xyz=None):
+        self.deg = deg
+        self.cell_nodes = i32(cell_nodes)
+        self.cell_h = f64(cell_h)
+        self.faces = i32(faces).reshape(-1) if len(faces) else np.zeros(0, np.int32)
+        self.n_faces = len(self.faces) // 3
+        self.n_cells = self.cell_nodes.size // ((deg + 1) ** 3)
+        self.n_nodes = n_nodes
+        self.con_of = i32(con_of)
+        self.con_ptr = i32(con_ptr)
+        self.con_master = i32(con_master) if len(con_master) else np.zeros(1, np.int32)
+        self.con_weight = f64(con_weight) if len(con_weight) else np.zeros(1)
+        self.row_ptr = i32(row_ptr)
+        self.col = i32(col)
+        self.node_xyz = node_xyz
+
+    @property
+    def n_dofs(self):
+        return self.n_nodes * 18
+
+    @property
+    def nnzb(self):
+        return int(self.row_ptr[-1])
+
+
+def box_mesh(deg, n, lo, hi, dirichlet_faces=(4, 5), bids=None):
+    """Independent (test-side) structured box mesh: n=(nx,ny,nz) cells,
+    lexicographic node numbering, homogeneous Dirichlet on all 18 components
+    of the nodes on the listed faces (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z)."""
+    nx, ny, nz = n
+    n1 = deg + 1
+    Nx, Ny, Nz = deg * nx + 1, deg * ny + 1, deg * nz + 1
+    h = [(hi[k] - lo[k]) / n[k] for k in range(3)]
+    cells = []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                cn = []
+                for c in range(n1):
+                    for b in range(n1):
+                        for a in range(n1):
+                            cn.append((deg * i + a) + Nx * ((deg * j + b) + Ny * (deg * k + c)))
+                cells.append(cn)
+    cells = np.array(cells, np.int32)
+    n_nodes = Nx * Ny * Nz
+    idx = np.arange(n_nodes)
+    ix, iy, iz = idx % Nx, (idx // Nx) % Ny, idx // (Nx * Ny)
+    xyz = np.stack([lo[0] + ix * h[0] / deg, lo[1] + iy * h[1] / deg, lo[2] + iz * h[2] / deg], 1)
+    on = [ix == 0, ix == Nx - 1, iy == 0, iy == Ny - 1, iz == 0, iz == Nz - 1]
+    cons = np.zeros(n_nodes, bool)
+    for f in dirichlet_faces:
+        cons |= on[f]
+    con_of = -np.ones(n_nodes * 18, np.int32)
+    cdofs = (np.nonzero(cons)[0][:, None] * 18 + np.arange(18)[None, :]).reshape(-1)
+    con_of[cdofs] = np.arange(cdofs.size, dtype=np.int32)
+    con_ptr = np.zeros(cdofs.size + 1, np.int32)
+    # node pattern: nodes sharing a cell
+    nb = [set() for _ in range(n_nodes)]
+    for cn in cells:
+        for a in cn:
+            nb[a].update(cn.tolist())
+    row_ptr = np.zeros(n_nodes + 1, np.int32)
+    cols = []
+    for r in range(n_nodes):
+        s = sorted(nb[r])
+        cols.extend(s)
+        row_ptr[r + 1] = row_ptr[r] + len(s)
+    faces = []
+    if bids:
+        for e in range(cells.shape[0]):
+            i, j, k = e % nx, (e // nx) % ny, e // (nx * ny)
+            for (lf, cond) in enumerate([i == 0, i == nx - 1, j == 0, j == ny - 1, k == 0, k == nz - 1]):
+                if cond and lf in bids:
+                    faces.append((e, lf, bids[lf]))
+    return Mesh(deg, cells.reshape(-1), np.tile(h, (cells.shape[0], 1)).reshape(-1), faces, n_nodes, con_of,
+                con_ptr, [], [], row_ptr, np.array(cols, np.int32), xyz)
+
+
+def assemble(mesh, co, U, J=True, R=True, faithful=False):
+    val = np.zeros(max(1, mesh.nnzb * 324)) if J else np.zeros(1)
+    rhs = np.zeros(mesh.n_dofs)
+    flags = (1 if J else 0) | (2 if R else 0) | (4 if faithful else 0)
+    rc = lib().orc_assemble(mesh.deg, mesh.n_cells, mesh.cell_nodes, mesh.cell_h, mesh.n_faces,
+                            mesh.faces if mesh.n_faces else np.zeros(3, np.int32), mesh.n_nodes, mesh.con_of,
+                            mesh.con_ptr, mesh.con_master, mesh.con_weight, f64(co), f64(U), flags, mesh.row_ptr,
+                            mesh.col, val, rhs)
+    assert rc == 0, rc
+    return (val.reshape(-1, 18, 18) if J else None), rhs
+
+
+def free_energy(mesh, co, U):
+    return lib().orc_free_energy(mesh.deg, mesh.n_cells, mesh.cell_nodes, mesh.cell_h, mesh.n_faces,
+                                 mesh.faces if mesh.n_faces else np.zeros(3, np.int32), f64(co), f64(U))
+
+
+def bsr_to_dense(mesh, val):
+    n = mesh.n_dofs
+    A = np.zeros((n, n))
+    for r in range(mesh.n_nodes):
+        for b in range(mesh.row_ptr[r], mesh.row_ptr[r + 1]):
+            s = mesh.col[b]
+            A[18 * r:18 * r + 18, 18 * s:18 * s + 18] = val[b]
+    return A
+
+
+def gmres(mesh, val, b, rtol=1e-10, maxit=1000, restart=30):
+    x = np.zeros(mesh.n_dofs)
+    res = C.c_double(0)
+    its = lib().orc_gmres(mesh.n_nodes, mesh.row_ptr, mesh.col, f64(val.reshape(-1)), f64(b), x, rtol, maxit,
+                          restart, C.byref(res))
+    return x, its, res.value
+
+
+def distribute(mesh, x):
+    x = f64(x).copy()
+    lib().orc_distribute(mesh.n_dofs, mesh.con_of, mesh.con_ptr, mesh.con_master, mesh.con_weight, x)
+    return x
+
+
+def mt64(seed, n):
+    out = np.zeros(n, np.uint64)
+    lib().orc_mt64(seed, n, out)
+    return out
+
+
+def initial_guess(n_nodes, gap, phase=0, amp=0.05, seed=0):
+    U = np.zeros(n_nodes * 18)
+    lib().orc_initial_guess(n_nodes, gap, phase, amp, seed, U)
+    return U
+
+
+def newton(mesh, co, U, rtol=1e-10, max_newton=50, lin_tol=0.1, lin_maxit=5000, restart=50, ls_step=0.83,
+           damped=True):
+    U = f64(U).copy()
+    stats = np.zeros(4)
+    rc = lib().orc_newton(mesh.deg, mesh.n_cells, mesh.cell_nodes, mesh.cell_h, mesh.n_faces,
+                          mesh.faces if mesh.n_faces else np.zeros(3, np.int32), mesh.n_nodes, mesh.con_of,
+                          mesh.con_ptr, mesh.con_master, mesh.con_weight, f64(co), U, mesh.row_ptr, mesh.col,
+                          rtol, max_newton, lin_tol, lin_maxit, restart, ls_step, int(damped), stats)
+    return U, rc, stats
