@@ -1,0 +1,146 @@
+/*
+ * glhip.h -- C ABI of the MI355X (gfx950) Ginzburg-Landau Newton-step library.
+ *
+ * The reference femgl has no plugin/FFI layer: its hot path is the private
+ * step methods of class FemGL<dim> (femgl/inc/femgl.h:121-349), operating on
+ * member state (Trilinos CSR system_matrix, system_rhs, residual_vector,
+ * local_solution, ... femgl.h:310-316).  This header is the C boundary that
+ * replaces those methods; each entry point names the reference method it
+ * stands in for.  All functions return 0 on success and a negative status
+ * on error (message via glh_last_error); nothing throws across the ABI.
+ * Host arrays are caller-owned and copied; device buffers belong to the
+ * context.  One host thread and one HIP stream per context.
+ *
+ * Degrees of freedom are numbered node-major: dof = node * 18 + c with
+ * c < 9 the real part u_{c/3, c%3} and c >= 9 the imaginary part
+ * v_{(c-9)/3, (c-9)%3} of the 3x3 complex order parameter A = u + i v
+ * (femgl/src/s_vector2matrix.cc:159-160).  The Jacobian is a BSR matrix of
+ * 18x18 blocks over mesh nodes.
+ */
+#ifndef GLHIP_H
+#define GLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct glh_ctx glh_ctx;
+typedef struct glh_mesh glh_mesh;
+
+#define GLH_OK 0
+#define GLH_ERR_ARG -1
+#define GLH_ERR_HIP -2
+#define GLH_ERR_STATE -3
+#define GLH_ERR_NOCONV -4     /* SolverControl::NoConvergence (solve.cc:159-174) */
+#define GLH_ERR_SINGULAR -5
+#define GLH_ERR_RANGE -6      /* pressure outside matep's table (matep.cc:365-406 UB) */
+
+/* assembly flags */
+#define GLH_ASM_J 1           /* Jacobian (assemble.cc:177-280) */
+#define GLH_ASM_R 2           /* right-hand side -R (assemble.cc:257-276) */
+#define GLH_ASM_FP32 4        /* Jacobian stored and solved in FP32 */
+
+/* ---------------------------------------------------------------- context */
+/* FemGL<dim> constructor state (femgl.cc:107-203): device + stream. */
+int glh_create(int device, glh_ctx **out);
+int glh_destroy(glh_ctx *ctx);
+const char *glh_last_error(const glh_ctx *ctx);
+
+/* Material and gradient constants: alpha, beta1..5 from matep
+ * (femgl.cc:161-168), K1..K3 (femgl.h:320-322) and the AdGR diffuse length
+ * bt (femgl.cc:150, Robin term active iff bt < 1e10, assemble.cc:293). */
+int glh_set_material(glh_ctx *ctx, double alpha, const double beta[5], const double K[3], double bt);
+
+/* matep restatement (matep/src/matep.cc): out[0]=alpha, [1..5]=beta1..5,
+ * [6]=beta_A, [7]=beta_B, [8]=gap_A, [9]=gap_B, [10]=f_A, [11]=f_B.
+ * Returns GLH_ERR_RANGE when SCC is on and p is outside [0, 34). */
+int glh_matep(double p, double t, int scc, double out[12]);
+
+/* ------------------------------------------------------- host mesh layer */
+/* Axis-aligned box of nx*ny*nz hex cells of degree 1 or 2 (the
+ * GridGenerator::hyper_rectangle + refine_global of femgl/src/makegrid_*.cc).
+ * side_id[6] for faces -x,+x,-y,+y,-z,+z: 0 = homogeneous Dirichlet on all
+ * 18 components, 1 = natural, 2/3/4 = AdGR wall (x/y/z-normal column
+ * Dirichlet, femgl.h:294-301, plus the Robin term when bt < 1e10). */
+int glh_mesh_box(int degree, int nx, int ny, int nz, const double lo[3], const double hi[3],
+                 const int side_id[6], glh_mesh **out);
+int glh_mesh_destroy(glh_mesh *m);
+/* sizes: [0]=degree [1]=n_cells [2]=n_nodes [3]=n_faces [4]=nnzb [5]=n_colors
+ *        [6]=n_constraint_dofs [7]=n_constraint_entries */
+int glh_mesh_sizes(const glh_mesh *m, int64_t sizes[8]);
+/* Copies the mesh arrays out (any pointer may be NULL):
+ * cell_nodes[n_cells*ns] (lexicographic local order), cell_h[n_cells*3],
+ * node_xyz[n_nodes*3], faces[n_faces*3] (cell, local face 0..5, id),
+ * node_mask[n_nodes] (bit c set = component c Dirichlet-constrained),
+ * color_ptr[n_colors+1], color_cells[n_cells], row_ptr[n_nodes+1], col[nnzb]. */
+int glh_mesh_get(const glh_mesh *m, int *cell_nodes, double *cell_h, double *node_xyz, int *faces,
+                 uint32_t *node_mask, int *color_ptr, int *color_cells, int *row_ptr, int *col);
+
+/* ------------------------------------------------- per refinement cycle */
+/* Uploads geometry, cell->node map, constraint masks, colour sets and the
+ * BSR pattern once per refinement cycle (setup_uniform_B-phase.cc:125-198).
+ * Also builds the cell->BSR scatter map used by the deterministic
+ * colour-ordered assembly. */
+int glh_upload_mesh(glh_ctx *ctx, const glh_mesh *m);
+
+/* --------------------------------------------------------- solution I/O */
+int glh_set_solution(glh_ctx *ctx, const double *U_host);       /* local_solution */
+int glh_get_solution(glh_ctx *ctx, double *U_host);
+int glh_get_update(glh_ctx *ctx, double *dU_host);              /* locally_relevant_newton_solution */
+int glh_get_rhs(glh_ctx *ctx, double *rhs_host);                /* system_rhs (= -R) */
+/* system_matrix values in BSR order [nnzb][18][18] (FP64; an FP32 Jacobian
+ * is widened).  For parity dumps (SURVEY glh_export_csr). */
+int glh_get_jacobian(glh_ctx *ctx, double *val_host);
+
+/* ---------------------------------------------------------- Newton step */
+/* FemGL::assemble_system (assemble.cc:108-372): J and/or system_rhs = -R at
+ * the current solution, with constrained rows condensed as
+ * AffineConstraints::distribute_local_to_global does. flags: GLH_ASM_*. */
+int glh_assemble(glh_ctx *ctx, int flags);
+/* ||system_rhs|| after glh_assemble with GLH_ASM_R. */
+int glh_rhs_norm(glh_ctx *ctx, double *norm);
+/* FemGL::compute_residual (residual.cc:109-297) at the current solution:
+ * writes ||R|| (R = residual_vector). */
+int glh_residual(glh_ctx *ctx, double *norm);
+/* FemGL::solve (solve.cc:108-187) with GMRES(restart) + 18x18 point-block
+ * Jacobi instead of FGMRES + ML AMG: J dU = system_rhs to
+ * ||r|| <= rtol * ||system_rhs||.  stats[0] = iterations, stats[1] = final
+ * relative residual.  GLH_ERR_NOCONV when maxit is reached. */
+int glh_solve(glh_ctx *ctx, double rtol, int maxit, int restart, double stats[2]);
+/* FemGL::newton_iteration (iteration.cc:109-215): damped back-tracking
+ * u <- u + s^i dU for i < 100 (first trial that lowers ||R||), or one full
+ * step when damped == 0.  out[0] = previous ||rhs||, out[1] = accepted ||R||,
+ * out[2] = step length, out[3] = trials. */
+int glh_newton_step(glh_ctx *ctx, double step, int damped, double out[4]);
+/* Free energy F = integral of f (bulk + gradient) + Robin wall term; the
+ * residual is R = (1/2) dF/dU.  The reference never forms F. */
+int glh_free_energy(glh_ctx *ctx, double *F);
+
+/* ------------------------------------------------------------ FemGL driver */
+/* The Newton loop of FemGL::run (run.cc:208-256) on the uploaded mesh:
+ * assemble -> solve -> newton_iteration, stopping at ||R|| <= rtol * ||R0||
+ * (relative criterion) or after max_newton steps.  fp32 != 0 assembles the
+ * Jacobian in FP32 and runs FP32 GMRES inside FP64 iterative refinement; the
+ * residual, the update and the line search stay FP64.
+ * stats: [0] Newton steps, [1] total GMRES iterations, [2] ||R0||,
+ * [3] final ||R||, [4] seconds assembling, [5] seconds solving,
+ * [6] seconds in line search, [7] free energy. */
+int glh_newton_solve(glh_ctx *ctx, double rtol, int max_newton, double lin_tol, int lin_maxit, int restart,
+                     double ls_step, int damped, int fp32, double stats[8]);
+
+/* timing of the last glh_assemble Jacobian launches on the context stream
+ * (HIP events): ms[0] = J+R assembly ms, ms[1] = number of launches. */
+int glh_last_assembly_ms(glh_ctx *ctx, double ms[2]);
+/* average SpMV kernel ms over the last glh_solve (HIP events around each launch). */
+int glh_last_spmv_ms(glh_ctx *ctx, double ms[2]);
+/* Run k back-to-back SpMV launches y = J x with x = current solution and
+ * time them with HIP events on the context stream: ms_avg. */
+int glh_bench_spmv(glh_ctx *ctx, int k, double *ms_avg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

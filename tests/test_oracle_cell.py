@@ -139,7 +139,7 @@ def test_global_assembly_dirichlet_convention_and_symmetry():
 
 
 def test_global_faithful_equals_factorized_two_cells():
-    mesh = O.box_mesh(2, (1, 1, 2), (0, 0, 0), (1.0, 1.0, 2.0), dirichlet_faces=(4,))
+    mesh = O.box_mesh(1, (2, 1, 2), (0, 0, 0), (2.0, 1.0, 2.0), dirichlet_faces=(4,))
     co = O.coef(22.0, 0.8)
     U = O.distribute(mesh, RNG.standard_normal(mesh.n_dofs) * 0.5)
     v1, r1 = O.assemble(mesh, co, U, faithful=True)

@@ -1,0 +1,444 @@
+// Cell-loop assembly of the Ginzburg-Landau Jacobian J and system_rhs = -R
+// for FE_Q(k) x 18 on axis-aligned hexes, hand-written for gfx950.
+//
+// Reference: femgl/src/assemble.cc:177-280 (cell loop), residual.cc:152-232,
+// the term functions of femgl/src/cell_mat_vec/, and the Robin face loops
+// assemble.cc:286-348 / residual.cc:235-278.
+//
+// The reference evaluates about 170 3x3 products for each (cell, q, i, j).
+// Because every FESystem(FE_Q(k),18) shape function is one-hot in the
+// component, each term is a bilinear form evaluated at unit tensors, so the
+// cell matrix factorises exactly as
+//   J[(a,c),(b,d)] = sum_q JxW_q N_a N_b H_cd(q)
+//                  + K1 delta_cd S_ab + (K2+K3) [same part and spin row] S^{k(c) l(d)}_ab
+// with H(q) the 18x18 Hessian of the bulk density at the iterate
+// (alpha + 2 sum beta_n m_n) and S the stiffness integrals; the residual
+// factorises the same way.  H is evaluated in closed form from six 3x3 Gram
+// matrices per quadrature point.
+//
+// One workgroup (3 wavefronts) owns one cell.  Thread t < 171 owns one
+// unique (c <= d) component pair, keeps H_cd(q) for all q in registers and
+// contracts it against the (a <= b) node-pair weights (uniform, scalar-cached
+// table loads).  Blocks are staged in LDS eight at a time and written to the
+// BSR matrix with 16-byte vector stores: block (a,b) and its transpose (b,a).
+// Cells of one colour share no node, so the read-modify-write scatter needs
+// no atomics and the result is bit-deterministic; the first colour to touch
+// a block stores instead of accumulating, so J needs no zero fill.
+// Dirichlet rows/columns are condensed in the scatter (constrained entries
+// 0, constrained diagonal += |local diagonal|, the documented behaviour of
+// deal.II's AffineConstraints::distribute_local_to_global).
+#include "glhip_dev.hpp"
+
+namespace glh {
+namespace {
+
+constexpr int NT = 192;   // threads per workgroup (3 wavefronts)
+constexpr int NB = 8;     // blocks staged per write round
+constexpr int GRAM = 57;  // per-q Gram data (see gram_entry)
+constexpr int NUPAIR = 171;
+
+// Gram layout per quadrature point:
+//  [0,9) Auu = u^T u (orbital k,l)   [9,18) Avv = v^T v   [18,27) Auv = u^T v
+//  [27,36) uut = u u^T (spin mu,nu)  [36,45) vvt          [45,54) uvt = u v^T
+//  54 |u|^2  55 |v|^2  56 u:v
+__device__ inline double gram_entry(const double *x, int g)
+{
+  const double *u = x, *v = x + 9;
+  if (g < 54) {
+    const int blk = g / 9, i = (g % 9) / 3, j = g % 3;
+    const double *A = (blk == 0 || blk == 2 || blk == 3 || blk == 5) ? u : v;
+    const double *B = (blk == 0 || blk == 3) ? u : v;
+    double s = 0.0;
+    if (blk < 3) {  // A^T B: sum over spin rows
+      for (int m = 0; m < 3; ++m) s += A[3 * m + i] * B[3 * m + j];
+    } else {        // A B^T: sum over orbital columns
+      for (int k = 0; k < 3; ++k) s += A[3 * i + k] * B[3 * j + k];
+    }
+    return s;
+  }
+  double s = 0.0;
+  if (g == 54) for (int a = 0; a < 9; ++a) s += u[a] * u[a];
+  else if (g == 55) for (int a = 0; a < 9; ++a) s += v[a] * v[a];
+  else for (int a = 0; a < 9; ++a) s += u[a] * v[a];
+  return s;
+}
+
+// H_cd (c <= d) of the bulk density: alpha delta + 2 sum_n beta_n m_n(E_c, E_d).
+__device__ inline double hess_entry(const double *__restrict__ G, const double *__restrict__ x, int c, int d,
+                                    const Material &M)
+{
+  const double *u = x, *v = x + 9;
+  const int pc = c / 9, pd = d / 9, ac = c % 9, ad = d % 9;
+  const int mu = ac / 3, k = ac % 3, nu = ad / 3, l = ad % 3;
+  const double dmn = (mu == nu) ? 1.0 : 0.0, dkl = (k == l) ? 1.0 : 0.0, dab = dmn * dkl;
+  const double U2 = G[54], V2 = G[55], UV = G[56], D = U2 - V2, N = U2 + V2;
+  const double b345 = M.b3 + M.b4 + M.b5;
+  if (pc == pd) {
+    // uu block, or vv block with the roles of u and v exchanged
+    const double *p = pc ? v : u, *o = pc ? u : v;
+    const double *App = pc ? G + 9 : G, *Aoo = pc ? G : G + 9;
+    const double *Spp = pc ? G + 36 : G + 27, *Soo = pc ? G + 27 : G + 36;
+    const double sD = pc ? -D : D;
+    return dab * (M.al + 2.0 * M.b1 * sD + 2.0 * M.b2 * N) + 4.0 * (M.b1 + M.b2) * p[ac] * p[ad] +
+           4.0 * M.b1 * o[ac] * o[ad] +
+           2.0 * dmn * (b345 * App[3 * l + k] + (M.b3 + M.b4 - M.b5) * Aoo[3 * l + k]) +
+           2.0 * b345 * p[3 * mu + l] * p[3 * nu + k] + 2.0 * (M.b3 - M.b4 + M.b5) * o[3 * mu + l] * o[3 * nu + k] +
+           2.0 * dkl * (b345 * Spp[3 * mu + nu] + (-M.b3 + M.b4 + M.b5) * Soo[3 * mu + nu]);
+  }
+  // uv block: row (mu,k) in u, column (nu,l) in v
+  const double Auv_lk = G[18 + 3 * l + k], Avu_lk = G[18 + 3 * k + l];
+  const double uvt_mn = G[45 + 3 * mu + nu], uvt_nm = G[45 + 3 * nu + mu];
+  const double Q = uvt_mn + uvt_nm, W = uvt_nm - uvt_mn;
+  const double ua = u[ac], va = v[ac], ub = u[ad], vb = v[ad];
+  const double uml = u[3 * mu + l], unk = u[3 * nu + k], vml = v[3 * mu + l], vnk = v[3 * nu + k];
+  return dab * 4.0 * M.b1 * UV + 4.0 * M.b1 * (ub * va - vb * ua) + 4.0 * M.b2 * ua * vb +
+         2.0 * M.b3 * (-dmn * Avu_lk - vml * unk + dmn * Auv_lk + uml * vnk + Q * dkl) +
+         2.0 * M.b4 * (dmn * Avu_lk + vml * unk - dmn * Auv_lk + uml * vnk - W * dkl) +
+         2.0 * M.b5 * (dmn * Avu_lk + vml * unk + dmn * Auv_lk - uml * vnk + W * dkl);
+}
+
+// rho_c = alpha x_c + 2 sum_n beta_n r_n(E_c): bulk residual density
+__device__ inline double rho_entry(const double *__restrict__ G, const double *__restrict__ x, int c,
+                                   const Material &M)
+{
+  const double *u = x, *v = x + 9;
+  const int part = c / 9, a = c % 9, mu = a / 3, k = a % 3;
+  const double U2 = G[54], V2 = G[55], UV = G[56], D = U2 - V2, N = U2 + V2;
+  const double *uut = G + 27, *vvt = G + 36, *uvt = G + 45;
+  double Pu = 0, Pv = 0, Qu = 0, Qv = 0, Su = 0, Sv = 0, Wu = 0, Wv = 0;
+  for (int nu = 0; nu < 3; ++nu) {
+    const double P = uut[3 * mu + nu] - vvt[3 * mu + nu];
+    const double S = uut[3 * mu + nu] + vvt[3 * mu + nu];
+    const double Q = uvt[3 * mu + nu] + uvt[3 * nu + mu];
+    const double W = uvt[3 * nu + mu] - uvt[3 * mu + nu];
+    const double un = u[3 * nu + k], vn = v[3 * nu + k];
+    Pu += P * un; Pv += P * vn; Qu += Q * un; Qv += Q * vn;
+    Su += S * un; Sv += S * vn; Wu += W * un; Wv += W * vn;
+  }
+  if (part == 0) {
+    const double g1 = D * u[a] + 2.0 * UV * v[a], g2 = N * u[a], g3 = Pu + Qv, g4 = Su - Wv, g5 = Su + Wv;
+    return M.al * u[a] + 2.0 * (M.b1 * g1 + M.b2 * g2 + M.b3 * g3 + M.b4 * g4 + M.b5 * g5);
+  }
+  const double g1 = -D * v[a] + 2.0 * UV * u[a], g2 = N * v[a], g3 = Qu - Pv, g4 = Sv + Wu, g5 = Sv - Wu;
+  return M.al * v[a] + 2.0 * (M.b1 * g1 + M.b2 * g2 + M.b3 * g3 + M.b4 * g4 + M.b5 * g5);
+}
+
+__device__ inline bool face_zeroed(int bid, int c) { return (c % 9) % 3 == bid - 2; }
+
+__device__ inline double face_area(int lf, const double *h)
+{
+  return lf < 2 ? h[1] * h[2] : (lf < 4 ? h[0] * h[2] : h[0] * h[1]);
+}
+
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
+
+template <int DEG, typename TJ, bool DO_J, bool DO_R>
+__global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Material M, const double *__restrict__ U,
+                                                 const int *__restrict__ cells, TJ *__restrict__ J,
+                                                 double *__restrict__ rhs)
+{
+  constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS, NPAIR = NS * (NS + 1) / 2;
+  constexpr int RBUF = DO_R ? NQ * 72 : 1, JBUF = DO_J ? NB * 324 : 1;
+  constexpr int NBUF = RBUF > JBUF ? RBUF : JBUF;
+  __shared__ double sU[NS * 18];
+  __shared__ double sX[NQ * 18];
+  __shared__ double sG[DO_R ? NQ * 54 : 1];
+  __shared__ double sGram[NQ * GRAM];
+  __shared__ double sBuf[NBUF];
+
+  const int tid = threadIdx.x;
+  const int e = cells[blockIdx.x];
+  const int *cn = Mh.cell_nodes + (size_t)e * NS;
+  double h[3], ih[3];
+  for (int k = 0; k < 3; ++k) { h[k] = Mh.cell_h[3 * e + k]; ih[k] = 1.0 / h[k]; }
+  const double vol = h[0] * h[1] * h[2];
+  const uint32_t cface = Mh.cell_face[e];
+
+  // gather the cell's 18 components at its NS nodes
+  for (int i = tid; i < NS * 18; i += NT) sU[i] = U[(size_t)cn[i / 18] * 18 + (i % 18)];
+  __syncthreads();
+  // iterate and gradient at the quadrature points (s_vector2matrix.cc:116-217)
+  for (int i = tid; i < NQ * 18; i += NT) {
+    const int q = i / 18, c = i % 18;
+    double xv = 0, g0 = 0, g1 = 0, g2 = 0;
+    for (int a = 0; a < NS; ++a) {
+      const double Ua = sU[a * 18 + c];
+      xv += T.N[q * NS + a] * Ua;
+      if (DO_R) {
+        g0 += T.dN[(q * NS + a) * 3 + 0] * Ua;
+        g1 += T.dN[(q * NS + a) * 3 + 1] * Ua;
+        g2 += T.dN[(q * NS + a) * 3 + 2] * Ua;
+      }
+    }
+    sX[i] = xv;
+    if (DO_R) {
+      sG[q * 54 + c] = g0 * ih[0];
+      sG[q * 54 + 18 + c] = g1 * ih[1];
+      sG[q * 54 + 36 + c] = g2 * ih[2];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < NQ * GRAM; i += NT) {
+    const int q = i / GRAM, g = i % GRAM;
+    sGram[i] = gram_entry(sX + q * 18, g);
+  }
+  __syncthreads();
+
+  if (DO_R) {
+    // rho_c(q) and gamma_{c,k}(q) = K1 d_k x_c + (K2+K3) [k == orbital(c)] div_part(mu)
+    for (int i = tid; i < NQ * 18; i += NT) {
+      const int q = i / 18, c = i % 18;
+      sBuf[q * 72 + c] = rho_entry(sGram + q * GRAM, sX + q * 18, c, M);
+      const int part = c / 9, mu = (c % 9) / 3, kc = c % 3;
+      const double *g = sG + q * 54;
+      const double dv = g[9 * part + 3 * mu + 0] + g[18 + 9 * part + 3 * mu + 1] + g[36 + 9 * part + 3 * mu + 2];
+      for (int k = 0; k < 3; ++k)
+        sBuf[q * 72 + 18 + k * 18 + c] = M.K1 * g[k * 18 + c] + (k == kc ? M.K23 * dv : 0.0);
+    }
+    __syncthreads();
+    for (int i = tid; i < NS * 18; i += NT) {
+      const int a = i / 18, c = i % 18;
+      double r = 0.0;
+      for (int q = 0; q < NQ; ++q) {
+        const double *b = sBuf + q * 72;
+        const double *d = T.dN + (q * NS + a) * 3;
+        r += T.w[q] * (T.N[q * NS + a] * b[c] + d[0] * ih[0] * b[18 + c] + d[1] * ih[1] * b[36 + c] +
+                       d[2] * ih[2] * b[54 + c]);
+      }
+      r *= vol;
+      if (M.robin != 0.0) {
+        for (int lf = 0; lf < 6; ++lf) {
+          const int bid = (int)((cface >> (3 * lf)) & 7u) - 1;
+          if (bid < 2 || bid > 4 || face_zeroed(bid, c)) continue;
+          const double *mf = T.Mf + (lf * NS + a) * NS;
+          double s = 0.0;
+          for (int b = 0; b < NS; ++b) s += mf[b] * sU[b * 18 + c];
+          r += M.robin * face_area(lf, h) * s;
+        }
+      }
+      const int node = cn[a];
+      if (!((Mh.node_mask[node] >> c) & 1u)) rhs[(size_t)node * 18 + c] -= r;
+    }
+    if (DO_J) __syncthreads();
+  }
+
+  if (DO_J) {
+    using V2 = typename Vec2<TJ>::type;
+    // unique component pair of this thread
+    int c = 0, d = 0;
+    const bool own = tid < NUPAIR;
+    if (own) {
+      int t = tid;
+      while (t >= 18 - c) { t -= 18 - c; ++c; }
+      d = c + t;
+    }
+    double Hq[NQ];
+    if (own)
+      for (int q = 0; q < NQ; ++q) Hq[q] = hess_entry(sGram + q * GRAM, sX + q * 18, c, d, M);
+    const bool grad_cd = (c / 9 == d / 9) && ((c % 9) / 3 == (d % 9) / 3);
+    const int kc = c % 3, kd = d % 3;
+    const double lap_scale[3] = {vol * ih[0] * ih[0], vol * ih[1] * ih[1], vol * ih[2] * ih[2]};
+    const double k23_cd = M.K23 * vol * ih[kc] * ih[kd];
+    const uint32_t *mask = Mh.node_mask;
+    const int *cblk = Mh.cell_blk + (size_t)e * NS * NS;
+
+    for (int p0 = 0; p0 < NPAIR; p0 += NB) {
+      const int nb = (NPAIR - p0) < NB ? (NPAIR - p0) : NB;
+      if (own) {
+        for (int s = 0; s < nb; ++s) {
+          const int p = p0 + s;
+          const double *wp = T.Wp + p * NQ;
+          double sym = 0.0;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) sym += wp[q] * Hq[q];
+          sym *= vol;
+          const double *sp = T.Sp + p * 9;
+          if (c == d) {
+            sym += M.K1 * (sp[0] * lap_scale[0] + sp[4] * lap_scale[1] + sp[8] * lap_scale[2]);
+            if (M.robin != 0.0) {
+              const int a = T.pa[p], b = T.pb[p];
+              for (int lf = 0; lf < 6; ++lf) {
+                const int bid = (int)((cface >> (3 * lf)) & 7u) - 1;
+                if (bid < 2 || bid > 4 || face_zeroed(bid, c)) continue;
+                sym += M.robin * face_area(lf, h) * T.Mf[(lf * NS + a) * NS + b];
+              }
+            }
+          }
+          double e1 = sym, e2 = sym;
+          if (grad_cd) {
+            e1 += k23_cd * sp[kc * 3 + kd];
+            e2 += k23_cd * sp[kd * 3 + kc];
+          }
+          sBuf[s * 324 + c * 18 + d] = e1;
+          sBuf[s * 324 + d * 18 + c] = e2;
+        }
+      }
+      __syncthreads();
+      // scatter: block (a,b) from the stage, block (b,a) as its transpose
+      for (int i = tid; i < nb * 324; i += NT) {
+        const int s = i / 324, r = i % 324, dir = r / 162, k2 = r % 162;
+        const int p = p0 + s, a = T.pa[p], b = T.pb[p];
+        if (dir == 1 && a == b) continue;
+        const int ra = dir ? b : a, cb = dir ? a : b;
+        const int rn = cn[ra], cnd = cn[cb];
+        const int code = cblk[ra * NS + cb];
+        const bool first = code < 0;
+        const size_t blk = (size_t)(first ? -code - 1 : code);
+        const int cc = (2 * k2) / 18, dd = (2 * k2) % 18;
+        const double *st = sBuf + s * 324;
+        double v0 = dir ? st[dd * 18 + cc] : st[cc * 18 + dd];
+        double v1 = dir ? st[(dd + 1) * 18 + cc] : st[cc * 18 + dd + 1];
+        const uint32_t mr = mask[rn], mc = mask[cnd];
+        const bool rc = (mr >> cc) & 1u;
+        if (rc || ((mc >> dd) & 1u)) v0 = (a == b && rc && cc == dd) ? fabs(v0) : 0.0;
+        if (rc || ((mc >> (dd + 1)) & 1u)) v1 = (a == b && rc && cc == dd + 1) ? fabs(v1) : 0.0;
+        V2 *dst = reinterpret_cast<V2 *>(J + blk * 324) + k2;
+        V2 out;
+        if (first) {
+          out.x = (TJ)v0;
+          out.y = (TJ)v1;
+        } else {
+          const V2 old = *dst;
+          out.x = (TJ)((double)old.x + v0);
+          out.y = (TJ)((double)old.y + v1);
+        }
+        *dst = out;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <int DEG>
+__global__ __launch_bounds__(NT) void k_energy(RefTables T, MeshDev Mh, Material M, const double *__restrict__ U,
+                                               int n_cells, double *__restrict__ Fcell)
+{
+  constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS;
+  __shared__ double sU[NS * 18];
+  __shared__ double sX[NQ * 18];
+  __shared__ double sG[NQ * 54];
+  __shared__ double sF[NT];
+  const int tid = threadIdx.x, e = blockIdx.x;
+  if (e >= n_cells) return;
+  const int *cn = Mh.cell_nodes + (size_t)e * NS;
+  double h[3], ih[3];
+  for (int k = 0; k < 3; ++k) { h[k] = Mh.cell_h[3 * e + k]; ih[k] = 1.0 / h[k]; }
+  const double vol = h[0] * h[1] * h[2];
+  for (int i = tid; i < NS * 18; i += NT) sU[i] = U[(size_t)cn[i / 18] * 18 + (i % 18)];
+  __syncthreads();
+  for (int i = tid; i < NQ * 18; i += NT) {
+    const int q = i / 18, c = i % 18;
+    double xv = 0, g0 = 0, g1 = 0, g2 = 0;
+    for (int a = 0; a < NS; ++a) {
+      const double Ua = sU[a * 18 + c];
+      xv += T.N[q * NS + a] * Ua;
+      g0 += T.dN[(q * NS + a) * 3 + 0] * Ua;
+      g1 += T.dN[(q * NS + a) * 3 + 1] * Ua;
+      g2 += T.dN[(q * NS + a) * 3 + 2] * Ua;
+    }
+    sX[i] = xv;
+    sG[q * 54 + c] = g0 * ih[0];
+    sG[q * 54 + 18 + c] = g1 * ih[1];
+    sG[q * 54 + 36 + c] = g2 * ih[2];
+  }
+  __syncthreads();
+  double acc = 0.0;
+  for (int q = tid; q < NQ; q += NT) {
+    const double *x = sX + q * 18, *u = x, *v = x + 9, *g = sG + q * 54;
+    double uut[9], vvt[9], uvt[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double a1 = 0, a2 = 0, a3 = 0;
+        for (int k = 0; k < 3; ++k) {
+          a1 += u[3 * i + k] * u[3 * j + k];
+          a2 += v[3 * i + k] * v[3 * j + k];
+          a3 += u[3 * i + k] * v[3 * j + k];
+        }
+        uut[3 * i + j] = a1; vvt[3 * i + j] = a2; uvt[3 * i + j] = a3;
+      }
+    double U2 = 0, V2 = 0, UV = 0;
+    for (int a = 0; a < 9; ++a) { U2 += u[a] * u[a]; V2 += v[a] * v[a]; UV += u[a] * v[a]; }
+    double trP2 = 0, trQ2 = 0, trS2 = 0, trW2 = 0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        const double P = uut[3 * i + j] - vvt[3 * i + j], S = uut[3 * i + j] + vvt[3 * i + j];
+        const double Q = uvt[3 * i + j] + uvt[3 * j + i];
+        const double Wij = uvt[3 * j + i] - uvt[3 * i + j], Wji = uvt[3 * i + j] - uvt[3 * j + i];
+        trP2 += P * P; trQ2 += Q * Q; trS2 += S * S; trW2 += Wij * Wji;
+      }
+    const double D = U2 - V2, N = U2 + V2;
+    double f = M.al * N + M.b1 * (D * D + 4 * UV * UV) + M.b2 * N * N + M.b3 * (trP2 + trQ2) +
+               M.b4 * (trS2 - trW2) + M.b5 * (trS2 + trW2);
+    double g2 = 0;
+    for (int i = 0; i < 54; ++i) g2 += g[i] * g[i];
+    double dv = 0;
+    for (int part = 0; part < 2; ++part)
+      for (int m = 0; m < 3; ++m) {
+        const double dd = g[9 * part + 3 * m] + g[18 + 9 * part + 3 * m + 1] + g[36 + 9 * part + 3 * m + 2];
+        dv += dd * dd;
+      }
+    f += M.K1 * g2 + M.K23 * dv;
+    acc += T.w[q] * vol * f;
+  }
+  // Robin wall energy: (K1/bt) sum_faces U^T M_f U over unzeroed components
+  if (M.robin != 0.0) {
+    const uint32_t cface = Mh.cell_face[e];
+    for (int lf = 0; lf < 6; ++lf) {
+      const int bid = (int)((cface >> (3 * lf)) & 7u) - 1;
+      if (bid < 2 || bid > 4) continue;
+      for (int i = tid; i < NS * 18; i += NT) {
+        const int a = i / 18, c = i % 18;
+        if (face_zeroed(bid, c)) continue;
+        double s = 0;
+        for (int b = 0; b < NS; ++b) s += T.Mf[(lf * NS + a) * NS + b] * sU[b * 18 + c];
+        acc += M.robin * face_area(lf, h) * sU[a * 18 + c] * s;
+      }
+    }
+  }
+  sF[tid] = acc;
+  __syncthreads();
+  // fixed-order tree: fold 192 -> 128, then halve
+  if (tid < NT - 128) sF[tid] += sF[tid + 128];
+  __syncthreads();
+  for (int st = 64; st > 0; st >>= 1) {
+    if (tid < st) sF[tid] += sF[tid + st];
+    __syncthreads();
+  }
+  if (tid == 0) Fcell[e] = sF[0];
+}
+
+}  // namespace
+
+hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefTables &T, const MeshDev &M,
+                           const Material &mat, const double *U, const int *cells, int n, void *J, double *rhs,
+                           hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  dim3 g(n), b(NT);
+#define GLH_ASM(D, TJ, DJ, DR) \
+  hipLaunchKernelGGL((k_assemble<D, TJ, DJ, DR>), g, b, 0, s, T, M, mat, U, cells, (TJ *)J, rhs)
+  if (deg == 1) {
+    if (do_j && do_r) { if (jfp32) GLH_ASM(1, float, true, true); else GLH_ASM(1, double, true, true); }
+    else if (do_j) { if (jfp32) GLH_ASM(1, float, true, false); else GLH_ASM(1, double, true, false); }
+    else GLH_ASM(1, double, false, true);
+  } else {
+    if (do_j && do_r) { if (jfp32) GLH_ASM(2, float, true, true); else GLH_ASM(2, double, true, true); }
+    else if (do_j) { if (jfp32) GLH_ASM(2, float, true, false); else GLH_ASM(2, double, true, false); }
+    else GLH_ASM(2, double, false, true);
+  }
+#undef GLH_ASM
+  return hipGetLastError();
+}
+
+hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
+                         int n_cells, double *Fcell, hipStream_t s)
+{
+  if (n_cells <= 0) return hipSuccess;
+  if (deg == 1) hipLaunchKernelGGL(k_energy<1>, dim3(n_cells), dim3(NT), 0, s, T, M, mat, U, n_cells, Fcell);
+  else hipLaunchKernelGGL(k_energy<2>, dim3(n_cells), dim3(NT), 0, s, T, M, mat, U, n_cells, Fcell);
+  return hipGetLastError();
+}
+
+}  // namespace glh

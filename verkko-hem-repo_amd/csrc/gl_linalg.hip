@@ -1,0 +1,474 @@
+// Linear-algebra kernels of the Newton-GMRES solve on gfx950: BSR(18x18)
+// SpMV, 18x18 point-block Jacobi (inverse and apply), deterministic
+// multi-dot reductions and the vector updates of GMRES and the line search.
+// They replace the Trilinos Epetra SpMV / dot products inside deal.II's
+// SolverFGMRES (femgl/src/solve.cc:159-174) and the vector updates of
+// femgl/src/iteration.cc:170-206.
+//
+// Layout: vectors are node-major (dof = node*18 + c); the matrix is BSR with
+// row-major 18x18 blocks of 324 contiguous values.  All kernels are
+// HBM-bound; the SpMV streams each block once with 16-byte loads.
+#include "glhip_dev.hpp"
+
+namespace glh {
+namespace {
+
+template <typename T> struct V2t;
+template <> struct V2t<double> { using type = double2; };
+template <> struct V2t<float> { using type = float2; };
+
+// 6 consecutive values starting at a 16-byte (double) / 8-byte (float) boundary
+template <typename T>
+__device__ inline void load6(const T *__restrict__ p, double *o)
+{
+  using V2 = typename V2t<T>::type;
+  const V2 *q = reinterpret_cast<const V2 *>(p);
+  const V2 a = q[0], b = q[1], c = q[2];
+  o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y; o[4] = c.x; o[5] = c.y;
+}
+
+// One wavefront per block row.  Lane = g*18 + c (g < 3): row c of every
+// block, columns 6g..6g+5.  Partial sums of the three column groups are
+// combined with cross-lane reads at the end of the row.
+template <typename TJ, typename TX>
+__global__ __launch_bounds__(256) void k_spmv(int n_rows, const int *__restrict__ rp, const int *__restrict__ col,
+                                              const TJ *__restrict__ val, const TX *__restrict__ x,
+                                              TX *__restrict__ y)
+{
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n_rows) return;
+  const int g = lane / 18, c = lane % 18;
+  const bool active = lane < 54;
+  double acc = 0.0;
+  const int b0 = rp[row], b1 = rp[row + 1];
+  if (active) {
+    int b = b0;
+    for (; b + 1 < b1; b += 2) {
+      const int s0 = col[b], s1 = col[b + 1];
+      double B0[6], B1[6], X0[6], X1[6];
+      load6(val + (size_t)b * 324 + c * 18 + 6 * g, B0);
+      load6(val + (size_t)(b + 1) * 324 + c * 18 + 6 * g, B1);
+      load6(x + (size_t)s0 * 18 + 6 * g, X0);
+      load6(x + (size_t)s1 * 18 + 6 * g, X1);
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc += B0[j] * X0[j];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc += B1[j] * X1[j];
+    }
+    if (b < b1) {
+      double B0[6], X0[6];
+      load6(val + (size_t)b * 324 + c * 18 + 6 * g, B0);
+      load6(x + (size_t)col[b] * 18 + 6 * g, X0);
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc += B0[j] * X0[j];
+    }
+  }
+  const double a1 = __shfl(acc, (c + 18) & 63);
+  const double a2 = __shfl(acc, (c + 36) & 63);
+  if (lane < 18) y[(size_t)row * 18 + c] = (TX)(acc + a1 + a2);
+}
+
+// r = b - J x with FP64 vectors (used for the outer iterative refinement with
+// an FP32 matrix and for the GMRES restart residual)
+template <typename TJ>
+__global__ __launch_bounds__(256) void k_res_spmv(int n_rows, const int *__restrict__ rp, const int *__restrict__ col,
+                                                  const TJ *__restrict__ val, const double *__restrict__ x,
+                                                  const double *__restrict__ bv, double *__restrict__ r)
+{
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n_rows) return;
+  const int g = lane / 18, c = lane % 18;
+  double acc = 0.0;
+  if (lane < 54)
+    for (int b = rp[row]; b < rp[row + 1]; ++b) {
+      double B0[6], X0[6];
+      load6(val + (size_t)b * 324 + c * 18 + 6 * g, B0);
+      load6(x + (size_t)col[b] * 18 + 6 * g, X0);
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc += B0[j] * X0[j];
+    }
+  const double a1 = __shfl(acc, (c + 18) & 63);
+  const double a2 = __shfl(acc, (c + 36) & 63);
+  if (lane < 18) r[(size_t)row * 18 + c] = bv[(size_t)row * 18 + c] - (acc + a1 + a2);
+}
+
+// Gauss-Jordan inverse of one 18x18 diagonal block per workgroup (64 threads,
+// thread j owns column j of the augmented [A | I] matrix).
+template <typename TJ>
+__global__ __launch_bounds__(64) void k_bj_invert(int n_rows, const int *__restrict__ diag,
+                                                  const TJ *__restrict__ val, TJ *__restrict__ Dinv,
+                                                  int *__restrict__ status)
+{
+  __shared__ double M[18][37];
+  __shared__ double colc[18];
+  __shared__ int piv;
+  const int r = blockIdx.x, t = threadIdx.x;
+  if (r >= n_rows) return;
+  const TJ *A = val + (size_t)diag[r] * 324;
+  for (int i = t; i < 18 * 36; i += 64) {
+    const int row = i / 36, cl = i % 36;
+    M[row][cl] = cl < 18 ? (double)A[row * 18 + cl] : (cl - 18 == row ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  bool singular = false;
+  for (int c = 0; c < 18; ++c) {
+    if (t == 0) {
+      int p = c;
+      double best = fabs(M[c][c]);
+      for (int i = c + 1; i < 18; ++i)
+        if (fabs(M[i][c]) > best) { best = fabs(M[i][c]); p = i; }
+      piv = p;
+    }
+    __syncthreads();
+    const int p = piv;
+    if (p != c && t < 36) {
+      const double tmp = M[c][t];
+      M[c][t] = M[p][t];
+      M[p][t] = tmp;
+    }
+    __syncthreads();
+    const double pv = M[c][c];
+    if (pv == 0.0) { singular = true; break; }
+    __syncthreads();
+    if (t < 36) M[c][t] /= pv;
+    if (t < 18) colc[t] = M[t][c];
+    __syncthreads();
+    if (t < 36)
+      for (int i = 0; i < 18; ++i)
+        if (i != c) M[i][t] -= colc[i] * M[c][t];
+    __syncthreads();
+  }
+  if (singular) {
+    if (t == 0) status[0] = 1;
+    return;
+  }
+  TJ *D = Dinv + (size_t)r * 324;
+  for (int i = t; i < 324; i += 64) D[i] = (TJ)M[i / 18][18 + i % 18];
+}
+
+template <typename TJ, typename TX>
+__global__ __launch_bounds__(256) void k_bj_apply(int n_rows, const TJ *__restrict__ Dinv, const TX *__restrict__ x,
+                                                  TX *__restrict__ y)
+{
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n_rows) return;
+  const int g = lane / 18, c = lane % 18;
+  double acc = 0.0;
+  if (lane < 54) {
+    double B0[6], X0[6];
+    load6(Dinv + (size_t)row * 324 + c * 18 + 6 * g, B0);
+    load6(x + (size_t)row * 18 + 6 * g, X0);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc += B0[j] * X0[j];
+  }
+  const double a1 = __shfl(acc, (c + 18) & 63);
+  const double a2 = __shfl(acc, (c + 36) & 63);
+  if (lane < 18) y[(size_t)row * 18 + c] = (TX)(acc + a1 + a2);
+}
+
+// -------------------------------------------------------------- reductions
+constexpr int RT = 256;
+constexpr int KCH = 8;  // vectors per multi-dot pass
+
+template <typename T>
+__device__ inline T block_reduce(T v, T *sh)
+{
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int st = RT / 2; st > 0; st >>= 1) {
+    if (t < st) sh[t] += sh[t + st];
+    __syncthreads();
+  }
+  const T r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// partial[blk*k + i] = sum over this block's fixed index set of V_i . w
+template <typename T>
+__global__ __launch_bounds__(RT) void k_mdot_partial(size_t n, int k, int i0, const T *__restrict__ V, size_t ldv,
+                                                     const T *__restrict__ w, T *__restrict__ partial)
+{
+  __shared__ T sh[RT];
+  T acc[KCH];
+#pragma unroll
+  for (int j = 0; j < KCH; ++j) acc[j] = 0;
+  const int kk = (k - i0) < KCH ? (k - i0) : KCH;
+  const size_t stride = (size_t)gridDim.x * RT;
+  for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n; idx += stride) {
+    const T wv = w[idx];
+#pragma unroll
+    for (int j = 0; j < KCH; ++j)
+      if (j < kk) acc[j] += V[(size_t)(i0 + j) * ldv + idx] * wv;
+  }
+  for (int j = 0; j < kk; ++j) {
+    const T r = block_reduce(acc[j], sh);
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * k + i0 + j] = r;
+  }
+}
+
+// out[i] (+)= sum over blocks of partial[blk*k + i], fixed order
+template <typename T>
+__global__ __launch_bounds__(RT) void k_mdot_final(int nblk, int k, const T *__restrict__ partial, T *__restrict__ out,
+                                                   int accumulate)
+{
+  __shared__ T sh[RT];
+  const int i = blockIdx.x;
+  T acc = 0;
+  for (int b = threadIdx.x; b < nblk; b += RT) acc += partial[(size_t)b * k + i];
+  const T r = block_reduce(acc, sh);
+  if (threadIdx.x == 0) out[i] = accumulate ? out[i] + r : r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(RT) void k_sub_lincomb(size_t n, int k, const T *__restrict__ V, size_t ldv,
+                                                    const T *__restrict__ h, T *__restrict__ w)
+{
+  const size_t stride = (size_t)gridDim.x * RT;
+  for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n; idx += stride) {
+    T s = w[idx];
+    for (int i = 0; i < k; ++i) s -= h[i] * V[(size_t)i * ldv + idx];
+    w[idx] = s;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(RT) void k_lincomb(size_t n, int k, const T *__restrict__ V, size_t ldv,
+                                                const T *__restrict__ y, T *__restrict__ w)
+{
+  const size_t stride = (size_t)gridDim.x * RT;
+  for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n; idx += stride) {
+    T s = 0;
+    for (int i = 0; i < k; ++i) s += y[i] * V[(size_t)i * ldv + idx];
+    w[idx] = s;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(RT) void k_scale_by(size_t n, const T *__restrict__ src, const T *__restrict__ scal,
+                                                 T *__restrict__ dst)
+{
+  const T a = scal[0];
+  const T inv = a != (T)0 ? (T)1 / a : (T)1;
+  const size_t stride = (size_t)gridDim.x * RT;
+  for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n; idx += stride) dst[idx] = src[idx] * inv;
+}
+
+template <typename TX, typename TY>
+__global__ __launch_bounds__(RT) void k_axpy(size_t n, double a, const TX *__restrict__ x, TY *__restrict__ y)
+{
+  const size_t stride = (size_t)gridDim.x * RT;
+  for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n; idx += stride)
+    y[idx] = (TY)((double)y[idx] + a * (double)x[idx]);
+}
+
+template <typename TS, typename TD>
+__global__ __launch_bounds__(RT) void k_convert(size_t n, const TS *__restrict__ s, TD *__restrict__ d)
+{
+  const size_t stride = (size_t)gridDim.x * RT;
+  for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n; idx += stride) d[idx] = (TD)s[idx];
+}
+
+__global__ __launch_bounds__(RT) void k_sub(size_t n, const double *__restrict__ b, const double *__restrict__ w,
+                                           double *__restrict__ d)
+{
+  const size_t stride = (size_t)gridDim.x * RT;
+  for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n; idx += stride) d[idx] = b[idx] - w[idx];
+}
+
+__global__ __launch_bounds__(RT) void k_trial(size_t n, const double *__restrict__ U, const double *__restrict__ dU,
+                                             double a, const uint32_t *__restrict__ mask, double *__restrict__ X)
+{
+  const size_t stride = (size_t)gridDim.x * RT;
+  for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n; idx += stride) {
+    const uint32_t m = mask[idx / 18];
+    X[idx] = ((m >> (idx % 18)) & 1u) ? 0.0 : U[idx] + a * dU[idx];
+  }
+}
+
+__global__ __launch_bounds__(RT) void k_mask_zero(size_t n, const uint32_t *__restrict__ mask, double *__restrict__ x)
+{
+  const size_t stride = (size_t)gridDim.x * RT;
+  for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n; idx += stride)
+    if ((mask[idx / 18] >> (idx % 18)) & 1u) x[idx] = 0.0;
+}
+
+__global__ __launch_bounds__(RT) void k_sum_partial(size_t n, const double *__restrict__ v,
+                                                   double *__restrict__ partial)
+{
+  __shared__ double sh[RT];
+  double acc = 0.0;
+  const size_t stride = (size_t)gridDim.x * RT;
+  for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n; idx += stride) acc += v[idx];
+  const double r = block_reduce(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+template <typename T>
+__global__ void k_set_scalar(T *p, T v)
+{
+  if (threadIdx.x == 0) p[0] = v;
+}
+
+inline int grid_for(size_t n)
+{
+  size_t g = (n + RT - 1) / RT;
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+}  // namespace
+
+template <typename TJ, typename TX>
+hipError_t launch_spmv(int n_rows, const int *row_ptr, const int *col, const TJ *val, const TX *x, TX *y,
+                       hipStream_t s)
+{
+  if (n_rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_spmv<TJ, TX>), dim3((n_rows + 3) / 4), dim3(256), 0, s, n_rows, row_ptr, col, val, x, y);
+  return hipGetLastError();
+}
+
+template <typename TJ>
+hipError_t launch_residual_spmv(int n_rows, const int *row_ptr, const int *col, const TJ *val, const double *x,
+                                const double *b, double *r, hipStream_t s)
+{
+  if (n_rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_res_spmv<TJ>), dim3((n_rows + 3) / 4), dim3(256), 0, s, n_rows, row_ptr, col, val, x, b, r);
+  return hipGetLastError();
+}
+
+template <typename TJ>
+hipError_t launch_bj_invert(int n_rows, const int *diag_blk, const TJ *val, TJ *Dinv, int *status, hipStream_t s)
+{
+  if (n_rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_bj_invert<TJ>), dim3(n_rows), dim3(64), 0, s, n_rows, diag_blk, val, Dinv, status);
+  return hipGetLastError();
+}
+
+template <typename TJ, typename TX>
+hipError_t launch_bj_apply(int n_rows, const TJ *Dinv, const TX *x, TX *y, hipStream_t s)
+{
+  if (n_rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_bj_apply<TJ, TX>), dim3((n_rows + 3) / 4), dim3(256), 0, s, n_rows, Dinv, x, y);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_mdot(size_t n, int k, const T *V, size_t ldv, const T *w, T *partial, T *out, bool accumulate,
+                       hipStream_t s)
+{
+  if (k <= 0) return hipSuccess;
+  for (int i0 = 0; i0 < k; i0 += KCH)
+    hipLaunchKernelGGL((k_mdot_partial<T>), dim3(MDOT_GRID), dim3(RT), 0, s, n, k, i0, V, ldv, w, partial);
+  hipLaunchKernelGGL((k_mdot_final<T>), dim3(k), dim3(RT), 0, s, MDOT_GRID, k, partial, out, accumulate ? 1 : 0);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_sub_lincomb(size_t n, int k, const T *V, size_t ldv, const T *h, T *w, hipStream_t s)
+{
+  hipLaunchKernelGGL((k_sub_lincomb<T>), dim3(grid_for(n)), dim3(RT), 0, s, n, k, V, ldv, h, w);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_lincomb(size_t n, int k, const T *V, size_t ldv, const T *y, T *w, hipStream_t s)
+{
+  hipLaunchKernelGGL((k_lincomb<T>), dim3(grid_for(n)), dim3(RT), 0, s, n, k, V, ldv, y, w);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_scale_by(size_t n, const T *src, const T *scal, T *dst, hipStream_t s)
+{
+  hipLaunchKernelGGL((k_scale_by<T>), dim3(grid_for(n)), dim3(RT), 0, s, n, src, scal, dst);
+  return hipGetLastError();
+}
+
+template <typename TX, typename TY>
+hipError_t launch_axpy(size_t n, double a, const TX *x, TY *y, hipStream_t s)
+{
+  hipLaunchKernelGGL((k_axpy<TX, TY>), dim3(grid_for(n)), dim3(RT), 0, s, n, a, x, y);
+  return hipGetLastError();
+}
+
+template <typename TS, typename TD>
+hipError_t launch_convert(size_t n, const TS *src, TD *dst, hipStream_t s)
+{
+  hipLaunchKernelGGL((k_convert<TS, TD>), dim3(grid_for(n)), dim3(RT), 0, s, n, src, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_sub(size_t n, const double *b, const double *w, double *dst, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sub, dim3(grid_for(n)), dim3(RT), 0, s, n, b, w, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_trial(int n_nodes, const double *U, const double *dU, double a, const uint32_t *mask, double *X,
+                        hipStream_t s)
+{
+  const size_t n = (size_t)n_nodes * 18;
+  hipLaunchKernelGGL(k_trial, dim3(grid_for(n)), dim3(RT), 0, s, n, U, dU, a, mask, X);
+  return hipGetLastError();
+}
+
+hipError_t launch_mask_zero(int n_nodes, const uint32_t *mask, double *x, hipStream_t s)
+{
+  const size_t n = (size_t)n_nodes * 18;
+  hipLaunchKernelGGL(k_mask_zero, dim3(grid_for(n)), dim3(RT), 0, s, n, mask, x);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum(size_t n, const double *v, double *partial, double *out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sum_partial, dim3(MDOT_GRID), dim3(RT), 0, s, n, v, partial);
+  hipLaunchKernelGGL((k_mdot_final<double>), dim3(1), dim3(RT), 0, s, MDOT_GRID, 1, partial, out, 0);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_set_scalar(T *p, T v, hipStream_t s)
+{
+  hipLaunchKernelGGL((k_set_scalar<T>), dim3(1), dim3(64), 0, s, p, v);
+  return hipGetLastError();
+}
+
+// explicit instantiations
+template hipError_t launch_set_scalar<double>(double *, double, hipStream_t);
+template hipError_t launch_set_scalar<float>(float *, float, hipStream_t);
+template hipError_t launch_axpy<float, float>(size_t, double, const float *, float *, hipStream_t);
+template hipError_t launch_spmv<double, double>(int, const int *, const int *, const double *, const double *,
+                                                double *, hipStream_t);
+template hipError_t launch_spmv<float, float>(int, const int *, const int *, const float *, const float *, float *,
+                                              hipStream_t);
+template hipError_t launch_residual_spmv<double>(int, const int *, const int *, const double *, const double *,
+                                                 const double *, double *, hipStream_t);
+template hipError_t launch_residual_spmv<float>(int, const int *, const int *, const float *, const double *,
+                                                const double *, double *, hipStream_t);
+template hipError_t launch_bj_invert<double>(int, const int *, const double *, double *, int *, hipStream_t);
+template hipError_t launch_bj_invert<float>(int, const int *, const float *, float *, int *, hipStream_t);
+template hipError_t launch_bj_apply<double, double>(int, const double *, const double *, double *, hipStream_t);
+template hipError_t launch_bj_apply<float, float>(int, const float *, const float *, float *, hipStream_t);
+template hipError_t launch_mdot<double>(size_t, int, const double *, size_t, const double *, double *, double *, bool,
+                                        hipStream_t);
+template hipError_t launch_mdot<float>(size_t, int, const float *, size_t, const float *, float *, float *, bool,
+                                       hipStream_t);
+template hipError_t launch_sub_lincomb<double>(size_t, int, const double *, size_t, const double *, double *,
+                                               hipStream_t);
+template hipError_t launch_sub_lincomb<float>(size_t, int, const float *, size_t, const float *, float *, hipStream_t);
+template hipError_t launch_lincomb<double>(size_t, int, const double *, size_t, const double *, double *, hipStream_t);
+template hipError_t launch_lincomb<float>(size_t, int, const float *, size_t, const float *, float *, hipStream_t);
+template hipError_t launch_scale_by<double>(size_t, const double *, const double *, double *, hipStream_t);
+template hipError_t launch_scale_by<float>(size_t, const float *, const float *, float *, hipStream_t);
+template hipError_t launch_axpy<double, double>(size_t, double, const double *, double *, hipStream_t);
+template hipError_t launch_axpy<float, double>(size_t, double, const float *, double *, hipStream_t);
+template hipError_t launch_convert<double, float>(size_t, const double *, float *, hipStream_t);
+template hipError_t launch_convert<float, double>(size_t, const float *, double *, hipStream_t);
+
+}  // namespace glh

@@ -1,0 +1,99 @@
+// Internal interface between the host driver (glhip_api.cpp) and the HIP
+// kernels (gl_assemble.hip, gl_linalg.hip).  Plain device pointers only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace glh {
+
+// Reference-element tables for FE_Q(k) x QGauss(k+1) on the unit cube
+// (assemble.cc:123-132), built once per context on the host.
+struct RefTables {
+  const double *N;    // [NQ*NS]   N_a(x_q)
+  const double *dN;   // [NQ*NS*3] dN_a/dxi_k(x_q)
+  const double *w;    // [NQ]      quadrature weights (sum 1)
+  const double *Wp;   // [NPAIR*NQ] w_q N_a N_b for a <= b pairs
+  const double *Sp;   // [NPAIR*9]  sum_q w_q dN_a/dxi_k dN_b/dxi_l, index k*3+l
+  const double *Mf;   // [6*NS*NS]  face mass on each unit face
+  const int *pa, *pb; // [NPAIR]    pair list (a <= b)
+};
+
+struct Material {
+  double al, b1, b2, b3, b4, b5;  // alpha, beta1..5
+  double K1, K23;                 // K1 and K2+K3 (assemble.cc:229-233)
+  double robin;                   // K1/bt when bt < 1e10, else 0 (assemble.cc:293)
+};
+
+struct MeshDev {
+  const int *cell_nodes;     // [n_cells*NS]
+  const double *cell_h;      // [n_cells*3]
+  const uint32_t *cell_face; // [n_cells] 3 bits per local face: boundary id + 1 (0 = interior)
+  const uint32_t *node_mask; // [n_nodes] Dirichlet component bits
+  const int *cell_blk;       // [n_cells*NS*NS] BSR block of (a,b); -(blk+1) when this cell writes first
+};
+
+// ---------------------------------------------------------------- assembly
+// One launch per colour; cells[] lists the cells of that colour.
+// J: double* or float* (jfp32), may be null when !do_j.  rhs (system_rhs,
+// = -R) is accumulated and must be zeroed before the first colour.
+hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefTables &T, const MeshDev &M,
+                           const Material &mat, const double *U, const int *cells, int n, void *J, double *rhs,
+                           hipStream_t s);
+// Per-cell free energy into Fcell[cell] for cells [0, n_cells).
+hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
+                         int n_cells, double *Fcell, hipStream_t s);
+
+// --------------------------------------------------------- linear algebra
+// y = J x over BSR(18x18); TJ in {double,float}, TX in {double,float}.
+template <typename TJ, typename TX>
+hipError_t launch_spmv(int n_rows, const int *row_ptr, const int *col, const TJ *val, const TX *x, TX *y,
+                       hipStream_t s);
+// r = b - J x in FP64 arithmetic with an FP32 or FP64 matrix.
+template <typename TJ>
+hipError_t launch_residual_spmv(int n_rows, const int *row_ptr, const int *col, const TJ *val, const double *x,
+                                const double *b, double *r, hipStream_t s);
+// Dinv[r] = inverse of the diagonal block of row r (Gauss-Jordan with
+// partial pivoting in FP64); status[0] set to 1 if a block is singular.
+template <typename TJ>
+hipError_t launch_bj_invert(int n_rows, const int *diag_blk, const TJ *val, TJ *Dinv, int *status,
+                            hipStream_t s);
+// y = Dinv x, block-diagonal apply.
+template <typename TJ, typename TX>
+hipError_t launch_bj_apply(int n_rows, const TJ *Dinv, const TX *x, TX *y, hipStream_t s);
+
+// Deterministic reductions.  partial must hold MDOT_GRID * k values.
+constexpr int MDOT_GRID = 512;
+// out[i] (+)= sum_j V[i*ldv + j] * w[j] for i < k.  accumulate: out += result.
+template <typename T>
+hipError_t launch_mdot(size_t n, int k, const T *V, size_t ldv, const T *w, T *partial, T *out, bool accumulate,
+                       hipStream_t s);
+// w[j] -= sum_i h[i] V[i*ldv + j]
+template <typename T>
+hipError_t launch_sub_lincomb(size_t n, int k, const T *V, size_t ldv, const T *h, T *w, hipStream_t s);
+// w[j] = sum_i y[i] V[i*ldv + j]
+template <typename T>
+hipError_t launch_lincomb(size_t n, int k, const T *V, size_t ldv, const T *y, T *w, hipStream_t s);
+// dst = src / *scal (scal on device; dst = src when *scal == 0)
+template <typename T>
+hipError_t launch_scale_by(size_t n, const T *src, const T *scal, T *dst, hipStream_t s);
+// y += a x (a on host), converting types
+template <typename TX, typename TY>
+hipError_t launch_axpy(size_t n, double a, const TX *x, TY *y, hipStream_t s);
+// dst = src (type conversion)
+template <typename TS, typename TD>
+hipError_t launch_convert(size_t n, const TS *src, TD *dst, hipStream_t s);
+// dst = b - w  (double)
+hipError_t launch_sub(size_t n, const double *b, const double *w, double *dst, hipStream_t s);
+// X = U + a dU, then Dirichlet components of X set to zero (constraints distribute)
+hipError_t launch_trial(int n_nodes, const double *U, const double *dU, double a, const uint32_t *mask, double *X,
+                        hipStream_t s);
+// zero the Dirichlet components of x
+hipError_t launch_mask_zero(int n_nodes, const uint32_t *mask, double *x, hipStream_t s);
+// *p = v (stream-ordered scalar write)
+template <typename T>
+hipError_t launch_set_scalar(T *p, T v, hipStream_t s);
+// out = sum_i v[i] in fixed order (deterministic)
+hipError_t launch_sum(size_t n, const double *v, double *partial, double *out, hipStream_t s);
+
+}  // namespace glh

@@ -1,0 +1,150 @@
+#include "mesh.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace glh {
+
+uint32_t dirichlet_mask(int bid)
+{
+  if (bid == 0) return (1u << NC) - 1u;
+  if (bid >= 2 && bid <= 4) {
+    uint32_t m = 0;
+    int col = bid - 2;
+    for (int c = 0; c < NC; ++c)
+      if ((c % 9) % 3 == col) m |= 1u << c;
+    return m;
+  }
+  return 0u;
+}
+
+int make_box(Mesh &m, int deg, int nx, int ny, int nz, const double lo[3], const double hi[3],
+             const int side_id[6], std::string &err)
+{
+  if (deg != 1 && deg != 2) { err = "degree must be 1 or 2"; return -1; }
+  if (nx < 1 || ny < 1 || nz < 1) { err = "cell counts must be positive"; return -1; }
+  for (int k = 0; k < 3; ++k)
+    if (!(hi[k] > lo[k])) { err = "empty box"; return -1; }
+  m = Mesh();
+  m.deg = deg; m.n1 = deg + 1; m.ns = m.n1 * m.n1 * m.n1;
+  const long Nx = (long)deg * nx + 1, Ny = (long)deg * ny + 1, Nz = (long)deg * nz + 1;
+  const long nn = Nx * Ny * Nz, nc = (long)nx * ny * nz;
+  if (nn >= (1L << 31) / NC || nc * m.ns >= (1L << 31)) { err = "mesh too large for 32-bit indices"; return -1; }
+  m.n_nodes = (int)nn; m.n_cells = (int)nc;
+  const double h[3] = {(hi[0] - lo[0]) / nx, (hi[1] - lo[1]) / ny, (hi[2] - lo[2]) / nz};
+  m.node_xyz.resize(3 * nn);
+  for (long k = 0; k < Nz; ++k)
+    for (long j = 0; j < Ny; ++j)
+      for (long i = 0; i < Nx; ++i) {
+        long n = i + Nx * (j + Ny * k);
+        m.node_xyz[3 * n + 0] = lo[0] + (double)i * h[0] / deg;
+        m.node_xyz[3 * n + 1] = lo[1] + (double)j * h[1] / deg;
+        m.node_xyz[3 * n + 2] = lo[2] + (double)k * h[2] / deg;
+      }
+  m.cell_nodes.resize(nc * m.ns);
+  m.cell_h.resize(3 * nc);
+  m.cell_lo.resize(3 * nc);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        long e = i + (long)nx * (j + (long)ny * k);
+        int a = 0;
+        for (int c = 0; c < m.n1; ++c)
+          for (int b = 0; b < m.n1; ++b)
+            for (int q = 0; q < m.n1; ++q, ++a)
+              m.cell_nodes[e * m.ns + a] =
+                  (int)(((long)deg * i + q) + Nx * (((long)deg * j + b) + Ny * ((long)deg * k + c)));
+        for (int d = 0; d < 3; ++d) m.cell_h[3 * e + d] = h[d];
+        m.cell_lo[3 * e + 0] = lo[0] + i * h[0];
+        m.cell_lo[3 * e + 1] = lo[1] + j * h[1];
+        m.cell_lo[3 * e + 2] = lo[2] + k * h[2];
+        const bool on[6] = {i == 0, i == nx - 1, j == 0, j == ny - 1, k == 0, k == nz - 1};
+        for (int f = 0; f < 6; ++f)
+          if (on[f] && side_id[f] != 1) {
+            m.faces.push_back((int)e);
+            m.faces.push_back(f);
+            m.faces.push_back(side_id[f]);
+          }
+      }
+  // Dirichlet masks: nodes on a side take that side's component mask
+  m.node_mask.assign(nn, 0u);
+  for (long k = 0; k < Nz; ++k)
+    for (long j = 0; j < Ny; ++j)
+      for (long i = 0; i < Nx; ++i) {
+        long n = i + Nx * (j + Ny * k);
+        const bool on[6] = {i == 0, i == Nx - 1, j == 0, j == Ny - 1, k == 0, k == Nz - 1};
+        uint32_t mk = 0;
+        for (int f = 0; f < 6; ++f)
+          if (on[f]) mk |= dirichlet_mask(side_id[f]);
+        m.node_mask[n] = mk;
+      }
+  // colours: parity of the cell index (no two cells of one parity class share a node)
+  std::vector<std::vector<int>> cls(8);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) cls[(i & 1) + 2 * (j & 1) + 4 * (k & 1)].push_back(i + nx * (j + ny * k));
+  m.color_ptr.assign(1, 0);
+  for (auto &c : cls) {
+    if (c.empty()) continue;
+    m.color_cells.insert(m.color_cells.end(), c.begin(), c.end());
+    m.color_ptr.push_back((int)m.color_cells.size());
+  }
+  build_pattern(m);
+  return 0;
+}
+
+void build_pattern(Mesh &m)
+{
+  std::vector<std::vector<int>> nb(m.n_nodes);
+  for (int e = 0; e < m.n_cells; ++e) {
+    const int *cn = &m.cell_nodes[(size_t)e * m.ns];
+    for (int a = 0; a < m.ns; ++a)
+      for (int b = 0; b < m.ns; ++b) nb[cn[a]].push_back(cn[b]);
+  }
+  m.row_ptr.assign(m.n_nodes + 1, 0);
+  m.col.clear();
+  for (int r = 0; r < m.n_nodes; ++r) {
+    auto &v = nb[r];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    m.row_ptr[r + 1] = m.row_ptr[r] + (int)v.size();
+    m.col.insert(m.col.end(), v.begin(), v.end());
+    std::vector<int>().swap(v);
+  }
+}
+
+void greedy_colors(Mesh &m)
+{
+  std::vector<std::vector<int>> node_cells(m.n_nodes);
+  for (int e = 0; e < m.n_cells; ++e)
+    for (int a = 0; a < m.ns; ++a) node_cells[m.cell_nodes[(size_t)e * m.ns + a]].push_back(e);
+  std::vector<int> color(m.n_cells, -1);
+  int ncol = 0;
+  std::vector<int> used;
+  for (int e = 0; e < m.n_cells; ++e) {
+    used.assign(ncol + 1, 0);
+    for (int a = 0; a < m.ns; ++a)
+      for (int o : node_cells[m.cell_nodes[(size_t)e * m.ns + a]])
+        if (color[o] >= 0) used[color[o]] = 1;
+    int c = 0;
+    while (c < ncol && used[c]) ++c;
+    color[e] = c;
+    if (c == ncol) ++ncol;
+  }
+  m.color_ptr.assign(ncol + 1, 0);
+  for (int e = 0; e < m.n_cells; ++e) ++m.color_ptr[color[e] + 1];
+  for (int c = 0; c < ncol; ++c) m.color_ptr[c + 1] += m.color_ptr[c];
+  m.color_cells.assign(m.n_cells, 0);
+  std::vector<int> pos(m.color_ptr.begin(), m.color_ptr.end() - 1);
+  for (int e = 0; e < m.n_cells; ++e) m.color_cells[pos[color[e]]++] = e;
+}
+
+int find_block(const Mesh &m, int r, int s)
+{
+  auto b = m.col.begin() + m.row_ptr[r], e = m.col.begin() + m.row_ptr[r + 1];
+  auto it = std::lower_bound(b, e, s);
+  if (it == e || *it != s) return -1;
+  return (int)(it - m.col.begin());
+}
+
+}  // namespace glh

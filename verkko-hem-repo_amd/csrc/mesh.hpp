@@ -1,0 +1,48 @@
+// Host mesh / DoF / constraint layer (deal.II-free) for axis-aligned hex
+// meshes of FE_Q(1) or FE_Q(2) x 18 components.  It produces the data the
+// reference builds in make_grid()/setup_system() (femgl/src/makegrid_*.cc,
+// femgl/src/setup_uniform_B-phase.cc:125-198) and uploads once per cycle:
+// cell->node maps, geometry, boundary ids, Dirichlet component masks,
+// colour sets for the conflict-free scatter, and the BSR(18x18) pattern.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace glh {
+
+constexpr int NC = 18;
+
+struct Mesh {
+  int deg = 1, n1 = 2, ns = 8;
+  int n_cells = 0, n_nodes = 0;
+  std::vector<int> cell_nodes;      // [n_cells*ns], lexicographic local order
+  std::vector<double> cell_h;       // [n_cells*3]
+  std::vector<double> cell_lo;      // [n_cells*3] lower corner
+  std::vector<double> node_xyz;     // [n_nodes*3]
+  std::vector<int> faces;           // boundary faces: (cell, local face 0..5, boundary id)
+  std::vector<uint32_t> node_mask;  // [n_nodes] bit c = component c Dirichlet-constrained
+  std::vector<int> color_ptr, color_cells;
+  std::vector<int> row_ptr, col;    // BSR pattern over nodes, sorted columns
+};
+
+// Dirichlet component mask of a boundary id: 0 -> all 18 components (A = 0
+// wall), 2/3/4 -> the x/y/z-normal orbital column (femgl/inc/femgl.h:294-301),
+// anything else -> none.
+uint32_t dirichlet_mask(int bid);
+
+// Structured box mesh.  side_id for faces -x,+x,-y,+y,-z,+z.
+int make_box(Mesh &m, int deg, int nx, int ny, int nz, const double lo[3], const double hi[3],
+             const int side_id[6], std::string &err);
+
+// BSR pattern: node pairs sharing a cell (DoFTools::make_sparsity_pattern
+// with full component coupling, setup_uniform_B-phase.cc:190-196).
+void build_pattern(Mesh &m);
+
+// Greedy colouring of cells so that no two cells of one colour share a node.
+void greedy_colors(Mesh &m);
+
+// Index of block (r, s) in the pattern, or -1.
+int find_block(const Mesh &m, int r, int s);
+
+}  // namespace glh

@@ -1,0 +1,227 @@
+"""ctypes binding of libglhip.so (include/glhip.h) for tests and bench.
+
+This is plumbing only: every computation runs in the C++/HIP library.  The
+binding refuses to work without the built library -- there is no CPU
+fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libglhip.so")
+
+GLH_ASM_J, GLH_ASM_R, GLH_ASM_FP32 = 1, 2, 4
+ERRORS = {-1: "bad argument", -2: "HIP error", -3: "bad state", -4: "no convergence", -5: "singular block",
+          -6: "pressure out of range"}
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "glh_matep", "glh_mesh_box",
+           "glh_mesh_destroy", "glh_mesh_sizes", "glh_mesh_get", "glh_upload_mesh", "glh_set_solution",
+           "glh_get_solution", "glh_get_update", "glh_get_rhs", "glh_get_jacobian", "glh_assemble", "glh_rhs_norm",
+           "glh_residual", "glh_solve", "glh_newton_step", "glh_free_energy", "glh_newton_solve",
+           "glh_last_assembly_ms", "glh_last_spmv_ms", "glh_bench_spmv"]
+
+_lib = None
+
+
+class GLHError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GLHError("libglhip.so is not built: run __graft_entry__.build() (no CPU fallback exists)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.glh_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.glh_destroy.argtypes = [vp]
+    L.glh_last_error.argtypes = [vp]
+    L.glh_last_error.restype = C.c_char_p
+    L.glh_set_material.argtypes = [vp, C.c_double, _dp, _dp, C.c_double]
+    L.glh_matep.argtypes = [C.c_double, C.c_double, C.c_int, _dp]
+    L.glh_mesh_box.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, C.POINTER(vp)]
+    L.glh_mesh_destroy.argtypes = [vp]
+    L.glh_mesh_sizes.argtypes = [vp, _i64p]
+    L.glh_mesh_get.argtypes = [vp] + [vp] * 9
+    L.glh_upload_mesh.argtypes = [vp, vp]
+    for f in ("glh_set_solution", "glh_get_solution", "glh_get_update", "glh_get_rhs", "glh_get_jacobian"):
+        getattr(L, f).argtypes = [vp, _dp]
+    L.glh_assemble.argtypes = [vp, C.c_int]
+    L.glh_rhs_norm.argtypes = [vp, C.POINTER(C.c_double)]
+    L.glh_residual.argtypes = [vp, C.POINTER(C.c_double)]
+    L.glh_solve.argtypes = [vp, C.c_double, C.c_int, C.c_int, _dp]
+    L.glh_newton_step.argtypes = [vp, C.c_double, C.c_int, _dp]
+    L.glh_free_energy.argtypes = [vp, C.POINTER(C.c_double)]
+    L.glh_newton_solve.argtypes = [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
+                                   C.c_int, _dp]
+    L.glh_last_assembly_ms.argtypes = [vp, _dp]
+    L.glh_last_spmv_ms.argtypes = [vp, _dp]
+    L.glh_bench_spmv.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+    _lib = L
+    return L
+
+
+def matep(p, t, scc=True):
+    out = np.zeros(12)
+    rc = lib().glh_matep(float(p), float(t), int(scc), out)
+    return rc, out
+
+
+K_CONST = (0.42072, 0.42072, 0.42072)  # femgl/inc/femgl.h:320-322
+
+
+class Mesh:
+    """Host mesh layer object (glh_mesh) with numpy views of its arrays."""
+
+    def __init__(self, degree, n, lo, hi, side_id):
+        self.h = C.c_void_p()
+        rc = lib().glh_mesh_box(int(degree), int(n[0]), int(n[1]), int(n[2]), np.asarray(lo, np.float64),
+                                np.asarray(hi, np.float64), np.asarray(side_id, np.int32), C.byref(self.h))
+        if rc:
+            raise GLHError("glh_mesh_box failed (%d)" % rc)
+        s = np.zeros(8, np.int64)
+        lib().glh_mesh_sizes(self.h, s)
+        self.deg, self.n_cells, self.n_nodes, self.n_faces, self.nnzb, self.n_colors = [int(v) for v in s[:6]]
+        self.ns = (self.deg + 1) ** 3
+
+    def arrays(self):
+        a = dict(cell_nodes=np.zeros(self.n_cells * self.ns, np.int32), cell_h=np.zeros(self.n_cells * 3),
+                 node_xyz=np.zeros(self.n_nodes * 3), faces=np.zeros(max(1, self.n_faces * 3), np.int32),
+                 node_mask=np.zeros(self.n_nodes, np.uint32), color_ptr=np.zeros(self.n_colors + 1, np.int32),
+                 color_cells=np.zeros(self.n_cells, np.int32), row_ptr=np.zeros(self.n_nodes + 1, np.int32),
+                 col=np.zeros(self.nnzb, np.int32))
+        lib().glh_mesh_get(self.h, *[v.ctypes.data_as(C.c_void_p) for v in a.values()])
+        a["faces"] = a["faces"][:self.n_faces * 3]
+        return a
+
+    def close(self):
+        if self.h:
+            lib().glh_mesh_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device context (glh_ctx): the FemGL step methods on the GPU."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        rc = lib().glh_create(int(device), C.byref(self.h))
+        if rc:
+            raise GLHError("glh_create failed (%d): is a GPU visible?" % rc)
+        self.n_dofs = 0
+        self.nnzb = 0
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = lib().glh_last_error(self.h)
+            raise GLHError("%s failed (%d, %s): %s" % (what, rc, ERRORS.get(rc, "?"),
+                                                      msg.decode() if msg else ""))
+
+    def set_material(self, alpha, beta, K=K_CONST, bt=1.0e10):
+        self._chk(lib().glh_set_material(self.h, float(alpha), np.asarray(beta, np.float64),
+                                         np.asarray(K, np.float64), float(bt)), "glh_set_material")
+
+    def set_material_pt(self, p, t, scc=True, bt=1.0e10, K=K_CONST):
+        rc, m = matep(p, t, scc)
+        if rc:
+            raise GLHError("glh_matep: pressure %g outside the strong-coupling table" % p)
+        self.set_material(m[0], m[1:6], K, bt)
+        return m
+
+    def upload_mesh(self, mesh):
+        self._chk(lib().glh_upload_mesh(self.h, mesh.h), "glh_upload_mesh")
+        self.n_dofs = mesh.n_nodes * 18
+        self.nnzb = mesh.nnzb
+
+    def set_solution(self, U):
+        self._chk(lib().glh_set_solution(self.h, np.ascontiguousarray(U, np.float64)), "glh_set_solution")
+
+    def _get(self, fn, n):
+        out = np.zeros(n)
+        self._chk(getattr(lib(), fn)(self.h, out), fn)
+        return out
+
+    def solution(self):
+        return self._get("glh_get_solution", self.n_dofs)
+
+    def update(self):
+        return self._get("glh_get_update", self.n_dofs)
+
+    def rhs(self):
+        return self._get("glh_get_rhs", self.n_dofs)
+
+    def jacobian(self):
+        return self._get("glh_get_jacobian", self.nnzb * 324).reshape(-1, 18, 18)
+
+    def assemble(self, J=True, R=True, fp32=False):
+        flags = (GLH_ASM_J if J else 0) | (GLH_ASM_R if R else 0) | (GLH_ASM_FP32 if fp32 else 0)
+        self._chk(lib().glh_assemble(self.h, flags), "glh_assemble")
+
+    def rhs_norm(self):
+        v = C.c_double()
+        self._chk(lib().glh_rhs_norm(self.h, C.byref(v)), "glh_rhs_norm")
+        return v.value
+
+    def residual_norm(self):
+        v = C.c_double()
+        self._chk(lib().glh_residual(self.h, C.byref(v)), "glh_residual")
+        return v.value
+
+    def solve(self, rtol=0.1, maxit=10000, restart=30):
+        st = np.zeros(2)
+        self._chk(lib().glh_solve(self.h, rtol, maxit, restart, st), "glh_solve")
+        return int(st[0]), st[1]
+
+    def newton_step(self, step=0.83, damped=True):
+        out = np.zeros(4)
+        self._chk(lib().glh_newton_step(self.h, step, int(damped), out), "glh_newton_step")
+        return out
+
+    def free_energy(self):
+        v = C.c_double()
+        self._chk(lib().glh_free_energy(self.h, C.byref(v)), "glh_free_energy")
+        return v.value
+
+    def newton_solve(self, rtol=1e-10, max_newton=50, lin_tol=0.1, lin_maxit=20000, restart=30, ls_step=0.83,
+                     damped=True, fp32=False):
+        st = np.zeros(8)
+        self._chk(lib().glh_newton_solve(self.h, rtol, max_newton, lin_tol, lin_maxit, restart, ls_step,
+                                         int(damped), int(fp32), st), "glh_newton_solve")
+        keys = ["newton_steps", "gmres_its", "r0", "r_final", "t_assemble", "t_solve", "t_linesearch", "F"]
+        return dict(zip(keys, st.tolist()))
+
+    def last_assembly_ms(self):
+        v = np.zeros(2)
+        self._chk(lib().glh_last_assembly_ms(self.h, v), "glh_last_assembly_ms")
+        return v[0]
+
+    def bench_spmv(self, k=10):
+        v = C.c_double()
+        self._chk(lib().glh_bench_spmv(self.h, int(k), C.byref(v)), "glh_bench_spmv")
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().glh_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
