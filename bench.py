@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Headline benchmark: converged Ginzburg-Landau Newton-GMRES solves per hour.
+
+One step = one full plain-FP64 Newton solve (assembly of J and -R, GMRES +
+18x18 point-block Jacobi, damped line search) of the order-parameter problem
+from the seeded initial guess to ||R|| / ||R0|| <= 1e-10, entirely on the GPU
+through the glhip C ABI.  Default workload: 3D cube [-20,20]^3, FE_Q(2) x 18
+components, 32^3 cells (4 943 250 DoFs), T/Tc = 0.7, p = 26 bar, A = 0 on the
+two z faces, B phase + 0.05 Delta_B uniform noise (seed 0).
+
+Also reports, measured live with HIP events on the library's stream over the
+timed region: the SpMV roofline (the dominant kernel: HBM-bound), the
+assembly rate (GDoF/s) and a CPU baseline: the oracle (a CPU restatement of
+the reference algorithm) timed on a bounded sample on the host cores and
+extrapolated with the GPU run's own Newton/GMRES counts.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "converged GL solves/hr @ ||F||<=1e-10 vs plain FP64; assembly GDoF/s, HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (degree, cells per side, half side, p [bar], T/Tc)
+    "cube_q2_ref5": (2, 32, 20.0, 26.0, 0.7),
+    "cube_q1_ref3": (1, 8, 10.0, 20.0, 0.5),
+}
+
+
+def load_glhip():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("glhip", os.path.join(ROOT, "verkko-hem-repo_amd", "glhip.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def spmv_bytes(n_nodes, nnzb, val_bytes, vec_bytes):
+    """Algorithmic HBM bytes of one BSR(18x18) SpMV y = J x: every block value
+    once, one column index per block, the row pointer, x read once and y
+    written once."""
+    return nnzb * 324 * val_bytes + nnzb * 4 + (n_nodes + 1) * 4 + 2 * n_nodes * 18 * vec_bytes
+
+
+def cpu_baseline(deg, h, mat, n_cells, n_nodes, nnzb, newton_steps, gmres_its, ls_trials, restart, budget_s=25.0):
+    """CPU baseline of the same algorithm (the oracle, kind 'port'), timed on
+    a bounded sample with OpenMP on the host cores, extrapolated with the
+    GPU run's Newton and GMRES counts."""
+    import oracle_lib as O
+    L = O.lib()
+    L.orc_time_cells.argtypes = [C.c_int, C.c_int, O._dp, O._dp, O._dp, C.c_int, C.c_int, O._dp]
+    L.orc_time_cells.restype = C.c_double
+    L.orc_time_spmv.argtypes = [C.c_int, O._ip, O._ip, O._dp, O._dp, O._dp, C.c_int, C.c_int]
+    L.orc_time_spmv.restype = C.c_double
+    L.orc_max_threads.restype = C.c_int
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or L.orc_max_threads()
+    co = O.f64(mat)
+    dpc = 18 * (deg + 1) ** 3
+    rng = np.random.default_rng(7)
+    hv = O.f64([h, h, h])
+    cs = np.zeros(1)
+    # factorized oracle (one Hessian per quadrature point), J+R per cell
+    ns_f = max(threads, 64 if deg == 2 else 512)
+    Uf = rng.standard_normal(ns_f * dpc) * 0.5
+    t_fac = L.orc_time_cells(deg, ns_f, hv, co, Uf, 0, threads, cs) / ns_f
+    # literal per-(i,j,q) restatement of assemble.cc:177-280, one cell per thread
+    ns_l = threads
+    Ul = rng.standard_normal(ns_l * dpc) * 0.5
+    t_lit = L.orc_time_cells(deg, ns_l, hv, co, Ul, 1, threads, cs) / ns_l
+    # BSR SpMV on a sample matrix of the same block structure (values random)
+    G = load_glhip()
+    ns_side = 8 if deg == 2 else 16
+    m = G.Mesh(deg, (ns_side,) * 3, (0, 0, 0), (1, 1, 1), [1] * 6)
+    a = m.arrays()
+    val = rng.standard_normal(m.nnzb * 324)
+    x = rng.standard_normal(m.n_nodes * 18)
+    y = np.zeros_like(x)
+    L.orc_time_spmv(m.n_nodes, a["row_ptr"], a["col"], val, x, y, 1, threads)
+    reps = 5
+    t_sp = L.orc_time_spmv(m.n_nodes, a["row_ptr"], a["col"], val, x, y, reps, threads) / reps
+    t_spmv_full = t_sp * (nnzb / m.nnzb)
+    # GMRES iteration = SpMV + preconditioner + CGS2 vector passes; the vector
+    # passes are counted by their bytes relative to the SpMV's
+    n = n_nodes * 18
+    vec_bytes = (4 * (restart / 2 + 1) + 6) * n * 8
+    t_iter = t_spmv_full * (1.0 + vec_bytes / spmv_bytes(n_nodes, nnzb, 8, 8))
+    t_res = 0.25 * t_fac  # residual-only cell work is about a quarter of J+R
+    solve_fac = newton_steps * t_fac * n_cells + (ls_trials + 1) * t_res * n_cells + gmres_its * t_iter
+    solve_lit = newton_steps * t_lit * n_cells + (ls_trials + 1) * t_res * n_cells + gmres_its * t_iter
+    return {
+        "value": 3600.0 / solve_lit,
+        "unit": "solves/hr",
+        "cores": threads,
+        "kind": "port",
+        "sample": ("literal per-(i,j,q) cell loop of assemble.cc:177-280 on %d sample cells (%.2f s/cell/core), "
+                   "GMRES iterations from a BSR SpMV on a %d^3-cell sample of the same block structure "
+                   "(%.3f s per full-size SpMV), extrapolated with the GPU run's %d Newton steps, %d GMRES "
+                   "iterations and %d line-search residuals" % (ns_l, t_lit * threads, ns_side, t_spmv_full,
+                                                                newton_steps, gmres_its, ls_trials)),
+        "value_factorized": 3600.0 / solve_fac,
+        "factorized_s_per_cell_core": t_fac * threads,
+        "extrapolated": True,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cube_q2_ref5", choices=sorted(WORKLOADS))
+    ap.add_argument("--lin-tol", type=float, default=0.1)
+    ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    G = load_glhip()
+    deg, ncell, half, p, t = WORKLOADS[args.workload]
+    print("bench: building %s" % args.workload, file=sys.stderr, flush=True)
+    mesh = G.Mesh(deg, (ncell,) * 3, (-half,) * 3, (half,) * 3, [1, 1, 1, 1, 0, 0])
+    ctx = G.Context(0)
+    m = ctx.set_material_pt(p, t)
+    ctx.upload_mesh(mesh)
+    U0 = G.initial_guess(mesh.n_nodes, m[9], phase=0, amp=0.05, seed=0)
+    ctx.set_solution(U0)
+    ctx.store_initial()
+    n_dofs = mesh.n_nodes * 18
+    kw = dict(rtol=1e-10, max_newton=100, lin_tol=args.lin_tol, lin_maxit=100000, restart=args.restart)
+    for w in range(args.warmup):
+        ctx.restore_initial()
+        st = ctx.newton_solve(**kw)
+        print("bench: warmup %d: %s" % (w, st), file=sys.stderr, flush=True)
+    ctx.set_timing(True)
+    runs = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ctx.restore_initial()
+        runs.append(ctx.newton_solve(**kw))
+    elapsed = time.perf_counter() - t0
+    tm = ctx.timing()
+    for k, st in enumerate(runs):
+        print("bench: step %d: %s" % (k, st), file=sys.stderr, flush=True)
+    converged = all(st["r_final"] <= 1e-10 * st["r0"] for st in runs)
+    ms_step = 1000.0 * elapsed / args.steps
+    spmv_avg_ms = tm["spmv_ms"] / max(1, tm["spmv_n"])
+    traffic_alg = spmv_bytes(mesh.n_nodes, mesh.nnzb, 8, 8)
+    achieved = traffic_alg / (spmv_avg_ms * 1e-3) / 1e9
+    jasm_ms = tm["jasm_ms"] / max(1, tm["jasm_n"])
+    st0 = runs[0]
+    line = {
+        "metric": METRIC,
+        "value": 3600.0 * args.steps / elapsed if converged else 0.0,
+        "unit": "solves/hr",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "GL Newton-GMRES solve, Q%d x 18 on [-%g,%g]^3, %d^3 cells, %d DoFs, T/Tc=%g, "
+                               "p=%g bar, FP64 J + GMRES(%d) + point-block Jacobi, eta=%g, ||R||/||R0||<=1e-10"
+                               % (deg, half, half, ncell, n_dofs, t, p, args.restart, args.lin_tol),
+                   "name": args.workload, "dofs": n_dofs, "cells": mesh.n_cells, "nnz": mesh.nnzb * 324},
+        "converged": converged,
+        "newton_steps": st0["newton_steps"],
+        "gmres_its": st0["gmres_its"],
+        "free_energy": st0["F"],
+        "assembly_ms": jasm_ms,
+        "assembly_gdof_s": n_dofs / (jasm_ms * 1e-3) / 1e9 if jasm_ms > 0 else None,
+        "assembly_hbm_gb_s": (mesh.nnzb * 324 * 8 + 2 * n_dofs * 8) / (jasm_ms * 1e-3) / 1e9 if jasm_ms else None,
+        "spmv_ms": spmv_avg_ms,
+        "spmv_launches": tm["spmv_n"],
+        "roofline": {"kernel": "k_spmv<double,double> (BSR 18x18 SpMV in GMRES)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": traffic_alg},
+    }
+    if not args.no_cpu_baseline:
+        ls = int(st0.get("ls_trials", st0["newton_steps"]))
+        line["cpu_baseline"] = cpu_baseline(deg, 2 * half / ncell, list(ctx_coef(m)), mesh.n_cells, mesh.n_nodes,
+                                            mesh.nnzb, int(st0["newton_steps"]), int(st0["gmres_its"]), ls,
+                                            args.restart)
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+def ctx_coef(m):
+    K = 0.42072
+    return [m[0], m[1], m[2], m[3], m[4], m[5], K, K, K, 1.0e10]
+
+
+if __name__ == "__main__":
+    main()

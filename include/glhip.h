@@ -87,7 +87,16 @@ int glh_mesh_get(const glh_mesh *m, int *cell_nodes, double *cell_h, double *nod
 int glh_upload_mesh(glh_ctx *ctx, const glh_mesh *m);
 
 /* --------------------------------------------------------- solution I/O */
+/* Seeded initial guess on the host (setup_uniform_B-phase.cc:200-265 for
+ * phase 0, setup_uniform_A-phase.cc:225-262 for phase 1): the bulk phase of
+ * gap `gap` plus amp*gap*(2u-1), u = (mt19937_64(seed) >> 11) * 2^-53 for
+ * every dof in node-major order.  U[n_nodes*18]. */
+int glh_initial_guess_host(int n_nodes, double gap, int phase, double amp, uint64_t seed, double *U);
 int glh_set_solution(glh_ctx *ctx, const double *U_host);       /* local_solution */
+/* keep a device copy of the current solution / restore it (resets a solve
+ * without a host transfer) */
+int glh_store_initial(glh_ctx *ctx);
+int glh_restore_initial(glh_ctx *ctx);
 int glh_get_solution(glh_ctx *ctx, double *U_host);
 int glh_get_update(glh_ctx *ctx, double *dU_host);              /* locally_relevant_newton_solution */
 int glh_get_rhs(glh_ctx *ctx, double *rhs_host);                /* system_rhs (= -R) */
@@ -127,15 +136,21 @@ int glh_free_energy(glh_ctx *ctx, double *F);
  * residual, the update and the line search stay FP64.
  * stats: [0] Newton steps, [1] total GMRES iterations, [2] ||R0||,
  * [3] final ||R||, [4] seconds assembling, [5] seconds solving,
- * [6] seconds in line search, [7] free energy. */
+ * [6] seconds in line search, [7] free energy, [8] line-search residual
+ * evaluations, [9] 1 if converged. */
 int glh_newton_solve(glh_ctx *ctx, double rtol, int max_newton, double lin_tol, int lin_maxit, int restart,
-                     double ls_step, int damped, int fp32, double stats[8]);
+                     double ls_step, int damped, int fp32, double stats[10]);
 
 /* timing of the last glh_assemble Jacobian launches on the context stream
  * (HIP events): ms[0] = J+R assembly ms, ms[1] = number of launches. */
 int glh_last_assembly_ms(glh_ctx *ctx, double ms[2]);
-/* average SpMV kernel ms over the last glh_solve (HIP events around each launch). */
-int glh_last_spmv_ms(glh_ctx *ctx, double ms[2]);
+/* Live kernel timing with HIP events on the context stream.  on != 0 resets
+ * the counters and records an event pair around every SpMV launch and every
+ * assembly sweep from then on.  out: [0] total SpMV ms, [1] SpMV launches,
+ * [2] total Jacobian(+rhs) assembly ms, [3] such sweeps, [4] total
+ * residual-only assembly ms, [5] such sweeps. */
+int glh_set_timing(glh_ctx *ctx, int on);
+int glh_timing(glh_ctx *ctx, double out[6]);
 /* Run k back-to-back SpMV launches y = J x with x = current solution and
  * time them with HIP events on the context stream: ms_avg. */
 int glh_bench_spmv(glh_ctx *ctx, int k, double *ms_avg);

@@ -1254,3 +1254,84 @@ int orc_newton(int deg, int n_cells, const int *cell_nodes, const double *cell_h
   free(val); free(rhs); free(dU); free(xt); free(rt);
   return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* CPU-baseline timers (bench.py cpu_baseline leg): the oracle's cell  */
+/* assembly and GMRES kernels over a bounded sample, OpenMP over cells */
+/* and rows on the host cores.                                         */
+/* ------------------------------------------------------------------ */
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+static double wall(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* Local J and R of n sample cells (Uloc_all[n*dpc], h[3] shared), in
+ * parallel over cells.  Returns wall seconds; checksum[0] receives a sum of
+ * the results so the work cannot be elided. */
+double orc_time_cells(int deg, int n, const double *h, const double *co, const double *Uloc_all, int faithful,
+                      int nthreads, double *checksum)
+{
+  int ns = (deg + 1) * (deg + 1) * (deg + 1), dpc = ns * NC;
+  double t0 = wall(), sum = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel reduction(+ : sum)
+#endif
+  {
+    double *J = (double *)malloc(sizeof(double) * dpc * dpc), *R = (double *)malloc(sizeof(double) * dpc);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (int e = 0; e < n; ++e) {
+      if (faithful) orc_cell_faithful(deg, h, co, Uloc_all + (size_t)e * dpc, J, R);
+      else orc_cell_factorized(deg, h, co, Uloc_all + (size_t)e * dpc, J, R);
+      sum += J[(size_t)(e % dpc) * dpc] + R[e % dpc];
+    }
+    free(J); free(R);
+  }
+  if (checksum) checksum[0] = sum;
+  return wall() - t0;
+}
+
+/* reps BSR SpMVs y = A x, parallel over block rows; returns wall seconds */
+double orc_time_spmv(int n_nodes, const int *row_ptr, const int *col, const double *val, const double *x, double *y,
+                     int reps, int nthreads)
+{
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  double t0 = wall();
+  for (int it = 0; it < reps; ++it) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int r = 0; r < n_nodes; ++r) {
+      double acc[NC] = {0};
+      for (int b = row_ptr[r]; b < row_ptr[r + 1]; ++b) {
+        const double *B = val + (size_t)b * NC * NC, *xc = x + (size_t)col[b] * NC;
+        for (int c = 0; c < NC; ++c) {
+          double s = 0;
+          for (int d = 0; d < NC; ++d) s += B[c * NC + d] * xc[d];
+          acc[c] += s;
+        }
+      }
+      memcpy(y + (size_t)r * NC, acc, sizeof acc);
+    }
+  }
+  return wall() - t0;
+}
+
+int orc_max_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

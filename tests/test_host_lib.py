@@ -89,3 +89,12 @@ def test_mesh_argument_errors():
         G.Mesh(3, (2, 2, 2), (0, 0, 0), (1, 1, 1), [1] * 6)
     with pytest.raises(G.GLHError):
         G.Mesh(1, (0, 2, 2), (0, 0, 0), (1, 1, 1), [1] * 6)
+
+
+@pytest.mark.parametrize("phase", [0, 1])
+def test_initial_guess_bitwise_equal_to_oracle(phase):
+    gap = O.matep(20.0, 0.5)["gap_B" if phase == 0 else "gap_A"]
+    for seed in (0, 1, 12345):
+        a = G.initial_guess(97, gap, phase=phase, amp=0.05, seed=seed)
+        b = O.initial_guess(97, gap, phase=phase, amp=0.05, seed=seed)
+        assert np.array_equal(a, b)

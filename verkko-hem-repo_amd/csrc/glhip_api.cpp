@@ -65,7 +65,7 @@ struct glh_ctx {
   DBuf<double> cell_h;
   DBuf<uint32_t> cell_face, node_mask;
   // state vectors (FP64)
-  DBuf<double> U, dU, X, rhs, res, Fcell, red_partial, red_out;
+  DBuf<double> U, U0, dU, X, rhs, res, Fcell, red_partial, red_out;
   // Jacobian and preconditioner
   DBuf<double> J64, D64;
   DBuf<float> J32, D32;
@@ -78,8 +78,13 @@ struct glh_ctx {
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double asm_ms = 0.0, asm_launches = 0;
+  // live kernel timing (glh_set_timing): event pairs around each SpMV
+  // launch on the context stream, read back when the pool fills or a solve ends
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  int ev_used = 0;
   double spmv_ms_total = 0.0, spmv_count = 0;
-  bool time_spmv = false;
+  double jasm_ms_total = 0.0, jasm_count = 0, rasm_ms_total = 0.0, rasm_count = 0;
   double last_rhs_norm = -1.0;
 
   glh::RefTables tables() const
@@ -242,6 +247,7 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
   if (do_r) HIPCHK(c, hipMemsetAsync(rhs_out, 0, ndofs(c) * sizeof(double), c->stream));
   const glh::RefTables T = c->tables();
   const glh::MeshDev M = c->meshdev();
+  timed = timed || c->timing;
   if (timed) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   const int ncol = (int)c->color_ptr.size() - 1;
   for (int k = 0; k < ncol; ++k) {
@@ -256,6 +262,10 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->asm_ms = ms;
     c->asm_launches = ncol;
+    if (c->timing) {
+      if (do_j) { c->jasm_ms_total += ms; c->jasm_count += 1; }
+      else { c->rasm_ms_total += ms; c->rasm_count += 1; }
+    }
   }
   if (do_j) {
     c->j_fp32 = fp32;
@@ -307,18 +317,35 @@ struct Work<float> {
   static DBuf<float> &y(glh_ctx *c) { return c->gy32; }
 };
 
+int flush_events(glh_ctx *c)
+{
+  if (c->ev_used == 0) return GLH_OK;
+  HIPCHK(c, hipEventSynchronize(c->ev_pool[2 * c->ev_used - 1]));
+  for (int i = 0; i < c->ev_used; ++i) {
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev_pool[2 * i], c->ev_pool[2 * i + 1]));
+    c->spmv_ms_total += ms;
+    c->spmv_count += 1;
+  }
+  c->ev_used = 0;
+  return GLH_OK;
+}
+
 template <typename TJ, typename TV>
 int spmv(glh_ctx *c, const TJ *J, const TV *x, TV *y)
 {
-  if (c->time_spmv) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  const bool t = c->timing && !c->ev_pool.empty();
+  if (t) {
+    if (2 * c->ev_used >= (int)c->ev_pool.size()) {
+      int rc = flush_events(c);
+      if (rc) return rc;
+    }
+    HIPCHK(c, hipEventRecord(c->ev_pool[2 * c->ev_used], c->stream));
+  }
   HIPCHK(c, glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, x, y, c->stream));
-  if (c->time_spmv) {
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    HIPCHK(c, hipEventSynchronize(c->ev1));
-    float ms = 0;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->spmv_ms_total += ms;
-    c->spmv_count += 1;
+  if (t) {
+    HIPCHK(c, hipEventRecord(c->ev_pool[2 * c->ev_used + 1], c->stream));
+    ++c->ev_used;
   }
   return GLH_OK;
 }
@@ -431,6 +458,10 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, double rt
     HIPCHK(c, glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, W.p, Z.p, s));
     HIPCHK(c, glh::launch_axpy<TV, TV>(n, 1.0, Z.p, x, s));
   }
+  {
+    int rc = flush_events(c);
+    if (rc) return rc;
+  }
   *its_out = it;
   *rel_out = bn > 0 ? res / bn : 0.0;
   return conv ? GLH_OK : fail(c, GLH_ERR_NOCONV, "GMRES reached the maximum iteration number");
@@ -444,8 +475,6 @@ int solve_impl(glh_ctx *c, double rtol, int maxit, int restart, double stats[2])
   if (rc) return rc;
   const size_t n = ndofs(c);
   hipStream_t s = c->stream;
-  c->spmv_ms_total = 0;
-  c->spmv_count = 0;
   int its = 0;
   double rel = 0;
   if (!c->j_fp32) {
@@ -539,7 +568,7 @@ int glh_destroy(glh_ctx *c)
   if (!c) return GLH_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  DBuf<double> *dbs[] = {&c->tN, &c->tdN, &c->tw, &c->tWp, &c->tSp, &c->tMf, &c->cell_h, &c->U, &c->dU, &c->X,
+  DBuf<double> *dbs[] = {&c->tN, &c->tdN, &c->tw, &c->tWp, &c->tSp, &c->tMf, &c->cell_h, &c->U, &c->U0, &c->dU, &c->X,
                          &c->rhs, &c->res, &c->Fcell, &c->red_partial, &c->red_out, &c->J64, &c->D64, &c->gV64,
                          &c->gZ64, &c->gW64, &c->gh64, &c->gp64, &c->gy64, &c->ir_r, &c->ir_c};
   for (auto *b : dbs) b->release();
@@ -551,6 +580,8 @@ int glh_destroy(glh_ctx *c)
   for (auto *b : ibs) b->release();
   c->cell_face.release();
   c->node_mask.release();
+  for (auto e : c->ev_pool)
+    if (e) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -701,6 +732,26 @@ int glh_set_solution(glh_ctx *c, const double *Uh)
   return GLH_OK;
 }
 
+int glh_store_initial(glh_ctx *c)
+{
+  int rc = check_ready(c, false);
+  if (rc) return rc;
+  HIPCHK(c, c->U0.alloc(ndofs(c)));
+  HIPCHK(c, hipMemcpyAsync(c->U0.p, c->U.p, ndofs(c) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GLH_OK;
+}
+
+int glh_restore_initial(glh_ctx *c)
+{
+  int rc = check_ready(c, false);
+  if (rc) return rc;
+  if (!c->U0.p || c->U0.n < ndofs(c)) return fail(c, GLH_ERR_STATE, "no stored initial state");
+  HIPCHK(c, hipMemcpyAsync(c->U.p, c->U0.p, ndofs(c) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GLH_OK;
+}
+
 static int get_vec(glh_ctx *c, const double *src, double *dst)
 {
   int rc = check_ready(c, false);
@@ -791,13 +842,13 @@ int glh_free_energy(glh_ctx *c, double *F)
 }
 
 int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, int lin_maxit, int restart,
-                     double ls_step, int damped, int fp32, double stats[8])
+                     double ls_step, int damped, int fp32, double stats[10])
 {
   int rc = check_ready(c);
   if (rc) return rc;
   using clk = std::chrono::steady_clock;
   double t_asm = 0, t_sol = 0, t_ls = 0;
-  int lin_total = 0, k = 0;
+  int lin_total = 0, k = 0, trials = 0;
   auto t0 = clk::now();
   rc = assemble_at(c, c->U.p, false, true, false, c->rhs.p, false);
   if (rc) return rc;
@@ -825,6 +876,7 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
     rc = newton_step_impl(c, ls_step, damped, out);
     if (rc) return rc;
     rcur = out[1];
+    trials += (int)out[3];
     t_ls += std::chrono::duration<double>(clk::now() - t2).count();
   }
   double F = 0;
@@ -833,6 +885,7 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
   if (stats) {
     stats[0] = k; stats[1] = lin_total; stats[2] = r0; stats[3] = rcur;
     stats[4] = t_asm; stats[5] = t_sol; stats[6] = t_ls; stats[7] = F;
+    stats[8] = trials; stats[9] = (rcur <= rtol * r0) ? 1.0 : 0.0;
   }
   return GLH_OK;
 }
@@ -845,11 +898,29 @@ int glh_last_assembly_ms(glh_ctx *c, double ms[2])
   return GLH_OK;
 }
 
-int glh_last_spmv_ms(glh_ctx *c, double ms[2])
+int glh_set_timing(glh_ctx *c, int on)
 {
-  if (!c || !ms) return GLH_ERR_ARG;
-  ms[0] = c->spmv_count > 0 ? c->spmv_ms_total / c->spmv_count : 0.0;
-  ms[1] = c->spmv_count;
+  if (!c) return GLH_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (on && c->ev_pool.empty()) {
+    c->ev_pool.resize(2 * 1024, nullptr);
+    for (auto &e : c->ev_pool) HIPCHK(c, hipEventCreate(&e));
+  }
+  c->timing = on != 0;
+  c->ev_used = 0;
+  c->spmv_ms_total = c->spmv_count = 0;
+  c->jasm_ms_total = c->jasm_count = c->rasm_ms_total = c->rasm_count = 0;
+  return GLH_OK;
+}
+
+int glh_timing(glh_ctx *c, double out[6])
+{
+  if (!c || !out) return GLH_ERR_ARG;
+  int rc = flush_events(c);
+  if (rc) return rc;
+  out[0] = c->spmv_ms_total; out[1] = c->spmv_count;
+  out[2] = c->jasm_ms_total; out[3] = c->jasm_count;
+  out[4] = c->rasm_ms_total; out[5] = c->rasm_count;
   return GLH_OK;
 }
 

@@ -22,10 +22,10 @@ _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
 EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "glh_matep", "glh_mesh_box",
-           "glh_mesh_destroy", "glh_mesh_sizes", "glh_mesh_get", "glh_upload_mesh", "glh_set_solution",
+           "glh_mesh_destroy", "glh_mesh_sizes", "glh_mesh_get", "glh_upload_mesh", "glh_initial_guess_host", "glh_set_solution", "glh_store_initial", "glh_restore_initial",
            "glh_get_solution", "glh_get_update", "glh_get_rhs", "glh_get_jacobian", "glh_assemble", "glh_rhs_norm",
            "glh_residual", "glh_solve", "glh_newton_step", "glh_free_energy", "glh_newton_solve",
-           "glh_last_assembly_ms", "glh_last_spmv_ms", "glh_bench_spmv"]
+           "glh_last_assembly_ms", "glh_set_timing", "glh_timing", "glh_bench_spmv"]
 
 _lib = None
 
@@ -55,6 +55,9 @@ def lib():
     L.glh_upload_mesh.argtypes = [vp, vp]
     for f in ("glh_set_solution", "glh_get_solution", "glh_get_update", "glh_get_rhs", "glh_get_jacobian"):
         getattr(L, f).argtypes = [vp, _dp]
+    L.glh_initial_guess_host.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_uint64, _dp]
+    L.glh_store_initial.argtypes = [vp]
+    L.glh_restore_initial.argtypes = [vp]
     L.glh_assemble.argtypes = [vp, C.c_int]
     L.glh_rhs_norm.argtypes = [vp, C.POINTER(C.c_double)]
     L.glh_residual.argtypes = [vp, C.POINTER(C.c_double)]
@@ -64,7 +67,8 @@ def lib():
     L.glh_newton_solve.argtypes = [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
                                    C.c_int, _dp]
     L.glh_last_assembly_ms.argtypes = [vp, _dp]
-    L.glh_last_spmv_ms.argtypes = [vp, _dp]
+    L.glh_set_timing.argtypes = [vp, C.c_int]
+    L.glh_timing.argtypes = [vp, _dp]
     L.glh_bench_spmv.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     _lib = L
     return L
@@ -77,6 +81,14 @@ def matep(p, t, scc=True):
 
 
 K_CONST = (0.42072, 0.42072, 0.42072)  # femgl/inc/femgl.h:320-322
+
+
+def initial_guess(n_nodes, gap, phase=0, amp=0.05, seed=0):
+    U = np.zeros(n_nodes * 18)
+    rc = lib().glh_initial_guess_host(int(n_nodes), float(gap), int(phase), float(amp), int(seed), U)
+    if rc:
+        raise GLHError("glh_initial_guess_host failed (%d)" % rc)
+    return U
 
 
 class Mesh:
@@ -151,6 +163,12 @@ class Context:
     def set_solution(self, U):
         self._chk(lib().glh_set_solution(self.h, np.ascontiguousarray(U, np.float64)), "glh_set_solution")
 
+    def store_initial(self):
+        self._chk(lib().glh_store_initial(self.h), "glh_store_initial")
+
+    def restore_initial(self):
+        self._chk(lib().glh_restore_initial(self.h), "glh_restore_initial")
+
     def _get(self, fn, n):
         out = np.zeros(n)
         self._chk(getattr(lib(), fn)(self.h, out), fn)
@@ -199,16 +217,25 @@ class Context:
 
     def newton_solve(self, rtol=1e-10, max_newton=50, lin_tol=0.1, lin_maxit=20000, restart=30, ls_step=0.83,
                      damped=True, fp32=False):
-        st = np.zeros(8)
+        st = np.zeros(10)
         self._chk(lib().glh_newton_solve(self.h, rtol, max_newton, lin_tol, lin_maxit, restart, ls_step,
                                          int(damped), int(fp32), st), "glh_newton_solve")
-        keys = ["newton_steps", "gmres_its", "r0", "r_final", "t_assemble", "t_solve", "t_linesearch", "F"]
+        keys = ["newton_steps", "gmres_its", "r0", "r_final", "t_assemble", "t_solve", "t_linesearch", "F",
+                "ls_trials", "converged"]
         return dict(zip(keys, st.tolist()))
 
     def last_assembly_ms(self):
         v = np.zeros(2)
         self._chk(lib().glh_last_assembly_ms(self.h, v), "glh_last_assembly_ms")
         return v[0]
+
+    def set_timing(self, on=True):
+        self._chk(lib().glh_set_timing(self.h, int(on)), "glh_set_timing")
+
+    def timing(self):
+        v = np.zeros(6)
+        self._chk(lib().glh_timing(self.h, v), "glh_timing")
+        return dict(spmv_ms=v[0], spmv_n=int(v[1]), jasm_ms=v[2], jasm_n=int(v[3]), rasm_ms=v[4], rasm_n=int(v[5]))
 
     def bench_spmv(self, k=10):
         v = C.c_double()
