@@ -121,8 +121,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cube_q2_ref5", choices=sorted(WORKLOADS))
     ap.add_argument("--lin-tol", type=float, default=0.1)
-    ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--restart", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernels", action="store_true",
+                    help="short fixed kernel sequence for rocprofv3: 3 FP64 and 3 FP32 J+R assemblies, 3 residual "
+                         "assemblies, 10 FP64 and 10 FP32 SpMVs, 5 copies of 1 GiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
@@ -139,6 +142,18 @@ def main():
     ctx.set_solution(U0)
     ctx.store_initial()
     n_dofs = mesh.n_nodes * 18
+    if args.kernels:
+        for fp32 in (False, True):
+            for _ in range(3):
+                ctx.assemble(J=True, R=True, fp32=fp32)
+                print("bench: assemble fp32=%d %.3f ms" % (fp32, ctx.last_assembly_ms()), file=sys.stderr,
+                      flush=True)
+            print("bench: spmv fp32=%d %.3f ms" % (fp32, ctx.bench_spmv(10)), file=sys.stderr, flush=True)
+        for _ in range(3):
+            ctx.assemble(J=False, R=True)
+        print("bench: copy 1 GiB %.3f ms" % ctx.bench_copy(1 << 30, 5), file=sys.stderr, flush=True)
+        ctx.close()
+        return
     kw = dict(rtol=1e-10, max_newton=100, lin_tol=args.lin_tol, lin_maxit=100000, restart=args.restart)
     for w in range(args.warmup):
         ctx.restore_initial()
@@ -160,6 +175,11 @@ def main():
     traffic_alg = spmv_bytes(mesh.n_nodes, mesh.nnzb, 8, 8)
     achieved = traffic_alg / (spmv_avg_ms * 1e-3) / 1e9
     jasm_ms = tm["jasm_ms"] / max(1, tm["jasm_n"])
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get(args.workload, {}).get("k_spmv<double,double>", {}).get(
+            "bytes_per_launch")
     st0 = runs[0]
     line = {
         "metric": METRIC,
@@ -189,7 +209,7 @@ def main():
         "spmv_launches": tm["spmv_n"],
         "roofline": {"kernel": "k_spmv<double,double> (BSR 18x18 SpMV in GMRES)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": traffic_alg},
+                     "traffic": traffic, "algorithmic_bytes": traffic_alg},
     }
     if not args.no_cpu_baseline:
         ls = int(st0.get("ls_trials", st0["newton_steps"]))

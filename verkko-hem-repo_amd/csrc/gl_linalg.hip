@@ -236,6 +236,26 @@ __global__ __launch_bounds__(RT) void k_sub_lincomb(size_t n, int k, const T *__
   }
 }
 
+// w[j] -= sum_i h[i] V[i*ldv + j], and the block partial of ||w||^2 after the
+// update (fixed index sets: deterministic)
+template <typename T>
+__global__ __launch_bounds__(RT) void k_sub_lincomb_norm(size_t n, int k, const T *__restrict__ V, size_t ldv,
+                                                         const T *__restrict__ h, T *__restrict__ w,
+                                                         T *__restrict__ partial)
+{
+  __shared__ T sh[RT];
+  T acc = 0;
+  const size_t stride = (size_t)gridDim.x * RT;
+  for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n; idx += stride) {
+    T s = w[idx];
+    for (int i = 0; i < k; ++i) s -= h[i] * V[(size_t)i * ldv + idx];
+    w[idx] = s;
+    acc += s * s;
+  }
+  const T r = block_reduce(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
 template <typename T>
 __global__ __launch_bounds__(RT) void k_lincomb(size_t n, int k, const T *__restrict__ V, size_t ldv,
                                                 const T *__restrict__ y, T *__restrict__ w)
@@ -308,6 +328,12 @@ __global__ __launch_bounds__(RT) void k_sum_partial(size_t n, const double *__re
   if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
 
+__global__ __launch_bounds__(RT) void k_copy16(size_t n16, const float4 *__restrict__ src, float4 *__restrict__ dst)
+{
+  const size_t stride = (size_t)gridDim.x * RT;
+  for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n16; idx += stride) dst[idx] = src[idx];
+}
+
 template <typename T>
 __global__ void k_set_scalar(T *p, T v)
 {
@@ -377,6 +403,15 @@ hipError_t launch_sub_lincomb(size_t n, int k, const T *V, size_t ldv, const T *
 }
 
 template <typename T>
+hipError_t launch_sub_lincomb_norm(size_t n, int k, const T *V, size_t ldv, const T *h, T *w, T *partial, T *norm2,
+                                   hipStream_t s)
+{
+  hipLaunchKernelGGL((k_sub_lincomb_norm<T>), dim3(MDOT_GRID), dim3(RT), 0, s, n, k, V, ldv, h, w, partial);
+  hipLaunchKernelGGL((k_mdot_final<T>), dim3(1), dim3(RT), 0, s, MDOT_GRID, 1, partial, norm2, 0);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_lincomb(size_t n, int k, const T *V, size_t ldv, const T *y, T *w, hipStream_t s)
 {
   hipLaunchKernelGGL((k_lincomb<T>), dim3(grid_for(n)), dim3(RT), 0, s, n, k, V, ldv, y, w);
@@ -439,6 +474,15 @@ hipError_t launch_set_scalar(T *p, T v, hipStream_t s)
   return hipGetLastError();
 }
 
+hipError_t launch_copy(size_t nbytes, const void *src, void *dst, hipStream_t s)
+{
+  const size_t n16 = nbytes / 16;
+  size_t g = (n16 + RT - 1) / RT;
+  if (g > 8192) g = 8192;
+  hipLaunchKernelGGL(k_copy16, dim3((int)(g ? g : 1)), dim3(RT), 0, s, n16, (const float4 *)src, (float4 *)dst);
+  return hipGetLastError();
+}
+
 // explicit instantiations
 template hipError_t launch_set_scalar<double>(double *, double, hipStream_t);
 template hipError_t launch_set_scalar<float>(float *, float, hipStream_t);
@@ -462,6 +506,10 @@ template hipError_t launch_mdot<float>(size_t, int, const float *, size_t, const
 template hipError_t launch_sub_lincomb<double>(size_t, int, const double *, size_t, const double *, double *,
                                                hipStream_t);
 template hipError_t launch_sub_lincomb<float>(size_t, int, const float *, size_t, const float *, float *, hipStream_t);
+template hipError_t launch_sub_lincomb_norm<double>(size_t, int, const double *, size_t, const double *, double *,
+                                                    double *, double *, hipStream_t);
+template hipError_t launch_sub_lincomb_norm<float>(size_t, int, const float *, size_t, const float *, float *, float *,
+                                                   float *, hipStream_t);
 template hipError_t launch_lincomb<double>(size_t, int, const double *, size_t, const double *, double *, hipStream_t);
 template hipError_t launch_lincomb<float>(size_t, int, const float *, size_t, const float *, float *, hipStream_t);
 template hipError_t launch_scale_by<double>(size_t, const double *, const double *, double *, hipStream_t);

@@ -81,6 +81,7 @@ struct glh_ctx {
   // live kernel timing (glh_set_timing): event pairs around each SpMV
   // launch on the context stream, read back when the pool fills or a solve ends
   bool timing = false;
+  long reorth_count = 0;
   std::vector<hipEvent_t> ev_pool;
   int ev_used = 0;
   double spmv_ms_total = 0.0, spmv_count = 0;
@@ -351,7 +352,7 @@ int spmv(glh_ctx *c, const TJ *J, const TV *x, TV *y)
 }
 
 // Right-preconditioned restarted GMRES(m) with 18x18 point-block Jacobi and
-// CGS2 orthogonalisation on the device.  Solves J x = b (b, x device vectors
+// classical Gram-Schmidt with selective reorthogonalisation on the device.  Solves J x = b (b, x device vectors
 // of type TV) to ||b - J x|| <= rtol ||b||.  Hessenberg/Givens on the host.
 template <typename TJ, typename TV>
 int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, double rtol, int maxit, int m, int *its_out,
@@ -369,10 +370,11 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, double rt
   HIPCHK(c, Z.alloc(n));
   HIPCHK(c, W.alloc(n));
   HIPCHK(c, hb.alloc(2 * (m + 2)));
+  HIPCHK(c, V.alloc(n * (m + 1)));
   HIPCHK(c, pt.alloc((size_t)glh::MDOT_GRID * (m + 2)));
   HIPCHK(c, yb.alloc(m + 1));
   TV *hsum = hb.p, *hpass = hb.p + (m + 2);
-  std::vector<TV> hh(m + 2);
+  std::vector<TV> hh(m + 3);
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m);
   std::vector<TV> yh(m);
   HIPCHK(c, hipMemsetAsync(x, 0, n * sizeof(TV), s));
@@ -413,25 +415,36 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, double rt
     bool est = false;
     while (j < m && it < maxit && !est) {
       TV *vj = V.p + n * j;
+      TV *wv = V.p + n * (j + 1);  // the new direction is built in place
       HIPCHK(c, glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, vj, Z.p, s));
-      int rc = spmv<TJ, TV>(c, J, Z.p, W.p);
+      int rc = spmv<TJ, TV>(c, J, Z.p, wv);
       if (rc) return rc;
-      // CGS2
-      HIPCHK(c, glh::launch_mdot<TV>(n, j + 1, V.p, n, W.p, pt.p, hsum, false, s));
-      HIPCHK(c, glh::launch_sub_lincomb<TV>(n, j + 1, V.p, n, hsum, W.p, s));
-      HIPCHK(c, glh::launch_mdot<TV>(n, j + 1, V.p, n, W.p, pt.p, hpass, false, s));
-      HIPCHK(c, glh::launch_sub_lincomb<TV>(n, j + 1, V.p, n, hpass, W.p, s));
-      HIPCHK(c, glh::launch_mdot<TV>(n, 1, W.p, n, W.p, pt.p, hpass + (j + 1), false, s));
-      std::vector<TV> h2(j + 2);
-      HIPCHK(c, hipMemcpyAsync(hh.data(), hsum, (j + 1) * sizeof(TV), hipMemcpyDeviceToHost, s));
-      HIPCHK(c, hipMemcpyAsync(h2.data(), hpass, (j + 2) * sizeof(TV), hipMemcpyDeviceToHost, s));
+      // classical Gram-Schmidt: h = V^T w together with ||w||^2 in one pass,
+      // then w -= V h with ||w'||^2; a second pass only when
+      // ||w'|| < ||w|| / sqrt(2) (Daniel-Gragg-Kaufman-Stewart criterion)
+      HIPCHK(c, glh::launch_mdot<TV>(n, j + 2, V.p, n, wv, pt.p, hsum, false, s));
+      HIPCHK(c, glh::launch_sub_lincomb_norm<TV>(n, j + 1, V.p, n, hsum, wv, pt.p, hpass + (j + 1), s));
+      HIPCHK(c, hipMemcpyAsync(hh.data(), hsum, (j + 2) * sizeof(TV), hipMemcpyDeviceToHost, s));
+      TV hn2 = 0;
+      HIPCHK(c, hipMemcpyAsync(&hn2, hpass + (j + 1), sizeof(TV), hipMemcpyDeviceToHost, s));
       HIPCHK(c, hipStreamSynchronize(s));
-      const double hn = std::sqrt((double)h2[j + 1]);
+      std::vector<double> h2(j + 1, 0.0);
+      if ((double)hn2 < 0.5 * (double)hh[j + 1]) {
+        HIPCHK(c, glh::launch_mdot<TV>(n, j + 1, V.p, n, wv, pt.p, hpass, false, s));
+        HIPCHK(c, glh::launch_sub_lincomb_norm<TV>(n, j + 1, V.p, n, hpass, wv, pt.p, hpass + (j + 1), s));
+        std::vector<TV> hp(j + 2);
+        HIPCHK(c, hipMemcpyAsync(hp.data(), hpass, (j + 2) * sizeof(TV), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        for (int i = 0; i <= j; ++i) h2[i] = (double)hp[i];
+        hn2 = hp[j + 1];
+        ++c->reorth_count;
+      }
+      const double hn = std::sqrt(std::max(0.0, (double)hn2));
       if (hn > 0) {
         HIPCHK(c, glh::launch_set_scalar<TV>(hpass, (TV)hn, s));
-        HIPCHK(c, glh::launch_scale_by<TV>(n, W.p, hpass, V.p + n * (j + 1), s));
+        HIPCHK(c, glh::launch_scale_by<TV>(n, wv, hpass, wv, s));
       }
-      for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = (double)hh[i] + (double)h2[i];
+      for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = (double)hh[i] + h2[i];
       for (int i = 0; i < j; ++i) {
         const double a = H[(size_t)i * m + j], b2 = H[(size_t)(i + 1) * m + j];
         H[(size_t)i * m + j] = cs[i] * a + sn[i] * b2;
@@ -947,6 +960,30 @@ int glh_bench_spmv(glh_ctx *c, int k, double *ms_avg)
   float ms = 0;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
   *ms_avg = ms / k;
+  return GLH_OK;
+}
+
+int glh_bench_copy(glh_ctx *c, size_t nbytes, int k, double *ms_avg)
+{
+  if (!c || !ms_avg || k < 1 || nbytes % 16) return GLH_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  void *a = nullptr, *b = nullptr;
+  HIPCHK(c, hipMalloc(&a, nbytes));
+  hipError_t e = hipMalloc(&b, nbytes);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    return fail(c, GLH_ERR_HIP, "hipMalloc failed");
+  }
+  HIPCHK(c, hipMemsetAsync(a, 0, nbytes, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (int i = 0; i < k; ++i) HIPCHK(c, glh::launch_copy(nbytes, a, b, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *ms_avg = ms / k;
+  (void)hipFree(a);
+  (void)hipFree(b);
   return GLH_OK;
 }
 

@@ -63,7 +63,7 @@ template <typename TJ, typename TX>
 hipError_t launch_bj_apply(int n_rows, const TJ *Dinv, const TX *x, TX *y, hipStream_t s);
 
 // Deterministic reductions.  partial must hold MDOT_GRID * k values.
-constexpr int MDOT_GRID = 512;
+constexpr int MDOT_GRID = 2048;
 // out[i] (+)= sum_j V[i*ldv + j] * w[j] for i < k.  accumulate: out += result.
 template <typename T>
 hipError_t launch_mdot(size_t n, int k, const T *V, size_t ldv, const T *w, T *partial, T *out, bool accumulate,
@@ -71,6 +71,10 @@ hipError_t launch_mdot(size_t n, int k, const T *V, size_t ldv, const T *w, T *p
 // w[j] -= sum_i h[i] V[i*ldv + j]
 template <typename T>
 hipError_t launch_sub_lincomb(size_t n, int k, const T *V, size_t ldv, const T *h, T *w, hipStream_t s);
+// w[j] -= sum_i h[i] V[i*ldv + j]; *norm2 = ||w||^2 afterwards (partial: MDOT_GRID values)
+template <typename T>
+hipError_t launch_sub_lincomb_norm(size_t n, int k, const T *V, size_t ldv, const T *h, T *w, T *partial, T *norm2,
+                                   hipStream_t s);
 // w[j] = sum_i y[i] V[i*ldv + j]
 template <typename T>
 hipError_t launch_lincomb(size_t n, int k, const T *V, size_t ldv, const T *y, T *w, hipStream_t s);
@@ -93,6 +97,8 @@ hipError_t launch_mask_zero(int n_nodes, const uint32_t *mask, double *x, hipStr
 // *p = v (stream-ordered scalar write)
 template <typename T>
 hipError_t launch_set_scalar(T *p, T v, hipStream_t s);
+// plain device copy of nbytes (multiple of 16): calibration kernel for HBM counters
+hipError_t launch_copy(size_t nbytes, const void *src, void *dst, hipStream_t s);
 // out = sum_i v[i] in fixed order (deterministic)
 hipError_t launch_sum(size_t n, const double *v, double *partial, double *out, hipStream_t s);
 

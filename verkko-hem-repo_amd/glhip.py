@@ -25,7 +25,7 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_mesh_destroy", "glh_mesh_sizes", "glh_mesh_get", "glh_upload_mesh", "glh_initial_guess_host", "glh_set_solution", "glh_store_initial", "glh_restore_initial",
            "glh_get_solution", "glh_get_update", "glh_get_rhs", "glh_get_jacobian", "glh_assemble", "glh_rhs_norm",
            "glh_residual", "glh_solve", "glh_newton_step", "glh_free_energy", "glh_newton_solve",
-           "glh_last_assembly_ms", "glh_set_timing", "glh_timing", "glh_bench_spmv"]
+           "glh_last_assembly_ms", "glh_set_timing", "glh_timing", "glh_bench_spmv", "glh_bench_copy"]
 
 _lib = None
 
@@ -70,6 +70,7 @@ def lib():
     L.glh_set_timing.argtypes = [vp, C.c_int]
     L.glh_timing.argtypes = [vp, _dp]
     L.glh_bench_spmv.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+    L.glh_bench_copy.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -240,6 +241,11 @@ class Context:
     def bench_spmv(self, k=10):
         v = C.c_double()
         self._chk(lib().glh_bench_spmv(self.h, int(k), C.byref(v)), "glh_bench_spmv")
+        return v.value
+
+    def bench_copy(self, nbytes, k=5):
+        v = C.c_double()
+        self._chk(lib().glh_bench_copy(self.h, int(nbytes), int(k), C.byref(v)), "glh_bench_copy")
         return v.value
 
     def close(self):
