@@ -33,7 +33,6 @@ namespace glh {
 namespace {
 
 constexpr int NT = 192;   // threads per workgroup (3 wavefronts)
-constexpr int NB = 8;     // blocks staged per write round
 constexpr int GRAM = 57;  // per-q Gram data (see gram_entry)
 constexpr int NUPAIR = 171;
 
@@ -140,8 +139,7 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
                                                  double *__restrict__ rhs)
 {
   constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS, NPAIR = NS * (NS + 1) / 2;
-  constexpr int RBUF = DO_R ? NQ * 72 : 1, JBUF = DO_J ? NB * 324 : 1;
-  constexpr int NBUF = RBUF > JBUF ? RBUF : JBUF;
+  constexpr int NBUF = DO_R ? NQ * 72 : 1;
   __shared__ double sU[NS * 18];
   __shared__ double sX[NQ * 18];
   __shared__ double sG[DO_R ? NQ * 54 : 1];
@@ -221,92 +219,116 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
       const int node = cn[a];
       if (!((Mh.node_mask[node] >> c) & 1u)) rhs[(size_t)node * 18 + c] -= r;
     }
-    if (DO_J) __syncthreads();
   }
 
   if (DO_J) {
-    using V2 = typename Vec2<TJ>::type;
-    // unique component pair of this thread
+    // Thread t < 171 owns the unique component pair (c <= d).  For every node
+    // pair (a <= b) it forms block(a,b)[c][d], block(a,b)[d][c] and writes them,
+    // with the transposed block(b,a), straight to the BSR values: no LDS
+    // staging and no barrier, so each wavefront keeps many stores in flight.
+    // The 64 lanes of a store cover consecutive entries of one 2.6 KB block,
+    // which L2 merges into full lines.
+    if (tid >= NUPAIR) return;
     int c = 0, d = 0;
-    const bool own = tid < NUPAIR;
-    if (own) {
+    {
       int t = tid;
       while (t >= 18 - c) { t -= 18 - c; ++c; }
       d = c + t;
     }
     double Hq[NQ];
-    if (own)
-      for (int q = 0; q < NQ; ++q) Hq[q] = hess_entry(sGram + q * GRAM, sX + q * 18, c, d, M);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) Hq[q] = hess_entry(sGram + q * GRAM, sX + q * 18, c, d, M);
     const bool grad_cd = (c / 9 == d / 9) && ((c % 9) / 3 == (d % 9) / 3);
     const int kc = c % 3, kd = d % 3;
-    const double lap_scale[3] = {vol * ih[0] * ih[0], vol * ih[1] * ih[1], vol * ih[2] * ih[2]};
+    const double lap0 = vol * ih[0] * ih[0], lap1 = vol * ih[1] * ih[1], lap2 = vol * ih[2] * ih[2];
     const double k23_cd = M.K23 * vol * ih[kc] * ih[kd];
     const uint32_t *mask = Mh.node_mask;
     const int *cblk = Mh.cell_blk + (size_t)e * NS * NS;
-
-    for (int p0 = 0; p0 < NPAIR; p0 += NB) {
-      const int nb = (NPAIR - p0) < NB ? (NPAIR - p0) : NB;
-      if (own) {
-        for (int s = 0; s < nb; ++s) {
-          const int p = p0 + s;
-          const double *wp = T.Wp + p * NQ;
-          double sym = 0.0;
+    const int o_cd = c * 18 + d, o_dc = d * 18 + c;
+    // Robin diagonal weight of this component on each local face (0 if off)
+    double rob[6];
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) sym += wp[q] * Hq[q];
-          sym *= vol;
-          const double *sp = T.Sp + p * 9;
-          if (c == d) {
-            sym += M.K1 * (sp[0] * lap_scale[0] + sp[4] * lap_scale[1] + sp[8] * lap_scale[2]);
-            if (M.robin != 0.0) {
-              const int a = T.pa[p], b = T.pb[p];
-              for (int lf = 0; lf < 6; ++lf) {
-                const int bid = (int)((cface >> (3 * lf)) & 7u) - 1;
-                if (bid < 2 || bid > 4 || face_zeroed(bid, c)) continue;
-                sym += M.robin * face_area(lf, h) * T.Mf[(lf * NS + a) * NS + b];
-              }
-            }
-          }
-          double e1 = sym, e2 = sym;
-          if (grad_cd) {
-            e1 += k23_cd * sp[kc * 3 + kd];
-            e2 += k23_cd * sp[kd * 3 + kc];
-          }
-          sBuf[s * 324 + c * 18 + d] = e1;
-          sBuf[s * 324 + d * 18 + c] = e2;
+    for (int lf = 0; lf < 6; ++lf) {
+      const int bid = (int)((cface >> (3 * lf)) & 7u) - 1;
+      rob[lf] = (c == d && M.robin != 0.0 && bid >= 2 && bid <= 4 && !face_zeroed(bid, c))
+                    ? M.robin * face_area(lf, h) : 0.0;
+    }
+    // Old values of blocks shared with earlier colours are loaded one node pair
+    // ahead, so their latency overlaps the contraction of the current pair.
+    auto blk_ptr = [&](int code) -> TJ * { return J + (size_t)(code < 0 ? -code - 1 : code) * 324; };
+    auto prefetch = [&](int p, double *o) {
+      const int a = T.pa[p], b = T.pb[p];
+      const int cab = cblk[a * NS + b];
+      o[0] = o[1] = o[2] = o[3] = 0.0;
+      if (cab >= 0) {
+        const TJ *B = J + (size_t)cab * 324;
+        o[0] = (double)B[o_cd];
+        if (c != d) o[1] = (double)B[o_dc];
+      }
+      if (a != b) {
+        const int cba = cblk[b * NS + a];
+        if (cba >= 0) {
+          const TJ *B = J + (size_t)cba * 324;
+          o[2] = (double)B[o_dc];
+          if (c != d) o[3] = (double)B[o_cd];
         }
       }
-      __syncthreads();
-      // scatter: block (a,b) from the stage, block (b,a) as its transpose
-      for (int i = tid; i < nb * 324; i += NT) {
-        const int s = i / 324, r = i % 324, dir = r / 162, k2 = r % 162;
-        const int p = p0 + s, a = T.pa[p], b = T.pb[p];
-        if (dir == 1 && a == b) continue;
-        const int ra = dir ? b : a, cb = dir ? a : b;
-        const int rn = cn[ra], cnd = cn[cb];
-        const int code = cblk[ra * NS + cb];
-        const bool first = code < 0;
-        const size_t blk = (size_t)(first ? -code - 1 : code);
-        const int cc = (2 * k2) / 18, dd = (2 * k2) % 18;
-        const double *st = sBuf + s * 324;
-        double v0 = dir ? st[dd * 18 + cc] : st[cc * 18 + dd];
-        double v1 = dir ? st[(dd + 1) * 18 + cc] : st[cc * 18 + dd + 1];
-        const uint32_t mr = mask[rn], mc = mask[cnd];
-        const bool rc = (mr >> cc) & 1u;
-        if (rc || ((mc >> dd) & 1u)) v0 = (a == b && rc && cc == dd) ? fabs(v0) : 0.0;
-        if (rc || ((mc >> (dd + 1)) & 1u)) v1 = (a == b && rc && cc == dd + 1) ? fabs(v1) : 0.0;
-        V2 *dst = reinterpret_cast<V2 *>(J + blk * 324) + k2;
-        V2 out;
-        if (first) {
-          out.x = (TJ)v0;
-          out.y = (TJ)v1;
-        } else {
-          const V2 old = *dst;
-          out.x = (TJ)((double)old.x + v0);
-          out.y = (TJ)((double)old.y + v1);
-        }
-        *dst = out;
+    };
+    auto process = [&](int p, const double *ol) {
+      const double *wp = T.Wp + p * NQ;
+      double sym = 0.0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) sym += wp[q] * Hq[q];
+      sym *= vol;
+      const double *sp = T.Sp + p * 9;
+      const int a = T.pa[p], b = T.pb[p];
+      if (c == d) {
+        sym += M.K1 * (sp[0] * lap0 + sp[4] * lap1 + sp[8] * lap2);
+#pragma unroll
+        for (int lf = 0; lf < 6; ++lf)
+          if (rob[lf] != 0.0) sym += rob[lf] * T.Mf[(lf * NS + a) * NS + b];
       }
-      __syncthreads();
+      double e1 = sym, e2 = sym;
+      if (grad_cd) {
+        e1 += k23_cd * sp[kc * 3 + kd];
+        e2 += k23_cd * sp[kd * 3 + kc];
+      }
+      const uint32_t ma = mask[cn[a]], mb = mask[cn[b]];
+      {  // block (a,b): rows of node a, columns of node b
+        TJ *B = blk_ptr(cblk[a * NS + b]);
+        double v1 = e1, v2 = e2;
+        const bool r1 = (ma >> c) & 1u, c1 = (mb >> d) & 1u, r2 = (ma >> d) & 1u, c2 = (mb >> c) & 1u;
+        if (r1 || c1) v1 = (a == b && r1 && c == d) ? fabs(v1) : 0.0;
+        if (r2 || c2) v2 = (a == b && r2 && c == d) ? fabs(v2) : 0.0;
+        B[o_cd] = (TJ)(ol[0] + v1);
+        if (c != d) B[o_dc] = (TJ)(ol[1] + v2);
+      }
+      if (a != b) {  // block (b,a) = block(a,b)^T: [d][c] = e1, [c][d] = e2
+        TJ *B = blk_ptr(cblk[b * NS + a]);
+        double v1 = e1, v2 = e2;
+        const bool r1 = (mb >> d) & 1u, c1 = (ma >> c) & 1u, r2 = (mb >> c) & 1u, c2 = (ma >> d) & 1u;
+        if (r1 || c1) v1 = 0.0;
+        if (r2 || c2) v2 = 0.0;
+        B[o_dc] = (TJ)(ol[2] + v1);
+        if (c != d) B[o_cd] = (TJ)(ol[3] + v2);
+      }
+    };
+    // PD node pairs in flight: the old values of pair p + PD are requested
+    // before pair p is contracted and stored (PD = 1 measured fastest)
+    constexpr int PD = 1;
+    double ob[PD][4];
+#pragma unroll
+    for (int k = 0; k < PD; ++k) prefetch(k < NPAIR ? k : NPAIR - 1, ob[k]);
+    for (int p0 = 0; p0 < NPAIR; p0 += PD) {
+#pragma unroll
+      for (int k = 0; k < PD; ++k) {
+        const int p = p0 + k;
+        if (p < NPAIR) {
+          double ol[4] = {ob[k][0], ob[k][1], ob[k][2], ob[k][3]};
+          if (p + PD < NPAIR) prefetch(p + PD, ob[k]);
+          process(p, ol);
+        }
+      }
     }
   }
 }
@@ -419,14 +441,15 @@ hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefT
   dim3 g(n), b(NT);
 #define GLH_ASM(D, TJ, DJ, DR) \
   hipLaunchKernelGGL((k_assemble<D, TJ, DJ, DR>), g, b, 0, s, T, M, mat, U, cells, (TJ *)J, rhs)
+  // J and R are launched as two kernels: the Jacobian-only kernel needs fewer
+  // registers (3 waves/SIMD instead of 2 for Q2), which hides more of the
+  // latency of its scattered read-modify-write stores.
   if (deg == 1) {
-    if (do_j && do_r) { if (jfp32) GLH_ASM(1, float, true, true); else GLH_ASM(1, double, true, true); }
-    else if (do_j) { if (jfp32) GLH_ASM(1, float, true, false); else GLH_ASM(1, double, true, false); }
-    else GLH_ASM(1, double, false, true);
+    if (do_j) { if (jfp32) GLH_ASM(1, float, true, false); else GLH_ASM(1, double, true, false); }
+    if (do_r) GLH_ASM(1, double, false, true);
   } else {
-    if (do_j && do_r) { if (jfp32) GLH_ASM(2, float, true, true); else GLH_ASM(2, double, true, true); }
-    else if (do_j) { if (jfp32) GLH_ASM(2, float, true, false); else GLH_ASM(2, double, true, false); }
-    else GLH_ASM(2, double, false, true);
+    if (do_j) { if (jfp32) GLH_ASM(2, float, true, false); else GLH_ASM(2, double, true, false); }
+    if (do_r) GLH_ASM(2, double, false, true);
   }
 #undef GLH_ASM
   return hipGetLastError();
