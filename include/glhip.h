@@ -49,10 +49,16 @@ int glh_create(int device, glh_ctx **out);
 int glh_destroy(glh_ctx *ctx);
 const char *glh_last_error(const glh_ctx *ctx);
 
-/* Material and gradient constants: alpha, beta1..5 from matep
+/* Material and gradient constants for every cell: alpha, beta1..5 from matep
  * (femgl.cc:161-168), K1..K3 (femgl.h:320-322) and the AdGR diffuse length
  * bt (femgl.cc:150, Robin term active iff bt < 1e10, assemble.cc:293). */
 int glh_set_material(glh_ctx *ctx, double alpha, const double beta[5], const double K[3], double bt);
+
+/* Per-instance materials of a batch (see glh_mesh_concat): coef[n_inst*10]
+ * = (alpha, beta1..5, K1, K2, K3, bt) per instance, the constructor state of
+ * one FemGL<dim> per (T, p) point (femgl.cc:145-168).  n_inst must equal the
+ * number of instances of the uploaded mesh. */
+int glh_set_materials(glh_ctx *ctx, int n_inst, const double *coef);
 
 /* matep restatement (matep/src/matep.cc): out[0]=alpha, [1..5]=beta1..5,
  * [6]=beta_A, [7]=beta_B, [8]=gap_A, [9]=gap_B, [10]=f_A, [11]=f_B.
@@ -67,9 +73,15 @@ int glh_matep(double p, double t, int scc, double out[12]);
  * Dirichlet, femgl.h:294-301, plus the Robin term when bt < 1e10). */
 int glh_mesh_box(int degree, int nx, int ny, int nz, const double lo[3], const double hi[3],
                  const int side_id[6], glh_mesh **out);
+/* Batch of independent instances: the meshes are concatenated with disjoint
+ * node and cell ranges (instance i owns nodes [node_off[i], node_off[i+1])).
+ * All meshes need the same degree.  The solver treats every instance as its
+ * own problem (own Krylov space, line search and stopping test) while each
+ * vector operation runs as one launch over the whole batch. */
+int glh_mesh_concat(const glh_mesh *const *meshes, int n, glh_mesh **out);
 int glh_mesh_destroy(glh_mesh *m);
 /* sizes: [0]=degree [1]=n_cells [2]=n_nodes [3]=n_faces [4]=nnzb [5]=n_colors
- *        [6]=n_constraint_dofs [7]=n_constraint_entries */
+ *        [6]=n_instances [7]=0 */
 int glh_mesh_sizes(const glh_mesh *m, int64_t sizes[8]);
 /* Copies the mesh arrays out (any pointer may be NULL):
  * cell_nodes[n_cells*ns] (lexicographic local order), cell_h[n_cells*3],
@@ -78,6 +90,9 @@ int glh_mesh_sizes(const glh_mesh *m, int64_t sizes[8]);
  * color_ptr[n_colors+1], color_cells[n_cells], row_ptr[n_nodes+1], col[nnzb]. */
 int glh_mesh_get(const glh_mesh *m, int *cell_nodes, double *cell_h, double *node_xyz, int *faces,
                  uint32_t *node_mask, int *color_ptr, int *color_cells, int *row_ptr, int *col);
+
+/* node_off[n_inst+1], cell_off[n_inst+1] (either may be NULL) */
+int glh_mesh_instances(const glh_mesh *m, int *node_off, int *cell_off);
 
 /* ------------------------------------------------- per refinement cycle */
 /* Uploads geometry, cell->node map, constraint masks, colour sets and the
@@ -125,8 +140,11 @@ int glh_solve(glh_ctx *ctx, double rtol, int maxit, int restart, double stats[2]
  * out[2] = step length, out[3] = trials. */
 int glh_newton_step(glh_ctx *ctx, double step, int damped, double out[4]);
 /* Free energy F = integral of f (bulk + gradient) + Robin wall term; the
- * residual is R = (1/2) dF/dU.  The reference never forms F. */
+ * residual is R = (1/2) dF/dU.  The reference never forms F.  For a batch,
+ * glh_free_energy returns the sum and glh_instance_energy one value per
+ * instance. */
 int glh_free_energy(glh_ctx *ctx, double *F);
+int glh_instance_energy(glh_ctx *ctx, double *F);
 
 /* ------------------------------------------------------------ FemGL driver */
 /* The Newton loop of FemGL::run (run.cc:208-256) on the uploaded mesh:
@@ -137,9 +155,17 @@ int glh_free_energy(glh_ctx *ctx, double *F);
  * stats: [0] Newton steps, [1] total GMRES iterations, [2] ||R0||,
  * [3] final ||R||, [4] seconds assembling, [5] seconds solving,
  * [6] seconds in line search, [7] free energy, [8] line-search residual
- * evaluations, [9] 1 if converged. */
+ * evaluations, [9] 1 if every instance converged.  Each instance of a batch
+ * stops on its own test and is frozen afterwards. */
 int glh_newton_solve(glh_ctx *ctx, double rtol, int max_newton, double lin_tol, int lin_maxit, int restart,
                      double ls_step, int damped, int fp32, double stats[10]);
+
+/* Per-instance results of the last glh_newton_solve, 8 values per instance:
+ * [0] Newton steps, [1] converged (0/1), [2] ||R0||, [3] final ||R||,
+ * [4] seconds from the start of the solve to convergence (-1 if not),
+ * [5] free energy, [6] line-search residual evaluations, [7] GMRES
+ * iterations the instance used. */
+int glh_instance_stats(glh_ctx *ctx, double *out);
 
 /* timing of the last glh_assemble Jacobian launches on the context stream
  * (HIP events): ms[0] = J+R assembly ms, ms[1] = number of launches. */

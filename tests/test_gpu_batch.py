@@ -1,0 +1,104 @@
+"""Batched instances (many (T, p, slab) problems in one context) on the GPU,
+checked instance by instance against the oracle and against single-instance
+solves: the batch must reproduce each instance's Jacobian, residual and
+converged free energy exactly as if it were solved alone."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from product_util import glhip, oracle_mesh
+
+pytestmark = pytest.mark.gpu
+G = glhip()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def slab(n, D, lateral=40.0, deg=1):
+    h = lateral / 2
+    return G.Mesh(deg, n, (-h, -h, -D / 2), (h, h, D / 2), [1, 1, 1, 1, 0, 0])
+
+
+CASES = [((4, 4, 2), 6.0, 20.0, 0.5), ((5, 3, 3), 12.0, 26.0, 0.7), ((3, 3, 4), 9.0, 3.3, 0.31)]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = G.Context(0)
+    yield c
+    c.close()
+
+
+def _batch(ctx, cases, seed0=10):
+    meshes = [slab(n, D) for (n, D, p, t) in cases]
+    batch = G.Mesh.concat(meshes)
+    ctx.upload_mesh(batch)
+    cos = [O.coef(p, t) for (n, D, p, t) in cases]
+    ctx.set_materials(np.array(cos))
+    Us = [O.initial_guess(m.n_nodes, O.matep(p, t)["gap_B"], amp=0.05, seed=seed0 + i)
+          for i, (m, (n, D, p, t)) in enumerate(zip(meshes, cases))]
+    ctx.set_solution(np.concatenate(Us))
+    return meshes, batch, cos, Us
+
+
+def test_batch_structure_and_assembly_match_each_instance(ctx):
+    meshes, batch, cos, Us = _batch(ctx, CASES)
+    no, co = batch.instances()
+    assert batch.n_inst == 3 and no[-1] == batch.n_nodes and co[-1] == batch.n_cells
+    ctx.assemble()
+    Jb, rb = ctx.jacobian(), ctx.rhs()
+    a = batch.arrays()
+    for i, m in enumerate(meshes):
+        say("instance", i)
+        om = oracle_mesh(m)
+        U = O.distribute(om, Us[i])
+        Jo, ro = O.assemble(om, cos[i], U)
+        r_i = rb[no[i] * 18:no[i + 1] * 18]
+        assert np.abs(r_i - ro).max() <= 1e-12 * np.abs(ro).max()
+        b0, b1 = a["row_ptr"][no[i]], a["row_ptr"][no[i + 1]]
+        assert np.array_equal(a["col"][b0:b1] - no[i], om.col)
+        assert np.abs(Jb[b0:b1] - Jo).max() <= 1e-12 * np.abs(Jo).max()
+
+
+def test_batch_newton_equals_single_instance_solves(ctx):
+    meshes, batch, cos, Us = _batch(ctx, CASES)
+    st = ctx.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=40)
+    ist = ctx.instance_stats()
+    say("batch", st)
+    say(ist)
+    assert st["converged"] == 1.0 and np.all(ist[:, 1] == 1)
+    Fb = ctx.instance_energy()
+    Ub = ctx.solution()
+    no, _ = batch.instances()
+    single = G.Context(0)
+    try:
+        for i, m in enumerate(meshes):
+            single.upload_mesh(m)
+            single.set_material(cos[i][0], cos[i][1:6], cos[i][6:9], cos[i][9])
+            single.set_solution(Us[i])
+            s1 = single.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=40)
+            say("single", i, s1)
+            assert s1["F"] == pytest.approx(Fb[i], rel=1e-10)
+            assert ist[i, 0] == s1["newton_steps"]
+            U1 = single.solution()
+            Ui = Ub[no[i] * 18:no[i + 1] * 18]
+            assert np.abs(U1 - Ui).max() <= 1e-8 * np.abs(U1).max()
+    finally:
+        single.close()
+
+
+def test_batch_fp32_mixed_precision_reaches_fp64_fixed_point(ctx):
+    meshes, batch, cos, Us = _batch(ctx, CASES)
+    st64 = ctx.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=40)
+    F64 = ctx.instance_energy()
+    ctx.set_solution(np.concatenate(Us))
+    st32 = ctx.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=40, fp32=True)
+    F32 = ctx.instance_energy()
+    say(st64, st32)
+    assert st32["converged"] == 1.0
+    assert np.allclose(F32, F64, rtol=1e-10, atol=0)

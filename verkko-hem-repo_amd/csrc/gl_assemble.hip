@@ -134,7 +134,7 @@ template <> struct Vec2<double> { using type = double2; };
 template <> struct Vec2<float> { using type = float2; };
 
 template <int DEG, typename TJ, bool DO_J, bool DO_R>
-__global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Material M, const double *__restrict__ U,
+__global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ U,
                                                  const int *__restrict__ cells, TJ *__restrict__ J,
                                                  double *__restrict__ rhs)
 {
@@ -148,6 +148,7 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
 
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
+  const Material M = Mh.mats ? Mh.mats[Mh.cell_mat[e]] : M0;  // per-instance (T, p) in a batch
   const int *cn = Mh.cell_nodes + (size_t)e * NS;
   double h[3], ih[3];
   for (int k = 0; k < 3; ++k) { h[k] = Mh.cell_h[3 * e + k]; ih[k] = 1.0 / h[k]; }
@@ -334,7 +335,7 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
 }
 
 template <int DEG>
-__global__ __launch_bounds__(NT) void k_energy(RefTables T, MeshDev Mh, Material M, const double *__restrict__ U,
+__global__ __launch_bounds__(NT) void k_energy(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ U,
                                                int n_cells, double *__restrict__ Fcell)
 {
   constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS;
@@ -344,6 +345,7 @@ __global__ __launch_bounds__(NT) void k_energy(RefTables T, MeshDev Mh, Material
   __shared__ double sF[NT];
   const int tid = threadIdx.x, e = blockIdx.x;
   if (e >= n_cells) return;
+  const Material M = Mh.mats ? Mh.mats[Mh.cell_mat[e]] : M0;
   const int *cn = Mh.cell_nodes + (size_t)e * NS;
   double h[3], ih[3];
   for (int k = 0; k < 3; ++k) { h[k] = Mh.cell_h[3 * e + k]; ih[k] = 1.0 / h[k]; }

@@ -340,6 +340,120 @@ __global__ void k_set_scalar(T *p, T v)
   if (threadIdx.x == 0) p[0] = v;
 }
 
+
+// ------------------------------------------------ segmented (batched) kernels
+// The dof range is cut into chunks that never straddle an instance boundary:
+// ch[3i] = instance, ch[3i+1] = first dof, ch[3i+2] = one past the last dof.
+// One workgroup per chunk; per-chunk partials are reduced per instance in
+// fixed order (deterministic).
+
+template <typename T>
+__global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__restrict__ ch, int k, int i0,
+                                                         const T *__restrict__ V, size_t ldv, const T *__restrict__ w,
+                                                         T *__restrict__ partial)
+{
+  __shared__ T sh[RT];
+  T acc[KCH];
+#pragma unroll
+  for (int j = 0; j < KCH; ++j) acc[j] = 0;
+  const int kk = (k - i0) < KCH ? (k - i0) : KCH;
+  const long long beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
+  for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
+    const T wv = w[idx];
+#pragma unroll
+    for (int j = 0; j < KCH; ++j)
+      if (j < kk) acc[j] += V[(size_t)(i0 + j) * ldv + idx] * wv;
+  }
+  for (int j = 0; j < kk; ++j) {
+    const T r = block_reduce(acc[j], sh);
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * k + i0 + j] = r;
+  }
+}
+
+// out[s*ldo + off + i] (+)= sum over the chunks of instance s of partial[chunk*k + i]
+template <typename T>
+__global__ __launch_bounds__(RT) void k_seg_final(const int *__restrict__ seg_chunk, int k,
+                                                  const T *__restrict__ partial, T *__restrict__ out, int ldo, int off,
+                                                  int accumulate)
+{
+  __shared__ T sh[RT];
+  const int sg = blockIdx.x, i = blockIdx.y;
+  T acc = 0;
+  for (int b = seg_chunk[sg] + threadIdx.x; b < seg_chunk[sg + 1]; b += RT) acc += partial[(size_t)b * k + i];
+  const T r = block_reduce(acc, sh);
+  if (threadIdx.x == 0) {
+    T *o = out + (size_t)sg * ldo + off + i;
+    *o = accumulate ? *o + r : r;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(RT) void k_seg_sub_lincomb_norm(const long long *__restrict__ ch, int k,
+                                                             const T *__restrict__ V, size_t ldv,
+                                                             const T *__restrict__ h, int ldh, T *__restrict__ w,
+                                                             T *__restrict__ partial)
+{
+  __shared__ T sh[RT];
+  const long long sg = ch[3 * blockIdx.x], beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
+  const T *hs = h + sg * ldh;
+  T acc = 0;
+  for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
+    T s = w[idx];
+    for (int i = 0; i < k; ++i) s -= hs[i] * V[(size_t)i * ldv + idx];
+    w[idx] = s;
+    acc += s * s;
+  }
+  const T r = block_reduce(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(RT) void k_seg_lincomb(const long long *__restrict__ ch, int k, const T *__restrict__ V,
+                                                    size_t ldv, const T *__restrict__ y, int ldy, T *__restrict__ w)
+{
+  const long long sg = ch[3 * blockIdx.x], beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
+  const T *ys = y + sg * ldy;
+  for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
+    T s = 0;
+    for (int i = 0; i < k; ++i) s += ys[i] * V[(size_t)i * ldv + idx];
+    w[idx] = s;
+  }
+}
+
+// dst = src / scal[s*lds] (unchanged when the scale is 0)
+template <typename T>
+__global__ __launch_bounds__(RT) void k_seg_scale(const long long *__restrict__ ch, const T *__restrict__ src,
+                                                  const T *__restrict__ scal, int lds, T *__restrict__ dst)
+{
+  const long long sg = ch[3 * blockIdx.x], beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
+  const T a = scal[sg * lds];
+  const T inv = a != (T)0 ? (T)1 / a : (T)1;
+  for (long long idx = beg + threadIdx.x; idx < end; idx += RT) dst[idx] = src[idx] * inv;
+}
+
+// X = U + alpha[s] dU with Dirichlet components zeroed
+__global__ __launch_bounds__(RT) void k_seg_trial(const long long *__restrict__ ch, const double *__restrict__ U,
+                                                 const double *__restrict__ dU, const double *__restrict__ alpha,
+                                                 const uint32_t *__restrict__ mask, double *__restrict__ X)
+{
+  const long long sg = ch[3 * blockIdx.x], beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
+  const double a = alpha[sg];
+  for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
+    const uint32_t m = mask[idx / 18];
+    X[idx] = ((m >> (idx % 18)) & 1u) ? 0.0 : U[idx] + a * dU[idx];
+  }
+}
+
+// v = 0 on the instances whose flag is 0
+template <typename T>
+__global__ __launch_bounds__(RT) void k_seg_zero(const long long *__restrict__ ch, const int *__restrict__ flag,
+                                                 T *__restrict__ v)
+{
+  const long long sg = ch[3 * blockIdx.x], beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
+  if (flag[sg]) return;
+  for (long long idx = beg + threadIdx.x; idx < end; idx += RT) v[idx] = (T)0;
+}
+
 inline int grid_for(size_t n)
 {
   size_t g = (n + RT - 1) / RT;
@@ -482,6 +596,76 @@ hipError_t launch_copy(size_t nbytes, const void *src, void *dst, hipStream_t s)
   hipLaunchKernelGGL(k_copy16, dim3((int)(g ? g : 1)), dim3(RT), 0, s, n16, (const float4 *)src, (float4 *)dst);
   return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, const T *w, T *partial, T *out, int ldo,
+                           int off, bool accumulate, hipStream_t s)
+{
+  if (k <= 0 || S.n_chunks <= 0) return hipSuccess;
+  for (int i0 = 0; i0 < k; i0 += KCH)
+    hipLaunchKernelGGL((k_seg_mdot_partial<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, k, i0, V, ldv, w,
+                       partial);
+  hipLaunchKernelGGL((k_seg_final<T>), dim3(S.n_seg, k), dim3(RT), 0, s, S.seg_chunk, k, partial, out, ldo, off,
+                     accumulate ? 1 : 0);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size_t ldv, const T *h, int ldh, T *w,
+                                       T *partial, T *norm2, int ldo, int off, hipStream_t s)
+{
+  if (S.n_chunks <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_seg_sub_lincomb_norm<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, k, V, ldv, h, ldh, w,
+                     partial);
+  hipLaunchKernelGGL((k_seg_final<T>), dim3(S.n_seg, 1), dim3(RT), 0, s, S.seg_chunk, 1, partial, norm2, ldo, off, 0);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_seg_lincomb(const SegInfo &S, int k, const T *V, size_t ldv, const T *y, int ldy, T *w,
+                              hipStream_t s)
+{
+  if (S.n_chunks <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_seg_lincomb<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, k, V, ldv, y, ldy, w);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_seg_scale(const SegInfo &S, const T *src, const T *scal, int lds, T *dst, hipStream_t s)
+{
+  if (S.n_chunks <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_seg_scale<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, src, scal, lds, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_trial(const SegInfo &S, const double *U, const double *dU, const double *alpha,
+                            const uint32_t *mask, double *X, hipStream_t s)
+{
+  if (S.n_chunks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_seg_trial, dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, U, dU, alpha, mask, X);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_seg_zero(const SegInfo &S, const int *flag, T *v, hipStream_t s)
+{
+  if (S.n_chunks <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_seg_zero<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, flag, v);
+  return hipGetLastError();
+}
+
+#define GLH_SEG_INST(T)                                                                                         \
+  template hipError_t launch_seg_mdot<T>(const SegInfo &, int, const T *, size_t, const T *, T *, T *, int, int,   \
+                                         bool, hipStream_t);                                                       \
+  template hipError_t launch_seg_sub_lincomb_norm<T>(const SegInfo &, int, const T *, size_t, const T *, int, T *, \
+                                                     T *, T *, int, int, hipStream_t);                             \
+  template hipError_t launch_seg_lincomb<T>(const SegInfo &, int, const T *, size_t, const T *, int, T *,          \
+                                            hipStream_t);                                                          \
+  template hipError_t launch_seg_scale<T>(const SegInfo &, const T *, const T *, int, T *, hipStream_t);           \
+  template hipError_t launch_seg_zero<T>(const SegInfo &, const int *, T *, hipStream_t);
+GLH_SEG_INST(double)
+GLH_SEG_INST(float)
+#undef GLH_SEG_INST
 
 // explicit instantiations
 template hipError_t launch_set_scalar<double>(double *, double, hipStream_t);

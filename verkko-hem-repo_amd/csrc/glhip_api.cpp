@@ -3,6 +3,12 @@
 // free energy.  Host orchestration of the HIP kernels in gl_assemble.hip and
 // gl_linalg.hip.  See include/glhip.h for the reference methods each entry
 // point replaces.
+//
+// Every context solves a batch of independent instances (a single mesh is a
+// batch of one).  Instances own disjoint node/cell ranges of one concatenated
+// mesh, so assembly, SpMV and the block-Jacobi preconditioner run unchanged on
+// the block-diagonal system; dot products, Gram-Schmidt coefficients, Givens
+// rotations, convergence tests and the Newton line search are per instance.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,6 +51,31 @@ struct DBuf {
   }
 };
 
+// pinned host staging for the small per-instance arrays exchanged every
+// GMRES iteration
+template <typename T>
+struct HBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count)
+  {
+    if (count <= n && p) return hipSuccess;
+    release();
+    hipError_t e = hipHostMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault);
+    if (e != hipSuccess) { p = nullptr; return e; }
+    n = count;
+    return hipSuccess;
+  }
+  void release()
+  {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+constexpr long long CHUNK = 4096;  // dofs per reduction chunk
+
 }  // namespace
 
 struct glh_ctx {
@@ -53,40 +84,51 @@ struct glh_ctx {
   std::string err;
   bool have_mat = false, have_mesh = false;
   glh::Material mat{};
+  std::vector<glh::Material> mats_host;  // per instance when set by glh_set_materials
   // mesh
   int deg = 1, ns = 8, nq = 8, npair = 36;
   int n_cells = 0, n_nodes = 0, nnzb = 0;
   std::vector<int> color_ptr;
+  std::vector<int> node_off, cell_off;  // instance ranges
+  int n_seg = 1, n_chunks = 0;
   // reference tables
   DBuf<double> tN, tdN, tw, tWp, tSp, tMf;
   DBuf<int> tpa, tpb;
   // mesh arrays
-  DBuf<int> cell_nodes, row_ptr, col, diag, cell_blk, color_cells;
+  DBuf<int> cell_nodes, row_ptr, col, diag, cell_blk, color_cells, cell_mat, seg_chunk;
+  DBuf<long long> chunks;
+  DBuf<glh::Material> mats;
   DBuf<double> cell_h;
   DBuf<uint32_t> cell_face, node_mask;
   // state vectors (FP64)
-  DBuf<double> U, U0, dU, X, rhs, res, Fcell, red_partial, red_out;
+  DBuf<double> U, U0, dU, X, rhs, res, Fcell;
+  // per-instance small device arrays
+  DBuf<double> s_norm, s_alpha, s_part;
+  DBuf<int> s_flag;
   // Jacobian and preconditioner
   DBuf<double> J64, D64;
   DBuf<float> J32, D32;
   bool j_fp32 = false, j_valid = false, d_valid = false;
   DBuf<int> status;
   // GMRES workspace
-  DBuf<double> gV64, gZ64, gW64, gh64, gp64, gy64;
-  DBuf<float> gV32, gZ32, gW32, gh32, gp32, gy32, gb32, gx32;
-  DBuf<double> ir_r, ir_c;
+  DBuf<double> gV64, gZ64, gW64, gh64, gp64, gy64, gs64, gq64;
+  DBuf<float> gV32, gZ32, gW32, gh32, gp32, gy32, gs32, gq32, gb32, gx32;
+  HBuf<double> pin64, pinv;
+  HBuf<float> pin32;
+  DBuf<double> ir_r;
+  // per-instance results of the last Newton solve: 8 values per instance
+  std::vector<double> inst_stats;
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double asm_ms = 0.0, asm_launches = 0;
-  // live kernel timing (glh_set_timing): event pairs around each SpMV
-  // launch on the context stream, read back when the pool fills or a solve ends
+  // live kernel timing (glh_set_timing): event pairs around each SpMV launch
+  // on the context stream, read back when the pool fills or a solve ends
   bool timing = false;
   long reorth_count = 0;
   std::vector<hipEvent_t> ev_pool;
   int ev_used = 0;
   double spmv_ms_total = 0.0, spmv_count = 0;
   double jasm_ms_total = 0.0, jasm_count = 0, rasm_ms_total = 0.0, rasm_count = 0;
-  double last_rhs_norm = -1.0;
 
   glh::RefTables tables() const
   {
@@ -94,8 +136,11 @@ struct glh_ctx {
   }
   glh::MeshDev meshdev() const
   {
-    return glh::MeshDev{cell_nodes.p, cell_h.p, cell_face.p, node_mask.p, cell_blk.p};
+    const bool per = !mats_host.empty();
+    return glh::MeshDev{per ? cell_mat.p : nullptr, per ? mats.p : nullptr, cell_nodes.p, cell_h.p,
+                        cell_face.p,          node_mask.p,                  cell_blk.p};
   }
+  glh::SegInfo seg() const { return glh::SegInfo{chunks.p, seg_chunk.p, n_chunks, n_seg}; }
 };
 
 namespace {
@@ -106,11 +151,17 @@ int fail(glh_ctx *c, int code, const std::string &msg)
   return code;
 }
 
-#define HIPCHK(ctx, ...)                                                                       \
-  do {                                                                                         \
-    hipError_t e_ = (__VA_ARGS__);                                                             \
-    if (e_ != hipSuccess)                                                                      \
+#define HIPCHK(ctx, ...)                                                                         \
+  do {                                                                                           \
+    hipError_t e_ = (__VA_ARGS__);                                                               \
+    if (e_ != hipSuccess)                                                                        \
       return fail((ctx), GLH_ERR_HIP, std::string(#__VA_ARGS__) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define RCHK(expr)       \
+  do {                   \
+    int rc_ = (expr);    \
+    if (rc_) return rc_; \
   } while (0)
 
 template <typename T>
@@ -120,6 +171,11 @@ hipError_t upload(DBuf<T> &b, const T *src, size_t n, hipStream_t s)
   if (e != hipSuccess) return e;
   if (n == 0) return hipSuccess;
   return hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+double now_s()
+{
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 // 1D Gauss points on [0,1] and Lagrange basis on equispaced support points
@@ -216,23 +272,35 @@ int check_ready(glh_ctx *c, bool need_mat = true)
   if (!c) return GLH_ERR_ARG;
   if (!c->have_mesh) return fail(c, GLH_ERR_STATE, "no mesh uploaded");
   if (need_mat && !c->have_mat) return fail(c, GLH_ERR_STATE, "material not set");
+  if (need_mat && !c->mats_host.empty() && (int)c->mats_host.size() != c->n_seg)
+    return fail(c, GLH_ERR_STATE, "per-instance materials do not match the uploaded batch");
   HIPCHK(c, hipSetDevice(c->device));
   return GLH_OK;
 }
 
 size_t ndofs(const glh_ctx *c) { return (size_t)c->n_nodes * NC; }
 
-// ||v|| of an FP64 device vector (deterministic), result on host
-int norm64(glh_ctx *c, const double *v, double *out)
+// per-instance ||v_s||^2 of an FP64 device vector into out (host, n_seg values)
+int seg_norm2(glh_ctx *c, const double *v, std::vector<double> &out)
 {
   const size_t n = ndofs(c);
-  HIPCHK(c, c->red_partial.alloc(glh::MDOT_GRID * 2));
-  HIPCHK(c, c->red_out.alloc(4));
-  HIPCHK(c, glh::launch_mdot<double>(n, 1, v, n, v, c->red_partial.p, c->red_out.p, false, c->stream));
-  double h = 0;
-  HIPCHK(c, hipMemcpyAsync(&h, c->red_out.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, c->s_part.alloc((size_t)std::max(1, c->n_chunks)));
+  HIPCHK(c, c->s_norm.alloc(c->n_seg));
+  HIPCHK(c, c->pinv.alloc(c->n_seg));
+  HIPCHK(c, glh::launch_seg_mdot<double>(c->seg(), 1, v, n, v, c->s_part.p, c->s_norm.p, 1, 0, false, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->pinv.p, c->s_norm.p, c->n_seg * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  *out = std::sqrt(h);
+  out.assign(c->pinv.p, c->pinv.p + c->n_seg);
+  return GLH_OK;
+}
+
+int norm64(glh_ctx *c, const double *v, double *out)
+{
+  std::vector<double> s;
+  RCHK(seg_norm2(c, v, s));
+  double t = 0;
+  for (double x : s) t += x;
+  *out = std::sqrt(t);
   return GLH_OK;
 }
 
@@ -305,8 +373,11 @@ struct Work<double> {
   static DBuf<double> &Z(glh_ctx *c) { return c->gZ64; }
   static DBuf<double> &W(glh_ctx *c) { return c->gW64; }
   static DBuf<double> &h(glh_ctx *c) { return c->gh64; }
+  static DBuf<double> &hp(glh_ctx *c) { return c->gq64; }
   static DBuf<double> &part(glh_ctx *c) { return c->gp64; }
   static DBuf<double> &y(glh_ctx *c) { return c->gy64; }
+  static DBuf<double> &sc(glh_ctx *c) { return c->gs64; }
+  static HBuf<double> &pin(glh_ctx *c) { return c->pin64; }
 };
 template <>
 struct Work<float> {
@@ -314,8 +385,11 @@ struct Work<float> {
   static DBuf<float> &Z(glh_ctx *c) { return c->gZ32; }
   static DBuf<float> &W(glh_ctx *c) { return c->gW32; }
   static DBuf<float> &h(glh_ctx *c) { return c->gh32; }
+  static DBuf<float> &hp(glh_ctx *c) { return c->gq32; }
   static DBuf<float> &part(glh_ctx *c) { return c->gp32; }
   static DBuf<float> &y(glh_ctx *c) { return c->gy32; }
+  static DBuf<float> &sc(glh_ctx *c) { return c->gs32; }
+  static HBuf<float> &pin(glh_ctx *c) { return c->pin32; }
 };
 
 int flush_events(glh_ctx *c)
@@ -337,10 +411,7 @@ int spmv(glh_ctx *c, const TJ *J, const TV *x, TV *y)
 {
   const bool t = c->timing && !c->ev_pool.empty();
   if (t) {
-    if (2 * c->ev_used >= (int)c->ev_pool.size()) {
-      int rc = flush_events(c);
-      if (rc) return rc;
-    }
+    if (2 * c->ev_used >= (int)c->ev_pool.size()) RCHK(flush_events(c));
     HIPCHK(c, hipEventRecord(c->ev_pool[2 * c->ev_used], c->stream));
   }
   HIPCHK(c, glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, x, y, c->stream));
@@ -352,204 +423,310 @@ int spmv(glh_ctx *c, const TJ *J, const TV *x, TV *y)
 }
 
 // Right-preconditioned restarted GMRES(m) with 18x18 point-block Jacobi and
-// classical Gram-Schmidt with selective reorthogonalisation on the device.  Solves J x = b (b, x device vectors
-// of type TV) to ||b - J x|| <= rtol ||b||.  Hessenberg/Givens on the host.
+// classical Gram-Schmidt with selective reorthogonalisation, batched over the
+// instances of the context: every instance has its own Krylov coefficients,
+// Givens rotations and stopping test ||b_s - J x_s|| <= rtol[s] ||b_s||; all
+// instances advance in lockstep through one launch per vector operation.
+// its_out: lockstep iterations; its_seg: iterations each instance used.
 template <typename TJ, typename TV>
-int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, double rtol, int maxit, int m, int *its_out,
-          double *rel_out)
+int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std::vector<double> &rtol, int maxit,
+          int m, int *its_out, std::vector<double> &rel_out, std::vector<int> &its_seg)
 {
   const size_t n = ndofs(c);
+  const int S = c->n_seg, ldh = m + 2;
+  const glh::SegInfo SI = c->seg();
   hipStream_t s = c->stream;
   auto &V = Work<TV>::V(c);
   auto &Z = Work<TV>::Z(c);
   auto &W = Work<TV>::W(c);
-  auto &hb = Work<TV>::h(c);
+  auto &hs = Work<TV>::h(c);
+  auto &hq = Work<TV>::hp(c);
   auto &pt = Work<TV>::part(c);
   auto &yb = Work<TV>::y(c);
+  auto &sc = Work<TV>::sc(c);
+  auto &pin = Work<TV>::pin(c);
   HIPCHK(c, V.alloc(n * (m + 1)));
   HIPCHK(c, Z.alloc(n));
   HIPCHK(c, W.alloc(n));
-  HIPCHK(c, hb.alloc(2 * (m + 2)));
-  HIPCHK(c, V.alloc(n * (m + 1)));
-  HIPCHK(c, pt.alloc((size_t)glh::MDOT_GRID * (m + 2)));
-  HIPCHK(c, yb.alloc(m + 1));
-  TV *hsum = hb.p, *hpass = hb.p + (m + 2);
-  std::vector<TV> hh(m + 3);
-  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m);
-  std::vector<TV> yh(m);
+  HIPCHK(c, hs.alloc((size_t)S * ldh));
+  HIPCHK(c, hq.alloc((size_t)S * ldh));
+  HIPCHK(c, yb.alloc((size_t)S * ldh));
+  HIPCHK(c, sc.alloc((size_t)S * 2));
+  HIPCHK(c, pt.alloc((size_t)std::max(1, c->n_chunks) * (m + 2)));
+  HIPCHK(c, pin.alloc((size_t)S * ldh * 2 + 2 * S));
+  TV *ph = pin.p, *pq = pin.p + (size_t)S * ldh, *pn = pin.p + (size_t)2 * S * ldh;
+  TV *nrm = sc.p;  // [S] per-instance squared norms
+  std::vector<double> H((size_t)S * (m + 1) * m, 0.0), cs((size_t)S * m), sn((size_t)S * m), g((size_t)S * (m + 1));
+  std::vector<double> bn(S), target(S), beta(S, 0.0), hn(S), y(m);
+  std::vector<int> jseg(S, 0);
+  std::vector<char> cyc(S, 0);
+  its_seg.assign(S, 0);
   HIPCHK(c, hipMemsetAsync(x, 0, n * sizeof(TV), s));
-  // ||b||
-  HIPCHK(c, glh::launch_mdot<TV>(n, 1, b, n, b, pt.p, hsum, false, s));
-  TV bn2 = 0;
-  HIPCHK(c, hipMemcpyAsync(&bn2, hsum, sizeof(TV), hipMemcpyDeviceToHost, s));
+  // ||b_s||
+  HIPCHK(c, glh::launch_seg_mdot<TV>(SI, 1, b, n, b, pt.p, nrm, 1, 0, false, s));
+  HIPCHK(c, hipMemcpyAsync(pn, nrm, S * sizeof(TV), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
-  const double bn = std::sqrt((double)bn2);
-  const double target = rtol * bn;
+  for (int i = 0; i < S; ++i) {
+    bn[i] = std::sqrt(std::max(0.0, (double)pn[i]));
+    target[i] = rtol[i] * bn[i];
+  }
   int it = 0;
-  double res = bn;
   bool conv = false;
   while (true) {
-    // V0 = b - J x
+    // V0 = b - J x, with per-instance ||V0||^2
+    HIPCHK(c, hipMemcpyAsync(V.p, b, n * sizeof(TV), hipMemcpyDeviceToDevice, s));
     if (it == 0) {
-      HIPCHK(c, hipMemcpyAsync(V.p, b, n * sizeof(TV), hipMemcpyDeviceToDevice, s));
+      HIPCHK(c, glh::launch_seg_mdot<TV>(SI, 1, V.p, n, V.p, pt.p, nrm, 1, 0, false, s));
     } else {
-      int rc = spmv<TJ, TV>(c, J, x, W.p);
-      if (rc) return rc;
-      HIPCHK(c, hipMemcpyAsync(V.p, b, n * sizeof(TV), hipMemcpyDeviceToDevice, s));
-      HIPCHK(c, glh::launch_set_scalar<TV>(hpass, (TV)1, s));
-      HIPCHK(c, glh::launch_sub_lincomb<TV>(n, 1, W.p, n, hpass, V.p, s));
+      RCHK((spmv<TJ, TV>(c, J, x, W.p)));
+      for (int i = 0; i < S; ++i) ph[(size_t)i * ldh] = (TV)1;
+      HIPCHK(c, hipMemcpyAsync(hq.p, ph, (size_t)S * ldh * sizeof(TV), hipMemcpyHostToDevice, s));
+      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SI, 1, W.p, n, hq.p, ldh, V.p, pt.p, nrm, 1, 0, s));
     }
-    HIPCHK(c, glh::launch_mdot<TV>(n, 1, V.p, n, V.p, pt.p, hsum, false, s));
-    TV beta2 = 0;
-    HIPCHK(c, hipMemcpyAsync(&beta2, hsum, sizeof(TV), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(pn, nrm, S * sizeof(TV), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
-    const double beta = std::sqrt((double)beta2);
-    res = beta;
-    if (beta <= target) { conv = true; break; }
+    bool any = false;
+    for (int i = 0; i < S; ++i) {
+      beta[i] = std::sqrt(std::max(0.0, (double)pn[i]));
+      cyc[i] = beta[i] > target[i];
+      any = any || cyc[i];
+    }
+    if (!any) { conv = true; break; }
     if (it >= maxit) break;
-    HIPCHK(c, glh::launch_set_scalar<TV>(hpass, (TV)beta, s));
-    HIPCHK(c, glh::launch_scale_by<TV>(n, V.p, hpass, V.p, s));
-    std::fill(g.begin(), g.end(), 0.0);
-    g[0] = beta;
+    for (int i = 0; i < S; ++i) pn[i] = (TV)beta[i];
+    HIPCHK(c, hipMemcpyAsync(sc.p + S, pn, S * sizeof(TV), hipMemcpyHostToDevice, s));
+    HIPCHK(c, glh::launch_seg_scale<TV>(SI, V.p, sc.p + S, 1, V.p, s));
+    for (int i = 0; i < S; ++i) {
+      std::fill(g.begin() + (size_t)i * (m + 1), g.begin() + (size_t)(i + 1) * (m + 1), 0.0);
+      g[(size_t)i * (m + 1)] = beta[i];
+      jseg[i] = 0;
+    }
     int j = 0;
-    bool est = false;
-    while (j < m && it < maxit && !est) {
+    while (j < m && it < maxit && std::any_of(cyc.begin(), cyc.end(), [](char v) { return v != 0; })) {
       TV *vj = V.p + n * j;
       TV *wv = V.p + n * (j + 1);  // the new direction is built in place
-      HIPCHK(c, glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, vj, Z.p, s));
-      int rc = spmv<TJ, TV>(c, J, Z.p, wv);
-      if (rc) return rc;
-      // classical Gram-Schmidt: h = V^T w together with ||w||^2 in one pass,
-      // then w -= V h with ||w'||^2; a second pass only when
-      // ||w'|| < ||w|| / sqrt(2) (Daniel-Gragg-Kaufman-Stewart criterion)
-      HIPCHK(c, glh::launch_mdot<TV>(n, j + 2, V.p, n, wv, pt.p, hsum, false, s));
-      HIPCHK(c, glh::launch_sub_lincomb_norm<TV>(n, j + 1, V.p, n, hsum, wv, pt.p, hpass + (j + 1), s));
-      HIPCHK(c, hipMemcpyAsync(hh.data(), hsum, (j + 2) * sizeof(TV), hipMemcpyDeviceToHost, s));
-      TV hn2 = 0;
-      HIPCHK(c, hipMemcpyAsync(&hn2, hpass + (j + 1), sizeof(TV), hipMemcpyDeviceToHost, s));
+      HIPCHK(c, (glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, vj, Z.p, s)));
+      RCHK((spmv<TJ, TV>(c, J, Z.p, wv)));
+      // h = V^T w with ||w||^2 in one pass, w -= V h with ||w'||^2; a second
+      // pass only when ||w'|| < ||w|| / sqrt(2) for some iterating instance
+      // (Daniel-Gragg-Kaufman-Stewart criterion)
+      HIPCHK(c, glh::launch_seg_mdot<TV>(SI, j + 2, V.p, n, wv, pt.p, hs.p, ldh, 0, false, s));
+      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SI, j + 1, V.p, n, hs.p, ldh, wv, pt.p, nrm, 1, 0, s));
+      HIPCHK(c, hipMemcpyAsync(ph, hs.p, (size_t)S * ldh * sizeof(TV), hipMemcpyDeviceToHost, s));
+      HIPCHK(c, hipMemcpyAsync(pn, nrm, S * sizeof(TV), hipMemcpyDeviceToHost, s));
       HIPCHK(c, hipStreamSynchronize(s));
-      std::vector<double> h2(j + 1, 0.0);
-      if ((double)hn2 < 0.5 * (double)hh[j + 1]) {
-        HIPCHK(c, glh::launch_mdot<TV>(n, j + 1, V.p, n, wv, pt.p, hpass, false, s));
-        HIPCHK(c, glh::launch_sub_lincomb_norm<TV>(n, j + 1, V.p, n, hpass, wv, pt.p, hpass + (j + 1), s));
-        std::vector<TV> hp(j + 2);
-        HIPCHK(c, hipMemcpyAsync(hp.data(), hpass, (j + 2) * sizeof(TV), hipMemcpyDeviceToHost, s));
+      bool reorth = false;
+      for (int i = 0; i < S; ++i)
+        if (cyc[i] && (double)pn[i] < 0.5 * (double)ph[(size_t)i * ldh + j + 1]) reorth = true;
+      if (reorth) {
+        HIPCHK(c, glh::launch_seg_mdot<TV>(SI, j + 1, V.p, n, wv, pt.p, hq.p, ldh, 0, false, s));
+        HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SI, j + 1, V.p, n, hq.p, ldh, wv, pt.p, nrm, 1, 0, s));
+        HIPCHK(c, hipMemcpyAsync(pq, hq.p, (size_t)S * ldh * sizeof(TV), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipMemcpyAsync(pn, nrm, S * sizeof(TV), hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipStreamSynchronize(s));
-        for (int i = 0; i <= j; ++i) h2[i] = (double)hp[i];
-        hn2 = hp[j + 1];
         ++c->reorth_count;
       }
-      const double hn = std::sqrt(std::max(0.0, (double)hn2));
-      if (hn > 0) {
-        HIPCHK(c, glh::launch_set_scalar<TV>(hpass, (TV)hn, s));
-        HIPCHK(c, glh::launch_scale_by<TV>(n, wv, hpass, wv, s));
+      for (int i = 0; i < S; ++i) hn[i] = std::sqrt(std::max(0.0, (double)pn[i]));
+      for (int i = 0; i < S; ++i) pn[i] = (TV)hn[i];
+      HIPCHK(c, hipMemcpyAsync(sc.p + S, pn, S * sizeof(TV), hipMemcpyHostToDevice, s));
+      HIPCHK(c, glh::launch_seg_scale<TV>(SI, wv, sc.p + S, 1, wv, s));
+      for (int i = 0; i < S; ++i) {
+        if (!cyc[i]) continue;
+        double *Hi = H.data() + (size_t)i * (m + 1) * m;
+        double *ci = cs.data() + (size_t)i * m, *si = sn.data() + (size_t)i * m, *gi = g.data() + (size_t)i * (m + 1);
+        for (int r = 0; r <= j; ++r)
+          Hi[(size_t)r * m + j] = (double)ph[(size_t)i * ldh + r] + (reorth ? (double)pq[(size_t)i * ldh + r] : 0.0);
+        for (int r = 0; r < j; ++r) {
+          const double a = Hi[(size_t)r * m + j], b2 = Hi[(size_t)(r + 1) * m + j];
+          Hi[(size_t)r * m + j] = ci[r] * a + si[r] * b2;
+          Hi[(size_t)(r + 1) * m + j] = -si[r] * a + ci[r] * b2;
+        }
+        const double a = Hi[(size_t)j * m + j], rr = std::hypot(a, hn[i]);
+        ci[j] = rr > 0 ? a / rr : 1.0;
+        si[j] = rr > 0 ? hn[i] / rr : 0.0;
+        Hi[(size_t)j * m + j] = rr;
+        gi[j + 1] = -si[j] * gi[j];
+        gi[j] = ci[j] * gi[j];
+        jseg[i] = j + 1;
+        ++its_seg[i];
+        if (std::fabs(gi[j + 1]) <= target[i] || hn[i] == 0) cyc[i] = 0;
       }
-      for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = (double)hh[i] + h2[i];
-      for (int i = 0; i < j; ++i) {
-        const double a = H[(size_t)i * m + j], b2 = H[(size_t)(i + 1) * m + j];
-        H[(size_t)i * m + j] = cs[i] * a + sn[i] * b2;
-        H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b2;
-      }
-      const double a = H[(size_t)j * m + j], rr = std::hypot(a, hn);
-      cs[j] = rr > 0 ? a / rr : 1.0;
-      sn[j] = rr > 0 ? hn / rr : 0.0;
-      H[(size_t)j * m + j] = rr;
-      g[j + 1] = -sn[j] * g[j];
-      g[j] = cs[j] * g[j];
       ++j;
       ++it;
-      if (std::fabs(g[j]) <= target || hn == 0) est = true;
     }
-    for (int i = j - 1; i >= 0; --i) {
-      double sum = g[i];
-      for (int k = i + 1; k < j; ++k) sum -= H[(size_t)i * m + k] * y[k];
-      y[i] = sum / H[(size_t)i * m + i];
+    // per-instance least-squares solution y, then x += M^-1 V y
+    std::fill(ph, ph + (size_t)S * ldh, (TV)0);
+    for (int i = 0; i < S; ++i) {
+      const int ji = jseg[i];
+      const double *Hi = H.data() + (size_t)i * (m + 1) * m, *gi = g.data() + (size_t)i * (m + 1);
+      for (int r = ji - 1; r >= 0; --r) {
+        double sum = gi[r];
+        for (int k = r + 1; k < ji; ++k) sum -= Hi[(size_t)r * m + k] * y[k];
+        y[r] = sum / Hi[(size_t)r * m + r];
+      }
+      for (int r = 0; r < ji; ++r) ph[(size_t)i * ldh + r] = (TV)y[r];
     }
-    for (int i = 0; i < j; ++i) yh[i] = (TV)y[i];
-    HIPCHK(c, hipMemcpyAsync(yb.p, yh.data(), j * sizeof(TV), hipMemcpyHostToDevice, s));
-    HIPCHK(c, glh::launch_lincomb<TV>(n, j, V.p, n, yb.p, W.p, s));
-    HIPCHK(c, glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, W.p, Z.p, s));
-    HIPCHK(c, glh::launch_axpy<TV, TV>(n, 1.0, Z.p, x, s));
+    HIPCHK(c, hipMemcpyAsync(yb.p, ph, (size_t)S * ldh * sizeof(TV), hipMemcpyHostToDevice, s));
+    HIPCHK(c, glh::launch_seg_lincomb<TV>(SI, j, V.p, n, yb.p, ldh, W.p, s));
+    HIPCHK(c, (glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, W.p, Z.p, s)));
+    HIPCHK(c, (glh::launch_axpy<TV, TV>(n, 1.0, Z.p, x, s)));
   }
-  {
-    int rc = flush_events(c);
-    if (rc) return rc;
-  }
+  RCHK(flush_events(c));
   *its_out = it;
-  *rel_out = bn > 0 ? res / bn : 0.0;
+  rel_out.assign(S, 0.0);
+  for (int i = 0; i < S; ++i) rel_out[i] = bn[i] > 0 ? beta[i] / bn[i] : 0.0;
   return conv ? GLH_OK : fail(c, GLH_ERR_NOCONV, "GMRES reached the maximum iteration number");
 }
 
-int solve_impl(glh_ctx *c, double rtol, int maxit, int restart, double stats[2])
+// J dU = rhs per instance to ||r_s|| <= rtol ||rhs_s||.  stats: [0] lockstep
+// iterations, [1] worst relative residual.
+int solve_impl(glh_ctx *c, double rtol, int maxit, int restart, double stats[2], std::vector<int> *its_seg_out)
 {
   if (!c->j_valid) return fail(c, GLH_ERR_STATE, "no Jacobian assembled");
   if (restart < 1 || maxit < 1 || !(rtol > 0)) return fail(c, GLH_ERR_ARG, "bad solver parameters");
-  int rc = ensure_precond(c);
-  if (rc) return rc;
+  RCHK(ensure_precond(c));
   const size_t n = ndofs(c);
+  const int S = c->n_seg;
   hipStream_t s = c->stream;
-  int its = 0;
-  double rel = 0;
+  int its = 0, rc = GLH_OK;
+  std::vector<double> rel;
+  std::vector<int> its_seg(S, 0);
   if (!c->j_fp32) {
-    rc = gmres<double, double>(c, c->J64.p, c->D64.p, c->rhs.p, c->dU.p, rtol, maxit, restart, &its, &rel);
+    rc = gmres<double, double>(c, c->J64.p, c->D64.p, c->rhs.p, c->dU.p, std::vector<double>(S, rtol), maxit, restart,
+                               &its, rel, its_seg);
   } else {
-    // FP32 GMRES inside an FP64 iterative-refinement loop:
+    // FP32 GMRES inside FP64 iterative refinement, per instance:
     //   r = rhs - J32 dU (FP64 arithmetic), solve J32 c = r in FP32, dU += c
     HIPCHK(c, c->gb32.alloc(n));
     HIPCHK(c, c->gx32.alloc(n));
     HIPCHK(c, c->ir_r.alloc(n));
+    HIPCHK(c, c->s_flag.alloc(S));
     HIPCHK(c, hipMemsetAsync(c->dU.p, 0, n * sizeof(double), s));
-    double bn = 0;
-    rc = norm64(c, c->rhs.p, &bn);
-    if (rc) return rc;
+    std::vector<double> bn2, rn2;
+    RCHK(seg_norm2(c, c->rhs.p, bn2));
     HIPCHK(c, hipMemcpyAsync(c->ir_r.p, c->rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    double rn = bn;
+    rn2 = bn2;
     const double inner = std::max(rtol, 1e-4);
-    for (int sweep = 0; sweep < 20 && rn > rtol * bn && its < maxit; ++sweep) {
-      HIPCHK(c, glh::launch_convert<double, float>(n, c->ir_r.p, c->gb32.p, s));
+    std::vector<int> flag(S);
+    for (int sweep = 0; sweep < 20 && its < maxit; ++sweep) {
+      bool any = false;
+      std::vector<double> rt(S, 1.0);
+      for (int i = 0; i < S; ++i) {
+        const double bn = std::sqrt(bn2[i]), rn = std::sqrt(rn2[i]);
+        flag[i] = rn > rtol * bn;
+        any = any || flag[i];
+        if (flag[i]) rt[i] = std::max(inner, rtol * bn / rn);
+      }
+      if (!any) break;
+      HIPCHK(c, hipMemcpyAsync(c->s_flag.p, flag.data(), S * sizeof(int), hipMemcpyHostToDevice, s));
+      HIPCHK(c, (glh::launch_convert<double, float>(n, c->ir_r.p, c->gb32.p, s)));
+      HIPCHK(c, glh::launch_seg_zero<float>(c->seg(), c->s_flag.p, c->gb32.p, s));
       int it2 = 0;
-      double rel2 = 0;
-      rc = gmres<float, float>(c, c->J32.p, c->D32.p, c->gb32.p, c->gx32.p, std::max(inner, rtol * bn / rn),
-                               maxit - its, restart, &it2, &rel2);
+      std::vector<double> rel2;
+      std::vector<int> is2;
+      rc = gmres<float, float>(c, c->J32.p, c->D32.p, c->gb32.p, c->gx32.p, rt, maxit - its, restart, &it2, rel2, is2);
       its += it2;
+      for (int i = 0; i < S; ++i) its_seg[i] += is2[i];
       if (rc && rc != GLH_ERR_NOCONV) return rc;
-      HIPCHK(c, glh::launch_axpy<float, double>(n, 1.0, c->gx32.p, c->dU.p, s));
+      HIPCHK(c, (glh::launch_axpy<float, double>(n, 1.0, c->gx32.p, c->dU.p, s)));
       HIPCHK(c, glh::launch_residual_spmv<float>(c->n_nodes, c->row_ptr.p, c->col.p, c->J32.p, c->dU.p, c->rhs.p,
                                                  c->ir_r.p, s));
-      rc = norm64(c, c->ir_r.p, &rn);
-      if (rc) return rc;
+      RCHK(seg_norm2(c, c->ir_r.p, rn2));
     }
-    rel = bn > 0 ? rn / bn : 0;
-    rc = rel <= rtol ? GLH_OK : fail(c, GLH_ERR_NOCONV, "mixed-precision GMRES did not reach the tolerance");
+    rel.assign(S, 0.0);
+    bool ok = true;
+    for (int i = 0; i < S; ++i) {
+      const double bn = std::sqrt(bn2[i]), rn = std::sqrt(rn2[i]);
+      rel[i] = bn > 0 ? rn / bn : 0.0;
+      ok = ok && rel[i] <= rtol;
+    }
+    rc = ok ? GLH_OK : fail(c, GLH_ERR_NOCONV, "mixed-precision GMRES did not reach the tolerance");
   }
-  if (stats) { stats[0] = its; stats[1] = rel; }
+  if (stats) {
+    stats[0] = its;
+    stats[1] = rel.empty() ? 0.0 : *std::max_element(rel.begin(), rel.end());
+  }
+  if (its_seg_out) *its_seg_out = its_seg;
   if (rc) return rc;
   // constraints_newton_update.distribute (solve.cc:181): Dirichlet components to zero
   HIPCHK(c, glh::launch_mask_zero(c->n_nodes, c->node_mask.p, c->dU.p, s));
   return GLH_OK;
 }
 
-int newton_step_impl(glh_ctx *c, double step, int damped, double out[4])
+// damped back-tracking line search of iteration.cc:170-206 per instance.
+// active[s] == 0: the instance is frozen (its update is zero).
+// prev/cur: per-instance ||rhs|| before, ||R|| after; alpha, trials per instance.
+int newton_step_impl(glh_ctx *c, double step, int damped, const std::vector<char> &active, std::vector<double> &prev,
+                     std::vector<double> &cur, std::vector<double> &alpha, std::vector<int> &trials)
 {
-  double prev = 0;
-  int rc = norm64(c, c->rhs.p, &prev);  // previous_residual = ||system_rhs|| (iteration.cc:130)
-  if (rc) return rc;
+  const int S = c->n_seg;
+  std::vector<double> p2, c2;
+  RCHK(seg_norm2(c, c->rhs.p, p2));  // previous_residual = ||system_rhs|| (iteration.cc:130)
+  prev.resize(S);
+  for (int i = 0; i < S; ++i) prev[i] = std::sqrt(p2[i]);
+  std::vector<char> acc(S);
+  alpha.assign(S, 0.0);
+  trials.assign(S, 0);
+  for (int i = 0; i < S; ++i) acc[i] = !active[i];
+  HIPCHK(c, c->s_alpha.alloc(S));
+  HIPCHK(c, c->pinv.alloc(S));
   const int n_try = damped ? 100 : 1;
-  double a = 1.0, cur = 0.0;
-  int i = 0;
-  for (i = 0; i < n_try; ++i) {
-    a = damped ? std::pow(step, (double)i) : 1.0;
-    HIPCHK(c, glh::launch_trial(c->n_nodes, c->U.p, c->dU.p, a, c->node_mask.p, c->X.p, c->stream));
-    rc = assemble_at(c, c->X.p, false, true, false, c->res.p, false);
-    if (rc) return rc;
-    rc = norm64(c, c->res.p, &cur);
-    if (rc) return rc;
-    if (cur < prev) break;
+  cur.assign(S, 0.0);
+  for (int t = 0; t < n_try; ++t) {
+    for (int i = 0; i < S; ++i)
+      if (!acc[i]) {
+        alpha[i] = damped ? std::pow(step, (double)t) : 1.0;
+        trials[i] = t + 1;
+      }
+    std::copy(alpha.begin(), alpha.end(), c->pinv.p);
+    HIPCHK(c, hipMemcpyAsync(c->s_alpha.p, c->pinv.p, S * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, glh::launch_seg_trial(c->seg(), c->U.p, c->dU.p, c->s_alpha.p, c->node_mask.p, c->X.p, c->stream));
+    RCHK(assemble_at(c, c->X.p, false, true, false, c->res.p, false));
+    RCHK(seg_norm2(c, c->res.p, c2));
+    bool all = true;
+    for (int i = 0; i < S; ++i) {
+      cur[i] = std::sqrt(c2[i]);
+      if (!acc[i] && cur[i] < prev[i]) acc[i] = 1;
+      all = all && acc[i];
+    }
+    if (all) break;
   }
-  if (i == n_try) i = n_try - 1;
   HIPCHK(c, hipMemcpyAsync(c->U.p, c->X.p, ndofs(c) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (out) { out[0] = prev; out[1] = cur; out[2] = a; out[3] = i + 1; }
+  return GLH_OK;
+}
+
+int instance_energy(glh_ctx *c, std::vector<double> &F)
+{
+  HIPCHK(c, glh::launch_energy(c->deg, c->tables(), c->meshdev(), c->mat, c->U.p, c->n_cells, c->Fcell.p,
+                               c->stream));
+  std::vector<double> fc(c->n_cells);
+  HIPCHK(c, hipMemcpyAsync(fc.data(), c->Fcell.p, c->n_cells * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  F.assign(c->n_seg, 0.0);
+  for (int s = 0; s < c->n_seg; ++s)
+    for (int e = c->cell_off[s]; e < c->cell_off[s + 1]; ++e) F[s] += fc[e];  // fixed order
+  return GLH_OK;
+}
+
+bool make_material(const double *co, glh::Material &m)
+{
+  // co: alpha, beta1..5, K1, K2, K3, bt
+  if (!(co[9] > 0)) return false;
+  m.al = co[0];
+  m.b1 = co[1]; m.b2 = co[2]; m.b3 = co[3]; m.b4 = co[4]; m.b5 = co[5];
+  m.K1 = co[6];
+  m.K23 = co[7] + co[8];
+  m.robin = (co[9] < 1e10) ? co[6] / co[9] : 0.0;
+  return true;
+}
+
+int get_vec(glh_ctx *c, const double *src, double *dst)
+{
+  RCHK(check_ready(c, false));
+  if (!dst) return fail(c, GLH_ERR_ARG, "null output");
+  HIPCHK(c, hipMemcpyAsync(dst, src, ndofs(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return GLH_OK;
 }
 
@@ -581,18 +758,24 @@ int glh_destroy(glh_ctx *c)
   if (!c) return GLH_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  DBuf<double> *dbs[] = {&c->tN, &c->tdN, &c->tw, &c->tWp, &c->tSp, &c->tMf, &c->cell_h, &c->U, &c->U0, &c->dU, &c->X,
-                         &c->rhs, &c->res, &c->Fcell, &c->red_partial, &c->red_out, &c->J64, &c->D64, &c->gV64,
-                         &c->gZ64, &c->gW64, &c->gh64, &c->gp64, &c->gy64, &c->ir_r, &c->ir_c};
+  DBuf<double> *dbs[] = {&c->tN,     &c->tdN,  &c->tw,   &c->tWp,  &c->tSp,   &c->tMf,   &c->cell_h,
+                         &c->U,      &c->U0,   &c->dU,   &c->X,    &c->rhs,   &c->res,   &c->Fcell,
+                         &c->s_norm, &c->s_alpha, &c->s_part, &c->J64, &c->D64, &c->gV64, &c->gZ64,
+                         &c->gW64,   &c->gh64, &c->gp64, &c->gy64, &c->gs64,  &c->gq64,  &c->ir_r};
   for (auto *b : dbs) b->release();
-  DBuf<float> *fbs[] = {&c->J32, &c->D32, &c->gV32, &c->gZ32, &c->gW32, &c->gh32, &c->gp32, &c->gy32, &c->gb32,
-                        &c->gx32};
+  DBuf<float> *fbs[] = {&c->J32,  &c->D32,  &c->gV32, &c->gZ32, &c->gW32, &c->gh32,
+                        &c->gp32, &c->gy32, &c->gs32, &c->gq32, &c->gb32, &c->gx32};
   for (auto *b : fbs) b->release();
-  DBuf<int> *ibs[] = {&c->tpa, &c->tpb, &c->cell_nodes, &c->row_ptr, &c->col, &c->diag, &c->cell_blk,
-                      &c->color_cells, &c->status};
+  DBuf<int> *ibs[] = {&c->tpa,      &c->tpb,         &c->cell_nodes, &c->row_ptr,  &c->col,       &c->diag,
+                      &c->cell_blk, &c->color_cells, &c->status,     &c->cell_mat, &c->seg_chunk, &c->s_flag};
   for (auto *b : ibs) b->release();
+  c->chunks.release();
+  c->mats.release();
   c->cell_face.release();
   c->node_mask.release();
+  c->pin64.release();
+  c->pinv.release();
+  c->pin32.release();
   for (auto e : c->ev_pool)
     if (e) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -607,12 +790,25 @@ const char *glh_last_error(const glh_ctx *c) { return c ? c->err.c_str() : "null
 int glh_set_material(glh_ctx *c, double alpha, const double beta[5], const double K[3], double bt)
 {
   if (!c || !beta || !K) return GLH_ERR_ARG;
-  if (!(bt > 0)) return fail(c, GLH_ERR_ARG, "AdGR diffuse length must be positive");
-  c->mat.al = alpha;
-  c->mat.b1 = beta[0]; c->mat.b2 = beta[1]; c->mat.b3 = beta[2]; c->mat.b4 = beta[3]; c->mat.b5 = beta[4];
-  c->mat.K1 = K[0];
-  c->mat.K23 = K[1] + K[2];
-  c->mat.robin = (bt < 1e10) ? K[0] / bt : 0.0;
+  const double co[10] = {alpha, beta[0], beta[1], beta[2], beta[3], beta[4], K[0], K[1], K[2], bt};
+  if (!make_material(co, c->mat)) return fail(c, GLH_ERR_ARG, "AdGR diffuse length must be positive");
+  c->mats_host.clear();
+  c->have_mat = true;
+  c->j_valid = c->d_valid = false;
+  return GLH_OK;
+}
+
+int glh_set_materials(glh_ctx *c, int n_inst, const double *coef)
+{
+  if (!c || !coef || n_inst < 1) return GLH_ERR_ARG;
+  std::vector<glh::Material> ms(n_inst);
+  for (int i = 0; i < n_inst; ++i)
+    if (!make_material(coef + 10 * i, ms[i])) return fail(c, GLH_ERR_ARG, "AdGR diffuse length must be positive");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, upload(c->mats, ms.data(), ms.size(), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->mats_host = ms;
+  c->mat = ms[0];
   c->have_mat = true;
   c->j_valid = c->d_valid = false;
   return GLH_OK;
@@ -633,6 +829,25 @@ int glh_mesh_box(int degree, int nx, int ny, int nz, const double lo[3], const d
   return GLH_OK;
 }
 
+int glh_mesh_concat(const glh_mesh *const *ms, int n, glh_mesh **out)
+{
+  if (!ms || !out || n < 1) return GLH_ERR_ARG;
+  *out = nullptr;
+  std::vector<const glh::Mesh *> v(n);
+  for (int i = 0; i < n; ++i) {
+    if (!ms[i]) return GLH_ERR_ARG;
+    v[i] = &ms[i]->m;
+  }
+  glh_mesh *m = new glh_mesh();
+  std::string err;
+  if (glh::concat(m->m, v, err)) {
+    delete m;
+    return GLH_ERR_ARG;
+  }
+  *out = m;
+  return GLH_OK;
+}
+
 int glh_mesh_destroy(glh_mesh *m)
 {
   delete m;
@@ -645,7 +860,7 @@ int glh_mesh_sizes(const glh_mesh *mm, int64_t sizes[8])
   const glh::Mesh &m = mm->m;
   sizes[0] = m.deg; sizes[1] = m.n_cells; sizes[2] = m.n_nodes; sizes[3] = (int64_t)m.faces.size() / 3;
   sizes[4] = m.row_ptr.empty() ? 0 : m.row_ptr.back(); sizes[5] = (int64_t)m.color_ptr.size() - 1;
-  sizes[6] = 0; sizes[7] = 0;
+  sizes[6] = (int64_t)m.node_off.size() - 1; sizes[7] = 0;
   return GLH_OK;
 }
 
@@ -663,20 +878,29 @@ int glh_mesh_get(const glh_mesh *mm, int *cell_nodes, double *cell_h, double *no
   return GLH_OK;
 }
 
+int glh_mesh_instances(const glh_mesh *mm, int *node_off, int *cell_off)
+{
+  if (!mm) return GLH_ERR_ARG;
+  const glh::Mesh &m = mm->m;
+  if (node_off) std::memcpy(node_off, m.node_off.data(), m.node_off.size() * sizeof(int));
+  if (cell_off) std::memcpy(cell_off, m.cell_off.data(), m.cell_off.size() * sizeof(int));
+  return GLH_OK;
+}
+
 int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
 {
   if (!c || !mm) return GLH_ERR_ARG;
   const glh::Mesh &m = mm->m;
   HIPCHK(c, hipSetDevice(c->device));
-  if (c->deg != m.deg || !c->tN.p) {
-    int rc = build_tables(c, m.deg);
-    if (rc) return rc;
-  }
+  if (c->deg != m.deg || !c->tN.p) RCHK(build_tables(c, m.deg));
   hipStream_t s = c->stream;
   c->n_cells = m.n_cells;
   c->n_nodes = m.n_nodes;
   c->nnzb = m.row_ptr.back();
   c->color_ptr = m.color_ptr;
+  c->node_off = m.node_off.empty() ? std::vector<int>{0, m.n_nodes} : m.node_off;
+  c->cell_off = m.cell_off.empty() ? std::vector<int>{0, m.n_cells} : m.cell_off;
+  c->n_seg = (int)c->node_off.size() - 1;
   // per-cell face codes (boundary id + 1, 3 bits per local face)
   std::vector<uint32_t> cface(m.n_cells, 0u);
   for (size_t f = 0; f + 2 < m.faces.size(); f += 3) {
@@ -706,6 +930,22 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
           cblk[((size_t)e * ns + a) * ns + b] = first ? -(blk + 1) : blk;
         }
     }
+  // instance of each cell, and reduction chunks that never straddle instances
+  std::vector<int> cmat(m.n_cells);
+  for (int sg = 0; sg < c->n_seg; ++sg)
+    for (int e = c->cell_off[sg]; e < c->cell_off[sg + 1]; ++e) cmat[e] = sg;
+  std::vector<long long> ch;
+  std::vector<int> segch(1, 0);
+  for (int sg = 0; sg < c->n_seg; ++sg) {
+    const long long b0 = (long long)c->node_off[sg] * NC, b1 = (long long)c->node_off[sg + 1] * NC;
+    for (long long b = b0; b < b1; b += CHUNK) {
+      ch.push_back(sg);
+      ch.push_back(b);
+      ch.push_back(std::min(b1, b + CHUNK));
+    }
+    segch.push_back((int)(ch.size() / 3));
+  }
+  c->n_chunks = (int)(ch.size() / 3);
   HIPCHK(c, upload(c->cell_nodes, m.cell_nodes.data(), m.cell_nodes.size(), s));
   HIPCHK(c, upload(c->cell_h, m.cell_h.data(), m.cell_h.size(), s));
   HIPCHK(c, upload(c->cell_face, cface.data(), cface.size(), s));
@@ -715,6 +955,9 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
   HIPCHK(c, upload(c->diag, diag.data(), diag.size(), s));
   HIPCHK(c, upload(c->cell_blk, cblk.data(), cblk.size(), s));
   HIPCHK(c, upload(c->color_cells, m.color_cells.data(), m.color_cells.size(), s));
+  HIPCHK(c, upload(c->cell_mat, cmat.data(), cmat.size(), s));
+  HIPCHK(c, upload(c->chunks, ch.data(), ch.size(), s));
+  HIPCHK(c, upload(c->seg_chunk, segch.data(), segch.size(), s));
   const size_t n = (size_t)m.n_nodes * NC;
   HIPCHK(c, c->U.alloc(n));
   HIPCHK(c, c->dU.alloc(n));
@@ -722,21 +965,19 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
   HIPCHK(c, c->rhs.alloc(n));
   HIPCHK(c, c->res.alloc(n));
   HIPCHK(c, c->Fcell.alloc(m.n_cells));
-  HIPCHK(c, c->red_partial.alloc(glh::MDOT_GRID * 2));
-  HIPCHK(c, c->red_out.alloc(4));
   HIPCHK(c, hipMemsetAsync(c->U.p, 0, n * sizeof(double), s));
   HIPCHK(c, hipMemsetAsync(c->dU.p, 0, n * sizeof(double), s));
   HIPCHK(c, hipMemsetAsync(c->rhs.p, 0, n * sizeof(double), s));
   HIPCHK(c, hipStreamSynchronize(s));
   c->j_valid = c->d_valid = false;
   c->have_mesh = true;
+  c->inst_stats.assign((size_t)c->n_seg * 8, 0.0);
   return GLH_OK;
 }
 
 int glh_set_solution(glh_ctx *c, const double *Uh)
 {
-  int rc = check_ready(c, false);
-  if (rc) return rc;
+  RCHK(check_ready(c, false));
   if (!Uh) return fail(c, GLH_ERR_ARG, "null solution");
   HIPCHK(c, hipMemcpyAsync(c->U.p, Uh, ndofs(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
   // constraints_solution.distribute (setup_uniform_B-phase.cc:262)
@@ -747,8 +988,7 @@ int glh_set_solution(glh_ctx *c, const double *Uh)
 
 int glh_store_initial(glh_ctx *c)
 {
-  int rc = check_ready(c, false);
-  if (rc) return rc;
+  RCHK(check_ready(c, false));
   HIPCHK(c, c->U0.alloc(ndofs(c)));
   HIPCHK(c, hipMemcpyAsync(c->U0.p, c->U.p, ndofs(c) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -757,20 +997,9 @@ int glh_store_initial(glh_ctx *c)
 
 int glh_restore_initial(glh_ctx *c)
 {
-  int rc = check_ready(c, false);
-  if (rc) return rc;
+  RCHK(check_ready(c, false));
   if (!c->U0.p || c->U0.n < ndofs(c)) return fail(c, GLH_ERR_STATE, "no stored initial state");
   HIPCHK(c, hipMemcpyAsync(c->U.p, c->U0.p, ndofs(c) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return GLH_OK;
-}
-
-static int get_vec(glh_ctx *c, const double *src, double *dst)
-{
-  int rc = check_ready(c, false);
-  if (rc) return rc;
-  if (!dst) return fail(c, GLH_ERR_ARG, "null output");
-  HIPCHK(c, hipMemcpyAsync(dst, src, ndofs(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return GLH_OK;
 }
@@ -781,8 +1010,7 @@ int glh_get_rhs(glh_ctx *c, double *r) { return c ? get_vec(c, c->rhs.p, r) : GL
 
 int glh_get_jacobian(glh_ctx *c, double *val)
 {
-  int rc = check_ready(c, false);
-  if (rc) return rc;
+  RCHK(check_ready(c, false));
   if (!c->j_valid) return fail(c, GLH_ERR_STATE, "no Jacobian assembled");
   const size_t nv = (size_t)c->nnzb * 324;
   if (!c->j_fp32) {
@@ -799,107 +1027,157 @@ int glh_get_jacobian(glh_ctx *c, double *val)
 
 int glh_assemble(glh_ctx *c, int flags)
 {
-  int rc = check_ready(c);
-  if (rc) return rc;
+  RCHK(check_ready(c));
   const bool dj = flags & GLH_ASM_J, dr = flags & GLH_ASM_R, f32 = flags & GLH_ASM_FP32;
   if (!dj && !dr) return fail(c, GLH_ERR_ARG, "nothing to assemble");
-  rc = assemble_at(c, c->U.p, dj, dr, f32, c->rhs.p, true);
-  if (rc) return rc;
+  RCHK(assemble_at(c, c->U.p, dj, dr, f32, c->rhs.p, true));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return GLH_OK;
 }
 
 int glh_rhs_norm(glh_ctx *c, double *nrm)
 {
-  int rc = check_ready(c);
-  if (rc) return rc;
+  RCHK(check_ready(c));
   return norm64(c, c->rhs.p, nrm);
 }
 
 int glh_residual(glh_ctx *c, double *nrm)
 {
-  int rc = check_ready(c);
-  if (rc) return rc;
-  rc = assemble_at(c, c->U.p, false, true, false, c->res.p, false);
-  if (rc) return rc;
+  RCHK(check_ready(c));
+  RCHK(assemble_at(c, c->U.p, false, true, false, c->res.p, false));
   return norm64(c, c->res.p, nrm);
 }
 
 int glh_solve(glh_ctx *c, double rtol, int maxit, int restart, double stats[2])
 {
-  int rc = check_ready(c);
-  if (rc) return rc;
-  return solve_impl(c, rtol, maxit, restart, stats);
+  RCHK(check_ready(c));
+  return solve_impl(c, rtol, maxit, restart, stats, nullptr);
 }
 
 int glh_newton_step(glh_ctx *c, double step, int damped, double out[4])
 {
-  int rc = check_ready(c);
-  if (rc) return rc;
+  RCHK(check_ready(c));
   if (damped && !(step > 0 && step < 1)) return fail(c, GLH_ERR_ARG, "line-search step must be in (0,1)");
-  return newton_step_impl(c, step, damped, out);
+  std::vector<double> prev, cur, alpha;
+  std::vector<int> trials;
+  RCHK(newton_step_impl(c, step, damped, std::vector<char>(c->n_seg, 1), prev, cur, alpha, trials));
+  if (out) {
+    double p2 = 0, c2 = 0;
+    for (int i = 0; i < c->n_seg; ++i) { p2 += prev[i] * prev[i]; c2 += cur[i] * cur[i]; }
+    out[0] = std::sqrt(p2);
+    out[1] = std::sqrt(c2);
+    out[2] = alpha[0];
+    out[3] = *std::max_element(trials.begin(), trials.end());
+  }
+  return GLH_OK;
 }
 
 int glh_free_energy(glh_ctx *c, double *F)
 {
-  int rc = check_ready(c);
-  if (rc) return rc;
+  RCHK(check_ready(c));
   if (!F) return fail(c, GLH_ERR_ARG, "null output");
-  HIPCHK(c, glh::launch_energy(c->deg, c->tables(), c->meshdev(), c->mat, c->U.p, c->n_cells, c->Fcell.p,
-                               c->stream));
-  HIPCHK(c, c->red_partial.alloc(glh::MDOT_GRID * 2));
-  HIPCHK(c, glh::launch_sum(c->n_cells, c->Fcell.p, c->red_partial.p, c->red_out.p, c->stream));
-  HIPCHK(c, hipMemcpyAsync(F, c->red_out.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<double> f;
+  RCHK(instance_energy(c, f));
+  double t = 0;
+  for (double v : f) t += v;
+  *F = t;
+  return GLH_OK;
+}
+
+int glh_instance_energy(glh_ctx *c, double *F)
+{
+  RCHK(check_ready(c));
+  if (!F) return fail(c, GLH_ERR_ARG, "null output");
+  std::vector<double> f;
+  RCHK(instance_energy(c, f));
+  std::copy(f.begin(), f.end(), F);
   return GLH_OK;
 }
 
 int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, int lin_maxit, int restart,
                      double ls_step, int damped, int fp32, double stats[10])
 {
-  int rc = check_ready(c);
-  if (rc) return rc;
-  using clk = std::chrono::steady_clock;
+  RCHK(check_ready(c));
+  const int S = c->n_seg;
   double t_asm = 0, t_sol = 0, t_ls = 0;
-  int lin_total = 0, k = 0, trials = 0;
-  auto t0 = clk::now();
-  rc = assemble_at(c, c->U.p, false, true, false, c->rhs.p, false);
-  if (rc) return rc;
-  double r0 = 0;
-  rc = norm64(c, c->rhs.p, &r0);
-  if (rc) return rc;
-  t_asm += std::chrono::duration<double>(clk::now() - t0).count();
-  double rcur = r0;
+  int lin_total = 0, k = 0, trials_total = 0;
+  const double t_start = now_s();
+  double t0 = t_start;
+  RCHK(assemble_at(c, c->U.p, false, true, false, c->rhs.p, false));
+  std::vector<double> r0(S), rcur(S), r2;
+  RCHK(seg_norm2(c, c->rhs.p, r2));
+  for (int i = 0; i < S; ++i) r0[i] = rcur[i] = std::sqrt(r2[i]);
+  t_asm += now_s() - t0;
+  std::vector<double> &st = c->inst_stats;
+  st.assign((size_t)S * 8, 0.0);
+  for (int i = 0; i < S; ++i) { st[8 * i + 2] = r0[i]; st[8 * i + 4] = -1.0; }
+  std::vector<char> active(S);
+  HIPCHK(c, c->s_flag.alloc(S));
   const bool f32 = fp32 != 0;
-  for (k = 0; k < max_newton; ++k) {
-    if (rcur <= rtol * r0) break;
-    t0 = clk::now();
-    rc = assemble_at(c, c->U.p, true, true, f32, c->rhs.p, false);
-    if (rc) return rc;
+  std::vector<int> flag(S);
+  for (k = 0; k <= max_newton; ++k) {
+    bool any = false;
+    const double tnow = now_s() - t_start;
+    for (int i = 0; i < S; ++i) {
+      active[i] = rcur[i] > rtol * r0[i];
+      if (!active[i] && st[8 * i + 4] < 0) st[8 * i + 4] = tnow;
+      any = any || active[i];
+    }
+    if (!any || k == max_newton) break;
+    t0 = now_s();
+    RCHK(assemble_at(c, c->U.p, true, true, f32, c->rhs.p, false));
+    // frozen instances see a zero right-hand side, so GMRES leaves them at dU = 0
+    for (int i = 0; i < S; ++i) flag[i] = active[i];
+    HIPCHK(c, hipMemcpyAsync(c->s_flag.p, flag.data(), S * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, glh::launch_seg_zero<double>(c->seg(), c->s_flag.p, c->rhs.p, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    auto t1 = clk::now();
-    t_asm += std::chrono::duration<double>(t1 - t0).count();
-    double st[2] = {0, 0};
-    rc = solve_impl(c, lin_tol, lin_maxit, restart, st);
-    lin_total += (int)st[0];
-    if (rc) return rc;
-    auto t2 = clk::now();
-    t_sol += std::chrono::duration<double>(t2 - t1).count();
-    double out[4];
-    rc = newton_step_impl(c, ls_step, damped, out);
-    if (rc) return rc;
-    rcur = out[1];
-    trials += (int)out[3];
-    t_ls += std::chrono::duration<double>(clk::now() - t2).count();
+    const double t1 = now_s();
+    t_asm += t1 - t0;
+    double sst[2] = {0, 0};
+    std::vector<int> its_seg;
+    RCHK(solve_impl(c, lin_tol, lin_maxit, restart, sst, &its_seg));
+    lin_total += (int)sst[0];
+    const double t2 = now_s();
+    t_sol += t2 - t1;
+    std::vector<double> prev, cur, alpha;
+    std::vector<int> trials;
+    RCHK(newton_step_impl(c, ls_step, damped, active, prev, cur, alpha, trials));
+    for (int i = 0; i < S; ++i)
+      if (active[i]) {
+        rcur[i] = cur[i];
+        st[8 * i + 0] += 1;
+        st[8 * i + 6] += trials[i];
+        st[8 * i + 7] += its_seg[i];
+      }
+    trials_total += *std::max_element(trials.begin(), trials.end());
+    t_ls += now_s() - t2;
   }
-  double F = 0;
-  rc = glh_free_energy(c, &F);
-  if (rc) return rc;
+  std::vector<double> F;
+  RCHK(instance_energy(c, F));
+  bool all = true;
+  double r0t = 0, rft = 0, Ft = 0;
+  for (int i = 0; i < S; ++i) {
+    const bool ok = rcur[i] <= rtol * r0[i];
+    all = all && ok;
+    st[8 * i + 1] = ok ? 1.0 : 0.0;
+    st[8 * i + 3] = rcur[i];
+    st[8 * i + 5] = F[i];
+    r0t += r0[i] * r0[i];
+    rft += rcur[i] * rcur[i];
+    Ft += F[i];
+  }
   if (stats) {
-    stats[0] = k; stats[1] = lin_total; stats[2] = r0; stats[3] = rcur;
-    stats[4] = t_asm; stats[5] = t_sol; stats[6] = t_ls; stats[7] = F;
-    stats[8] = trials; stats[9] = (rcur <= rtol * r0) ? 1.0 : 0.0;
+    stats[0] = k; stats[1] = lin_total; stats[2] = std::sqrt(r0t); stats[3] = std::sqrt(rft);
+    stats[4] = t_asm; stats[5] = t_sol; stats[6] = t_ls; stats[7] = Ft;
+    stats[8] = trials_total; stats[9] = all ? 1.0 : 0.0;
   }
+  return GLH_OK;
+}
+
+int glh_instance_stats(glh_ctx *c, double *out)
+{
+  if (!c || !out) return GLH_ERR_ARG;
+  std::copy(c->inst_stats.begin(), c->inst_stats.end(), out);
   return GLH_OK;
 }
 
@@ -929,8 +1207,7 @@ int glh_set_timing(glh_ctx *c, int on)
 int glh_timing(glh_ctx *c, double out[6])
 {
   if (!c || !out) return GLH_ERR_ARG;
-  int rc = flush_events(c);
-  if (rc) return rc;
+  RCHK(flush_events(c));
   out[0] = c->spmv_ms_total; out[1] = c->spmv_count;
   out[2] = c->jasm_ms_total; out[3] = c->jasm_count;
   out[4] = c->rasm_ms_total; out[5] = c->rasm_count;
@@ -939,21 +1216,21 @@ int glh_timing(glh_ctx *c, double out[6])
 
 int glh_bench_spmv(glh_ctx *c, int k, double *ms_avg)
 {
-  int rc = check_ready(c);
-  if (rc) return rc;
+  RCHK(check_ready(c));
   if (!c->j_valid || k < 1 || !ms_avg) return fail(c, GLH_ERR_STATE, "no Jacobian assembled");
+  if (c->j_fp32) {
+    HIPCHK(c, c->gW32.alloc(ndofs(c)));
+    HIPCHK(c, c->gZ32.alloc(ndofs(c)));
+    HIPCHK(c, (glh::launch_convert<double, float>(ndofs(c), c->U.p, c->gZ32.p, c->stream)));
+  }
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   for (int i = 0; i < k; ++i) {
-    if (c->j_fp32) {
-      HIPCHK(c, c->gW32.alloc(ndofs(c)));
-      HIPCHK(c, c->gZ32.alloc(ndofs(c)));
-      if (i == 0) HIPCHK(c, glh::launch_convert<double, float>(ndofs(c), c->U.p, c->gZ32.p, c->stream));
-      HIPCHK(c, glh::launch_spmv<float, float>(c->n_nodes, c->row_ptr.p, c->col.p, c->J32.p, c->gZ32.p, c->gW32.p,
-                                               c->stream));
-    } else {
-      HIPCHK(c, glh::launch_spmv<double, double>(c->n_nodes, c->row_ptr.p, c->col.p, c->J64.p, c->U.p, c->res.p,
-                                                 c->stream));
-    }
+    if (c->j_fp32)
+      HIPCHK(c, (glh::launch_spmv<float, float>(c->n_nodes, c->row_ptr.p, c->col.p, c->J32.p, c->gZ32.p,
+                                                c->gW32.p, c->stream)));
+    else
+      HIPCHK(c, (glh::launch_spmv<double, double>(c->n_nodes, c->row_ptr.p, c->col.p, c->J64.p, c->U.p, c->res.p,
+                                                  c->stream)));
   }
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));

@@ -26,6 +26,8 @@ struct Material {
 };
 
 struct MeshDev {
+  const int *cell_mat;       // [n_cells] material index, or null: one material for all cells
+  const Material *mats;      // per-instance materials (with cell_mat)
   const int *cell_nodes;     // [n_cells*NS]
   const double *cell_h;      // [n_cells*3]
   const uint32_t *cell_face; // [n_cells] 3 bits per local face: boundary id + 1 (0 = interior)
@@ -94,6 +96,32 @@ hipError_t launch_trial(int n_nodes, const double *U, const double *dU, double a
                         hipStream_t s);
 // zero the Dirichlet components of x
 hipError_t launch_mask_zero(int n_nodes, const uint32_t *mask, double *x, hipStream_t s);
+// ------------------------------------------------ segmented (batched) vectors
+// Instances own contiguous dof ranges.  chunks: [3*n_chunks] (instance, first
+// dof, end dof), no chunk straddles an instance; seg_chunk: [n_seg+1] chunk
+// ranges per instance.  Outputs are indexed out[s*ldo + off + i].
+struct SegInfo {
+  const long long *chunks;
+  const int *seg_chunk;
+  int n_chunks, n_seg;
+};
+template <typename T>
+hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, const T *w, T *partial, T *out, int ldo,
+                           int off, bool accumulate, hipStream_t s);
+// w -= sum_i h[s*ldh + i] V_i per instance; norm2[s*ldo + off] = ||w_s||^2 after
+template <typename T>
+hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size_t ldv, const T *h, int ldh, T *w,
+                                       T *partial, T *norm2, int ldo, int off, hipStream_t s);
+template <typename T>
+hipError_t launch_seg_lincomb(const SegInfo &S, int k, const T *V, size_t ldv, const T *y, int ldy, T *w,
+                              hipStream_t s);
+template <typename T>
+hipError_t launch_seg_scale(const SegInfo &S, const T *src, const T *scal, int lds, T *dst, hipStream_t s);
+hipError_t launch_seg_trial(const SegInfo &S, const double *U, const double *dU, const double *alpha,
+                            const uint32_t *mask, double *X, hipStream_t s);
+template <typename T>
+hipError_t launch_seg_zero(const SegInfo &S, const int *flag, T *v, hipStream_t s);
+
 // *p = v (stream-ordered scalar write)
 template <typename T>
 hipError_t launch_set_scalar(T *p, T v, hipStream_t s);

@@ -90,6 +90,61 @@ int make_box(Mesh &m, int deg, int nx, int ny, int nz, const double lo[3], const
     m.color_ptr.push_back((int)m.color_cells.size());
   }
   build_pattern(m);
+  m.node_off = {0, m.n_nodes};
+  m.cell_off = {0, m.n_cells};
+  return 0;
+}
+
+int concat(Mesh &out, const std::vector<const Mesh *> &ms, std::string &err)
+{
+  if (ms.empty()) { err = "empty batch"; return -1; }
+  out = Mesh();
+  out.deg = ms[0]->deg; out.n1 = ms[0]->n1; out.ns = ms[0]->ns;
+  long nn = 0, nc = 0, nb = 0;
+  size_t ncol = 0;
+  for (const Mesh *m : ms) {
+    if (m->deg != out.deg) { err = "all instances of a batch need the same degree"; return -1; }
+    nn += m->n_nodes; nc += m->n_cells; nb += m->row_ptr.back();
+    ncol = std::max(ncol, m->color_ptr.size() - 1);
+  }
+  if (nn >= (1L << 31) / NC || nc * out.ns >= (1L << 31) || nb >= (1L << 31)) {
+    err = "batch too large for 32-bit indices";
+    return -1;
+  }
+  out.n_nodes = (int)nn; out.n_cells = (int)nc;
+  out.node_off.assign(1, 0);
+  out.cell_off.assign(1, 0);
+  out.row_ptr.assign(1, 0);
+  int noff = 0, coff = 0;
+  for (const Mesh *m : ms) {
+    for (int v : m->cell_nodes) out.cell_nodes.push_back(v + noff);
+    out.cell_h.insert(out.cell_h.end(), m->cell_h.begin(), m->cell_h.end());
+    out.cell_lo.insert(out.cell_lo.end(), m->cell_lo.begin(), m->cell_lo.end());
+    out.node_xyz.insert(out.node_xyz.end(), m->node_xyz.begin(), m->node_xyz.end());
+    for (size_t f = 0; f + 2 < m->faces.size(); f += 3) {
+      out.faces.push_back(m->faces[f] + coff);
+      out.faces.push_back(m->faces[f + 1]);
+      out.faces.push_back(m->faces[f + 2]);
+    }
+    out.node_mask.insert(out.node_mask.end(), m->node_mask.begin(), m->node_mask.end());
+    const int base = out.row_ptr.back();
+    for (int r = 0; r < m->n_nodes; ++r) out.row_ptr.push_back(base + m->row_ptr[r + 1]);
+    for (int v : m->col) out.col.push_back(v + noff);
+    noff += m->n_nodes;
+    coff += m->n_cells;
+    out.node_off.push_back(noff);
+    out.cell_off.push_back(coff);
+  }
+  out.color_ptr.assign(1, 0);
+  for (size_t k = 0; k < ncol; ++k) {
+    int cbase = 0;
+    for (const Mesh *m : ms) {
+      if (k + 1 < m->color_ptr.size())
+        for (int i = m->color_ptr[k]; i < m->color_ptr[k + 1]; ++i) out.color_cells.push_back(m->color_cells[i] + cbase);
+      cbase += m->n_cells;
+    }
+    out.color_ptr.push_back((int)out.color_cells.size());
+  }
   return 0;
 }
 

@@ -24,6 +24,9 @@ struct Mesh {
   std::vector<uint32_t> node_mask;  // [n_nodes] bit c = component c Dirichlet-constrained
   std::vector<int> color_ptr, color_cells;
   std::vector<int> row_ptr, col;    // BSR pattern over nodes, sorted columns
+  // instances of a batch: instance i owns nodes [node_off[i], node_off[i+1])
+  // and cells [cell_off[i], cell_off[i+1]); a single mesh is one instance
+  std::vector<int> node_off, cell_off;
 };
 
 // Dirichlet component mask of a boundary id: 0 -> all 18 components (A = 0
@@ -41,6 +44,10 @@ void build_pattern(Mesh &m);
 
 // Greedy colouring of cells so that no two cells of one colour share a node.
 void greedy_colors(Mesh &m);
+
+// Concatenate meshes of one degree into a batch with disjoint node and cell
+// ranges; colour k of the batch is the union of colour k of every instance.
+int concat(Mesh &out, const std::vector<const Mesh *> &ms, std::string &err);
 
 // Index of block (r, s) in the pattern, or -1.
 int find_block(const Mesh &m, int r, int s);

@@ -21,7 +21,8 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 
-EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "glh_matep", "glh_mesh_box",
+EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "glh_set_materials", "glh_matep",
+           "glh_mesh_box", "glh_mesh_concat", "glh_mesh_instances", "glh_instance_energy", "glh_instance_stats",
            "glh_mesh_destroy", "glh_mesh_sizes", "glh_mesh_get", "glh_upload_mesh", "glh_initial_guess_host", "glh_set_solution", "glh_store_initial", "glh_restore_initial",
            "glh_get_solution", "glh_get_update", "glh_get_rhs", "glh_get_jacobian", "glh_assemble", "glh_rhs_norm",
            "glh_residual", "glh_solve", "glh_newton_step", "glh_free_energy", "glh_newton_solve",
@@ -50,6 +51,11 @@ def lib():
     L.glh_matep.argtypes = [C.c_double, C.c_double, C.c_int, _dp]
     L.glh_mesh_box.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, C.POINTER(vp)]
     L.glh_mesh_destroy.argtypes = [vp]
+    L.glh_set_materials.argtypes = [vp, C.c_int, _dp]
+    L.glh_mesh_concat.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
+    L.glh_mesh_instances.argtypes = [vp, _ip, _ip]
+    L.glh_instance_energy.argtypes = [vp, _dp]
+    L.glh_instance_stats.argtypes = [vp, _dp]
     L.glh_mesh_sizes.argtypes = [vp, _i64p]
     L.glh_mesh_get.argtypes = [vp] + [vp] * 9
     L.glh_upload_mesh.argtypes = [vp, vp]
@@ -95,16 +101,35 @@ def initial_guess(n_nodes, gap, phase=0, amp=0.05, seed=0):
 class Mesh:
     """Host mesh layer object (glh_mesh) with numpy views of its arrays."""
 
-    def __init__(self, degree, n, lo, hi, side_id):
+    def __init__(self, degree=None, n=None, lo=None, hi=None, side_id=None, handle=None):
         self.h = C.c_void_p()
-        rc = lib().glh_mesh_box(int(degree), int(n[0]), int(n[1]), int(n[2]), np.asarray(lo, np.float64),
-                                np.asarray(hi, np.float64), np.asarray(side_id, np.int32), C.byref(self.h))
-        if rc:
-            raise GLHError("glh_mesh_box failed (%d)" % rc)
+        if handle is not None:
+            self.h = handle
+        else:
+            rc = lib().glh_mesh_box(int(degree), int(n[0]), int(n[1]), int(n[2]), np.asarray(lo, np.float64),
+                                    np.asarray(hi, np.float64), np.asarray(side_id, np.int32), C.byref(self.h))
+            if rc:
+                raise GLHError("glh_mesh_box failed (%d)" % rc)
         s = np.zeros(8, np.int64)
         lib().glh_mesh_sizes(self.h, s)
         self.deg, self.n_cells, self.n_nodes, self.n_faces, self.nnzb, self.n_colors = [int(v) for v in s[:6]]
+        self.n_inst = int(s[6])
         self.ns = (self.deg + 1) ** 3
+
+    @staticmethod
+    def concat(meshes):
+        arr = (C.c_void_p * len(meshes))(*[m.h for m in meshes])
+        h = C.c_void_p()
+        rc = lib().glh_mesh_concat(arr, len(meshes), C.byref(h))
+        if rc:
+            raise GLHError("glh_mesh_concat failed (%d)" % rc)
+        return Mesh(handle=h)
+
+    def instances(self):
+        no = np.zeros(self.n_inst + 1, np.int32)
+        co = np.zeros(self.n_inst + 1, np.int32)
+        lib().glh_mesh_instances(self.h, no, co)
+        return no, co
 
     def arrays(self):
         a = dict(cell_nodes=np.zeros(self.n_cells * self.ns, np.int32), cell_h=np.zeros(self.n_cells * 3),
@@ -138,6 +163,7 @@ class Context:
             raise GLHError("glh_create failed (%d): is a GPU visible?" % rc)
         self.n_dofs = 0
         self.nnzb = 0
+        self.n_inst = 1
 
     def _chk(self, rc, what):
         if rc:
@@ -156,10 +182,24 @@ class Context:
         self.set_material(m[0], m[1:6], K, bt)
         return m
 
+    def set_materials(self, coefs):
+        """coefs: (n_inst, 10) rows (alpha, beta1..5, K1, K2, K3, bt)."""
+        c = np.ascontiguousarray(coefs, np.float64)
+        self._chk(lib().glh_set_materials(self.h, c.shape[0], c.reshape(-1)), "glh_set_materials")
+
+    def instance_energy(self):
+        return self._get("glh_instance_energy", self.n_inst)
+
+    def instance_stats(self):
+        out = np.zeros(self.n_inst * 8)
+        self._chk(lib().glh_instance_stats(self.h, out), "glh_instance_stats")
+        return out.reshape(-1, 8)
+
     def upload_mesh(self, mesh):
         self._chk(lib().glh_upload_mesh(self.h, mesh.h), "glh_upload_mesh")
         self.n_dofs = mesh.n_nodes * 18
         self.nnzb = mesh.nnzb
+        self.n_inst = mesh.n_inst
 
     def set_solution(self, U):
         self._chk(lib().glh_set_solution(self.h, np.ascontiguousarray(U, np.float64)), "glh_set_solution")
