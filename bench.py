@@ -114,12 +114,126 @@ def cpu_baseline(deg, h, mat, n_cells, n_nodes, nnzb, newton_steps, gmres_its, l
     }
 
 
+def slab_batch(G, n_inst=256, seed=2):
+    """The batched phase-diagram sweep: n_inst slab instances, T/Tc ~ U[0.3,0.95],
+    p ~ U[0,34) bar, thickness D ~ U[6,40] xi0, lateral extent 40 xi0, Q1 meshes
+    of nx = ny in 8..16 and nz in 2..8 cells, A = 0 on the two slab faces and
+    natural on the sides; B phase + 0.05 Delta_B noise (seed 1000 + i)."""
+    rng = np.random.default_rng(seed)
+    T = rng.uniform(0.3, 0.95, n_inst)
+    P = np.minimum(rng.uniform(0.0, 34.0, n_inst), np.nextafter(34.0, 0.0))
+    D = rng.uniform(6.0, 40.0, n_inst)
+    NX = rng.integers(8, 17, n_inst)
+    NZ = rng.integers(2, 9, n_inst)
+    meshes, coefs, U = [], [], []
+    K = 0.42072
+    for i in range(n_inst):
+        m = G.Mesh(1, (int(NX[i]), int(NX[i]), int(NZ[i])), (-20.0, -20.0, -D[i] / 2), (20.0, 20.0, D[i] / 2),
+                   [1, 1, 1, 1, 0, 0])
+        rc, mp = G.matep(P[i], T[i])
+        meshes.append(m)
+        coefs.append([mp[0], mp[1], mp[2], mp[3], mp[4], mp[5], K, K, K, 1.0e10])
+        U.append(G.initial_guess(m.n_nodes, mp[9], phase=0, amp=0.05, seed=1000 + i))
+    return meshes, np.array(coefs), U, dict(T=T, p=P, D=D, nx=NX, nz=NZ)
+
+
+def run_batch(G, args):
+    meshes, coefs, Us, params = slab_batch(G, args.instances)
+    batch = G.Mesh.concat(meshes)
+    n_dofs = batch.n_nodes * 18
+    print("bench: batch of %d instances, %d DoFs" % (len(meshes), n_dofs), file=sys.stderr, flush=True)
+    kw = dict(rtol=1e-10, max_newton=60, lin_tol=args.lin_tol, lin_maxit=args.lin_maxit, restart=args.restart)
+    ctx = G.Context(0)
+    ctx.set_verbose(1)
+    ctx.upload_mesh(batch)
+    ctx.set_materials(coefs)
+    ctx.set_solution(np.concatenate(Us))
+    ctx.store_initial()
+    out = {}
+    for prec in ("fp64", "fp32"):
+        for w in range(args.warmup):
+            ctx.restore_initial()
+            ctx.newton_solve(fp32=(prec == "fp32"), **kw)
+        ctx.restore_initial()
+        t0 = time.perf_counter()
+        st = ctx.newton_solve(fp32=(prec == "fp32"), **kw)
+        el = time.perf_counter() - t0
+        ist = ctx.instance_stats()
+        out[prec] = dict(elapsed=el, st=st, inst=ist, F=ctx.instance_energy())
+        print("bench: batch %s %.3f s, %d/%d converged, %s" % (prec, el, int(ist[:, 1].sum()), len(meshes), st),
+              file=sys.stderr, flush=True)
+    ctx.close()
+    # the plain FP64 drop-in: every instance solved on its own, one after another
+    single = G.Context(0)
+    t0 = time.perf_counter()
+    seq_conv = 0
+    nseq = min(len(meshes), args.sequential)
+    Fseq = []
+    for i in range(nseq):
+        if i % 32 == 0:
+            print("bench: sequential instance %d (%.1f s)" % (i, time.perf_counter() - t0), file=sys.stderr,
+                  flush=True)
+        single.upload_mesh(meshes[i])
+        single.set_material(coefs[i][0], coefs[i][1:6], coefs[i][6:9], coefs[i][9])
+        single.set_solution(Us[i])
+        try:
+            s1 = single.newton_solve(**kw)
+        except G.GLHError as e:
+            print("bench: sequential instance %d failed: %s" % (i, e), file=sys.stderr, flush=True)
+            Fseq.append(np.nan)
+            continue
+        seq_conv += int(s1["converged"])
+        Fseq.append(s1["F"])
+    seq_el = time.perf_counter() - t0
+    single.close()
+    F64, F32 = out["fp64"]["F"], out["fp32"]["F"]
+    conv64 = out["fp64"]["inst"][:, 1] == 1
+    both = conv64 & (out["fp32"]["inst"][:, 1] == 1)
+    # free energies are compared only where both solves reached the fixed point
+    ferr = (np.abs(F32 - F64) / np.maximum(np.abs(F64), 1e-300))[both]
+    ferr_seq = np.abs(np.array(Fseq) - F64[:nseq]) / np.maximum(np.abs(F64[:nseq]), 1e-300)
+    ferr_seq = ferr_seq[np.isfinite(ferr_seq) & conv64[:nseq]]
+    conv32 = out["fp32"]["inst"][:, 1]
+    t_conv = out["fp32"]["inst"][:, 4]
+    rate = lambda n, s: 3600.0 * n / s
+    plain_rate = rate(seq_conv, seq_el)
+    line = {
+        "metric": METRIC,
+        "value": rate(conv32.sum(), out["fp32"]["elapsed"]),
+        "unit": "solves/hr",
+        "n_gpus": 1, "steps": 1, "warmup": args.warmup,
+        "ms_per_step": 1000.0 * out["fp32"]["elapsed"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "batched phase-diagram sweep: %d Q1 slab instances (8..16 x 8..16 x 2..8 cells, "
+                               "%d DoFs total), T/Tc U[0.3,0.95], p U[0,34) bar, D U[6,40], seed 2; FP32 J + FP32 "
+                               "GMRES in FP64 iterative refinement, FP64 residual and line search, one batched "
+                               "launch per operation" % (len(meshes), n_dofs),
+                   "name": "slab_batch%d" % len(meshes), "dofs": n_dofs, "instances": len(meshes)},
+        "converged_instances": int(conv32.sum()),
+        "p50_instance_s": float(np.percentile(t_conv[t_conv >= 0], 50)) if (t_conv >= 0).any() else None,
+        "p99_instance_s": float(np.percentile(t_conv[t_conv >= 0], 99)) if (t_conv >= 0).any() else None,
+        "free_energy_max_rel_err_fp32_vs_fp64": float(ferr.max()) if ferr.size else None,
+        "fp64_batched_solves_hr": rate(out["fp64"]["inst"][:, 1].sum(), out["fp64"]["elapsed"]),
+        "plain_fp64_sequential_solves_hr": plain_rate,
+        "plain_fp64_sequential_sample": nseq,
+        "plain_vs_batched_free_energy_max_rel_err": float(ferr_seq.max()) if ferr_seq.size else None,
+        "speedup_vs_plain_fp64": rate(conv32.sum(), out["fp32"]["elapsed"]) / plain_rate if plain_rate else None,
+        "lockstep_newton_steps": out["fp32"]["st"]["newton_steps"],
+        "lockstep_gmres_its": out["fp32"]["st"]["gmres_its"],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cube_q2_ref5", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="cube_q2_ref5", choices=sorted(WORKLOADS) + ["slab_batch"])
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--lin-maxit", type=int, default=3000)
+    ap.add_argument("--sequential", type=int, default=256,
+                    help="instances solved one by one for the plain FP64 comparison (slab_batch)")
     ap.add_argument("--lin-tol", type=float, default=0.1)
     ap.add_argument("--restart", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -132,6 +246,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     G = load_glhip()
+    if args.workload == "slab_batch":
+        run_batch(G, args)
+        return
     deg, ncell, half, p, t = WORKLOADS[args.workload]
     print("bench: building %s" % args.workload, file=sys.stderr, flush=True)
     mesh = G.Mesh(deg, (ncell,) * 3, (-half,) * 3, (half,) * 3, [1, 1, 1, 1, 0, 0])

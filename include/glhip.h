@@ -149,7 +149,9 @@ int glh_instance_energy(glh_ctx *ctx, double *F);
 /* ------------------------------------------------------------ FemGL driver */
 /* The Newton loop of FemGL::run (run.cc:208-256) on the uploaded mesh:
  * assemble -> solve -> newton_iteration, stopping at ||R|| <= rtol * ||R0||
- * (relative criterion) or after max_newton steps.  fp32 != 0 assembles the
+ * (relative criterion) or after max_newton steps.  A linear solve that reaches
+ * lin_maxit is not fatal: its partial update goes to the line search.
+ * fp32 != 0 assembles the
  * Jacobian in FP32 and runs FP32 GMRES inside FP64 iterative refinement; the
  * residual, the update and the line search stay FP64.
  * stats: [0] Newton steps, [1] total GMRES iterations, [2] ||R0||,
@@ -170,6 +172,8 @@ int glh_instance_stats(glh_ctx *ctx, double *out);
 /* timing of the last glh_assemble Jacobian launches on the context stream
  * (HIP events): ms[0] = J+R assembly ms, ms[1] = number of launches. */
 int glh_last_assembly_ms(glh_ctx *ctx, double ms[2]);
+/* level > 0: one progress line per Newton step on stderr (glh_newton_solve). */
+int glh_set_verbose(glh_ctx *ctx, int level);
 /* Live kernel timing with HIP events on the context stream.  on != 0 resets
  * the counters and records an event pair around every SpMV launch and every
  * assembly sweep from then on.  out: [0] total SpMV ms, [1] SpMV launches,

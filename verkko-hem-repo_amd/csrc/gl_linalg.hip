@@ -33,11 +33,13 @@ __device__ inline void load6(const T *__restrict__ p, double *o)
 template <typename TJ, typename TX>
 __global__ __launch_bounds__(256) void k_spmv(int n_rows, const int *__restrict__ rp, const int *__restrict__ col,
                                               const TJ *__restrict__ val, const TX *__restrict__ x,
-                                              TX *__restrict__ y)
+                                              TX *__restrict__ y, const int *__restrict__ row_seg,
+                                              const int *__restrict__ seg_flag)
 {
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= n_rows) return;
+  if (seg_flag && !seg_flag[row_seg[row]]) return;  // instance not iterating
   const int g = lane / 18, c = lane % 18;
   const bool active = lane < 54;
   double acc = 0.0;
@@ -150,11 +152,13 @@ __global__ __launch_bounds__(64) void k_bj_invert(int n_rows, const int *__restr
 
 template <typename TJ, typename TX>
 __global__ __launch_bounds__(256) void k_bj_apply(int n_rows, const TJ *__restrict__ Dinv, const TX *__restrict__ x,
-                                                  TX *__restrict__ y)
+                                                  TX *__restrict__ y, const int *__restrict__ row_seg,
+                                                  const int *__restrict__ seg_flag)
 {
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= n_rows) return;
+  if (seg_flag && !seg_flag[row_seg[row]]) return;
   const int g = lane / 18, c = lane % 18;
   double acc = 0.0;
   if (lane < 54) {
@@ -350,9 +354,10 @@ __global__ void k_set_scalar(T *p, T v)
 template <typename T>
 __global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__restrict__ ch, int k, int i0,
                                                          const T *__restrict__ V, size_t ldv, const T *__restrict__ w,
-                                                         T *__restrict__ partial)
+                                                         T *__restrict__ partial, const int *__restrict__ flag)
 {
   __shared__ T sh[RT];
+  if (flag && !flag[ch[3 * blockIdx.x]]) return;  // skipped instance: its partials are never read
   T acc[KCH];
 #pragma unroll
   for (int j = 0; j < KCH; ++j) acc[j] = 0;
@@ -374,10 +379,11 @@ __global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__rest
 template <typename T>
 __global__ __launch_bounds__(RT) void k_seg_final(const int *__restrict__ seg_chunk, int k,
                                                   const T *__restrict__ partial, T *__restrict__ out, int ldo, int off,
-                                                  int accumulate)
+                                                  int accumulate, const int *__restrict__ flag)
 {
   __shared__ T sh[RT];
   const int sg = blockIdx.x, i = blockIdx.y;
+  if (flag && !flag[sg]) return;
   T acc = 0;
   for (int b = seg_chunk[sg] + threadIdx.x; b < seg_chunk[sg + 1]; b += RT) acc += partial[(size_t)b * k + i];
   const T r = block_reduce(acc, sh);
@@ -391,10 +397,11 @@ template <typename T>
 __global__ __launch_bounds__(RT) void k_seg_sub_lincomb_norm(const long long *__restrict__ ch, int k,
                                                              const T *__restrict__ V, size_t ldv,
                                                              const T *__restrict__ h, int ldh, T *__restrict__ w,
-                                                             T *__restrict__ partial)
+                                                             T *__restrict__ partial, const int *__restrict__ flag)
 {
   __shared__ T sh[RT];
   const long long sg = ch[3 * blockIdx.x], beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
+  if (flag && !flag[sg]) return;
   const T *hs = h + sg * ldh;
   T acc = 0;
   for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
@@ -409,24 +416,30 @@ __global__ __launch_bounds__(RT) void k_seg_sub_lincomb_norm(const long long *__
 
 template <typename T>
 __global__ __launch_bounds__(RT) void k_seg_lincomb(const long long *__restrict__ ch, int k, const T *__restrict__ V,
-                                                    size_t ldv, const T *__restrict__ y, int ldy, T *__restrict__ w)
+                                                    size_t ldv, const T *__restrict__ y, int ldy, T *__restrict__ w,
+                                                    const int *__restrict__ kseg)
 {
   const long long sg = ch[3 * blockIdx.x], beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
   const T *ys = y + sg * ldy;
+  const int kk = kseg ? (kseg[sg] < k ? kseg[sg] : k) : k;  // basis vectors this instance built
   for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
     T s = 0;
-    for (int i = 0; i < k; ++i) s += ys[i] * V[(size_t)i * ldv + idx];
+    for (int i = 0; i < kk; ++i) s += ys[i] * V[(size_t)i * ldv + idx];
     w[idx] = s;
   }
 }
 
-// dst = src / scal[s*lds] (unchanged when the scale is 0)
+// dst = src / scal[s*lds] (unchanged when the scale is 0); with sq != 0 the
+// scale is sqrt(scal)
 template <typename T>
 __global__ __launch_bounds__(RT) void k_seg_scale(const long long *__restrict__ ch, const T *__restrict__ src,
-                                                  const T *__restrict__ scal, int lds, T *__restrict__ dst)
+                                                  const T *__restrict__ scal, int lds, T *__restrict__ dst,
+                                                  const int *__restrict__ flag, int sq)
 {
   const long long sg = ch[3 * blockIdx.x], beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
-  const T a = scal[sg * lds];
+  if (flag && !flag[sg]) return;
+  T a = scal[sg * lds];
+  if (sq) a = a > (T)0 ? sqrt(a) : (T)0;
   const T inv = a != (T)0 ? (T)1 / a : (T)1;
   for (long long idx = beg + threadIdx.x; idx < end; idx += RT) dst[idx] = src[idx] * inv;
 }
@@ -454,6 +467,95 @@ __global__ __launch_bounds__(RT) void k_seg_zero(const long long *__restrict__ c
   for (long long idx = beg + threadIdx.x; idx < end; idx += RT) v[idx] = (T)0;
 }
 
+
+// ----------------------------------------- device-side GMRES control (batched)
+// One thread per instance.  op 0: start of a restart cycle (beta, g, cycle
+// flags); op 1: DGKS flag for column j; op 2: Givens update of column j;
+// op 3: back-substitution for y.
+template <typename T>
+__global__ void k_gmres_ctl(GmresCtl<T> C, int op, int j)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= C.S) return;
+  const int m = C.m, ldh = C.ldh;
+  double *g = C.g + (size_t)s * (m + 1);
+  double *H = C.H + (size_t)s * (m + 1) * m;
+  double *cs = C.cs + (size_t)s * m, *sn = C.sn + (size_t)s * m;
+  if (op == 0) {
+    const double b = C.nrm2[s] > (T)0 ? sqrt((double)C.nrm2[s]) : 0.0;
+    C.beta[s] = b;
+    const int it = !C.frozen[s] && b > C.target[s];
+    C.cyc[s] = it;
+    C.conv[s] = !it;
+    for (int r = 0; r <= m; ++r) g[r] = 0.0;
+    g[0] = b;
+    C.jlast[s] = C.jseg[s];
+    C.jseg[s] = 0;
+    return;
+  }
+  if (op == 1) {
+    C.ro[s] = C.cyc[s] && (double)C.nrm2[s] < 0.5 * (double)C.hs[(size_t)s * ldh + j + 1];
+    return;
+  }
+  if (op == 2) {
+    if (!C.cyc[s]) return;
+    const double hn = C.nrm2[s] > (T)0 ? sqrt((double)C.nrm2[s]) : 0.0;
+    for (int r = 0; r <= j; ++r)
+      H[(size_t)r * m + j] = (double)C.hs[(size_t)s * ldh + r] + (C.ro[s] ? (double)C.hq[(size_t)s * ldh + r] : 0.0);
+    for (int r = 0; r < j; ++r) {
+      const double a = H[(size_t)r * m + j], b2 = H[(size_t)(r + 1) * m + j];
+      H[(size_t)r * m + j] = cs[r] * a + sn[r] * b2;
+      H[(size_t)(r + 1) * m + j] = -sn[r] * a + cs[r] * b2;
+    }
+    const double a = H[(size_t)j * m + j], rr = hypot(a, hn);
+    cs[j] = rr > 0 ? a / rr : 1.0;
+    sn[j] = rr > 0 ? hn / rr : 0.0;
+    H[(size_t)j * m + j] = rr;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+    C.jseg[s] = j + 1;
+    C.its[s] += 1;
+    if (fabs(g[j + 1]) <= C.target[s] || hn == 0.0) C.cyc[s] = 0;
+    return;
+  }
+  // op 3: y = R^-1 g on the first jseg columns, zero beyond
+  const int ji = C.jseg[s];
+  T *y = C.y + (size_t)s * ldh;
+  for (int r = ji; r < ldh; ++r) y[r] = (T)0;
+  double yd[128];
+  for (int r = ji - 1; r >= 0; --r) {
+    double sum = g[r];
+    for (int k = r + 1; k < ji; ++k) sum -= H[(size_t)r * m + k] * yd[k];
+    yd[r] = sum / H[(size_t)r * m + r];
+    y[r] = (T)yd[r];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(RT) void k_fill(size_t n, T v, T *__restrict__ d)
+{
+  const size_t stride = (size_t)gridDim.x * RT;
+  for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n; idx += stride) d[idx] = v;
+}
+
+// r = b - w per chunk with partial ||r||^2
+template <typename T>
+__global__ __launch_bounds__(RT) void k_seg_residual(const long long *__restrict__ ch, const T *__restrict__ b,
+                                                     const T *__restrict__ w, T *__restrict__ r,
+                                                     T *__restrict__ partial)
+{
+  __shared__ T sh[RT];
+  const long long beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
+  T acc = 0;
+  for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
+    const T v = b[idx] - w[idx];
+    r[idx] = v;
+    acc += v * v;
+  }
+  const T res = block_reduce(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = res;
+}
+
 inline int grid_for(size_t n)
 {
   size_t g = (n + RT - 1) / RT;
@@ -466,10 +568,11 @@ inline int grid_for(size_t n)
 
 template <typename TJ, typename TX>
 hipError_t launch_spmv(int n_rows, const int *row_ptr, const int *col, const TJ *val, const TX *x, TX *y,
-                       hipStream_t s)
+                       hipStream_t s, const int *row_seg, const int *seg_flag)
 {
   if (n_rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_spmv<TJ, TX>), dim3((n_rows + 3) / 4), dim3(256), 0, s, n_rows, row_ptr, col, val, x, y);
+  hipLaunchKernelGGL((k_spmv<TJ, TX>), dim3((n_rows + 3) / 4), dim3(256), 0, s, n_rows, row_ptr, col, val, x, y,
+                     row_seg, seg_flag);
   return hipGetLastError();
 }
 
@@ -491,10 +594,12 @@ hipError_t launch_bj_invert(int n_rows, const int *diag_blk, const TJ *val, TJ *
 }
 
 template <typename TJ, typename TX>
-hipError_t launch_bj_apply(int n_rows, const TJ *Dinv, const TX *x, TX *y, hipStream_t s)
+hipError_t launch_bj_apply(int n_rows, const TJ *Dinv, const TX *x, TX *y, hipStream_t s, const int *row_seg,
+                           const int *seg_flag)
 {
   if (n_rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_bj_apply<TJ, TX>), dim3((n_rows + 3) / 4), dim3(256), 0, s, n_rows, Dinv, x, y);
+  hipLaunchKernelGGL((k_bj_apply<TJ, TX>), dim3((n_rows + 3) / 4), dim3(256), 0, s, n_rows, Dinv, x, y, row_seg,
+                     seg_flag);
   return hipGetLastError();
 }
 
@@ -599,42 +704,45 @@ hipError_t launch_copy(size_t nbytes, const void *src, void *dst, hipStream_t s)
 
 template <typename T>
 hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, const T *w, T *partial, T *out, int ldo,
-                           int off, bool accumulate, hipStream_t s)
+                           int off, bool accumulate, hipStream_t s, const int *flag)
 {
   if (k <= 0 || S.n_chunks <= 0) return hipSuccess;
   for (int i0 = 0; i0 < k; i0 += KCH)
     hipLaunchKernelGGL((k_seg_mdot_partial<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, k, i0, V, ldv, w,
-                       partial);
+                       partial, flag);
   hipLaunchKernelGGL((k_seg_final<T>), dim3(S.n_seg, k), dim3(RT), 0, s, S.seg_chunk, k, partial, out, ldo, off,
-                     accumulate ? 1 : 0);
+                     accumulate ? 1 : 0, flag);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size_t ldv, const T *h, int ldh, T *w,
-                                       T *partial, T *norm2, int ldo, int off, hipStream_t s)
+                                       T *partial, T *norm2, int ldo, int off, hipStream_t s, const int *flag)
 {
   if (S.n_chunks <= 0) return hipSuccess;
   hipLaunchKernelGGL((k_seg_sub_lincomb_norm<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, k, V, ldv, h, ldh, w,
-                     partial);
-  hipLaunchKernelGGL((k_seg_final<T>), dim3(S.n_seg, 1), dim3(RT), 0, s, S.seg_chunk, 1, partial, norm2, ldo, off, 0);
+                     partial, flag);
+  hipLaunchKernelGGL((k_seg_final<T>), dim3(S.n_seg, 1), dim3(RT), 0, s, S.seg_chunk, 1, partial, norm2, ldo, off, 0,
+                     flag);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_seg_lincomb(const SegInfo &S, int k, const T *V, size_t ldv, const T *y, int ldy, T *w,
-                              hipStream_t s)
+                              hipStream_t s, const int *kseg)
 {
   if (S.n_chunks <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_seg_lincomb<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, k, V, ldv, y, ldy, w);
+  hipLaunchKernelGGL((k_seg_lincomb<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, k, V, ldv, y, ldy, w, kseg);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_seg_scale(const SegInfo &S, const T *src, const T *scal, int lds, T *dst, hipStream_t s)
+hipError_t launch_seg_scale(const SegInfo &S, const T *src, const T *scal, int lds, T *dst, hipStream_t s,
+                            const int *flag, bool sqrt_of_scale)
 {
   if (S.n_chunks <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_seg_scale<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, src, scal, lds, dst);
+  hipLaunchKernelGGL((k_seg_scale<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, src, scal, lds, dst, flag,
+                     sqrt_of_scale ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -654,35 +762,65 @@ hipError_t launch_seg_zero(const SegInfo &S, const int *flag, T *v, hipStream_t 
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_gmres_ctl(const GmresCtl<T> &C, int op, int j, hipStream_t s)
+{
+  hipLaunchKernelGGL((k_gmres_ctl<T>), dim3((C.S + 63) / 64), dim3(64), 0, s, C, op, j);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fill(size_t n, T v, T *d, hipStream_t s)
+{
+  hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(n)), dim3(RT), 0, s, n, v, d);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_seg_residual(const SegInfo &S, const T *b, const T *w, T *r, T *partial, T *norm2, hipStream_t s)
+{
+  hipLaunchKernelGGL((k_seg_residual<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, b, w, r, partial);
+  hipLaunchKernelGGL((k_seg_final<T>), dim3(S.n_seg, 1), dim3(RT), 0, s, S.seg_chunk, 1, partial, norm2, 1, 0, 0,
+                     (const int *)nullptr);
+  return hipGetLastError();
+}
+
 #define GLH_SEG_INST(T)                                                                                         \
   template hipError_t launch_seg_mdot<T>(const SegInfo &, int, const T *, size_t, const T *, T *, T *, int, int,   \
-                                         bool, hipStream_t);                                                       \
+                                         bool, hipStream_t, const int *);                                          \
   template hipError_t launch_seg_sub_lincomb_norm<T>(const SegInfo &, int, const T *, size_t, const T *, int, T *, \
-                                                     T *, T *, int, int, hipStream_t);                             \
+                                                     T *, T *, int, int, hipStream_t, const int *);                \
   template hipError_t launch_seg_lincomb<T>(const SegInfo &, int, const T *, size_t, const T *, int, T *,          \
-                                            hipStream_t);                                                          \
-  template hipError_t launch_seg_scale<T>(const SegInfo &, const T *, const T *, int, T *, hipStream_t);           \
-  template hipError_t launch_seg_zero<T>(const SegInfo &, const int *, T *, hipStream_t);
+                                            hipStream_t, const int *);                                             \
+  template hipError_t launch_seg_scale<T>(const SegInfo &, const T *, const T *, int, T *, hipStream_t,            \
+                                          const int *, bool);                                                      \
+  template hipError_t launch_seg_zero<T>(const SegInfo &, const int *, T *, hipStream_t);                          \
+  template hipError_t launch_gmres_ctl<T>(const GmresCtl<T> &, int, int, hipStream_t);                             \
+  template hipError_t launch_fill<T>(size_t, T, T *, hipStream_t);                                                 \
+  template hipError_t launch_seg_residual<T>(const SegInfo &, const T *, const T *, T *, T *, T *, hipStream_t);
 GLH_SEG_INST(double)
 GLH_SEG_INST(float)
 #undef GLH_SEG_INST
+template hipError_t launch_fill<int>(size_t, int, int *, hipStream_t);
 
 // explicit instantiations
 template hipError_t launch_set_scalar<double>(double *, double, hipStream_t);
 template hipError_t launch_set_scalar<float>(float *, float, hipStream_t);
 template hipError_t launch_axpy<float, float>(size_t, double, const float *, float *, hipStream_t);
 template hipError_t launch_spmv<double, double>(int, const int *, const int *, const double *, const double *,
-                                                double *, hipStream_t);
+                                                double *, hipStream_t, const int *, const int *);
 template hipError_t launch_spmv<float, float>(int, const int *, const int *, const float *, const float *, float *,
-                                              hipStream_t);
+                                              hipStream_t, const int *, const int *);
 template hipError_t launch_residual_spmv<double>(int, const int *, const int *, const double *, const double *,
                                                  const double *, double *, hipStream_t);
 template hipError_t launch_residual_spmv<float>(int, const int *, const int *, const float *, const double *,
                                                 const double *, double *, hipStream_t);
 template hipError_t launch_bj_invert<double>(int, const int *, const double *, double *, int *, hipStream_t);
 template hipError_t launch_bj_invert<float>(int, const int *, const float *, float *, int *, hipStream_t);
-template hipError_t launch_bj_apply<double, double>(int, const double *, const double *, double *, hipStream_t);
-template hipError_t launch_bj_apply<float, float>(int, const float *, const float *, float *, hipStream_t);
+template hipError_t launch_bj_apply<double, double>(int, const double *, const double *, double *, hipStream_t,
+                                                    const int *, const int *);
+template hipError_t launch_bj_apply<float, float>(int, const float *, const float *, float *, hipStream_t,
+                                                  const int *, const int *);
 template hipError_t launch_mdot<double>(size_t, int, const double *, size_t, const double *, double *, double *, bool,
                                         hipStream_t);
 template hipError_t launch_mdot<float>(size_t, int, const float *, size_t, const float *, float *, float *, bool,

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -114,6 +115,16 @@ struct glh_ctx {
   DBuf<double> gV64, gZ64, gW64, gh64, gp64, gy64, gs64, gq64;
   DBuf<float> gV32, gZ32, gW32, gh32, gp32, gy32, gs32, gq32, gb32, gx32;
   HBuf<double> pin64, pinv;
+  // device-side GMRES control (per instance)
+  DBuf<double> gmH, gmcs, gmsn, gmg, gmbeta, gmtarget;
+  DBuf<int> gmcyc, gmconv, gmjseg, gmits, gmro, gmfrozen, gmjlast, row_seg;
+  // one restart cycle as graphs: head (true residual, start of cycle), body
+  // blocks of Arnoldi steps, tail (update of x)
+  struct Graph {
+    std::vector<hipGraphExec_t> exec;  // head, tail, blocks...
+    std::vector<const void *> key;
+    int q = 1;
+  } graph64, graph32;
   HBuf<float> pin32;
   DBuf<double> ir_r;
   // per-instance results of the last Newton solve: 8 values per instance
@@ -124,6 +135,7 @@ struct glh_ctx {
   // live kernel timing (glh_set_timing): event pairs around each SpMV launch
   // on the context stream, read back when the pool fills or a solve ends
   bool timing = false;
+  int verbose = 0;
   long reorth_count = 0;
   std::vector<hipEvent_t> ev_pool;
   int ev_used = 0;
@@ -378,6 +390,7 @@ struct Work<double> {
   static DBuf<double> &y(glh_ctx *c) { return c->gy64; }
   static DBuf<double> &sc(glh_ctx *c) { return c->gs64; }
   static HBuf<double> &pin(glh_ctx *c) { return c->pin64; }
+  static glh_ctx::Graph &graph(glh_ctx *c) { return c->graph64; }
 };
 template <>
 struct Work<float> {
@@ -390,6 +403,7 @@ struct Work<float> {
   static DBuf<float> &y(glh_ctx *c) { return c->gy32; }
   static DBuf<float> &sc(glh_ctx *c) { return c->gs32; }
   static HBuf<float> &pin(glh_ctx *c) { return c->pin32; }
+  static glh_ctx::Graph &graph(glh_ctx *c) { return c->graph32; }
 };
 
 int flush_events(glh_ctx *c)
@@ -407,14 +421,14 @@ int flush_events(glh_ctx *c)
 }
 
 template <typename TJ, typename TV>
-int spmv(glh_ctx *c, const TJ *J, const TV *x, TV *y)
+int spmv(glh_ctx *c, const TJ *J, const TV *x, TV *y, const int *flag = nullptr)
 {
   const bool t = c->timing && !c->ev_pool.empty();
   if (t) {
     if (2 * c->ev_used >= (int)c->ev_pool.size()) RCHK(flush_events(c));
     HIPCHK(c, hipEventRecord(c->ev_pool[2 * c->ev_used], c->stream));
   }
-  HIPCHK(c, glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, x, y, c->stream));
+  HIPCHK(c, glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, x, y, c->stream, c->row_seg.p, flag));
   if (t) {
     HIPCHK(c, hipEventRecord(c->ev_pool[2 * c->ev_used + 1], c->stream));
     ++c->ev_used;
@@ -423,15 +437,70 @@ int spmv(glh_ctx *c, const TJ *J, const TV *x, TV *y)
 }
 
 // Right-preconditioned restarted GMRES(m) with 18x18 point-block Jacobi and
-// classical Gram-Schmidt with selective reorthogonalisation, batched over the
-// instances of the context: every instance has its own Krylov coefficients,
-// Givens rotations and stopping test ||b_s - J x_s|| <= rtol[s] ||b_s||; all
-// instances advance in lockstep through one launch per vector operation.
-// its_out: lockstep iterations; its_seg: iterations each instance used.
+// classical Gram-Schmidt with selective (DGKS) reorthogonalisation, batched
+// over the instances of the context: every instance has its own Krylov
+// coefficients, Givens rotations and stopping test
+// ||b_s - J x_s|| <= rtol[s] ||b_s||.  The Hessenberg updates, rotations and
+// tests run on the device (k_gmres_ctl), and every kernel skips the instances
+// that have stopped, so a restart cycle needs no host round trip: it is
+// enqueued once, captured in a HIP graph and replayed (or launched directly
+// when kernel timing is on).  its_out: lockstep iterations; its_seg:
+// iterations each instance used.
+template <typename TJ, typename TV>
+int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, int m, const glh::GmresCtl<TV> &C,
+                  bool timed, int part, int j0 = 0, int j1 = 0)
+{
+  const size_t n = ndofs(c);
+  const int ldh = m + 2;
+  const glh::SegInfo SI = c->seg();
+  hipStream_t s = c->stream;
+  auto &V = Work<TV>::V(c);
+  auto &Z = Work<TV>::Z(c);
+  auto &W = Work<TV>::W(c);
+  auto &pt = Work<TV>::part(c);
+  TV *nrm2 = const_cast<TV *>(C.nrm2);
+  TV *hs = const_cast<TV *>(C.hs), *hq = const_cast<TV *>(C.hq);
+  const int *rs = c->row_seg.p, *cyc = C.cyc, *ro = C.ro;
+  auto mv = [&](const TV *in, TV *out, const int *flag) -> int {
+    if (timed) return spmv<TJ, TV>(c, J, in, out, flag);
+    HIPCHK(c, (glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, in, out, s, rs, flag)));
+    return GLH_OK;
+  };
+  if (part == 0) {  // head: V0 = b - J x, beta_s, cycle flags
+    RCHK(mv(x, W.p, nullptr));
+    HIPCHK(c, glh::launch_seg_residual<TV>(SI, b, W.p, V.p, pt.p, nrm2, s));
+    HIPCHK(c, glh::launch_gmres_ctl<TV>(C, 0, 0, s));
+    return GLH_OK;
+  }
+  if (part == 2) {  // tail: x += M^-1 V y
+    HIPCHK(c, glh::launch_gmres_ctl<TV>(C, 3, 0, s));
+    HIPCHK(c, glh::launch_seg_lincomb<TV>(SI, m, V.p, n, C.y, ldh, W.p, s, C.jseg));
+    HIPCHK(c, (glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, W.p, Z.p, s)));
+    HIPCHK(c, (glh::launch_axpy<TV, TV>(n, 1.0, Z.p, x, s)));
+    return GLH_OK;
+  }
+  // body block: Arnoldi steps j0 <= j < j1
+  if (j0 == 0) HIPCHK(c, glh::launch_seg_scale<TV>(SI, V.p, nrm2, 1, V.p, s, cyc, true));
+  for (int j = j0; j < j1; ++j) {
+    TV *vj = V.p + n * j, *wv = V.p + n * (j + 1);
+    HIPCHK(c, (glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, vj, Z.p, s, rs, cyc)));
+    RCHK(mv(Z.p, wv, cyc));
+    HIPCHK(c, glh::launch_seg_mdot<TV>(SI, j + 2, V.p, n, wv, pt.p, hs, ldh, 0, false, s, cyc));
+    HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SI, j + 1, V.p, n, hs, ldh, wv, pt.p, nrm2, 1, 0, s, cyc));
+    HIPCHK(c, glh::launch_gmres_ctl<TV>(C, 1, j, s));
+    HIPCHK(c, glh::launch_seg_mdot<TV>(SI, j + 1, V.p, n, wv, pt.p, hq, ldh, 0, false, s, ro));
+    HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SI, j + 1, V.p, n, hq, ldh, wv, pt.p, nrm2, 1, 0, s, ro));
+    HIPCHK(c, glh::launch_seg_scale<TV>(SI, wv, nrm2, 1, wv, s, cyc, true));
+    HIPCHK(c, glh::launch_gmres_ctl<TV>(C, 2, j, s));
+  }
+  return GLH_OK;
+}
+
 template <typename TJ, typename TV>
 int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std::vector<double> &rtol, int maxit,
           int m, int *its_out, std::vector<double> &rel_out, std::vector<int> &its_seg)
 {
+  if (m > 126) m = 126;
   const size_t n = ndofs(c);
   const int S = c->n_seg, ldh = m + 2;
   const glh::SegInfo SI = c->seg();
@@ -444,7 +513,6 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
   auto &pt = Work<TV>::part(c);
   auto &yb = Work<TV>::y(c);
   auto &sc = Work<TV>::sc(c);
-  auto &pin = Work<TV>::pin(c);
   HIPCHK(c, V.alloc(n * (m + 1)));
   HIPCHK(c, Z.alloc(n));
   HIPCHK(c, W.alloc(n));
@@ -453,128 +521,122 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
   HIPCHK(c, yb.alloc((size_t)S * ldh));
   HIPCHK(c, sc.alloc((size_t)S * 2));
   HIPCHK(c, pt.alloc((size_t)std::max(1, c->n_chunks) * (m + 2)));
-  HIPCHK(c, pin.alloc((size_t)S * ldh * 2 + 2 * S));
-  TV *ph = pin.p, *pq = pin.p + (size_t)S * ldh, *pn = pin.p + (size_t)2 * S * ldh;
-  TV *nrm = sc.p;  // [S] per-instance squared norms
-  std::vector<double> H((size_t)S * (m + 1) * m, 0.0), cs((size_t)S * m), sn((size_t)S * m), g((size_t)S * (m + 1));
-  std::vector<double> bn(S), target(S), beta(S, 0.0), hn(S), y(m);
-  std::vector<int> jseg(S, 0);
-  std::vector<char> cyc(S, 0);
-  its_seg.assign(S, 0);
-  HIPCHK(c, hipMemsetAsync(x, 0, n * sizeof(TV), s));
-  // ||b_s||
-  HIPCHK(c, glh::launch_seg_mdot<TV>(SI, 1, b, n, b, pt.p, nrm, 1, 0, false, s));
-  HIPCHK(c, hipMemcpyAsync(pn, nrm, S * sizeof(TV), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, c->gmH.alloc((size_t)S * (m + 1) * m));
+  HIPCHK(c, c->gmcs.alloc((size_t)S * m));
+  HIPCHK(c, c->gmsn.alloc((size_t)S * m));
+  HIPCHK(c, c->gmg.alloc((size_t)S * (m + 1)));
+  HIPCHK(c, c->gmbeta.alloc(S));
+  HIPCHK(c, c->gmtarget.alloc(S));
+  DBuf<int> *ib[] = {&c->gmcyc, &c->gmconv, &c->gmjseg, &c->gmits, &c->gmro, &c->gmfrozen, &c->gmjlast};
+  for (auto *q : ib) HIPCHK(c, q->alloc(S));
+  HIPCHK(c, c->pinv.alloc((size_t)4 * S));
+  glh::GmresCtl<TV> C{S,        m,          ldh,        hs.p,         hq.p,         sc.p,        yb.p,
+                      c->gmH.p, c->gmcs.p,  c->gmsn.p,  c->gmg.p,     c->gmbeta.p,  c->gmtarget.p, c->gmcyc.p,
+                      c->gmconv.p, c->gmjseg.p, c->gmits.p, c->gmro.p, c->gmfrozen.p, c->gmjlast.p};
+  // ||b_s|| and the absolute targets
+  HIPCHK(c, glh::launch_seg_mdot<TV>(SI, 1, b, n, b, pt.p, sc.p, 1, 0, false, s));
+  std::vector<TV> bn2(S);
+  HIPCHK(c, hipMemcpyAsync(bn2.data(), sc.p, S * sizeof(TV), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
+  std::vector<double> bn(S);
   for (int i = 0; i < S; ++i) {
-    bn[i] = std::sqrt(std::max(0.0, (double)pn[i]));
-    target[i] = rtol[i] * bn[i];
+    bn[i] = std::sqrt(std::max(0.0, (double)bn2[i]));
+    c->pinv.p[i] = rtol[i] * bn[i];
   }
+  HIPCHK(c, hipMemcpyAsync(c->gmtarget.p, c->pinv.p, S * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c, glh::launch_fill<int>(S, 0, c->gmits.p, s));
+  HIPCHK(c, glh::launch_fill<int>(S, 0, c->gmjseg.p, s));
+  HIPCHK(c, glh::launch_fill<int>(S, 0, c->gmfrozen.p, s));
+  HIPCHK(c, glh::launch_fill<TV>(n, (TV)0, x, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  // graphs of one restart cycle, rebuilt whenever their buffers change.  The
+  // Arnoldi steps go in blocks of q; the host checks between blocks whether
+  // any instance still iterates, so at most q - 1 steps of a cycle run as
+  // skipped launches (none for a single instance, q = 1).
+  const bool timed = c->timing && !c->ev_pool.empty();
+  const int q = S == 1 ? 1 : std::min(m, 8);
+  const int nblk = (m + q - 1) / q;
+  auto &gc = Work<TV>::graph(c);
+  const std::vector<const void *> key = {J, Dinv, b, x, V.p, Z.p, W.p, pt.p, hs.p, c->gmH.p, c->gmbeta.p,
+                                         (const void *)(size_t)m, (const void *)(size_t)S,
+                                         (const void *)(size_t)c->n_chunks};
+  if (!timed && (gc.exec.empty() || gc.key != key)) {
+    for (auto e : gc.exec) (void)hipGraphExecDestroy(e);
+    gc.exec.clear();
+    gc.key.clear();
+    for (int g = 0; g < 2 + nblk; ++g) {
+      const int part = g == 0 ? 0 : g == 1 ? 2 : 1;
+      const int j0 = g >= 2 ? (g - 2) * q : 0, j1 = std::min(m, j0 + q);
+      hipGraph_t graph = nullptr;
+      HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_cycle<TJ, TV>(c, J, Dinv, b, x, m, C, false, part, j0, j1);
+      hipError_t e = hipStreamEndCapture(s, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      HIPCHK(c, e);
+      hipGraphExec_t ge = nullptr;
+      e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHK(c, e);
+      gc.exec.push_back(ge);
+    }
+    gc.key = key;
+    gc.q = q;
+  }
+  auto launch = [&](int part, int blk) -> int {
+    if (timed) return enqueue_cycle<TJ, TV>(c, J, Dinv, b, x, m, C, true, part, blk * q, std::min(m, blk * q + q));
+    HIPCHK(c, hipGraphLaunch(gc.exec[part == 0 ? 0 : part == 2 ? 1 : 2 + blk], s));
+    return GLH_OK;
+  };
+  // cycle loop
   int it = 0;
   bool conv = false;
+  std::vector<int> hconv(S), hj(S);
   while (true) {
-    // V0 = b - J x, with per-instance ||V0||^2
-    HIPCHK(c, hipMemcpyAsync(V.p, b, n * sizeof(TV), hipMemcpyDeviceToDevice, s));
-    if (it == 0) {
-      HIPCHK(c, glh::launch_seg_mdot<TV>(SI, 1, V.p, n, V.p, pt.p, nrm, 1, 0, false, s));
-    } else {
-      RCHK((spmv<TJ, TV>(c, J, x, W.p)));
-      for (int i = 0; i < S; ++i) ph[(size_t)i * ldh] = (TV)1;
-      HIPCHK(c, hipMemcpyAsync(hq.p, ph, (size_t)S * ldh * sizeof(TV), hipMemcpyHostToDevice, s));
-      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SI, 1, W.p, n, hq.p, ldh, V.p, pt.p, nrm, 1, 0, s));
-    }
-    HIPCHK(c, hipMemcpyAsync(pn, nrm, S * sizeof(TV), hipMemcpyDeviceToHost, s));
+    RCHK(launch(0, 0));
+    HIPCHK(c, hipMemcpyAsync(hconv.data(), c->gmconv.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(hj.data(), c->gmjlast.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
-    bool any = false;
+    bool all = true;
+    int jmax = 0;
     for (int i = 0; i < S; ++i) {
-      beta[i] = std::sqrt(std::max(0.0, (double)pn[i]));
-      cyc[i] = beta[i] > target[i];
-      any = any || cyc[i];
+      all = all && hconv[i];
+      jmax = std::max(jmax, hj[i]);
     }
-    if (!any) { conv = true; break; }
+    it += jmax;
+    if (all) {
+      conv = true;
+      break;
+    }
     if (it >= maxit) break;
-    for (int i = 0; i < S; ++i) pn[i] = (TV)beta[i];
-    HIPCHK(c, hipMemcpyAsync(sc.p + S, pn, S * sizeof(TV), hipMemcpyHostToDevice, s));
-    HIPCHK(c, glh::launch_seg_scale<TV>(SI, V.p, sc.p + S, 1, V.p, s));
-    for (int i = 0; i < S; ++i) {
-      std::fill(g.begin() + (size_t)i * (m + 1), g.begin() + (size_t)(i + 1) * (m + 1), 0.0);
-      g[(size_t)i * (m + 1)] = beta[i];
-      jseg[i] = 0;
-    }
-    int j = 0;
-    while (j < m && it < maxit && std::any_of(cyc.begin(), cyc.end(), [](char v) { return v != 0; })) {
-      TV *vj = V.p + n * j;
-      TV *wv = V.p + n * (j + 1);  // the new direction is built in place
-      HIPCHK(c, (glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, vj, Z.p, s)));
-      RCHK((spmv<TJ, TV>(c, J, Z.p, wv)));
-      // h = V^T w with ||w||^2 in one pass, w -= V h with ||w'||^2; a second
-      // pass only when ||w'|| < ||w|| / sqrt(2) for some iterating instance
-      // (Daniel-Gragg-Kaufman-Stewart criterion)
-      HIPCHK(c, glh::launch_seg_mdot<TV>(SI, j + 2, V.p, n, wv, pt.p, hs.p, ldh, 0, false, s));
-      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SI, j + 1, V.p, n, hs.p, ldh, wv, pt.p, nrm, 1, 0, s));
-      HIPCHK(c, hipMemcpyAsync(ph, hs.p, (size_t)S * ldh * sizeof(TV), hipMemcpyDeviceToHost, s));
-      HIPCHK(c, hipMemcpyAsync(pn, nrm, S * sizeof(TV), hipMemcpyDeviceToHost, s));
+    for (int blk = 0; blk < nblk; ++blk) {
+      RCHK(launch(1, blk));
+      if (blk + 1 == nblk) break;
+      HIPCHK(c, hipMemcpyAsync(hconv.data(), c->gmcyc.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
       HIPCHK(c, hipStreamSynchronize(s));
-      bool reorth = false;
-      for (int i = 0; i < S; ++i)
-        if (cyc[i] && (double)pn[i] < 0.5 * (double)ph[(size_t)i * ldh + j + 1]) reorth = true;
-      if (reorth) {
-        HIPCHK(c, glh::launch_seg_mdot<TV>(SI, j + 1, V.p, n, wv, pt.p, hq.p, ldh, 0, false, s));
-        HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SI, j + 1, V.p, n, hq.p, ldh, wv, pt.p, nrm, 1, 0, s));
-        HIPCHK(c, hipMemcpyAsync(pq, hq.p, (size_t)S * ldh * sizeof(TV), hipMemcpyDeviceToHost, s));
-        HIPCHK(c, hipMemcpyAsync(pn, nrm, S * sizeof(TV), hipMemcpyDeviceToHost, s));
-        HIPCHK(c, hipStreamSynchronize(s));
-        ++c->reorth_count;
-      }
-      for (int i = 0; i < S; ++i) hn[i] = std::sqrt(std::max(0.0, (double)pn[i]));
-      for (int i = 0; i < S; ++i) pn[i] = (TV)hn[i];
-      HIPCHK(c, hipMemcpyAsync(sc.p + S, pn, S * sizeof(TV), hipMemcpyHostToDevice, s));
-      HIPCHK(c, glh::launch_seg_scale<TV>(SI, wv, sc.p + S, 1, wv, s));
-      for (int i = 0; i < S; ++i) {
-        if (!cyc[i]) continue;
-        double *Hi = H.data() + (size_t)i * (m + 1) * m;
-        double *ci = cs.data() + (size_t)i * m, *si = sn.data() + (size_t)i * m, *gi = g.data() + (size_t)i * (m + 1);
-        for (int r = 0; r <= j; ++r)
-          Hi[(size_t)r * m + j] = (double)ph[(size_t)i * ldh + r] + (reorth ? (double)pq[(size_t)i * ldh + r] : 0.0);
-        for (int r = 0; r < j; ++r) {
-          const double a = Hi[(size_t)r * m + j], b2 = Hi[(size_t)(r + 1) * m + j];
-          Hi[(size_t)r * m + j] = ci[r] * a + si[r] * b2;
-          Hi[(size_t)(r + 1) * m + j] = -si[r] * a + ci[r] * b2;
-        }
-        const double a = Hi[(size_t)j * m + j], rr = std::hypot(a, hn[i]);
-        ci[j] = rr > 0 ? a / rr : 1.0;
-        si[j] = rr > 0 ? hn[i] / rr : 0.0;
-        Hi[(size_t)j * m + j] = rr;
-        gi[j + 1] = -si[j] * gi[j];
-        gi[j] = ci[j] * gi[j];
-        jseg[i] = j + 1;
-        ++its_seg[i];
-        if (std::fabs(gi[j + 1]) <= target[i] || hn[i] == 0) cyc[i] = 0;
-      }
-      ++j;
-      ++it;
+      bool any = false;
+      for (int i = 0; i < S && !any; ++i) any = hconv[i] != 0;
+      if (!any) break;
     }
-    // per-instance least-squares solution y, then x += M^-1 V y
-    std::fill(ph, ph + (size_t)S * ldh, (TV)0);
-    for (int i = 0; i < S; ++i) {
-      const int ji = jseg[i];
-      const double *Hi = H.data() + (size_t)i * (m + 1) * m, *gi = g.data() + (size_t)i * (m + 1);
-      for (int r = ji - 1; r >= 0; --r) {
-        double sum = gi[r];
-        for (int k = r + 1; k < ji; ++k) sum -= Hi[(size_t)r * m + k] * y[k];
-        y[r] = sum / Hi[(size_t)r * m + r];
-      }
-      for (int r = 0; r < ji; ++r) ph[(size_t)i * ldh + r] = (TV)y[r];
-    }
-    HIPCHK(c, hipMemcpyAsync(yb.p, ph, (size_t)S * ldh * sizeof(TV), hipMemcpyHostToDevice, s));
-    HIPCHK(c, glh::launch_seg_lincomb<TV>(SI, j, V.p, n, yb.p, ldh, W.p, s));
-    HIPCHK(c, (glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, W.p, Z.p, s)));
-    HIPCHK(c, (glh::launch_axpy<TV, TV>(n, 1.0, Z.p, x, s)));
+    RCHK(launch(2, 0));
   }
   RCHK(flush_events(c));
+  // the last head computed the true residual norms of the final iterate
+  std::vector<int> hits(S);
+  HIPCHK(c, hipMemcpyAsync(c->pinv.p, c->gmbeta.p, S * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(hits.data(), c->gmits.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
   *its_out = it;
+  its_seg = hits;
   rel_out.assign(S, 0.0);
-  for (int i = 0; i < S; ++i) rel_out[i] = bn[i] > 0 ? beta[i] / bn[i] : 0.0;
+  for (int i = 0; i < S; ++i) {
+    const double r = c->pinv.p[i];
+    rel_out[i] = bn[i] > 0 ? r / bn[i] : 0.0;
+    if (!std::isfinite(rel_out[i]) || !std::isfinite(bn[i]))
+      return fail(c, GLH_ERR_NOCONV, "GMRES: non-finite residual in instance " + std::to_string(i));
+  }
   return conv ? GLH_OK : fail(c, GLH_ERR_NOCONV, "GMRES reached the maximum iteration number");
 }
 
@@ -769,6 +831,12 @@ int glh_destroy(glh_ctx *c)
   DBuf<int> *ibs[] = {&c->tpa,      &c->tpb,         &c->cell_nodes, &c->row_ptr,  &c->col,       &c->diag,
                       &c->cell_blk, &c->color_cells, &c->status,     &c->cell_mat, &c->seg_chunk, &c->s_flag};
   for (auto *b : ibs) b->release();
+  for (auto *g : {&c->graph64, &c->graph32})
+    for (auto e : g->exec) (void)hipGraphExecDestroy(e);
+  DBuf<double> *gd[] = {&c->gmH, &c->gmcs, &c->gmsn, &c->gmg, &c->gmbeta, &c->gmtarget};
+  for (auto *b : gd) b->release();
+  DBuf<int> *gi[] = {&c->gmcyc, &c->gmconv, &c->gmjseg, &c->gmits, &c->gmro, &c->gmfrozen, &c->gmjlast, &c->row_seg};
+  for (auto *b : gi) b->release();
   c->chunks.release();
   c->mats.release();
   c->cell_face.release();
@@ -958,6 +1026,10 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
   HIPCHK(c, upload(c->cell_mat, cmat.data(), cmat.size(), s));
   HIPCHK(c, upload(c->chunks, ch.data(), ch.size(), s));
   HIPCHK(c, upload(c->seg_chunk, segch.data(), segch.size(), s));
+  std::vector<int> rseg(m.n_nodes);
+  for (int sg = 0; sg < c->n_seg; ++sg)
+    for (int r = c->node_off[sg]; r < c->node_off[sg + 1]; ++r) rseg[r] = sg;
+  HIPCHK(c, upload(c->row_seg, rseg.data(), rseg.size(), s));
   const size_t n = (size_t)m.n_nodes * NC;
   HIPCHK(c, c->U.alloc(n));
   HIPCHK(c, c->dU.alloc(n));
@@ -1111,15 +1183,24 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
   std::vector<double> &st = c->inst_stats;
   st.assign((size_t)S * 8, 0.0);
   for (int i = 0; i < S; ++i) { st[8 * i + 2] = r0[i]; st[8 * i + 4] = -1.0; }
-  std::vector<char> active(S);
+  std::vector<char> active(S), stalled(S, 0);
   HIPCHK(c, c->s_flag.alloc(S));
   const bool f32 = fp32 != 0;
   std::vector<int> flag(S);
+  // batches only: an instance whose ||R|| has fallen by less than 10% over the
+  // last STALL_WINDOW steps is stopped and reported as not converged, so that
+  // it cannot hold the others in lockstep until max_newton
+  constexpr int STALL_WINDOW = 10;
+  std::vector<double> hist;  // [k][S] ||R|| after step k
+  hist.insert(hist.end(), rcur.begin(), rcur.end());
   for (k = 0; k <= max_newton; ++k) {
     bool any = false;
     const double tnow = now_s() - t_start;
     for (int i = 0; i < S; ++i) {
-      active[i] = rcur[i] > rtol * r0[i];
+      if (S > 1 && k >= STALL_WINDOW && rcur[i] > rtol * r0[i] &&
+          rcur[i] > 0.9 * hist[(size_t)(k - STALL_WINDOW) * S + i])
+        stalled[i] = 1;
+      active[i] = !stalled[i] && rcur[i] > rtol * r0[i];
       if (!active[i] && st[8 * i + 4] < 0) st[8 * i + 4] = tnow;
       any = any || active[i];
     }
@@ -1135,7 +1216,15 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
     t_asm += t1 - t0;
     double sst[2] = {0, 0};
     std::vector<int> its_seg;
-    RCHK(solve_impl(c, lin_tol, lin_maxit, restart, sst, &its_seg));
+    // an inexact solve that hit lin_maxit still gives a usable update: the
+    // line search below decides whether it lowers ||R||
+    const int src = solve_impl(c, lin_tol, lin_maxit, restart, sst, &its_seg);
+    if (src == GLH_ERR_NOCONV) {
+      HIPCHK(c, glh::launch_mask_zero(c->n_nodes, c->node_mask.p, c->dU.p, c->stream));
+    } else if (src) {
+      return src;
+    }
+    if (its_seg.size() != (size_t)S) its_seg.assign(S, 0);
     lin_total += (int)sst[0];
     const double t2 = now_s();
     t_sol += t2 - t1;
@@ -1150,7 +1239,22 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
         st[8 * i + 7] += its_seg[i];
       }
     trials_total += *std::max_element(trials.begin(), trials.end());
+    hist.insert(hist.end(), rcur.begin(), rcur.end());
     t_ls += now_s() - t2;
+    if (c->verbose) {
+      int na = 0;
+      double rmax = 0;
+      for (int i = 0; i < S; ++i)
+        if (active[i]) {
+          ++na;
+          rmax = std::max(rmax, rcur[i] / std::max(r0[i], 1e-300));
+        }
+      int ns = 0;
+      for (int i = 0; i < S; ++i) ns += stalled[i];
+      fprintf(stderr, "glhip: newton %d: %d active, %d stalled, worst ||R||/||R0|| %.3e, gmres %d%s, %.2f s\n", k + 1,
+              na, ns, rmax, (int)sst[0], src == GLH_ERR_NOCONV ? " (maxit)" : "", now_s() - t_start);
+      fflush(stderr);
+    }
   }
   std::vector<double> F;
   RCHK(instance_energy(c, F));
@@ -1186,6 +1290,13 @@ int glh_last_assembly_ms(glh_ctx *c, double ms[2])
   if (!c || !ms) return GLH_ERR_ARG;
   ms[0] = c->asm_ms;
   ms[1] = c->asm_launches;
+  return GLH_OK;
+}
+
+int glh_set_verbose(glh_ctx *c, int level)
+{
+  if (!c) return GLH_ERR_ARG;
+  c->verbose = level;
   return GLH_OK;
 }
 

@@ -48,9 +48,10 @@ hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Ma
 
 // --------------------------------------------------------- linear algebra
 // y = J x over BSR(18x18); TJ in {double,float}, TX in {double,float}.
+// row_seg/seg_flag (optional): rows of instances whose flag is 0 are skipped.
 template <typename TJ, typename TX>
 hipError_t launch_spmv(int n_rows, const int *row_ptr, const int *col, const TJ *val, const TX *x, TX *y,
-                       hipStream_t s);
+                       hipStream_t s, const int *row_seg = nullptr, const int *seg_flag = nullptr);
 // r = b - J x in FP64 arithmetic with an FP32 or FP64 matrix.
 template <typename TJ>
 hipError_t launch_residual_spmv(int n_rows, const int *row_ptr, const int *col, const TJ *val, const double *x,
@@ -62,7 +63,8 @@ hipError_t launch_bj_invert(int n_rows, const int *diag_blk, const TJ *val, TJ *
                             hipStream_t s);
 // y = Dinv x, block-diagonal apply.
 template <typename TJ, typename TX>
-hipError_t launch_bj_apply(int n_rows, const TJ *Dinv, const TX *x, TX *y, hipStream_t s);
+hipError_t launch_bj_apply(int n_rows, const TJ *Dinv, const TX *x, TX *y, hipStream_t s,
+                           const int *row_seg = nullptr, const int *seg_flag = nullptr);
 
 // Deterministic reductions.  partial must hold MDOT_GRID * k values.
 constexpr int MDOT_GRID = 2048;
@@ -105,18 +107,46 @@ struct SegInfo {
   const int *seg_chunk;
   int n_chunks, n_seg;
 };
+// flag (optional): instances whose flag is 0 are skipped (outputs untouched).
 template <typename T>
 hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, const T *w, T *partial, T *out, int ldo,
-                           int off, bool accumulate, hipStream_t s);
+                           int off, bool accumulate, hipStream_t s, const int *flag = nullptr);
 // w -= sum_i h[s*ldh + i] V_i per instance; norm2[s*ldo + off] = ||w_s||^2 after
 template <typename T>
 hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size_t ldv, const T *h, int ldh, T *w,
-                                       T *partial, T *norm2, int ldo, int off, hipStream_t s);
+                                       T *partial, T *norm2, int ldo, int off, hipStream_t s,
+                                       const int *flag = nullptr);
+// w = sum_{i < min(k, kseg[s])} y[s*ldy + i] V_i
 template <typename T>
 hipError_t launch_seg_lincomb(const SegInfo &S, int k, const T *V, size_t ldv, const T *y, int ldy, T *w,
-                              hipStream_t s);
+                              hipStream_t s, const int *kseg = nullptr);
+// dst = src / scal[s*lds] (sqrt_of_scale: / sqrt(scal)); unchanged where the scale is 0
 template <typename T>
-hipError_t launch_seg_scale(const SegInfo &S, const T *src, const T *scal, int lds, T *dst, hipStream_t s);
+hipError_t launch_seg_scale(const SegInfo &S, const T *src, const T *scal, int lds, T *dst, hipStream_t s,
+                            const int *flag = nullptr, bool sqrt_of_scale = false);
+// r = b - w with norm2[s] = ||r_s||^2
+template <typename T>
+hipError_t launch_seg_residual(const SegInfo &S, const T *b, const T *w, T *r, T *partial, T *norm2, hipStream_t s);
+template <typename T>
+hipError_t launch_fill(size_t n, T v, T *d, hipStream_t s);
+
+// Device state of the batched GMRES: Hessenberg columns, Givens rotations
+// and stopping tests live on the device so a restart cycle needs no host
+// round trip (and can be captured in a HIP graph).
+template <typename T>
+struct GmresCtl {
+  int S, m, ldh;
+  const T *hs, *hq;      // [S*ldh] Gram-Schmidt coefficients of the two passes
+  const T *nrm2;         // [S] squared norms
+  T *y;                  // [S*ldh] least-squares coefficients
+  double *H, *cs, *sn, *g, *beta;
+  const double *target;  // [S] absolute tolerances
+  int *cyc, *conv, *jseg, *its, *ro;
+  const int *frozen;
+  int *jlast;  // [S] basis size of the previous cycle (written by op 0)
+};
+template <typename T>
+hipError_t launch_gmres_ctl(const GmresCtl<T> &C, int op, int j, hipStream_t s);
 hipError_t launch_seg_trial(const SegInfo &S, const double *U, const double *dU, const double *alpha,
                             const uint32_t *mask, double *X, hipStream_t s);
 template <typename T>

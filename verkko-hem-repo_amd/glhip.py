@@ -26,7 +26,7 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_mesh_destroy", "glh_mesh_sizes", "glh_mesh_get", "glh_upload_mesh", "glh_initial_guess_host", "glh_set_solution", "glh_store_initial", "glh_restore_initial",
            "glh_get_solution", "glh_get_update", "glh_get_rhs", "glh_get_jacobian", "glh_assemble", "glh_rhs_norm",
            "glh_residual", "glh_solve", "glh_newton_step", "glh_free_energy", "glh_newton_solve",
-           "glh_last_assembly_ms", "glh_set_timing", "glh_timing", "glh_bench_spmv", "glh_bench_copy"]
+           "glh_last_assembly_ms", "glh_set_verbose", "glh_set_timing", "glh_timing", "glh_bench_spmv", "glh_bench_copy"]
 
 _lib = None
 
@@ -74,6 +74,7 @@ def lib():
                                    C.c_int, _dp]
     L.glh_last_assembly_ms.argtypes = [vp, _dp]
     L.glh_set_timing.argtypes = [vp, C.c_int]
+    L.glh_set_verbose.argtypes = [vp, C.c_int]
     L.glh_timing.argtypes = [vp, _dp]
     L.glh_bench_spmv.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.glh_bench_copy.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
@@ -269,6 +270,9 @@ class Context:
         v = np.zeros(2)
         self._chk(lib().glh_last_assembly_ms(self.h, v), "glh_last_assembly_ms")
         return v[0]
+
+    def set_verbose(self, level=1):
+        self._chk(lib().glh_set_verbose(self.h, int(level)), "glh_set_verbose")
 
     def set_timing(self, on=True):
         self._chk(lib().glh_set_timing(self.h, int(on)), "glh_set_timing")
