@@ -27,24 +27,33 @@ __device__ inline void load6(const T *__restrict__ p, double *o)
   o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y; o[4] = c.x; o[5] = c.y;
 }
 
+__device__ inline int work_count(const ActiveSet &a, int n_chunks) { return a.counts ? a.counts[1] : n_chunks; }
+__device__ inline int work_chunk(const ActiveSet &a, int i) { return a.counts ? a.chunk[i] : i; }
+
+// block row of the g-th active row: binary search in the row prefix of the
+// active instances
+__device__ inline int active_row(const ActiveSet &a, int g)
+{
+  int lo = 0, hi = a.counts[0];
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (a.rpref[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return a.node_off[a.seg[lo]] + g - a.rpref[lo];
+}
+
 // One wavefront per block row.  Lane = g*18 + c (g < 3): row c of every
 // block, columns 6g..6g+5.  Partial sums of the three column groups are
 // combined with cross-lane reads at the end of the row.
 template <typename TJ, typename TX>
-__global__ __launch_bounds__(256) void k_spmv(int n_rows, const int *__restrict__ rp, const int *__restrict__ col,
-                                              const TJ *__restrict__ val, const TX *__restrict__ x,
-                                              TX *__restrict__ y, const int *__restrict__ row_seg,
-                                              const int *__restrict__ seg_flag)
+__device__ inline void spmv_row(int row, int lane, const int *__restrict__ rp, const int *__restrict__ col,
+                                const TJ *__restrict__ val, const TX *__restrict__ x, TX *__restrict__ y)
 {
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= n_rows) return;
-  if (seg_flag && !seg_flag[row_seg[row]]) return;  // instance not iterating
   const int g = lane / 18, c = lane % 18;
-  const bool active = lane < 54;
   double acc = 0.0;
   const int b0 = rp[row], b1 = rp[row + 1];
-  if (active) {
+  if (lane < 54) {
     int b = b0;
     for (; b + 1 < b1; b += 2) {
       const int s0 = col[b], s1 = col[b + 1];
@@ -69,6 +78,31 @@ __global__ __launch_bounds__(256) void k_spmv(int n_rows, const int *__restrict_
   const double a1 = __shfl(acc, (c + 18) & 63);
   const double a2 = __shfl(acc, (c + 36) & 63);
   if (lane < 18) y[(size_t)row * 18 + c] = (TX)(acc + a1 + a2);
+}
+
+// Without an active set: one row per wave, grid = rows / 4.  With one:
+// grid-stride over the block rows of the listed instances.  Rows of
+// instances whose seg_flag is 0 are skipped.
+template <typename TJ, typename TX>
+__global__ __launch_bounds__(256) void k_spmv(int n_rows, const int *__restrict__ rp, const int *__restrict__ col,
+                                              const TJ *__restrict__ val, const TX *__restrict__ x,
+                                              TX *__restrict__ y, const int *__restrict__ row_seg,
+                                              const int *__restrict__ seg_flag, ActiveSet act)
+{
+  const int lane = threadIdx.x & 63;
+  if (!act.counts) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n_rows) return;
+    if (seg_flag && !seg_flag[row_seg[row]]) return;  // instance not iterating
+    spmv_row(row, lane, rp, col, val, x, y);
+    return;
+  }
+  const int total = act.counts[2];
+  for (int gi = blockIdx.x * 4 + (threadIdx.x >> 6); gi < total; gi += gridDim.x * 4) {
+    const int row = active_row(act, gi);
+    if (seg_flag && !seg_flag[row_seg[row]]) continue;
+    spmv_row(row, lane, rp, col, val, x, y);
+  }
 }
 
 // r = b - J x with FP64 vectors (used for the outer iterative refinement with
@@ -151,14 +185,9 @@ __global__ __launch_bounds__(64) void k_bj_invert(int n_rows, const int *__restr
 }
 
 template <typename TJ, typename TX>
-__global__ __launch_bounds__(256) void k_bj_apply(int n_rows, const TJ *__restrict__ Dinv, const TX *__restrict__ x,
-                                                  TX *__restrict__ y, const int *__restrict__ row_seg,
-                                                  const int *__restrict__ seg_flag)
+__device__ inline void bj_row(int row, int lane, const TJ *__restrict__ Dinv, const TX *__restrict__ x,
+                              TX *__restrict__ y)
 {
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= n_rows) return;
-  if (seg_flag && !seg_flag[row_seg[row]]) return;
   const int g = lane / 18, c = lane % 18;
   double acc = 0.0;
   if (lane < 54) {
@@ -173,21 +202,52 @@ __global__ __launch_bounds__(256) void k_bj_apply(int n_rows, const TJ *__restri
   if (lane < 18) y[(size_t)row * 18 + c] = (TX)(acc + a1 + a2);
 }
 
+template <typename TJ, typename TX>
+__global__ __launch_bounds__(256) void k_bj_apply(int n_rows, const TJ *__restrict__ Dinv, const TX *__restrict__ x,
+                                                  TX *__restrict__ y, const int *__restrict__ row_seg,
+                                                  const int *__restrict__ seg_flag, ActiveSet act)
+{
+  const int lane = threadIdx.x & 63;
+  if (!act.counts) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n_rows) return;
+    if (seg_flag && !seg_flag[row_seg[row]]) return;
+    bj_row(row, lane, Dinv, x, y);
+    return;
+  }
+  const int total = act.counts[2];
+  for (int gi = blockIdx.x * 4 + (threadIdx.x >> 6); gi < total; gi += gridDim.x * 4) {
+    const int row = active_row(act, gi);
+    if (seg_flag && !seg_flag[row_seg[row]]) continue;
+    bj_row(row, lane, Dinv, x, y);
+  }
+}
+
 // -------------------------------------------------------------- reductions
 constexpr int RT = 256;
 constexpr int KCH = 8;  // vectors per multi-dot pass
 
+// Sum over the 64 lanes of a wave (butterfly: every lane gets the same,
+// order-fixed result).
+template <typename T>
+__device__ inline T wave_sum(T v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Sum over the RT threads of a workgroup, returned to every thread, in fixed
+// order (deterministic).  sh: at least RT/64 entries.
 template <typename T>
 __device__ inline T block_reduce(T v, T *sh)
 {
-  const int t = threadIdx.x;
-  sh[t] = v;
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int st = RT / 2; st > 0; st >>= 1) {
-    if (t < st) sh[t] += sh[t + st];
-    __syncthreads();
-  }
-  const T r = sh[0];
+  T r = 0;
+#pragma unroll
+  for (int i = 0; i < RT / 64; ++i) r += sh[i];
   __syncthreads();
   return r;
 }
@@ -348,100 +408,151 @@ __global__ void k_set_scalar(T *p, T v)
 // ------------------------------------------------ segmented (batched) kernels
 // The dof range is cut into chunks that never straddle an instance boundary:
 // ch[3i] = instance, ch[3i+1] = first dof, ch[3i+2] = one past the last dof.
-// One workgroup per chunk; per-chunk partials are reduced per instance in
-// fixed order (deterministic).
+// A workgroup handles one chunk at a time, either every chunk (grid =
+// n_chunks) or, with an active set, the chunks of the instances listed in it
+// (grid-stride over the compacted list, so skipped instances cost no
+// workgroups).  Per-chunk partials are reduced per instance in fixed order
+// (deterministic).
 
+// 2D grid: x = chunks (or the active list), y = groups of KCH vectors.
 template <typename T>
-__global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__restrict__ ch, int k, int i0,
-                                                         const T *__restrict__ V, size_t ldv, const T *__restrict__ w,
-                                                         T *__restrict__ partial, const int *__restrict__ flag)
+__global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__restrict__ ch, int n_chunks,
+                                                         ActiveSet act, int k, const T *__restrict__ V, size_t ldv,
+                                                         const T *__restrict__ w, T *__restrict__ partial,
+                                                         const int *__restrict__ flag)
 {
-  __shared__ T sh[RT];
-  if (flag && !flag[ch[3 * blockIdx.x]]) return;  // skipped instance: its partials are never read
-  T acc[KCH];
-#pragma unroll
-  for (int j = 0; j < KCH; ++j) acc[j] = 0;
+  __shared__ T sh[KCH][RT / 64];
+  const int nw = work_count(act, n_chunks);
+  const int i0 = blockIdx.y * KCH;
   const int kk = (k - i0) < KCH ? (k - i0) : KCH;
-  const long long beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
-  for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
-    const T wv = w[idx];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int wi = blockIdx.x; wi < nw; wi += gridDim.x) {
+    const int cb = work_chunk(act, wi);
+    if (flag && !flag[ch[3 * cb]]) continue;  // skipped instance: its partials are never read
+    const long long beg = ch[3 * cb + 1], end = ch[3 * cb + 2];
+    T acc[KCH];
 #pragma unroll
-    for (int j = 0; j < KCH; ++j)
-      if (j < kk) acc[j] += V[(size_t)(i0 + j) * ldv + idx] * wv;
-  }
-  for (int j = 0; j < kk; ++j) {
-    const T r = block_reduce(acc[j], sh);
-    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * k + i0 + j] = r;
+    for (int j = 0; j < KCH; ++j) acc[j] = 0;
+    for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
+      const T x = w[idx];
+#pragma unroll
+      for (int j = 0; j < KCH; ++j)
+        if (j < kk) acc[j] += V[(size_t)(i0 + j) * ldv + idx] * x;
+    }
+#pragma unroll
+    for (int j = 0; j < KCH; ++j) {
+      const T r = wave_sum(acc[j]);
+      if (lane == 0) sh[j][wv] = r;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < kk) {
+      T r = 0;
+#pragma unroll
+      for (int q = 0; q < RT / 64; ++q) r += sh[threadIdx.x][q];
+      partial[(size_t)cb * k + i0 + threadIdx.x] = r;
+    }
+    __syncthreads();
   }
 }
 
 // out[s*ldo + off + i] (+)= sum over the chunks of instance s of partial[chunk*k + i]
 template <typename T>
-__global__ __launch_bounds__(RT) void k_seg_final(const int *__restrict__ seg_chunk, int k,
+__global__ __launch_bounds__(RT) void k_seg_final(const int *__restrict__ seg_chunk, int n_seg, ActiveSet act, int k,
                                                   const T *__restrict__ partial, T *__restrict__ out, int ldo, int off,
                                                   int accumulate, const int *__restrict__ flag)
 {
   __shared__ T sh[RT];
-  const int sg = blockIdx.x, i = blockIdx.y;
-  if (flag && !flag[sg]) return;
-  T acc = 0;
-  for (int b = seg_chunk[sg] + threadIdx.x; b < seg_chunk[sg + 1]; b += RT) acc += partial[(size_t)b * k + i];
-  const T r = block_reduce(acc, sh);
-  if (threadIdx.x == 0) {
-    T *o = out + (size_t)sg * ldo + off + i;
-    *o = accumulate ? *o + r : r;
+  const int i = blockIdx.y;
+  const int na = act.counts ? act.counts[0] : n_seg;
+  for (int a = blockIdx.x; a < na; a += gridDim.x) {
+    const int sg = act.counts ? act.seg[a] : a;
+    if (flag && !flag[sg]) continue;
+    T acc = 0;
+    for (int b = seg_chunk[sg] + threadIdx.x; b < seg_chunk[sg + 1]; b += RT) acc += partial[(size_t)b * k + i];
+    const T r = block_reduce(acc, sh);
+    if (threadIdx.x == 0) {
+      T *o = out + (size_t)sg * ldo + off + i;
+      *o = accumulate ? *o + r : r;
+    }
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(RT) void k_seg_sub_lincomb_norm(const long long *__restrict__ ch, int k,
-                                                             const T *__restrict__ V, size_t ldv,
-                                                             const T *__restrict__ h, int ldh, T *__restrict__ w,
-                                                             T *__restrict__ partial, const int *__restrict__ flag)
+__global__ __launch_bounds__(RT) void k_seg_sub_lincomb_norm(const long long *__restrict__ ch, int n_chunks,
+                                                             ActiveSet act, int k, const T *__restrict__ V,
+                                                             size_t ldv, const T *__restrict__ h, int ldh,
+                                                             T *__restrict__ w, T *__restrict__ partial,
+                                                             const int *__restrict__ flag)
 {
   __shared__ T sh[RT];
-  const long long sg = ch[3 * blockIdx.x], beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
-  if (flag && !flag[sg]) return;
-  const T *hs = h + sg * ldh;
-  T acc = 0;
-  for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
-    T s = w[idx];
-    for (int i = 0; i < k; ++i) s -= hs[i] * V[(size_t)i * ldv + idx];
-    w[idx] = s;
-    acc += s * s;
+  const int nw = work_count(act, n_chunks);
+  for (int wi = blockIdx.x; wi < nw; wi += gridDim.x) {
+    const int cb = work_chunk(act, wi);
+    const long long sg = ch[3 * cb], beg = ch[3 * cb + 1], end = ch[3 * cb + 2];
+    if (flag && !flag[sg]) continue;
+    const T *hs = h + sg * ldh;
+    T acc = 0;
+    for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
+      T s = w[idx];
+      for (int i = 0; i < k; ++i) s -= hs[i] * V[(size_t)i * ldv + idx];
+      w[idx] = s;
+      acc += s * s;
+    }
+    const T r = block_reduce(acc, sh);
+    if (threadIdx.x == 0) partial[cb] = r;
   }
-  const T r = block_reduce(acc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
 
 template <typename T>
-__global__ __launch_bounds__(RT) void k_seg_lincomb(const long long *__restrict__ ch, int k, const T *__restrict__ V,
-                                                    size_t ldv, const T *__restrict__ y, int ldy, T *__restrict__ w,
+__global__ __launch_bounds__(RT) void k_seg_lincomb(const long long *__restrict__ ch, int n_chunks, ActiveSet act,
+                                                    int k, const T *__restrict__ V, size_t ldv,
+                                                    const T *__restrict__ y, int ldy, T *__restrict__ w,
                                                     const int *__restrict__ kseg)
 {
-  const long long sg = ch[3 * blockIdx.x], beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
-  const T *ys = y + sg * ldy;
-  const int kk = kseg ? (kseg[sg] < k ? kseg[sg] : k) : k;  // basis vectors this instance built
-  for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
-    T s = 0;
-    for (int i = 0; i < kk; ++i) s += ys[i] * V[(size_t)i * ldv + idx];
-    w[idx] = s;
+  const int nw = work_count(act, n_chunks);
+  for (int wi = blockIdx.x; wi < nw; wi += gridDim.x) {
+    const int cb = work_chunk(act, wi);
+    const long long sg = ch[3 * cb], beg = ch[3 * cb + 1], end = ch[3 * cb + 2];
+    const T *ys = y + sg * ldy;
+    const int kk = kseg ? (kseg[sg] < k ? kseg[sg] : k) : k;  // basis vectors this instance built
+    for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
+      T s = 0;
+      for (int i = 0; i < kk; ++i) s += ys[i] * V[(size_t)i * ldv + idx];
+      w[idx] = s;
+    }
   }
 }
 
 // dst = src / scal[s*lds] (unchanged when the scale is 0); with sq != 0 the
 // scale is sqrt(scal)
 template <typename T>
-__global__ __launch_bounds__(RT) void k_seg_scale(const long long *__restrict__ ch, const T *__restrict__ src,
-                                                  const T *__restrict__ scal, int lds, T *__restrict__ dst,
-                                                  const int *__restrict__ flag, int sq)
+__global__ __launch_bounds__(RT) void k_seg_scale(const long long *__restrict__ ch, int n_chunks, ActiveSet act,
+                                                  const T *__restrict__ src, const T *__restrict__ scal, int lds,
+                                                  T *__restrict__ dst, const int *__restrict__ flag, int sq)
 {
-  const long long sg = ch[3 * blockIdx.x], beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
-  if (flag && !flag[sg]) return;
-  T a = scal[sg * lds];
-  if (sq) a = a > (T)0 ? sqrt(a) : (T)0;
-  const T inv = a != (T)0 ? (T)1 / a : (T)1;
-  for (long long idx = beg + threadIdx.x; idx < end; idx += RT) dst[idx] = src[idx] * inv;
+  const int nw = work_count(act, n_chunks);
+  for (int wi = blockIdx.x; wi < nw; wi += gridDim.x) {
+    const int cb = work_chunk(act, wi);
+    const long long sg = ch[3 * cb], beg = ch[3 * cb + 1], end = ch[3 * cb + 2];
+    if (flag && !flag[sg]) continue;
+    T a = scal[sg * lds];
+    if (sq) a = a > (T)0 ? sqrt(a) : (T)0;
+    const T inv = a != (T)0 ? (T)1 / a : (T)1;
+    for (long long idx = beg + threadIdx.x; idx < end; idx += RT) dst[idx] = src[idx] * inv;
+  }
+}
+
+// y += x on the (active) chunks
+template <typename T>
+__global__ __launch_bounds__(RT) void k_seg_add(const long long *__restrict__ ch, int n_chunks, ActiveSet act,
+                                                const T *__restrict__ x, T *__restrict__ y)
+{
+  const int nw = work_count(act, n_chunks);
+  for (int wi = blockIdx.x; wi < nw; wi += gridDim.x) {
+    const int cb = work_chunk(act, wi);
+    const long long beg = ch[3 * cb + 1], end = ch[3 * cb + 2];
+    for (long long idx = beg + threadIdx.x; idx < end; idx += RT) y[idx] += x[idx];
+  }
 }
 
 // X = U + alpha[s] dU with Dirichlet components zeroed
@@ -465,6 +576,61 @@ __global__ __launch_bounds__(RT) void k_seg_zero(const long long *__restrict__ c
   const long long sg = ch[3 * blockIdx.x], beg = ch[3 * blockIdx.x + 1], end = ch[3 * blockIdx.x + 2];
   if (flag[sg]) return;
   for (long long idx = beg + threadIdx.x; idx < end; idx += RT) v[idx] = (T)0;
+}
+
+// Compacted work lists of the instances whose flag is set: one workgroup of
+// AT threads scans the instances (ids and block-row prefix) and the chunks.
+constexpr int AT = 1024;
+__device__ inline int block_scan_incl(int v, int *sh)
+{
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < AT; off <<= 1) {
+    const int u = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += u;
+    __syncthreads();
+  }
+  return sh[t];
+}
+
+__global__ __launch_bounds__(AT) void k_build_active(int n_seg, int n_chunks, const long long *__restrict__ ch,
+                                                     const int *__restrict__ flag, const int *__restrict__ node_off,
+                                                     int *__restrict__ counts, int *__restrict__ seg,
+                                                     int *__restrict__ rpref, int *__restrict__ chunk)
+{
+  __shared__ int sa[AT], sb[AT];
+  int base_n = 0, base_r = 0;
+  for (int t0 = 0; t0 < n_seg; t0 += AT) {
+    const int sgi = t0 + threadIdx.x;
+    const int f = (sgi < n_seg && flag[sgi]) ? 1 : 0;
+    const int r = f ? node_off[sgi + 1] - node_off[sgi] : 0;
+    const int pa = block_scan_incl(f, sa);
+    const int pb = block_scan_incl(r, sb);
+    if (f) {
+      seg[base_n + pa - 1] = sgi;
+      rpref[base_n + pa - 1] = base_r + pb - r;
+    }
+    base_n += sa[AT - 1];
+    base_r += sb[AT - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    counts[0] = base_n;
+    counts[2] = base_r;
+    rpref[base_n] = base_r;
+  }
+  int base_c = 0;
+  for (int t0 = 0; t0 < n_chunks; t0 += AT) {
+    const int ci = t0 + threadIdx.x;
+    const int f = (ci < n_chunks && flag[ch[3 * ci]]) ? 1 : 0;
+    const int pa = block_scan_incl(f, sa);
+    if (f) chunk[base_c + pa - 1] = ci;
+    base_c += sa[AT - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[1] = base_c;
 }
 
 
@@ -500,15 +666,20 @@ __global__ void k_gmres_ctl(GmresCtl<T> C, int op, int j)
   if (op == 2) {
     if (!C.cyc[s]) return;
     const double hn = C.nrm2[s] > (T)0 ? sqrt((double)C.nrm2[s]) : 0.0;
-    for (int r = 0; r <= j; ++r)
-      H[(size_t)r * m + j] = (double)C.hs[(size_t)s * ldh + r] + (C.ro[s] ? (double)C.hq[(size_t)s * ldh + r] : 0.0);
+    const T *hs = C.hs + (size_t)s * ldh, *hq = C.hq + (size_t)s * ldh;
+    const bool ro = C.ro[s] != 0;
+    // apply the previous rotations to the new column; the entry carried from
+    // rotation r to r+1 stays in a register, the column entries and the
+    // rotations are independent loads
+    double cur = (double)hs[0] + (ro ? (double)hq[0] : 0.0);
     for (int r = 0; r < j; ++r) {
-      const double a = H[(size_t)r * m + j], b2 = H[(size_t)(r + 1) * m + j];
-      H[(size_t)r * m + j] = cs[r] * a + sn[r] * b2;
-      H[(size_t)(r + 1) * m + j] = -sn[r] * a + cs[r] * b2;
+      const double nxt = (double)hs[r + 1] + (ro ? (double)hq[r + 1] : 0.0);
+      const double c_ = cs[r], s_ = sn[r];
+      H[(size_t)r * m + j] = c_ * cur + s_ * nxt;
+      cur = -s_ * cur + c_ * nxt;
     }
-    const double a = H[(size_t)j * m + j], rr = hypot(a, hn);
-    cs[j] = rr > 0 ? a / rr : 1.0;
+    const double rr = hypot(cur, hn);
+    cs[j] = rr > 0 ? cur / rr : 1.0;
     sn[j] = rr > 0 ? hn / rr : 0.0;
     H[(size_t)j * m + j] = rr;
     g[j + 1] = -sn[j] * g[j];
@@ -568,11 +739,12 @@ inline int grid_for(size_t n)
 
 template <typename TJ, typename TX>
 hipError_t launch_spmv(int n_rows, const int *row_ptr, const int *col, const TJ *val, const TX *x, TX *y,
-                       hipStream_t s, const int *row_seg, const int *seg_flag)
+                       hipStream_t s, const int *row_seg, const int *seg_flag, ActiveSet act)
 {
   if (n_rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_spmv<TJ, TX>), dim3((n_rows + 3) / 4), dim3(256), 0, s, n_rows, row_ptr, col, val, x, y,
-                     row_seg, seg_flag);
+  const int grid = act.counts ? std::max(1, std::min((n_rows + 3) / 4, ACT_ROW_GRID) >> act.shift) : (n_rows + 3) / 4;
+  hipLaunchKernelGGL((k_spmv<TJ, TX>), dim3(grid), dim3(256), 0, s, n_rows, row_ptr, col, val, x, y, row_seg,
+                     seg_flag, act);
   return hipGetLastError();
 }
 
@@ -595,11 +767,11 @@ hipError_t launch_bj_invert(int n_rows, const int *diag_blk, const TJ *val, TJ *
 
 template <typename TJ, typename TX>
 hipError_t launch_bj_apply(int n_rows, const TJ *Dinv, const TX *x, TX *y, hipStream_t s, const int *row_seg,
-                           const int *seg_flag)
+                           const int *seg_flag, ActiveSet act)
 {
   if (n_rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_bj_apply<TJ, TX>), dim3((n_rows + 3) / 4), dim3(256), 0, s, n_rows, Dinv, x, y, row_seg,
-                     seg_flag);
+  const int grid = act.counts ? std::max(1, std::min((n_rows + 3) / 4, ACT_ROW_GRID) >> act.shift) : (n_rows + 3) / 4;
+  hipLaunchKernelGGL((k_bj_apply<TJ, TX>), dim3(grid), dim3(256), 0, s, n_rows, Dinv, x, y, row_seg, seg_flag, act);
   return hipGetLastError();
 }
 
@@ -702,16 +874,24 @@ hipError_t launch_copy(size_t nbytes, const void *src, void *dst, hipStream_t s)
   return hipGetLastError();
 }
 
+inline int chunk_grid(const SegInfo &S)
+{
+  return S.act.counts ? std::max(1, std::min(S.n_chunks, ACT_CHUNK_GRID) >> S.act.shift) : S.n_chunks;
+}
+inline int seg_grid(const SegInfo &S)
+{
+  return S.act.counts ? std::max(1, std::min(S.n_seg, ACT_SEG_GRID) >> S.act.shift) : S.n_seg;
+}
+
 template <typename T>
 hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, const T *w, T *partial, T *out, int ldo,
                            int off, bool accumulate, hipStream_t s, const int *flag)
 {
   if (k <= 0 || S.n_chunks <= 0) return hipSuccess;
-  for (int i0 = 0; i0 < k; i0 += KCH)
-    hipLaunchKernelGGL((k_seg_mdot_partial<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, k, i0, V, ldv, w,
-                       partial, flag);
-  hipLaunchKernelGGL((k_seg_final<T>), dim3(S.n_seg, k), dim3(RT), 0, s, S.seg_chunk, k, partial, out, ldo, off,
-                     accumulate ? 1 : 0, flag);
+  hipLaunchKernelGGL((k_seg_mdot_partial<T>), dim3(chunk_grid(S), (k + KCH - 1) / KCH), dim3(RT), 0, s, S.chunks,
+                     S.n_chunks, S.act, k, V, ldv, w, partial, flag);
+  hipLaunchKernelGGL((k_seg_final<T>), dim3(seg_grid(S), k), dim3(RT), 0, s, S.seg_chunk, S.n_seg, S.act, k, partial,
+                     out, ldo, off, accumulate ? 1 : 0, flag);
   return hipGetLastError();
 }
 
@@ -720,10 +900,10 @@ hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size
                                        T *partial, T *norm2, int ldo, int off, hipStream_t s, const int *flag)
 {
   if (S.n_chunks <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_seg_sub_lincomb_norm<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, k, V, ldv, h, ldh, w,
-                     partial, flag);
-  hipLaunchKernelGGL((k_seg_final<T>), dim3(S.n_seg, 1), dim3(RT), 0, s, S.seg_chunk, 1, partial, norm2, ldo, off, 0,
-                     flag);
+  hipLaunchKernelGGL((k_seg_sub_lincomb_norm<T>), dim3(chunk_grid(S)), dim3(RT), 0, s, S.chunks, S.n_chunks, S.act, k,
+                     V, ldv, h, ldh, w, partial, flag);
+  hipLaunchKernelGGL((k_seg_final<T>), dim3(seg_grid(S), 1), dim3(RT), 0, s, S.seg_chunk, S.n_seg, S.act, 1, partial,
+                     norm2, ldo, off, 0, flag);
   return hipGetLastError();
 }
 
@@ -732,7 +912,8 @@ hipError_t launch_seg_lincomb(const SegInfo &S, int k, const T *V, size_t ldv, c
                               hipStream_t s, const int *kseg)
 {
   if (S.n_chunks <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_seg_lincomb<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, k, V, ldv, y, ldy, w, kseg);
+  hipLaunchKernelGGL((k_seg_lincomb<T>), dim3(chunk_grid(S)), dim3(RT), 0, s, S.chunks, S.n_chunks, S.act, k, V, ldv,
+                     y, ldy, w, kseg);
   return hipGetLastError();
 }
 
@@ -741,8 +922,24 @@ hipError_t launch_seg_scale(const SegInfo &S, const T *src, const T *scal, int l
                             const int *flag, bool sqrt_of_scale)
 {
   if (S.n_chunks <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_seg_scale<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, src, scal, lds, dst, flag,
-                     sqrt_of_scale ? 1 : 0);
+  hipLaunchKernelGGL((k_seg_scale<T>), dim3(chunk_grid(S)), dim3(RT), 0, s, S.chunks, S.n_chunks, S.act, src, scal,
+                     lds, dst, flag, sqrt_of_scale ? 1 : 0);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_seg_add(const SegInfo &S, const T *x, T *y, hipStream_t s)
+{
+  if (S.n_chunks <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_seg_add<T>), dim3(chunk_grid(S)), dim3(RT), 0, s, S.chunks, S.n_chunks, S.act, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_active(const SegInfo &S, const int *flag, const int *node_off, int *counts, int *seg,
+                               int *rpref, int *chunk, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_build_active, dim3(1), dim3(AT), 0, s, S.n_seg, S.n_chunks, S.chunks, flag, node_off, counts,
+                     seg, rpref, chunk);
   return hipGetLastError();
 }
 
@@ -780,8 +977,8 @@ template <typename T>
 hipError_t launch_seg_residual(const SegInfo &S, const T *b, const T *w, T *r, T *partial, T *norm2, hipStream_t s)
 {
   hipLaunchKernelGGL((k_seg_residual<T>), dim3(S.n_chunks), dim3(RT), 0, s, S.chunks, b, w, r, partial);
-  hipLaunchKernelGGL((k_seg_final<T>), dim3(S.n_seg, 1), dim3(RT), 0, s, S.seg_chunk, 1, partial, norm2, 1, 0, 0,
-                     (const int *)nullptr);
+  hipLaunchKernelGGL((k_seg_final<T>), dim3(S.n_seg, 1), dim3(RT), 0, s, S.seg_chunk, S.n_seg, ActiveSet{}, 1, partial,
+                     norm2, 1, 0, 0, (const int *)nullptr);
   return hipGetLastError();
 }
 
@@ -797,7 +994,8 @@ hipError_t launch_seg_residual(const SegInfo &S, const T *b, const T *w, T *r, T
   template hipError_t launch_seg_zero<T>(const SegInfo &, const int *, T *, hipStream_t);                          \
   template hipError_t launch_gmres_ctl<T>(const GmresCtl<T> &, int, int, hipStream_t);                             \
   template hipError_t launch_fill<T>(size_t, T, T *, hipStream_t);                                                 \
-  template hipError_t launch_seg_residual<T>(const SegInfo &, const T *, const T *, T *, T *, T *, hipStream_t);
+  template hipError_t launch_seg_residual<T>(const SegInfo &, const T *, const T *, T *, T *, T *, hipStream_t);  \
+  template hipError_t launch_seg_add<T>(const SegInfo &, const T *, T *, hipStream_t);
 GLH_SEG_INST(double)
 GLH_SEG_INST(float)
 #undef GLH_SEG_INST
@@ -808,9 +1006,9 @@ template hipError_t launch_set_scalar<double>(double *, double, hipStream_t);
 template hipError_t launch_set_scalar<float>(float *, float, hipStream_t);
 template hipError_t launch_axpy<float, float>(size_t, double, const float *, float *, hipStream_t);
 template hipError_t launch_spmv<double, double>(int, const int *, const int *, const double *, const double *,
-                                                double *, hipStream_t, const int *, const int *);
+                                                double *, hipStream_t, const int *, const int *, ActiveSet);
 template hipError_t launch_spmv<float, float>(int, const int *, const int *, const float *, const float *, float *,
-                                              hipStream_t, const int *, const int *);
+                                              hipStream_t, const int *, const int *, ActiveSet);
 template hipError_t launch_residual_spmv<double>(int, const int *, const int *, const double *, const double *,
                                                  const double *, double *, hipStream_t);
 template hipError_t launch_residual_spmv<float>(int, const int *, const int *, const float *, const double *,
@@ -818,9 +1016,9 @@ template hipError_t launch_residual_spmv<float>(int, const int *, const int *, c
 template hipError_t launch_bj_invert<double>(int, const int *, const double *, double *, int *, hipStream_t);
 template hipError_t launch_bj_invert<float>(int, const int *, const float *, float *, int *, hipStream_t);
 template hipError_t launch_bj_apply<double, double>(int, const double *, const double *, double *, hipStream_t,
-                                                    const int *, const int *);
+                                                    const int *, const int *, ActiveSet);
 template hipError_t launch_bj_apply<float, float>(int, const float *, const float *, float *, hipStream_t,
-                                                  const int *, const int *);
+                                                  const int *, const int *, ActiveSet);
 template hipError_t launch_mdot<double>(size_t, int, const double *, size_t, const double *, double *, double *, bool,
                                         hipStream_t);
 template hipError_t launch_mdot<float>(size_t, int, const float *, size_t, const float *, float *, float *, bool,

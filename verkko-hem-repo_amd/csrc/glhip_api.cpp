@@ -75,7 +75,7 @@ struct HBuf {
   }
 };
 
-constexpr long long CHUNK = 4096;  // dofs per reduction chunk
+constexpr long long CHUNK = 1024;  // dofs per reduction chunk (one workgroup each)
 
 }  // namespace
 
@@ -118,6 +118,8 @@ struct glh_ctx {
   // device-side GMRES control (per instance)
   DBuf<double> gmH, gmcs, gmsn, gmg, gmbeta, gmtarget;
   DBuf<int> gmcyc, gmconv, gmjseg, gmits, gmro, gmfrozen, gmjlast, row_seg;
+  // active set of a batched GMRES cycle (instances still iterating)
+  DBuf<int> d_node_off, act_counts, act_seg, act_rpref, act_chunk;
   // one restart cycle as graphs: head (true residual, start of cycle), body
   // blocks of Arnoldi steps, tail (update of x)
   struct Graph {
@@ -126,6 +128,7 @@ struct glh_ctx {
     int q = 1;
   } graph64, graph32;
   HBuf<float> pin32;
+  HBuf<int> pin_i;
   DBuf<double> ir_r;
   // per-instance results of the last Newton solve: 8 values per instance
   std::vector<double> inst_stats;
@@ -421,14 +424,15 @@ int flush_events(glh_ctx *c)
 }
 
 template <typename TJ, typename TV>
-int spmv(glh_ctx *c, const TJ *J, const TV *x, TV *y, const int *flag = nullptr)
+int spmv(glh_ctx *c, const TJ *J, const TV *x, TV *y, const int *flag = nullptr, glh::ActiveSet act = {})
 {
   const bool t = c->timing && !c->ev_pool.empty();
   if (t) {
     if (2 * c->ev_used >= (int)c->ev_pool.size()) RCHK(flush_events(c));
     HIPCHK(c, hipEventRecord(c->ev_pool[2 * c->ev_used], c->stream));
   }
-  HIPCHK(c, glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, x, y, c->stream, c->row_seg.p, flag));
+  HIPCHK(c, glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, x, y, c->stream, c->row_seg.p, flag,
+                                     act));
   if (t) {
     HIPCHK(c, hipEventRecord(c->ev_pool[2 * c->ev_used + 1], c->stream));
     ++c->ev_used;
@@ -448,7 +452,7 @@ int spmv(glh_ctx *c, const TJ *J, const TV *x, TV *y, const int *flag = nullptr)
 // iterations each instance used.
 template <typename TJ, typename TV>
 int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, int m, const glh::GmresCtl<TV> &C,
-                  bool timed, int part, int j0 = 0, int j1 = 0)
+                  bool timed, int part, int j0 = 0, int j1 = 0, int shift = 0)
 {
   const size_t n = ndofs(c);
   const int ldh = m + 2;
@@ -461,36 +465,48 @@ int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, i
   TV *nrm2 = const_cast<TV *>(C.nrm2);
   TV *hs = const_cast<TV *>(C.hs), *hq = const_cast<TV *>(C.hq);
   const int *rs = c->row_seg.p, *cyc = C.cyc, *ro = C.ro;
-  auto mv = [&](const TV *in, TV *out, const int *flag) -> int {
-    if (timed) return spmv<TJ, TV>(c, J, in, out, flag);
-    HIPCHK(c, (glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, in, out, s, rs, flag)));
+  // batches: the body and tail visit only the instances that iterate in this
+  // cycle (active set built at the end of the head)
+  const bool use_act = c->n_seg > 1;
+  glh::ActiveSet act{};
+  if (use_act)
+    act = glh::ActiveSet{c->act_counts.p, c->act_seg.p, c->act_rpref.p, c->act_chunk.p, c->d_node_off.p, shift};
+  glh::SegInfo SA = SI;
+  SA.act = act;
+  auto mv = [&](const TV *in, TV *out, const int *flag, glh::ActiveSet a) -> int {
+    if (timed) return spmv<TJ, TV>(c, J, in, out, flag, a);
+    HIPCHK(c, (glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, in, out, s, rs, flag, a)));
     return GLH_OK;
   };
   if (part == 0) {  // head: V0 = b - J x, beta_s, cycle flags
-    RCHK(mv(x, W.p, nullptr));
+    RCHK(mv(x, W.p, nullptr, glh::ActiveSet{}));
     HIPCHK(c, glh::launch_seg_residual<TV>(SI, b, W.p, V.p, pt.p, nrm2, s));
     HIPCHK(c, glh::launch_gmres_ctl<TV>(C, 0, 0, s));
+    if (use_act)
+      HIPCHK(c, glh::launch_build_active(SI, cyc, c->d_node_off.p, c->act_counts.p, c->act_seg.p, c->act_rpref.p,
+                                         c->act_chunk.p, s));
     return GLH_OK;
   }
   if (part == 2) {  // tail: x += M^-1 V y
     HIPCHK(c, glh::launch_gmres_ctl<TV>(C, 3, 0, s));
-    HIPCHK(c, glh::launch_seg_lincomb<TV>(SI, m, V.p, n, C.y, ldh, W.p, s, C.jseg));
-    HIPCHK(c, (glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, W.p, Z.p, s)));
-    HIPCHK(c, (glh::launch_axpy<TV, TV>(n, 1.0, Z.p, x, s)));
+    HIPCHK(c, glh::launch_seg_lincomb<TV>(SA, m, V.p, n, C.y, ldh, W.p, s, C.jseg));
+    HIPCHK(c, (glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, W.p, Z.p, s, rs, nullptr, act)));
+    if (use_act) HIPCHK(c, glh::launch_seg_add<TV>(SA, Z.p, x, s));
+    else HIPCHK(c, (glh::launch_axpy<TV, TV>(n, 1.0, Z.p, x, s)));
     return GLH_OK;
   }
   // body block: Arnoldi steps j0 <= j < j1
-  if (j0 == 0) HIPCHK(c, glh::launch_seg_scale<TV>(SI, V.p, nrm2, 1, V.p, s, cyc, true));
+  if (j0 == 0) HIPCHK(c, glh::launch_seg_scale<TV>(SA, V.p, nrm2, 1, V.p, s, cyc, true));
   for (int j = j0; j < j1; ++j) {
     TV *vj = V.p + n * j, *wv = V.p + n * (j + 1);
-    HIPCHK(c, (glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, vj, Z.p, s, rs, cyc)));
-    RCHK(mv(Z.p, wv, cyc));
-    HIPCHK(c, glh::launch_seg_mdot<TV>(SI, j + 2, V.p, n, wv, pt.p, hs, ldh, 0, false, s, cyc));
-    HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SI, j + 1, V.p, n, hs, ldh, wv, pt.p, nrm2, 1, 0, s, cyc));
+    HIPCHK(c, (glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, vj, Z.p, s, rs, cyc, act)));
+    RCHK(mv(Z.p, wv, cyc, act));
+    HIPCHK(c, glh::launch_seg_mdot<TV>(SA, j + 2, V.p, n, wv, pt.p, hs, ldh, 0, false, s, cyc));
+    HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hs, ldh, wv, pt.p, nrm2, 1, 0, s, cyc));
     HIPCHK(c, glh::launch_gmres_ctl<TV>(C, 1, j, s));
-    HIPCHK(c, glh::launch_seg_mdot<TV>(SI, j + 1, V.p, n, wv, pt.p, hq, ldh, 0, false, s, ro));
-    HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SI, j + 1, V.p, n, hq, ldh, wv, pt.p, nrm2, 1, 0, s, ro));
-    HIPCHK(c, glh::launch_seg_scale<TV>(SI, wv, nrm2, 1, wv, s, cyc, true));
+    HIPCHK(c, glh::launch_seg_mdot<TV>(SA, j + 1, V.p, n, wv, pt.p, hq, ldh, 0, false, s, ro));
+    HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hq, ldh, wv, pt.p, nrm2, 1, 0, s, ro));
+    HIPCHK(c, glh::launch_seg_scale<TV>(SA, wv, nrm2, 1, wv, s, cyc, true));
     HIPCHK(c, glh::launch_gmres_ctl<TV>(C, 2, j, s));
   }
   return GLH_OK;
@@ -530,6 +546,11 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
   DBuf<int> *ib[] = {&c->gmcyc, &c->gmconv, &c->gmjseg, &c->gmits, &c->gmro, &c->gmfrozen, &c->gmjlast};
   for (auto *q : ib) HIPCHK(c, q->alloc(S));
   HIPCHK(c, c->pinv.alloc((size_t)4 * S));
+  HIPCHK(c, c->act_counts.alloc(4));
+  HIPCHK(c, c->pin_i.alloc(4));
+  HIPCHK(c, c->act_seg.alloc(S));
+  HIPCHK(c, c->act_rpref.alloc(S + 1));
+  HIPCHK(c, c->act_chunk.alloc(std::max(1, c->n_chunks)));
   glh::GmresCtl<TV> C{S,        m,          ldh,        hs.p,         hq.p,         sc.p,        yb.p,
                       c->gmH.p, c->gmcs.p,  c->gmsn.p,  c->gmg.p,     c->gmbeta.p,  c->gmtarget.p, c->gmcyc.p,
                       c->gmconv.p, c->gmjseg.p, c->gmits.p, c->gmro.p, c->gmfrozen.p, c->gmjlast.p};
@@ -549,55 +570,69 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
   HIPCHK(c, glh::launch_fill<int>(S, 0, c->gmfrozen.p, s));
   HIPCHK(c, glh::launch_fill<TV>(n, (TV)0, x, s));
   HIPCHK(c, hipStreamSynchronize(s));
-  // graphs of one restart cycle, rebuilt whenever their buffers change.  The
+  // Graphs of one restart cycle, rebuilt whenever their buffers change.  The
   // Arnoldi steps go in blocks of q; the host checks between blocks whether
   // any instance still iterates, so at most q - 1 steps of a cycle run as
-  // skipped launches (none for a single instance, q = 1).
+  // skipped launches (none for a single instance, q = 1).  Batches record
+  // the body and tail at three grid sizes (shift 0, 3, 6) and replay the
+  // one that fits the number of rows still iterating, so that a few
+  // straggling instances do not pay for the workgroups of the full batch.
   const bool timed = c->timing && !c->ev_pool.empty();
   const int q = S == 1 ? 1 : std::min(m, 8);
   const int nblk = (m + q - 1) / q;
+  constexpr int NLEV = 3;
   auto &gc = Work<TV>::graph(c);
-  const std::vector<const void *> key = {J, Dinv, b, x, V.p, Z.p, W.p, pt.p, hs.p, c->gmH.p, c->gmbeta.p,
+  // every buffer and size the captured launches use
+  const std::vector<const void *> key = {J,           Dinv,          b,           x,            V.p,
+                                         Z.p,         W.p,           pt.p,        hs.p,         hq.p,
+                                         yb.p,        sc.p,          c->gmH.p,    c->gmcs.p,    c->gmsn.p,
+                                         c->gmg.p,    c->gmbeta.p,   c->gmtarget.p, c->gmcyc.p, c->gmconv.p,
+                                         c->gmjseg.p, c->gmits.p,    c->gmro.p,   c->gmfrozen.p, c->gmjlast.p,
+                                         c->row_seg.p, c->d_node_off.p, c->act_counts.p, c->act_seg.p,
+                                         c->act_rpref.p, c->act_chunk.p, c->chunks.p, c->seg_chunk.p,
+                                         c->row_ptr.p, c->col.p,
                                          (const void *)(size_t)m, (const void *)(size_t)S,
-                                         (const void *)(size_t)c->n_chunks};
-  if (!timed && (gc.exec.empty() || gc.key != key)) {
-    for (auto e : gc.exec) (void)hipGraphExecDestroy(e);
-    gc.exec.clear();
-    gc.key.clear();
-    for (int g = 0; g < 2 + nblk; ++g) {
-      const int part = g == 0 ? 0 : g == 1 ? 2 : 1;
-      const int j0 = g >= 2 ? (g - 2) * q : 0, j1 = std::min(m, j0 + q);
+                                         (const void *)(size_t)c->n_chunks, (const void *)(size_t)c->n_nodes};
+  if (!timed && gc.key != key) {
+    for (auto e : gc.exec)
+      if (e) (void)hipGraphExecDestroy(e);
+    gc.exec.assign(1 + NLEV * (1 + nblk), nullptr);
+    gc.key = key;
+    gc.q = q;
+  }
+  // graph slot: 0 head; per level L: tail, then the body blocks
+  auto slot = [&](int part, int lev, int blk) { return part == 0 ? 0 : 1 + lev * (1 + nblk) + (part == 2 ? 0 : 1 + blk); };
+  auto launch = [&](int part, int lev, int blk) -> int {
+    const int j0 = blk * q, j1 = std::min(m, j0 + q), shift = 3 * lev;
+    if (timed) return enqueue_cycle<TJ, TV>(c, J, Dinv, b, x, m, C, true, part, j0, j1, shift);
+    hipGraphExec_t &ge = gc.exec[slot(part, lev, blk)];
+    if (!ge) {
       hipGraph_t graph = nullptr;
       HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      const int rc = enqueue_cycle<TJ, TV>(c, J, Dinv, b, x, m, C, false, part, j0, j1);
+      const int rc = enqueue_cycle<TJ, TV>(c, J, Dinv, b, x, m, C, false, part, j0, j1, shift);
       hipError_t e = hipStreamEndCapture(s, &graph);
       if (rc) {
         if (graph) (void)hipGraphDestroy(graph);
         return rc;
       }
       HIPCHK(c, e);
-      hipGraphExec_t ge = nullptr;
       e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       HIPCHK(c, e);
-      gc.exec.push_back(ge);
     }
-    gc.key = key;
-    gc.q = q;
-  }
-  auto launch = [&](int part, int blk) -> int {
-    if (timed) return enqueue_cycle<TJ, TV>(c, J, Dinv, b, x, m, C, true, part, blk * q, std::min(m, blk * q + q));
-    HIPCHK(c, hipGraphLaunch(gc.exec[part == 0 ? 0 : part == 2 ? 1 : 2 + blk], s));
+    HIPCHK(c, hipGraphLaunch(ge, s));
     return GLH_OK;
   };
   // cycle loop
   int it = 0;
   bool conv = false;
   std::vector<int> hconv(S), hj(S);
+  int *hcnt = c->pin_i.p;
   while (true) {
-    RCHK(launch(0, 0));
+    RCHK(launch(0, 0, 0));
     HIPCHK(c, hipMemcpyAsync(hconv.data(), c->gmconv.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipMemcpyAsync(hj.data(), c->gmjlast.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (S > 1) HIPCHK(c, hipMemcpyAsync(hcnt, c->act_counts.p, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     bool all = true;
     int jmax = 0;
@@ -611,8 +646,13 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
       break;
     }
     if (it >= maxit) break;
+    int lev = 0;
+    if (S > 1) {
+      const double f = (double)hcnt[2] / std::max(1, c->n_nodes);
+      lev = f > 1.0 / 8 ? 0 : f > 1.0 / 64 ? 1 : 2;
+    }
     for (int blk = 0; blk < nblk; ++blk) {
-      RCHK(launch(1, blk));
+      RCHK(launch(1, lev, blk));
       if (blk + 1 == nblk) break;
       HIPCHK(c, hipMemcpyAsync(hconv.data(), c->gmcyc.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
       HIPCHK(c, hipStreamSynchronize(s));
@@ -620,7 +660,7 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
       for (int i = 0; i < S && !any; ++i) any = hconv[i] != 0;
       if (!any) break;
     }
-    RCHK(launch(2, 0));
+    RCHK(launch(2, lev, 0));
   }
   RCHK(flush_events(c));
   // the last head computed the true residual norms of the final iterate
@@ -832,10 +872,13 @@ int glh_destroy(glh_ctx *c)
                       &c->cell_blk, &c->color_cells, &c->status,     &c->cell_mat, &c->seg_chunk, &c->s_flag};
   for (auto *b : ibs) b->release();
   for (auto *g : {&c->graph64, &c->graph32})
-    for (auto e : g->exec) (void)hipGraphExecDestroy(e);
+    for (auto e : g->exec)
+      if (e) (void)hipGraphExecDestroy(e);
   DBuf<double> *gd[] = {&c->gmH, &c->gmcs, &c->gmsn, &c->gmg, &c->gmbeta, &c->gmtarget};
   for (auto *b : gd) b->release();
-  DBuf<int> *gi[] = {&c->gmcyc, &c->gmconv, &c->gmjseg, &c->gmits, &c->gmro, &c->gmfrozen, &c->gmjlast, &c->row_seg};
+  DBuf<int> *gi[] = {&c->gmcyc,   &c->gmconv,     &c->gmjseg,     &c->gmits,    &c->gmro,     &c->gmfrozen,
+                     &c->gmjlast, &c->row_seg,    &c->d_node_off, &c->act_counts, &c->act_seg, &c->act_rpref,
+                     &c->act_chunk};
   for (auto *b : gi) b->release();
   c->chunks.release();
   c->mats.release();
@@ -844,6 +887,7 @@ int glh_destroy(glh_ctx *c)
   c->pin64.release();
   c->pinv.release();
   c->pin32.release();
+  c->pin_i.release();
   for (auto e : c->ev_pool)
     if (e) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -962,6 +1006,13 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
   HIPCHK(c, hipSetDevice(c->device));
   if (c->deg != m.deg || !c->tN.p) RCHK(build_tables(c, m.deg));
   hipStream_t s = c->stream;
+  // captured GMRES graphs refer to the previous mesh's buffers
+  for (auto *g : {&c->graph64, &c->graph32}) {
+    for (auto e : g->exec)
+      if (e) (void)hipGraphExecDestroy(e);
+    g->exec.clear();
+    g->key.clear();
+  }
   c->n_cells = m.n_cells;
   c->n_nodes = m.n_nodes;
   c->nnzb = m.row_ptr.back();
@@ -1030,6 +1081,7 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
   for (int sg = 0; sg < c->n_seg; ++sg)
     for (int r = c->node_off[sg]; r < c->node_off[sg + 1]; ++r) rseg[r] = sg;
   HIPCHK(c, upload(c->row_seg, rseg.data(), rseg.size(), s));
+  HIPCHK(c, upload(c->d_node_off, c->node_off.data(), c->node_off.size(), s));
   const size_t n = (size_t)m.n_nodes * NC;
   HIPCHK(c, c->U.alloc(n));
   HIPCHK(c, c->dU.alloc(n));

@@ -47,11 +47,28 @@ hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Ma
                          int n_cells, double *Fcell, hipStream_t s);
 
 // --------------------------------------------------------- linear algebra
+// Compacted work lists of the instances that take part in a batched
+// operation, built on the device (launch_build_active) so that a replayed
+// graph spends workgroups only on them: counts[0] instances, counts[1]
+// chunks, counts[2] block rows; seg[] the instance ids, rpref[] the prefix
+// of their block rows (counts[0]+1 values), chunk[] their chunk ids.
+// counts == nullptr: no list, every instance takes part.
+// shift: the grids of the grid-stride launches are divided by 2^shift
+// (chosen by the host from the size of the set; any value is correct).
+struct ActiveSet {
+  const int *counts = nullptr, *seg = nullptr, *rpref = nullptr, *chunk = nullptr, *node_off = nullptr;
+  int shift = 0;
+};
+constexpr int ACT_ROW_GRID = 4096;   // workgroups of 4 rows, grid-stride over listed rows
+constexpr int ACT_CHUNK_GRID = 1024; // workgroups, grid-stride over listed chunks
+constexpr int ACT_SEG_GRID = 256;    // workgroups per reduced value, grid-stride over listed instances
+
 // y = J x over BSR(18x18); TJ in {double,float}, TX in {double,float}.
 // row_seg/seg_flag (optional): rows of instances whose flag is 0 are skipped.
 template <typename TJ, typename TX>
 hipError_t launch_spmv(int n_rows, const int *row_ptr, const int *col, const TJ *val, const TX *x, TX *y,
-                       hipStream_t s, const int *row_seg = nullptr, const int *seg_flag = nullptr);
+                       hipStream_t s, const int *row_seg = nullptr, const int *seg_flag = nullptr,
+                       ActiveSet act = ActiveSet{});
 // r = b - J x in FP64 arithmetic with an FP32 or FP64 matrix.
 template <typename TJ>
 hipError_t launch_residual_spmv(int n_rows, const int *row_ptr, const int *col, const TJ *val, const double *x,
@@ -64,7 +81,8 @@ hipError_t launch_bj_invert(int n_rows, const int *diag_blk, const TJ *val, TJ *
 // y = Dinv x, block-diagonal apply.
 template <typename TJ, typename TX>
 hipError_t launch_bj_apply(int n_rows, const TJ *Dinv, const TX *x, TX *y, hipStream_t s,
-                           const int *row_seg = nullptr, const int *seg_flag = nullptr);
+                           const int *row_seg = nullptr, const int *seg_flag = nullptr,
+                           ActiveSet act = ActiveSet{});
 
 // Deterministic reductions.  partial must hold MDOT_GRID * k values.
 constexpr int MDOT_GRID = 2048;
@@ -106,6 +124,7 @@ struct SegInfo {
   const long long *chunks;
   const int *seg_chunk;
   int n_chunks, n_seg;
+  ActiveSet act;  // optional: only the listed instances' chunks are visited
 };
 // flag (optional): instances whose flag is 0 are skipped (outputs untouched).
 template <typename T>
@@ -129,6 +148,14 @@ template <typename T>
 hipError_t launch_seg_residual(const SegInfo &S, const T *b, const T *w, T *r, T *partial, T *norm2, hipStream_t s);
 template <typename T>
 hipError_t launch_fill(size_t n, T v, T *d, hipStream_t s);
+// y += x per instance (over the active set when S.act is set)
+template <typename T>
+hipError_t launch_seg_add(const SegInfo &S, const T *x, T *y, hipStream_t s);
+// Active set of the instances whose flag is nonzero (node_off: [n_seg+1]
+// first block row of every instance; seg: n_seg, rpref: n_seg+1, chunk:
+// n_chunks, counts: 3 ints).
+hipError_t launch_build_active(const SegInfo &S, const int *flag, const int *node_off, int *counts, int *seg,
+                               int *rpref, int *chunk, hipStream_t s);
 
 // Device state of the batched GMRES: Hessenberg columns, Givens rotations
 // and stopping tests live on the device so a restart cycle needs no host
