@@ -29,6 +29,7 @@ extern "C" {
 
 typedef struct glh_ctx glh_ctx;
 typedef struct glh_mesh glh_mesh;
+typedef struct glh_forest glh_forest;
 
 #define GLH_OK 0
 #define GLH_ERR_ARG -1
@@ -94,6 +95,66 @@ int glh_mesh_get(const glh_mesh *m, int *cell_nodes, double *cell_h, double *nod
 /* node_off[n_inst+1], cell_off[n_inst+1] (either may be NULL) */
 int glh_mesh_instances(const glh_mesh *m, int *node_off, int *cell_off);
 
+/* Hanging-node data of a mesh (adaptive meshes; zero for conforming ones):
+ * sizes[0] = hanging nodes, [1] = constraint-line entries, [2] = blocks of
+ * the pattern the cell loop assembles into. */
+int glh_mesh_constraint_sizes(const glh_mesh *m, int64_t sizes[3]);
+/* Closed hanging-node lines U[h] = sum_k hang_w[k] U[hang_m[k]],
+ * k in [hang_ptr[h], hang_ptr[h+1]) (hang_ptr[n_nodes+1]), and the cell-loop
+ * pattern arow_ptr[n_nodes+1], acol[blocks].  The system pattern of
+ * glh_mesh_get is the condensed one (make_sparsity_pattern with constraints,
+ * keep_constrained_dofs = false, setup_uniform_B-phase.cc:190-196). */
+int glh_mesh_constraints(const glh_mesh *m, int *hang_ptr, int *hang_m, double *hang_w, int *arow_ptr,
+                         int *acol);
+
+/* ------------------------------------------------ adaptive meshes (Q1) */
+/* Forest of octrees over nx*ny*nz root cells of the box [lo, hi] with the
+ * side ids of glh_mesh_box: the parallel::distributed::Triangulation of the
+ * reference's adaptive cycle (femgl/src/refine.cc:128-173). */
+int glh_forest_create(int nx, int ny, int nz, const double lo[3], const double hi[3], const int side_id[6],
+                      glh_forest **out);
+int glh_forest_destroy(glh_forest *f);
+int64_t glh_forest_n_cells(const glh_forest *f);
+/* New forest: the cells with flags[e] != 0 split into 8, then neighbours
+ * (faces, edges, vertices) more than one level apart are refined
+ * (execute_coarsening_and_refinement with 2:1 balance).  f is unchanged. */
+int glh_forest_refine(const glh_forest *f, const uint8_t *flags, glh_forest **out);
+/* Q1 mesh of the leaves (cell e = leaf e, Morton order) with closed
+ * hanging-node constraints (DoFTools::make_hanging_node_constraints,
+ * AffineConstraints::close; setup_uniform_B-phase.cc:137-160). */
+int glh_forest_mesh(const glh_forest *f, glh_mesh **out);
+/* SolutionTransfer::interpolate + constraints_solution.distribute
+ * (refine.cc:163-170): Un on the mesh mn of fn from Uo on the mesh mo of fo,
+ * where fn refines fo. */
+int glh_forest_transfer(const glh_forest *fo, const glh_mesh *mo, const double *Uo, const glh_forest *fn,
+                        const glh_mesh *mn, double *Un);
+/* KellyErrorEstimator<3>::estimate (refine.cc:146-151) of the Q1 field U:
+ * eta[n_cells] in float, as the reference's Vector<float>. */
+int glh_kelly(const glh_forest *f, const glh_mesh *m, const double *U, float *eta);
+/* refine_and_coarsen_fixed_number(eta, frac, 0) (refine.cc:153-157): flags
+ * the floor(frac * n) cells of largest eta (ties in cell order). */
+int glh_flag_fixed_number(int64_t n, const float *eta, double frac, uint8_t *flags);
+
+/* FemGL::run (femgl/src/run.cc:109-260) for adaptive Q1 meshes: cycle 0
+ * solves on the mesh of f0 from U0 (n_nodes*18 of that mesh); each of the
+ * n_refine further cycles flags refine_frac of the cells by Kelly, refines
+ * (2:1 balance), transfers the converged field and solves again.  Each
+ * solve is glh_newton_solve with the given controls (fp32: mixed
+ * precision).  check_fp64 (with fp32): every cycle is also solved in FP64
+ * from the same start.  cycle_stats[(n_refine+1) * GLH_CYCLE_STATS]:
+ * [0] cycle [1] cells [2] dofs [3] hanging nodes [4] Newton steps
+ * [5] GMRES iterations [6] ||R0|| [7] ||R|| [8] F [9] converged
+ * [10] solve seconds [11] cycle seconds (mesh, solve, Kelly, refinement,
+ * transfer) [12] F of the FP64 solve [13] |F - F64| / |F64|
+ * [14] max|U - U64| / max|U64| [15] FP64 converged (NaN without check).
+ * On return ctx holds the last mesh and its solution; *f_out / *m_out (may
+ * be NULL) receive the last forest and mesh. */
+#define GLH_CYCLE_STATS 16
+int glh_run_adaptive(glh_ctx *ctx, const glh_forest *f0, const double *U0, int n_refine, double refine_frac,
+                     double rtol, int max_newton, double lin_tol, int lin_maxit, int restart, double ls_step,
+                     int damped, int fp32, int check_fp64, double *cycle_stats, glh_forest **f_out,
+                     glh_mesh **m_out);
+
 /* ------------------------------------------------- per refinement cycle */
 /* Uploads geometry, cell->node map, constraint masks, colour sets and the
  * BSR pattern once per refinement cycle (setup_uniform_B-phase.cc:125-198).
@@ -107,6 +168,12 @@ int glh_upload_mesh(glh_ctx *ctx, const glh_mesh *m);
  * gap `gap` plus amp*gap*(2u-1), u = (mt19937_64(seed) >> 11) * 2^-53 for
  * every dof in node-major order.  U[n_nodes*18]. */
 int glh_initial_guess_host(int n_nodes, double gap, int phase, double amp, uint64_t seed, double *U);
+/* A-phase / B-phase flat-wall guess (femgl/inc/BnA.h:130-163, used by
+ * setup_uniform_BnA-flatwall-configuration.cc:228-235): bulk B phase where
+ * z >= z_a, bulk A phase below, plus amp*gap_B*(2u-1) noise drawn as above.
+ * node_xyz[n_nodes*3]. */
+int glh_initial_guess_bna(int n_nodes, const double *node_xyz, double z_a, double gap_A, double gap_B, double amp,
+                          uint64_t seed, double *U);
 int glh_set_solution(glh_ctx *ctx, const double *U_host);       /* local_solution */
 /* keep a device copy of the current solution / restore it (resets a solve
  * without a host transfer) */
@@ -116,7 +183,7 @@ int glh_get_solution(glh_ctx *ctx, double *U_host);
 int glh_get_update(glh_ctx *ctx, double *dU_host);              /* locally_relevant_newton_solution */
 int glh_get_rhs(glh_ctx *ctx, double *rhs_host);                /* system_rhs (= -R) */
 /* system_matrix values in BSR order [nnzb][18][18] (FP64; an FP32 Jacobian
- * is widened).  For parity dumps (SURVEY glh_export_csr). */
+ * is widened).  For parity dumps of the assembled system. */
 int glh_get_jacobian(glh_ctx *ctx, double *val_host);
 
 /* ---------------------------------------------------------- Newton step */

@@ -1206,6 +1206,26 @@ void orc_initial_guess(int n_nodes, double gap, int phase, double amp, uint64_t 
     }
 }
 
+/* A-phase / B-phase flat wall (femgl/inc/BnA.h:130-163): B phase at
+   z >= z_a (u11 = u22 = u33 = gap_B * 0.5773502691896258f), A phase below
+   (u11 = v12 = gap_A * 0.7071067811865475f), plus amp*gap_B*(2u-1) noise. */
+void orc_initial_guess_bna(int n_nodes, const double *xyz, double z_a, double gap_A, double gap_B, double amp,
+                           uint64_t seed, double *U)
+{
+  mt64 s;
+  mt64_seed(&s, seed);
+  for (int nd = 0; nd < n_nodes; ++nd) {
+    int b = xyz[3 * nd + 2] >= z_a;
+    for (int c = 0; c < NC; ++c) {
+      double base = 0;
+      if (b && (c == 0 || c == 4 || c == 8)) base = gap_B * (double)0.5773502691896258f;
+      if (!b && (c == 0 || c == 10)) base = gap_A * (double)0.7071067811865475f;
+      double r = (double)(mt64_next(&s) >> 11) * (1.0 / 9007199254740992.0);
+      U[(size_t)nd * NC + c] = base + amp * gap_B * (2.0 * r - 1.0);
+    }
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* Newton driver: assemble -> GMRES -> damped line search, as           */
 /* run.cc:208-218 + iteration.cc:109-215, with the relative stopping    */

@@ -49,6 +49,8 @@ def lib():
         L.orc_distribute.argtypes = [C.c_int, _ip, _ip, _ip, _dp, _dp]
         L.orc_mt64.argtypes = [C.c_uint64, C.c_int, _up]
         L.orc_initial_guess.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_uint64, _dp]
+        L.orc_initial_guess_bna.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                            C.c_uint64, _dp]
         L.orc_newton.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_int, _ip, C.c_int, _ip, _ip, _ip, _dp, _dp,
                                  _dp, _ip, _ip, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int,
                                  C.c_double, C.c_int, _dp]
@@ -258,6 +260,14 @@ def initial_guess(n_nodes, gap, phase=0, amp=0.05, seed=0):
     return U
 
 
+def initial_guess_bna(xyz, z_a, gap_A, gap_B, amp=0.05, seed=1):
+    xyz = f64(xyz).reshape(-1)
+    n = xyz.size // 3
+    U = np.zeros(n * 18)
+    lib().orc_initial_guess_bna(n, xyz, z_a, gap_A, gap_B, amp, seed, U)
+    return U
+
+
 def newton(mesh, co, U, rtol=1e-10, max_newton=50, lin_tol=0.1, lin_maxit=5000, restart=50, ls_step=0.83,
            damped=True):
     U = f64(U).copy()
@@ -267,3 +277,188 @@ def newton(mesh, co, U, rtol=1e-10, max_newton=50, lin_tol=0.1, lin_maxit=5000, 
                           mesh.con_ptr, mesh.con_master, mesh.con_weight, f64(co), U, mesh.row_ptr, mesh.col,
                           rtol, max_newton, lin_tol, lin_maxit, restart, ls_step, int(damped), stats)
     return U, rc, stats
+
+
+# ---------------------------------------------------------------- adaptive
+def hanging_lines(cell_nodes, xyz, cell_lo, cell_h, tol=1e-9):
+    """Independent (test-side) hanging-node detection for Q1 hex meshes: a node
+    inside an edge or face of a cell that does not list it follows the
+    trilinear interpolant of that cell.  Lines are closed (masters that hang
+    themselves are expanded).  Returns {node: {master: weight}}."""
+    cells = np.asarray(cell_nodes).reshape(-1, 8)
+    lo = np.asarray(cell_lo).reshape(-1, 3)
+    h = np.asarray(cell_h).reshape(-1, 3)
+    xyz = np.asarray(xyz).reshape(-1, 3)
+    raw = {}
+    for e in range(cells.shape[0]):
+        t = (xyz - lo[e]) / h[e]                       # node positions in cell coordinates
+        inside = np.all((t > -tol) & (t < 1 + tol), 1)
+        inside[cells[e]] = False
+        for v in np.nonzero(inside)[0]:
+            if v in raw:
+                continue
+            tv = np.clip(t[v], 0.0, 1.0)
+            line = {}
+            for a in range(8):
+                bits = (a & 1, (a >> 1) & 1, a >> 2)
+                w = 1.0
+                for d in range(3):
+                    w *= tv[d] if bits[d] else 1.0 - tv[d]
+                if w > 1e-14:
+                    line[int(cells[e][a])] = line.get(int(cells[e][a]), 0.0) + w
+            raw[int(v)] = line
+    closed = {}
+    for v in raw:
+        line = dict(raw[v])
+        for _ in range(32):
+            nxt = {}
+            changed = False
+            for m, w in line.items():
+                if m in raw:
+                    changed = True
+                    for m2, w2 in raw[m].items():
+                        nxt[m2] = nxt.get(m2, 0.0) + w * w2
+                else:
+                    nxt[m] = nxt.get(m, 0.0) + w
+            line = nxt
+            if not changed:
+                break
+        closed[v] = line
+    return closed
+
+
+def adaptive_oracle_mesh(cell_nodes, xyz, cell_lo, cell_h, faces, box_lo, box_hi, side_masks, deg=1):
+    """Oracle description of an adaptive Q1 mesh with deal.II constraint
+    semantics: hanging-node lines first (make_hanging_node_constraints), then
+    homogeneous Dirichlet lines for boundary dofs not yet constrained
+    (interpolate_boundary_values), then close (constrained masters replaced
+    by their lines; Dirichlet masters drop out).  side_masks[6]: 18-bit
+    component masks of the faces -x,+x,-y,+y,-z,+z."""
+    xyz = np.asarray(xyz).reshape(-1, 3)
+    n_nodes = xyz.shape[0]
+    hang = hanging_lines(cell_nodes, xyz, cell_lo, cell_h)
+    lines = {}                                      # dof -> {master dof: w}
+    for v, line in hang.items():
+        for c in range(18):
+            lines[v * 18 + c] = {m * 18 + c: w for m, w in line.items()}
+    tol = 1e-9 * max(abs(b - a) for a, b in zip(box_lo, box_hi))
+    for v in range(n_nodes):
+        for s in range(6):
+            d = s // 2
+            on = abs(xyz[v, d] - (box_lo[d] if s % 2 == 0 else box_hi[d])) < tol
+            if not on:
+                continue
+            for c in range(18):
+                if (side_masks[s] >> c) & 1 and v * 18 + c not in lines:
+                    lines[v * 18 + c] = {}
+    for g in list(lines):
+        line = lines[g]
+        for _ in range(32):
+            nxt, changed = {}, False
+            for m, w in line.items():
+                if m in lines:
+                    changed = True
+                    for m2, w2 in lines[m].items():
+                        nxt[m2] = nxt.get(m2, 0.0) + w * w2
+                else:
+                    nxt[m] = nxt.get(m, 0.0) + w
+            line = nxt
+            if not changed:
+                break
+        lines[g] = line
+    n_dofs = n_nodes * 18
+    con_of = -np.ones(n_dofs, np.int32)
+    con_ptr, con_master, con_weight = [0], [], []
+    for i, g in enumerate(sorted(lines)):
+        con_of[g] = i
+        for m, w in sorted(lines[g].items()):
+            con_master.append(m)
+            con_weight.append(w)
+        con_ptr.append(len(con_master))
+    # node pattern able to hold every condensed entry
+    cells = np.asarray(cell_nodes).reshape(-1, 8)
+    tgt = {}
+    for a in range(n_nodes):
+        t = set()
+        for c in range(18):
+            g = a * 18 + c
+            if g in lines:
+                t.update(m // 18 for m in lines[g])
+            else:
+                t.add(a)
+        tgt[a] = t
+    nb = [set([r]) for r in range(n_nodes)]
+    for cn in cells:
+        for a in cn:
+            for b in cn:
+                for r in tgt[int(a)]:
+                    nb[r].update(tgt[int(b)])
+    row_ptr = np.zeros(n_nodes + 1, np.int32)
+    cols = []
+    for r in range(n_nodes):
+        s = sorted(nb[r])
+        cols.extend(s)
+        row_ptr[r + 1] = row_ptr[r] + len(s)
+    return Mesh(deg, np.asarray(cell_nodes).reshape(-1), np.asarray(cell_h).reshape(-1), faces, n_nodes, con_of,
+                np.array(con_ptr, np.int32), con_master, con_weight, row_ptr, np.array(cols, np.int32), xyz)
+
+
+def kelly(cell_nodes, cell_lo, cell_h, U, tol=1e-9):
+    """Test-side Kelly indicator (deal.II's documented formula, no Neumann
+    data): eta_K^2 = sum_F h_K/24 int_F sum_c [du_c/dn]^2 over interior face
+    pieces, integrated with 2x2 Gauss points on the smaller face."""
+    cells = np.asarray(cell_nodes).reshape(-1, 8)
+    lo = np.asarray(cell_lo).reshape(-1, 3)
+    h = np.asarray(cell_h).reshape(-1, 3)
+    U = np.asarray(U).reshape(-1, 18)
+    n = cells.shape[0]
+    e2 = np.zeros(n)
+    g = [0.5 - 0.5 / np.sqrt(3.0), 0.5 + 0.5 / np.sqrt(3.0)]
+
+    def grad(e, x):
+        t = (x - lo[e]) / h[e]
+        out = np.zeros((3, 18))
+        for a in range(8):
+            b = (a & 1, (a >> 1) & 1, a >> 2)
+            L = [t[d] if b[d] else 1 - t[d] for d in range(3)]
+            dL = [(1.0 if b[d] else -1.0) / h[e][d] for d in range(3)]
+            out[0] += dL[0] * L[1] * L[2] * U[cells[e][a]]
+            out[1] += L[0] * dL[1] * L[2] * U[cells[e][a]]
+            out[2] += L[0] * L[1] * dL[2] * U[cells[e][a]]
+        return out
+
+    hi = lo + h
+    for e in range(n):
+        for d in range(3):
+            for sg in (0, 1):
+                x_f = hi[e][d] if sg else lo[e][d]
+                t1, t2 = (d + 1) % 3, (d + 2) % 3
+                # neighbours sharing this face plane and overlapping it
+                for k in range(n):
+                    if k == e:
+                        continue
+                    x_k = lo[k][d] if sg else hi[k][d]
+                    if abs(x_k - x_f) > tol * max(1.0, abs(x_f)):
+                        continue
+                    a1, b1 = max(lo[e][t1], lo[k][t1]), min(hi[e][t1], hi[k][t1])
+                    a2, b2 = max(lo[e][t2], lo[k][t2]), min(hi[e][t2], hi[k][t2])
+                    if b1 - a1 <= tol or b2 - a2 <= tol:
+                        continue
+                    # integrate each overlap piece once: from the smaller side,
+                    # same-size faces from the lower cell
+                    small_e = h[e][t1] * h[e][t2] < h[k][t1] * h[k][t2] - tol
+                    same = abs(h[e][t1] * h[e][t2] - h[k][t1] * h[k][t2]) <= tol
+                    if not (small_e or (same and sg == 1)):
+                        continue
+                    s = 0.0
+                    for q1 in g:
+                        for q2 in g:
+                            x = np.zeros(3)
+                            x[d] = x_f
+                            x[t1] = a1 + q1 * (b1 - a1)
+                            x[t2] = a2 + q2 * (b2 - a2)
+                            jmp = grad(e, x)[d] - grad(k, x)[d]
+                            s += 0.25 * (b1 - a1) * (b2 - a2) * float(jmp @ jmp)
+                    e2[e] += np.linalg.norm(h[e]) / 24.0 * s
+                    e2[k] += np.linalg.norm(h[k]) / 24.0 * s
+    return np.sqrt(e2)

@@ -34,8 +34,8 @@ def oracle_mesh(mesh):
 
 
 def cfg_mesh(G, name):
-    """Meshes of the benchmark configurations and small
-    stand-ins used by the parity tests."""
+    """Meshes of the benchmark workloads (8^3 Q1 cube, 32^3 Q2 cube) and
+    small stand-ins used by the parity tests."""
     Z = [1, 1, 1, 1, 0, 0]  # A = 0 on the two z faces, natural elsewhere
     if name == "cube_q1_ref3":      # [-10,10]^3, Q1, 8^3 cells
         return G.Mesh(1, (8, 8, 8), (-10, -10, -10), (10, 10, 10), Z)
@@ -52,3 +52,40 @@ def initial_state(mesh_arrays_n_nodes, p, t, amp, seed, phase=0):
     m = O.matep(p, t)
     gap = m["gap_B"] if phase == 0 else m["gap_A"]
     return O.initial_guess(mesh_arrays_n_nodes, gap, phase=phase, amp=amp, seed=seed)
+
+
+def side_masks(side_id):
+    """18-bit Dirichlet component masks of the six box faces (femgl.h:294-301):
+    id 0 all components, 2/3/4 the x/y/z-normal orbital column."""
+    out = []
+    for s in side_id:
+        if s == 0:
+            out.append((1 << 18) - 1)
+        elif s in (2, 3, 4):
+            out.append(sum(1 << c for c in range(18) if (c % 9) % 3 == s - 2))
+        else:
+            out.append(0)
+    return out
+
+
+def adaptive_forest(G, n, lo, hi, side_id, steps, seed=0, frac=0.25):
+    """Forest refined `steps` times, each time a seeded random fraction of its
+    cells (with 2:1 balance applied by the library)."""
+    rng = np.random.default_rng(seed)
+    f = G.Forest(n, lo, hi, side_id)
+    for _ in range(steps):
+        fl = (rng.random(f.n_cells) < frac).astype(np.uint8)
+        f = f.refine(fl)
+    return f
+
+
+def oracle_adaptive(mesh, lo, hi, side_id):
+    """glhip adaptive Mesh -> oracle_lib.Mesh with independently derived
+    hanging-node and Dirichlet constraints (deal.II order and closure)."""
+    a = mesh.arrays()
+    xyz = a["node_xyz"].reshape(-1, 3)
+    cn = a["cell_nodes"].reshape(-1, 8)
+    cell_lo = xyz[cn[:, 0]]
+    faces = a["faces"].reshape(-1, 3) if mesh.n_faces else np.zeros((0, 3), np.int32)
+    return O.adaptive_oracle_mesh(a["cell_nodes"], xyz, cell_lo, a["cell_h"], [tuple(f) for f in faces], lo, hi,
+                                  side_masks(side_id))

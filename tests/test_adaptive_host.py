@@ -1,0 +1,144 @@
+"""Adaptive Q1 meshes on the host (no GPU): forest refinement with 2:1
+balance, hanging-node constraint lines against an independent derivation,
+the condensed pattern, solution transfer, Kelly indicators and the
+fixed-number refinement flags."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from product_util import adaptive_forest, glhip, oracle_adaptive, side_masks
+
+G = glhip()
+LO, HI = (-20.0, -20.0, -6.0), (20.0, 20.0, 6.0)
+SIDES = [1, 1, 1, 1, 0, 0]
+
+
+def _continuous(mesh, U):
+    """Hanging values from their masters (a conforming Q1 field)."""
+    c = mesh.constraints()
+    U = np.array(U, dtype=np.float64).reshape(-1, 18)
+    hp = c["hang_ptr"]
+    for v in range(mesh.n_nodes):
+        if hp[v + 1] > hp[v]:
+            U[v] = sum(c["hang_w"][k] * U[c["hang_m"][k]] for k in range(hp[v], hp[v + 1]))
+    return U.reshape(-1)
+
+
+def _geometry(mesh):
+    a = mesh.arrays()
+    xyz = a["node_xyz"].reshape(-1, 3)
+    cn = a["cell_nodes"].reshape(-1, 8)
+    h = a["cell_h"].reshape(-1, 3)
+    return a, xyz, cn, xyz[cn[:, 0]], h
+
+
+@pytest.mark.parametrize("steps,seed", [(1, 0), (2, 1), (3, 2)])
+def test_forest_is_balanced_and_tiles_the_box(steps, seed):
+    f = adaptive_forest(G, (4, 4, 2), LO, HI, SIDES, steps, seed)
+    m = f.mesh()
+    a, xyz, cn, lo, h = _geometry(m)
+    vol = np.prod(h, 1)
+    assert vol.sum() == pytest.approx(np.prod(np.subtract(HI, LO)), rel=1e-12)
+    # 2:1 balance: cells whose closures touch differ by at most one level
+    lvl = np.round(np.log2((HI[0] - LO[0]) / 4 / h[:, 0])).astype(int)
+    hi = lo + h
+    for e in range(m.n_cells):
+        touch = np.all((lo <= hi[e] + 1e-9) & (hi >= lo[e] - 1e-9), 1)
+        assert np.abs(lvl[touch] - lvl[e]).max() <= 1
+
+
+@pytest.mark.parametrize("steps,seed", [(1, 0), (2, 1), (3, 2)])
+def test_hanging_lines_match_independent_derivation(steps, seed):
+    f = adaptive_forest(G, (4, 4, 2), LO, HI, SIDES, steps, seed)
+    m = f.mesh()
+    a, xyz, cn, lo, h = _geometry(m)
+    ref = O.hanging_lines(a["cell_nodes"], xyz, lo, h)
+    c = m.constraints()
+    hp = c["hang_ptr"]
+    got = {v: {int(c["hang_m"][k]): float(c["hang_w"][k]) for k in range(hp[v], hp[v + 1])}
+           for v in range(m.n_nodes) if hp[v + 1] > hp[v]}
+    assert set(got) == set(ref) and len(ref) == c["n_hanging"] > 0
+    for v in ref:
+        assert set(got[v]) == set(ref[v])
+        for mm in ref[v]:
+            assert got[v][mm] == pytest.approx(ref[v][mm], abs=1e-15)
+        assert sum(got[v].values()) == pytest.approx(1.0, abs=1e-15)  # partition of unity
+    # hanging nodes carry the flag bit and no Dirichlet bits
+    mask = a["node_mask"]
+    hang = np.array(sorted(ref))
+    assert np.all(mask[hang] == np.uint32(1 << 31))
+
+
+def test_condensed_pattern_holds_every_oracle_entry():
+    f = adaptive_forest(G, (4, 4, 2), LO, HI, SIDES, 2, 3)
+    m = f.mesh()
+    a = m.arrays()
+    om = oracle_adaptive(m, LO, HI, SIDES)
+    prod = set()
+    for r in range(m.n_nodes):
+        for k in range(a["row_ptr"][r], a["row_ptr"][r + 1]):
+            prod.add((r, int(a["col"][k])))
+    for r in range(om.n_nodes):
+        for k in range(om.row_ptr[r], om.row_ptr[r + 1]):
+            assert (r, int(om.col[k])) in prod
+    # hanging rows and columns hold only the diagonal block
+    hp = m.constraints()["hang_ptr"]
+    hang = set(np.nonzero(np.diff(hp))[0].tolist())
+    for (r, s) in prod:
+        if r in hang or s in hang:
+            assert r == s
+
+
+def test_transfer_reproduces_linear_fields_and_interpolates():
+    f = adaptive_forest(G, (3, 3, 2), LO, HI, SIDES, 1, 4)
+    m = f.mesh()
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((3, 18))
+    b = rng.standard_normal(18)
+    xyz = m.arrays()["node_xyz"].reshape(-1, 3)
+    U = (xyz @ A + b).reshape(-1)
+    f2 = f.refine((rng.random(f.n_cells) < 0.4).astype(np.uint8))
+    m2 = f2.mesh()
+    U2 = f.transfer(m, U, f2, m2)
+    xyz2 = m2.arrays()["node_xyz"].reshape(-1, 3)
+    assert np.abs(U2 - (xyz2 @ A + b).reshape(-1)).max() < 1e-12
+    # a general conforming field: new nodes take the trilinear interpolant of
+    # the old cell that contains them
+    U = _continuous(m, rng.standard_normal(m.n_nodes * 18))
+    U2 = f.transfer(m, U, f2, m2).reshape(-1, 18)
+    a1, _, cn, lo, h = _geometry(m)
+    for v in rng.choice(m2.n_nodes, 40, replace=False):
+        x = xyz2[v]
+        t = (x - lo) / h
+        e = int(np.nonzero(np.all((t > -1e-12) & (t < 1 + 1e-12), 1))[0][0])
+        tv = np.clip(t[e], 0, 1)
+        val = np.zeros(18)
+        for k in range(8):
+            bits = (k & 1, (k >> 1) & 1, k >> 2)
+            w = np.prod([tv[d] if bits[d] else 1 - tv[d] for d in range(3)])
+            val += w * U.reshape(-1, 18)[cn[e][k]]
+        assert np.abs(U2[v] - val).max() < 1e-12
+
+
+def test_kelly_matches_independent_restatement():
+    f = adaptive_forest(G, (3, 3, 2), LO, HI, SIDES, 2, 5, frac=0.3)
+    m = f.mesh()
+    a, xyz, cn, lo, h = _geometry(m)
+    rng = np.random.default_rng(1)
+    U = _continuous(m, rng.standard_normal(m.n_nodes * 18))
+    eta = f.kelly(m, U)
+    ref = O.kelly(a["cell_nodes"], lo, h, U)
+    assert eta.dtype == np.float32
+    assert np.abs(eta - ref).max() <= 1e-6 * ref.max()
+    # a globally linear field has no jumps
+    U_lin = (xyz @ rng.standard_normal((3, 18))).reshape(-1)
+    assert np.abs(f.kelly(m, U_lin)).max() < 1e-10
+
+
+def test_fixed_number_flags():
+    eta = np.array([0.5, 3.0, 1.0, 3.0, 0.1, 2.0, 0.0, 7.0, 1.5, 0.2], np.float32)
+    fl = G.flag_fixed_number(eta, 0.3)
+    assert fl.sum() == 3 and set(np.nonzero(fl)[0]) == {7, 1, 3}
+    assert G.flag_fixed_number(eta, 0.0).sum() == 0
+    with pytest.raises(G.GLHError):
+        G.flag_fixed_number(eta, 1.5)

@@ -1,0 +1,150 @@
+"""Adaptive meshes with hanging nodes on the GPU against the oracle: the
+condensed Jacobian and residual (deal.II distribute_local_to_global
+semantics, constraints derived independently on the test side), Newton to
+the oracle's fixed point, FP32 mixed precision, and one adaptive cycle
+(Kelly, refinement, transfer) that keeps the solution continuous."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from product_util import adaptive_forest, glhip, oracle_adaptive
+
+pytestmark = pytest.mark.gpu
+G = glhip()
+LO, HI = (-20.0, -20.0, -6.0), (20.0, 20.0, 6.0)
+SIDES = [1, 1, 1, 1, 0, 0]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = G.Context(0)
+    yield c
+    c.close()
+
+
+def _case(ctx, steps, seed, p=22.0, t=0.8, amp=0.05, sides=SIDES, n=(3, 3, 2), lo=LO, hi=HI):
+    f = adaptive_forest(G, n, lo, hi, sides, steps, seed)
+    m = f.mesh()
+    say("adaptive case: steps %d seed %d cells %d nodes %d hanging %d" % (steps, seed, m.n_cells, m.n_nodes,
+                                                                         m.constraints()["n_hanging"]))
+    ctx.set_material_pt(p, t)
+    ctx.upload_mesh(m)
+    om = oracle_adaptive(m, lo, hi, sides)
+    U = O.initial_guess(m.n_nodes, O.matep(p, t)["gap_B"], amp=amp, seed=seed)
+    Ud = O.distribute(om, U)
+    ctx.set_solution(U)
+    return f, m, om, Ud, O.coef(p, t)
+
+
+def _dense(row_ptr, col, val, n_nodes):
+    A = np.zeros((n_nodes * 18, n_nodes * 18))
+    v = val.reshape(-1, 18, 18)
+    for r in range(n_nodes):
+        for b in range(row_ptr[r], row_ptr[r + 1]):
+            s = col[b]
+            A[18 * r:18 * r + 18, 18 * s:18 * s + 18] = v[b]
+    return A
+
+
+@pytest.mark.parametrize("steps,seed,sides", [(1, 0, SIDES), (2, 1, SIDES), (2, 2, [1, 1, 1, 1, 4, 4])])
+def test_condensed_jacobian_and_rhs_match_oracle(ctx, steps, seed, sides):
+    f, m, om, Ud, co = _case(ctx, steps, seed, sides=sides)
+    assert np.abs(ctx.solution() - Ud).max() <= 1e-15 * np.abs(Ud).max()  # distribute
+    ctx.assemble()
+    Jo, ro = O.assemble(om, co, Ud)
+    rg = ctx.rhs()
+    assert np.abs(rg - ro).max() <= 1e-12 * np.abs(ro).max()
+    a = m.arrays()
+    Ag = _dense(a["row_ptr"], a["col"], ctx.jacobian(), m.n_nodes)
+    Ao = _dense(om.row_ptr, om.col, Jo, om.n_nodes)
+    err = np.abs(Ag - Ao).max() / np.abs(Ao).max()
+    say("J rel err", err)
+    assert err <= 1e-12
+
+
+def test_newton_on_adaptive_mesh_matches_oracle(ctx):
+    f, m, om, Ud, co = _case(ctx, 2, 3)
+    st = ctx.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-3, restart=50)
+    say("gpu newton", st)
+    assert st["converged"] == 1.0
+    Ug = ctx.solution()
+    Uo, rc, ost = O.newton(om, co, Ud, rtol=1e-10, max_newton=60, lin_tol=1e-3, restart=50)
+    say("oracle newton", ost)
+    assert rc == 0
+    Fo = O.free_energy(om, co, Uo)
+    assert st["F"] == pytest.approx(Fo, rel=1e-10)
+    assert np.abs(Ug - Uo).max() <= 1e-8 * np.abs(Uo).max()
+    # the converged field is continuous: hanging values follow their lines
+    assert np.abs(O.distribute(om, Ug) - Ug).max() <= 1e-15 * np.abs(Ug).max()
+
+
+def test_fp32_mixed_precision_on_adaptive_mesh(ctx):
+    f, m, om, Ud, co = _case(ctx, 2, 4)
+    st64 = ctx.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=50)
+    F64 = st64["F"]
+    ctx.set_solution(Ud)
+    st32 = ctx.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=50, fp32=True)
+    say(st64, st32)
+    assert st32["converged"] == 1.0
+    assert st32["F"] == pytest.approx(F64, rel=1e-10)
+
+
+def test_adaptive_cycle_refines_and_transfers(ctx):
+    # one cycle of the reference's run(): converge, Kelly, refine 30%,
+    # transfer, converge again, on a mesh fine enough (h <= 3 xi0) for the
+    # transferred state to be a good guess; the oracle repeats it on the CPU
+    lo, hi = (-5.0, -5.0, -6.0), (5.0, 5.0, 6.0)
+    f, m, om, Ud, co = _case(ctx, 1, 5, n=(4, 4, 4), lo=lo, hi=hi)
+    st = ctx.newton_solve(rtol=1e-8, max_newton=60, lin_tol=1e-2, restart=50)
+    assert st["converged"] == 1.0
+    U = ctx.solution()
+    eta = f.kelly(m, U)
+    fl = G.flag_fixed_number(eta, 0.3)
+    f2 = f.refine(fl)
+    m2 = f2.mesh()
+    U2 = f.transfer(m, U, f2, m2)
+    ctx.upload_mesh(m2)
+    ctx.set_solution(U2)
+    # the transferred field is already continuous on the new mesh
+    assert np.abs(ctx.solution() - U2).max() <= 1e-13 * np.abs(U2).max()
+    st2 = ctx.newton_solve(rtol=1e-8, max_newton=60, lin_tol=1e-2, restart=50)
+    say("cycle 1", st2, "cells", m2.n_cells, "hanging", m2.constraints()["n_hanging"])
+    assert st2["converged"] == 1.0
+    om2 = oracle_adaptive(m2, lo, hi, SIDES)
+    Uo2, rc, ost = O.newton(om2, co, O.distribute(om2, U2), rtol=1e-8, max_newton=60, lin_tol=1e-2, restart=50)
+    assert rc == 0
+    # same stationary state, reached from the same transferred guess
+    assert st2["F"] == pytest.approx(O.free_energy(om2, co, Uo2), rel=1e-10)
+
+
+def test_run_adaptive_driver_fp32_against_fp64(ctx):
+    # the cycle loop of FemGL::run in the library: BnA guess (seed 1),
+    # two refinement cycles, FP32 mixed precision checked against FP64 per cycle
+    lo, hi = (-5.0, -5.0, -6.0), (5.0, 5.0, 6.0)
+    p, t = 22.0, 0.8
+    f = G.Forest((4, 4, 4), lo, hi, SIDES)
+    m = f.mesh()
+    mp = O.matep(p, t)
+    U0 = G.initial_guess_bna(m.arrays()["node_xyz"], 0.0, mp["gap_A"], mp["gap_B"], amp=0.05, seed=1)
+    ctx.set_material_pt(p, t)
+    rows, f2, m2 = ctx.run_adaptive(f, U0, n_refine=2, refine_frac=0.3, rtol=1e-9, lin_tol=1e-2, restart=50,
+                                    fp32=True, check_fp64=True)
+    for r in rows:
+        say(r)
+        assert r["converged"] == 1.0 and r["converged64"] == 1.0
+        assert r["F_rel_err_vs_fp64"] <= 1e-10
+    assert rows[1]["hanging"] > 0 and rows[2]["cells"] > rows[1]["cells"] > rows[0]["cells"] == 64
+    # the last field is a fixed point of the oracle's condensed system
+    om = oracle_adaptive(m2, lo, hi, SIDES)
+    co = O.coef(p, t)
+    U = ctx.solution()
+    _, ro = O.assemble(om, co, U, J=False)
+    assert np.linalg.norm(ro) <= 1.01e-9 * rows[2]["r0"]
+    assert rows[2]["F"] == pytest.approx(O.free_energy(om, co, U), rel=1e-12)

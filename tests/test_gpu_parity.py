@@ -1,6 +1,6 @@
 """GPU parity of the HIP path against the FP64 CPU oracle, through the C ABI.
 
-Tolerances (BASELINE north star): assembled Jacobian and residual 1e-12
+Tolerances: assembled Jacobian and residual 1e-12
 relative (normwise, max|diff| / max|ref|); converged free energy 1e-10
 relative; converged order parameter 1e-8 relative.  FP32 Jacobian: 2e-6.
 """

@@ -15,7 +15,7 @@ G = glhip()
 
 def test_library_exports_every_declared_symbol():
     hdr = open(os.path.join(ROOT, "include", "glhip.h")).read()
-    declared = set(re.findall(r"^\s*(?:int|const char \*)\s*(glh_\w+)\(", hdr, re.M))
+    declared = set(re.findall(r"^\s*(?:int|int64_t|const char \*)\s*(glh_\w+)\(", hdr, re.M))
     assert declared, "no declarations found"
     assert declared == set(G.EXPORTS)
     L = G.lib()
@@ -79,7 +79,7 @@ def test_boundary_ids_and_masks():
     assert np.all(a["node_mask"][on] == zmask) and np.all(a["node_mask"][~on] == 0)
     f = a["faces"].reshape(-1, 3)
     assert len(f) == 8 and set(f[:, 2]) == {4} and set(f[:, 1]) == {4, 5}
-    # cfg sizes: Q1 ref 3 -> 729 nodes, Q2 ref 5 -> 274625 nodes
+    # workload sizes: Q1 ref 3 -> 729 nodes, Q2 ref 5 -> 274625 nodes
     m1 = cfg_mesh(G, "cube_q1_ref3")
     assert m1.n_nodes * 18 == 13122 and m1.nnzb * 324 == 5_062_500
 
@@ -98,3 +98,17 @@ def test_initial_guess_bitwise_equal_to_oracle(phase):
         a = G.initial_guess(97, gap, phase=phase, amp=0.05, seed=seed)
         b = O.initial_guess(97, gap, phase=phase, amp=0.05, seed=seed)
         assert np.array_equal(a, b)
+
+
+def test_bna_guess_bitwise_equal_to_oracle_and_phases():
+    m = O.matep(22.0, 0.8)
+    xyz = np.random.default_rng(3).uniform(-6, 6, (211, 3))
+    a = G.initial_guess_bna(xyz, 0.0, m["gap_A"], m["gap_B"], amp=0.05, seed=1)
+    b = O.initial_guess_bna(xyz, 0.0, m["gap_A"], m["gap_B"], amp=0.05, seed=1)
+    assert np.array_equal(a, b)
+    # without noise: B phase above the wall, A phase below (femgl/inc/BnA.h)
+    u = G.initial_guess_bna(xyz, 0.0, m["gap_A"], m["gap_B"], amp=0.0).reshape(-1, 18)
+    up = xyz[:, 2] >= 0
+    assert np.allclose(u[up][:, [0, 4, 8]], m["gap_B"] / np.sqrt(3), rtol=1e-7)
+    assert np.allclose(u[~up][:, [0, 10]], m["gap_A"] / np.sqrt(2), rtol=1e-7)
+    assert np.count_nonzero(u[up]) == 3 * up.sum() and np.count_nonzero(u[~up]) == 2 * (~up).sum()

@@ -1,0 +1,305 @@
+#include "amr.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+
+namespace glh {
+
+namespace {
+
+// Morton code of a lattice point (21 bits per coordinate)
+uint64_t morton(uint64_t x, uint64_t y, uint64_t z)
+{
+  auto spread = [](uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+  };
+  return spread(x) | spread(y) << 1 | spread(z) << 2;
+}
+
+uint64_t vkey(int64_t x, int64_t y, int64_t z) { return ((uint64_t)x << 42) | ((uint64_t)y << 21) | (uint64_t)z; }
+
+}  // namespace
+
+int Forest::locate(int l, int i, int j, int k) const
+{
+  for (int lv = l; lv >= 0; --lv) {
+    const int s = l - lv;
+    auto it = where.find(key(lv, i >> s, j >> s, k >> s));
+    if (it != where.end()) return it->second;
+  }
+  return -1;
+}
+
+void Forest::reindex()
+{
+  std::vector<std::pair<uint64_t, int>> ord(leaves.size());
+  for (size_t e = 0; e < leaves.size(); ++e) {
+    const Leaf &L = leaves[e];
+    const int s = LMAX - L.level;
+    // lower corner on the finest lattice; ties (same corner) ordered by level
+    ord[e] = {morton((uint64_t)L.ix << s, (uint64_t)L.iy << s, (uint64_t)L.iz << s) * 16 + (uint64_t)L.level, (int)e};
+  }
+  std::sort(ord.begin(), ord.end());
+  std::vector<Leaf> nl(leaves.size());
+  for (size_t e = 0; e < ord.size(); ++e) nl[e] = leaves[ord[e].second];
+  leaves.swap(nl);
+  where.clear();
+  where.reserve(leaves.size() * 2);
+  for (size_t e = 0; e < leaves.size(); ++e) where[key(leaves[e].level, leaves[e].ix, leaves[e].iy, leaves[e].iz)] = (int)e;
+}
+
+int forest_init(Forest &f, int nx, int ny, int nz, const double lo[3], const double hi[3], const int side_id[6],
+                std::string &err)
+{
+  if (nx < 1 || ny < 1 || nz < 1) { err = "cell counts must be positive"; return -1; }
+  if (((int64_t)std::max(nx, std::max(ny, nz)) << Forest::LMAX) >= (1 << 19)) {
+    err = "too many root cells for the refinement lattice";
+    return -1;
+  }
+  for (int k = 0; k < 3; ++k)
+    if (!(hi[k] > lo[k])) { err = "empty box"; return -1; }
+  f = Forest();
+  f.nx = nx; f.ny = ny; f.nz = nz;
+  for (int k = 0; k < 3; ++k) { f.lo[k] = lo[k]; f.hi[k] = hi[k]; }
+  for (int k = 0; k < 6; ++k) f.side_id[k] = side_id[k];
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) f.leaves.push_back({0, i, j, k});
+  f.reindex();
+  return 0;
+}
+
+int forest_refine(Forest &f, const std::vector<char> &flag, std::string &err)
+{
+  if (flag.size() != f.leaves.size()) { err = "one refinement flag per cell expected"; return -1; }
+  std::vector<char> fl(flag);
+  while (true) {
+    bool any = false;
+    std::vector<Forest::Leaf> nl;
+    nl.reserve(f.leaves.size() + 8 * std::count(fl.begin(), fl.end(), 1));
+    for (size_t e = 0; e < f.leaves.size(); ++e) {
+      const Forest::Leaf L = f.leaves[e];
+      if (!fl[e]) { nl.push_back(L); continue; }
+      if (L.level >= Forest::LMAX) { err = "refinement deeper than the lattice allows"; return -1; }
+      any = true;
+      for (int c = 0; c < 8; ++c)
+        nl.push_back({L.level + 1, 2 * L.ix + (c & 1), 2 * L.iy + ((c >> 1) & 1), 2 * L.iz + (c >> 2)});
+    }
+    if (!any) break;
+    f.leaves.swap(nl);
+    f.reindex();
+    // 2:1 balance: a neighbour (26 directions) more than one level coarser
+    // than a leaf is refined in the next pass
+    fl.assign(f.leaves.size(), 0);
+    for (const auto &L : f.leaves) {
+      if (L.level < 2) continue;
+      const int n[3] = {f.nx << L.level, f.ny << L.level, f.nz << L.level};
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            if (!dx && !dy && !dz) continue;
+            const int i = L.ix + dx, j = L.iy + dy, k = L.iz + dz;
+            if (i < 0 || j < 0 || k < 0 || i >= n[0] || j >= n[1] || k >= n[2]) continue;
+            const int K = f.locate(L.level, i, j, k);
+            if (K >= 0 && f.leaves[K].level < L.level - 1) fl[K] = 1;
+          }
+    }
+  }
+  return 0;
+}
+
+int forest_mesh(const Forest &f, Mesh &m, std::string &err)
+{
+  m = Mesh();
+  m.deg = 1; m.n1 = 2; m.ns = 8;
+  const int LM = Forest::LMAX;
+  const int64_t NX = (int64_t)f.nx << LM, NY = (int64_t)f.ny << LM, NZ = (int64_t)f.nz << LM;
+  const double dl[3] = {(f.hi[0] - f.lo[0]) / (double)NX, (f.hi[1] - f.lo[1]) / (double)NY,
+                        (f.hi[2] - f.lo[2]) / (double)NZ};
+  const int nc = (int)f.leaves.size();
+  if ((int64_t)nc * 8 >= (1LL << 31)) { err = "mesh too large for 32-bit indices"; return -1; }
+  m.n_cells = nc;
+  // vertices, numbered in the (Morton) order of first appearance
+  std::unordered_map<uint64_t, int> vid;
+  vid.reserve((size_t)nc * 2);
+  std::vector<int64_t> vx;
+  m.cell_nodes.resize((size_t)nc * 8);
+  m.cell_h.resize((size_t)nc * 3);
+  m.cell_lo.resize((size_t)nc * 3);
+  m.cell_level.resize(nc);
+  for (int e = 0; e < nc; ++e) {
+    const auto &L = f.leaves[e];
+    const int64_t s = (int64_t)1 << (LM - L.level);
+    const int64_t x0 = L.ix * s, y0 = L.iy * s, z0 = L.iz * s;
+    for (int a = 0; a < 8; ++a) {
+      const int64_t x = x0 + (a & 1) * s, y = y0 + ((a >> 1) & 1) * s, z = z0 + (a >> 2) * s;
+      auto it = vid.find(vkey(x, y, z));
+      int v;
+      if (it == vid.end()) {
+        v = (int)(vx.size() / 3);
+        vid.emplace(vkey(x, y, z), v);
+        vx.push_back(x); vx.push_back(y); vx.push_back(z);
+      } else {
+        v = it->second;
+      }
+      m.cell_nodes[(size_t)e * 8 + a] = v;
+    }
+    for (int d = 0; d < 3; ++d) m.cell_h[3 * e + d] = (double)s * dl[d];
+    m.cell_lo[3 * e + 0] = f.lo[0] + (double)x0 * dl[0];
+    m.cell_lo[3 * e + 1] = f.lo[1] + (double)y0 * dl[1];
+    m.cell_lo[3 * e + 2] = f.lo[2] + (double)z0 * dl[2];
+    m.cell_level[e] = L.level;
+  }
+  const int nn = (int)(vx.size() / 3);
+  if ((int64_t)nn * NC >= (1LL << 31)) { err = "mesh too large for 32-bit indices"; return -1; }
+  m.n_nodes = nn;
+  m.node_xyz.resize((size_t)nn * 3);
+  for (int v = 0; v < nn; ++v)
+    for (int d = 0; d < 3; ++d) m.node_xyz[3 * v + d] = f.lo[d] + (double)vx[3 * v + d] * dl[d];
+  // hanging vertices: a vertex that lies inside an edge or a face of a leaf
+  // touching it follows the trilinear interpolant of that leaf there
+  // (weights 1/2 on an edge, 1/4 on a face under 2:1 balance)
+  std::vector<std::vector<std::pair<int, double>>> line(nn);
+  std::vector<char> hanging(nn, 0);
+  for (int v = 0; v < nn; ++v) {
+    const int64_t X = vx[3 * v], Y = vx[3 * v + 1], Z = vx[3 * v + 2];
+    for (int o = 0; o < 8 && !hanging[v]; ++o) {
+      const int64_t cx = X - (o & 1), cy = Y - ((o >> 1) & 1), cz = Z - (o >> 2);
+      if (cx < 0 || cy < 0 || cz < 0 || cx >= NX || cy >= NY || cz >= NZ) continue;
+      const int K = f.locate(LM, (int)cx, (int)cy, (int)cz);
+      if (K < 0) { err = "forest lookup failed"; return -1; }
+      const auto &L = f.leaves[K];
+      const int64_t s = (int64_t)1 << (LM - L.level);
+      const int64_t r[3] = {X - L.ix * s, Y - L.iy * s, Z - L.iz * s};
+      int inside = 0;
+      for (int d = 0; d < 3; ++d)
+        if (r[d] > 0 && r[d] < s) {
+          if (2 * r[d] != s) { err = "unbalanced forest (hanging vertex off the midpoint)"; return -1; }
+          ++inside;
+        }
+      if (inside == 0) continue;
+      if (inside == 3) { err = "vertex inside a leaf"; return -1; }
+      hanging[v] = 1;
+      const double w = inside == 1 ? 0.5 : 0.25;
+      const int nm = 1 << inside;
+      for (int t = 0; t < nm; ++t) {
+        int64_t q[3];
+        int bit = 0;
+        for (int d = 0; d < 3; ++d) {
+          if (r[d] > 0 && r[d] < s) q[d] = ((t >> bit++) & 1) ? s : 0;
+          else q[d] = r[d];
+        }
+        auto it = vid.find(vkey(L.ix * s + q[0], L.iy * s + q[1], L.iz * s + q[2]));
+        if (it == vid.end()) { err = "master vertex missing"; return -1; }
+        line[v].push_back({it->second, w});
+      }
+    }
+  }
+  // close the lines: masters that are hanging themselves are replaced by
+  // their own lines until no hanging master remains
+  for (int v = 0; v < nn; ++v) {
+    if (!hanging[v]) continue;
+    for (int pass = 0; pass < 32; ++pass) {
+      bool changed = false;
+      std::map<int, double> acc;
+      for (auto &mw : line[v]) {
+        if (hanging[mw.first]) {
+          for (auto &mw2 : line[mw.first]) acc[mw2.first] += mw.second * mw2.second;
+          changed = true;
+        } else {
+          acc[mw.first] += mw.second;
+        }
+      }
+      line[v].assign(acc.begin(), acc.end());
+      if (!changed) break;
+      if (pass == 31) { err = "cyclic hanging-node constraints"; return -1; }
+    }
+  }
+  m.hang_ptr.assign(nn + 1, 0);
+  for (int v = 0; v < nn; ++v) {
+    m.hang_ptr[v + 1] = m.hang_ptr[v] + (hanging[v] ? (int)line[v].size() : 0);
+    if (hanging[v])
+      for (auto &mw : line[v]) {
+        m.hang_m.push_back(mw.first);
+        m.hang_w.push_back(mw.second);
+      }
+  }
+  // boundary faces and Dirichlet masks of the non-hanging vertices
+  m.node_mask.assign(nn, 0u);
+  for (int v = 0; v < nn; ++v) {
+    if (hanging[v]) { m.node_mask[v] = HANG_BIT; continue; }
+    const bool on[6] = {vx[3 * v] == 0, vx[3 * v] == NX, vx[3 * v + 1] == 0, vx[3 * v + 1] == NY,
+                        vx[3 * v + 2] == 0, vx[3 * v + 2] == NZ};
+    uint32_t mk = 0;
+    for (int s = 0; s < 6; ++s)
+      if (on[s]) mk |= dirichlet_mask(f.side_id[s]);
+    m.node_mask[v] = mk;
+  }
+  for (int e = 0; e < nc; ++e) {
+    const auto &L = f.leaves[e];
+    const int64_t s = (int64_t)1 << (LM - L.level);
+    const bool on[6] = {L.ix == 0, (L.ix + 1) * s == NX, L.iy == 0, (L.iy + 1) * s == NY,
+                        L.iz == 0, (L.iz + 1) * s == NZ};
+    for (int lf = 0; lf < 6; ++lf)
+      if (on[lf] && f.side_id[lf] != 1) {
+        m.faces.push_back(e);
+        m.faces.push_back(lf);
+        m.faces.push_back(f.side_id[lf]);
+      }
+  }
+  greedy_colors(m);
+  if (m.has_hanging()) build_constrained_patterns(m);
+  else {
+    build_pattern(m);
+    m.hang_ptr.clear();
+  }
+  m.node_off = {0, m.n_nodes};
+  m.cell_off = {0, m.n_cells};
+  return 0;
+}
+
+int forest_transfer(const Forest &fo, const Mesh &mo, const double *Uo, const Forest &fn, const Mesh &mn,
+                    double *Un, std::string &err)
+{
+  const int LM = Forest::LMAX;
+  if (fo.nx != fn.nx || fo.ny != fn.ny || fo.nz != fn.nz) { err = "forests over different boxes"; return -1; }
+  if ((int)fo.leaves.size() != mo.n_cells || (int)fn.leaves.size() != mn.n_cells) {
+    err = "mesh does not belong to the forest";
+    return -1;
+  }
+  const int64_t NX = (int64_t)fn.nx << LM, NY = (int64_t)fn.ny << LM, NZ = (int64_t)fn.nz << LM;
+  const double dl[3] = {(fn.hi[0] - fn.lo[0]) / (double)NX, (fn.hi[1] - fn.lo[1]) / (double)NY,
+                        (fn.hi[2] - fn.lo[2]) / (double)NZ};
+  for (int v = 0; v < mn.n_nodes; ++v) {
+    // lattice coordinates of the vertex
+    int64_t P[3];
+    for (int d = 0; d < 3; ++d) P[d] = (int64_t)std::llround((mn.node_xyz[3 * v + d] - fn.lo[d]) / dl[d]);
+    const int64_t cx = std::min(P[0], NX - 1), cy = std::min(P[1], NY - 1), cz = std::min(P[2], NZ - 1);
+    const int K = fo.locate(LM, (int)cx, (int)cy, (int)cz);
+    if (K < 0) { err = "vertex outside the old forest"; return -1; }
+    const auto &L = fo.leaves[K];
+    const int64_t s = (int64_t)1 << (LM - L.level);
+    const double t[3] = {(double)(P[0] - L.ix * s) / (double)s, (double)(P[1] - L.iy * s) / (double)s,
+                         (double)(P[2] - L.iz * s) / (double)s};
+    const int *cn = &mo.cell_nodes[(size_t)K * 8];
+    for (int c = 0; c < NC; ++c) {
+      double u = 0.0;
+      for (int a = 0; a < 8; ++a) {
+        const double w = ((a & 1) ? t[0] : 1 - t[0]) * (((a >> 1) & 1) ? t[1] : 1 - t[1]) *
+                         ((a >> 2) ? t[2] : 1 - t[2]);
+        if (w != 0.0) u += w * Uo[(size_t)cn[a] * NC + c];
+      }
+      Un[(size_t)v * NC + c] = u;
+    }
+  }
+  return 0;
+}
+
+}  // namespace glh

@@ -295,6 +295,8 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
         e2 += k23_cd * sp[kd * 3 + kc];
       }
       const uint32_t ma = mask[cn[a]], mb = mask[cn[b]];
+      // hanging node: its condensed row keeps sum over cells |local diagonal|
+      if (a == b && c == d && (ma & HANG_BIT) && Mh.hdiag) Mh.hdiag[(size_t)cn[a] * 18 + c] += fabs(e1);
       {  // block (a,b): rows of node a, columns of node b
         TJ *B = blk_ptr(cblk[a * NS + b]);
         double v1 = e1, v2 = e2;

@@ -21,13 +21,12 @@
 
 #include "glhip.h"
 #include "glhip_dev.hpp"
+#include "amr.hpp"
+#include "handles.hpp"
 #include "mesh.hpp"
 
 using glh::NC;
 
-struct glh_mesh {
-  glh::Mesh m;
-};
 
 namespace {
 
@@ -97,6 +96,17 @@ struct glh_ctx {
   DBuf<int> tpa, tpb;
   // mesh arrays
   DBuf<int> cell_nodes, row_ptr, col, diag, cell_blk, color_cells, cell_mat, seg_chunk;
+  // hanging-node constraints: the cell loop assembles Jf (nnzb_a blocks over
+  // the unconstrained cell pattern) and rf, which are then condensed into the
+  // system matrix J and rhs
+  bool constrained = false;
+  int nnzb_a = 0, n_hang = 0;
+  glh::ConstraintDev cons;
+  DBuf<int> c_gptr, c_gsrc, c_brow, c_rptr, c_rsrc, c_hnode, c_hptr, c_hm;
+  DBuf<double> c_gw, c_rw, c_hw, hdiag, rfull;
+  DBuf<unsigned char> c_gfl, c_rfl;
+  DBuf<double> Jf64;
+  DBuf<float> Jf32;
   DBuf<long long> chunks;
   DBuf<glh::Material> mats;
   DBuf<double> cell_h;
@@ -153,7 +163,8 @@ struct glh_ctx {
   {
     const bool per = !mats_host.empty();
     return glh::MeshDev{per ? cell_mat.p : nullptr, per ? mats.p : nullptr, cell_nodes.p, cell_h.p,
-                        cell_face.p,          node_mask.p,                  cell_blk.p};
+                        cell_face.p,          node_mask.p,                  cell_blk.p,
+                        constrained ? hdiag.p : nullptr};
   }
   glh::SegInfo seg() const { return glh::SegInfo{chunks.p, seg_chunk.p, n_chunks, n_seg}; }
 };
@@ -328,7 +339,19 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
     if (fp32) { HIPCHK(c, c->J32.alloc(nv)); J = c->J32.p; }
     else { HIPCHK(c, c->J64.alloc(nv)); J = c->J64.p; }
   }
-  if (do_r) HIPCHK(c, hipMemsetAsync(rhs_out, 0, ndofs(c) * sizeof(double), c->stream));
+  // with hanging nodes the cell loop writes Jf / rf, condensed afterwards
+  void *Jcell = J;
+  double *rcell = rhs_out;
+  if (c->constrained) {
+    if (do_j) {
+      const size_t nv = (size_t)c->nnzb_a * 324;
+      if (fp32) { HIPCHK(c, c->Jf32.alloc(nv)); Jcell = c->Jf32.p; }
+      else { HIPCHK(c, c->Jf64.alloc(nv)); Jcell = c->Jf64.p; }
+      HIPCHK(c, glh::launch_fill<double>(ndofs(c), 0.0, c->hdiag.p, c->stream));
+    }
+    if (do_r) rcell = c->rfull.p;
+  }
+  if (do_r) HIPCHK(c, hipMemsetAsync(rcell, 0, ndofs(c) * sizeof(double), c->stream));
   const glh::RefTables T = c->tables();
   const glh::MeshDev M = c->meshdev();
   timed = timed || c->timing;
@@ -336,8 +359,15 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
   const int ncol = (int)c->color_ptr.size() - 1;
   for (int k = 0; k < ncol; ++k) {
     const int off = c->color_ptr[k], n = c->color_ptr[k + 1] - off;
-    HIPCHK(c, glh::launch_assemble(c->deg, do_j, do_r, fp32, T, M, c->mat, Ustate, c->color_cells.p + off, n, J,
-                                   rhs_out, c->stream));
+    HIPCHK(c, glh::launch_assemble(c->deg, do_j, do_r, fp32, T, M, c->mat, Ustate, c->color_cells.p + off, n,
+                                   Jcell, rcell, c->stream));
+  }
+  if (c->constrained) {
+    if (do_j) {
+      if (fp32) HIPCHK(c, glh::launch_condense<float>(c->cons, c->Jf32.p, c->J32.p, c->hdiag.p, c->stream));
+      else HIPCHK(c, glh::launch_condense<double>(c->cons, c->Jf64.p, c->J64.p, c->hdiag.p, c->stream));
+    }
+    if (do_r) HIPCHK(c, glh::launch_condense_rhs(c->cons, c->rfull.p, rhs_out, c->stream));
   }
   if (timed) {
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -356,6 +386,15 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
     c->j_valid = true;
     c->d_valid = false;
   }
+  return GLH_OK;
+}
+
+// constraints distribute: Dirichlet components to 0, hanging values from
+// their masters (AffineConstraints::distribute)
+int distribute(glh_ctx *c, double *v)
+{
+  HIPCHK(c, glh::launch_mask_zero(c->n_nodes, c->node_mask.p, v, c->stream));
+  if (c->constrained) HIPCHK(c, glh::launch_distribute(c->cons, v, c->stream));
   return GLH_OK;
 }
 
@@ -750,8 +789,8 @@ int solve_impl(glh_ctx *c, double rtol, int maxit, int restart, double stats[2],
   }
   if (its_seg_out) *its_seg_out = its_seg;
   if (rc) return rc;
-  // constraints_newton_update.distribute (solve.cc:181): Dirichlet components to zero
-  HIPCHK(c, glh::launch_mask_zero(c->n_nodes, c->node_mask.p, c->dU.p, s));
+  // constraints_newton_update.distribute (solve.cc:181)
+  RCHK(distribute(c, c->dU.p));
   return GLH_OK;
 }
 
@@ -783,6 +822,7 @@ int newton_step_impl(glh_ctx *c, double step, int damped, const std::vector<char
     std::copy(alpha.begin(), alpha.end(), c->pinv.p);
     HIPCHK(c, hipMemcpyAsync(c->s_alpha.p, c->pinv.p, S * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, glh::launch_seg_trial(c->seg(), c->U.p, c->dU.p, c->s_alpha.p, c->node_mask.p, c->X.p, c->stream));
+    if (c->constrained) HIPCHK(c, glh::launch_distribute(c->cons, c->X.p, c->stream));  // iteration.cc:180
     RCHK(assemble_at(c, c->X.p, false, true, false, c->res.p, false));
     RCHK(seg_norm2(c, c->res.p, c2));
     bool all = true;
@@ -880,6 +920,13 @@ int glh_destroy(glh_ctx *c)
                      &c->gmjlast, &c->row_seg,    &c->d_node_off, &c->act_counts, &c->act_seg, &c->act_rpref,
                      &c->act_chunk};
   for (auto *b : gi) b->release();
+  DBuf<int> *ci[] = {&c->c_gptr, &c->c_gsrc, &c->c_brow, &c->c_rptr, &c->c_rsrc, &c->c_hnode, &c->c_hptr, &c->c_hm};
+  for (auto *b : ci) b->release();
+  DBuf<double> *cd[] = {&c->c_gw, &c->c_rw, &c->c_hw, &c->hdiag, &c->rfull, &c->Jf64};
+  for (auto *b : cd) b->release();
+  c->c_gfl.release();
+  c->c_rfl.release();
+  c->Jf32.release();
   c->chunks.release();
   c->mats.release();
   c->cell_face.release();
@@ -990,6 +1037,113 @@ int glh_mesh_get(const glh_mesh *mm, int *cell_nodes, double *cell_h, double *no
   return GLH_OK;
 }
 
+int glh_forest_create(int nx, int ny, int nz, const double lo[3], const double hi[3], const int side_id[6],
+                      glh_forest **out)
+{
+  if (!lo || !hi || !side_id || !out) return GLH_ERR_ARG;
+  *out = nullptr;
+  auto *f = new glh_forest();
+  std::string err;
+  if (glh::forest_init(f->f, nx, ny, nz, lo, hi, side_id, err)) {
+    delete f;
+    return GLH_ERR_ARG;
+  }
+  *out = f;
+  return GLH_OK;
+}
+
+int glh_forest_destroy(glh_forest *f)
+{
+  delete f;
+  return GLH_OK;
+}
+
+int64_t glh_forest_n_cells(const glh_forest *f) { return f ? (int64_t)f->f.leaves.size() : -1; }
+
+int glh_forest_refine(const glh_forest *f, const uint8_t *flags, glh_forest **out)
+{
+  if (!f || !flags || !out) return GLH_ERR_ARG;
+  *out = nullptr;
+  auto *g = new glh_forest(*f);
+  std::vector<char> fl(flags, flags + f->f.leaves.size());
+  std::string err;
+  if (glh::forest_refine(g->f, fl, err)) {
+    delete g;
+    return GLH_ERR_ARG;
+  }
+  *out = g;
+  return GLH_OK;
+}
+
+int glh_forest_mesh(const glh_forest *f, glh_mesh **out)
+{
+  if (!f || !out) return GLH_ERR_ARG;
+  *out = nullptr;
+  auto *mm = new glh_mesh();
+  std::string err;
+  if (glh::forest_mesh(f->f, mm->m, err)) {
+    delete mm;
+    return GLH_ERR_ARG;
+  }
+  *out = mm;
+  return GLH_OK;
+}
+
+int glh_forest_transfer(const glh_forest *fo, const glh_mesh *mo, const double *Uo, const glh_forest *fn,
+                        const glh_mesh *mn, double *Un)
+{
+  if (!fo || !mo || !Uo || !fn || !mn || !Un) return GLH_ERR_ARG;
+  std::string err;
+  return glh::forest_transfer(fo->f, mo->m, Uo, fn->f, mn->m, Un, err) ? GLH_ERR_ARG : GLH_OK;
+}
+
+int glh_kelly(const glh_forest *f, const glh_mesh *m, const double *U, float *eta)
+{
+  if (!f || !m || !U || !eta) return GLH_ERR_ARG;
+  std::string err;
+  return glh::kelly_estimate(f->f, m->m, U, eta, err) ? GLH_ERR_ARG : GLH_OK;
+}
+
+int glh_flag_fixed_number(int64_t n, const float *eta, double frac, uint8_t *flags)
+{
+  if (n < 0 || (n > 0 && (!eta || !flags)) || !(frac >= 0.0 && frac <= 1.0)) return GLH_ERR_ARG;
+  std::vector<char> fl;
+  glh::flag_fixed_number((int)n, eta, frac, fl);
+  for (int64_t i = 0; i < n; ++i) flags[i] = (uint8_t)fl[i];
+  return GLH_OK;
+}
+
+int glh_mesh_constraint_sizes(const glh_mesh *mm, int64_t sizes[3])
+{
+  if (!mm || !sizes) return GLH_ERR_ARG;
+  const glh::Mesh &m = mm->m;
+  int64_t nh = 0;
+  if (m.has_hanging())
+    for (int v = 0; v < m.n_nodes; ++v) nh += m.hang_ptr[v + 1] > m.hang_ptr[v];
+  sizes[0] = nh;
+  sizes[1] = (int64_t)m.hang_m.size();
+  sizes[2] = m.arow_ptr.empty() ? (m.row_ptr.empty() ? 0 : m.row_ptr.back()) : m.arow_ptr.back();
+  return GLH_OK;
+}
+
+int glh_mesh_constraints(const glh_mesh *mm, int *hang_ptr, int *hang_m, double *hang_w, int *arow_ptr, int *acol)
+{
+  if (!mm) return GLH_ERR_ARG;
+  const glh::Mesh &m = mm->m;
+  const bool h = m.has_hanging();
+  if (hang_ptr) {
+    if (h) std::memcpy(hang_ptr, m.hang_ptr.data(), m.hang_ptr.size() * sizeof(int));
+    else std::fill(hang_ptr, hang_ptr + m.n_nodes + 1, 0);
+  }
+  if (hang_m && h) std::memcpy(hang_m, m.hang_m.data(), m.hang_m.size() * sizeof(int));
+  if (hang_w && h) std::memcpy(hang_w, m.hang_w.data(), m.hang_w.size() * sizeof(double));
+  const std::vector<int> &rp = m.arow_ptr.empty() ? m.row_ptr : m.arow_ptr;
+  const std::vector<int> &cl = m.arow_ptr.empty() ? m.col : m.acol;
+  if (arow_ptr) std::memcpy(arow_ptr, rp.data(), rp.size() * sizeof(int));
+  if (acol) std::memcpy(acol, cl.data(), cl.size() * sizeof(int));
+  return GLH_OK;
+}
+
 int glh_mesh_instances(const glh_mesh *mm, int *node_off, int *cell_off)
 {
   if (!mm) return GLH_ERR_ARG;
@@ -998,6 +1152,109 @@ int glh_mesh_instances(const glh_mesh *mm, int *node_off, int *cell_off)
   if (cell_off) std::memcpy(cell_off, m.cell_off.data(), m.cell_off.size() * sizeof(int));
   return GLH_OK;
 }
+
+namespace {
+
+// Gather lists of the condensation Jc = C^T Jf C, rc = C^T rf and the
+// hanging lines, built on the host from the mesh's closed constraint lines.
+int upload_constraints(glh_ctx *c, const glh::Mesh &m, const std::vector<int> &diag)
+{
+  hipStream_t s = c->stream;
+  const int nn = m.n_nodes;
+  auto hanging = [&](int a) { return m.hang_ptr[a + 1] > m.hang_ptr[a]; };
+  // Jc block t gathers Jf block (A,B) with weight wA*wB for every master pair
+  // (r in T(A), s in T(B)); T(A) = {A} or the line of a hanging A
+  std::vector<std::vector<int>> tsrc(c->nnzb);
+  std::vector<std::vector<double>> tw(c->nnzb);
+  std::vector<std::vector<unsigned char>> tfl(c->nnzb);
+  for (int A = 0; A < nn; ++A)
+    for (int sb = m.arow_ptr[A]; sb < m.arow_ptr[A + 1]; ++sb) {
+      const int B = m.acol[sb];
+      const bool ha = hanging(A), hb = hanging(B);
+      const int ka0 = ha ? m.hang_ptr[A] : 0, ka1 = ha ? m.hang_ptr[A + 1] : 1;
+      const int kb0 = hb ? m.hang_ptr[B] : 0, kb1 = hb ? m.hang_ptr[B + 1] : 1;
+      for (int ka = ka0; ka < ka1; ++ka)
+        for (int kb = kb0; kb < kb1; ++kb) {
+          const int r = ha ? m.hang_m[ka] : A, cc = hb ? m.hang_m[kb] : B;
+          const double w = (ha ? m.hang_w[ka] : 1.0) * (hb ? m.hang_w[kb] : 1.0);
+          const int t = glh::find_block(m, r, cc);
+          if (t < 0) return fail(c, GLH_ERR_ARG, "condensed pattern lacks a constrained coupling");
+          tsrc[t].push_back(sb);
+          tw[t].push_back(w);
+          tfl[t].push_back((unsigned char)((ha ? 1 : 0) | (hb ? 2 : 0)));
+        }
+    }
+  std::vector<int> gptr(c->nnzb + 1, 0), gsrc, brow(c->nnzb);
+  std::vector<double> gw;
+  std::vector<unsigned char> gfl;
+  for (int t = 0; t < c->nnzb; ++t) {
+    gptr[t + 1] = gptr[t] + (int)tsrc[t].size();
+    gsrc.insert(gsrc.end(), tsrc[t].begin(), tsrc[t].end());
+    gw.insert(gw.end(), tw[t].begin(), tw[t].end());
+    gfl.insert(gfl.end(), tfl[t].begin(), tfl[t].end());
+    std::vector<int>().swap(tsrc[t]);
+    std::vector<double>().swap(tw[t]);
+    std::vector<unsigned char>().swap(tfl[t]);
+  }
+  for (int r = 0; r < nn; ++r)
+    for (int t = m.row_ptr[r]; t < m.row_ptr[r + 1]; ++t) brow[t] = r;
+  // rhs: row r gathers itself and every hanging node whose line holds r
+  std::vector<std::vector<int>> rs(nn);
+  std::vector<std::vector<double>> rwv(nn);
+  std::vector<int> hnode, hptr(1, 0), hm;
+  std::vector<double> hw;
+  for (int a = 0; a < nn; ++a) {
+    if (!hanging(a)) {
+      rs[a].push_back(a);
+      rwv[a].push_back(1.0);
+      continue;
+    }
+    hnode.push_back(a);
+    for (int k = m.hang_ptr[a]; k < m.hang_ptr[a + 1]; ++k) {
+      hm.push_back(m.hang_m[k]);
+      hw.push_back(m.hang_w[k]);
+    }
+    hptr.push_back((int)hm.size());
+  }
+  for (int a = 0; a < nn; ++a)
+    if (hanging(a))
+      for (int k = m.hang_ptr[a]; k < m.hang_ptr[a + 1]; ++k) {
+        rs[m.hang_m[k]].push_back(-(a + 1));  // marks a constraint contribution
+        rwv[m.hang_m[k]].push_back(m.hang_w[k]);
+      }
+  std::vector<int> rptr(nn + 1, 0), rsrc;
+  std::vector<double> rw;
+  std::vector<unsigned char> rfl;
+  for (int r = 0; r < nn; ++r) {
+    for (size_t k = 0; k < rs[r].size(); ++k) {
+      const int v = rs[r][k];
+      rsrc.push_back(v >= 0 ? v : -v - 1);
+      rfl.push_back(v >= 0 ? 0 : 1);
+      rw.push_back(rwv[r][k]);
+    }
+    rptr[r + 1] = (int)rsrc.size();
+  }
+  HIPCHK(c, upload(c->c_gptr, gptr.data(), gptr.size(), s));
+  HIPCHK(c, upload(c->c_gsrc, gsrc.data(), gsrc.size(), s));
+  HIPCHK(c, upload(c->c_gw, gw.data(), gw.size(), s));
+  HIPCHK(c, upload(c->c_gfl, gfl.data(), gfl.size(), s));
+  HIPCHK(c, upload(c->c_brow, brow.data(), brow.size(), s));
+  HIPCHK(c, upload(c->c_rptr, rptr.data(), rptr.size(), s));
+  HIPCHK(c, upload(c->c_rsrc, rsrc.data(), rsrc.size(), s));
+  HIPCHK(c, upload(c->c_rw, rw.data(), rw.size(), s));
+  HIPCHK(c, upload(c->c_rfl, rfl.data(), rfl.size(), s));
+  c->n_hang = (int)hnode.size();
+  HIPCHK(c, upload(c->c_hnode, hnode.data(), hnode.size(), s));
+  HIPCHK(c, upload(c->c_hptr, hptr.data(), hptr.size(), s));
+  HIPCHK(c, upload(c->c_hm, hm.data(), hm.size(), s));
+  HIPCHK(c, upload(c->c_hw, hw.data(), hw.size(), s));
+  HIPCHK(c, c->hdiag.alloc((size_t)nn * NC));
+  HIPCHK(c, c->rfull.alloc((size_t)nn * NC));
+  HIPCHK(c, hipStreamSynchronize(s));
+  return GLH_OK;
+}
+
+}  // namespace
 
 int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
 {
@@ -1032,17 +1289,20 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
     diag[r] = glh::find_block(m, r, r);
     if (diag[r] < 0) return fail(c, GLH_ERR_ARG, "pattern lacks a diagonal block");
   }
-  // cell -> block map with first-writer marks in colour order
+  c->constrained = m.has_hanging();
+  c->nnzb_a = c->constrained ? m.arow_ptr.back() : c->nnzb;
+  if (c->constrained) RCHK(upload_constraints(c, m, diag));
+  // cell -> block map (cell-loop pattern) with first-writer marks in colour order
   const int ns = m.ns;
   std::vector<int> cblk((size_t)m.n_cells * ns * ns);
-  std::vector<uint8_t> touched(c->nnzb, 0);
+  std::vector<uint8_t> touched(c->nnzb_a, 0);
   for (size_t k = 0; k + 1 < m.color_ptr.size(); ++k)
     for (int i = m.color_ptr[k]; i < m.color_ptr[k + 1]; ++i) {
       const int e = m.color_cells[i];
       const int *cn = &m.cell_nodes[(size_t)e * ns];
       for (int a = 0; a < ns; ++a)
         for (int b = 0; b < ns; ++b) {
-          const int blk = glh::find_block(m, cn[a], cn[b]);
+          const int blk = glh::find_ablock(m, cn[a], cn[b]);
           if (blk < 0) return fail(c, GLH_ERR_ARG, "pattern lacks a cell coupling");
           const bool first = !touched[blk];
           touched[blk] = 1;
@@ -1072,6 +1332,17 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
   HIPCHK(c, upload(c->row_ptr, m.row_ptr.data(), m.row_ptr.size(), s));
   HIPCHK(c, upload(c->col, m.col.data(), m.col.size(), s));
   HIPCHK(c, upload(c->diag, diag.data(), diag.size(), s));
+  c->cons = glh::ConstraintDev{};
+  if (c->constrained) {
+    glh::ConstraintDev &C = c->cons;
+    C.n_nodes = m.n_nodes;
+    C.nnzb = c->nnzb;
+    C.n_hang = c->n_hang;
+    C.gptr = c->c_gptr.p; C.gsrc = c->c_gsrc.p; C.gw = c->c_gw.p; C.gfl = c->c_gfl.p;
+    C.brow = c->c_brow.p; C.col = c->col.p; C.mask = c->node_mask.p; C.diag = c->diag.p;
+    C.rptr = c->c_rptr.p; C.rsrc = c->c_rsrc.p; C.rw = c->c_rw.p; C.rfl = c->c_rfl.p;
+    C.hnode = c->c_hnode.p; C.hptr = c->c_hptr.p; C.hm = c->c_hm.p; C.hw = c->c_hw.p;
+  }
   HIPCHK(c, upload(c->cell_blk, cblk.data(), cblk.size(), s));
   HIPCHK(c, upload(c->color_cells, m.color_cells.data(), m.color_cells.size(), s));
   HIPCHK(c, upload(c->cell_mat, cmat.data(), cmat.size(), s));
@@ -1105,7 +1376,7 @@ int glh_set_solution(glh_ctx *c, const double *Uh)
   if (!Uh) return fail(c, GLH_ERR_ARG, "null solution");
   HIPCHK(c, hipMemcpyAsync(c->U.p, Uh, ndofs(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
   // constraints_solution.distribute (setup_uniform_B-phase.cc:262)
-  HIPCHK(c, glh::launch_mask_zero(c->n_nodes, c->node_mask.p, c->U.p, c->stream));
+  RCHK(distribute(c, c->U.p));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return GLH_OK;
 }
@@ -1272,7 +1543,7 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
     // line search below decides whether it lowers ||R||
     const int src = solve_impl(c, lin_tol, lin_maxit, restart, sst, &its_seg);
     if (src == GLH_ERR_NOCONV) {
-      HIPCHK(c, glh::launch_mask_zero(c->n_nodes, c->node_mask.p, c->dU.p, c->stream));
+      RCHK(distribute(c, c->dU.p));
     } else if (src) {
       return src;
     }

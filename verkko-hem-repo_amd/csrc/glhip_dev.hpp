@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "common.hpp"
+
 namespace glh {
 
 // Reference-element tables for FE_Q(k) x QGauss(k+1) on the unit cube
@@ -33,7 +35,31 @@ struct MeshDev {
   const uint32_t *cell_face; // [n_cells] 3 bits per local face: boundary id + 1 (0 = interior)
   const uint32_t *node_mask; // [n_nodes] Dirichlet component bits
   const int *cell_blk;       // [n_cells*NS*NS] BSR block of (a,b); -(blk+1) when this cell writes first
+  double *hdiag;             // [n_nodes*18] sum of |local diagonal| of hanging nodes, or null
 };
+
+// Hanging-node constraints on the device (see gl_constraints.hip).
+struct ConstraintDev {
+  int n_nodes = 0, nnzb = 0, n_hang = 0;
+  const int *gptr = nullptr, *gsrc = nullptr;  // Jc gather lists per block
+  const double *gw = nullptr;
+  const unsigned char *gfl = nullptr;
+  const int *brow = nullptr, *col = nullptr;  // block row / column of every Jc block
+  const uint32_t *mask = nullptr;             // node masks
+  const int *diag = nullptr;                  // diagonal block of every node in Jc
+  const int *rptr = nullptr, *rsrc = nullptr; // rhs gather lists per node
+  const double *rw = nullptr;
+  const unsigned char *rfl = nullptr;
+  const int *hnode = nullptr, *hptr = nullptr, *hm = nullptr;  // hanging nodes and their closed lines
+  const double *hw = nullptr;
+};
+// Jc = C^T Jf C over the condensed pattern, hanging rows diagonal (hdiag)
+template <typename TJ>
+hipError_t launch_condense(const ConstraintDev &C, const TJ *Jf, TJ *Jc, const double *hdiag, hipStream_t s);
+// rc = C^T rf, hanging rows 0
+hipError_t launch_condense_rhs(const ConstraintDev &C, const double *rf, double *rc, hipStream_t s);
+// hanging values of v from their masters
+hipError_t launch_distribute(const ConstraintDev &C, double *v, hipStream_t s);
 
 // ---------------------------------------------------------------- assembly
 // One launch per colour; cells[] lists the cells of that colour.

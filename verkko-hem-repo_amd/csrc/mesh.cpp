@@ -135,6 +135,31 @@ int concat(Mesh &out, const std::vector<const Mesh *> &ms, std::string &err)
     out.node_off.push_back(noff);
     out.cell_off.push_back(coff);
   }
+  // hanging-node lines and cell-loop patterns, when any instance has them
+  bool any_hang = false;
+  for (const Mesh *m : ms) any_hang = any_hang || m->has_hanging();
+  if (any_hang) {
+    out.hang_ptr.assign(1, 0);
+    out.arow_ptr.assign(1, 0);
+    int nof = 0;
+    for (const Mesh *m : ms) {
+      const bool h = m->has_hanging();
+      for (int v = 0; v < m->n_nodes; ++v) {
+        if (h)
+          for (int k = m->hang_ptr[v]; k < m->hang_ptr[v + 1]; ++k) {
+            out.hang_m.push_back(m->hang_m[k] + nof);
+            out.hang_w.push_back(m->hang_w[k]);
+          }
+        out.hang_ptr.push_back((int)out.hang_m.size());
+      }
+      const std::vector<int> &rp = h ? m->arow_ptr : m->row_ptr;
+      const std::vector<int> &cl = h ? m->acol : m->col;
+      const int base = out.arow_ptr.back();
+      for (int r = 0; r < m->n_nodes; ++r) out.arow_ptr.push_back(base + rp[r + 1]);
+      for (int v : cl) out.acol.push_back(v + nof);
+      nof += m->n_nodes;
+    }
+  }
   out.color_ptr.assign(1, 0);
   for (size_t k = 0; k < ncol; ++k) {
     int cbase = 0;
@@ -200,6 +225,53 @@ int find_block(const Mesh &m, int r, int s)
   auto it = std::lower_bound(b, e, s);
   if (it == e || *it != s) return -1;
   return (int)(it - m.col.begin());
+}
+
+int find_ablock(const Mesh &m, int r, int s)
+{
+  if (m.arow_ptr.empty()) return find_block(m, r, s);
+  auto b = m.acol.begin() + m.arow_ptr[r], e = m.acol.begin() + m.arow_ptr[r + 1];
+  auto it = std::lower_bound(b, e, s);
+  if (it == e || *it != s) return -1;
+  return (int)(it - m.acol.begin());
+}
+
+void build_constrained_patterns(Mesh &m)
+{
+  build_pattern(m);
+  m.arow_ptr.swap(m.row_ptr);
+  m.acol.swap(m.col);
+  // targets of a node: itself, or the masters of its constraint line
+  auto targets = [&](int a, std::vector<int> &t) {
+    t.clear();
+    if (m.hang_ptr[a + 1] > m.hang_ptr[a])
+      for (int k = m.hang_ptr[a]; k < m.hang_ptr[a + 1]; ++k) t.push_back(m.hang_m[k]);
+    else
+      t.push_back(a);
+  };
+  std::vector<std::vector<int>> nb(m.n_nodes);
+  std::vector<int> ta, tb;
+  for (int e = 0; e < m.n_cells; ++e) {
+    const int *cn = &m.cell_nodes[(size_t)e * m.ns];
+    for (int a = 0; a < m.ns; ++a) {
+      targets(cn[a], ta);
+      for (int b = 0; b < m.ns; ++b) {
+        targets(cn[b], tb);
+        for (int r : ta) nb[r].insert(nb[r].end(), tb.begin(), tb.end());
+      }
+    }
+  }
+  for (int r = 0; r < m.n_nodes; ++r) nb[r].push_back(r);  // every row keeps its diagonal block
+  m.row_ptr.assign(m.n_nodes + 1, 0);
+  m.col.clear();
+  for (int r = 0; r < m.n_nodes; ++r) {
+    auto &v = nb[r];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    m.row_ptr[r + 1] = m.row_ptr[r] + (int)v.size();
+    m.col.insert(m.col.end(), v.begin(), v.end());
+    std::vector<int>().swap(v);
+  }
 }
 
 }  // namespace glh

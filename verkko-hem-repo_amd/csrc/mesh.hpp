@@ -9,9 +9,9 @@
 #include <string>
 #include <vector>
 
-namespace glh {
+#include "common.hpp"
 
-constexpr int NC = 18;
+namespace glh {
 
 struct Mesh {
   int deg = 1, n1 = 2, ns = 8;
@@ -27,6 +27,17 @@ struct Mesh {
   // instances of a batch: instance i owns nodes [node_off[i], node_off[i+1])
   // and cells [cell_off[i], cell_off[i+1]); a single mesh is one instance
   std::vector<int> node_off, cell_off;
+  // Adaptive meshes.  Hanging node h: U[h] = sum_k hang_w[k] U[hang_m[k]] over
+  // k in [hang_ptr[h], hang_ptr[h+1]) (closed lines: no master is hanging).
+  // row_ptr/col is then the condensed pattern of the system matrix
+  // (constrained couplings moved to the masters, hanging rows diagonal only)
+  // and arow_ptr/acol the pattern of the cell loop over all nodes.  All four
+  // hang/arow vectors are empty for conforming meshes.
+  std::vector<int> hang_ptr, hang_m;
+  std::vector<double> hang_w;
+  std::vector<int> arow_ptr, acol;
+  std::vector<int> cell_level;  // refinement level of each cell (adaptive meshes)
+  bool has_hanging() const { return !hang_ptr.empty() && hang_ptr.back() > 0; }
 };
 
 // Dirichlet component mask of a boundary id: 0 -> all 18 components (A = 0
@@ -41,6 +52,10 @@ int make_box(Mesh &m, int deg, int nx, int ny, int nz, const double lo[3], const
 // BSR pattern: node pairs sharing a cell (DoFTools::make_sparsity_pattern
 // with full component coupling, setup_uniform_B-phase.cc:190-196).
 void build_pattern(Mesh &m);
+// With hanging nodes: the cell-loop pattern into arow_ptr/acol and the
+// condensed pattern (make_sparsity_pattern with constraints,
+// keep_constrained_dofs = false) into row_ptr/col.
+void build_constrained_patterns(Mesh &m);
 
 // Greedy colouring of cells so that no two cells of one colour share a node.
 void greedy_colors(Mesh &m);
@@ -51,5 +66,7 @@ int concat(Mesh &out, const std::vector<const Mesh *> &ms, std::string &err);
 
 // Index of block (r, s) in the pattern, or -1.
 int find_block(const Mesh &m, int r, int s);
+// Same in the cell-loop pattern (arow_ptr/acol when present).
+int find_ablock(const Mesh &m, int r, int s);
 
 }  // namespace glh

@@ -20,13 +20,18 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 
 EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "glh_set_materials", "glh_matep",
            "glh_mesh_box", "glh_mesh_concat", "glh_mesh_instances", "glh_instance_energy", "glh_instance_stats",
            "glh_mesh_destroy", "glh_mesh_sizes", "glh_mesh_get", "glh_upload_mesh", "glh_initial_guess_host", "glh_set_solution", "glh_store_initial", "glh_restore_initial",
            "glh_get_solution", "glh_get_update", "glh_get_rhs", "glh_get_jacobian", "glh_assemble", "glh_rhs_norm",
            "glh_residual", "glh_solve", "glh_newton_step", "glh_free_energy", "glh_newton_solve",
-           "glh_last_assembly_ms", "glh_set_verbose", "glh_set_timing", "glh_timing", "glh_bench_spmv", "glh_bench_copy"]
+           "glh_last_assembly_ms", "glh_set_verbose", "glh_set_timing", "glh_timing", "glh_bench_spmv", "glh_bench_copy",
+           "glh_mesh_constraint_sizes", "glh_mesh_constraints", "glh_forest_create", "glh_forest_destroy",
+           "glh_forest_n_cells", "glh_forest_refine", "glh_forest_mesh", "glh_forest_transfer", "glh_kelly",
+           "glh_flag_fixed_number", "glh_initial_guess_bna", "glh_run_adaptive"]
 
 _lib = None
 
@@ -78,6 +83,21 @@ def lib():
     L.glh_timing.argtypes = [vp, _dp]
     L.glh_bench_spmv.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.glh_bench_copy.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+    L.glh_mesh_constraint_sizes.argtypes = [vp, _i64p]
+    L.glh_mesh_constraints.argtypes = [vp] + [vp] * 5
+    L.glh_forest_create.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, C.POINTER(vp)]
+    L.glh_forest_destroy.argtypes = [vp]
+    L.glh_forest_n_cells.argtypes = [vp]
+    L.glh_forest_n_cells.restype = C.c_int64
+    L.glh_forest_refine.argtypes = [vp, _u8p, C.POINTER(vp)]
+    L.glh_forest_mesh.argtypes = [vp, C.POINTER(vp)]
+    L.glh_forest_transfer.argtypes = [vp, vp, _dp, vp, vp, _dp]
+    L.glh_kelly.argtypes = [vp, vp, _dp, _f32p]
+    L.glh_flag_fixed_number.argtypes = [C.c_int64, _f32p, C.c_double, _u8p]
+    L.glh_run_adaptive.argtypes = [vp, vp, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int,
+                                   C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, _dp, C.POINTER(vp), C.POINTER(vp)]
+    L.glh_initial_guess_bna.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                        _dp]
     _lib = L
     return L
 
@@ -96,6 +116,16 @@ def initial_guess(n_nodes, gap, phase=0, amp=0.05, seed=0):
     rc = lib().glh_initial_guess_host(int(n_nodes), float(gap), int(phase), float(amp), int(seed), U)
     if rc:
         raise GLHError("glh_initial_guess_host failed (%d)" % rc)
+    return U
+
+
+def initial_guess_bna(node_xyz, z_a, gap_A, gap_B, amp=0.05, seed=1):
+    xyz = np.ascontiguousarray(node_xyz, np.float64).reshape(-1)
+    n = xyz.size // 3
+    U = np.zeros(n * 18)
+    rc = lib().glh_initial_guess_bna(n, xyz, float(z_a), float(gap_A), float(gap_B), float(amp), int(seed), U)
+    if rc:
+        raise GLHError("glh_initial_guess_bna failed (%d)" % rc)
     return U
 
 
@@ -142,6 +172,19 @@ class Mesh:
         a["faces"] = a["faces"][:self.n_faces * 3]
         return a
 
+    def constraints(self):
+        """Hanging-node lines (hang_ptr, hang_m, hang_w) and the cell-loop
+        pattern (arow_ptr, acol); empty lines for conforming meshes."""
+        s = np.zeros(3, np.int64)
+        lib().glh_mesh_constraint_sizes(self.h, s)
+        a = dict(hang_ptr=np.zeros(self.n_nodes + 1, np.int32), hang_m=np.zeros(max(1, s[1]), np.int32),
+                 hang_w=np.zeros(max(1, s[1])), arow_ptr=np.zeros(self.n_nodes + 1, np.int32),
+                 acol=np.zeros(max(1, s[2]), np.int32))
+        lib().glh_mesh_constraints(self.h, *[v.ctypes.data_as(C.c_void_p) for v in a.values()])
+        a["hang_m"], a["hang_w"], a["acol"] = a["hang_m"][:s[1]], a["hang_w"][:s[1]], a["acol"][:s[2]]
+        a["n_hanging"] = int(s[0])
+        return a
+
     def close(self):
         if self.h:
             lib().glh_mesh_destroy(self.h)
@@ -152,6 +195,78 @@ class Mesh:
             self.close()
         except Exception:
             pass
+
+
+class Forest:
+    """Adaptive Q1 hex mesh (glh_forest): refinement with 2:1 balance, the
+    mesh with hanging-node constraints, Kelly indicators and transfer."""
+
+    def __init__(self, n=None, lo=None, hi=None, side_id=None, handle=None):
+        self.h = C.c_void_p()
+        if handle is not None:
+            self.h = handle
+        else:
+            rc = lib().glh_forest_create(int(n[0]), int(n[1]), int(n[2]), np.asarray(lo, np.float64),
+                                         np.asarray(hi, np.float64), np.asarray(side_id, np.int32),
+                                         C.byref(self.h))
+            if rc:
+                raise GLHError("glh_forest_create failed (%d)" % rc)
+
+    @property
+    def n_cells(self):
+        return int(lib().glh_forest_n_cells(self.h))
+
+    def refine(self, flags):
+        f = np.ascontiguousarray(flags, np.uint8)
+        if f.size != self.n_cells:
+            raise GLHError("one flag per cell expected")
+        h = C.c_void_p()
+        rc = lib().glh_forest_refine(self.h, f, C.byref(h))
+        if rc:
+            raise GLHError("glh_forest_refine failed (%d)" % rc)
+        return Forest(handle=h)
+
+    def mesh(self):
+        h = C.c_void_p()
+        rc = lib().glh_forest_mesh(self.h, C.byref(h))
+        if rc:
+            raise GLHError("glh_forest_mesh failed (%d)" % rc)
+        return Mesh(handle=h)
+
+    def kelly(self, mesh, U):
+        eta = np.zeros(self.n_cells, np.float32)
+        rc = lib().glh_kelly(self.h, mesh.h, np.ascontiguousarray(U, np.float64), eta)
+        if rc:
+            raise GLHError("glh_kelly failed (%d)" % rc)
+        return eta
+
+    def transfer(self, mesh, U, new_forest, new_mesh):
+        Un = np.zeros(new_mesh.n_nodes * 18)
+        rc = lib().glh_forest_transfer(self.h, mesh.h, np.ascontiguousarray(U, np.float64), new_forest.h,
+                                       new_mesh.h, Un)
+        if rc:
+            raise GLHError("glh_forest_transfer failed (%d)" % rc)
+        return Un
+
+    def close(self):
+        if self.h:
+            lib().glh_forest_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def flag_fixed_number(eta, frac):
+    eta = np.ascontiguousarray(eta, np.float32)
+    fl = np.zeros(eta.size, np.uint8)
+    rc = lib().glh_flag_fixed_number(eta.size, eta, float(frac), fl)
+    if rc:
+        raise GLHError("glh_flag_fixed_number failed (%d)" % rc)
+    return fl
 
 
 class Context:
@@ -256,6 +371,26 @@ class Context:
         v = C.c_double()
         self._chk(lib().glh_free_energy(self.h, C.byref(v)), "glh_free_energy")
         return v.value
+
+    CYCLE_KEYS = ["cycle", "cells", "dofs", "hanging", "newton_steps", "gmres_its", "r0", "r_final", "F",
+                  "converged", "t_solve", "t_cycle", "F64", "F_rel_err_vs_fp64", "U_rel_err_vs_fp64", "converged64"]
+
+    def run_adaptive(self, forest, U0, n_refine=4, refine_frac=0.3, rtol=1e-10, max_newton=60, lin_tol=0.1,
+                     lin_maxit=10000, restart=100, ls_step=0.83, damped=True, fp32=False, check_fp64=False):
+        """Adaptive cycles (glh_run_adaptive): returns (per-cycle stats dicts,
+        last Forest, last Mesh); the context keeps the last mesh and field."""
+        st = np.zeros((n_refine + 1) * 16)
+        fo, mo = C.c_void_p(), C.c_void_p()
+        self._chk(lib().glh_run_adaptive(self.h, forest.h, np.ascontiguousarray(U0, np.float64), int(n_refine),
+                                         float(refine_frac), float(rtol), int(max_newton), float(lin_tol),
+                                         int(lin_maxit), int(restart), float(ls_step), int(damped), int(fp32),
+                                         int(check_fp64), st, C.byref(fo), C.byref(mo)), "glh_run_adaptive")
+        mesh = Mesh(handle=mo)
+        self.n_dofs = mesh.n_nodes * 18
+        self.nnzb = mesh.nnzb
+        self.n_inst = 1
+        rows = [dict(zip(self.CYCLE_KEYS, r.tolist())) for r in st.reshape(-1, 16)]
+        return rows, Forest(handle=fo), mesh
 
     def newton_solve(self, rtol=1e-10, max_newton=50, lin_tol=0.1, lin_maxit=20000, restart=30, ls_step=0.83,
                      damped=True, fp32=False):
