@@ -240,6 +240,13 @@ int glh_instance_stats(glh_ctx *ctx, double *out);
  * (HIP events): ms[0] = J+R assembly ms, ms[1] = number of launches. */
 int glh_last_assembly_ms(glh_ctx *ctx, double ms[2]);
 /* level > 0: one progress line per Newton step on stderr (glh_newton_solve). */
+/* Pseudo-transient continuation in glh_newton_solve (an addition to the
+ * reference's damped Newton, iteration.cc:109-215): every Newton matrix
+ * becomes J + sigma_k |diag J| on the unconstrained dofs with
+ * sigma_k = sigma0 ||R_k|| / ||R_0|| per instance, which follows the
+ * gradient flow of F far from a minimum and is plain Newton near it.  The
+ * fixed point is unchanged.  0 (default) = off. */
+int glh_set_continuation(glh_ctx *ctx, double sigma0);
 int glh_set_verbose(glh_ctx *ctx, int level);
 /* Live kernel timing with HIP events on the context stream.  on != 0 resets
  * the counters and records an event pair around every SpMV launch and every

@@ -126,7 +126,10 @@ def test_adaptive_cycle_refines_and_transfers(ctx):
 
 def test_run_adaptive_driver_fp32_against_fp64(ctx):
     # the cycle loop of FemGL::run in the library: BnA guess (seed 1),
-    # two refinement cycles, FP32 mixed precision checked against FP64 per cycle
+    # two refinement cycles, FP32 mixed precision checked against FP64 per
+    # cycle.  The A/B wall is not stationary (B is the lower phase at 0.8 Tc,
+    # 22 bar): the damped Newton of the reference stalls on it, pseudo-
+    # transient continuation reaches the minimum.
     lo, hi = (-5.0, -5.0, -6.0), (5.0, 5.0, 6.0)
     p, t = 22.0, 0.8
     f = G.Forest((4, 4, 4), lo, hi, SIDES)
@@ -134,8 +137,12 @@ def test_run_adaptive_driver_fp32_against_fp64(ctx):
     mp = O.matep(p, t)
     U0 = G.initial_guess_bna(m.arrays()["node_xyz"], 0.0, mp["gap_A"], mp["gap_B"], amp=0.05, seed=1)
     ctx.set_material_pt(p, t)
-    rows, f2, m2 = ctx.run_adaptive(f, U0, n_refine=2, refine_frac=0.3, rtol=1e-9, lin_tol=1e-2, restart=50,
-                                    fp32=True, check_fp64=True)
+    ctx.set_continuation(1.0)
+    try:
+        rows, f2, m2 = ctx.run_adaptive(f, U0, n_refine=2, refine_frac=0.3, rtol=1e-9, lin_tol=1e-2, restart=50,
+                                        fp32=True, check_fp64=True)
+    finally:
+        ctx.set_continuation(0.0)
     for r in rows:
         say(r)
         assert r["converged"] == 1.0 and r["converged64"] == 1.0
@@ -148,3 +155,32 @@ def test_run_adaptive_driver_fp32_against_fp64(ctx):
     _, ro = O.assemble(om, co, U, J=False)
     assert np.linalg.norm(ro) <= 1.01e-9 * rows[2]["r0"]
     assert rows[2]["F"] == pytest.approx(O.free_energy(om, co, U), rel=1e-12)
+
+
+def test_continuation_reaches_the_minimum_where_damped_newton_stalls(ctx):
+    lo, hi = (-5.0, -5.0, -6.0), (5.0, 5.0, 6.0)
+    p, t = 22.0, 0.8
+    m = G.Forest((4, 4, 4), lo, hi, SIDES).mesh()
+    mp = O.matep(p, t)
+    U0 = G.initial_guess_bna(m.arrays()["node_xyz"], 0.0, mp["gap_A"], mp["gap_B"], amp=0.05, seed=1)
+    ctx.set_material_pt(p, t)
+    ctx.upload_mesh(m)
+    ctx.set_solution(U0)
+    st0 = ctx.newton_solve(rtol=1e-10, max_newton=40, lin_tol=1e-2, restart=50)
+    ctx.set_solution(U0)
+    ctx.set_continuation(1.0)
+    try:
+        st1 = ctx.newton_solve(rtol=1e-10, max_newton=40, lin_tol=1e-2, restart=50)
+    finally:
+        ctx.set_continuation(0.0)
+    say("damped newton", st0)
+    say("continuation", st1)
+    assert st0["converged"] == 0.0 and st1["converged"] == 1.0
+    assert st1["F"] < st0["F"]
+    # the continuation fixed point is the oracle's: plain Newton from it stays put
+    om = oracle_adaptive(m, lo, hi, SIDES)
+    co = O.coef(p, t)
+    U = ctx.solution()
+    Uo, rc, ost = O.newton(om, co, U, rtol=1e-6, max_newton=5, lin_tol=1e-3, restart=50)
+    assert O.free_energy(om, co, Uo) == pytest.approx(st1["F"], rel=1e-10)
+    assert np.abs(Uo - U).max() <= 1e-8 * np.abs(U).max()

@@ -634,6 +634,24 @@ __global__ __launch_bounds__(AT) void k_build_active(int n_seg, int n_chunks, co
 }
 
 
+// J_cc += sigma[instance] |J_cc| on the diagonal of every unconstrained dof
+// (pseudo-transient continuation); Dirichlet components and hanging nodes
+// keep their rows
+template <typename TJ>
+__global__ __launch_bounds__(RT) void k_diag_shift(int n_rows, const int *__restrict__ diag,
+                                                   const uint32_t *__restrict__ mask, const int *__restrict__ row_seg,
+                                                   const double *__restrict__ sigma, TJ *__restrict__ J)
+{
+  const int i = blockIdx.x * RT + threadIdx.x;
+  if (i >= n_rows * 18) return;
+  const int r = i / 18, c = i % 18;
+  const uint32_t mk = mask[r];
+  if ((mk & HANG_BIT) || ((mk >> c) & 1u)) return;
+  TJ *p = J + (size_t)diag[r] * 324 + c * 19;
+  const double v = (double)*p;
+  *p = (TJ)(v + sigma[row_seg[r]] * fabs(v));
+}
+
 // ----------------------------------------- device-side GMRES control (batched)
 // One thread per instance.  op 0: start of a restart cycle (beta, g, cycle
 // flags); op 1: DGKS flag for column j; op 2: Givens update of column j;
@@ -1000,6 +1018,20 @@ GLH_SEG_INST(double)
 GLH_SEG_INST(float)
 #undef GLH_SEG_INST
 template hipError_t launch_fill<int>(size_t, int, int *, hipStream_t);
+
+template <typename TJ>
+hipError_t launch_diag_shift(int n_rows, const int *diag, const uint32_t *mask, const int *row_seg,
+                             const double *sigma, TJ *J, hipStream_t s)
+{
+  if (n_rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_diag_shift<TJ>), dim3((n_rows * 18 + RT - 1) / RT), dim3(RT), 0, s, n_rows, diag, mask, row_seg,
+                     sigma, J);
+  return hipGetLastError();
+}
+template hipError_t launch_diag_shift<double>(int, const int *, const uint32_t *, const int *, const double *, double *,
+                                              hipStream_t);
+template hipError_t launch_diag_shift<float>(int, const int *, const uint32_t *, const int *, const double *, float *,
+                                             hipStream_t);
 
 // explicit instantiations
 template hipError_t launch_set_scalar<double>(double *, double, hipStream_t);

@@ -149,6 +149,7 @@ struct glh_ctx {
   // on the context stream, read back when the pool fills or a solve ends
   bool timing = false;
   int verbose = 0;
+  double ptc_sigma0 = 0.0;  // pseudo-transient continuation in glh_newton_solve (0: off)
   long reorth_count = 0;
   std::vector<hipEvent_t> ev_pool;
   int ev_used = 0;
@@ -1530,6 +1531,21 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
     if (!any || k == max_newton) break;
     t0 = now_s();
     RCHK(assemble_at(c, c->U.p, true, true, f32, c->rhs.p, false));
+    // pseudo-transient continuation: J + sigma_k |diag J|, sigma_k =
+    // sigma0 ||R_k|| / ||R_0|| per instance (switched evolution relaxation)
+    if (c->ptc_sigma0 > 0) {
+      std::vector<double> sg(S);
+      for (int i = 0; i < S; ++i) sg[i] = c->ptc_sigma0 * rcur[i] / std::max(r0[i], 1e-300);
+      HIPCHK(c, c->s_norm.alloc(S));
+      HIPCHK(c, hipMemcpyAsync(c->s_norm.p, sg.data(), S * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      if (f32)
+        HIPCHK(c, glh::launch_diag_shift<float>(c->n_nodes, c->diag.p, c->node_mask.p, c->row_seg.p, c->s_norm.p,
+                                                c->J32.p, c->stream));
+      else
+        HIPCHK(c, glh::launch_diag_shift<double>(c->n_nodes, c->diag.p, c->node_mask.p, c->row_seg.p, c->s_norm.p,
+                                                 c->J64.p, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     // frozen instances see a zero right-hand side, so GMRES leaves them at dU = 0
     for (int i = 0; i < S; ++i) flag[i] = active[i];
     HIPCHK(c, hipMemcpyAsync(c->s_flag.p, flag.data(), S * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -1613,6 +1629,13 @@ int glh_last_assembly_ms(glh_ctx *c, double ms[2])
   if (!c || !ms) return GLH_ERR_ARG;
   ms[0] = c->asm_ms;
   ms[1] = c->asm_launches;
+  return GLH_OK;
+}
+
+int glh_set_continuation(glh_ctx *c, double sigma0)
+{
+  if (!c || !(sigma0 >= 0.0)) return GLH_ERR_ARG;
+  c->ptc_sigma0 = sigma0;
   return GLH_OK;
 }
 

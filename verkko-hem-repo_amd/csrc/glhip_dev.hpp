@@ -174,6 +174,10 @@ template <typename T>
 hipError_t launch_seg_residual(const SegInfo &S, const T *b, const T *w, T *r, T *partial, T *norm2, hipStream_t s);
 template <typename T>
 hipError_t launch_fill(size_t n, T v, T *d, hipStream_t s);
+// J_cc += sigma[row_seg[r]] |J_cc| on unconstrained diagonal entries
+template <typename TJ>
+hipError_t launch_diag_shift(int n_rows, const int *diag, const uint32_t *mask, const int *row_seg,
+                             const double *sigma, TJ *J, hipStream_t s);
 // y += x per instance (over the active set when S.act is set)
 template <typename T>
 hipError_t launch_seg_add(const SegInfo &S, const T *x, T *y, hipStream_t s);

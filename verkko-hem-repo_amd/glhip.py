@@ -31,7 +31,8 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_last_assembly_ms", "glh_set_verbose", "glh_set_timing", "glh_timing", "glh_bench_spmv", "glh_bench_copy",
            "glh_mesh_constraint_sizes", "glh_mesh_constraints", "glh_forest_create", "glh_forest_destroy",
            "glh_forest_n_cells", "glh_forest_refine", "glh_forest_mesh", "glh_forest_transfer", "glh_kelly",
-           "glh_flag_fixed_number", "glh_initial_guess_bna", "glh_run_adaptive"]
+           "glh_flag_fixed_number", "glh_initial_guess_bna", "glh_run_adaptive",
+           "glh_set_continuation"]
 
 _lib = None
 
@@ -80,6 +81,7 @@ def lib():
     L.glh_last_assembly_ms.argtypes = [vp, _dp]
     L.glh_set_timing.argtypes = [vp, C.c_int]
     L.glh_set_verbose.argtypes = [vp, C.c_int]
+    L.glh_set_continuation.argtypes = [vp, C.c_double]
     L.glh_timing.argtypes = [vp, _dp]
     L.glh_bench_spmv.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.glh_bench_copy.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
@@ -408,6 +410,10 @@ class Context:
 
     def set_verbose(self, level=1):
         self._chk(lib().glh_set_verbose(self.h, int(level)), "glh_set_verbose")
+
+    def set_continuation(self, sigma0):
+        """Pseudo-transient continuation (J + sigma0 ||R||/||R0|| |diag J|); 0: off."""
+        self._chk(lib().glh_set_continuation(self.h, float(sigma0)), "glh_set_continuation")
 
     def set_timing(self, on=True):
         self._chk(lib().glh_set_timing(self.h, int(on)), "glh_set_timing")
