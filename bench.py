@@ -285,14 +285,134 @@ def run_adaptive(G, args):
     print(json.dumps(line), flush=True)
 
 
+def load_module(name):
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "verkko-hem-repo_amd", name + ".py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def run_warm_start(G, args):
+    """Learned warm start: a coordinate MLP (64 Fourier frequencies, 5 x 256
+    SiLU, 18 outputs; BF16 kernels, FP32 accumulation, Adam) trained on
+    converged slab solutions (seed 3) predicts the initial guesses of held-out
+    instances (seed 4), with the seeded B-phase guess as the fallback where the
+    predicted guess's first residual is not lower.  Reports Newton iterations,
+    batched FP32 solves/hr with and without the network, the plain FP64 path
+    and the free-energy agreement with plain FP64."""
+    WS = load_module("warmstart")
+    t_all = time.perf_counter()
+    log = lambda *a: print("bench:", *a, file=sys.stderr, flush=True)
+    # 1. training data: converged FP32 batched solves of the seed-3 instances
+    train = WS.slab_instances(G, args.train_instances, 3)
+    Xs, Ys, n_conv = [], [], 0
+    t0 = time.perf_counter()
+    for b0 in range(0, len(train), 256):
+        part = train[b0:b0 + 256]
+        guesses = [WS.plain_guess(G, it, 3000 + b0 + i) for i, it in enumerate(part)]
+        st, ist, F, Us = WS.solve_batch(G, part, guesses, fp32=True, rtol=1e-8, lin_maxit=args.lin_maxit)
+        ok = ist[:, 1] == 1
+        n_conv += int(ok.sum())
+        X, Y = WS.training_set(part, Us, ok)
+        Xs.append(X)
+        Ys.append(Y)
+        log("training data batch %d: %d/%d converged, %.1f s" % (b0 // 256, int(ok.sum()), len(part),
+                                                                time.perf_counter() - t0))
+    X, Y = np.concatenate(Xs), np.concatenate(Ys)
+    t_data = time.perf_counter() - t0
+    # 2. training on the GPU
+    net = G.MLP(n_freq=64, sigma=args.sigma, width=256, depth=5, n_out=18, seed=3)
+    net.set_data(X, Y)
+    t0 = time.perf_counter()
+    curve = []
+    done = 0
+    while done < args.train_steps:
+        k = min(2000, args.train_steps - done)
+        L = net.train(k, batch=args.train_batch, lr=1e-3, seed=3, loss_every=k)
+        done += k
+        curve.append(float(L[-1]))
+        log("training step %d loss %.3e (%.1f s)" % (done, curve[-1], time.perf_counter() - t0))
+    t_train = time.perf_counter() - t0
+    # 3. held-out instances
+    test = WS.slab_instances(G, args.instances, 4)
+    plain = [WS.plain_guess(G, it, 4000 + i) for i, it in enumerate(test)]
+    kw = dict(rtol=1e-10, lin_maxit=args.lin_maxit)
+    # plain guess, FP32 batched (the config-4 path on these instances)
+    t0 = time.perf_counter()
+    st_p, ist_p, F_p, _ = WS.solve_batch(G, test, plain, fp32=True, **kw)
+    t_plain32 = time.perf_counter() - t0
+    log("plain FP32 batched: %.2f s, %d converged" % (t_plain32, int(ist_p[:, 1].sum())))
+    # learned guess with fallback, FP32 batched; prediction and choice timed
+    t0 = time.perf_counter()
+    learned = WS.predict_guesses(net, test)
+    r_l = WS.first_residuals(G, test, learned)
+    r_p = WS.first_residuals(G, test, plain)
+    use = WS.choose(r_l, r_p)
+    guesses = [l if u else p for l, p, u in zip(learned, plain, use)]
+    st_w, ist_w, F_w, _ = WS.solve_batch(G, test, guesses, fp32=True, **kw)
+    t_warm32 = time.perf_counter() - t0
+    log("warm FP32 batched: %.2f s, %d converged, learned guess used for %d" % (t_warm32, int(ist_w[:, 1].sum()),
+                                                                            int(use.sum())))
+    # plain FP64 batched: the free-energy reference
+    st_64, ist_64, F_64, _ = WS.solve_batch(G, test, plain, fp32=False, **kw)
+    # plain FP64 drop-in, one instance at a time, on a seeded sample
+    ns = min(args.sequential, len(test))
+    sample = np.sort(np.random.default_rng(7).choice(len(test), ns, replace=False))
+    t0 = time.perf_counter()
+    seq_conv = 0
+    for k, i in enumerate(sample):
+        st1, ist1, F1, _ = WS.solve_batch(G, [test[i]], [plain[i]], fp32=False, **kw)
+        seq_conv += int(ist1[0, 1] == 1)
+    t_seq = time.perf_counter() - t0
+    log("plain FP64 sequential sample: %d instances %.1f s" % (ns, t_seq))
+    net.close()
+    rate = lambda n, t: 3600.0 * n / t if t > 0 else 0.0
+    conv_w, conv_p, conv_64 = ist_w[:, 1] == 1, ist_p[:, 1] == 1, ist_64[:, 1] == 1
+    both = conv_w & conv_64
+    rel = np.abs(F_w - F_64) / np.maximum(np.abs(F_64), 1e-300)
+    same = both & (rel <= 1e-10)
+    plain_rate = rate(seq_conv, t_seq)
+    line = {
+        "metric": METRIC, "value": rate(conv_w.sum(), t_warm32), "unit": "solves/hr", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": 1000.0 * t_warm32, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 MLP + f32/f64 solve", "data": "synthetic",
+        "config": {"workload": "learned warm start: coordinate MLP (64 Fourier frequencies, 5 x 256 SiLU, 18 "
+                               "outputs, BF16 with FP32 accumulation, Adam lr 1e-3, %d steps, batch %d) trained on "
+                               "%d slab solutions (seed 3); %d held-out instances (seed 4), batched FP32 "
+                               "Newton-GMRES, B-phase guess as fallback" % (args.train_steps, args.train_batch,
+                                                                            n_conv, len(test)),
+                   "name": "warm_start", "train_points": int(X.shape[0])},
+        "newton_steps_mean": {"learned": float(ist_w[:, 0].mean()), "plain": float(ist_p[:, 0].mean())},
+        "newton_steps_lockstep": {"learned": st_w["newton_steps"], "plain": st_p["newton_steps"]},
+        "gmres_its_lockstep": {"learned": st_w["gmres_its"], "plain": st_p["gmres_its"]},
+        "learned_guess_used": int(use.sum()),
+        "first_residual_median": {"learned": float(np.median(r_l)), "plain": float(np.median(r_p))},
+        "solves_hr": {"learned_fp32_batched": rate(conv_w.sum(), t_warm32),
+                      "plain_fp32_batched": rate(conv_p.sum(), t_plain32),
+                      "plain_fp64_sequential": plain_rate},
+        "speedup_vs_plain_fp64": rate(conv_w.sum(), t_warm32) / plain_rate if plain_rate else None,
+        "free_energy_vs_plain_fp64": {"compared": int(both.sum()), "within_1e-10": int(same.sum()),
+                                      "max_rel_err_within": float(rel[same].max()) if same.any() else None,
+                                      "lower_energy_state": int((both & (F_w < F_64) & ~same).sum())},
+        "training": {"data_s": t_data, "train_s": t_train, "loss_curve": curve},
+        "wall_s": time.perf_counter() - t_all,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cube_q2_ref5",
-                    choices=sorted(WORKLOADS) + ["slab_batch", "adaptive_slab"])
+                    choices=sorted(WORKLOADS) + ["slab_batch", "adaptive_slab", "warm_start"])
     ap.add_argument("--cycles", type=int, default=4, help="refinement cycles (adaptive_slab)")
+    ap.add_argument("--train-instances", type=int, default=4096, help="training solutions (warm_start)")
+    ap.add_argument("--train-steps", type=int, default=20000)
+    ap.add_argument("--train-batch", type=int, default=65536)
+    ap.add_argument("--sigma", type=float, default=0.5, help="Fourier feature scale (warm_start)")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--lin-maxit", type=int, default=3000)
     ap.add_argument("--sequential", type=int, default=256,
@@ -314,6 +434,9 @@ def main():
         return
     if args.workload == "adaptive_slab":
         run_adaptive(G, args)
+        return
+    if args.workload == "warm_start":
+        run_warm_start(G, args)
         return
     deg, ncell, half, p, t = WORKLOADS[args.workload]
     print("bench: building %s" % args.workload, file=sys.stderr, flush=True)

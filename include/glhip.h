@@ -155,6 +155,38 @@ int glh_run_adaptive(glh_ctx *ctx, const glh_forest *f0, const double *U0, int n
                      int damped, int fp32, int check_fp64, double *cycle_stats, glh_forest **f_out,
                      glh_mesh **m_out);
 
+/* ------------------------------------------- coordinate-MLP warm start */
+/* A small network that predicts the order parameter from (x, y, z, T/Tc, p,
+ * D), trained on converged solutions and used as the Newton initial guess
+ * (an addition beside the reference's seeded guesses; it has no
+ * counterpart there).  Inputs pass through Fourier features
+ * [sin, cos](2 pi x B), B a fixed 6 x n_freq matrix with N(0, sigma^2)
+ * entries; `depth` hidden layers of `width` units with SiLU; n_out linear
+ * outputs.  Training and inference run in hand-written BF16 kernels with
+ * FP32 accumulation (matrix cores); master weights and Adam moments are FP32.
+ * Constraints: width a multiple of 64, 2*n_freq a multiple of 64,
+ * n_out <= 64.  Inputs X are n x 6, targets and predictions n x n_out,
+ * row-major FP32 on the host. */
+typedef struct glh_mlp glh_mlp;
+int glh_mlp_create(int device, int n_freq, double sigma, int width, int depth, int n_out, uint64_t seed,
+                   glh_mlp **out);
+int glh_mlp_destroy(glh_mlp *m);
+const char *glh_mlp_last_error(const glh_mlp *m);
+int64_t glh_mlp_n_params(const glh_mlp *m);
+/* flat FP32 parameters: for layer l, W_l (K_l x N_l row-major, y = x W + b)
+ * then b_l; fourier: the 6 x n_freq matrix B (either may be NULL) */
+int glh_mlp_get_params(glh_mlp *m, float *params, float *fourier);
+int glh_mlp_set_params(glh_mlp *m, const float *params);
+int glh_mlp_set_data(glh_mlp *m, int64_t n, const float *X, const float *Y);
+/* `steps` Adam steps (beta 0.9/0.999, eps 1e-8) on random batches of
+ * `batch` points (a multiple of 64) of the data, mean-squared-error loss;
+ * loss_out[steps / loss_every] receives the batch loss every loss_every
+ * steps (may be NULL). */
+int glh_mlp_train(glh_mlp *m, int steps, int batch, double lr, uint64_t seed, int loss_every, float *loss_out);
+/* mean-squared error and its gradient (n_params values) at the given points */
+int glh_mlp_loss_grad(glh_mlp *m, int n, const float *X, const float *Y, float *loss, float *grad);
+int glh_mlp_predict(glh_mlp *m, int64_t n, const float *X, float *Y);
+
 /* ------------------------------------------------- per refinement cycle */
 /* Uploads geometry, cell->node map, constraint masks, colour sets and the
  * BSR pattern once per refinement cycle (setup_uniform_B-phase.cc:125-198).
@@ -212,6 +244,9 @@ int glh_newton_step(glh_ctx *ctx, double step, int damped, double out[4]);
  * instance. */
 int glh_free_energy(glh_ctx *ctx, double *F);
 int glh_instance_energy(glh_ctx *ctx, double *F);
+/* ||R|| of every instance at the current solution (residual.cc:109-297 per
+ * instance): R[n_inst]. */
+int glh_instance_residuals(glh_ctx *ctx, double *R);
 
 /* ------------------------------------------------------------ FemGL driver */
 /* The Newton loop of FemGL::run (run.cc:208-256) on the uploaded mesh:

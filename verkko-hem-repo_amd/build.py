@@ -26,7 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", o
 HIPFLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-ffp-contract=fast"]
 
-SOURCES = ["gl_assemble.hip", "gl_linalg.hip", "gl_constraints.hip", "glhip_api.cpp", "mesh.cpp", "amr.cpp",
+SOURCES = ["gl_assemble.hip", "gl_linalg.hip", "gl_constraints.hip", "mlp.hip", "mlp.cpp", "glhip_api.cpp", "mesh.cpp", "amr.cpp",
            "matep.cpp", "setup.cpp", "kelly.cpp", "driver.cpp"]
 
 

@@ -1490,6 +1490,17 @@ int glh_instance_energy(glh_ctx *c, double *F)
   return GLH_OK;
 }
 
+int glh_instance_residuals(glh_ctx *c, double *R)
+{
+  RCHK(check_ready(c));
+  if (!R) return fail(c, GLH_ERR_ARG, "null output");
+  RCHK(assemble_at(c, c->U.p, false, true, false, c->res.p, false));
+  std::vector<double> r2;
+  RCHK(seg_norm2(c, c->res.p, r2));
+  for (int i = 0; i < c->n_seg; ++i) R[i] = std::sqrt(r2[i]);
+  return GLH_OK;
+}
+
 int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, int lin_maxit, int restart,
                      double ls_step, int damped, int fp32, double stats[10])
 {

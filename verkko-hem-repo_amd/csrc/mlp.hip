@@ -1,0 +1,380 @@
+// Coordinate-MLP warm start on gfx950: hand-written BF16 kernels with FP32
+// accumulation for training (forward, backward, Adam) and inference.
+//
+// Network: x (6 inputs) -> Fourier features [sin, cos](2 pi x B), B a fixed
+// 6 x F matrix -> `depth` hidden layers of width W with SiLU -> n_out linear
+// outputs.  Activations and weights enter the matrix cores as BF16; every
+// product accumulates in FP32 (v_mfma_f32_16x16x32_bf16); master weights,
+// gradients and Adam moments are FP32.
+//
+// GEMM: C (FP32) = A B with A (M x K) and B (K x N) in BF16, either operand
+// contiguous along k or along its other index (template flags), 64 x 64
+// tiles, 4 wavefronts of 2 x 2 MFMA 16 x 16 blocks, 32-wide k steps staged
+// in LDS.  M, N multiples of 64 and K a multiple of 32 (the host pads).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mlp_dev.hpp"
+
+namespace glh {
+namespace mlp {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;  // LDS rows padded by 16 B
+
+__device__ inline bf16 to_bf16(float f)
+{
+  // round to nearest even on the upper 16 bits (finite inputs)
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return bf16{(uint16_t)(u >> 16)};
+}
+__device__ inline float from_bf16(bf16 h) { return __uint_as_float((uint32_t)h.bits << 16); }
+
+// A(m,k) = A[m*sam + k*sak] with A_K (sak == 1) or A_M (sam == 1);
+// B(k,n) = B[k*sbk + n*sbn] with B_K (sbk == 1) or B_N (sbn == 1).
+// C[m*ldc + n] = (accumulate ? C : 0) + sum_k A(m,k) B(k,n), over the k
+// range [blockIdx.z * kchunk, +kchunk), written to C + blockIdx.z * c_split.
+template <bool A_K, bool B_K>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int kchunk, const bf16 *__restrict__ A, long long sa,
+                                              const bf16 *__restrict__ B, long long sb, float *__restrict__ C,
+                                              int ldc, long long c_split, int accumulate)
+{
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM][LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][LDK];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int kbeg = blockIdx.z * kchunk;
+  const uint16_t *Ag = reinterpret_cast<const uint16_t *>(A);
+  const uint16_t *Bg = reinterpret_cast<const uint16_t *>(B);
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  for (int k0 = kbeg; k0 < kbeg + kchunk; k0 += BK) {
+    // stage A (64 x 32) and B (32 x 64) tiles, 8 values per thread each
+    if (A_K) {
+      const int m = t >> 2, kk = (t & 3) * 8;
+      const uint4 v = *reinterpret_cast<const uint4 *>(Ag + (size_t)(m0 + m) * sa + (k0 + kk));
+      *reinterpret_cast<uint4 *>(&As[m][kk]) = v;
+    } else {
+      const int kk = t >> 3, m = (t & 7) * 8;
+      const uint4 v = *reinterpret_cast<const uint4 *>(Ag + (size_t)(k0 + kk) * sa + (m0 + m));
+      const uint16_t *e = reinterpret_cast<const uint16_t *>(&v);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) As[m + i][kk] = e[i];
+    }
+    if (B_K) {
+      const int n = t >> 2, kk = (t & 3) * 8;
+      const uint4 v = *reinterpret_cast<const uint4 *>(Bg + (size_t)(n0 + n) * sb + (k0 + kk));
+      *reinterpret_cast<uint4 *>(&Bs[n][kk]) = v;
+    } else {
+      const int kk = t >> 3, n = (t & 7) * 8;
+      const uint4 v = *reinterpret_cast<const uint4 *>(Bg + (size_t)(k0 + kk) * sb + (n0 + n));
+      const uint16_t *e = reinterpret_cast<const uint16_t *>(&v);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Bs[n + i][kk] = e[i];
+    }
+    __syncthreads();
+    // lane l: row/col l % 16 of the 16 x 16 block, k = 8 (l / 16) .. + 7
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    bf16x8 a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const uint4 va = *reinterpret_cast<const uint4 *>(&As[wm + i * 16 + r][kq]);
+      a[i] = __builtin_bit_cast(bf16x8, va);
+      const uint4 vb = *reinterpret_cast<const uint4 *>(&Bs[wn + i * 16 + r][kq]);
+      b[i] = __builtin_bit_cast(bf16x8, vb);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // D layout: lane l holds rows 4 (l / 16) + q, column l % 16
+  float *Cz = C + (size_t)blockIdx.z * c_split;
+  const int cc = lane & 15, rq = (lane >> 4) * 4;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float *p = Cz + (size_t)(m0 + wm + i * 16 + rq + q) * ldc + (n0 + wn + j * 16 + cc);
+        *p = accumulate ? *p + acc[i][j][q] : acc[i][j][q];
+      }
+}
+
+constexpr int ET = 256;
+
+__global__ __launch_bounds__(ET) void k_fourier(int n, int n_real, const float *__restrict__ X,
+                                                const float *__restrict__ Bf, int F, bf16 *__restrict__ H)
+{
+  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
+  if (i >= (long long)n * F) return;
+  const int s = (int)(i / F), j = (int)(i % F);
+  float a = 0.f;
+  if (s < n_real)
+    for (int d = 0; d < 6; ++d) a += X[(size_t)s * 6 + d] * Bf[d * F + j];
+  const float ang = 6.283185307179586f * a;
+  H[(size_t)s * 2 * F + j] = to_bf16(s < n_real ? sinf(ang) : 0.f);
+  H[(size_t)s * 2 * F + F + j] = to_bf16(s < n_real ? cosf(ang) : 0.f);
+}
+
+// Z += bias; H = bf16(SiLU(Z)) (Z kept for the backward pass)
+__global__ __launch_bounds__(ET) void k_bias_silu(long long total, int N, float *__restrict__ Z,
+                                                  const float *__restrict__ bias, bf16 *__restrict__ H)
+{
+  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
+  if (i >= total) return;
+  const float z = Z[i] + bias[i % N];
+  Z[i] = z;
+  H[i] = to_bf16(z / (1.f + expf(-z)));
+}
+
+// output layer: Y[s, o] += bias[o] (o < n_out; padded columns untouched)
+__global__ __launch_bounds__(ET) void k_bias_out(int n, int n_out, int ldy, float *__restrict__ Y,
+                                                 const float *__restrict__ bias)
+{
+  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
+  if (i >= (long long)n * n_out) return;
+  const int s = (int)(i / n_out), o = (int)(i % n_out);
+  Y[(size_t)s * ldy + o] += bias[o];
+}
+
+// mean-squared error over the n_real x n_out real outputs: dY = bf16(2 (Y - T)
+// / (n_real n_out)) (0 in the padding), per-block partial sums of (Y - T)^2
+__global__ __launch_bounds__(ET) void k_mse(int n, int n_real, int n_out, int ldy, const float *__restrict__ Y,
+                                            const float *__restrict__ T, bf16 *__restrict__ dY,
+                                            float *__restrict__ partial)
+{
+  __shared__ float sh[ET];
+  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
+  float e2 = 0.f;
+  if (i < (long long)n * ldy) {
+    const int s = (int)(i / ldy), o = (int)(i % ldy);
+    float g = 0.f;
+    if (s < n_real && o < n_out) {
+      const float d = Y[i] - T[(size_t)s * n_out + o];
+      e2 = d * d;
+      g = 2.f * d / ((float)n_real * (float)n_out);
+    }
+    dY[i] = to_bf16(g);
+  }
+  sh[threadIdx.x] = e2;
+  __syncthreads();
+  for (int st = ET / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// dZ = bf16(dH * SiLU'(Z))
+__global__ __launch_bounds__(ET) void k_silu_back(long long total, const float *__restrict__ dH,
+                                                  const float *__restrict__ Z, bf16 *__restrict__ dZ)
+{
+  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
+  if (i >= total) return;
+  const float z = Z[i], sg = 1.f / (1.f + expf(-z));
+  dZ[i] = to_bf16(dH[i] * sg * (1.f + z * (1.f - sg)));
+}
+
+// column sums of a BF16 n x N matrix, FP32, in fixed order: partial[blk][col]
+__global__ __launch_bounds__(ET) void k_colsum_partial(int n, int N, int rows_per_blk, const bf16 *__restrict__ D,
+                                                       float *__restrict__ partial)
+{
+  const int col = threadIdx.x + blockIdx.y * ET;
+  if (col >= N) return;
+  const int r0 = blockIdx.x * rows_per_blk, r1 = min(n, r0 + rows_per_blk);
+  float s = 0.f;
+  for (int r = r0; r < r1; ++r) s += from_bf16(D[(size_t)r * N + col]);
+  partial[(size_t)blockIdx.x * N + col] = s;
+}
+
+// out[i] = sum_p partial[p*len + i] (fixed order)
+__global__ __launch_bounds__(ET) void k_sum_parts(int parts, long long len, const float *__restrict__ partial,
+                                                  float *__restrict__ out)
+{
+  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
+  if (i >= len) return;
+  float s = 0.f;
+  for (int p = 0; p < parts; ++p) s += partial[(size_t)p * len + i];
+  out[i] = s;
+}
+
+// Adam on the FP32 master parameters (bias-corrected, step t >= 1)
+__global__ __launch_bounds__(ET) void k_adam(long long n, float *__restrict__ p, const float *__restrict__ g,
+                                             float *__restrict__ m, float *__restrict__ v, float lr, float b1,
+                                             float b2, float eps, const int *__restrict__ step)
+{
+  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
+  if (i >= n) return;
+  const int t = step[0];
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi, vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float mh = mi / (1.f - powf(b1, (float)t)), vh = vi / (1.f - powf(b2, (float)t));
+  p[i] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// BF16 copy of a K x N FP32 weight into a K x Np padded matrix (zeros beyond N)
+__global__ __launch_bounds__(ET) void k_pack_w(int K, int N, int Np, const float *__restrict__ W,
+                                               bf16 *__restrict__ Wb)
+{
+  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
+  if (i >= (long long)K * Np) return;
+  const int k = (int)(i / Np), c = (int)(i % Np);
+  Wb[i] = to_bf16(c < N ? W[(size_t)k * N + c] : 0.f);
+}
+
+// random training batch: sample i of step t is point hash(seed, t, i) mod n_data
+__device__ inline uint64_t mix64(uint64_t z)
+{
+  z += 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+__global__ __launch_bounds__(ET) void k_gather(int nb, long long n_data, uint64_t seed, const int *__restrict__ step,
+                                               const float *__restrict__ X, const float *__restrict__ Y, int n_out,
+                                               float *__restrict__ Xb, float *__restrict__ Yb)
+{
+  const int i = blockIdx.x * ET + threadIdx.x;
+  if (i >= nb) return;
+  const uint64_t h = mix64(seed ^ mix64(((uint64_t)(uint32_t)step[0] << 32) | (uint32_t)i));
+  const long long j = (long long)(h % (uint64_t)n_data);
+  for (int d = 0; d < 6; ++d) Xb[(size_t)i * 6 + d] = X[(size_t)j * 6 + d];
+  for (int o = 0; o < n_out; ++o) Yb[(size_t)i * n_out + o] = Y[(size_t)j * n_out + o];
+}
+
+// dst (K x N) = the first N columns of src (K x Np)
+__global__ __launch_bounds__(ET) void k_unpad(int K, int N, int Np, const float *__restrict__ src,
+                                              float *__restrict__ dst)
+{
+  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
+  if (i >= (long long)K * N) return;
+  const int k = (int)(i / N), c = (int)(i % N);
+  dst[i] = src[(size_t)k * Np + c];
+}
+
+// Y[s, o] = Yp[s*ldy + o] for s < n, o < n_out
+__global__ __launch_bounds__(ET) void k_take_out(int n, int n_out, int ldy, const float *__restrict__ Yp,
+                                                 float *__restrict__ Y)
+{
+  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
+  if (i >= (long long)n * n_out) return;
+  Y[i] = Yp[(size_t)(i / n_out) * ldy + (i % n_out)];
+}
+
+__global__ void k_step_inc(int *step)
+{
+  if (threadIdx.x == 0) step[0] += 1;
+}
+
+inline unsigned grid1(long long n) { return (unsigned)((n + ET - 1) / ET); }
+
+}  // namespace
+
+hipError_t gemm(int M, int N, int K, const bf16 *A, bool a_kcontig, long long sa, const bf16 *B, bool b_kcontig,
+                long long sb, float *C, int ldc, bool accumulate, int splits, long long c_split, hipStream_t s)
+{
+  if (M % BM || N % BN || K % (BK * splits)) return hipErrorInvalidValue;
+  const dim3 grid(N / BN, M / BM, splits);
+  const int kchunk = K / splits;
+  const int acc = accumulate ? 1 : 0;
+  if (a_kcontig && b_kcontig)
+    hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(256), 0, s, M, N, kchunk, A, sa, B, sb, C, ldc, c_split, acc);
+  else if (a_kcontig)
+    hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(256), 0, s, M, N, kchunk, A, sa, B, sb, C, ldc, c_split, acc);
+  else if (b_kcontig)
+    hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(256), 0, s, M, N, kchunk, A, sa, B, sb, C, ldc, c_split, acc);
+  else
+    hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(256), 0, s, M, N, kchunk, A, sa, B, sb, C, ldc, c_split,
+                       acc);
+  return hipGetLastError();
+}
+
+hipError_t fourier(int n, int n_real, const float *X, const float *Bf, int F, bf16 *H, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_fourier, dim3(grid1((long long)n * F)), dim3(ET), 0, s, n, n_real, X, Bf, F, H);
+  return hipGetLastError();
+}
+hipError_t bias_silu(long long total, int N, float *Z, const float *bias, bf16 *H, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_bias_silu, dim3(grid1(total)), dim3(ET), 0, s, total, N, Z, bias, H);
+  return hipGetLastError();
+}
+hipError_t bias_out(int n, int n_out, int ldy, float *Y, const float *bias, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_bias_out, dim3(grid1((long long)n * n_out)), dim3(ET), 0, s, n, n_out, ldy, Y, bias);
+  return hipGetLastError();
+}
+hipError_t mse(int n, int n_real, int n_out, int ldy, const float *Y, const float *T, bf16 *dY, float *partial,
+               hipStream_t s)
+{
+  hipLaunchKernelGGL(k_mse, dim3(grid1((long long)n * ldy)), dim3(ET), 0, s, n, n_real, n_out, ldy, Y, T, dY,
+                     partial);
+  return hipGetLastError();
+}
+int mse_blocks(int n, int ldy) { return (int)grid1((long long)n * ldy); }
+hipError_t silu_back(long long total, const float *dH, const float *Z, bf16 *dZ, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_silu_back, dim3(grid1(total)), dim3(ET), 0, s, total, dH, Z, dZ);
+  return hipGetLastError();
+}
+hipError_t colsum(int n, int N, const bf16 *D, float *partial, int parts, float *out, hipStream_t s)
+{
+  const int rows = (n + parts - 1) / parts;
+  hipLaunchKernelGGL(k_colsum_partial, dim3(parts, (N + ET - 1) / ET), dim3(ET), 0, s, n, N, rows, D, partial);
+  hipLaunchKernelGGL(k_sum_parts, dim3(grid1(N)), dim3(ET), 0, s, parts, (long long)N, partial, out);
+  return hipGetLastError();
+}
+hipError_t sum_parts(int parts, long long len, const float *partial, float *out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sum_parts, dim3(grid1(len)), dim3(ET), 0, s, parts, len, partial, out);
+  return hipGetLastError();
+}
+hipError_t adam(long long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
+                const int *step, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_adam, dim3(grid1(n)), dim3(ET), 0, s, n, p, g, m, v, lr, b1, b2, eps, step);
+  return hipGetLastError();
+}
+hipError_t pack_w(int K, int N, int Np, const float *W, bf16 *Wb, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_pack_w, dim3(grid1((long long)K * Np)), dim3(ET), 0, s, K, N, Np, W, Wb);
+  return hipGetLastError();
+}
+hipError_t gather(int nb, long long n_data, uint64_t seed, const int *step, const float *X, const float *Y, int n_out,
+                  float *Xb, float *Yb, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_gather, dim3(grid1(nb)), dim3(ET), 0, s, nb, n_data, seed, step, X, Y, n_out, Xb, Yb);
+  return hipGetLastError();
+}
+hipError_t unpad(int K, int N, int Np, const float *src, float *dst, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_unpad, dim3(grid1((long long)K * N)), dim3(ET), 0, s, K, N, Np, src, dst);
+  return hipGetLastError();
+}
+hipError_t take_out(int n, int n_out, int ldy, const float *Yp, float *Y, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_take_out, dim3(grid1((long long)n * n_out)), dim3(ET), 0, s, n, n_out, ldy, Yp, Y);
+  return hipGetLastError();
+}
+hipError_t step_inc(int *step, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, s, step);
+  return hipGetLastError();
+}
+
+}  // namespace mlp
+}  // namespace glh

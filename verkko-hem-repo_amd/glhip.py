@@ -32,7 +32,9 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_mesh_constraint_sizes", "glh_mesh_constraints", "glh_forest_create", "glh_forest_destroy",
            "glh_forest_n_cells", "glh_forest_refine", "glh_forest_mesh", "glh_forest_transfer", "glh_kelly",
            "glh_flag_fixed_number", "glh_initial_guess_bna", "glh_run_adaptive",
-           "glh_set_continuation"]
+           "glh_set_continuation", "glh_mlp_create", "glh_mlp_destroy", "glh_mlp_last_error", "glh_mlp_n_params",
+           "glh_mlp_get_params", "glh_mlp_set_params", "glh_mlp_set_data", "glh_mlp_train", "glh_mlp_loss_grad",
+           "glh_mlp_predict", "glh_instance_residuals"]
 
 _lib = None
 
@@ -61,6 +63,7 @@ def lib():
     L.glh_mesh_concat.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
     L.glh_mesh_instances.argtypes = [vp, _ip, _ip]
     L.glh_instance_energy.argtypes = [vp, _dp]
+    L.glh_instance_residuals.argtypes = [vp, _dp]
     L.glh_instance_stats.argtypes = [vp, _dp]
     L.glh_mesh_sizes.argtypes = [vp, _i64p]
     L.glh_mesh_get.argtypes = [vp] + [vp] * 9
@@ -82,6 +85,18 @@ def lib():
     L.glh_set_timing.argtypes = [vp, C.c_int]
     L.glh_set_verbose.argtypes = [vp, C.c_int]
     L.glh_set_continuation.argtypes = [vp, C.c_double]
+    L.glh_mlp_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]
+    L.glh_mlp_destroy.argtypes = [vp]
+    L.glh_mlp_last_error.argtypes = [vp]
+    L.glh_mlp_last_error.restype = C.c_char_p
+    L.glh_mlp_n_params.argtypes = [vp]
+    L.glh_mlp_n_params.restype = C.c_int64
+    L.glh_mlp_get_params.argtypes = [vp, vp, vp]
+    L.glh_mlp_set_params.argtypes = [vp, _f32p]
+    L.glh_mlp_set_data.argtypes = [vp, C.c_int64, _f32p, _f32p]
+    L.glh_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int, vp]
+    L.glh_mlp_loss_grad.argtypes = [vp, C.c_int, _f32p, _f32p, C.POINTER(C.c_float), vp]
+    L.glh_mlp_predict.argtypes = [vp, C.c_int64, _f32p, _f32p]
     L.glh_timing.argtypes = [vp, _dp]
     L.glh_bench_spmv.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.glh_bench_copy.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
@@ -262,6 +277,73 @@ class Forest:
             pass
 
 
+class MLP:
+    """Coordinate MLP (glh_mlp): Fourier features of (x, y, z, T, p, D), SiLU
+    hidden layers, trained and evaluated by BF16 matrix-core kernels."""
+
+    def __init__(self, n_freq=64, sigma=4.0, width=256, depth=5, n_out=18, seed=0, device=0):
+        self.h = C.c_void_p()
+        rc = lib().glh_mlp_create(int(device), int(n_freq), float(sigma), int(width), int(depth), int(n_out),
+                                  int(seed), C.byref(self.h))
+        if rc:
+            raise GLHError("glh_mlp_create failed (%d)" % rc)
+        self.n_freq, self.width, self.depth, self.n_out = n_freq, width, depth, n_out
+        self.n_params = int(lib().glh_mlp_n_params(self.h))
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = lib().glh_mlp_last_error(self.h)
+            raise GLHError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        b = np.zeros(6 * self.n_freq, np.float32)
+        self._chk(lib().glh_mlp_get_params(self.h, p.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p)),
+                  "glh_mlp_get_params")
+        return p, b.reshape(6, self.n_freq)
+
+    def set_params(self, p):
+        self._chk(lib().glh_mlp_set_params(self.h, np.ascontiguousarray(p, np.float32)), "glh_mlp_set_params")
+
+    def set_data(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        self._chk(lib().glh_mlp_set_data(self.h, X.shape[0], X.reshape(-1), Y.reshape(-1)), "glh_mlp_set_data")
+
+    def train(self, steps, batch=65536, lr=1e-3, seed=0, loss_every=100):
+        n = steps // loss_every if loss_every else 0
+        L = np.zeros(max(1, n), np.float32)
+        self._chk(lib().glh_mlp_train(self.h, int(steps), int(batch), float(lr), int(seed), int(loss_every),
+                                      L.ctypes.data_as(C.c_void_p)), "glh_mlp_train")
+        return L[:n]
+
+    def loss_grad(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        L = C.c_float()
+        self._chk(lib().glh_mlp_loss_grad(self.h, X.shape[0], X.reshape(-1), Y.reshape(-1), C.byref(L),
+                                          g.ctypes.data_as(C.c_void_p)), "glh_mlp_loss_grad")
+        return float(L.value), g
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], self.n_out), np.float32)
+        self._chk(lib().glh_mlp_predict(self.h, X.shape[0], X.reshape(-1), Y.reshape(-1)), "glh_mlp_predict")
+        return Y
+
+    def close(self):
+        if self.h:
+            lib().glh_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def flag_fixed_number(eta, frac):
     eta = np.ascontiguousarray(eta, np.float32)
     fl = np.zeros(eta.size, np.uint8)
@@ -307,6 +389,9 @@ class Context:
 
     def instance_energy(self):
         return self._get("glh_instance_energy", self.n_inst)
+
+    def instance_residuals(self):
+        return self._get("glh_instance_residuals", self.n_inst)
 
     def instance_stats(self):
         out = np.zeros(self.n_inst * 8)
