@@ -275,6 +275,16 @@ int glh_instance_stats(glh_ctx *ctx, double *out);
  * (HIP events): ms[0] = J+R assembly ms, ms[1] = number of launches. */
 int glh_last_assembly_ms(glh_ctx *ctx, double ms[2]);
 /* level > 0: one progress line per Newton step on stderr (glh_newton_solve). */
+/* Forcing terms of glh_newton_solve: 0 (default) the fixed lin_tol of the
+ * reference (solve.cc:119-128); 1 Eisenstat-Walker (choice 2: eta_k =
+ * 0.9 (||R_k|| / ||R_k-1||)^2 with the usual safeguard), capped at lin_tol,
+ * per instance. */
+int glh_set_forcing(glh_ctx *ctx, int eisenstat_walker);
+/* Stopping reference of glh_newton_solve: with n = n_inst norms, instance s
+ * stops at ||R_s|| <= rtol * r_ref[s] instead of rtol * ||R0_s|| (so solves
+ * from different initial guesses reach the same absolute accuracy); n = 0
+ * restores the default.  Cleared by glh_upload_mesh. */
+int glh_set_residual_reference(glh_ctx *ctx, int n, const double *r_ref);
 /* Pseudo-transient continuation in glh_newton_solve (an addition to the
  * reference's damped Newton, iteration.cc:109-215): every Newton matrix
  * becomes J + sigma_k |diag J| on the unconstrained dofs with

@@ -150,6 +150,8 @@ struct glh_ctx {
   bool timing = false;
   int verbose = 0;
   double ptc_sigma0 = 0.0;  // pseudo-transient continuation in glh_newton_solve (0: off)
+  bool ew_forcing = false;  // Eisenstat-Walker forcing terms in glh_newton_solve
+  std::vector<double> r_ref;  // per-instance stopping reference (empty: ||R0|| of the solve)
   long reorth_count = 0;
   std::vector<hipEvent_t> ev_pool;
   int ev_used = 0;
@@ -722,10 +724,13 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
 
 // J dU = rhs per instance to ||r_s|| <= rtol ||rhs_s||.  stats: [0] lockstep
 // iterations, [1] worst relative residual.
-int solve_impl(glh_ctx *c, double rtol, int maxit, int restart, double stats[2], std::vector<int> *its_seg_out)
+int solve_impl(glh_ctx *c, const std::vector<double> &rtolv, int maxit, int restart, double stats[2],
+               std::vector<int> *its_seg_out)
 {
   if (!c->j_valid) return fail(c, GLH_ERR_STATE, "no Jacobian assembled");
-  if (restart < 1 || maxit < 1 || !(rtol > 0)) return fail(c, GLH_ERR_ARG, "bad solver parameters");
+  if (restart < 1 || maxit < 1 || (int)rtolv.size() != c->n_seg) return fail(c, GLH_ERR_ARG, "bad solver parameters");
+  for (double r : rtolv)
+    if (!(r > 0)) return fail(c, GLH_ERR_ARG, "bad solver parameters");
   RCHK(ensure_precond(c));
   const size_t n = ndofs(c);
   const int S = c->n_seg;
@@ -734,8 +739,7 @@ int solve_impl(glh_ctx *c, double rtol, int maxit, int restart, double stats[2],
   std::vector<double> rel;
   std::vector<int> its_seg(S, 0);
   if (!c->j_fp32) {
-    rc = gmres<double, double>(c, c->J64.p, c->D64.p, c->rhs.p, c->dU.p, std::vector<double>(S, rtol), maxit, restart,
-                               &its, rel, its_seg);
+    rc = gmres<double, double>(c, c->J64.p, c->D64.p, c->rhs.p, c->dU.p, rtolv, maxit, restart, &its, rel, its_seg);
   } else {
     // FP32 GMRES inside FP64 iterative refinement, per instance:
     //   r = rhs - J32 dU (FP64 arithmetic), solve J32 c = r in FP32, dU += c
@@ -748,16 +752,16 @@ int solve_impl(glh_ctx *c, double rtol, int maxit, int restart, double stats[2],
     RCHK(seg_norm2(c, c->rhs.p, bn2));
     HIPCHK(c, hipMemcpyAsync(c->ir_r.p, c->rhs.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     rn2 = bn2;
-    const double inner = std::max(rtol, 1e-4);
     std::vector<int> flag(S);
     for (int sweep = 0; sweep < 20 && its < maxit; ++sweep) {
       bool any = false;
       std::vector<double> rt(S, 1.0);
       for (int i = 0; i < S; ++i) {
         const double bn = std::sqrt(bn2[i]), rn = std::sqrt(rn2[i]);
-        flag[i] = rn > rtol * bn;
+        const double inner = std::max(rtolv[i], 1e-4);
+        flag[i] = rn > rtolv[i] * bn;
         any = any || flag[i];
-        if (flag[i]) rt[i] = std::max(inner, rtol * bn / rn);
+        if (flag[i]) rt[i] = std::max(inner, rtolv[i] * bn / rn);
       }
       if (!any) break;
       HIPCHK(c, hipMemcpyAsync(c->s_flag.p, flag.data(), S * sizeof(int), hipMemcpyHostToDevice, s));
@@ -780,7 +784,7 @@ int solve_impl(glh_ctx *c, double rtol, int maxit, int restart, double stats[2],
     for (int i = 0; i < S; ++i) {
       const double bn = std::sqrt(bn2[i]), rn = std::sqrt(rn2[i]);
       rel[i] = bn > 0 ? rn / bn : 0.0;
-      ok = ok && rel[i] <= rtol;
+      ok = ok && rel[i] <= rtolv[i];
     }
     rc = ok ? GLH_OK : fail(c, GLH_ERR_NOCONV, "mixed-precision GMRES did not reach the tolerance");
   }
@@ -1278,6 +1282,7 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
   c->node_off = m.node_off.empty() ? std::vector<int>{0, m.n_nodes} : m.node_off;
   c->cell_off = m.cell_off.empty() ? std::vector<int>{0, m.n_cells} : m.cell_off;
   c->n_seg = (int)c->node_off.size() - 1;
+  c->r_ref.clear();
   // per-cell face codes (boundary id + 1, 3 bits per local face)
   std::vector<uint32_t> cface(m.n_cells, 0u);
   for (size_t f = 0; f + 2 < m.faces.size(); f += 3) {
@@ -1447,7 +1452,7 @@ int glh_residual(glh_ctx *c, double *nrm)
 int glh_solve(glh_ctx *c, double rtol, int maxit, int restart, double stats[2])
 {
   RCHK(check_ready(c));
-  return solve_impl(c, rtol, maxit, restart, stats, nullptr);
+  return solve_impl(c, std::vector<double>(c->n_seg, rtol), maxit, restart, stats, nullptr);
 }
 
 int glh_newton_step(glh_ctx *c, double step, int damped, double out[4])
@@ -1528,14 +1533,18 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
   constexpr int STALL_WINDOW = 10;
   std::vector<double> hist;  // [k][S] ||R|| after step k
   hist.insert(hist.end(), rcur.begin(), rcur.end());
+  // stopping reference: ||R0|| of this solve, or the caller's per-instance norms
+  std::vector<double> rref = r0;
+  if ((int)c->r_ref.size() == S) rref = c->r_ref;
+  std::vector<double> rprev(S, 0.0), eta_prev(S, lin_tol);
   for (k = 0; k <= max_newton; ++k) {
     bool any = false;
     const double tnow = now_s() - t_start;
     for (int i = 0; i < S; ++i) {
-      if (S > 1 && k >= STALL_WINDOW && rcur[i] > rtol * r0[i] &&
+      if (S > 1 && k >= STALL_WINDOW && rcur[i] > rtol * rref[i] &&
           rcur[i] > 0.9 * hist[(size_t)(k - STALL_WINDOW) * S + i])
         stalled[i] = 1;
-      active[i] = !stalled[i] && rcur[i] > rtol * r0[i];
+      active[i] = !stalled[i] && rcur[i] > rtol * rref[i];
       if (!active[i] && st[8 * i + 4] < 0) st[8 * i + 4] = tnow;
       any = any || active[i];
     }
@@ -1568,7 +1577,21 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
     std::vector<int> its_seg;
     // an inexact solve that hit lin_maxit still gives a usable update: the
     // line search below decides whether it lowers ||R||
-    const int src = solve_impl(c, lin_tol, lin_maxit, restart, sst, &its_seg);
+    // forcing terms: the fixed lin_tol of the reference (solve.cc:119-128), or
+    // Eisenstat-Walker (choice 2, gamma 0.9, alpha 2) capped at lin_tol
+    std::vector<double> eta(S, lin_tol);
+    if (c->ew_forcing)
+      for (int i = 0; i < S; ++i) {
+        if (k == 0 || !(rprev[i] > 0)) continue;
+        double e = 0.9 * (rcur[i] / rprev[i]) * (rcur[i] / rprev[i]);
+        const double safe = 0.9 * eta_prev[i] * eta_prev[i];
+        if (safe > 0.1) e = std::max(e, safe);
+        // no tighter than needed to reach the stopping test this step
+        e = std::max(e, 0.5 * rtol * rref[i] / std::max(rcur[i], 1e-300));
+        eta[i] = std::min(lin_tol, std::max(e, 1e-8));
+      }
+    eta_prev = eta;
+    const int src = solve_impl(c, eta, lin_maxit, restart, sst, &its_seg);
     if (src == GLH_ERR_NOCONV) {
       RCHK(distribute(c, c->dU.p));
     } else if (src) {
@@ -1583,6 +1606,7 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
     RCHK(newton_step_impl(c, ls_step, damped, active, prev, cur, alpha, trials));
     for (int i = 0; i < S; ++i)
       if (active[i]) {
+        rprev[i] = rcur[i];
         rcur[i] = cur[i];
         st[8 * i + 0] += 1;
         st[8 * i + 6] += trials[i];
@@ -1611,7 +1635,7 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
   bool all = true;
   double r0t = 0, rft = 0, Ft = 0;
   for (int i = 0; i < S; ++i) {
-    const bool ok = rcur[i] <= rtol * r0[i];
+    const bool ok = rcur[i] <= rtol * rref[i];
     all = all && ok;
     st[8 * i + 1] = ok ? 1.0 : 0.0;
     st[8 * i + 3] = rcur[i];
@@ -1640,6 +1664,25 @@ int glh_last_assembly_ms(glh_ctx *c, double ms[2])
   if (!c || !ms) return GLH_ERR_ARG;
   ms[0] = c->asm_ms;
   ms[1] = c->asm_launches;
+  return GLH_OK;
+}
+
+int glh_set_forcing(glh_ctx *c, int eisenstat_walker)
+{
+  if (!c) return GLH_ERR_ARG;
+  c->ew_forcing = eisenstat_walker != 0;
+  return GLH_OK;
+}
+
+int glh_set_residual_reference(glh_ctx *c, int n, const double *r_ref)
+{
+  if (!c || n < 0 || (n > 0 && !r_ref)) return GLH_ERR_ARG;
+  c->r_ref.assign(r_ref, r_ref + n);
+  for (double r : c->r_ref)
+    if (!(r > 0)) {
+      c->r_ref.clear();
+      return fail(c, GLH_ERR_ARG, "reference residual norms must be positive");
+    }
   return GLH_OK;
 }
 

@@ -34,7 +34,7 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_flag_fixed_number", "glh_initial_guess_bna", "glh_run_adaptive",
            "glh_set_continuation", "glh_mlp_create", "glh_mlp_destroy", "glh_mlp_last_error", "glh_mlp_n_params",
            "glh_mlp_get_params", "glh_mlp_set_params", "glh_mlp_set_data", "glh_mlp_train", "glh_mlp_loss_grad",
-           "glh_mlp_predict", "glh_instance_residuals"]
+           "glh_mlp_predict", "glh_instance_residuals", "glh_set_forcing", "glh_set_residual_reference"]
 
 _lib = None
 
@@ -85,6 +85,8 @@ def lib():
     L.glh_set_timing.argtypes = [vp, C.c_int]
     L.glh_set_verbose.argtypes = [vp, C.c_int]
     L.glh_set_continuation.argtypes = [vp, C.c_double]
+    L.glh_set_forcing.argtypes = [vp, C.c_int]
+    L.glh_set_residual_reference.argtypes = [vp, C.c_int, vp]
     L.glh_mlp_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]
     L.glh_mlp_destroy.argtypes = [vp]
     L.glh_mlp_last_error.argtypes = [vp]
@@ -495,6 +497,17 @@ class Context:
 
     def set_verbose(self, level=1):
         self._chk(lib().glh_set_verbose(self.h, int(level)), "glh_set_verbose")
+
+    def set_forcing(self, eisenstat_walker):
+        self._chk(lib().glh_set_forcing(self.h, int(bool(eisenstat_walker))), "glh_set_forcing")
+
+    def set_residual_reference(self, r_ref=None):
+        if r_ref is None:
+            self._chk(lib().glh_set_residual_reference(self.h, 0, None), "glh_set_residual_reference")
+        else:
+            r = np.ascontiguousarray(r_ref, np.float64)
+            self._chk(lib().glh_set_residual_reference(self.h, r.size, r.ctypes.data_as(C.c_void_p)),
+                      "glh_set_residual_reference")
 
     def set_continuation(self, sigma0):
         """Pseudo-transient continuation (J + sigma0 ||R||/||R0|| |diag J|); 0: off."""

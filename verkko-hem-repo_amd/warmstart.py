@@ -57,14 +57,20 @@ def plain_guess(G, inst, seed):
 
 
 def solve_batch(G, insts, guesses, fp32=True, rtol=1e-10, lin_tol=0.1, lin_maxit=3000, restart=100,
-                max_newton=60, verbose=0):
-    """Batched Newton solve; returns (stats, per-instance stats, F, U list, first residuals)."""
+                max_newton=60, verbose=0, ew=False, r_ref=None):
+    """Batched Newton solve; returns (stats, per-instance stats, F, U list).
+    ew: Eisenstat-Walker forcing; r_ref: per-instance stopping reference
+    (||R|| <= rtol r_ref), e.g. the first residuals of the plain guesses, so
+    that different guesses are held to the same absolute accuracy."""
     ctx = G.Context(0)
     try:
         ctx.set_verbose(verbose)
         batch = G.Mesh.concat([it.mesh for it in insts])
         ctx.upload_mesh(batch)
         ctx.set_materials(np.array([it.coef for it in insts]))
+        ctx.set_forcing(ew)
+        if r_ref is not None:
+            ctx.set_residual_reference(r_ref)
         ctx.set_solution(np.concatenate(guesses))
         st = ctx.newton_solve(rtol=rtol, max_newton=max_newton, lin_tol=lin_tol, lin_maxit=lin_maxit,
                               restart=restart, fp32=fp32)
