@@ -148,6 +148,7 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
 
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
+  if (Mh.inst_active && !Mh.inst_active[Mh.cell_inst[e]]) return;  // finished instance of a batch
   const Material M = Mh.mats ? Mh.mats[Mh.cell_mat[e]] : M0;  // per-instance (T, p) in a batch
   const int *cn = Mh.cell_nodes + (size_t)e * NS;
   double h[3], ih[3];

@@ -151,6 +151,7 @@ struct glh_ctx {
   int verbose = 0;
   double ptc_sigma0 = 0.0;  // pseudo-transient continuation in glh_newton_solve (0: off)
   bool ew_forcing = false;  // Eisenstat-Walker forcing terms in glh_newton_solve
+  const int *asm_active = nullptr;  // device flags: assemble only these instances (inside glh_newton_solve)
   std::vector<double> r_ref;  // per-instance stopping reference (empty: ||R0|| of the solve)
   long reorth_count = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -167,7 +168,7 @@ struct glh_ctx {
     const bool per = !mats_host.empty();
     return glh::MeshDev{per ? cell_mat.p : nullptr, per ? mats.p : nullptr, cell_nodes.p, cell_h.p,
                         cell_face.p,          node_mask.p,                  cell_blk.p,
-                        constrained ? hdiag.p : nullptr};
+                        constrained ? hdiag.p : nullptr, cell_mat.p, asm_active};
   }
   glh::SegInfo seg() const { return glh::SegInfo{chunks.p, seg_chunk.p, n_chunks, n_seg}; }
 };
@@ -1510,6 +1511,11 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
                      double ls_step, int damped, int fp32, double stats[10])
 {
   RCHK(check_ready(c));
+  // the instance filter of the assembly lives only inside this solve
+  struct FilterReset {
+    glh_ctx *c;
+    ~FilterReset() { c->asm_active = nullptr; }
+  } filter_reset{c};
   const int S = c->n_seg;
   double t_asm = 0, t_sol = 0, t_ls = 0;
   int lin_total = 0, k = 0, trials_total = 0;
@@ -1550,12 +1556,17 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
     }
     if (!any || k == max_newton) break;
     t0 = now_s();
+    // finished instances are skipped by the assembly kernels from here on:
+    // their rows keep the last Jacobian and get a zero right-hand side
+    for (int i = 0; i < S; ++i) flag[i] = active[i];
+    HIPCHK(c, hipMemcpyAsync(c->s_flag.p, flag.data(), S * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    c->asm_active = S > 1 ? c->s_flag.p : nullptr;
     RCHK(assemble_at(c, c->U.p, true, true, f32, c->rhs.p, false));
     // pseudo-transient continuation: J + sigma_k |diag J|, sigma_k =
     // sigma0 ||R_k|| / ||R_0|| per instance (switched evolution relaxation)
     if (c->ptc_sigma0 > 0) {
       std::vector<double> sg(S);
-      for (int i = 0; i < S; ++i) sg[i] = c->ptc_sigma0 * rcur[i] / std::max(r0[i], 1e-300);
+      for (int i = 0; i < S; ++i) sg[i] = active[i] ? c->ptc_sigma0 * rcur[i] / std::max(r0[i], 1e-300) : 0.0;
       HIPCHK(c, c->s_norm.alloc(S));
       HIPCHK(c, hipMemcpyAsync(c->s_norm.p, sg.data(), S * sizeof(double), hipMemcpyHostToDevice, c->stream));
       if (f32)
@@ -1567,8 +1578,6 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
       HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     // frozen instances see a zero right-hand side, so GMRES leaves them at dU = 0
-    for (int i = 0; i < S; ++i) flag[i] = active[i];
-    HIPCHK(c, hipMemcpyAsync(c->s_flag.p, flag.data(), S * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, glh::launch_seg_zero<double>(c->seg(), c->s_flag.p, c->rhs.p, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const double t1 = now_s();

@@ -36,6 +36,8 @@ struct MeshDev {
   const uint32_t *node_mask; // [n_nodes] Dirichlet component bits
   const int *cell_blk;       // [n_cells*NS*NS] BSR block of (a,b); -(blk+1) when this cell writes first
   double *hdiag;             // [n_nodes*18] sum of |local diagonal| of hanging nodes, or null
+  const int *cell_inst;      // [n_cells] instance of each cell
+  const int *inst_active;    // [n_inst] 0: the instance's cells are skipped (J and R left as they are), or null
 };
 
 // Hanging-node constraints on the device (see gl_constraints.hip).

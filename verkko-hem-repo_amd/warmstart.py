@@ -85,6 +85,40 @@ def solve_batch(G, insts, guesses, fp32=True, rtol=1e-10, lin_tol=0.1, lin_maxit
         ctx.close()
 
 
+def solve_from_candidates(G, insts, candidates, fp32=True, rtol=1e-10, lin_tol=0.1, lin_maxit=3000, restart=100,
+                          max_newton=60, ew=True, verbose=0):
+    """One context for the whole batch: the first residual of every candidate
+    guess set (candidates[0] is the plain guess), per instance the candidate
+    with the lowest residual (the plain guess wins ties: it is the fallback),
+    then the batched Newton solve from the chosen guesses, stopped at
+    ||R|| <= rtol ||R(plain guess)|| for every instance.
+    Returns (stats, per-instance stats, F, first residuals [k][n], choice)."""
+    ctx = G.Context(0)
+    try:
+        ctx.set_verbose(verbose)
+        batch = G.Mesh.concat([it.mesh for it in insts])
+        ctx.upload_mesh(batch)
+        ctx.set_materials(np.array([it.coef for it in insts]))
+        res = []
+        for cand in candidates:
+            ctx.set_solution(np.concatenate(cand))
+            res.append(ctx.instance_residuals())
+        res = np.array(res)
+        choice = np.argmin(res, axis=0)  # first minimum: the plain guess on ties
+        guess = np.concatenate([candidates[c][i] for i, c in enumerate(choice)])
+        ctx.set_forcing(ew)
+        ctx.set_residual_reference(res[0])
+        ctx.set_solution(guess)
+        st = ctx.newton_solve(rtol=rtol, max_newton=max_newton, lin_tol=lin_tol, lin_maxit=lin_maxit,
+                              restart=restart, fp32=fp32)
+        ist = ctx.instance_stats()
+        F = ctx.instance_energy()
+        batch.close()
+        return st, ist, F, res, choice
+    finally:
+        ctx.close()
+
+
 def first_residuals(G, insts, guesses):
     """||R|| of every instance at the given guesses (one batched assembly)."""
     ctx = G.Context(0)
