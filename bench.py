@@ -342,20 +342,20 @@ def run_warm_start(G, args):
     # ||R|| <= 1e-10 ||R(plain guess)||, the same absolute accuracy; each
     # timed region holds the upload, the first residuals and the solve.
     t0 = time.perf_counter()
-    st_p, ist_p, F_p, res_p, _ = WS.solve_from_candidates(G, test, [plain], fp32=True, **kw)
+    st_p, ist_p, F_p, res_p, _, _ = WS.solve_from_candidates(G, test, [plain], fp32=True, **kw)
     t_plain32 = time.perf_counter() - t0
     log("plain FP32 batched: %.2f s, %d converged" % (t_plain32, int(ist_p[:, 1].sum())))
     # learned guess with the plain guess as fallback; inference is timed too
     t0 = time.perf_counter()
     learned = WS.predict_guesses(net, test)
-    st_w, ist_w, F_w, res_w, choice = WS.solve_from_candidates(G, test, [plain, learned], fp32=True, **kw)
+    st_w, ist_w, F_w, res_w, choice, r_end = WS.solve_from_candidates(G, test, [plain, learned], fp32=True, **kw)
     t_warm32 = time.perf_counter() - t0
     use = choice == 1
     r_p, r_l = res_w[0], res_w[1]
     log("warm FP32 batched: %.2f s, %d converged, learned guess used for %d" % (t_warm32, int(ist_w[:, 1].sum()),
                                                                             int(use.sum())))
     # plain FP64 batched: the free-energy reference
-    st_64, ist_64, F_64, _, _ = WS.solve_from_candidates(G, test, [plain], fp32=False, **kw)
+    st_64, ist_64, F_64, _, _, _ = WS.solve_from_candidates(G, test, [plain], fp32=False, **kw)
     # plain FP64 drop-in, one instance at a time, on a seeded sample
     ns = min(args.sequential, len(test))
     sample = np.sort(np.random.default_rng(7).choice(len(test), ns, replace=False))
@@ -369,6 +369,8 @@ def run_warm_start(G, args):
     net.close()
     rate = lambda n, t: 3600.0 * n / t if t > 0 else 0.0
     conv_w, conv_p, conv_64 = ist_w[:, 1] == 1, ist_p[:, 1] == 1, ist_64[:, 1] == 1
+    # converged means the recomputed residual meets the test too
+    conv_w &= r_end <= 1e-10 * r_p * (1 + 1e-6)
     both = conv_w & conv_64
     rel = np.abs(F_w - F_64) / np.maximum(np.abs(F_64), 1e-300)
     same = both & (rel <= 1e-10)

@@ -49,12 +49,17 @@ def test_learned_guess_needs_fewer_newton_steps(trained):
     test = WS.slab_instances(G, 64, 4)
     plain = [WS.plain_guess(G, it, 4000 + i) for i, it in enumerate(test)]
     learned = WS.predict_guesses(trained, test)
-    st_p, ist_p, F_p, res_p, _ = WS.solve_from_candidates(G, test, [plain], fp32=True)
-    st_w, ist_w, F_w, res_w, choice = WS.solve_from_candidates(G, test, [plain, learned], fp32=True)
-    st_64, ist_64, F_64, _, _ = WS.solve_from_candidates(G, test, [plain], fp32=False)
+    st_p, ist_p, F_p, res_p, _, rend_p = WS.solve_from_candidates(G, test, [plain], fp32=True)
+    st_w, ist_w, F_w, res_w, choice, rend_w = WS.solve_from_candidates(G, test, [plain, learned], fp32=True)
+    st_64, ist_64, F_64, _, _, rend_64 = WS.solve_from_candidates(G, test, [plain], fp32=False)
     say("plain", st_p, "mean steps", ist_p[:, 0].mean())
     say("learned", st_w, "mean steps", ist_w[:, 0].mean(), "used", int((choice == 1).sum()))
     assert np.all(ist_w[:, 1] == 1) and np.all(ist_p[:, 1] == 1) and np.all(ist_64[:, 1] == 1)
+    # every reported convergence holds for the residual recomputed over the batch
+    for rend in (rend_p, rend_w, rend_64):
+        assert np.all(rend <= 1e-10 * res_p[0] * (1 + 1e-6))
+    # the plain FP32 path reaches the FP64 fixed point from the same guess
+    assert np.mean(np.abs(F_p - F_64) <= 1e-10 * np.abs(F_64)) >= 0.95
     assert (choice == 1).sum() >= 48                        # the prediction beats the plain guess mostly
     assert np.median(res_w[1]) < 0.5 * np.median(res_w[0])  # lower first residual
     assert ist_w[:, 0].mean() < ist_p[:, 0].mean() - 0.5    # fewer Newton steps per instance
@@ -67,5 +72,5 @@ def test_useless_prediction_falls_back_to_plain_guess(trained):
     plain = [WS.plain_guess(G, it, 5000 + i) for i, it in enumerate(test)]
     junk = [10.0 * it.gap_B * np.random.default_rng(i).standard_normal(p.size) for i, (it, p) in
             enumerate(zip(test, plain))]
-    st, ist, F, res, choice = WS.solve_from_candidates(G, test, [plain, junk], fp32=True)
+    st, ist, F, res, choice, rend = WS.solve_from_candidates(G, test, [plain, junk], fp32=True)
     assert np.all(choice == 0) and np.all(ist[:, 1] == 1)

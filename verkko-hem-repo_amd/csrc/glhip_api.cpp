@@ -152,6 +152,7 @@ struct glh_ctx {
   double ptc_sigma0 = 0.0;  // pseudo-transient continuation in glh_newton_solve (0: off)
   bool ew_forcing = false;  // Eisenstat-Walker forcing terms in glh_newton_solve
   const int *asm_active = nullptr;  // device flags: assemble only these instances (inside glh_newton_solve)
+  DBuf<int> asm_flag;
   std::vector<double> r_ref;  // per-instance stopping reference (empty: ||R0|| of the solve)
   long reorth_count = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -932,6 +933,7 @@ int glh_destroy(glh_ctx *c)
   for (auto *b : cd) b->release();
   c->c_gfl.release();
   c->c_rfl.release();
+  c->asm_flag.release();
   c->Jf32.release();
   c->chunks.release();
   c->mats.release();
@@ -1558,9 +1560,12 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
     t0 = now_s();
     // finished instances are skipped by the assembly kernels from here on:
     // their rows keep the last Jacobian and get a zero right-hand side
+    // (a buffer of its own: solve_impl reuses s_flag for its refinement sweeps)
     for (int i = 0; i < S; ++i) flag[i] = active[i];
+    HIPCHK(c, c->asm_flag.alloc(S));
+    HIPCHK(c, hipMemcpyAsync(c->asm_flag.p, flag.data(), S * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->s_flag.p, flag.data(), S * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    c->asm_active = S > 1 ? c->s_flag.p : nullptr;
+    c->asm_active = S > 1 ? c->asm_flag.p : nullptr;
     RCHK(assemble_at(c, c->U.p, true, true, f32, c->rhs.p, false));
     // pseudo-transient continuation: J + sigma_k |diag J|, sigma_k =
     // sigma0 ||R_k|| / ||R_0|| per instance (switched evolution relaxation)

@@ -92,7 +92,8 @@ def solve_from_candidates(G, insts, candidates, fp32=True, rtol=1e-10, lin_tol=0
     with the lowest residual (the plain guess wins ties: it is the fallback),
     then the batched Newton solve from the chosen guesses, stopped at
     ||R|| <= rtol ||R(plain guess)|| for every instance.
-    Returns (stats, per-instance stats, F, first residuals [k][n], choice)."""
+    Returns (stats, per-instance stats, F, first residuals [k][n], choice,
+    final residuals recomputed over the whole batch)."""
     ctx = G.Context(0)
     try:
         ctx.set_verbose(verbose)
@@ -113,8 +114,9 @@ def solve_from_candidates(G, insts, candidates, fp32=True, rtol=1e-10, lin_tol=0
                               restart=restart, fp32=fp32)
         ist = ctx.instance_stats()
         F = ctx.instance_energy()
+        r_end = ctx.instance_residuals()
         batch.close()
-        return st, ist, F, res, choice
+        return st, ist, F, res, choice, r_end
     finally:
         ctx.close()
 
