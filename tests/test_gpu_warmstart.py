@@ -26,7 +26,7 @@ def say(*a):
 
 @pytest.fixture(scope="module")
 def trained():
-    train = WS.slab_instances(G, 512, 3)
+    train = WS.slab_instances(G, 1024, 3)
     Xs, Ys = [], []
     for b0 in range(0, len(train), 256):
         part = train[b0:b0 + 256]
@@ -38,7 +38,7 @@ def trained():
         say("training batch", b0, int(ist[:, 1].sum()))
     net = G.MLP(n_freq=64, sigma=0.5, width=256, depth=5, n_out=18, seed=3)
     net.set_data(np.concatenate(Xs), np.concatenate(Ys))
-    L = net.train(3000, batch=16384, lr=1e-3, seed=3, loss_every=1000)
+    L = net.train(6000, batch=32768, lr=1e-3, seed=3, loss_every=2000)
     say("loss", L)
     assert L[-1] < L[0]
     yield net
