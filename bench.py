@@ -403,13 +403,38 @@ def run_warm_start(G, args):
     print(json.dumps(line), flush=True)
 
 
+def run_mlp_kernels(G, args):
+    """Short fixed sequence for rocprofv3: 50 training steps of the warm-start
+    network (64 Fourier frequencies, 5 x 256 SiLU, 18 outputs) on a batch of
+    65536 random points, then one inference pass over 262144 points."""
+    rng = np.random.default_rng(0)
+    X = rng.uniform(-1, 1, (1 << 20, 6)).astype(np.float32)
+    Y = rng.standard_normal((1 << 20, 18)).astype(np.float32)
+    net = G.MLP(n_freq=64, sigma=0.5, width=256, depth=5, n_out=18, seed=0)
+    net.set_data(X, Y)
+    net.train(5, batch=65536, lr=1e-3, seed=1, loss_every=5)
+    t0 = time.perf_counter()
+    L = net.train(50, batch=65536, lr=1e-3, seed=1, loss_every=50)
+    el = time.perf_counter() - t0
+    nb, W, F2 = 65536, 256, 128
+    flops_fwd = 2.0 * nb * (F2 * W + 4 * W * W + W * 64)
+    flops_step = 3.0 * flops_fwd - 2.0 * nb * F2 * W  # forward, dW, dX (no dX for the first layer)
+    print("bench: mlp step %.3f ms, %.1f TFLOP/s over the step (dense GEMM flops %.3g), loss %.4f"
+          % (1000 * el / 50, flops_step / (el / 50) / 1e12, flops_step, L[-1]), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    net.predict(X[:262144])
+    print("bench: inference of 262144 points %.3f ms" % (1000 * (time.perf_counter() - t0)), file=sys.stderr,
+          flush=True)
+    net.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cube_q2_ref5",
-                    choices=sorted(WORKLOADS) + ["slab_batch", "adaptive_slab", "warm_start"])
+                    choices=sorted(WORKLOADS) + ["slab_batch", "adaptive_slab", "warm_start", "mlp_kernels"])
     ap.add_argument("--cycles", type=int, default=4, help="refinement cycles (adaptive_slab)")
     ap.add_argument("--train-instances", type=int, default=4096, help="training solutions (warm_start)")
     ap.add_argument("--train-steps", type=int, default=20000)
@@ -439,6 +464,9 @@ def main():
         return
     if args.workload == "warm_start":
         run_warm_start(G, args)
+        return
+    if args.workload == "mlp_kernels":
+        run_mlp_kernels(G, args)
         return
     deg, ncell, half, p, t = WORKLOADS[args.workload]
     print("bench: building %s" % args.workload, file=sys.stderr, flush=True)
