@@ -133,7 +133,7 @@ template <typename T> struct Vec2;
 template <> struct Vec2<double> { using type = double2; };
 template <> struct Vec2<float> { using type = float2; };
 
-template <int DEG, typename TJ, bool DO_J, bool DO_R>
+template <int DEG, typename TJ, bool DO_J, bool DO_R, bool HANG = false>
 __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ U,
                                                  const int *__restrict__ cells, TJ *__restrict__ J,
                                                  double *__restrict__ rhs)
@@ -276,7 +276,8 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
         }
       }
     };
-    auto process = [&](int p, const double *ol) {
+    // entries [c][d] (e1) and [d][c] (e2) of the local block of node pair p
+    auto pair_values = [&](int p, double &e1, double &e2) {
       const double *wp = T.Wp + p * NQ;
       double sym = 0.0;
 #pragma unroll
@@ -290,14 +291,18 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
         for (int lf = 0; lf < 6; ++lf)
           if (rob[lf] != 0.0) sym += rob[lf] * T.Mf[(lf * NS + a) * NS + b];
       }
-      double e1 = sym, e2 = sym;
+      e1 = sym;
+      e2 = sym;
       if (grad_cd) {
         e1 += k23_cd * sp[kc * 3 + kd];
         e2 += k23_cd * sp[kd * 3 + kc];
       }
+    };
+    auto process = [&](int p, const double *ol) {
+      double e1, e2;
+      pair_values(p, e1, e2);
+      const int a = T.pa[p], b = T.pb[p];
       const uint32_t ma = mask[cn[a]], mb = mask[cn[b]];
-      // hanging node: its condensed row keeps sum over cells |local diagonal|
-      if (a == b && c == d && (ma & HANG_BIT) && Mh.hdiag) Mh.hdiag[(size_t)cn[a] * 18 + c] += fabs(e1);
       {  // block (a,b): rows of node a, columns of node b
         TJ *B = blk_ptr(cblk[a * NS + b]);
         double v1 = e1, v2 = e2;
@@ -334,6 +339,16 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
         }
       }
     }
+    // hanging nodes (adaptive meshes only): the condensed row keeps the sum
+    // over cells of |local diagonal|, gathered outside the hot loop
+    if (HANG && c == d)
+      for (int p = 0; p < NPAIR; ++p) {
+        const int a = T.pa[p];
+        if (a != T.pb[p] || !(mask[cn[a]] & HANG_BIT)) continue;
+        double e1, e2;
+        pair_values(p, e1, e2);
+        Mh.hdiag[(size_t)cn[a] * 18 + c] += fabs(e1);
+      }
   }
 }
 
@@ -444,17 +459,25 @@ hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefT
 {
   if (n <= 0) return hipSuccess;
   dim3 g(n), b(NT);
-#define GLH_ASM(D, TJ, DJ, DR) \
-  hipLaunchKernelGGL((k_assemble<D, TJ, DJ, DR>), g, b, 0, s, T, M, mat, U, cells, (TJ *)J, rhs)
+#define GLH_ASM(D, TJ, DJ, DR, HG) \
+  hipLaunchKernelGGL((k_assemble<D, TJ, DJ, DR, HG>), g, b, 0, s, T, M, mat, U, cells, (TJ *)J, rhs)
   // J and R are launched as two kernels: the Jacobian-only kernel needs fewer
   // registers (3 waves/SIMD instead of 2 for Q2), which hides more of the
-  // latency of its scattered read-modify-write stores.
+  // latency of its scattered read-modify-write stores.  The hanging-node
+  // variant (adaptive meshes) also gathers the constrained diagonals.
+  const bool hang = M.hdiag != nullptr;
   if (deg == 1) {
-    if (do_j) { if (jfp32) GLH_ASM(1, float, true, false); else GLH_ASM(1, double, true, false); }
-    if (do_r) GLH_ASM(1, double, false, true);
+    if (do_j) {
+      if (hang) { if (jfp32) GLH_ASM(1, float, true, false, true); else GLH_ASM(1, double, true, false, true); }
+      else { if (jfp32) GLH_ASM(1, float, true, false, false); else GLH_ASM(1, double, true, false, false); }
+    }
+    if (do_r) GLH_ASM(1, double, false, true, false);
   } else {
-    if (do_j) { if (jfp32) GLH_ASM(2, float, true, false); else GLH_ASM(2, double, true, false); }
-    if (do_r) GLH_ASM(2, double, false, true);
+    if (do_j) {
+      if (hang) { if (jfp32) GLH_ASM(2, float, true, false, true); else GLH_ASM(2, double, true, false, true); }
+      else { if (jfp32) GLH_ASM(2, float, true, false, false); else GLH_ASM(2, double, true, false, false); }
+    }
+    if (do_r) GLH_ASM(2, double, false, true, false);
   }
 #undef GLH_ASM
   return hipGetLastError();

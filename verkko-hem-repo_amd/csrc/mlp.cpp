@@ -44,7 +44,8 @@ struct Buf {
 };
 
 constexpr int OUTP = 64;     // padded width of the output layer
-constexpr int DW_SPLITS = 64;  // K splits of the weight-gradient products
+constexpr int DW_SPLITS = 64;    // K splits of the weight-gradient products
+constexpr int COL_PARTS = 256;   // row slices of the bias-gradient column sums
 
 }  // namespace
 
@@ -119,7 +120,7 @@ int ensure_batch(glh_mlp *m, int nb)
   MCHK(m, m->dH.alloc((size_t)nb * m->width));
   size_t pmax = 0;
   for (int l = 0; l <= m->depth; ++l) pmax = std::max(pmax, (size_t)m->K[l] * m->Np[l]);
-  MCHK(m, m->part.alloc(pmax * DW_SPLITS));
+  MCHK(m, m->part.alloc(std::max(pmax * DW_SPLITS, (size_t)COL_PARTS * std::max(m->width, OUTP))));
   MCHK(m, m->gtmp.alloc(pmax));
   MCHK(m, m->lpart.alloc((size_t)glh::mlp::mse_blocks(nb, OUTP)));
   MCHK(m, m->loss.alloc(1));
@@ -147,7 +148,7 @@ int forward_backward(glh_mlp *m, int n_real, bool grad)
   if (!grad) return GLH_OK;
   const int nlp = glh::mlp::mse_blocks(nb, OUTP);
   MCHK(m, glh::mlp::mse(nb, n_real, m->n_out, OUTP, m->Yo.p, m->Yb.p, m->dY.p, m->lpart.p, s));
-  MCHK(m, glh::mlp::sum_parts(nlp, 1, m->lpart.p, m->loss.p, s));
+  MCHK(m, glh::mlp::reduce1(nlp, m->lpart.p, m->loss.p, s));
   // backward: dZ of layer l is dY (l = D) or a BF16 buffer alternating A/B.
   // The weight gradients split the batch into `sp` fixed slices.
   int sp = 1;
@@ -166,7 +167,7 @@ int forward_backward(glh_mlp *m, int n_real, bool grad)
       MCHK(m, glh::mlp::unpad(Kl, Nl, Npl, m->gtmp.p, m->G.p + m->woff[l], s));
     }
     // db_l = column sums of dZ_l
-    MCHK(m, glh::mlp::colsum(nb, Npl, dZ, m->part.p, DW_SPLITS, m->gtmp.p, s));
+    MCHK(m, glh::mlp::colsum(nb, Npl, dZ, m->part.p, std::min(COL_PARTS, nb), m->gtmp.p, s));
     MCHK(m, glh::mlp::unpad(1, Nl, Npl, m->gtmp.p, m->G.p + m->boff[l], s));
     if (l == 0) break;
     // dH_{l-1} = dZ_l W_l^T, dZ_{l-1} = dH * SiLU'(Z_{l-1})

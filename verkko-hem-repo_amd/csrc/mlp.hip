@@ -199,6 +199,21 @@ __global__ __launch_bounds__(ET) void k_colsum_partial(int n, int N, int rows_pe
   partial[(size_t)blockIdx.x * N + col] = s;
 }
 
+// out[0] = sum of n values, one workgroup, fixed order (strided sums, then a tree)
+__global__ __launch_bounds__(ET) void k_reduce1(int n, const float *__restrict__ in, float *__restrict__ out)
+{
+  __shared__ float sh[ET];
+  float s = 0.f;
+  for (int i = threadIdx.x; i < n; i += ET) s += in[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = ET / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
 // out[i] = sum_p partial[p*len + i] (fixed order)
 __global__ __launch_bounds__(ET) void k_sum_parts(int parts, long long len, const float *__restrict__ partial,
                                                   float *__restrict__ out)
@@ -336,6 +351,11 @@ hipError_t colsum(int n, int N, const bf16 *D, float *partial, int parts, float 
   const int rows = (n + parts - 1) / parts;
   hipLaunchKernelGGL(k_colsum_partial, dim3(parts, (N + ET - 1) / ET), dim3(ET), 0, s, n, N, rows, D, partial);
   hipLaunchKernelGGL(k_sum_parts, dim3(grid1(N)), dim3(ET), 0, s, parts, (long long)N, partial, out);
+  return hipGetLastError();
+}
+hipError_t reduce1(int n, const float *in, float *out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_reduce1, dim3(1), dim3(ET), 0, s, n, in, out);
   return hipGetLastError();
 }
 hipError_t sum_parts(int parts, long long len, const float *partial, float *out, hipStream_t s)

@@ -27,6 +27,7 @@ int mse_blocks(int n, int ldy);
 hipError_t silu_back(long long total, const float *dH, const float *Z, bf16 *dZ, hipStream_t s);
 hipError_t colsum(int n, int N, const bf16 *D, float *partial, int parts, float *out, hipStream_t s);
 hipError_t sum_parts(int parts, long long len, const float *partial, float *out, hipStream_t s);
+hipError_t reduce1(int n, const float *in, float *out, hipStream_t s);
 hipError_t adam(long long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
                 const int *step, hipStream_t s);
 hipError_t pack_w(int K, int N, int Np, const float *W, bf16 *Wb, hipStream_t s);
