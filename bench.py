@@ -150,13 +150,24 @@ def run_batch(G, args):
     ctx.set_solution(np.concatenate(Us))
     ctx.store_initial()
     out = {}
+    tuned = {}
     for prec in ("fp64", "fp32"):
+        kw_p = dict(kw)
+        if args.autotune:
+            # schedule picked by the library's autotuner on this batch (not timed)
+            t0 = time.perf_counter()
+            ctx.restore_initial()
+            tuned[prec] = ctx.autotune(fp32=(prec == "fp32"), trial_newton=2, lin_tol=args.lin_tol,
+                                       lin_maxit=args.lin_maxit)
+            tuned[prec]["tuning_s"] = time.perf_counter() - t0
+            print("bench: autotune %s %s" % (prec, tuned[prec]), file=sys.stderr, flush=True)
+            kw_p["restart"] = 0
         for w in range(args.warmup):
             ctx.restore_initial()
-            ctx.newton_solve(fp32=(prec == "fp32"), **kw)
+            ctx.newton_solve(fp32=(prec == "fp32"), **kw_p)
         ctx.restore_initial()
         t0 = time.perf_counter()
-        st = ctx.newton_solve(fp32=(prec == "fp32"), **kw)
+        st = ctx.newton_solve(fp32=(prec == "fp32"), **kw_p)
         el = time.perf_counter() - t0
         ist = ctx.instance_stats()
         out[prec] = dict(elapsed=el, st=st, inst=ist, F=ctx.instance_energy())
@@ -234,6 +245,7 @@ def run_batch(G, args):
                                               "max_rel_err_within": float(eseq[sameseq].max()) if sameseq.any()
                                               else None},
         "speedup_vs_plain_fp64": rate(conv32.sum(), out["fp32"]["elapsed"]) / plain_rate if plain_rate else None,
+        "schedule": tuned or {"restart": args.restart, "q_block": 8},
         "lockstep_newton_steps": out["fp32"]["st"]["newton_steps"],
         "lockstep_gmres_its": out["fp32"]["st"]["gmres_its"],
     }
@@ -436,6 +448,8 @@ def main():
     ap.add_argument("--workload", default="cube_q2_ref5",
                     choices=sorted(WORKLOADS) + ["slab_batch", "adaptive_slab", "warm_start", "mlp_kernels"])
     ap.add_argument("--cycles", type=int, default=4, help="refinement cycles (adaptive_slab)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="slab_batch: fixed schedule (restart --restart, blocks of 8) instead of the autotuner")
     ap.add_argument("--train-instances", type=int, default=4096, help="training solutions (warm_start)")
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--train-batch", type=int, default=65536)

@@ -275,6 +275,19 @@ int glh_instance_stats(glh_ctx *ctx, double *out);
  * (HIP events): ms[0] = J+R assembly ms, ms[1] = number of launches. */
 int glh_last_assembly_ms(glh_ctx *ctx, double ms[2]);
 /* level > 0: one progress line per Newton step on stderr (glh_newton_solve). */
+/* Schedule of the batched solve: GMRES restart used when glh_newton_solve
+ * gets restart <= 0 (1..126), Arnoldi steps per replayed graph block
+ * (0: 8), grid caps of the active-set launches in block rows / 4 and in
+ * chunks (0: 4096 / 1024). */
+int glh_set_schedule(glh_ctx *ctx, int restart, int q_block, int row_cap, int chunk_cap);
+/* Autotuner: times `trial_newton` Newton steps (after one untimed step)
+ * from the current solution for every candidate schedule (restart 30 / 60 /
+ * 100; for batches also blocks of 4 / 8 / 16 steps and grid caps 2048 /
+ * 8192), keeps the fastest in the context and restores the solution.
+ * out: [0] restart [1] block [2] row cap [3] chunk cap [4] best trial
+ * seconds [5] candidates tried.  Its own choice of restart is used by
+ * glh_newton_solve calls with restart <= 0. */
+int glh_autotune(glh_ctx *ctx, int fp32, int trial_newton, double lin_tol, int lin_maxit, double out[6]);
 /* Forcing terms of glh_newton_solve: 0 (default) the fixed lin_tol of the
  * reference (solve.cc:119-128); 1 Eisenstat-Walker (choice 2: eta_k =
  * 0.9 (||R_k|| / ||R_k-1||)^2 with the usual safeguard), capped at lin_tol,

@@ -760,7 +760,8 @@ hipError_t launch_spmv(int n_rows, const int *row_ptr, const int *col, const TJ 
                        hipStream_t s, const int *row_seg, const int *seg_flag, ActiveSet act)
 {
   if (n_rows <= 0) return hipSuccess;
-  const int grid = act.counts ? std::max(1, std::min((n_rows + 3) / 4, ACT_ROW_GRID) >> act.shift) : (n_rows + 3) / 4;
+  const int cap = act.row_cap > 0 ? act.row_cap : ACT_ROW_GRID;
+  const int grid = act.counts ? std::max(1, std::min((n_rows + 3) / 4, cap) >> act.shift) : (n_rows + 3) / 4;
   hipLaunchKernelGGL((k_spmv<TJ, TX>), dim3(grid), dim3(256), 0, s, n_rows, row_ptr, col, val, x, y, row_seg,
                      seg_flag, act);
   return hipGetLastError();
@@ -788,7 +789,8 @@ hipError_t launch_bj_apply(int n_rows, const TJ *Dinv, const TX *x, TX *y, hipSt
                            const int *seg_flag, ActiveSet act)
 {
   if (n_rows <= 0) return hipSuccess;
-  const int grid = act.counts ? std::max(1, std::min((n_rows + 3) / 4, ACT_ROW_GRID) >> act.shift) : (n_rows + 3) / 4;
+  const int cap = act.row_cap > 0 ? act.row_cap : ACT_ROW_GRID;
+  const int grid = act.counts ? std::max(1, std::min((n_rows + 3) / 4, cap) >> act.shift) : (n_rows + 3) / 4;
   hipLaunchKernelGGL((k_bj_apply<TJ, TX>), dim3(grid), dim3(256), 0, s, n_rows, Dinv, x, y, row_seg, seg_flag, act);
   return hipGetLastError();
 }
@@ -894,7 +896,8 @@ hipError_t launch_copy(size_t nbytes, const void *src, void *dst, hipStream_t s)
 
 inline int chunk_grid(const SegInfo &S)
 {
-  return S.act.counts ? std::max(1, std::min(S.n_chunks, ACT_CHUNK_GRID) >> S.act.shift) : S.n_chunks;
+  const int cap = S.act.chunk_cap > 0 ? S.act.chunk_cap : ACT_CHUNK_GRID;
+  return S.act.counts ? std::max(1, std::min(S.n_chunks, cap) >> S.act.shift) : S.n_chunks;
 }
 inline int seg_grid(const SegInfo &S)
 {

@@ -151,6 +151,10 @@ struct glh_ctx {
   int verbose = 0;
   double ptc_sigma0 = 0.0;  // pseudo-transient continuation in glh_newton_solve (0: off)
   bool ew_forcing = false;  // Eisenstat-Walker forcing terms in glh_newton_solve
+  // schedule of the batched GMRES (set by glh_autotune or glh_set_schedule):
+  // Arnoldi steps per replayed block, grid caps of the active-set launches,
+  // restart used when glh_newton_solve is called with restart <= 0
+  int q_block = 0, row_cap = 0, chunk_cap = 0, tuned_restart = 100;
   const int *asm_active = nullptr;  // device flags: assemble only these instances (inside glh_newton_solve)
   DBuf<int> asm_flag;
   std::vector<double> r_ref;  // per-instance stopping reference (empty: ||R0|| of the solve)
@@ -514,7 +518,8 @@ int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, i
   const bool use_act = c->n_seg > 1;
   glh::ActiveSet act{};
   if (use_act)
-    act = glh::ActiveSet{c->act_counts.p, c->act_seg.p, c->act_rpref.p, c->act_chunk.p, c->d_node_off.p, shift};
+    act = glh::ActiveSet{c->act_counts.p, c->act_seg.p, c->act_rpref.p, c->act_chunk.p, c->d_node_off.p, shift,
+                         c->row_cap, c->chunk_cap};
   glh::SegInfo SA = SI;
   SA.act = act;
   auto mv = [&](const TV *in, TV *out, const int *flag, glh::ActiveSet a) -> int {
@@ -622,7 +627,7 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
   // one that fits the number of rows still iterating, so that a few
   // straggling instances do not pay for the workgroups of the full batch.
   const bool timed = c->timing && !c->ev_pool.empty();
-  const int q = S == 1 ? 1 : std::min(m, 8);
+  const int q = S == 1 ? 1 : std::min(m, c->q_block > 0 ? c->q_block : 8);
   const int nblk = (m + q - 1) / q;
   constexpr int NLEV = 3;
   auto &gc = Work<TV>::graph(c);
@@ -636,7 +641,9 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
                                          c->act_rpref.p, c->act_chunk.p, c->chunks.p, c->seg_chunk.p,
                                          c->row_ptr.p, c->col.p,
                                          (const void *)(size_t)m, (const void *)(size_t)S,
-                                         (const void *)(size_t)c->n_chunks, (const void *)(size_t)c->n_nodes};
+                                         (const void *)(size_t)c->n_chunks, (const void *)(size_t)c->n_nodes,
+                                         (const void *)(size_t)q, (const void *)(size_t)c->row_cap,
+                                         (const void *)(size_t)c->chunk_cap};
   if (!timed && gc.key != key) {
     for (auto e : gc.exec)
       if (e) (void)hipGraphExecDestroy(e);
@@ -1513,6 +1520,7 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
                      double ls_step, int damped, int fp32, double stats[10])
 {
   RCHK(check_ready(c));
+  if (restart <= 0) restart = c->tuned_restart;
   // the instance filter of the assembly lives only inside this solve
   struct FilterReset {
     glh_ctx *c;
@@ -1678,6 +1686,70 @@ int glh_last_assembly_ms(glh_ctx *c, double ms[2])
   if (!c || !ms) return GLH_ERR_ARG;
   ms[0] = c->asm_ms;
   ms[1] = c->asm_launches;
+  return GLH_OK;
+}
+
+int glh_set_schedule(glh_ctx *c, int restart, int q_block, int row_cap, int chunk_cap)
+{
+  if (!c || restart < 1 || restart > 126 || q_block < 0 || row_cap < 0 || chunk_cap < 0) return GLH_ERR_ARG;
+  c->tuned_restart = restart;
+  c->q_block = q_block;
+  c->row_cap = row_cap;
+  c->chunk_cap = chunk_cap;
+  return GLH_OK;
+}
+
+int glh_autotune(glh_ctx *c, int fp32, int trial_newton, double lin_tol, int lin_maxit, double out[6])
+{
+  RCHK(check_ready(c));
+  if (trial_newton < 1 || trial_newton > 8 || !(lin_tol > 0) || lin_maxit < 1 || !out)
+    return fail(c, GLH_ERR_ARG, "bad autotune parameters");
+  struct Cand {
+    int m, q, rc, cc;
+  };
+  std::vector<Cand> cands;
+  const int ms[3] = {30, 60, 100};
+  for (int m : ms) {
+    if (c->n_seg == 1) {
+      cands.push_back({m, 0, 0, 0});
+      continue;
+    }
+    for (int q : {4, 8, 16})
+      for (int rc : {2048, 8192}) cands.push_back({m, q, rc, rc / 4});
+  }
+  RCHK(glh_store_initial(c));
+  const int old_verbose = c->verbose;
+  c->verbose = 0;
+  double tbest = 1e300;
+  Cand best = cands[0];
+  int rc = GLH_OK;
+  for (const Cand &k : cands) {
+    c->q_block = k.q;
+    c->row_cap = k.rc;
+    c->chunk_cap = k.cc;
+    double st[10];
+    // one untimed step records the candidate's graphs, then the timed trial
+    if ((rc = glh_restore_initial(c)) != GLH_OK) break;
+    if ((rc = glh_newton_solve(c, 1e-300, 1, lin_tol, lin_maxit, k.m, 0.83, 1, fp32, st)) != GLH_OK) break;
+    if ((rc = glh_restore_initial(c)) != GLH_OK) break;
+    const double t0 = now_s();
+    if ((rc = glh_newton_solve(c, 1e-300, trial_newton, lin_tol, lin_maxit, k.m, 0.83, 1, fp32, st)) != GLH_OK)
+      break;
+    const double t = now_s() - t0;
+    if (t < tbest) {
+      tbest = t;
+      best = k;
+    }
+  }
+  c->verbose = old_verbose;
+  c->tuned_restart = best.m;
+  c->q_block = best.q;
+  c->row_cap = best.rc;
+  c->chunk_cap = best.cc;
+  RCHK(rc);
+  RCHK(glh_restore_initial(c));
+  out[0] = best.m; out[1] = best.q; out[2] = best.rc; out[3] = best.cc; out[4] = tbest;
+  out[5] = (double)cands.size();
   return GLH_OK;
 }
 

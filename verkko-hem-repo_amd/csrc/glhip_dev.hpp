@@ -86,6 +86,7 @@ hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Ma
 struct ActiveSet {
   const int *counts = nullptr, *seg = nullptr, *rpref = nullptr, *chunk = nullptr, *node_off = nullptr;
   int shift = 0;
+  int row_cap = 0, chunk_cap = 0;  // grid caps (0: ACT_ROW_GRID / ACT_CHUNK_GRID)
 };
 constexpr int ACT_ROW_GRID = 4096;   // workgroups of 4 rows, grid-stride over listed rows
 constexpr int ACT_CHUNK_GRID = 1024; // workgroups, grid-stride over listed chunks

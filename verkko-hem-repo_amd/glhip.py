@@ -34,7 +34,8 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_flag_fixed_number", "glh_initial_guess_bna", "glh_run_adaptive",
            "glh_set_continuation", "glh_mlp_create", "glh_mlp_destroy", "glh_mlp_last_error", "glh_mlp_n_params",
            "glh_mlp_get_params", "glh_mlp_set_params", "glh_mlp_set_data", "glh_mlp_train", "glh_mlp_loss_grad",
-           "glh_mlp_predict", "glh_instance_residuals", "glh_set_forcing", "glh_set_residual_reference"]
+           "glh_mlp_predict", "glh_instance_residuals", "glh_set_forcing", "glh_set_residual_reference",
+           "glh_set_schedule", "glh_autotune"]
 
 _lib = None
 
@@ -86,6 +87,8 @@ def lib():
     L.glh_set_verbose.argtypes = [vp, C.c_int]
     L.glh_set_continuation.argtypes = [vp, C.c_double]
     L.glh_set_forcing.argtypes = [vp, C.c_int]
+    L.glh_set_schedule.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.glh_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, _dp]
     L.glh_set_residual_reference.argtypes = [vp, C.c_int, vp]
     L.glh_mlp_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]
     L.glh_mlp_destroy.argtypes = [vp]
@@ -497,6 +500,19 @@ class Context:
 
     def set_verbose(self, level=1):
         self._chk(lib().glh_set_verbose(self.h, int(level)), "glh_set_verbose")
+
+    def set_schedule(self, restart=100, q_block=0, row_cap=0, chunk_cap=0):
+        self._chk(lib().glh_set_schedule(self.h, int(restart), int(q_block), int(row_cap), int(chunk_cap)),
+                  "glh_set_schedule")
+
+    def autotune(self, fp32=True, trial_newton=2, lin_tol=0.1, lin_maxit=3000):
+        """Time candidate schedules on the current problem and keep the fastest
+        (see glh_autotune); returns the choice as a dict."""
+        out = np.zeros(6)
+        self._chk(lib().glh_autotune(self.h, int(bool(fp32)), int(trial_newton), float(lin_tol), int(lin_maxit), out),
+                  "glh_autotune")
+        return dict(restart=int(out[0]), q_block=int(out[1]), row_cap=int(out[2]), chunk_cap=int(out[3]),
+                    trial_s=float(out[4]), candidates=int(out[5]))
 
     def set_forcing(self, eisenstat_walker):
         self._chk(lib().glh_set_forcing(self.h, int(bool(eisenstat_walker))), "glh_set_forcing")
