@@ -155,6 +155,33 @@ int glh_run_adaptive(glh_ctx *ctx, const glh_forest *f0, const double *U0, int n
                      int damped, int fp32, int check_fp64, double *cycle_stats, glh_forest **f_out,
                      glh_mesh **m_out);
 
+/* ------------------------------------------------ .prm configuration */
+/* The reference's parameter file (deal.II ParameterHandler text: nested
+ * "subsection NAME" ... "end", "set KEY = VALUE", '#' comments) with the keys,
+ * defaults and patterns of confreader/src/declare.cc:110-300; undeclared keys
+ * and pattern violations are errors (message in err).  The subsection
+ * "glhip" holds what the reference fixes at compile time: geometry
+ * (retangle | cube), boundary ids (six ids, faces -x +x -y +y -z +z),
+ * finite element degree (1), initial guess (B | A | BnA), random seed, noise
+ * amplitude, mixed precision, GMRES restart. */
+typedef struct glh_prm glh_prm;
+int glh_prm_parse(const char *text, glh_prm **out, char *err, int err_len);
+int glh_prm_destroy(glh_prm *p);
+/* value as text / as a number (double and integer entries) */
+int glh_prm_get(const glh_prm *p, const char *subsection, const char *key, char *value, int len);
+int glh_prm_get_double(const glh_prm *p, const char *subsection, const char *key, double *v);
+/* FemGL::run (femgl/src/run.cc:109-260) driven by a parsed configuration:
+ * make_grid (box refined globally "Number of initial global refinments"
+ * times), then per cycle up to "Number of interations" + 1 rounds of
+ * assemble / solve ("Cycle k linear solver tol") / damped Newton step, leaving
+ * the cycle when |R - R_last| < "Cycle k refinement threshold" (refine) or
+ * ||R|| <= "converge accuracy" (stop), with adaptive (Kelly, "adaptive
+ * refinment ratio") or global refinement between cycles.  rows[max_rows][6]:
+ * cycle, cells, dofs, Newton iterations, final ||R||, free energy; *n_rows
+ * the number written.  GLH_ERR_NOCONV when a linear solve fails, as the
+ * reference's SolverControl::NoConvergence ends its run. */
+int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int *n_rows);
+
 /* ------------------------------------------- coordinate-MLP warm start */
 /* A small network that predicts the order parameter from (x, y, z, T/Tc, p,
  * D), trained on converged solutions and used as the Newton initial guess

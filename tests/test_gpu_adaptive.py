@@ -184,3 +184,39 @@ def test_continuation_reaches_the_minimum_where_damped_newton_stalls(ctx):
     Uo, rc, ost = O.newton(om, co, U, rtol=1e-6, max_newton=5, lin_tol=1e-3, restart=50)
     assert O.free_energy(om, co, Uo) == pytest.approx(st1["F"], rel=1e-10)
     assert np.abs(Uo - U).max() <= 1e-8 * np.abs(U).max()
+
+
+def test_run_prm_follows_the_reference_cycle_loop(ctx):
+    # FemGL::run from a .prm file: cycle 0 leaves through the stuck test
+    # (|R - R_last| < 1, the default threshold) and refines by Kelly; the last
+    # cycle runs to the absolute converge accuracy
+    prm = G.Prm("""
+subsection physical parameters
+  set pressure in bar = 22.0
+  set t_reduced = 0.8
+end
+subsection control parameters
+  set half x length of retangle = 5
+  set half y length of retangle = 5
+  set half z length of retangle = 6
+  set Number of initial global refinments = 2
+  set Number of refinements = 1
+  set Number of interations = 60
+  set converge accuracy = 1e-7
+  set Cycle 0 linear solver tol = 1e-2
+  set Cycle 1 linear solver tol = 1e-2
+end
+subsection glhip
+  set boundary ids = 1 1 1 1 0 0
+  set random seed = 1
+  set GMRES restart = 50
+end
+""")
+    rows = ctx.run_prm(prm)
+    for r in rows:
+        say(r)
+    assert len(rows) == 2 and rows[0]["cells"] == 64 and rows[1]["cells"] > 64
+    assert rows[0]["residual"] > 1e-7 >= rows[1]["residual"]
+    # the context holds the last cycle's state: recompute its residual and energy
+    assert ctx.residual_norm() == pytest.approx(rows[1]["residual"], rel=1e-6)
+    assert ctx.free_energy() == pytest.approx(rows[1]["F"], rel=1e-12)

@@ -1,0 +1,93 @@
+"""The .prm reader (no GPU): every key of confreader/src/declare.cc with its
+default, overrides inside nested subsections, comments, and the errors of
+deal.II's ParameterHandler (undeclared entries, pattern violations, unclosed
+subsections)."""
+import pytest
+
+from product_util import glhip
+
+G = glhip()
+
+# (subsection, key, default) as declared in confreader/src/declare.cc:115-291
+DECLARED = [("physical parameters", "pressure in bar", 0.0), ("physical parameters", "t_reduced", 0.0),
+            ("physical parameters", "AdGR diffuse length", 1.0e10),
+            ("physical parameters", "gaussian random mean value", 2.0),
+            ("physical parameters", "gaussian random STD", 0.1),
+            ("physical parameters", "amplitude u parameter", 1.0),
+            ("control parameters", "cube half side length", 20.0),
+            ("control parameters", "half x length of retangle", 20.0),
+            ("control parameters", "half y length of retangle", 20.0),
+            ("control parameters", "half z length of retangle", 20.0),
+            ("control parameters", "B-phase inner plate radius ratio", 0.4666),
+            ("control parameters", "B-phase ball radius ratio", 0.5),
+            ("control parameters", "A-phase block range ratio", 0.0),
+            ("control parameters", "Number of refinements", 4.0),
+            ("control parameters", "Number of interations", 40.0),
+            ("control parameters", "converge accuracy", 5.0e-6),
+            ("control parameters", "adaptive refinment ratio", 0.3),
+            ("control parameters", "adaptive coarsen ratio", 0.0),
+            ("control parameters", "Number of initial global refinments", 4.0),
+            ("control parameters", "Number of n-cycle in AdditionalData", 4.0),
+            ("control parameters", "maximum linear iteration number", 10000.0),
+            ("control parameters", "primary step length of dampped newton iteration", 0.83)] + \
+           [("control parameters", "Cycle %d refinement threshold" % k, 1.0) for k in range(4)] + \
+           [("control parameters", "Cycle %d linear solver tol" % k, 0.1) for k in range(5)]
+
+
+def test_defaults_of_every_declared_key():
+    p = G.Prm("")
+    for sub, key, val in DECLARED:
+        assert p.get_double(sub, key) == pytest.approx(val, rel=1e-15), key
+    assert p.get("physical parameters", "trun on Strong Coupling Correction") == "true"
+    assert p.get("control parameters", "Using dampped Newton iteration") == "true"
+    for k in range(1, 5):
+        assert p.get("control parameters", "Cycle %d do global refinement" % k) == "false"
+
+
+def test_overrides_comments_and_nesting():
+    p = G.Prm("""
+# a configuration of the reference's format
+subsection physical parameters
+  set pressure in bar = 22.0      # bar
+  set t_reduced       = 0.8
+  set trun on Strong Coupling Correction = false
+end
+subsection control parameters
+  set Number of refinements = 2
+  set Cycle 1 linear solver tol = 1e-3
+end
+subsection glhip
+  set initial guess = BnA
+  set boundary ids = 1 1 1 1 0 0
+end
+""")
+    assert p.get_double("physical parameters", "pressure in bar") == 22.0
+    assert p.get_double("physical parameters", "t_reduced") == 0.8
+    assert p.get("physical parameters", "trun on Strong Coupling Correction") == "false"
+    assert p.get_double("control parameters", "Number of refinements") == 2
+    assert p.get_double("control parameters", "Cycle 1 linear solver tol") == 1e-3
+    assert p.get_double("control parameters", "Cycle 2 linear solver tol") == 0.1
+    assert p.get("glhip", "initial guess") == "BnA"
+    assert p.get("glhip", "boundary ids") == "1 1 1 1 0 0"
+
+
+@pytest.mark.parametrize("text,what", [
+    ("set pressure in bar = 1", "undeclared"),
+    ("subsection physical parameters\n set presure in bar = 1\nend", "undeclared"),
+    ("subsection physical parameters\n set pressure in bar = -1\nend", "non-negative"),
+    ("subsection control parameters\n set Number of refinements = 2.5\nend", "integer"),
+    ("subsection control parameters\n set Using dampped Newton iteration = yes\nend", "true or false"),
+    ("subsection control parameters\n set converge accuracy = abc\nend", "number"),
+    ("subsection control parameters\n", "not closed"),
+    ("end", "without a subsection"),
+    ("subsection control parameters\n converge accuracy = 1\nend", "expected"),
+])
+def test_errors_are_reported(text, what):
+    with pytest.raises(G.GLHError) as e:
+        G.Prm(text)
+    assert what in str(e.value)
+
+
+def test_amplitude_may_be_negative():
+    p = G.Prm("subsection physical parameters\n set amplitude u parameter = -0.5\nend")
+    assert p.get_double("physical parameters", "amplitude u parameter") == -0.5

@@ -35,7 +35,8 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_set_continuation", "glh_mlp_create", "glh_mlp_destroy", "glh_mlp_last_error", "glh_mlp_n_params",
            "glh_mlp_get_params", "glh_mlp_set_params", "glh_mlp_set_data", "glh_mlp_train", "glh_mlp_loss_grad",
            "glh_mlp_predict", "glh_instance_residuals", "glh_set_forcing", "glh_set_residual_reference",
-           "glh_set_schedule", "glh_autotune"]
+           "glh_set_schedule", "glh_autotune", "glh_prm_parse", "glh_prm_destroy", "glh_prm_get",
+           "glh_prm_get_double", "glh_run_prm"]
 
 _lib = None
 
@@ -87,6 +88,11 @@ def lib():
     L.glh_set_verbose.argtypes = [vp, C.c_int]
     L.glh_set_continuation.argtypes = [vp, C.c_double]
     L.glh_set_forcing.argtypes = [vp, C.c_int]
+    L.glh_prm_parse.argtypes = [C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_int]
+    L.glh_prm_destroy.argtypes = [vp]
+    L.glh_prm_get.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
+    L.glh_prm_get_double.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(C.c_double)]
+    L.glh_run_prm.argtypes = [vp, vp, C.c_int, _dp, C.POINTER(C.c_int)]
     L.glh_set_schedule.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.glh_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, _dp]
     L.glh_set_residual_reference.argtypes = [vp, C.c_int, vp]
@@ -349,6 +355,40 @@ class MLP:
             pass
 
 
+class Prm:
+    """Parsed .prm configuration (glh_prm)."""
+
+    def __init__(self, text):
+        self.h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = lib().glh_prm_parse(text.encode(), C.byref(self.h), err, 512)
+        if rc:
+            raise GLHError("glh_prm_parse: " + err.value.decode())
+
+    def get(self, subsection, key):
+        buf = C.create_string_buffer(256)
+        if lib().glh_prm_get(self.h, subsection.encode(), key.encode(), buf, 256):
+            raise GLHError("no entry '%s' in '%s'" % (key, subsection))
+        return buf.value.decode()
+
+    def get_double(self, subsection, key):
+        v = C.c_double()
+        if lib().glh_prm_get_double(self.h, subsection.encode(), key.encode(), C.byref(v)):
+            raise GLHError("no numeric entry '%s' in '%s'" % (key, subsection))
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().glh_prm_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def flag_fixed_number(eta, frac):
     eta = np.ascontiguousarray(eta, np.float32)
     fl = np.zeros(eta.size, np.uint8)
@@ -463,6 +503,15 @@ class Context:
         v = C.c_double()
         self._chk(lib().glh_free_energy(self.h, C.byref(v)), "glh_free_energy")
         return v.value
+
+    def run_prm(self, prm, max_rows=16):
+        """FemGL::run from a parsed configuration (glh_run_prm): per-cycle rows
+        (cycle, cells, dofs, Newton iterations, ||R||, F)."""
+        rows = np.zeros(max_rows * 6)
+        n = C.c_int()
+        self._chk(lib().glh_run_prm(self.h, prm.h, int(max_rows), rows, C.byref(n)), "glh_run_prm")
+        return [dict(zip(["cycle", "cells", "dofs", "iterations", "residual", "F"], r))
+                for r in rows.reshape(-1, 6)[:n.value].tolist()]
 
     CYCLE_KEYS = ["cycle", "cells", "dofs", "hanging", "newton_steps", "gmres_its", "r0", "r_final", "F",
                   "converged", "t_solve", "t_cycle", "F64", "F_rel_err_vs_fp64", "U_rel_err_vs_fp64", "converged64"]
