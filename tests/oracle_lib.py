@@ -462,3 +462,85 @@ def kelly(cell_nodes, cell_lo, cell_h, U, tol=1e-9):
                     e2[e] += np.linalg.norm(h[e]) / 24.0 * s
                     e2[k] += np.linalg.norm(h[k]) / 24.0 * s
     return np.sqrt(e2)
+
+
+def periodic_oracle_mesh(deg, n, lo, hi, side_masks, periodic_axes, faces=()):
+    """Box mesh with periodic axes for the oracle, derived on the test side:
+    Dirichlet lines first (masks of the non-periodic faces), then periodicity
+    (a dof on a + face equals its image on the - face, skipped where the dof
+    is already constrained), then closure (images that are constrained are
+    replaced by their own lines)."""
+    m = box_mesh(deg, n, lo, hi, dirichlet_faces=())
+    xyz = m.node_xyz
+    n_nodes = m.n_nodes
+    tol = 1e-9 * max(abs(b - a) for a, b in zip(lo, hi))
+    lines = {}
+    for v in range(n_nodes):
+        for s in range(6):
+            d = s // 2
+            if d in periodic_axes:
+                continue
+            if abs(xyz[v, d] - (lo[d] if s % 2 == 0 else hi[d])) < tol:
+                for c in range(18):
+                    if (side_masks[s] >> c) & 1:
+                        lines[v * 18 + c] = {}
+    key = {tuple(np.round(x / tol).astype(np.int64)): i for i, x in enumerate(xyz)}
+    for v in range(n_nodes):
+        img = xyz[v].copy()
+        for d in periodic_axes:
+            if abs(img[d] - hi[d]) < tol:
+                img[d] = lo[d]
+        w = key[tuple(np.round(img / tol).astype(np.int64))]
+        if w == v:
+            continue
+        for c in range(18):
+            if v * 18 + c not in lines:
+                lines[v * 18 + c] = {w * 18 + c: 1.0}
+    for g in list(lines):
+        line = lines[g]
+        for _ in range(8):
+            nxt, changed = {}, False
+            for mm, ww in line.items():
+                if mm in lines:
+                    changed = True
+                    for m2, w2 in lines[mm].items():
+                        nxt[m2] = nxt.get(m2, 0.0) + ww * w2
+                else:
+                    nxt[mm] = nxt.get(mm, 0.0) + ww
+            line = nxt
+            if not changed:
+                break
+        lines[g] = line
+    con_of = -np.ones(n_nodes * 18, np.int32)
+    con_ptr, con_master, con_weight = [0], [], []
+    for i, g in enumerate(sorted(lines)):
+        con_of[g] = i
+        for mm, ww in sorted(lines[g].items()):
+            con_master.append(mm)
+            con_weight.append(ww)
+        con_ptr.append(len(con_master))
+    cells = m.cell_nodes.reshape(-1, (deg + 1) ** 3)
+    tgt = {}
+    for a in range(n_nodes):
+        t = set()
+        for c in range(18):
+            g = a * 18 + c
+            if g in lines:
+                t.update(mm // 18 for mm in lines[g])
+            else:
+                t.add(a)
+        tgt[a] = t
+    nb = [set([r]) for r in range(n_nodes)]
+    for cn in cells:
+        for a in cn:
+            for b in cn:
+                for r in tgt[int(a)]:
+                    nb[r].update(tgt[int(b)])
+    row_ptr = np.zeros(n_nodes + 1, np.int32)
+    cols = []
+    for r in range(n_nodes):
+        s = sorted(nb[r])
+        cols.extend(s)
+        row_ptr[r + 1] = row_ptr[r] + len(s)
+    return Mesh(deg, m.cell_nodes, m.cell_h, list(faces), n_nodes, con_of, np.array(con_ptr, np.int32),
+                con_master, con_weight, row_ptr, np.array(cols, np.int32), xyz)

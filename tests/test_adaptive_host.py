@@ -142,3 +142,38 @@ def test_fixed_number_flags():
     assert G.flag_fixed_number(eta, 0.0).sum() == 0
     with pytest.raises(G.GLHError):
         G.flag_fixed_number(eta, 1.5)
+
+
+@pytest.mark.parametrize("deg,axes", [(1, (0, 1)), (2, (0,)), (1, (0, 1, 2))])
+def test_periodic_box_lines_match_independent_derivation(deg, axes):
+    n = (3, 2, 2)
+    lo, hi = (-4.0, -3.0, -2.0), (4.0, 3.0, 2.0)
+    sides = [1] * 6
+    for a in axes:
+        sides[2 * a] = sides[2 * a + 1] = 5
+    if 2 not in axes:
+        sides[4] = sides[5] = 0
+    m = G.Mesh(deg, n, lo, hi, sides)
+    c = m.constraints()
+    om = O.periodic_oracle_mesh(deg, n, lo, hi, side_masks(sides), axes)
+    hp = c["hang_ptr"]
+    # node-level lines of the product against the dof-level oracle lines
+    for v in range(m.n_nodes):
+        got = {int(c["hang_m"][k]): float(c["hang_w"][k]) for k in range(hp[v], hp[v + 1])}
+        comp = []
+        for cc in range(18):
+            g = v * 18 + cc
+            ci = om.con_of[g]
+            if ci < 0:
+                comp.append(None)
+            else:
+                comp.append({int(om.con_master[k]) // 18: float(om.con_weight[k])
+                             for k in range(om.con_ptr[ci], om.con_ptr[ci + 1])})
+        if got:
+            # a periodic image: every component either follows the same image
+            # or is Dirichlet there (its image is Dirichlet too, so the value is 0)
+            for line in comp:
+                assert line is not None and (line == got or line == {})
+        else:
+            assert all(line is None or line == {} for line in comp)
+    assert c["n_hanging"] > 0

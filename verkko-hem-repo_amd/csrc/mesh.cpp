@@ -7,6 +7,8 @@ namespace glh {
 
 uint32_t dirichlet_mask(int bid)
 {
+  // 0: A = 0 on all components; 2/3/4: the x/y/z-normal orbital column;
+  // 1 (natural) and 5 (periodic) constrain nothing here
   if (bid == 0) return (1u << NC) - 1u;
   if (bid >= 2 && bid <= 4) {
     uint32_t m = 0;
@@ -89,7 +91,39 @@ int make_box(Mesh &m, int deg, int nx, int ny, int nz, const double lo[3], const
     m.color_cells.insert(m.color_cells.end(), c.begin(), c.end());
     m.color_ptr.push_back((int)m.color_cells.size());
   }
-  build_pattern(m);
+  // periodic axes (side id 5 on both faces, the matched face pairs of
+  // makegrid_*-periodic.cc / make_periodicity_constraints): a node on a +
+  // face takes the value of its image on the - faces (weight 1; corners and
+  // edges map straight to the image on every periodic axis, so the lines are
+  // closed)
+  bool per[3];
+  for (int k = 0; k < 3; ++k) {
+    per[k] = side_id[2 * k] == 5 && side_id[2 * k + 1] == 5;
+    if ((side_id[2 * k] == 5) != (side_id[2 * k + 1] == 5)) {
+      err = "a periodic axis needs side id 5 on both faces";
+      return -1;
+    }
+  }
+  if (per[0] || per[1] || per[2]) {
+    m.hang_ptr.assign(nn + 1, 0);
+    for (long k = 0; k < Nz; ++k)
+      for (long j = 0; j < Ny; ++j)
+        for (long i = 0; i < Nx; ++i) {
+          const long n = i + Nx * (j + Ny * k);
+          const long mi = (per[0] && i == Nx - 1) ? 0 : i, mj = (per[1] && j == Ny - 1) ? 0 : j,
+                     mk = (per[2] && k == Nz - 1) ? 0 : k;
+          const long mn = mi + Nx * (mj + Ny * mk);
+          if (mn != n) {
+            m.hang_m.push_back((int)mn);
+            m.hang_w.push_back(1.0);
+            m.node_mask[n] = HANG_BIT;
+          }
+          m.hang_ptr[n + 1] = (int)m.hang_m.size();
+        }
+    build_constrained_patterns(m);
+  } else {
+    build_pattern(m);
+  }
   m.node_off = {0, m.n_nodes};
   m.cell_off = {0, m.n_cells};
   return 0;
