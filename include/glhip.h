@@ -155,6 +155,12 @@ int glh_run_adaptive(glh_ctx *ctx, const glh_forest *f0, const double *U0, int n
                      int damped, int fp32, int check_fp64, double *cycle_stats, glh_forest **f_out,
                      glh_mesh **m_out);
 
+/* FemGL::output_results (femgl/src/io.cc:105-168): dir/solution_NN.0.vtu
+ * and dir/solution_NN.pvtu (NN = counter, two digits) with point data
+ * du_ij, dv_ij (dU, may be NULL) and u_ij, v_ij (U), cell data "subdomain".
+ * Q2 cells are written as 8 sub-hexahedra. */
+int glh_write_vtu(const glh_mesh *m, const double *U, const double *dU, const char *dir, int counter);
+
 /* ------------------------------------------------ .prm configuration */
 /* The reference's parameter file (deal.II ParameterHandler text: nested
  * "subsection NAME" ... "end", "set KEY = VALUE", '#' comments) with the keys,

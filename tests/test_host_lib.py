@@ -112,3 +112,30 @@ def test_bna_guess_bitwise_equal_to_oracle_and_phases():
     assert np.allclose(u[up][:, [0, 4, 8]], m["gap_B"] / np.sqrt(3), rtol=1e-7)
     assert np.allclose(u[~up][:, [0, 10]], m["gap_A"] / np.sqrt(2), rtol=1e-7)
     assert np.count_nonzero(u[up]) == 3 * up.sum() and np.count_nonzero(u[~up]) == 2 * (~up).sum()
+
+
+@pytest.mark.parametrize("deg", [1, 2])
+def test_vtu_output_structure(tmp_path, deg):
+    import xml.etree.ElementTree as ET
+    m = G.Mesh(deg, (2, 2, 1), (-1, -1, -1), (1, 1, 1), [1, 1, 1, 1, 0, 0])
+    U = np.arange(m.n_nodes * 18, dtype=np.float64)
+    m.write_vtu(U, tmp_path, counter=3, dU=-U)
+    root = ET.parse(tmp_path / "solution_03.0.vtu").getroot()
+    piece = root.find("UnstructuredGrid/Piece")
+    n_hex = 4 * (8 if deg == 2 else 1)
+    assert int(piece.get("NumberOfPoints")) == m.n_nodes and int(piece.get("NumberOfCells")) == n_hex
+    arrays = {a.get("Name"): a for a in piece.find("PointData")}
+    names = ["%s_%d%d" % (p, i, j) for p in ("u", "v") for i in (1, 2, 3) for j in (1, 2, 3)]
+    assert all(n in arrays for n in names) and all("d" + n in arrays for n in names)
+    # component c of node v is U[v*18 + c]: u_12 is component 1, v_31 is 15
+    assert np.array_equal(np.array(arrays["u_12"].text.split(), float), U.reshape(-1, 18)[:, 1])
+    assert np.array_equal(np.array(arrays["v_31"].text.split(), float), U.reshape(-1, 18)[:, 15])
+    # hexahedra are positively oriented (VTK order) and cover the box
+    pts = np.array(piece.find("Points/DataArray").text.split(), float).reshape(-1, 3)
+    con = np.array(piece.find("Cells/DataArray").text.split(), int).reshape(-1, 8)
+    vol = 0.0
+    for h in con:
+        p = pts[h]
+        vol += np.linalg.det(np.array([p[1] - p[0], p[3] - p[0], p[4] - p[0]]))
+    assert vol == pytest.approx(8.0, rel=1e-12)
+    assert (tmp_path / "solution_03.pvtu").exists()

@@ -27,7 +27,7 @@ HIPFLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "
             "-ffp-contract=fast"]
 
 SOURCES = ["gl_assemble.hip", "gl_linalg.hip", "gl_constraints.hip", "mlp.hip", "mlp.cpp", "glhip_api.cpp", "mesh.cpp", "amr.cpp",
-           "matep.cpp", "setup.cpp", "kelly.cpp", "driver.cpp", "prm.cpp"]
+           "matep.cpp", "setup.cpp", "kelly.cpp", "driver.cpp", "prm.cpp", "io.cpp"]
 
 
 def _compile(src):

@@ -36,7 +36,7 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_mlp_get_params", "glh_mlp_set_params", "glh_mlp_set_data", "glh_mlp_train", "glh_mlp_loss_grad",
            "glh_mlp_predict", "glh_instance_residuals", "glh_set_forcing", "glh_set_residual_reference",
            "glh_set_schedule", "glh_autotune", "glh_prm_parse", "glh_prm_destroy", "glh_prm_get",
-           "glh_prm_get_double", "glh_run_prm"]
+           "glh_prm_get_double", "glh_run_prm", "glh_write_vtu"]
 
 _lib = None
 
@@ -88,6 +88,7 @@ def lib():
     L.glh_set_verbose.argtypes = [vp, C.c_int]
     L.glh_set_continuation.argtypes = [vp, C.c_double]
     L.glh_set_forcing.argtypes = [vp, C.c_int]
+    L.glh_write_vtu.argtypes = [vp, _dp, vp, C.c_char_p, C.c_int]
     L.glh_prm_parse.argtypes = [C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_int]
     L.glh_prm_destroy.argtypes = [vp]
     L.glh_prm_get.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
@@ -199,6 +200,14 @@ class Mesh:
         lib().glh_mesh_get(self.h, *[v.ctypes.data_as(C.c_void_p) for v in a.values()])
         a["faces"] = a["faces"][:self.n_faces * 3]
         return a
+
+    def write_vtu(self, U, directory, counter=0, dU=None):
+        """VTU/PVTU output of U (and dU) on this mesh (glh_write_vtu)."""
+        dptr = None if dU is None else np.ascontiguousarray(dU, np.float64).ctypes.data_as(C.c_void_p)
+        rc = lib().glh_write_vtu(self.h, np.ascontiguousarray(U, np.float64), dptr, str(directory).encode(),
+                                 int(counter))
+        if rc:
+            raise GLHError("glh_write_vtu failed (%d)" % rc)
 
     def constraints(self):
         """Hanging-node lines (hang_ptr, hang_m, hang_w) and the cell-loop
