@@ -239,6 +239,23 @@ int glh_initial_guess_host(int n_nodes, double gap, int phase, double amp, uint6
  * node_xyz[n_nodes*3]. */
 int glh_initial_guess_bna(int n_nodes, const double *node_xyz, double z_a, double gap_A, double gap_B, double amp,
                           uint64_t seed, double *U);
+/* Analytic initial configurations at the nodes (VectorTools::interpolate
+ * at the FE_Q support points), plus amp*scale*(2u-1) noise drawn as above
+ * (amp = 0: the reference, which adds none).  kind / par:
+ *   GLH_GUESS_BINA  B-in-A ball (femgl/inc/BinA.h:130-161, used by
+ *                   setup_uniform_BinA-bubble-like-configuration.cc:229-235):
+ *                   {radius, gap_A, gap_B}, scale gap_B;
+ *   GLH_GUESS_BNB   B-n-B polka dots (femgl/inc/BinB.h:130-162, used by
+ *                   setup_uniform_BnB-polkka-dot-like-configuration.cc:309-313):
+ *                   {radius}, scale 1;
+ *   GLH_GUESS_PDW   weak-coupling pair-density wave (femgl/inc/pdw_config.h:
+ *                   124-299, used by setup_weak-coupling-PDW-configuration.cc:
+ *                   311-315): {u, half_D, gap_B}, scale gap_B. */
+#define GLH_GUESS_BINA 1
+#define GLH_GUESS_BNB 2
+#define GLH_GUESS_PDW 3
+int glh_initial_guess_field(int kind, int n_nodes, const double *node_xyz, const double *par, double amp,
+                            uint64_t seed, double *U);
 int glh_set_solution(glh_ctx *ctx, const double *U_host);       /* local_solution */
 /* keep a device copy of the current solution / restore it (resets a solve
  * without a host transfer) */

@@ -265,3 +265,42 @@ def test_periodic_slab_converges_to_a_laterally_uniform_state(ctx):
     for z in np.unique(np.round(xyz[:, 2], 9)):
         sel = np.abs(xyz[:, 2] - z) < 1e-9
         assert np.ptp(a2[sel]) <= 1e-7 * max(a2.max(), 1.0)
+
+
+@pytest.mark.parametrize("guess", ["BinA", "PDW"])
+def test_run_prm_with_analytic_guesses(ctx, guess):
+    # the setup variants' initial configurations (B-in-A ball on a cube with
+    # A = 0 on every face; weak-coupling PDW on a z-walled slab), one cycle
+    # each; the context's final state is a fixed point at converge accuracy
+    geo = "cube" if guess == "BinA" else "retangle"
+    ids = "0 0 0 0 0 0" if guess == "BinA" else "1 1 1 1 0 0"
+    prm = G.Prm("""
+subsection physical parameters
+  set pressure in bar = 20.0
+  set t_reduced = 0.5
+  set trun on Strong Coupling Correction = %s
+end
+subsection control parameters
+  set cube half side length = 6
+  set half x length of retangle = 8
+  set half y length of retangle = 8
+  set half z length of retangle = 5
+  set Number of initial global refinments = 2
+  set Number of refinements = 0
+  set Number of interations = 80
+  set converge accuracy = 1e-8
+  set Cycle 0 linear solver tol = 1e-3
+end
+subsection glhip
+  set geometry = %s
+  set boundary ids = %s
+  set initial guess = %s
+  set noise amplitude = 0
+  set GMRES restart = 50
+end
+""" % ("true" if guess == "BinA" else "false", geo, ids, guess))
+    rows = ctx.run_prm(prm)
+    for r in rows:
+        say(guess, r)
+    assert len(rows) == 1 and rows[0]["residual"] <= 1e-8
+    assert ctx.residual_norm() == pytest.approx(rows[0]["residual"], rel=1e-6)

@@ -77,7 +77,7 @@ void declare(std::map<std::string, Entry> &m)
   d(X, "geometry", "retangle", 's');                 // retangle | cube (makegrid_* variants)
   d(X, "boundary ids", "1 1 1 1 4 4", 's');          // faces -x +x -y +y -z +z
   d(X, "finite element degree", "1", 'i');
-  d(X, "initial guess", "B", 's');                   // B | A | BnA
+  d(X, "initial guess", "B", 's');                   // B | A | BnA | BinA | BnB | PDW
   d(X, "random seed", "0", 'i');
   d(X, "noise amplitude", "0.05", 'd');
   d(X, "mixed precision", "false", 'b');
@@ -220,7 +220,8 @@ int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int 
   const int restart = (int)std::max(1L, std::min(126L, p->i(X, "GMRES restart")));
   if (deg != 1) return GLH_ERR_ARG;  // adaptive meshes are Q1
   const std::string geo = p->s(X, "geometry"), guess = p->s(X, "initial guess");
-  if ((geo != "retangle" && geo != "cube") || (guess != "B" && guess != "A" && guess != "BnA")) return GLH_ERR_ARG;
+  if ((geo != "retangle" && geo != "cube") || (guess != "B" && guess != "A" && guess != "BnA" && guess != "BinA" && guess != "BnB" && guess != "PDW"))
+    return GLH_ERR_ARG;
   if (n_glob > 6 || n_cycles > 8) return GLH_ERR_ARG;
   double thr[4], tol[5];
   bool glob[5] = {false, false, false, false, false};
@@ -285,7 +286,16 @@ int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int 
       if (g == "BnA")
         rc = glh_initial_guess_bna(nn, m->m.node_xyz.data(), p->d(C, "A-phase block range ratio") * hi[2], mp[8],
                                    mp[9], amp, seed, U.data());
-      else
+      else if (g == "BinA") {  // setup_uniform_BinA-bubble-like-configuration.cc:214-235
+        const double par[3] = {p->d(C, "B-phase ball radius ratio") * p->d(C, "cube half side length"), mp[8], mp[9]};
+        rc = glh_initial_guess_field(GLH_GUESS_BINA, nn, m->m.node_xyz.data(), par, amp, seed, U.data());
+      } else if (g == "BnB") {  // setup_uniform_BnB-polkka-dot-like-configuration.cc:289-313
+        const double par[1] = {p->d(C, "B-phase inner plate radius ratio") * p->d(C, "half x length of retangle")};
+        rc = glh_initial_guess_field(GLH_GUESS_BNB, nn, m->m.node_xyz.data(), par, amp, seed, U.data());
+      } else if (g == "PDW") {  // setup_weak-coupling-PDW-configuration.cc:285-315
+        const double par[3] = {p->d(P, "amplitude u parameter"), p->d(C, "half z length of retangle"), mp[9]};
+        rc = glh_initial_guess_field(GLH_GUESS_PDW, nn, m->m.node_xyz.data(), par, amp, seed, U.data());
+      } else
         rc = glh_initial_guess_host(nn, g == "A" ? mp[8] : mp[9], g == "A" ? 1 : 0, amp, seed, U.data());
       if (rc) break;
     }

@@ -36,7 +36,7 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_mlp_get_params", "glh_mlp_set_params", "glh_mlp_set_data", "glh_mlp_train", "glh_mlp_loss_grad",
            "glh_mlp_predict", "glh_instance_residuals", "glh_set_forcing", "glh_set_residual_reference",
            "glh_set_schedule", "glh_autotune", "glh_prm_parse", "glh_prm_destroy", "glh_prm_get",
-           "glh_prm_get_double", "glh_run_prm", "glh_write_vtu"]
+           "glh_prm_get_double", "glh_run_prm", "glh_write_vtu", "glh_initial_guess_field"]
 
 _lib = None
 
@@ -127,6 +127,7 @@ def lib():
                                    C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, _dp, C.POINTER(vp), C.POINTER(vp)]
     L.glh_initial_guess_bna.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                         _dp]
+    L.glh_initial_guess_field.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_uint64, _dp]
     _lib = L
     return L
 
@@ -155,6 +156,23 @@ def initial_guess_bna(node_xyz, z_a, gap_A, gap_B, amp=0.05, seed=1):
     rc = lib().glh_initial_guess_bna(n, xyz, float(z_a), float(gap_A), float(gap_B), float(amp), int(seed), U)
     if rc:
         raise GLHError("glh_initial_guess_bna failed (%d)" % rc)
+    return U
+
+
+GUESS_KINDS = {"BinA": 1, "BnB": 2, "PDW": 3}
+
+
+def initial_guess_field(kind, node_xyz, par, amp=0.0, seed=0):
+    """Analytic guess at the nodes: kind "BinA" par (radius, gap_A, gap_B),
+    "BnB" par (radius,), "PDW" par (u, half_D, gap_B) (glh_initial_guess_field)."""
+    xyz = np.ascontiguousarray(node_xyz, np.float64).reshape(-1)
+    n = xyz.size // 3
+    U = np.zeros(n * 18)
+    pv = np.zeros(4)
+    pv[:len(par)] = par
+    rc = lib().glh_initial_guess_field(GUESS_KINDS[kind], n, xyz, pv, float(amp), int(seed), U)
+    if rc:
+        raise GLHError("glh_initial_guess_field failed (%d)" % rc)
     return U
 
 
