@@ -128,6 +128,13 @@ int glh_forest_mesh(const glh_forest *f, glh_mesh **out);
  * where fn refines fo. */
 int glh_forest_transfer(const glh_forest *fo, const glh_mesh *mo, const double *Uo, const glh_forest *fn,
                         const glh_mesh *mn, double *Un);
+/* The same transfer on the device (refine.cc:163-175 without the host round
+ * trip of the field): ctx holds mo and its solution; uploads mn
+ * (glh_upload_mesh), interpolates on the GPU and distributes the
+ * constraints, which leaves the state glh_set_solution would leave with
+ * glh_forest_transfer's field, bit for bit.  Single-instance Q1 meshes. */
+int glh_transfer_solution(glh_ctx *ctx, const glh_forest *fo, const glh_mesh *mo, const glh_forest *fn,
+                          const glh_mesh *mn);
 /* KellyErrorEstimator<3>::estimate (refine.cc:146-151) of the Q1 field U:
  * eta[n_cells] in float, as the reference's Vector<float>. */
 int glh_kelly(const glh_forest *f, const glh_mesh *m, const double *U, float *eta);

@@ -304,3 +304,31 @@ end
         say(guess, r)
     assert len(rows) == 1 and rows[0]["residual"] <= 1e-8
     assert ctx.residual_norm() == pytest.approx(rows[0]["residual"], rel=1e-6)
+
+
+def test_device_transfer_equals_host_transfer(ctx):
+    # two refinements: the second starts from a mesh with hanging nodes; the
+    # device path must leave the state set_solution leaves with the host field
+    lo, hi = (-5.0, -5.0, -6.0), (5.0, 5.0, 6.0)
+    f = G.Forest((3, 3, 2), lo, hi, SIDES)
+    m = f.mesh()
+    rng = np.random.default_rng(7)
+    xyz = m.arrays()["node_xyz"].reshape(-1, 3)
+    A = rng.standard_normal((3, 18))
+    U = (np.sin(xyz @ A) + 0.1 * rng.standard_normal((m.n_nodes, 18))).reshape(-1)
+    ctx.set_material_pt(22.0, 0.8)
+    ctx.upload_mesh(m)
+    ctx.set_solution(U)
+    for step in range(2):
+        Uc = ctx.solution()
+        f2 = f.refine((rng.random(f.n_cells) < 0.35).astype(np.uint8))
+        m2 = f2.mesh()
+        say("transfer step", step, "cells", f.n_cells, "->", f2.n_cells, "nodes", m2.n_nodes)
+        Uh = f.transfer(m, Uc, f2, m2)
+        ctx.transfer_solution(f, m, f2, m2)
+        Ud = ctx.solution()
+        ctx.set_solution(Uh)
+        assert np.array_equal(Ud, ctx.solution())
+        if step == 0:
+            assert m2.constraints()["n_hanging"] > 0
+        f, m = f2, m2

@@ -265,8 +265,8 @@ int forest_mesh(const Forest &f, Mesh &m, std::string &err)
   return 0;
 }
 
-int forest_transfer(const Forest &fo, const Mesh &mo, const double *Uo, const Forest &fn, const Mesh &mn,
-                    double *Un, std::string &err)
+int forest_transfer_map(const Forest &fo, const Mesh &mo, const Forest &fn, const Mesh &mn, std::vector<int> &cell,
+                        std::vector<float> &t, std::string &err)
 {
   const int LM = Forest::LMAX;
   if (fo.nx != fn.nx || fo.ny != fn.ny || fo.nz != fn.nz) { err = "forests over different boxes"; return -1; }
@@ -277,6 +277,8 @@ int forest_transfer(const Forest &fo, const Mesh &mo, const double *Uo, const Fo
   const int64_t NX = (int64_t)fn.nx << LM, NY = (int64_t)fn.ny << LM, NZ = (int64_t)fn.nz << LM;
   const double dl[3] = {(fn.hi[0] - fn.lo[0]) / (double)NX, (fn.hi[1] - fn.lo[1]) / (double)NY,
                         (fn.hi[2] - fn.lo[2]) / (double)NZ};
+  cell.resize(mn.n_nodes);
+  t.resize((size_t)mn.n_nodes * 3);
   for (int v = 0; v < mn.n_nodes; ++v) {
     // lattice coordinates of the vertex
     int64_t P[3];
@@ -286,9 +288,24 @@ int forest_transfer(const Forest &fo, const Mesh &mo, const double *Uo, const Fo
     if (K < 0) { err = "vertex outside the old forest"; return -1; }
     const auto &L = fo.leaves[K];
     const int64_t s = (int64_t)1 << (LM - L.level);
-    const double t[3] = {(double)(P[0] - L.ix * s) / (double)s, (double)(P[1] - L.iy * s) / (double)s,
-                         (double)(P[2] - L.iz * s) / (double)s};
-    const int *cn = &mo.cell_nodes[(size_t)K * 8];
+    // k / 2^(LM - level) with k <= 2^LM: exact in float
+    cell[v] = K;
+    t[3 * (size_t)v] = (float)((double)(P[0] - L.ix * s) / (double)s);
+    t[3 * (size_t)v + 1] = (float)((double)(P[1] - L.iy * s) / (double)s);
+    t[3 * (size_t)v + 2] = (float)((double)(P[2] - L.iz * s) / (double)s);
+  }
+  return 0;
+}
+
+int forest_transfer(const Forest &fo, const Mesh &mo, const double *Uo, const Forest &fn, const Mesh &mn,
+                    double *Un, std::string &err)
+{
+  std::vector<int> cell;
+  std::vector<float> tt;
+  if (forest_transfer_map(fo, mo, fn, mn, cell, tt, err)) return -1;
+  for (int v = 0; v < mn.n_nodes; ++v) {
+    const double t[3] = {tt[3 * (size_t)v], tt[3 * (size_t)v + 1], tt[3 * (size_t)v + 2]};
+    const int *cn = &mo.cell_nodes[(size_t)cell[v] * 8];
     for (int c = 0; c < NC; ++c) {
       double u = 0.0;
       for (int a = 0; a < 8; ++a) {

@@ -53,6 +53,11 @@ int forest_refine(Forest &f, const std::vector<char> &flag, std::string &err);
 // Dirichlet bits; m.row_ptr/col is the condensed system pattern and
 // m.arow_ptr/acol the pattern the cell loop assembles into.
 int forest_mesh(const Forest &f, Mesh &m, std::string &err);
+// The interpolation of forest_transfer: for each node of mn, the fo cell
+// that contains it and the node's coordinates t[3] in [0,1]^3 within it
+// (dyadic fractions of level <= LMAX, exact in float).
+int forest_transfer_map(const Forest &fo, const Mesh &mo, const Forest &fn, const Mesh &mn, std::vector<int> &cell,
+                        std::vector<float> &t, std::string &err);
 // Interpolate a Q1 solution of the mesh of `fo` (mesh mo) to the mesh of
 // `fn` (mesh mn), which refines it: every vertex of mn takes the trilinear
 // interpolant of the leaf of fo that contains it (SolutionTransfer of Q1).

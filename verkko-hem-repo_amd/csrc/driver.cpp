@@ -3,7 +3,7 @@
 // Newton solve runs on the GPU, then the Kelly indicators of the converged
 // field flag the fixed fraction of cells (refine.cc:146-157), the forest is
 // refined with 2:1 balance and the field is interpolated onto the new mesh
-// (refine.cc:163-170).  Cycle 0 starts from the caller's guess.
+// on the device (refine.cc:163-170).  Cycle 0 starts from the caller's guess.
 //
 // The reference stops each cycle on an absolute ||R|| or a stagnation test
 // (run.cc:236-255); here every cycle converges to ||R||/||R0|| <= rtol, the
@@ -47,24 +47,25 @@ extern "C" int glh_run_adaptive(glh_ctx *ctx, const glh_forest *f0, const double
   const double nan = std::numeric_limits<double>::quiet_NaN();
   glh_forest *f = new glh_forest(*f0);
   glh_mesh *m = nullptr;
-  std::vector<double> U;  // converged field of the previous cycle (host)
+  // mesh of the current cycle; from cycle 1 on it is uploaded, with the
+  // field transferred on the device, at the end of the previous cycle
+  glh_mesh *next = nullptr;
   int rc = GLH_OK;
   for (int cyc = 0; cyc <= n_refine; ++cyc) {
     const double t0 = seconds();
     double *st = cycle_stats + (size_t)cyc * GLH_CYCLE_STATS;
     for (int k = 0; k < GLH_CYCLE_STATS; ++k) st[k] = nan;
-    glh_mesh *mn = nullptr;
-    if ((rc = glh_forest_mesh(f, &mn)) != GLH_OK) break;
-    const glh::Mesh &M = mn->m;
-    std::vector<double> Ustart((size_t)M.n_nodes * 18);
+    glh_mesh *mn = next;
+    next = nullptr;
     if (cyc == 0) {
-      std::copy(U0, U0 + Ustart.size(), Ustart.begin());
-    } else {
-      // the previous forest is kept in `prev` for the transfer
-      std::copy(U.begin(), U.end(), Ustart.begin());
+      if ((rc = glh_forest_mesh(f, &mn)) != GLH_OK) break;
+      if ((rc = glh_upload_mesh(ctx, mn)) != GLH_OK || (rc = glh_set_solution(ctx, U0)) != GLH_OK) {
+        glh_mesh_destroy(mn);
+        break;
+      }
     }
-    if ((rc = glh_upload_mesh(ctx, mn)) != GLH_OK || (rc = glh_set_solution(ctx, Ustart.data())) != GLH_OK ||
-        (rc = glh_store_initial(ctx)) != GLH_OK) {
+    const glh::Mesh &M = mn->m;
+    if ((rc = glh_store_initial(ctx)) != GLH_OK) {
       glh_mesh_destroy(mn);
       break;
     }
@@ -76,7 +77,7 @@ extern "C" int glh_run_adaptive(glh_ctx *ctx, const glh_forest *f0, const double
       break;
     }
     const double t_solve = seconds() - ts;
-    std::vector<double> Uc(Ustart.size());
+    std::vector<double> Uc((size_t)M.n_nodes * 18);
     if ((rc = glh_get_solution(ctx, Uc.data())) != GLH_OK) {
       glh_mesh_destroy(mn);
       break;
@@ -119,13 +120,16 @@ extern "C" int glh_run_adaptive(glh_ctx *ctx, const glh_forest *f0, const double
         glh_mesh_destroy(mn);
         break;
       }
-      U.assign((size_t)mnext->m.n_nodes * 18, 0.0);
-      rc = glh_forest_transfer(f, mn, Uc.data(), fn, mnext, U.data());
-      glh_mesh_destroy(mnext);
+      // the context holds Uc: refine.cc:163-175 on the device
+      rc = glh_transfer_solution(ctx, f, mn, fn, mnext);
       glh_mesh_destroy(mn);
       glh_forest_destroy(f);
       f = fn;
-      if (rc != GLH_OK) break;
+      if (rc != GLH_OK) {
+        glh_mesh_destroy(mnext);
+        break;
+      }
+      next = mnext;
     } else {
       m = mn;
     }

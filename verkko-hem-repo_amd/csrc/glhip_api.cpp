@@ -1112,6 +1112,55 @@ int glh_forest_transfer(const glh_forest *fo, const glh_mesh *mo, const double *
   return glh::forest_transfer(fo->f, mo->m, Uo, fn->f, mn->m, Un, err) ? GLH_ERR_ARG : GLH_OK;
 }
 
+int glh_transfer_solution(glh_ctx *c, const glh_forest *fo, const glh_mesh *mo, const glh_forest *fn,
+                          const glh_mesh *mn)
+{
+  RCHK(check_ready(c, false));
+  if (!fo || !mo || !fn || !mn) return fail(c, GLH_ERR_ARG, "null forest or mesh");
+  if (mo->m.deg != 1 || c->n_nodes != mo->m.n_nodes || c->n_cells != mo->m.n_cells || c->n_seg != 1)
+    return fail(c, GLH_ERR_STATE, "the context does not hold the old mesh");
+  std::string err;
+  std::vector<int> cell;
+  std::vector<float> t;
+  if (glh::forest_transfer_map(fo->f, mo->m, fn->f, mn->m, cell, t, err)) return fail(c, GLH_ERR_ARG, err);
+  hipStream_t s = c->stream;
+  // the old field and cell corners outlive the upload of the new mesh
+  DBuf<double> Uo;
+  DBuf<int> cno, dcell;
+  DBuf<float> dt;
+  auto release = [&]() {
+    Uo.release();
+    cno.release();
+    dcell.release();
+    dt.release();
+  };
+  const size_t no = (size_t)mo->m.n_nodes * NC, nco = (size_t)mo->m.n_cells * 8;
+  hipError_t e = Uo.alloc(no);
+  if (e == hipSuccess) e = cno.alloc(nco);
+  if (e == hipSuccess) e = hipMemcpyAsync(Uo.p, c->U.p, no * sizeof(double), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(cno.p, c->cell_nodes.p, nco * sizeof(int), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    release();
+    return fail(c, GLH_ERR_HIP, std::string("transfer staging: ") + hipGetErrorString(e));
+  }
+  int rc = glh_upload_mesh(c, mn);
+  if (rc) {
+    release();
+    return rc;
+  }
+  e = upload(dcell, cell.data(), cell.size(), s);
+  if (e == hipSuccess) e = upload(dt, t.data(), t.size(), s);
+  if (e == hipSuccess) e = glh::launch_transfer(mn->m.n_nodes, dcell.p, dt.p, cno.p, Uo.p, c->U.p, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  release();
+  if (e != hipSuccess) return fail(c, GLH_ERR_HIP, std::string("transfer: ") + hipGetErrorString(e));
+  // constraints_solution.distribute (refine.cc:173)
+  RCHK(distribute(c, c->U.p));
+  HIPCHK(c, hipStreamSynchronize(s));
+  return GLH_OK;
+}
+
 int glh_kelly(const glh_forest *f, const glh_mesh *m, const double *U, float *eta)
 {
   if (!f || !m || !U || !eta) return GLH_ERR_ARG;

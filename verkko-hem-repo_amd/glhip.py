@@ -36,7 +36,8 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_mlp_get_params", "glh_mlp_set_params", "glh_mlp_set_data", "glh_mlp_train", "glh_mlp_loss_grad",
            "glh_mlp_predict", "glh_instance_residuals", "glh_set_forcing", "glh_set_residual_reference",
            "glh_set_schedule", "glh_autotune", "glh_prm_parse", "glh_prm_destroy", "glh_prm_get",
-           "glh_prm_get_double", "glh_run_prm", "glh_write_vtu", "glh_initial_guess_field"]
+           "glh_prm_get_double", "glh_run_prm", "glh_write_vtu", "glh_initial_guess_field",
+           "glh_transfer_solution"]
 
 _lib = None
 
@@ -127,6 +128,7 @@ def lib():
                                    C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, _dp, C.POINTER(vp), C.POINTER(vp)]
     L.glh_initial_guess_bna.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                         _dp]
+    L.glh_transfer_solution.argtypes = [vp, vp, vp, vp, vp]
     L.glh_initial_guess_field.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_uint64, _dp]
     _lib = L
     return L
@@ -475,6 +477,14 @@ class Context:
         self.n_dofs = mesh.n_nodes * 18
         self.nnzb = mesh.nnzb
         self.n_inst = mesh.n_inst
+
+    def transfer_solution(self, fo, mo, fn, mn):
+        """Refine on the device: upload mn and carry the solution of mo over
+        (glh_transfer_solution)."""
+        self._chk(lib().glh_transfer_solution(self.h, fo.h, mo.h, fn.h, mn.h), "glh_transfer_solution")
+        self.n_dofs = mn.n_nodes * 18
+        self.nnzb = mn.nnzb
+        self.n_inst = mn.n_inst
 
     def set_solution(self, U):
         self._chk(lib().glh_set_solution(self.h, np.ascontiguousarray(U, np.float64)), "glh_set_solution")
