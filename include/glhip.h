@@ -338,12 +338,13 @@ int glh_last_assembly_ms(glh_ctx *ctx, double ms[2]);
  * chunks (0: 4096 / 1024). */
 int glh_set_schedule(glh_ctx *ctx, int restart, int q_block, int row_cap, int chunk_cap);
 /* Autotuner: times `trial_newton` Newton steps (after one untimed step)
- * from the current solution for every candidate schedule (restart 30 / 60 /
- * 100; for batches also blocks of 4 / 8 / 16 steps and grid caps 2048 /
- * 8192), keeps the fastest in the context and restores the solution.
- * out: [0] restart [1] block [2] row cap [3] chunk cap [4] best trial
- * seconds [5] candidates tried.  Its own choice of restart is used by
- * glh_newton_solve calls with restart <= 0. */
+ * from the current solution per candidate, in two stages: the GMRES restart
+ * (30 / 60 / 100) with linear solves to lin_tol / 10, where the restart
+ * length matters; then, for batches, blocks of 4 / 8 / 16 Arnoldi steps and
+ * grid caps 2048 / 8192 at lin_tol.  Keeps the fastest in the context and
+ * restores the solution.  out: [0] restart [1] block [2] row cap [3] chunk
+ * cap [4] best trial seconds (last stage) [5] trials run.  Its choice of
+ * restart is used by glh_newton_solve calls with restart <= 0. */
 int glh_autotune(glh_ctx *ctx, int fp32, int trial_newton, double lin_tol, int lin_maxit, double out[6]);
 /* Forcing terms of glh_newton_solve: 0 (default) the fixed lin_tol of the
  * reference (solve.cc:119-128); 1 Eisenstat-Walker (choice 2: eta_k =

@@ -92,6 +92,29 @@ def test_batch_newton_equals_single_instance_solves(ctx):
         single.close()
 
 
+def test_autotuned_schedule_restores_state_and_keeps_the_fixed_point(ctx):
+    meshes, batch, cos, Us = _batch(ctx, CASES)
+    U0 = ctx.solution()
+    tuned = ctx.autotune(fp32=True, trial_newton=1, lin_tol=1e-2)
+    say("autotune", tuned)
+    assert tuned["restart"] in (30, 60, 100) and tuned["q_block"] in (4, 8, 16)
+    assert tuned["row_cap"] in (2048, 8192) and tuned["chunk_cap"] == tuned["row_cap"] // 4
+    assert tuned["candidates"] == 9
+    assert np.array_equal(ctx.solution(), U0)  # the trials leave the solution as it was
+    # restart <= 0: the tuned restart; the schedule changes only the launch
+    # shapes, so the fixed point is the one of a fixed schedule
+    st = ctx.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=0, fp32=True)
+    F_tuned = ctx.instance_energy()
+    ctx.set_schedule(restart=tuned["restart"], q_block=8, row_cap=4096, chunk_cap=1024)
+    ctx.set_solution(U0)
+    st_fixed = ctx.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=tuned["restart"], fp32=True)
+    F_fixed = ctx.instance_energy()
+    say(st, st_fixed)
+    assert st["converged"] == 1.0 and st_fixed["converged"] == 1.0
+    assert np.allclose(F_tuned, F_fixed, rtol=1e-10, atol=0)
+    ctx.set_schedule(restart=100, q_block=0, row_cap=0, chunk_cap=0)
+
+
 def test_batch_fp32_mixed_precision_reaches_fp64_fixed_point(ctx):
     meshes, batch, cos, Us = _batch(ctx, CASES)
     st64 = ctx.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=40)

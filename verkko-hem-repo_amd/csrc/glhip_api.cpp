@@ -1756,39 +1756,61 @@ int glh_autotune(glh_ctx *c, int fp32, int trial_newton, double lin_tol, int lin
   struct Cand {
     int m, q, rc, cc;
   };
-  std::vector<Cand> cands;
-  const int ms[3] = {30, 60, 100};
-  for (int m : ms) {
-    if (c->n_seg == 1) {
-      cands.push_back({m, 0, 0, 0});
-      continue;
-    }
-    for (int q : {4, 8, 16})
-      for (int rc : {2048, 8192}) cands.push_back({m, q, rc, rc / 4});
-  }
   RCHK(glh_store_initial(c));
   const int old_verbose = c->verbose;
   c->verbose = 0;
-  double tbest = 1e300;
-  Cand best = cands[0];
-  int rc = GLH_OK;
-  for (const Cand &k : cands) {
+  int rc = GLH_OK, tried = 0;
+  // one untimed step records the candidate's graphs, then the timed trial
+  auto trial = [&](const Cand &k, double tol, double &t) {
     c->q_block = k.q;
     c->row_cap = k.rc;
     c->chunk_cap = k.cc;
     double st[10];
-    // one untimed step records the candidate's graphs, then the timed trial
-    if ((rc = glh_restore_initial(c)) != GLH_OK) break;
-    if ((rc = glh_newton_solve(c, 1e-300, 1, lin_tol, lin_maxit, k.m, 0.83, 1, fp32, st)) != GLH_OK) break;
-    if ((rc = glh_restore_initial(c)) != GLH_OK) break;
+    ++tried;
+    if ((rc = glh_restore_initial(c)) != GLH_OK) return;
+    if ((rc = glh_newton_solve(c, 1e-300, 1, tol, lin_maxit, k.m, 0.83, 1, fp32, st)) != GLH_OK) return;
+    if ((rc = glh_restore_initial(c)) != GLH_OK) return;
     const double t0 = now_s();
-    if ((rc = glh_newton_solve(c, 1e-300, trial_newton, lin_tol, lin_maxit, k.m, 0.83, 1, fp32, st)) != GLH_OK)
-      break;
-    const double t = now_s() - t0;
+    if ((rc = glh_newton_solve(c, 1e-300, trial_newton, tol, lin_maxit, k.m, 0.83, 1, fp32, st)) != GLH_OK) return;
+    t = now_s() - t0;
+  };
+  // Stage 1, the restart length: it changes the Krylov iteration itself, and
+  // the first Newton steps at the working tolerance converge in a few
+  // iterations whatever the restart, so the trial solves to a tenth of it
+  // (the later, harder linear systems of a solve).
+  const bool batch = c->n_seg > 1;
+  Cand best{100, batch ? 8 : 0, batch ? 8192 : 0, batch ? 2048 : 0};
+  double tbest = 1e300;
+  for (int m : {30, 60, 100}) {
+    Cand k = best;
+    k.m = m;
+    double t = 1e300;
+    trial(k, 0.1 * lin_tol, t);
+    if (rc) break;
     if (t < tbest) {
       tbest = t;
       best = k;
     }
+  }
+  // Stage 2, the launch schedule of a batch (same arithmetic, different
+  // speed): Arnoldi steps per replayed graph block and grid caps, timed at
+  // the working tolerance.
+  if (rc == GLH_OK && batch) {
+    tbest = 1e300;
+    const Cand base = best;
+    for (int q : {4, 8, 16})
+      for (int cap : {2048, 8192}) {
+        const Cand k{base.m, q, cap, cap / 4};
+        double t = 1e300;
+        trial(k, lin_tol, t);
+        if (rc) break;
+        if (t < tbest) {
+          tbest = t;
+          best = k;
+        }
+      }
+  } else if (rc == GLH_OK) {
+    trial(best, lin_tol, tbest);
   }
   c->verbose = old_verbose;
   c->tuned_restart = best.m;
@@ -1798,7 +1820,7 @@ int glh_autotune(glh_ctx *c, int fp32, int trial_newton, double lin_tol, int lin
   RCHK(rc);
   RCHK(glh_restore_initial(c));
   out[0] = best.m; out[1] = best.q; out[2] = best.rc; out[3] = best.cc; out[4] = tbest;
-  out[5] = (double)cands.size();
+  out[5] = (double)tried;
   return GLH_OK;
 }
 
