@@ -19,8 +19,9 @@
 // One workgroup (3 wavefronts) owns one cell.  Thread t < 171 owns one
 // unique (c <= d) component pair, keeps H_cd(q) for all q in registers and
 // contracts it against the (a <= b) node-pair weights (uniform, scalar-cached
-// table loads).  Blocks are staged in LDS eight at a time and written to the
-// BSR matrix with 16-byte vector stores: block (a,b) and its transpose (b,a).
+// table loads).  It writes entries [c][d] and [d][c] of block (a,b) and of
+// its transpose (b,a) straight to the BSR values, one node pair after
+// another, with the old values of the next pair already requested.
 // Cells of one colour share no node, so the read-modify-write scatter needs
 // no atomics and the result is bit-deterministic; the first colour to touch
 // a block stores instead of accumulating, so J needs no zero fill.
