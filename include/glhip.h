@@ -337,14 +337,15 @@ int glh_last_assembly_ms(glh_ctx *ctx, double ms[2]);
  * (0: 8), grid caps of the active-set launches in block rows / 4 and in
  * chunks (0: 4096 / 1024). */
 int glh_set_schedule(glh_ctx *ctx, int restart, int q_block, int row_cap, int chunk_cap);
-/* Autotuner: times `trial_newton` Newton steps (after one untimed step)
- * from the current solution per candidate, in two stages: the GMRES restart
- * (30 / 60 / 100) with linear solves to lin_tol / 10, where the restart
- * length matters; then, for batches, blocks of 4 / 8 / 16 Arnoldi steps and
- * grid caps 2048 / 8192 at lin_tol.  Keeps the fastest in the context and
- * restores the solution.  out: [0] restart [1] block [2] row cap [3] chunk
- * cap [4] best trial seconds (last stage) [5] trials run.  Its choice of
- * restart is used by glh_newton_solve calls with restart <= 0. */
+/* Autotuner of the batched launch schedule: times `trial_newton` Newton
+ * steps (after one untimed step that records the graphs) from the current
+ * solution for blocks of 4 / 8 / 16 Arnoldi steps per replayed graph and
+ * grid caps 2048 / 8192 block rows, keeps the fastest in the context and
+ * restores the solution.  These knobs change launch shapes only, not the
+ * arithmetic.  The restart is the context's (glh_set_schedule, default
+ * 100): it changes the iteration, which short trials do not predict.
+ * out: [0] restart [1] block [2] row cap [3] chunk cap [4] best trial
+ * seconds [5] trials run. */
 int glh_autotune(glh_ctx *ctx, int fp32, int trial_newton, double lin_tol, int lin_maxit, double out[6]);
 /* Forcing terms of glh_newton_solve: 0 (default) the fixed lin_tol of the
  * reference (solve.cc:119-128); 1 Eisenstat-Walker (choice 2: eta_k =

@@ -97,9 +97,9 @@ def test_autotuned_schedule_restores_state_and_keeps_the_fixed_point(ctx):
     U0 = ctx.solution()
     tuned = ctx.autotune(fp32=True, trial_newton=1, lin_tol=1e-2)
     say("autotune", tuned)
-    assert tuned["restart"] in (30, 60, 100) and tuned["q_block"] in (4, 8, 16)
+    assert tuned["restart"] == 100 and tuned["q_block"] in (4, 8, 16)
     assert tuned["row_cap"] in (2048, 8192) and tuned["chunk_cap"] == tuned["row_cap"] // 4
-    assert tuned["candidates"] == 9
+    assert tuned["candidates"] == 6
     assert np.array_equal(ctx.solution(), U0)  # the trials leave the solution as it was
     # restart <= 0: the tuned restart; the schedule changes only the launch
     # shapes, so the fixed point is the one of a fixed schedule

@@ -1774,33 +1774,21 @@ int glh_autotune(glh_ctx *c, int fp32, int trial_newton, double lin_tol, int lin
     if ((rc = glh_newton_solve(c, 1e-300, trial_newton, tol, lin_maxit, k.m, 0.83, 1, fp32, st)) != GLH_OK) return;
     t = now_s() - t0;
   };
-  // Stage 1, the restart length: it changes the Krylov iteration itself, and
-  // the first Newton steps at the working tolerance converge in a few
-  // iterations whatever the restart, so the trial solves to a tenth of it
-  // (the later, harder linear systems of a solve).
+  // The restart length is not tuned: it changes the Krylov iteration, and
+  // a trial of a few Newton steps does not predict its effect on the later,
+  // harder linear systems of a solve (measured on 256 slabs: short trials
+  // picked restart 30 or 60, and the full solve then took up to 2.7x as long
+  // as with 100).  The candidates are the
+  // launch schedule of a batch, which leaves the arithmetic as it is:
+  // Arnoldi steps per replayed graph block and grid caps.
   const bool batch = c->n_seg > 1;
-  Cand best{100, batch ? 8 : 0, batch ? 8192 : 0, batch ? 2048 : 0};
+  const int m = c->tuned_restart;
+  Cand best{m, batch ? 8 : 0, batch ? 8192 : 0, batch ? 2048 : 0};
   double tbest = 1e300;
-  for (int m : {30, 60, 100}) {
-    Cand k = best;
-    k.m = m;
-    double t = 1e300;
-    trial(k, 0.1 * lin_tol, t);
-    if (rc) break;
-    if (t < tbest) {
-      tbest = t;
-      best = k;
-    }
-  }
-  // Stage 2, the launch schedule of a batch (same arithmetic, different
-  // speed): Arnoldi steps per replayed graph block and grid caps, timed at
-  // the working tolerance.
-  if (rc == GLH_OK && batch) {
-    tbest = 1e300;
-    const Cand base = best;
+  if (batch) {
     for (int q : {4, 8, 16})
       for (int cap : {2048, 8192}) {
-        const Cand k{base.m, q, cap, cap / 4};
+        const Cand k{m, q, cap, cap / 4};
         double t = 1e300;
         trial(k, lin_tol, t);
         if (rc) break;
@@ -1809,7 +1797,7 @@ int glh_autotune(glh_ctx *c, int fp32, int trial_newton, double lin_tol, int lin
           best = k;
         }
       }
-  } else if (rc == GLH_OK) {
+  } else {
     trial(best, lin_tol, tbest);
   }
   c->verbose = old_verbose;
