@@ -146,6 +146,9 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
   __shared__ double sG[DO_R ? NQ * 54 : 1];
   __shared__ double sGram[NQ * GRAM];
   __shared__ double sBuf[NBUF];
+  // per-cell scatter data staged once: block codes and node constraint masks
+  __shared__ int sBlk[DO_J ? NS * NS : 1];
+  __shared__ uint32_t sMask[DO_J ? NS : 1];
 
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
@@ -159,6 +162,11 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
 
   // gather the cell's 18 components at its NS nodes
   for (int i = tid; i < NS * 18; i += NT) sU[i] = U[(size_t)cn[i / 18] * 18 + (i % 18)];
+  if (DO_J) {
+    const int *cb = Mh.cell_blk + (size_t)e * NS * NS;
+    for (int i = tid; i < NS * NS; i += NT) sBlk[i] = cb[i];
+    for (int i = tid; i < NS; i += NT) sMask[i] = Mh.node_mask[cn[i]];
+  }
   __syncthreads();
   // iterate and gradient at the quadrature points (s_vector2matrix.cc:116-217)
   for (int i = tid; i < NQ * 18; i += NT) {
@@ -246,7 +254,7 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
     const double lap0 = vol * ih[0] * ih[0], lap1 = vol * ih[1] * ih[1], lap2 = vol * ih[2] * ih[2];
     const double k23_cd = M.K23 * vol * ih[kc] * ih[kd];
     const uint32_t *mask = Mh.node_mask;
-    const int *cblk = Mh.cell_blk + (size_t)e * NS * NS;
+    const int *cblk = sBlk;
     const int o_cd = c * 18 + d, o_dc = d * 18 + c;
     // Robin diagonal weight of this component on each local face (0 if off)
     double rob[6];
@@ -303,7 +311,7 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
       double e1, e2;
       pair_values(p, e1, e2);
       const int a = T.pa[p], b = T.pb[p];
-      const uint32_t ma = mask[cn[a]], mb = mask[cn[b]];
+      const uint32_t ma = sMask[a], mb = sMask[b];
       {  // block (a,b): rows of node a, columns of node b
         TJ *B = blk_ptr(cblk[a * NS + b]);
         double v1 = e1, v2 = e2;
@@ -324,8 +332,9 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
       }
     };
     // PD node pairs in flight: the old values of pair p + PD are requested
-    // before pair p is contracted and stored (PD = 1 measured fastest)
-    constexpr int PD = 1;
+    // before pair p is contracted and stored (PD = 2 measured fastest with
+    // the block codes and masks in LDS; the registers stay at 3 waves/SIMD)
+    constexpr int PD = 2;
     double ob[PD][4];
 #pragma unroll
     for (int k = 0; k < PD; ++k) prefetch(k < NPAIR ? k : NPAIR - 1, ob[k]);
