@@ -149,6 +149,7 @@ def run_batch(G, args):
     ctx.set_materials(coefs)
     ctx.set_solution(np.concatenate(Us))
     ctx.store_initial()
+    ctx.set_continuation(args.continuation)
     out = {}
     tuned = {}
     for prec in ("fp64", "fp32"):
@@ -177,6 +178,7 @@ def run_batch(G, args):
     # the plain FP64 drop-in: every instance solved on its own, one after
     # another; a seeded random sample when fewer than all are requested
     single = G.Context(0)
+    single.set_continuation(args.continuation)
     nseq = min(len(meshes), args.sequential)
     sample = np.arange(len(meshes)) if nseq == len(meshes) else np.sort(
         np.random.default_rng(7).choice(len(meshes), nseq, replace=False))
@@ -460,6 +462,9 @@ def main():
                     help="instances solved one by one for the plain FP64 comparison (slab_batch)")
     ap.add_argument("--lin-tol", type=float, default=0.1)
     ap.add_argument("--restart", type=int, default=100)
+    ap.add_argument("--continuation", type=float, default=0.0,
+                    help="slab_batch: pseudo-transient continuation sigma0 for the batched and the one-at-a-time "
+                         "solves (0: plain damped Newton)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernels", action="store_true",
                     help="short fixed kernel sequence for rocprofv3: 3 FP64 and 3 FP32 J+R assemblies, 3 residual "
