@@ -43,6 +43,17 @@ __device__ inline int active_row(const ActiveSet &a, int g)
   return a.node_off[a.seg[lo]] + g - a.rpref[lo];
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of the device
+// (workgroup b runs on XCD b % 8, each XCD with its own L2).  The SpMV
+// renumbers them so that every XCD takes one contiguous range of block rows:
+// neighbouring rows share most of their x entries, which then meet in the
+// same L2 instead of being fetched into all eight.
+__device__ inline int xcd_block(int b, int n_blocks)
+{
+  const int q = n_blocks >> 3, r = n_blocks & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 // One wavefront per block row.  Lane = g*18 + c (g < 3): row c of every
 // block, columns 6g..6g+5.  Partial sums of the three column groups are
 // combined with cross-lane reads at the end of the row.
@@ -91,7 +102,7 @@ __global__ __launch_bounds__(256) void k_spmv(int n_rows, const int *__restrict_
 {
   const int lane = threadIdx.x & 63;
   if (!act.counts) {
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int row = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (row >= n_rows) return;
     if (seg_flag && !seg_flag[row_seg[row]]) return;  // instance not iterating
     spmv_row(row, lane, rp, col, val, x, y);
@@ -113,7 +124,7 @@ __global__ __launch_bounds__(256) void k_res_spmv(int n_rows, const int *__restr
                                                   const double *__restrict__ bv, double *__restrict__ r)
 {
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int row = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   if (row >= n_rows) return;
   const int g = lane / 18, c = lane % 18;
   double acc = 0.0;
