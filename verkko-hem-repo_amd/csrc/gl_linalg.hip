@@ -689,7 +689,7 @@ __global__ void k_gmres_ctl(GmresCtl<T> C, int op, int j)
     return;
   }
   if (op == 1) {
-    C.ro[s] = C.cyc[s] && (double)C.nrm2[s] < 0.5 * (double)C.hs[(size_t)s * ldh + j + 1];
+    C.ro[s] = C.cyc[s] && (double)C.nrm2[s] < C.dgks * (double)C.hs[(size_t)s * ldh + j + 1];
     return;
   }
   if (op == 2) {

@@ -208,6 +208,7 @@ struct GmresCtl {
   int *cyc, *conv, *jseg, *its, *ro;
   const int *frozen;
   int *jlast;  // [S] basis size of the previous cycle (written by op 0)
+  double dgks;  // second Gram-Schmidt pass when ||w'||^2 < dgks ||w||^2
 };
 template <typename T>
 hipError_t launch_gmres_ctl(const GmresCtl<T> &C, int op, int j, hipStream_t s);
