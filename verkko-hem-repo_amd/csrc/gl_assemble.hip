@@ -18,10 +18,12 @@
 //
 // One workgroup (3 wavefronts) owns one cell.  Thread t < 171 owns one
 // unique (c <= d) component pair, keeps H_cd(q) for all q in registers and
-// contracts it against the (a <= b) node-pair weights (uniform, scalar-cached
-// table loads).  It writes entries [c][d] and [d][c] of block (a,b) and of
-// its transpose (b,a) straight to the BSR values, one node pair after
-// another, with the old values of the next pair already requested.
+// contracts it against the (a <= b) node-pair weights, which the workgroup
+// stages in LDS 32 pairs at a time (the whole table, 82 KB for Q2, does not
+// fit the scalar cache).  It writes entries [c][d] and [d][c] of block (a,b)
+// and of its transpose (b,a) straight to the BSR values, one node pair after
+// another, with the old values of the next two pairs already requested; the
+// block codes and node masks of the cell sit in LDS as well.
 // Cells of one colour share no node, so the read-modify-write scatter needs
 // no atomics and the result is bit-deterministic; the first colour to touch
 // a block stores instead of accumulating, so J needs no zero fill.
@@ -148,6 +150,11 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
   __shared__ double sBuf[NBUF];
   // per-cell scatter data staged once: block codes and node constraint masks
   __shared__ int sBlk[DO_J ? NS * NS : 1];
+  // node-pair weights of the current chunk of pairs (the whole table does not
+  // fit the scalar cache)
+  constexpr int PCH = 32;
+  __shared__ double sWp[DO_J ? PCH * NQ : 1];
+  __shared__ double sSp[DO_J ? PCH * 9 : 1];
   __shared__ uint32_t sMask[DO_J ? NS : 1];
 
   const int tid = threadIdx.x;
@@ -235,14 +242,14 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
   if (DO_J) {
     // Thread t < 171 owns the unique component pair (c <= d).  For every node
     // pair (a <= b) it forms block(a,b)[c][d], block(a,b)[d][c] and writes them,
-    // with the transposed block(b,a), straight to the BSR values: no LDS
-    // staging and no barrier, so each wavefront keeps many stores in flight.
-    // The 64 lanes of a store cover consecutive entries of one 2.6 KB block,
-    // which L2 merges into full lines.
-    if (tid >= NUPAIR) return;
+    // with the transposed block(b,a), straight to the BSR values (no staging
+    // of the output, so each wavefront keeps many stores in flight; L2 merges
+    // the lanes' 8-byte stores into lines).  All 192 threads load each chunk
+    // of node-pair weights into LDS between two barriers.
+    const bool owner = tid < NUPAIR;
     int c = 0, d = 0;
     {
-      int t = tid;
+      int t = owner ? tid : 0;
       while (t >= 18 - c) { t -= 18 - c; ++c; }
       d = c + t;
     }
@@ -286,13 +293,11 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
       }
     };
     // entries [c][d] (e1) and [d][c] (e2) of the local block of node pair p
-    auto pair_values = [&](int p, double &e1, double &e2) {
-      const double *wp = T.Wp + p * NQ;
+    auto pair_values = [&](int p, const double *wp, const double *sp, double &e1, double &e2) {
       double sym = 0.0;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) sym += wp[q] * Hq[q];
       sym *= vol;
-      const double *sp = T.Sp + p * 9;
       const int a = T.pa[p], b = T.pb[p];
       if (c == d) {
         sym += M.K1 * (sp[0] * lap0 + sp[4] * lap1 + sp[8] * lap2);
@@ -309,7 +314,7 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
     };
     auto process = [&](int p, const double *ol) {
       double e1, e2;
-      pair_values(p, e1, e2);
+      pair_values(p, sWp + (p % PCH) * NQ, sSp + (p % PCH) * 9, e1, e2);
       const int a = T.pa[p], b = T.pb[p];
       const uint32_t ma = sMask[a], mb = sMask[b];
       {  // block (a,b): rows of node a, columns of node b
@@ -336,19 +341,30 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
     // the block codes and masks in LDS; the registers stay at 3 waves/SIMD)
     constexpr int PD = 2;
     double ob[PD][4];
+    if (owner) {
 #pragma unroll
-    for (int k = 0; k < PD; ++k) prefetch(k < NPAIR ? k : NPAIR - 1, ob[k]);
-    for (int p0 = 0; p0 < NPAIR; p0 += PD) {
+      for (int k = 0; k < PD; ++k) prefetch(k < NPAIR ? k : NPAIR - 1, ob[k]);
+    }
+    for (int c0 = 0; c0 < NPAIR; c0 += PCH) {
+      const int np = NPAIR - c0 < PCH ? NPAIR - c0 : PCH;
+      __syncthreads();  // the previous chunk has been used
+      for (int i = tid; i < np * NQ; i += NT) sWp[i] = T.Wp[(size_t)c0 * NQ + i];
+      for (int i = tid; i < np * 9; i += NT) sSp[i] = T.Sp[(size_t)c0 * 9 + i];
+      __syncthreads();
+      if (!owner) continue;
+      for (int p0 = c0; p0 < c0 + np; p0 += PD) {
 #pragma unroll
-      for (int k = 0; k < PD; ++k) {
-        const int p = p0 + k;
-        if (p < NPAIR) {
-          double ol[4] = {ob[k][0], ob[k][1], ob[k][2], ob[k][3]};
-          if (p + PD < NPAIR) prefetch(p + PD, ob[k]);
-          process(p, ol);
+        for (int k = 0; k < PD; ++k) {
+          const int p = p0 + k;
+          if (p < c0 + np) {
+            double ol[4] = {ob[k][0], ob[k][1], ob[k][2], ob[k][3]};
+            if (p + PD < NPAIR) prefetch(p + PD, ob[k]);
+            process(p, ol);
+          }
         }
       }
     }
+    if (!owner) return;
     // hanging nodes (adaptive meshes only): the condensed row keeps the sum
     // over cells of |local diagonal|, gathered outside the hot loop
     if (HANG && c == d)
@@ -356,7 +372,7 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
         const int a = T.pa[p];
         if (a != T.pb[p] || !(mask[cn[a]] & HANG_BIT)) continue;
         double e1, e2;
-        pair_values(p, e1, e2);
+        pair_values(p, T.Wp + p * NQ, T.Sp + p * 9, e1, e2);
         Mh.hdiag[(size_t)cn[a] * 18 + c] += fabs(e1);
       }
   }
