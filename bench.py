@@ -462,6 +462,8 @@ def main():
                     help="instances solved one by one for the plain FP64 comparison (slab_batch)")
     ap.add_argument("--lin-tol", type=float, default=0.1)
     ap.add_argument("--restart", type=int, default=100)
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="cube workloads: skip the FP32 mixed-precision solve reported beside the FP64 headline")
     ap.add_argument("--continuation", type=float, default=0.0,
                     help="slab_batch: pseudo-transient continuation sigma0 for the batched and the one-at-a-time "
                          "solves (0: plain damped Newton)")
@@ -566,6 +568,22 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "algorithmic_bytes": traffic_alg},
     }
+    if not args.no_mixed:
+        # the same solve with FP32 J and GMRES inside FP64 iterative refinement
+        # (reported beside the FP64 headline, not part of its value)
+        U64 = ctx.solution()
+        ctx.set_timing(False)
+        ctx.restore_initial()
+        t1 = time.perf_counter()
+        s32 = ctx.newton_solve(fp32=True, **kw)
+        el32 = time.perf_counter() - t1
+        U32 = ctx.solution()
+        line["mixed_precision"] = {
+            "ms_per_solve": 1000.0 * el32, "converged": bool(s32["r_final"] <= 1e-10 * s32["r0"]),
+            "newton_steps": s32["newton_steps"], "gmres_its": s32["gmres_its"], "free_energy": s32["F"],
+            "F_rel_err_vs_fp64": abs(s32["F"] - st0["F"]) / abs(st0["F"]),
+            "U_rel_err_vs_fp64": float(np.abs(U32 - U64).max() / np.abs(U64).max())}
+        print("bench: mixed precision %s" % line["mixed_precision"], file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         ls = int(st0.get("ls_trials", st0["newton_steps"]))
         line["cpu_baseline"] = cpu_baseline(deg, 2 * half / ncell, list(ctx_coef(m)), mesh.n_cells, mesh.n_nodes,
