@@ -119,9 +119,15 @@ int64_t glh_forest_n_cells(const glh_forest *f);
  * (faces, edges, vertices) more than one level apart are refined
  * (execute_coarsening_and_refinement with 2:1 balance).  f is unchanged. */
 int glh_forest_refine(const glh_forest *f, const uint8_t *flags, glh_forest **out);
-/* Q1 mesh of the leaves (cell e = leaf e, Morton order) with closed
+/* Element degree (1 or 2, default 1) of the meshes glh_forest_mesh builds
+ * from f and from the forests refined from it (FE_Q(degree) of
+ * femgl/src/femgl.cc:112). */
+int glh_forest_set_degree(glh_forest *f, int degree);
+/* Q1 or Q2 mesh of the leaves (cell e = leaf e, Morton order) with closed
  * hanging-node constraints (DoFTools::make_hanging_node_constraints,
- * AffineConstraints::close; setup_uniform_B-phase.cc:137-160). */
+ * AffineConstraints::close; setup_uniform_B-phase.cc:137-160): a node on a
+ * face or edge of a coarser neighbour follows that cell's Q_k field there
+ * (Q1: 1/2, 1/4; Q2: products of 3/8, 3/4, -1/8 under 2:1 balance). */
 int glh_forest_mesh(const glh_forest *f, glh_mesh **out);
 /* SolutionTransfer::interpolate + constraints_solution.distribute
  * (refine.cc:163-170): Un on the mesh mn of fn from Uo on the mesh mo of fo,
@@ -132,7 +138,7 @@ int glh_forest_transfer(const glh_forest *fo, const glh_mesh *mo, const double *
  * trip of the field): ctx holds mo and its solution; uploads mn
  * (glh_upload_mesh), interpolates on the GPU and distributes the
  * constraints, which leaves the state glh_set_solution would leave with
- * glh_forest_transfer's field, bit for bit.  Single-instance Q1 meshes. */
+ * glh_forest_transfer's field, bit for bit.  Single-instance meshes. */
 int glh_transfer_solution(glh_ctx *ctx, const glh_forest *fo, const glh_mesh *mo, const glh_forest *fn,
                           const glh_mesh *mn);
 /* KellyErrorEstimator<3>::estimate (refine.cc:146-151) of the Q1 field U:

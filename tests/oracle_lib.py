@@ -280,12 +280,25 @@ def newton(mesh, co, U, rtol=1e-10, max_newton=50, lin_tol=0.1, lin_maxit=5000, 
 
 
 # ---------------------------------------------------------------- adaptive
-def hanging_lines(cell_nodes, xyz, cell_lo, cell_h, tol=1e-9):
-    """Independent (test-side) hanging-node detection for Q1 hex meshes: a node
-    inside an edge or face of a cell that does not list it follows the
-    trilinear interpolant of that cell.  Lines are closed (masters that hang
-    themselves are expanded).  Returns {node: {master: weight}}."""
-    cells = np.asarray(cell_nodes).reshape(-1, 8)
+def lagrange_1d(deg, i, t):
+    """Lagrange basis function i of degree deg on the equidistant nodes of [0, 1]."""
+    nodes = [k / deg for k in range(deg + 1)]
+    out = 1.0
+    for k in range(deg + 1):
+        if k != i:
+            out *= (t - nodes[k]) / (nodes[i] - nodes[k])
+    return out
+
+
+def hanging_lines(cell_nodes, xyz, cell_lo, cell_h, tol=1e-9, deg=1):
+    """Independent (test-side) hanging-node detection for Q1/Q2 hex meshes: a
+    node on the closure of a cell that the cell does not list follows that
+    cell's Q_deg interpolant (the full tensor-product Lagrange basis evaluated
+    at the node).  Lines are closed (masters that hang themselves are
+    expanded).  Returns {node: {master: weight}}."""
+    n1 = deg + 1
+    ns = n1 ** 3
+    cells = np.asarray(cell_nodes).reshape(-1, ns)
     lo = np.asarray(cell_lo).reshape(-1, 3)
     h = np.asarray(cell_h).reshape(-1, 3)
     xyz = np.asarray(xyz).reshape(-1, 3)
@@ -299,12 +312,12 @@ def hanging_lines(cell_nodes, xyz, cell_lo, cell_h, tol=1e-9):
                 continue
             tv = np.clip(t[v], 0.0, 1.0)
             line = {}
-            for a in range(8):
-                bits = (a & 1, (a >> 1) & 1, a >> 2)
+            for a in range(ns):
+                idx = (a % n1, (a // n1) % n1, a // (n1 * n1))
                 w = 1.0
                 for d in range(3):
-                    w *= tv[d] if bits[d] else 1.0 - tv[d]
-                if w > 1e-14:
+                    w *= lagrange_1d(deg, idx[d], tv[d])
+                if abs(w) > 1e-14:
                     line[int(cells[e][a])] = line.get(int(cells[e][a]), 0.0) + w
             raw[int(v)] = line
     closed = {}
@@ -336,7 +349,7 @@ def adaptive_oracle_mesh(cell_nodes, xyz, cell_lo, cell_h, faces, box_lo, box_hi
     component masks of the faces -x,+x,-y,+y,-z,+z."""
     xyz = np.asarray(xyz).reshape(-1, 3)
     n_nodes = xyz.shape[0]
-    hang = hanging_lines(cell_nodes, xyz, cell_lo, cell_h)
+    hang = hanging_lines(cell_nodes, xyz, cell_lo, cell_h, deg=deg)
     lines = {}                                      # dof -> {master dof: w}
     for v, line in hang.items():
         for c in range(18):
@@ -376,7 +389,7 @@ def adaptive_oracle_mesh(cell_nodes, xyz, cell_lo, cell_h, faces, box_lo, box_hi
             con_weight.append(w)
         con_ptr.append(len(con_master))
     # node pattern able to hold every condensed entry
-    cells = np.asarray(cell_nodes).reshape(-1, 8)
+    cells = np.asarray(cell_nodes).reshape(-1, (deg + 1) ** 3)
     tgt = {}
     for a in range(n_nodes):
         t = set()
@@ -403,28 +416,40 @@ def adaptive_oracle_mesh(cell_nodes, xyz, cell_lo, cell_h, faces, box_lo, box_hi
                 np.array(con_ptr, np.int32), con_master, con_weight, row_ptr, np.array(cols, np.int32), xyz)
 
 
-def kelly(cell_nodes, cell_lo, cell_h, U, tol=1e-9):
+def kelly(cell_nodes, cell_lo, cell_h, U, tol=1e-9, deg=1):
     """Test-side Kelly indicator (deal.II's documented formula, no Neumann
     data): eta_K^2 = sum_F h_K/24 int_F sum_c [du_c/dn]^2 over interior face
-    pieces, integrated with 2x2 Gauss points on the smaller face."""
-    cells = np.asarray(cell_nodes).reshape(-1, 8)
+    pieces, integrated with (deg+1)^2 Gauss points (numpy's Gauss-Legendre)
+    on the smaller face; gradients by central differences of the Q_deg
+    interpolant (exact for polynomials of degree <= 2 per direction)."""
+    n1 = deg + 1
+    ns = n1 ** 3
+    cells = np.asarray(cell_nodes).reshape(-1, ns)
     lo = np.asarray(cell_lo).reshape(-1, 3)
     h = np.asarray(cell_h).reshape(-1, 3)
     U = np.asarray(U).reshape(-1, 18)
     n = cells.shape[0]
     e2 = np.zeros(n)
-    g = [0.5 - 0.5 / np.sqrt(3.0), 0.5 + 0.5 / np.sqrt(3.0)]
+    gx, gwt = np.polynomial.legendre.leggauss(n1)
+    g = [(0.5 * (x + 1.0), 0.5 * w) for x, w in zip(gx, gwt)]
+
+    def value(e, t):
+        out = np.zeros(18)
+        for a in range(ns):
+            idx = (a % n1, (a // n1) % n1, a // (n1 * n1))
+            out += lagrange_1d(deg, idx[0], t[0]) * lagrange_1d(deg, idx[1], t[1]) * \
+                lagrange_1d(deg, idx[2], t[2]) * U[cells[e][a]]
+        return out
 
     def grad(e, x):
         t = (x - lo[e]) / h[e]
         out = np.zeros((3, 18))
-        for a in range(8):
-            b = (a & 1, (a >> 1) & 1, a >> 2)
-            L = [t[d] if b[d] else 1 - t[d] for d in range(3)]
-            dL = [(1.0 if b[d] else -1.0) / h[e][d] for d in range(3)]
-            out[0] += dL[0] * L[1] * L[2] * U[cells[e][a]]
-            out[1] += L[0] * dL[1] * L[2] * U[cells[e][a]]
-            out[2] += L[0] * L[1] * dL[2] * U[cells[e][a]]
+        eps = 1e-3
+        for d in range(3):
+            tp, tm = t.copy(), t.copy()
+            tp[d] += eps
+            tm[d] -= eps
+            out[d] = (value(e, tp) - value(e, tm)) / (2 * eps * h[e][d])
         return out
 
     hi = lo + h
@@ -451,14 +476,14 @@ def kelly(cell_nodes, cell_lo, cell_h, U, tol=1e-9):
                     if not (small_e or (same and sg == 1)):
                         continue
                     s = 0.0
-                    for q1 in g:
-                        for q2 in g:
+                    for q1, w1 in g:
+                        for q2, w2 in g:
                             x = np.zeros(3)
                             x[d] = x_f
                             x[t1] = a1 + q1 * (b1 - a1)
                             x[t2] = a2 + q2 * (b2 - a2)
                             jmp = grad(e, x)[d] - grad(k, x)[d]
-                            s += 0.25 * (b1 - a1) * (b2 - a2) * float(jmp @ jmp)
+                            s += w1 * w2 * (b1 - a1) * (b2 - a2) * float(jmp @ jmp)
                     e2[e] += np.linalg.norm(h[e]) / 24.0 * s
                     e2[k] += np.linalg.norm(h[k]) / 24.0 * s
     return np.sqrt(e2)

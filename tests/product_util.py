@@ -68,11 +68,11 @@ def side_masks(side_id):
     return out
 
 
-def adaptive_forest(G, n, lo, hi, side_id, steps, seed=0, frac=0.25):
+def adaptive_forest(G, n, lo, hi, side_id, steps, seed=0, frac=0.25, degree=1):
     """Forest refined `steps` times, each time a seeded random fraction of its
     cells (with 2:1 balance applied by the library)."""
     rng = np.random.default_rng(seed)
-    f = G.Forest(n, lo, hi, side_id)
+    f = G.Forest(n, lo, hi, side_id, degree=degree)
     for _ in range(steps):
         fl = (rng.random(f.n_cells) < frac).astype(np.uint8)
         f = f.refine(fl)
@@ -84,8 +84,8 @@ def oracle_adaptive(mesh, lo, hi, side_id):
     hanging-node and Dirichlet constraints (deal.II order and closure)."""
     a = mesh.arrays()
     xyz = a["node_xyz"].reshape(-1, 3)
-    cn = a["cell_nodes"].reshape(-1, 8)
+    cn = a["cell_nodes"].reshape(-1, (mesh.deg + 1) ** 3)
     cell_lo = xyz[cn[:, 0]]
     faces = a["faces"].reshape(-1, 3) if mesh.n_faces else np.zeros((0, 3), np.int32)
     return O.adaptive_oracle_mesh(a["cell_nodes"], xyz, cell_lo, a["cell_h"], [tuple(f) for f in faces], lo, hi,
-                                  side_masks(side_id))
+                                  side_masks(side_id), deg=mesh.deg)

@@ -177,3 +177,109 @@ def test_periodic_box_lines_match_independent_derivation(deg, axes):
         else:
             assert all(line is None or line == {} for line in comp)
     assert c["n_hanging"] > 0
+
+
+# ------------------------------------------------------------------ Q2 forests
+def _q2(steps, seed, n=(3, 3, 2)):
+    f = adaptive_forest(G, n, LO, HI, SIDES, steps, seed, degree=2)
+    return f, f.mesh()
+
+
+def _q2_geometry(mesh):
+    a = mesh.arrays()
+    xyz = a["node_xyz"].reshape(-1, 3)
+    cn = a["cell_nodes"].reshape(-1, 27)
+    return a, xyz, cn, xyz[cn[:, 0]], a["cell_h"].reshape(-1, 3)
+
+
+def _poly(xyz, C):
+    """sum_{i,j,k <= 2} C[i,j,k,:] x^i y^j z^k, scaled to the box (a Q2 field)."""
+    x, y, z = (xyz[:, 0] / 20.0, xyz[:, 1] / 20.0, xyz[:, 2] / 6.0)
+    out = np.zeros((xyz.shape[0], C.shape[-1]))
+    for i in range(C.shape[0]):
+        for j in range(C.shape[1]):
+            for k in range(C.shape[2]):
+                out += np.outer(x ** i * y ** j * z ** k, C[i, j, k])
+    return out
+
+
+@pytest.mark.parametrize("steps,seed", [(1, 0), (2, 1), (2, 5)])
+def test_q2_hanging_lines_match_independent_derivation(steps, seed):
+    f, m = _q2(steps, seed)
+    assert m.deg == 2 and m.ns == 27
+    a, xyz, cn, lo, h = _q2_geometry(m)
+    ref = O.hanging_lines(a["cell_nodes"], xyz, lo, h, deg=2)
+    c = m.constraints()
+    hp = c["hang_ptr"]
+    got = {v: {int(c["hang_m"][k]): c["hang_w"][k] for k in range(hp[v], hp[v + 1])}
+           for v in range(m.n_nodes) if hp[v + 1] > hp[v]}
+    assert len(got) > 0 and set(got) == set(ref)
+    for v in got:
+        assert set(got[v]) == set(ref[v]), v
+        for mst, w in got[v].items():
+            assert w == pytest.approx(ref[v][mst], abs=1e-14)
+    # Q2 weights at a quarter of a coarse edge: 3/8, 3/4, -1/8 appear
+    ws = np.round(np.concatenate([list(l.values()) for l in got.values()]), 12)
+    assert any(np.isclose(ws, -1.0 / 8.0)) and any(np.isclose(ws, 3.0 / 8.0))
+
+
+@pytest.mark.parametrize("steps,seed", [(2, 1), (3, 2)])
+def test_q2_constraints_reproduce_triquadratics_and_only_those(steps, seed):
+    f, m = _q2(steps, seed)
+    a, xyz, cn, lo, h = _q2_geometry(m)
+    rng = np.random.default_rng(seed)
+    C = rng.standard_normal((3, 3, 3, 18))
+    P = _poly(xyz, C)
+    U = _continuous(m, P.reshape(-1)).reshape(-1, 18)
+    hang = m.constraints()["hang_ptr"]
+    hv = np.nonzero(np.diff(hang) > 0)[0]
+    assert hv.size > 0
+    # a field of degree <= 2 in each direction is its own Q2 interpolant on
+    # every cell, so the hanging values must equal it
+    assert np.abs(U[hv] - P[hv]).max() <= 1e-12 * np.abs(P).max()
+    # a cubic is not: the coarse face interpolant differs from it
+    cub = np.outer((xyz[:, 0] / 20.0) ** 3 + (xyz[:, 1] / 20.0) ** 3, np.ones(18))
+    Uc = _continuous(m, cub.reshape(-1)).reshape(-1, 18)
+    assert np.abs(Uc[hv] - cub[hv]).max() > 1e-6
+
+
+def test_q2_transfer_reproduces_triquadratics_and_interpolates():
+    f, m = _q2(1, 4)
+    a, xyz, cn, lo, h = _q2_geometry(m)
+    rng = np.random.default_rng(0)
+    C = rng.standard_normal((3, 3, 3, 18))
+    U = _continuous(m, _poly(xyz, C).reshape(-1))
+    f2 = f.refine((rng.random(f.n_cells) < 0.4).astype(np.uint8))
+    m2 = f2.mesh()
+    assert m2.deg == 2
+    xyz2 = m2.arrays()["node_xyz"].reshape(-1, 3)
+    U2 = f.transfer(m, U, f2, m2)
+    assert np.abs(U2 - _poly(xyz2, C).reshape(-1)).max() <= 1e-12 * np.abs(U).max()
+    # a general conforming field: new nodes take the Q2 interpolant of the
+    # old cell that contains them
+    U = _continuous(m, rng.standard_normal(m.n_nodes * 18)).reshape(-1, 18)
+    U2 = f.transfer(m, U.reshape(-1), f2, m2).reshape(-1, 18)
+    for v in rng.choice(m2.n_nodes, 40, replace=False):
+        t = (xyz2[v] - lo) / h
+        e = int(np.nonzero(np.all((t > -1e-12) & (t < 1 + 1e-12), 1))[0][0])
+        tv = np.clip(t[e], 0, 1)
+        val = np.zeros(18)
+        for k in range(27):
+            idx = (k % 3, (k // 3) % 3, k // 9)
+            w = np.prod([O.lagrange_1d(2, idx[d], tv[d]) for d in range(3)])
+            val += w * U[cn[e][k]]
+        assert np.abs(U2[v] - val).max() < 1e-12 * max(1.0, np.abs(U).max())
+
+
+def test_q2_kelly_matches_independent_restatement():
+    f, m = _q2(2, 5)
+    a, xyz, cn, lo, h = _q2_geometry(m)
+    rng = np.random.default_rng(1)
+    U = _continuous(m, rng.standard_normal(m.n_nodes * 18))
+    eta = f.kelly(m, U)
+    ref = O.kelly(a["cell_nodes"], lo, h, U, deg=2)
+    assert eta.dtype == np.float32
+    assert np.abs(eta - ref).max() <= 1e-6 * ref.max()
+    # a global field of degree <= 2 per direction has no normal-derivative jumps
+    Up = _continuous(m, _poly(xyz, rng.standard_normal((3, 3, 3, 18))).reshape(-1))
+    assert np.abs(f.kelly(m, Up)).max() < 1e-8 * max(1.0, ref.max())

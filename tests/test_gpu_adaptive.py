@@ -29,8 +29,8 @@ def ctx():
     c.close()
 
 
-def _case(ctx, steps, seed, p=22.0, t=0.8, amp=0.05, sides=SIDES, n=(3, 3, 2), lo=LO, hi=HI):
-    f = adaptive_forest(G, n, lo, hi, sides, steps, seed)
+def _case(ctx, steps, seed, p=22.0, t=0.8, amp=0.05, sides=SIDES, n=(3, 3, 2), lo=LO, hi=HI, degree=1):
+    f = adaptive_forest(G, n, lo, hi, sides, steps, seed, degree=degree)
     m = f.mesh()
     say("adaptive case: steps %d seed %d cells %d nodes %d hanging %d" % (steps, seed, m.n_cells, m.n_nodes,
                                                                          m.constraints()["n_hanging"]))
@@ -332,3 +332,88 @@ def test_device_transfer_equals_host_transfer(ctx):
         if step == 0:
             assert m2.constraints()["n_hanging"] > 0
         f, m = f2, m2
+
+
+def _blocks(row_ptr, col, val, n_nodes):
+    v = val.reshape(-1, 18, 18)
+    return {(r, int(col[b])): v[b] for r in range(n_nodes) for b in range(row_ptr[r], row_ptr[r + 1])}
+
+
+@pytest.mark.parametrize("steps,seed,n", [(1, 0, (2, 2, 2)), (2, 1, (2, 2, 1))])
+def test_q2_condensed_jacobian_and_rhs_match_oracle(ctx, steps, seed, n):
+    f, m, om, Ud, co = _case(ctx, steps, seed, n=n, degree=2)
+    assert m.deg == 2 and m.constraints()["n_hanging"] > 0
+    assert np.abs(ctx.solution() - Ud).max() <= 1e-15 * np.abs(Ud).max()  # distribute
+    ctx.assemble()
+    Jo, ro = O.assemble(om, co, Ud)
+    rg = ctx.rhs()
+    assert np.abs(rg - ro).max() <= 1e-12 * np.abs(ro).max()
+    a = m.arrays()
+    bg = _blocks(a["row_ptr"], a["col"], ctx.jacobian(), m.n_nodes)
+    bo = _blocks(om.row_ptr, om.col, Jo, om.n_nodes)
+    scale = max(np.abs(b).max() for b in bo.values())
+    err = 0.0
+    for k in set(bg) | set(bo):
+        z = np.zeros((18, 18))
+        err = max(err, np.abs(bg.get(k, z) - bo.get(k, z)).max() / scale)
+    say("Q2 J rel err", err)
+    assert err <= 1e-12
+
+
+def test_q2_newton_on_adaptive_mesh_matches_oracle(ctx):
+    f, m, om, Ud, co = _case(ctx, 1, 3, n=(2, 2, 2), degree=2)
+    st = ctx.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-3, restart=50)
+    say("gpu newton Q2", st)
+    assert st["converged"] == 1.0
+    Ug = ctx.solution()
+    Uo, rc, ost = O.newton(om, co, Ud, rtol=1e-10, max_newton=60, lin_tol=1e-3, restart=50)
+    say("oracle newton Q2", ost)
+    assert rc == 0
+    assert st["F"] == pytest.approx(O.free_energy(om, co, Uo), rel=1e-10)
+    assert np.abs(Ug - Uo).max() <= 1e-8 * np.abs(Uo).max()
+    assert np.abs(O.distribute(om, Ug) - Ug).max() <= 1e-15 * np.abs(Ug).max()
+
+
+def test_q2_device_transfer_equals_host_transfer(ctx):
+    lo, hi = (-5.0, -5.0, -6.0), (5.0, 5.0, 6.0)
+    f = G.Forest((2, 2, 2), lo, hi, SIDES, degree=2)
+    m = f.mesh()
+    rng = np.random.default_rng(11)
+    xyz = m.arrays()["node_xyz"].reshape(-1, 3)
+    U = (np.sin(xyz @ rng.standard_normal((3, 18))) + 0.1 * rng.standard_normal((m.n_nodes, 18))).reshape(-1)
+    ctx.set_material_pt(22.0, 0.8)
+    ctx.upload_mesh(m)
+    ctx.set_solution(U)
+    for step in range(2):
+        Uc = ctx.solution()
+        f2 = f.refine((rng.random(f.n_cells) < 0.35).astype(np.uint8))
+        m2 = f2.mesh()
+        say("Q2 transfer step", step, "cells", f.n_cells, "->", f2.n_cells, "nodes", m2.n_nodes)
+        Uh = f.transfer(m, Uc, f2, m2)
+        ctx.transfer_solution(f, m, f2, m2)
+        Ud = ctx.solution()
+        ctx.set_solution(Uh)
+        assert np.array_equal(Ud, ctx.solution())
+        f, m = f2, m2
+
+
+def test_q2_adaptive_cycles_fp32_against_fp64(ctx):
+    lo, hi = (-5.0, -5.0, -6.0), (5.0, 5.0, 6.0)
+    p, t = 22.0, 0.8
+    f = G.Forest((2, 2, 2), lo, hi, SIDES, degree=2)
+    m = f.mesh()
+    ctx.set_material_pt(p, t)
+    U0 = O.initial_guess(m.n_nodes, O.matep(p, t)["gap_B"], amp=0.05, seed=2)
+    rows, f2, m2 = ctx.run_adaptive(f, U0, n_refine=2, refine_frac=0.3, rtol=1e-9, lin_tol=1e-2, restart=50,
+                                    fp32=True, check_fp64=True)
+    for r in rows:
+        say(r)
+        assert r["converged"] == 1.0 and r["converged64"] == 1.0
+        assert r["F_rel_err_vs_fp64"] <= 1e-10
+    assert m2.deg == 2 and any(r["hanging"] > 0 for r in rows)
+    assert rows[2]["cells"] > rows[1]["cells"] > rows[0]["cells"]
+    om = oracle_adaptive(m2, lo, hi, SIDES)
+    co = O.coef(p, t)
+    U = ctx.solution()
+    _, ro = O.assemble(om, co, U, J=False)
+    assert np.linalg.norm(ro) <= 1.01e-9 * rows[2]["r0"]

@@ -23,8 +23,10 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include"),
           "-I", SRC]
+# fused multiply-adds everywhere except where a kernel asks otherwise with
+# "#pragma clang fp contract(off)" (plain -ffp-contract=fast ignores it)
 HIPFLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-ffp-contract=fast"]
+            "-ffp-contract=fast-honor-pragmas"]
 
 SOURCES = ["gl_assemble.hip", "gl_linalg.hip", "gl_constraints.hip", "mlp.hip", "mlp.cpp", "glhip_api.cpp", "mesh.cpp", "amr.cpp",
            "matep.cpp", "setup.cpp", "kelly.cpp", "driver.cpp", "prm.cpp", "io.cpp"]

@@ -115,31 +115,44 @@ int forest_refine(Forest &f, const std::vector<char> &flag, std::string &err)
   return 0;
 }
 
-int forest_mesh(const Forest &f, Mesh &m, std::string &err)
+// Lagrange basis of degree deg (1 or 2) on the equidistant nodes of [0,1]
+double lagrange(int deg, int i, double t)
 {
+  if (deg == 1) return i ? t : 1.0 - t;
+  if (i == 0) return (2.0 * (t - 0.5)) * (t - 1.0);
+  if (i == 1) return (-4.0 * t) * (t - 1.0);
+  return (2.0 * t) * (t - 0.5);
+}
+
+int forest_mesh(const Forest &f, Mesh &m, std::string &err, int deg)
+{
+  if (deg != 1 && deg != 2) { err = "forest meshes are Q1 or Q2"; return -1; }
   m = Mesh();
-  m.deg = 1; m.n1 = 2; m.ns = 8;
+  const int n1 = deg + 1, ns = n1 * n1 * n1;
+  m.deg = deg; m.n1 = n1; m.ns = ns;
   const int LM = Forest::LMAX;
-  const int64_t NX = (int64_t)f.nx << LM, NY = (int64_t)f.ny << LM, NZ = (int64_t)f.nz << LM;
+  // node coordinates are integers in units of 1/deg of the finest lattice:
+  // a leaf of level l spans deg * 2^(LM-l) units, its nodes sit 2^(LM-l) apart
+  const int64_t NX = ((int64_t)f.nx << LM) * deg, NY = ((int64_t)f.ny << LM) * deg, NZ = ((int64_t)f.nz << LM) * deg;
   const double dl[3] = {(f.hi[0] - f.lo[0]) / (double)NX, (f.hi[1] - f.lo[1]) / (double)NY,
                         (f.hi[2] - f.lo[2]) / (double)NZ};
   const int nc = (int)f.leaves.size();
-  if ((int64_t)nc * 8 >= (1LL << 31)) { err = "mesh too large for 32-bit indices"; return -1; }
+  if ((int64_t)nc * ns >= (1LL << 31)) { err = "mesh too large for 32-bit indices"; return -1; }
   m.n_cells = nc;
-  // vertices, numbered in the (Morton) order of first appearance
+  // nodes, numbered in the (Morton) order of first appearance
   std::unordered_map<uint64_t, int> vid;
-  vid.reserve((size_t)nc * 2);
+  vid.reserve((size_t)nc * (deg == 1 ? 2 : 9));
   std::vector<int64_t> vx;
-  m.cell_nodes.resize((size_t)nc * 8);
+  m.cell_nodes.resize((size_t)nc * ns);
   m.cell_h.resize((size_t)nc * 3);
   m.cell_lo.resize((size_t)nc * 3);
   m.cell_level.resize(nc);
   for (int e = 0; e < nc; ++e) {
     const auto &L = f.leaves[e];
     const int64_t s = (int64_t)1 << (LM - L.level);
-    const int64_t x0 = L.ix * s, y0 = L.iy * s, z0 = L.iz * s;
-    for (int a = 0; a < 8; ++a) {
-      const int64_t x = x0 + (a & 1) * s, y = y0 + ((a >> 1) & 1) * s, z = z0 + (a >> 2) * s;
+    const int64_t x0 = L.ix * s * deg, y0 = L.iy * s * deg, z0 = L.iz * s * deg;
+    for (int a = 0; a < ns; ++a) {
+      const int64_t x = x0 + (a % n1) * s, y = y0 + ((a / n1) % n1) * s, z = z0 + (a / (n1 * n1)) * s;
       auto it = vid.find(vkey(x, y, z));
       int v;
       if (it == vid.end()) {
@@ -149,9 +162,9 @@ int forest_mesh(const Forest &f, Mesh &m, std::string &err)
       } else {
         v = it->second;
       }
-      m.cell_nodes[(size_t)e * 8 + a] = v;
+      m.cell_nodes[(size_t)e * ns + a] = v;
     }
-    for (int d = 0; d < 3; ++d) m.cell_h[3 * e + d] = (double)s * dl[d];
+    for (int d = 0; d < 3; ++d) m.cell_h[3 * e + d] = (double)(s * deg) * dl[d];
     m.cell_lo[3 * e + 0] = f.lo[0] + (double)x0 * dl[0];
     m.cell_lo[3 * e + 1] = f.lo[1] + (double)y0 * dl[1];
     m.cell_lo[3 * e + 2] = f.lo[2] + (double)z0 * dl[2];
@@ -163,41 +176,63 @@ int forest_mesh(const Forest &f, Mesh &m, std::string &err)
   m.node_xyz.resize((size_t)nn * 3);
   for (int v = 0; v < nn; ++v)
     for (int d = 0; d < 3; ++d) m.node_xyz[3 * v + d] = f.lo[d] + (double)vx[3 * v + d] * dl[d];
-  // hanging vertices: a vertex that lies inside an edge or a face of a leaf
-  // touching it follows the trilinear interpolant of that leaf there
-  // (weights 1/2 on an edge, 1/4 on a face under 2:1 balance)
+  // hanging nodes: a node that lies on a face or an edge of a leaf touching
+  // it without being one of that leaf's nodes follows the leaf's field there:
+  // the tensor product of the degree-deg Lagrange values at its position
+  // along the face or edge (Q1: 1/2 on an edge, 1/4 on a face; Q2: 3/8, 3/4,
+  // -1/8 at a quarter of the edge, under 2:1 balance)
   std::vector<std::vector<std::pair<int, double>>> line(nn);
   std::vector<char> hanging(nn, 0);
+  const int64_t NL[3] = {(int64_t)f.nx << LM, (int64_t)f.ny << LM, (int64_t)f.nz << LM};
   for (int v = 0; v < nn; ++v) {
-    const int64_t X = vx[3 * v], Y = vx[3 * v + 1], Z = vx[3 * v + 2];
+    const int64_t P[3] = {vx[3 * v], vx[3 * v + 1], vx[3 * v + 2]};
+    // finest-lattice cells whose closure holds the node: floor((P-1)/deg)
+    // and floor(P/deg) in every direction
+    int64_t cand[3][2];
+    for (int d = 0; d < 3; ++d) {
+      cand[d][0] = P[d] > 0 ? (P[d] - 1) / deg : -1;
+      cand[d][1] = P[d] / deg;
+    }
     for (int o = 0; o < 8 && !hanging[v]; ++o) {
-      const int64_t cx = X - (o & 1), cy = Y - ((o >> 1) & 1), cz = Z - (o >> 2);
-      if (cx < 0 || cy < 0 || cz < 0 || cx >= NX || cy >= NY || cz >= NZ) continue;
+      const int64_t cx = cand[0][o & 1], cy = cand[1][(o >> 1) & 1], cz = cand[2][o >> 2];
+      if (cx < 0 || cy < 0 || cz < 0 || cx >= NL[0] || cy >= NL[1] || cz >= NL[2]) continue;
       const int K = f.locate(LM, (int)cx, (int)cy, (int)cz);
       if (K < 0) { err = "forest lookup failed"; return -1; }
       const auto &L = f.leaves[K];
-      const int64_t s = (int64_t)1 << (LM - L.level);
-      const int64_t r[3] = {X - L.ix * s, Y - L.iy * s, Z - L.iz * s};
-      int inside = 0;
-      for (int d = 0; d < 3; ++d)
-        if (r[d] > 0 && r[d] < s) {
-          if (2 * r[d] != s) { err = "unbalanced forest (hanging vertex off the midpoint)"; return -1; }
-          ++inside;
-        }
-      if (inside == 0) continue;
-      if (inside == 3) { err = "vertex inside a leaf"; return -1; }
+      const int64_t s = (int64_t)1 << (LM - L.level), ext = s * deg;
+      const int64_t O[3] = {L.ix * ext, L.iy * ext, L.iz * ext};
+      const int64_t r[3] = {P[0] - O[0], P[1] - O[1], P[2] - O[2]};
+      bool node = true, bnd = false;
+      for (int d = 0; d < 3; ++d) {
+        if (r[d] < 0 || r[d] > ext) { err = "forest lookup off the node"; return -1; }
+        if (r[d] % s) node = false;
+        if (r[d] == 0 || r[d] == ext) bnd = true;
+      }
+      if (node) continue;
+      if (!bnd) { err = "node inside a leaf"; return -1; }
       hanging[v] = 1;
-      const double w = inside == 1 ? 0.5 : 0.25;
-      const int nm = 1 << inside;
-      for (int t = 0; t < nm; ++t) {
-        int64_t q[3];
-        int bit = 0;
-        for (int d = 0; d < 3; ++d) {
-          if (r[d] > 0 && r[d] < s) q[d] = ((t >> bit++) & 1) ? s : 0;
-          else q[d] = r[d];
+      // free directions: along the face or edge (0 < r < ext)
+      int fd[3], nf = 0;
+      for (int d = 0; d < 3; ++d)
+        if (r[d] > 0 && r[d] < ext) {
+          if ((2 * r[d]) % s) { err = "unbalanced forest (hanging node off the half lattice)"; return -1; }
+          fd[nf++] = d;
         }
-        auto it = vid.find(vkey(L.ix * s + q[0], L.iy * s + q[1], L.iz * s + q[2]));
-        if (it == vid.end()) { err = "master vertex missing"; return -1; }
+      int cnt = 1;
+      for (int k = 0; k < nf; ++k) cnt *= n1;
+      for (int t = 0; t < cnt; ++t) {
+        int64_t q[3] = {r[0], r[1], r[2]};
+        double w = 1.0;
+        int tt = t;
+        for (int k = 0; k < nf; ++k) {
+          const int i = tt % n1, d = fd[k];
+          tt /= n1;
+          q[d] = i * s;
+          w *= lagrange(deg, i, (double)r[d] / (double)ext);
+        }
+        if (w == 0.0) continue;
+        auto it = vid.find(vkey(O[0] + q[0], O[1] + q[1], O[2] + q[2]));
+        if (it == vid.end()) { err = "master node missing"; return -1; }
         line[v].push_back({it->second, w});
       }
     }
@@ -231,7 +266,7 @@ int forest_mesh(const Forest &f, Mesh &m, std::string &err)
         m.hang_w.push_back(mw.second);
       }
   }
-  // boundary faces and Dirichlet masks of the non-hanging vertices
+  // boundary faces and Dirichlet masks of the non-hanging nodes
   m.node_mask.assign(nn, 0u);
   for (int v = 0; v < nn; ++v) {
     if (hanging[v]) { m.node_mask[v] = HANG_BIT; continue; }
@@ -245,8 +280,8 @@ int forest_mesh(const Forest &f, Mesh &m, std::string &err)
   for (int e = 0; e < nc; ++e) {
     const auto &L = f.leaves[e];
     const int64_t s = (int64_t)1 << (LM - L.level);
-    const bool on[6] = {L.ix == 0, (L.ix + 1) * s == NX, L.iy == 0, (L.iy + 1) * s == NY,
-                        L.iz == 0, (L.iz + 1) * s == NZ};
+    const bool on[6] = {L.ix == 0, (L.ix + 1) * s == NL[0], L.iy == 0, (L.iy + 1) * s == NL[1],
+                        L.iz == 0, (L.iz + 1) * s == NL[2]};
     for (int lf = 0; lf < 6; ++lf)
       if (on[lf] && f.side_id[lf] != 1) {
         m.faces.push_back(e);
@@ -274,25 +309,29 @@ int forest_transfer_map(const Forest &fo, const Mesh &mo, const Forest &fn, cons
     err = "mesh does not belong to the forest";
     return -1;
   }
-  const int64_t NX = (int64_t)fn.nx << LM, NY = (int64_t)fn.ny << LM, NZ = (int64_t)fn.nz << LM;
-  const double dl[3] = {(fn.hi[0] - fn.lo[0]) / (double)NX, (fn.hi[1] - fn.lo[1]) / (double)NY,
-                        (fn.hi[2] - fn.lo[2]) / (double)NZ};
+  const int deg = mn.deg;
+  const int64_t NL[3] = {(int64_t)fn.nx << LM, (int64_t)fn.ny << LM, (int64_t)fn.nz << LM};
+  // node coordinates in units of 1/deg of the finest lattice
+  const double dl[3] = {(fn.hi[0] - fn.lo[0]) / (double)(NL[0] * deg), (fn.hi[1] - fn.lo[1]) / (double)(NL[1] * deg),
+                        (fn.hi[2] - fn.lo[2]) / (double)(NL[2] * deg)};
   cell.resize(mn.n_nodes);
   t.resize((size_t)mn.n_nodes * 3);
   for (int v = 0; v < mn.n_nodes; ++v) {
-    // lattice coordinates of the vertex
-    int64_t P[3];
-    for (int d = 0; d < 3; ++d) P[d] = (int64_t)std::llround((mn.node_xyz[3 * v + d] - fn.lo[d]) / dl[d]);
-    const int64_t cx = std::min(P[0], NX - 1), cy = std::min(P[1], NY - 1), cz = std::min(P[2], NZ - 1);
-    const int K = fo.locate(LM, (int)cx, (int)cy, (int)cz);
-    if (K < 0) { err = "vertex outside the old forest"; return -1; }
+    int64_t P[3], C[3];
+    for (int d = 0; d < 3; ++d) {
+      P[d] = (int64_t)std::llround((mn.node_xyz[3 * v + d] - fn.lo[d]) / dl[d]);
+      C[d] = std::min(P[d] / deg, NL[d] - 1);
+    }
+    const int K = fo.locate(LM, (int)C[0], (int)C[1], (int)C[2]);
+    if (K < 0) { err = "node outside the old forest"; return -1; }
     const auto &L = fo.leaves[K];
-    const int64_t s = (int64_t)1 << (LM - L.level);
-    // k / 2^(LM - level) with k <= 2^LM: exact in float
+    const int64_t ext = ((int64_t)1 << (LM - L.level)) * deg;
+    // k / (deg 2^(LM - level)) with k <= deg 2^LM: dyadic (Q1) or a quarter
+    // step (Q2) of at most 14 bits, exact in float
     cell[v] = K;
-    t[3 * (size_t)v] = (float)((double)(P[0] - L.ix * s) / (double)s);
-    t[3 * (size_t)v + 1] = (float)((double)(P[1] - L.iy * s) / (double)s);
-    t[3 * (size_t)v + 2] = (float)((double)(P[2] - L.iz * s) / (double)s);
+    t[3 * (size_t)v] = (float)((double)(P[0] - L.ix * ext) / (double)ext);
+    t[3 * (size_t)v + 1] = (float)((double)(P[1] - L.iy * ext) / (double)ext);
+    t[3 * (size_t)v + 2] = (float)((double)(P[2] - L.iz * ext) / (double)ext);
   }
   return 0;
 }
@@ -300,17 +339,19 @@ int forest_transfer_map(const Forest &fo, const Mesh &mo, const Forest &fn, cons
 int forest_transfer(const Forest &fo, const Mesh &mo, const double *Uo, const Forest &fn, const Mesh &mn,
                     double *Un, std::string &err)
 {
+  if (mo.deg != mn.deg) { err = "old and new mesh of different degree"; return -1; }
   std::vector<int> cell;
   std::vector<float> tt;
   if (forest_transfer_map(fo, mo, fn, mn, cell, tt, err)) return -1;
+  const int deg = mo.deg, n1 = deg + 1, ns = mo.ns;
   for (int v = 0; v < mn.n_nodes; ++v) {
     const double t[3] = {tt[3 * (size_t)v], tt[3 * (size_t)v + 1], tt[3 * (size_t)v + 2]};
-    const int *cn = &mo.cell_nodes[(size_t)cell[v] * 8];
+    const int *cn = &mo.cell_nodes[(size_t)cell[v] * ns];
     for (int c = 0; c < NC; ++c) {
       double u = 0.0;
-      for (int a = 0; a < 8; ++a) {
-        const double w = ((a & 1) ? t[0] : 1 - t[0]) * (((a >> 1) & 1) ? t[1] : 1 - t[1]) *
-                         ((a >> 2) ? t[2] : 1 - t[2]);
+      for (int a = 0; a < ns; ++a) {
+        const double w = lagrange(deg, a % n1, t[0]) * lagrange(deg, (a / n1) % n1, t[1]) *
+                         lagrange(deg, a / (n1 * n1), t[2]);
         if (w != 0.0) u += w * Uo[(size_t)cn[a] * NC + c];
       }
       Un[(size_t)v * NC + c] = u;

@@ -48,11 +48,14 @@ int forest_init(Forest &f, int nx, int ny, int nz, const double lo[3], const dou
 // neighbouring leaves (across faces, edges and vertices) differ by at most
 // one level.
 int forest_refine(Forest &f, const std::vector<char> &flag, std::string &err);
-// Q1 mesh of the leaves (cell e = leaf e).  Hanging vertices get closed
+// Q1 or Q2 (deg) mesh of the leaves (cell e = leaf e).  Hanging nodes get closed
 // constraint lines in m.hang_*, the flag HANG_BIT in node_mask and no
 // Dirichlet bits; m.row_ptr/col is the condensed system pattern and
 // m.arow_ptr/acol the pattern the cell loop assembles into.
-int forest_mesh(const Forest &f, Mesh &m, std::string &err);
+int forest_mesh(const Forest &f, Mesh &m, std::string &err, int deg = 1);
+// Lagrange basis function i of degree deg (1 or 2) on the equidistant nodes
+// of [0,1], evaluated at t
+double lagrange(int deg, int i, double t);
 // The interpolation of forest_transfer: for each node of mn, the fo cell
 // that contains it and the node's coordinates t[3] in [0,1]^3 within it
 // (dyadic fractions of level <= LMAX, exact in float).

@@ -1,5 +1,5 @@
 // The adaptive cycle loop of FemGL::run (femgl/src/run.cc:109-260) over the
-// C ABI: for every cycle, the Q1 mesh of the current forest is uploaded, the
+// C ABI: for every cycle, the mesh of the current forest is uploaded, the
 // Newton solve runs on the GPU, then the Kelly indicators of the converged
 // field flag the fixed fraction of cells (refine.cc:146-157), the forest is
 // refined with 2:1 balance and the field is interpolated onto the new mesh

@@ -92,37 +92,51 @@ __global__ __launch_bounds__(CT) void k_distribute(int n_hang, const int *__rest
   v[(size_t)hnode[j] * 18 + c] = acc;
 }
 
-// Solution transfer to a refined mesh (SolutionTransfer of Q1, femgl/src/
-// refine.cc:163-175): value (v, c) is the trilinear interpolant of the old
-// cell cell[v] at t[v]; rounded products and sums in the order of the host
-// transfer, so both give the same bits.
-__global__ __launch_bounds__(CT) void k_transfer(int n_nodes, const int *__restrict__ cell,
+// Solution transfer to a refined mesh (SolutionTransfer of FE_Q(deg),
+// femgl/src/refine.cc:163-175): value (v, c) is the Q_deg interpolant of the
+// old cell cell[v] at t[v], with every product and sum rounded on its own in
+// the order of the host transfer, so both give the same bits.  Contraction
+// into fused multiply-adds is switched off here: it would round w * U and the
+// sum once (the build compiles with -ffp-contract=fast).
+__device__ inline double lagrange_rn(int deg, int i, double t)
+{
+#pragma clang fp contract(off)
+  if (deg == 1) return i ? t : 1.0 - t;
+  if (i == 0) return (2.0 * (t - 0.5)) * (t - 1.0);
+  if (i == 1) return (-4.0 * t) * (t - 1.0);
+  return (2.0 * t) * (t - 0.5);
+}
+
+__global__ __launch_bounds__(CT) void k_transfer(int n_nodes, int deg, const int *__restrict__ cell,
                                                  const float *__restrict__ t, const int *__restrict__ old_cn,
                                                  const double *__restrict__ Uo, double *__restrict__ Un)
 {
+#pragma clang fp contract(off)
   const long long i = (long long)blockIdx.x * CT + threadIdx.x;
   if (i >= (long long)n_nodes * 18) return;
   const int v = (int)(i / 18), c = (int)(i % 18);
+  const int n1 = deg + 1, ns = n1 * n1 * n1;
   const double t0 = t[3 * (size_t)v], t1 = t[3 * (size_t)v + 1], t2 = t[3 * (size_t)v + 2];
-  const int *cn = old_cn + (size_t)cell[v] * 8;
+  const int *cn = old_cn + (size_t)cell[v] * ns;
   double u = 0.0;
-  for (int a = 0; a < 8; ++a) {
-    const double w = __dmul_rn(__dmul_rn((a & 1) ? t0 : 1.0 - t0, ((a >> 1) & 1) ? t1 : 1.0 - t1),
-                               (a >> 2) ? t2 : 1.0 - t2);
-    if (w != 0.0) u = __dadd_rn(u, __dmul_rn(w, Uo[(size_t)cn[a] * 18 + c]));
+  for (int a = 0; a < ns; ++a) {
+    const double w = (lagrange_rn(deg, a % n1, t0) * lagrange_rn(deg, (a / n1) % n1, t1)) *
+                     lagrange_rn(deg, a / (n1 * n1), t2);
+    if (w != 0.0) u = u + w * Uo[(size_t)cn[a] * 18 + c];
   }
   Un[i] = u;
 }
 
 }  // namespace
 
-hipError_t launch_transfer(int n_nodes, const int *cell, const float *t, const int *old_cn, const double *Uo,
-                           double *Un, hipStream_t s)
+hipError_t launch_transfer(int n_nodes, int deg, const int *cell, const float *t, const int *old_cn,
+                           const double *Uo, double *Un, hipStream_t s)
 {
   if (n_nodes <= 0) return hipSuccess;
+  if (deg != 1 && deg != 2) return hipErrorInvalidValue;
   const long long n = (long long)n_nodes * 18;
-  hipLaunchKernelGGL(k_transfer, dim3((unsigned)((n + CT - 1) / CT)), dim3(CT), 0, s, n_nodes, cell, t, old_cn, Uo,
-                     Un);
+  hipLaunchKernelGGL(k_transfer, dim3((unsigned)((n + CT - 1) / CT)), dim3(CT), 0, s, n_nodes, deg, cell, t, old_cn,
+                     Uo, Un);
   return hipGetLastError();
 }
 

@@ -1084,6 +1084,13 @@ int glh_forest_destroy(glh_forest *f)
 
 int64_t glh_forest_n_cells(const glh_forest *f) { return f ? (int64_t)f->f.leaves.size() : -1; }
 
+int glh_forest_set_degree(glh_forest *f, int deg)
+{
+  if (!f || (deg != 1 && deg != 2)) return GLH_ERR_ARG;
+  f->deg = deg;
+  return GLH_OK;
+}
+
 int glh_forest_refine(const glh_forest *f, const uint8_t *flags, glh_forest **out)
 {
   if (!f || !flags || !out) return GLH_ERR_ARG;
@@ -1105,7 +1112,7 @@ int glh_forest_mesh(const glh_forest *f, glh_mesh **out)
   *out = nullptr;
   auto *mm = new glh_mesh();
   std::string err;
-  if (glh::forest_mesh(f->f, mm->m, err)) {
+  if (glh::forest_mesh(f->f, mm->m, err, f->deg)) {
     delete mm;
     return GLH_ERR_ARG;
   }
@@ -1126,8 +1133,9 @@ int glh_transfer_solution(glh_ctx *c, const glh_forest *fo, const glh_mesh *mo, 
 {
   RCHK(check_ready(c, false));
   if (!fo || !mo || !fn || !mn) return fail(c, GLH_ERR_ARG, "null forest or mesh");
-  if (mo->m.deg != 1 || c->n_nodes != mo->m.n_nodes || c->n_cells != mo->m.n_cells || c->n_seg != 1)
+  if (c->n_nodes != mo->m.n_nodes || c->n_cells != mo->m.n_cells || c->n_seg != 1 || mo->m.deg != c->deg)
     return fail(c, GLH_ERR_STATE, "the context does not hold the old mesh");
+  if (mn->m.deg != mo->m.deg) return fail(c, GLH_ERR_ARG, "old and new mesh of different degree");
   std::string err;
   std::vector<int> cell;
   std::vector<float> t;
@@ -1143,7 +1151,7 @@ int glh_transfer_solution(glh_ctx *c, const glh_forest *fo, const glh_mesh *mo, 
     dcell.release();
     dt.release();
   };
-  const size_t no = (size_t)mo->m.n_nodes * NC, nco = (size_t)mo->m.n_cells * 8;
+  const size_t no = (size_t)mo->m.n_nodes * NC, nco = (size_t)mo->m.n_cells * mo->m.ns;
   hipError_t e = Uo.alloc(no);
   if (e == hipSuccess) e = cno.alloc(nco);
   if (e == hipSuccess) e = hipMemcpyAsync(Uo.p, c->U.p, no * sizeof(double), hipMemcpyDeviceToDevice, s);
@@ -1160,7 +1168,7 @@ int glh_transfer_solution(glh_ctx *c, const glh_forest *fo, const glh_mesh *mo, 
   }
   e = upload(dcell, cell.data(), cell.size(), s);
   if (e == hipSuccess) e = upload(dt, t.data(), t.size(), s);
-  if (e == hipSuccess) e = glh::launch_transfer(mn->m.n_nodes, dcell.p, dt.p, cno.p, Uo.p, c->U.p, s);
+  if (e == hipSuccess) e = glh::launch_transfer(mn->m.n_nodes, mn->m.deg, dcell.p, dt.p, cno.p, Uo.p, c->U.p, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   release();
   if (e != hipSuccess) return fail(c, GLH_ERR_HIP, std::string("transfer: ") + hipGetErrorString(e));

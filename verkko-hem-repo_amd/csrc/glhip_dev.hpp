@@ -62,10 +62,10 @@ hipError_t launch_condense(const ConstraintDev &C, const TJ *Jf, TJ *Jc, const d
 hipError_t launch_condense_rhs(const ConstraintDev &C, const double *rf, double *rc, hipStream_t s);
 // hanging values of v from their masters
 hipError_t launch_distribute(const ConstraintDev &C, double *v, hipStream_t s);
-// Un[v*18+c] = trilinear interpolant of the old cell cell[v] (corner nodes
-// old_cn[cell*8..], lexicographic) at t[3v..3v+2], from Uo
-hipError_t launch_transfer(int n_nodes, const int *cell, const float *t, const int *old_cn, const double *Uo,
-                           double *Un, hipStream_t s);
+// Un[v*18+c] = Q_deg interpolant of the old cell cell[v] (nodes
+// old_cn[cell*(deg+1)^3..], lexicographic) at t[3v..3v+2], from Uo
+hipError_t launch_transfer(int n_nodes, int deg, const int *cell, const float *t, const int *old_cn,
+                           const double *Uo, double *Un, hipStream_t s);
 
 // ---------------------------------------------------------------- assembly
 // One launch per colour; cells[] lists the cells of that colour.

@@ -10,4 +10,5 @@ struct glh_mesh {
 
 struct glh_forest {
   glh::Forest f;
+  int deg = 1;  // element degree of the meshes built from it (glh_forest_set_degree)
 };

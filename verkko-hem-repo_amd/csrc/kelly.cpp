@@ -1,5 +1,5 @@
-// Kelly error estimator and fixed-number refinement flags for the Q1 forest
-// meshes of amr.hpp.
+// Kelly error estimator and fixed-number refinement flags for the Q1 and Q2
+// forest meshes of amr.hpp.
 //
 // Restates what femgl/src/refine.cc:146-158 asks of deal.II:
 // KellyErrorEstimator<3>::estimate(dof_handler, QGauss<2>(k+1), {} (no
@@ -8,8 +8,9 @@
 // (refine 0.3, coarsen 0).  The estimator, as documented by deal.II:
 //   eta_K^2 = sum over faces F of K of  (h_K / 24) int_F sum_c [d u_c / dn]^2
 // with h_K the cell diameter, [.] the jump across interior faces (0 on the
-// boundary: no Neumann data), all 18 components; faces between cells of
-// different levels are integrated on the smaller face.  eta is stored in
+// boundary: no Neumann data), all 18 components, by the (k+1)^2-point Gauss
+// rule on the face; faces between cells of different levels are integrated
+// on the smaller face.  eta is stored in
 // float, as the reference's Vector<float>.
 #include <algorithm>
 #include <cmath>
@@ -21,19 +22,28 @@ namespace glh {
 
 namespace {
 
-// gradient (3 x 18) of the Q1 field of leaf e at reference point xi
-void q1_grad(const Mesh &m, const double *U, int e, const double xi[3], double g[3][NC])
+double dlagrange(int deg, int i, double t)
+{
+  if (deg == 1) return i ? 1.0 : -1.0;
+  if (i == 0) return 4.0 * t - 3.0;
+  if (i == 1) return 4.0 - 8.0 * t;
+  return 4.0 * t - 1.0;
+}
+
+// gradient (3 x 18) of the Q_deg field of leaf e at reference point xi
+void field_grad(const Mesh &m, const double *U, int e, const double xi[3], double g[3][NC])
 {
   const double *h = &m.cell_h[3 * e];
+  const int deg = m.deg, n1 = m.n1;
   for (int k = 0; k < 3; ++k)
     for (int c = 0; c < NC; ++c) g[k][c] = 0.0;
-  for (int a = 0; a < 8; ++a) {
-    const int ax = a & 1, ay = (a >> 1) & 1, az = a >> 2;
-    const double lx = ax ? xi[0] : 1 - xi[0], ly = ay ? xi[1] : 1 - xi[1], lz = az ? xi[2] : 1 - xi[2];
-    const double dx = (ax ? 1.0 : -1.0) * ly * lz / h[0];
-    const double dy = (ay ? 1.0 : -1.0) * lx * lz / h[1];
-    const double dz = (az ? 1.0 : -1.0) * lx * ly / h[2];
-    const double *u = U + (size_t)m.cell_nodes[(size_t)e * 8 + a] * NC;
+  for (int a = 0; a < m.ns; ++a) {
+    const int ax = a % n1, ay = (a / n1) % n1, az = a / (n1 * n1);
+    const double lx = lagrange(deg, ax, xi[0]), ly = lagrange(deg, ay, xi[1]), lz = lagrange(deg, az, xi[2]);
+    const double dx = dlagrange(deg, ax, xi[0]) * ly * lz / h[0];
+    const double dy = lx * dlagrange(deg, ay, xi[1]) * lz / h[1];
+    const double dz = lx * ly * dlagrange(deg, az, xi[2]) / h[2];
+    const double *u = U + (size_t)m.cell_nodes[(size_t)e * m.ns + a] * NC;
     for (int c = 0; c < NC; ++c) {
       g[0][c] += dx * u[c];
       g[1][c] += dy * u[c];
@@ -47,9 +57,17 @@ void q1_grad(const Mesh &m, const double *U, int e, const double xi[3], double g
 int kelly_estimate(const Forest &f, const Mesh &m, const double *U, float *eta, std::string &err)
 {
   const int nc = (int)f.leaves.size();
-  if (m.n_cells != nc || m.deg != 1) { err = "Kelly estimator: Q1 mesh of this forest expected"; return -1; }
+  if (m.n_cells != nc || (m.deg != 1 && m.deg != 2)) {
+    err = "Kelly estimator: Q1 or Q2 mesh of this forest expected";
+    return -1;
+  }
   std::vector<double> e2(nc, 0.0);
-  const double gq[2] = {0.5 - 0.5 / std::sqrt(3.0), 0.5 + 0.5 / std::sqrt(3.0)};
+  // QGauss<2>(deg + 1) on the face, as refine.cc:148-151 asks
+  const int ng = m.deg + 1;
+  const double gq2[2] = {0.5 - 0.5 / std::sqrt(3.0), 0.5 + 0.5 / std::sqrt(3.0)}, gw2[2] = {0.5, 0.5};
+  const double gq3[3] = {0.5 - 0.5 * std::sqrt(0.6), 0.5, 0.5 + 0.5 * std::sqrt(0.6)},
+               gw3[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
+  const double *gq = ng == 2 ? gq2 : gq3, *gw = ng == 2 ? gw2 : gw3;
   for (int e = 0; e < nc; ++e) {
     const auto &L = f.leaves[e];
     const int l = L.level;
@@ -69,8 +87,8 @@ int kelly_estimate(const Forest &f, const Mesh &m, const double *U, float *eta, 
         const int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
         const double area = h[t1] * h[t2];
         double s = 0.0;
-        for (int q1 = 0; q1 < 2; ++q1)
-          for (int q2 = 0; q2 < 2; ++q2) {
+        for (int q1 = 0; q1 < ng; ++q1)
+          for (int q2 = 0; q2 < ng; ++q2) {
             double P[3];
             P[d] = m.cell_lo[3 * e + d] + (sg ? h[d] : 0.0);
             P[t1] = m.cell_lo[3 * e + t1] + gq[q1] * h[t1];
@@ -83,14 +101,14 @@ int kelly_estimate(const Forest &f, const Mesh &m, const double *U, float *eta, 
             xa[d] = sg ? 1.0 : 0.0;
             xb[d] = sg ? 0.0 : 1.0;
             double ga[3][NC], gb[3][NC];
-            q1_grad(m, U, e, xa, ga);
-            q1_grad(m, U, N, xb, gb);
+            field_grad(m, U, e, xa, ga);
+            field_grad(m, U, N, xb, gb);
             double j2 = 0.0;
             for (int c = 0; c < NC; ++c) {
               const double jmp = ga[d][c] - gb[d][c];
               j2 += jmp * jmp;
             }
-            s += 0.25 * area * j2;
+            s += gw[q1] * gw[q2] * area * j2;
           }
         auto diam = [&](int k) {
           const double *hk = &m.cell_h[3 * k];

@@ -218,7 +218,7 @@ int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int 
   const int deg = (int)p->i(X, "finite element degree");
   const int fp32 = p->b(X, "mixed precision");
   const int restart = (int)std::max(1L, std::min(126L, p->i(X, "GMRES restart")));
-  if (deg != 1) return GLH_ERR_ARG;  // adaptive meshes are Q1
+  if (deg != 1 && deg != 2) return GLH_ERR_ARG;
   const std::string geo = p->s(X, "geometry"), guess = p->s(X, "initial guess");
   if ((geo != "retangle" && geo != "cube") || (guess != "B" && guess != "A" && guess != "BnA" && guess != "BinA" && guess != "BnB" && guess != "PDW"))
     return GLH_ERR_ARG;
@@ -252,6 +252,10 @@ int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int 
   const int nroot = 1 << n_glob;
   glh_forest *f = nullptr;
   if ((rc = glh_forest_create(nroot, nroot, nroot, lo, hi, sid, &f)) != GLH_OK) return rc;
+  if ((rc = glh_forest_set_degree(f, deg)) != GLH_OK) {
+    glh_forest_destroy(f);
+    return rc;
+  }
   glh_mesh *m = nullptr;
   std::vector<double> U;
   double res = 0.0;

@@ -37,7 +37,7 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_mlp_predict", "glh_instance_residuals", "glh_set_forcing", "glh_set_residual_reference",
            "glh_set_schedule", "glh_autotune", "glh_prm_parse", "glh_prm_destroy", "glh_prm_get",
            "glh_prm_get_double", "glh_run_prm", "glh_write_vtu", "glh_initial_guess_field",
-           "glh_transfer_solution"]
+           "glh_transfer_solution", "glh_forest_set_degree"]
 
 _lib = None
 
@@ -129,6 +129,7 @@ def lib():
     L.glh_initial_guess_bna.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                         _dp]
     L.glh_transfer_solution.argtypes = [vp, vp, vp, vp, vp]
+    L.glh_forest_set_degree.argtypes = [vp, C.c_int]
     L.glh_initial_guess_field.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_uint64, _dp]
     _lib = L
     return L
@@ -255,10 +256,10 @@ class Mesh:
 
 
 class Forest:
-    """Adaptive Q1 hex mesh (glh_forest): refinement with 2:1 balance, the
-    mesh with hanging-node constraints, Kelly indicators and transfer."""
+    """Adaptive Q1 or Q2 hex mesh (glh_forest): refinement with 2:1 balance,
+    the mesh with hanging-node constraints, Kelly indicators and transfer."""
 
-    def __init__(self, n=None, lo=None, hi=None, side_id=None, handle=None):
+    def __init__(self, n=None, lo=None, hi=None, side_id=None, handle=None, degree=1):
         self.h = C.c_void_p()
         if handle is not None:
             self.h = handle
@@ -268,6 +269,11 @@ class Forest:
                                          C.byref(self.h))
             if rc:
                 raise GLHError("glh_forest_create failed (%d)" % rc)
+            if degree != 1:
+                rc = lib().glh_forest_set_degree(self.h, int(degree))
+                if rc:
+                    lib().glh_forest_destroy(self.h)
+                    raise GLHError("glh_forest_set_degree failed (%d)" % rc)
 
     @property
     def n_cells(self):
