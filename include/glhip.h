@@ -382,6 +382,10 @@ int glh_timing(glh_ctx *ctx, double out[6]);
  * time them with HIP events on the context stream: ms_avg. */
 int glh_bench_spmv(glh_ctx *ctx, int k, double *ms_avg);
 
+/* Average time of a streaming read of nbytes (multiple of 16) over k
+ * launches: the device's sustained HBM read rate, the practical ceiling of
+ * the read-dominated SpMV. */
+int glh_bench_read(glh_ctx *ctx, size_t nbytes, int k, double *ms_avg);
 /* Run k launches of a plain 16-byte-vector device copy of nbytes (a multiple
  * of 16) between two scratch buffers; ms_avg per launch.  A kernel of known
  * traffic, used to calibrate the HBM byte counters. */

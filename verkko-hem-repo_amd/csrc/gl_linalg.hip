@@ -409,6 +409,28 @@ __global__ __launch_bounds__(RT) void k_copy16(size_t n16, const float4 *__restr
   for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n16; idx += stride) dst[idx] = src[idx];
 }
 
+// streaming read of n16 16-byte words: every workgroup reads contiguous
+// blocks of RT * RD words, each thread RD independent loads in flight; one
+// float per thread is written so that the loads stay live
+constexpr int RD = 8;
+__global__ __launch_bounds__(RT) void k_read16(size_t n16, const float4 *__restrict__ src, float *__restrict__ out)
+{
+  float acc = 0.f;
+  const size_t blk = (size_t)RT * RD, nblk = (n16 + blk - 1) / blk;
+  for (size_t b = blockIdx.x; b < nblk; b += gridDim.x) {
+    const size_t base = b * blk + threadIdx.x;
+    float4 v[RD];
+#pragma unroll
+    for (int r = 0; r < RD; ++r) {
+      const size_t idx = base + (size_t)r * RT;
+      v[r] = idx < n16 ? src[idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int r = 0; r < RD; ++r) acc += (v[r].x + v[r].y) + (v[r].z + v[r].w);
+  }
+  out[(size_t)blockIdx.x * RT + threadIdx.x] = acc;
+}
+
 template <typename T>
 __global__ void k_set_scalar(T *p, T v)
 {
@@ -893,6 +915,12 @@ template <typename T>
 hipError_t launch_set_scalar(T *p, T v, hipStream_t s)
 {
   hipLaunchKernelGGL((k_set_scalar<T>), dim3(1), dim3(64), 0, s, p, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_read(size_t nbytes, const void *src, float *out, int grid, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_read16, dim3(grid), dim3(RT), 0, s, nbytes / 16, (const float4 *)src, out);
   return hipGetLastError();
 }
 

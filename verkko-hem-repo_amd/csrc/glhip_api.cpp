@@ -1913,6 +1913,29 @@ int glh_bench_spmv(glh_ctx *c, int k, double *ms_avg)
   return GLH_OK;
 }
 
+int glh_bench_read(glh_ctx *c, size_t nbytes, int k, double *ms_avg)
+{
+  if (!c || !ms_avg || k < 1 || nbytes % 16 || nbytes == 0) return GLH_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  constexpr int grid = 4096;
+  DBuf<unsigned char> a;
+  DBuf<float> out;
+  hipError_t e = a.alloc(nbytes);
+  if (e == hipSuccess) e = out.alloc((size_t)grid * 256);
+  if (e == hipSuccess) e = hipMemsetAsync(a.p, 0, nbytes, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+  for (int i = 0; i < k && e == hipSuccess; ++i) e = glh::launch_read(nbytes, a.p, out.p, grid, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  a.release();
+  out.release();
+  if (e != hipSuccess) return fail(c, GLH_ERR_HIP, std::string("read benchmark: ") + hipGetErrorString(e));
+  *ms_avg = ms / k;
+  return GLH_OK;
+}
+
 int glh_bench_copy(glh_ctx *c, size_t nbytes, int k, double *ms_avg)
 {
   if (!c || !ms_avg || k < 1 || nbytes % 16) return GLH_ERR_ARG;

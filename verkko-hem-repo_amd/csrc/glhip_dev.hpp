@@ -222,6 +222,9 @@ template <typename T>
 hipError_t launch_set_scalar(T *p, T v, hipStream_t s);
 // plain device copy of nbytes (multiple of 16): calibration kernel for HBM counters
 hipError_t launch_copy(size_t nbytes, const void *src, void *dst, hipStream_t s);
+// streaming read of nbytes (multiple of 16) by grid workgroups of 256 threads;
+// out holds grid * 256 floats
+hipError_t launch_read(size_t nbytes, const void *src, float *out, int grid, hipStream_t s);
 // out = sum_i v[i] in fixed order (deterministic)
 hipError_t launch_sum(size_t n, const double *v, double *partial, double *out, hipStream_t s);
 

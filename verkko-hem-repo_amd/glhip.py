@@ -37,7 +37,7 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_mlp_predict", "glh_instance_residuals", "glh_set_forcing", "glh_set_residual_reference",
            "glh_set_schedule", "glh_autotune", "glh_prm_parse", "glh_prm_destroy", "glh_prm_get",
            "glh_prm_get_double", "glh_run_prm", "glh_write_vtu", "glh_initial_guess_field",
-           "glh_transfer_solution", "glh_forest_set_degree"]
+           "glh_transfer_solution", "glh_forest_set_degree", "glh_bench_read"]
 
 _lib = None
 
@@ -130,6 +130,7 @@ def lib():
                                         _dp]
     L.glh_transfer_solution.argtypes = [vp, vp, vp, vp, vp]
     L.glh_forest_set_degree.argtypes = [vp, C.c_int]
+    L.glh_bench_read.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
     L.glh_initial_guess_field.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_uint64, _dp]
     _lib = L
     return L
@@ -632,6 +633,11 @@ class Context:
     def bench_spmv(self, k=10):
         v = C.c_double()
         self._chk(lib().glh_bench_spmv(self.h, int(k), C.byref(v)), "glh_bench_spmv")
+        return v.value
+
+    def bench_read(self, nbytes, k=5):
+        v = C.c_double()
+        self._chk(lib().glh_bench_read(self.h, int(nbytes), int(k), C.byref(v)), "glh_bench_read")
         return v.value
 
     def bench_copy(self, nbytes, k=5):
