@@ -138,7 +138,8 @@ int glh_forest_transfer(const glh_forest *fo, const glh_mesh *mo, const double *
  * trip of the field): ctx holds mo and its solution; uploads mn
  * (glh_upload_mesh), interpolates on the GPU and distributes the
  * constraints, which leaves the state glh_set_solution would leave with
- * glh_forest_transfer's field, bit for bit.  Single-instance meshes. */
+ * glh_forest_transfer's field, bit for bit.  Single-instance meshes.  On an
+ * error after the upload the context holds mn with a zero solution. */
 int glh_transfer_solution(glh_ctx *ctx, const glh_forest *fo, const glh_mesh *mo, const glh_forest *fn,
                           const glh_mesh *mn);
 /* KellyErrorEstimator<3>::estimate (refine.cc:146-151) of the Q1 field U:

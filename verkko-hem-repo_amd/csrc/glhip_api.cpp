@@ -30,10 +30,15 @@ using glh::NC;
 
 namespace {
 
+// device buffer owned by a context (or a scope); freed with it
 template <typename T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
   hipError_t alloc(size_t count)
   {
     if (count <= n && p) return hipSuccess;
@@ -57,6 +62,10 @@ template <typename T>
 struct HBuf {
   T *p = nullptr;
   size_t n = 0;
+  HBuf() = default;
+  HBuf(const HBuf &) = delete;
+  HBuf &operator=(const HBuf &) = delete;
+  ~HBuf() { release(); }
   hipError_t alloc(size_t count)
   {
     if (count <= n && p) return hipSuccess;
@@ -911,7 +920,7 @@ int glh_create(int device, glh_ctx **out)
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-    delete c;
+    glh_destroy(c);  // releases whatever was created
     return GLH_ERR_HIP;
   }
   *out = c;
@@ -1938,25 +1947,22 @@ int glh_bench_read(glh_ctx *c, size_t nbytes, int k, double *ms_avg)
 
 int glh_bench_copy(glh_ctx *c, size_t nbytes, int k, double *ms_avg)
 {
-  if (!c || !ms_avg || k < 1 || nbytes % 16) return GLH_ERR_ARG;
+  if (!c || !ms_avg || k < 1 || nbytes % 16 || nbytes == 0) return GLH_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->device));
-  void *a = nullptr, *b = nullptr;
-  HIPCHK(c, hipMalloc(&a, nbytes));
-  hipError_t e = hipMalloc(&b, nbytes);
-  if (e != hipSuccess) {
-    (void)hipFree(a);
-    return fail(c, GLH_ERR_HIP, "hipMalloc failed");
-  }
-  HIPCHK(c, hipMemsetAsync(a, 0, nbytes, c->stream));
-  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  for (int i = 0; i < k; ++i) HIPCHK(c, glh::launch_copy(nbytes, a, b, c->stream));
-  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-  HIPCHK(c, hipEventSynchronize(c->ev1));
+  DBuf<unsigned char> a, b;
+  hipError_t e = a.alloc(nbytes);
+  if (e == hipSuccess) e = b.alloc(nbytes);
+  if (e == hipSuccess) e = hipMemsetAsync(a.p, 0, nbytes, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+  for (int i = 0; i < k && e == hipSuccess; ++i) e = glh::launch_copy(nbytes, a.p, b.p, c->stream);
+  if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
   float ms = 0;
-  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  a.release();
+  b.release();
+  if (e != hipSuccess) return fail(c, GLH_ERR_HIP, std::string("copy benchmark: ") + hipGetErrorString(e));
   *ms_avg = ms / k;
-  (void)hipFree(a);
-  (void)hipFree(b);
   return GLH_OK;
 }
 

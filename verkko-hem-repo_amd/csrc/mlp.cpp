@@ -22,10 +22,25 @@ using glh::mlp::bf16;
 
 namespace {
 
+// device buffer owned by the network handle; movable (vectors of layers)
 template <typename T>
 struct Buf {
   T *p = nullptr;
   size_t n = 0;
+  Buf() = default;
+  Buf(const Buf &) = delete;
+  Buf &operator=(const Buf &) = delete;
+  Buf(Buf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  Buf &operator=(Buf &&o) noexcept
+  {
+    if (this != &o) {
+      release();
+      p = o.p; n = o.n;
+      o.p = nullptr; o.n = 0;
+    }
+    return *this;
+  }
+  ~Buf() { release(); }
   hipError_t alloc(size_t count)
   {
     if (count <= n && p) return hipSuccess;
