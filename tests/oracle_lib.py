@@ -290,12 +290,14 @@ def lagrange_1d(deg, i, t):
     return out
 
 
-def hanging_lines(cell_nodes, xyz, cell_lo, cell_h, tol=1e-9, deg=1):
+def hanging_lines(cell_nodes, xyz, cell_lo, cell_h, tol=1e-9, deg=1, period=(0.0, 0.0, 0.0)):
     """Independent (test-side) hanging-node detection for Q1/Q2 hex meshes: a
     node on the closure of a cell that the cell does not list follows that
     cell's Q_deg interpolant (the full tensor-product Lagrange basis evaluated
-    at the node).  Lines are closed (masters that hang themselves are
-    expanded).  Returns {node: {master: weight}}."""
+    at the node).  period[d] > 0: nodes on the - face of periodic axis d
+    also stand for the + face (they are also tested one period up).  Lines
+    are closed (masters that hang themselves are expanded).  Returns
+    {node: {master: weight}}."""
     n1 = deg + 1
     ns = n1 ** 3
     cells = np.asarray(cell_nodes).reshape(-1, ns)
@@ -303,23 +305,33 @@ def hanging_lines(cell_nodes, xyz, cell_lo, cell_h, tol=1e-9, deg=1):
     h = np.asarray(cell_h).reshape(-1, 3)
     xyz = np.asarray(xyz).reshape(-1, 3)
     raw = {}
+    shifts = [np.array([sx, sy, sz]) for sx in {0.0, period[0]} for sy in {0.0, period[1]}
+              for sz in {0.0, period[2]}]
     for e in range(cells.shape[0]):
-        t = (xyz - lo[e]) / h[e]                       # node positions in cell coordinates
-        inside = np.all((t > -tol) & (t < 1 + tol), 1)
-        inside[cells[e]] = False
-        for v in np.nonzero(inside)[0]:
-            if v in raw:
-                continue
-            tv = np.clip(t[v], 0.0, 1.0)
-            line = {}
-            for a in range(ns):
-                idx = (a % n1, (a // n1) % n1, a // (n1 * n1))
-                w = 1.0
-                for d in range(3):
-                    w *= lagrange_1d(deg, idx[d], tv[d])
-                if abs(w) > 1e-14:
-                    line[int(cells[e][a])] = line.get(int(cells[e][a]), 0.0) + w
-            raw[int(v)] = line
+        for sh in shifts:
+            t = (xyz + sh - lo[e]) / h[e]              # node positions in cell coordinates
+            inside = np.all((t > -tol) & (t < 1 + tol), 1)
+            inside[cells[e]] = False
+            for v in np.nonzero(inside)[0]:
+                if v in raw:
+                    continue
+                raw[int(v)] = _interp_line(cells[e], np.clip(t[v], 0.0, 1.0), deg, ns, n1)
+    return _close_lines(raw)
+
+
+def _interp_line(cell, tv, deg, ns, n1):
+    line = {}
+    for a in range(ns):
+        idx = (a % n1, (a // n1) % n1, a // (n1 * n1))
+        w = 1.0
+        for d in range(3):
+            w *= lagrange_1d(deg, idx[d], tv[d])
+        if abs(w) > 1e-14:
+            line[int(cell[a])] = line.get(int(cell[a]), 0.0) + w
+    return line
+
+
+def _close_lines(raw):
     closed = {}
     for v in raw:
         line = dict(raw[v])
@@ -340,7 +352,8 @@ def hanging_lines(cell_nodes, xyz, cell_lo, cell_h, tol=1e-9, deg=1):
     return closed
 
 
-def adaptive_oracle_mesh(cell_nodes, xyz, cell_lo, cell_h, faces, box_lo, box_hi, side_masks, deg=1):
+def adaptive_oracle_mesh(cell_nodes, xyz, cell_lo, cell_h, faces, box_lo, box_hi, side_masks, deg=1,
+                         period=(0.0, 0.0, 0.0)):
     """Oracle description of an adaptive Q1 mesh with deal.II constraint
     semantics: hanging-node lines first (make_hanging_node_constraints), then
     homogeneous Dirichlet lines for boundary dofs not yet constrained
@@ -349,7 +362,7 @@ def adaptive_oracle_mesh(cell_nodes, xyz, cell_lo, cell_h, faces, box_lo, box_hi
     component masks of the faces -x,+x,-y,+y,-z,+z."""
     xyz = np.asarray(xyz).reshape(-1, 3)
     n_nodes = xyz.shape[0]
-    hang = hanging_lines(cell_nodes, xyz, cell_lo, cell_h, deg=deg)
+    hang = hanging_lines(cell_nodes, xyz, cell_lo, cell_h, deg=deg, period=period)
     lines = {}                                      # dof -> {master dof: w}
     for v, line in hang.items():
         for c in range(18):
@@ -416,12 +429,14 @@ def adaptive_oracle_mesh(cell_nodes, xyz, cell_lo, cell_h, faces, box_lo, box_hi
                 np.array(con_ptr, np.int32), con_master, con_weight, row_ptr, np.array(cols, np.int32), xyz)
 
 
-def kelly(cell_nodes, cell_lo, cell_h, U, tol=1e-9, deg=1):
+def kelly(cell_nodes, cell_lo, cell_h, U, tol=1e-9, deg=1, period=(0.0, 0.0, 0.0)):
     """Test-side Kelly indicator (deal.II's documented formula, no Neumann
     data): eta_K^2 = sum_F h_K/24 int_F sum_c [du_c/dn]^2 over interior face
     pieces, integrated with (deg+1)^2 Gauss points (numpy's Gauss-Legendre)
     on the smaller face; gradients by central differences of the Q_deg
-    interpolant (exact for polynomials of degree <= 2 per direction)."""
+    interpolant (exact for polynomials of degree <= 2 per direction).
+    period[d] > 0: axis d is periodic with that length (faces at the two ends
+    of the box are neighbours)."""
     n1 = deg + 1
     ns = n1 ** 3
     cells = np.asarray(cell_nodes).reshape(-1, ns)
@@ -463,8 +478,13 @@ def kelly(cell_nodes, cell_lo, cell_h, U, tol=1e-9, deg=1):
                     if k == e:
                         continue
                     x_k = lo[k][d] if sg else hi[k][d]
+                    wrap = 0.0
                     if abs(x_k - x_f) > tol * max(1.0, abs(x_f)):
-                        continue
+                        if period[d] <= 0:
+                            continue
+                        wrap = period[d] if sg else -period[d]
+                        if abs(x_k + wrap - x_f) > tol * max(1.0, abs(x_f)):
+                            continue
                     a1, b1 = max(lo[e][t1], lo[k][t1]), min(hi[e][t1], hi[k][t1])
                     a2, b2 = max(lo[e][t2], lo[k][t2]), min(hi[e][t2], hi[k][t2])
                     if b1 - a1 <= tol or b2 - a2 <= tol:
@@ -482,7 +502,9 @@ def kelly(cell_nodes, cell_lo, cell_h, U, tol=1e-9, deg=1):
                             x[d] = x_f
                             x[t1] = a1 + q1 * (b1 - a1)
                             x[t2] = a2 + q2 * (b2 - a2)
-                            jmp = grad(e, x)[d] - grad(k, x)[d]
+                            xk = x.copy()
+                            xk[d] -= wrap
+                            jmp = grad(e, x)[d] - grad(k, xk)[d]
                             s += w1 * w2 * (b1 - a1) * (b2 - a2) * float(jmp @ jmp)
                     e2[e] += np.linalg.norm(h[e]) / 24.0 * s
                     e2[k] += np.linalg.norm(h[k]) / 24.0 * s

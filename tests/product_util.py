@@ -80,6 +80,7 @@ def adaptive_forest(G, n, lo, hi, side_id, steps, seed=0, frac=0.25, degree=1):
 
 
 def oracle_adaptive(mesh, lo, hi, side_id):
+    period = tuple((hi[d] - lo[d]) if side_id[2 * d] == 5 and side_id[2 * d + 1] == 5 else 0.0 for d in range(3))
     """glhip adaptive Mesh -> oracle_lib.Mesh with independently derived
     hanging-node and Dirichlet constraints (deal.II order and closure)."""
     a = mesh.arrays()
@@ -88,4 +89,4 @@ def oracle_adaptive(mesh, lo, hi, side_id):
     cell_lo = xyz[cn[:, 0]]
     faces = a["faces"].reshape(-1, 3) if mesh.n_faces else np.zeros((0, 3), np.int32)
     return O.adaptive_oracle_mesh(a["cell_nodes"], xyz, cell_lo, a["cell_h"], [tuple(f) for f in faces], lo, hi,
-                                  side_masks(side_id), deg=mesh.deg)
+                                  side_masks(side_id), deg=mesh.deg, period=period)

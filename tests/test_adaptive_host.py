@@ -283,3 +283,150 @@ def test_q2_kelly_matches_independent_restatement():
     # a global field of degree <= 2 per direction has no normal-derivative jumps
     Up = _continuous(m, _poly(xyz, rng.standard_normal((3, 3, 3, 18))).reshape(-1))
     assert np.abs(f.kelly(m, Up)).max() < 1e-8 * max(1.0, ref.max())
+
+
+# ------------------------------------------------------------ periodic forests
+PER = [5, 5, 5, 5, 0, 0]  # x and y periodic, A = 0 on the z faces
+
+
+def _cells(mesh):
+    a = mesh.arrays()
+    ns = (mesh.deg + 1) ** 3
+    xyz = a["node_xyz"].reshape(-1, 3)
+    cn = a["cell_nodes"].reshape(-1, ns)
+    return xyz, cn, xyz[cn[:, 0]], a["cell_h"].reshape(-1, 3)
+
+
+def _value(deg, cn_e, lo_e, h_e, U, x):
+    n1 = deg + 1
+    t = (x - lo_e) / h_e
+    out = np.zeros(18)
+    for a in range(n1 ** 3):
+        idx = (a % n1, (a // n1) % n1, a // (n1 * n1))
+        out += np.prod([O.lagrange_1d(deg, idx[d], t[d]) for d in range(3)]) * U[cn_e[a]]
+    return out
+
+
+def _face_jumps(mesh, U, period, tol=1e-9):
+    """Largest difference of the cell-wise Q_k fields of two face neighbours
+    (across periodic faces too) at sample points of their common face."""
+    deg = mesh.deg
+    xyz, cn, lo, h = _cells(mesh)
+    U = U.reshape(-1, 18)
+    hi = lo + h
+    worst, pairs = 0.0, 0
+    rng = np.random.default_rng(0)
+    for e in range(cn.shape[0]):
+        for d in range(3):
+            t1, t2 = (d + 1) % 3, (d + 2) % 3
+            x_f = hi[e][d]
+            for k in range(cn.shape[0]):
+                wrap = 0.0
+                if abs(lo[k][d] - x_f) > tol:
+                    if period[d] <= 0 or abs(lo[k][d] + period[d] - x_f) > tol:
+                        continue
+                    wrap = period[d]
+                a1, b1 = max(lo[e][t1], lo[k][t1]), min(hi[e][t1], hi[k][t1])
+                a2, b2 = max(lo[e][t2], lo[k][t2]), min(hi[e][t2], hi[k][t2])
+                if b1 - a1 <= tol or b2 - a2 <= tol:
+                    continue
+                pairs += 1
+                for _ in range(3):
+                    x = np.zeros(3)
+                    x[d] = x_f
+                    x[t1] = a1 + rng.random() * (b1 - a1)
+                    x[t2] = a2 + rng.random() * (b2 - a2)
+                    xk = x.copy()
+                    xk[d] -= wrap
+                    ve = _value(deg, cn[e], lo[e], h[e], U, x)
+                    vk = _value(deg, cn[k], lo[k], h[k], U, xk)
+                    worst = max(worst, np.abs(ve - vk).max())
+    return worst, pairs
+
+
+@pytest.mark.parametrize("deg", [1, 2])
+def test_periodic_forest_uniform_node_count_and_identification(deg):
+    f = G.Forest((4, 4, 2), LO, HI, PER, degree=deg)
+    f = f.refine(np.ones(f.n_cells, np.uint8))
+    m = f.mesh()
+    assert m.n_nodes == (8 * deg) * (8 * deg) * (4 * deg + 1)
+    assert m.constraints()["n_hanging"] == 0
+    xyz, cn, lo, h = _cells(m)
+    assert xyz[:, 0].max() < HI[0] - 1e-9 and xyz[:, 1].max() < HI[1] - 1e-9  # + faces are the - faces
+    assert np.isclose(xyz[:, 2].max(), HI[2])
+
+
+@pytest.mark.parametrize("deg,steps,seed", [(1, 2, 1), (1, 3, 2), (2, 2, 3)])
+def test_periodic_forest_fields_are_continuous_across_every_face(deg, steps, seed):
+    f = adaptive_forest(G, (3, 3, 2), LO, HI, PER, steps, seed, degree=deg)
+    m = f.mesh()
+    assert m.constraints()["n_hanging"] > 0
+    rng = np.random.default_rng(seed)
+    U = _continuous(m, rng.standard_normal(m.n_nodes * 18))
+    L = (HI[0] - LO[0], HI[1] - LO[1], 0.0)
+    worst, pairs = _face_jumps(m, U, L)
+    # the periodic faces are among the checked pairs
+    _, per_pairs = _face_jumps(m, U, (L[0], 0.0, 0.0))
+    _, plain_pairs = _face_jumps(m, U, (0.0, 0.0, 0.0))
+    assert pairs > per_pairs > plain_pairs > 0
+    assert worst <= 1e-12 * np.abs(U).max()
+    # without the lines the fields jump: the check sees a wrong constraint
+    Ub = rng.standard_normal(m.n_nodes * 18)
+    assert _face_jumps(m, Ub, L)[0] > 1e-3
+
+
+def test_periodic_forest_is_balanced_across_the_periodic_faces():
+    f = adaptive_forest(G, (3, 3, 2), LO, HI, PER, 3, 4)
+    m = f.mesh()
+    xyz, cn, lo, h = _cells(m)
+    hi = lo + h
+    checked = 0
+    for d in (0, 1):
+        t1, t2 = (d + 1) % 3, (d + 2) % 3
+        top = np.nonzero(np.isclose(hi[:, d], HI[d]))[0]
+        bot = np.nonzero(np.isclose(lo[:, d], LO[d]))[0]
+        for e in top:
+            for k in bot:
+                ov1 = min(hi[e][t1], hi[k][t1]) - max(lo[e][t1], lo[k][t1])
+                ov2 = min(hi[e][t2], hi[k][t2]) - max(lo[e][t2], lo[k][t2])
+                if ov1 > 1e-9 and ov2 > 1e-9:
+                    checked += 1
+                    assert 0.5 - 1e-12 <= h[e][d] / h[k][d] <= 2.0 + 1e-12
+    assert checked > 0
+
+
+@pytest.mark.parametrize("deg", [1, 2])
+def test_periodic_forest_kelly_matches_independent_restatement(deg):
+    f = adaptive_forest(G, (3, 3, 2), LO, HI, PER, 2, 6, degree=deg)
+    m = f.mesh()
+    xyz, cn, lo, h = _cells(m)
+    rng = np.random.default_rng(2)
+    U = _continuous(m, rng.standard_normal(m.n_nodes * 18))
+    eta = f.kelly(m, U)
+    ref = O.kelly(m.arrays()["cell_nodes"], lo, h, U, deg=deg, period=(HI[0] - LO[0], HI[1] - LO[1], 0.0))
+    assert np.abs(eta - ref).max() <= 1e-6 * ref.max()
+    nonper = O.kelly(m.arrays()["cell_nodes"], lo, h, U, deg=deg)
+    assert np.abs(eta - nonper).max() > 1e-3 * ref.max()  # the periodic faces count
+
+
+def test_periodic_forest_rejects_one_sided_periodicity():
+    with pytest.raises(G.GLHError):
+        G.Forest((3, 3, 2), LO, HI, [5, 1, 1, 1, 0, 0])
+
+
+@pytest.mark.parametrize("deg,steps,seed", [(1, 2, 1), (2, 2, 3)])
+def test_periodic_forest_lines_match_independent_derivation(deg, steps, seed):
+    f = adaptive_forest(G, (3, 3, 2), LO, HI, PER, steps, seed, degree=deg)
+    m = f.mesh()
+    xyz, cn, lo, h = _cells(m)
+    ref = O.hanging_lines(m.arrays()["cell_nodes"], xyz, lo, h, deg=deg,
+                          period=(HI[0] - LO[0], HI[1] - LO[1], 0.0))
+    c = m.constraints()
+    hp = c["hang_ptr"]
+    got = {v: {int(c["hang_m"][k]): c["hang_w"][k] for k in range(hp[v], hp[v + 1])}
+           for v in range(m.n_nodes) if hp[v + 1] > hp[v]}
+    assert len(got) > 0 and set(got) == set(ref)
+    for v in got:
+        assert set(got[v]) == set(ref[v]), v
+        for mst, w in got[v].items():
+            assert w == pytest.approx(ref[v][mst], abs=1e-14)

@@ -417,3 +417,74 @@ def test_q2_adaptive_cycles_fp32_against_fp64(ctx):
     U = ctx.solution()
     _, ro = O.assemble(om, co, U, J=False)
     assert np.linalg.norm(ro) <= 1.01e-9 * rows[2]["r0"]
+
+
+PERIODIC = [5, 5, 5, 5, 0, 0]
+
+
+@pytest.mark.parametrize("deg,steps,seed", [(1, 2, 1), (2, 1, 2)])
+def test_periodic_forest_condensed_parity_with_oracle(ctx, deg, steps, seed):
+    lo, hi = (-6.0, -6.0, -4.0), (6.0, 6.0, 4.0)
+    f, m, om, Ud, co = _case(ctx, steps, seed, sides=PERIODIC, n=(3, 3, 2) if deg == 1 else (2, 2, 2), lo=lo, hi=hi,
+                             degree=deg)
+    assert m.constraints()["n_hanging"] > 0
+    assert np.abs(ctx.solution() - Ud).max() <= 1e-15 * np.abs(Ud).max()
+    ctx.assemble()
+    Jo, ro = O.assemble(om, co, Ud)
+    assert np.abs(ctx.rhs() - ro).max() <= 1e-12 * np.abs(ro).max()
+    a = m.arrays()
+    bg = _blocks(a["row_ptr"], a["col"], ctx.jacobian(), m.n_nodes)
+    bo = _blocks(om.row_ptr, om.col, Jo, om.n_nodes)
+    scale = max(np.abs(b).max() for b in bo.values())
+    z = np.zeros((18, 18))
+    err = max(np.abs(bg.get(k, z) - bo.get(k, z)).max() / scale for k in set(bg) | set(bo))
+    say("periodic forest J rel err", err)
+    assert err <= 1e-12
+
+
+def test_periodic_forest_newton_matches_oracle(ctx):
+    lo, hi = (-6.0, -6.0, -4.0), (6.0, 6.0, 4.0)
+    f, m, om, Ud, co = _case(ctx, 2, 5, sides=PERIODIC, lo=lo, hi=hi)
+    st = ctx.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-3, restart=50)
+    say("gpu newton periodic forest", st)
+    assert st["converged"] == 1.0
+    Uo, rc, ost = O.newton(om, co, Ud, rtol=1e-10, max_newton=60, lin_tol=1e-3, restart=50)
+    assert rc == 0
+    assert st["F"] == pytest.approx(O.free_energy(om, co, Uo), rel=1e-10)
+    assert np.abs(ctx.solution() - Uo).max() <= 1e-8 * np.abs(Uo).max()
+
+
+def test_run_prm_periodic_adaptive_like_the_reference_default_build(ctx):
+    # makegrid_retangle-z-AdGR_xy-periodic.cc: x and y periodic, AdGR walls
+    # (id 4: the z-normal column vanishes) on the z faces; two adaptive cycles
+    prm = G.Prm("""
+subsection physical parameters
+  set pressure in bar = 22.0
+  set t_reduced = 0.8
+end
+subsection control parameters
+  set half x length of retangle = 6
+  set half y length of retangle = 6
+  set half z length of retangle = 5
+  set Number of initial global refinments = 2
+  set Number of refinements = 2
+  set Number of interations = 60
+  set converge accuracy = 1e-8
+  set Cycle 0 refinement threshold = 1e-3
+  set Cycle 1 refinement threshold = 1e-3
+  set Cycle 0 linear solver tol = 1e-2
+  set Cycle 1 linear solver tol = 1e-2
+  set Cycle 2 linear solver tol = 1e-2
+end
+subsection glhip
+  set boundary ids = 5 5 5 5 4 4
+  set random seed = 3
+  set GMRES restart = 50
+end
+""")
+    rows = ctx.run_prm(prm)
+    for r in rows:
+        say(r)
+    assert len(rows) >= 2 and rows[-1]["cells"] > rows[0]["cells"]
+    assert rows[-1]["residual"] <= 1e-8
+    assert ctx.residual_norm() == pytest.approx(rows[-1]["residual"], rel=1e-6)

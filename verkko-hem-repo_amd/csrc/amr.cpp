@@ -65,6 +65,17 @@ int forest_init(Forest &f, int nx, int ny, int nz, const double lo[3], const dou
   }
   for (int k = 0; k < 3; ++k)
     if (!(hi[k] > lo[k])) { err = "empty box"; return -1; }
+  const int nroot[3] = {nx, ny, nz};
+  for (int k = 0; k < 3; ++k) {
+    if ((side_id[2 * k] == 5) != (side_id[2 * k + 1] == 5)) {
+      err = "a periodic axis needs side id 5 on both faces";
+      return -1;
+    }
+    if (side_id[2 * k] == 5 && nroot[k] < 2) {
+      err = "a periodic axis needs at least two root cells";
+      return -1;
+    }
+  }
   f = Forest();
   f.nx = nx; f.ny = ny; f.nz = nz;
   for (int k = 0; k < 3; ++k) { f.lo[k] = lo[k]; f.hi[k] = hi[k]; }
@@ -105,9 +116,14 @@ int forest_refine(Forest &f, const std::vector<char> &flag, std::string &err)
         for (int dy = -1; dy <= 1; ++dy)
           for (int dx = -1; dx <= 1; ++dx) {
             if (!dx && !dy && !dz) continue;
-            const int i = L.ix + dx, j = L.iy + dy, k = L.iz + dz;
-            if (i < 0 || j < 0 || k < 0 || i >= n[0] || j >= n[1] || k >= n[2]) continue;
-            const int K = f.locate(L.level, i, j, k);
+            int q[3] = {L.ix + dx, L.iy + dy, L.iz + dz};
+            bool out = false;
+            for (int d = 0; d < 3; ++d) {
+              if (f.periodic(d)) q[d] = (q[d] + n[d]) % n[d];  // across the periodic faces
+              else if (q[d] < 0 || q[d] >= n[d]) out = true;
+            }
+            if (out) continue;
+            const int K = f.locate(L.level, q[0], q[1], q[2]);
             if (K >= 0 && f.leaves[K].level < L.level - 1) fl[K] = 1;
           }
     }
@@ -147,12 +163,21 @@ int forest_mesh(const Forest &f, Mesh &m, std::string &err, int deg)
   m.cell_h.resize((size_t)nc * 3);
   m.cell_lo.resize((size_t)nc * 3);
   m.cell_level.resize(nc);
+  // a node on the + face of a periodic axis is the node of its - face
+  const int64_t NN[3] = {NX, NY, NZ};
+  const bool per[3] = {f.periodic(0), f.periodic(1), f.periodic(2)};
+  auto canon = [&](int64_t &x, int64_t &y, int64_t &z) {
+    int64_t *q[3] = {&x, &y, &z};
+    for (int d = 0; d < 3; ++d)
+      if (per[d]) *q[d] = ((*q[d] % NN[d]) + NN[d]) % NN[d];
+  };
   for (int e = 0; e < nc; ++e) {
     const auto &L = f.leaves[e];
     const int64_t s = (int64_t)1 << (LM - L.level);
     const int64_t x0 = L.ix * s * deg, y0 = L.iy * s * deg, z0 = L.iz * s * deg;
     for (int a = 0; a < ns; ++a) {
-      const int64_t x = x0 + (a % n1) * s, y = y0 + ((a / n1) % n1) * s, z = z0 + (a / (n1 * n1)) * s;
+      int64_t x = x0 + (a % n1) * s, y = y0 + ((a / n1) % n1) * s, z = z0 + (a / (n1 * n1)) * s;
+      canon(x, y, z);
       auto it = vid.find(vkey(x, y, z));
       int v;
       if (it == vid.end()) {
@@ -187,21 +212,26 @@ int forest_mesh(const Forest &f, Mesh &m, std::string &err, int deg)
   for (int v = 0; v < nn; ++v) {
     const int64_t P[3] = {vx[3 * v], vx[3 * v + 1], vx[3 * v + 2]};
     // finest-lattice cells whose closure holds the node: floor((P-1)/deg)
-    // and floor(P/deg) in every direction
-    int64_t cand[3][2];
+    // and floor(P/deg) in every direction; on a periodic axis the cell below
+    // the - face is the last cell, and the node is seen one period up
+    int64_t cand[3][2], shift[3][2];
     for (int d = 0; d < 3; ++d) {
       cand[d][0] = P[d] > 0 ? (P[d] - 1) / deg : -1;
       cand[d][1] = P[d] / deg;
+      shift[d][0] = shift[d][1] = 0;
+      if (per[d] && cand[d][0] < 0) { cand[d][0] = NL[d] - 1; shift[d][0] = NN[d]; }
     }
     for (int o = 0; o < 8 && !hanging[v]; ++o) {
-      const int64_t cx = cand[0][o & 1], cy = cand[1][(o >> 1) & 1], cz = cand[2][o >> 2];
+      const int b[3] = {o & 1, (o >> 1) & 1, o >> 2};
+      const int64_t cx = cand[0][b[0]], cy = cand[1][b[1]], cz = cand[2][b[2]];
       if (cx < 0 || cy < 0 || cz < 0 || cx >= NL[0] || cy >= NL[1] || cz >= NL[2]) continue;
       const int K = f.locate(LM, (int)cx, (int)cy, (int)cz);
       if (K < 0) { err = "forest lookup failed"; return -1; }
       const auto &L = f.leaves[K];
       const int64_t s = (int64_t)1 << (LM - L.level), ext = s * deg;
       const int64_t O[3] = {L.ix * ext, L.iy * ext, L.iz * ext};
-      const int64_t r[3] = {P[0] - O[0], P[1] - O[1], P[2] - O[2]};
+      const int64_t r[3] = {P[0] + shift[0][b[0]] - O[0], P[1] + shift[1][b[1]] - O[1],
+                            P[2] + shift[2][b[2]] - O[2]};
       bool node = true, bnd = false;
       for (int d = 0; d < 3; ++d) {
         if (r[d] < 0 || r[d] > ext) { err = "forest lookup off the node"; return -1; }
@@ -231,7 +261,9 @@ int forest_mesh(const Forest &f, Mesh &m, std::string &err, int deg)
           w *= lagrange(deg, i, (double)r[d] / (double)ext);
         }
         if (w == 0.0) continue;
-        auto it = vid.find(vkey(O[0] + q[0], O[1] + q[1], O[2] + q[2]));
+        int64_t mx = O[0] + q[0], my = O[1] + q[1], mz = O[2] + q[2];
+        canon(mx, my, mz);
+        auto it = vid.find(vkey(mx, my, mz));
         if (it == vid.end()) { err = "master node missing"; return -1; }
         line[v].push_back({it->second, w});
       }

@@ -39,6 +39,10 @@ struct Forest {
   }
   // leaf covering cell (i,j,k) of level l (at that level or coarser), or -1
   int locate(int l, int i, int j, int k) const;
+  // axis d is periodic: side id 5 on both of its faces (the matched faces of
+  // makegrid_retangle-z-AdGR_xy-periodic.cc); nodes on its + face are the
+  // nodes of its - face, and neighbours wrap around
+  bool periodic(int d) const { return side_id[2 * d] == 5 && side_id[2 * d + 1] == 5; }
   void reindex();  // Morton sort of the leaves and rebuild of `where`
 };
 

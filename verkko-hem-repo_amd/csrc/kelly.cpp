@@ -77,7 +77,13 @@ int kelly_estimate(const Forest &f, const Mesh &m, const double *U, float *eta, 
       for (int sg = 0; sg < 2; ++sg) {
         int nb[3] = {idx[0], idx[1], idx[2]};
         nb[d] += sg ? 1 : -1;
-        if (nb[d] < 0 || nb[d] >= n[d]) continue;  // boundary face: no Neumann data, no jump
+        // across a periodic face the neighbour sits one period away
+        double wrap = 0.0;
+        if (nb[d] < 0 || nb[d] >= n[d]) {
+          if (!f.periodic(d)) continue;  // boundary face: no Neumann data, no jump
+          wrap = (sg ? 1.0 : -1.0) * (f.hi[d] - f.lo[d]);
+          nb[d] = (nb[d] + n[d]) % n[d];
+        }
         const int N = f.locate(l, nb[0], nb[1], nb[2]);
         if (N < 0) continue;                         // finer neighbours integrate their pieces
         const int ln = f.leaves[N].level;
@@ -96,7 +102,7 @@ int kelly_estimate(const Forest &f, const Mesh &m, const double *U, float *eta, 
             double xa[3], xb[3];
             for (int k = 0; k < 3; ++k) {
               xa[k] = (P[k] - m.cell_lo[3 * e + k]) / h[k];
-              xb[k] = (P[k] - m.cell_lo[3 * N + k]) / m.cell_h[3 * N + k];
+              xb[k] = (P[k] - (k == d ? wrap : 0.0) - m.cell_lo[3 * N + k]) / m.cell_h[3 * N + k];
             }
             xa[d] = sg ? 1.0 : 0.0;
             xb[d] = sg ? 0.0 : 1.0;

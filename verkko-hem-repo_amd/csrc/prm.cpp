@@ -75,7 +75,7 @@ void declare(std::map<std::string, Entry> &m)
   d(C, "primary step length of dampped newton iteration", "0.83", 'd');
   // the build's own keys (compile-time choices in the reference)
   d(X, "geometry", "retangle", 's');                 // retangle | cube (makegrid_* variants)
-  d(X, "boundary ids", "1 1 1 1 4 4", 's');          // faces -x +x -y +y -z +z
+  d(X, "boundary ids", "1 1 1 1 4 4", 's');          // faces -x +x -y +y -z +z (5 5: periodic axis)
   d(X, "finite element degree", "1", 'i');
   d(X, "initial guess", "B", 's');                   // B | A | BnA | BinA | BnB | PDW
   d(X, "random seed", "0", 'i');
@@ -242,7 +242,7 @@ int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int 
   {
     std::istringstream ss(p->s(X, "boundary ids"));
     for (int k = 0; k < 6; ++k)
-      if (!(ss >> sid[k]) || sid[k] < 0 || sid[k] > 4) return GLH_ERR_ARG;
+      if (!(ss >> sid[k]) || sid[k] < 0 || sid[k] > 5) return GLH_ERR_ARG;  // 5: periodic pair
   }
   double mp[12];
   if (glh_matep(pres, t, scc, mp) != GLH_OK) return GLH_ERR_RANGE;
