@@ -180,10 +180,13 @@ int glh_write_vtu(const glh_mesh *m, const double *U, const double *dU, const ch
  * "subsection NAME" ... "end", "set KEY = VALUE", '#' comments) with the keys,
  * defaults and patterns of confreader/src/declare.cc:110-300; undeclared keys
  * and pattern violations are errors (message in err).  The subsection
- * "glhip" holds what the reference fixes at compile time: geometry
- * (retangle | cube), boundary ids (six ids, faces -x +x -y +y -z +z),
- * finite element degree (1), initial guess (B | A | BnA), random seed, noise
- * amplitude, mixed precision, GMRES restart. */
+ * "glhip" holds what the reference fixes at compile time or in its driver:
+ * geometry (retangle | cube), boundary ids (six ids, faces -x +x -y +y -z +z;
+ * 5 5 = periodic axis), finite element degree (1 | 2), initial guess (B | A |
+ * BnA | BinA | BnB | PDW), random seed, noise amplitude, mixed precision,
+ * GMRES restart, output directory (VTU files per Newton iteration in
+ * DIR/refine-cycle_<cycle>/ as run.cc:219-224 writes them; empty: none) and
+ * verbose (progress lines of run() on stdout). */
 typedef struct glh_prm glh_prm;
 int glh_prm_parse(const char *text, glh_prm **out, char *err, int err_len);
 int glh_prm_destroy(glh_prm *p);

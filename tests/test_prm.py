@@ -91,3 +91,28 @@ def test_errors_are_reported(text, what):
 def test_amplitude_may_be_negative():
     p = G.Prm("subsection physical parameters\n set amplitude u parameter = -0.5\nend")
     assert p.get_double("physical parameters", "amplitude u parameter") == -0.5
+
+
+def _exe():
+    import os
+    p = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "verkko-hem-repo_amd", "glhip_run")
+    assert os.path.exists(p), "glhip_run is built with the library"
+    return p
+
+
+def test_solver_executable_reports_bad_parameter_files(tmp_path):
+    # sol/src/main.cc:120-145: an error ends the run with status 1 and a message
+    import subprocess
+    bad = tmp_path / "configuration.prm"
+    bad.write_text("subsection control parameters\n  set Number of refinements = -3\nend\n")
+    r = subprocess.run([_exe(), str(bad)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Aborting!" in r.stderr and "Number of refinements" in r.stderr
+    r = subprocess.run([_exe(), str(tmp_path / "missing.prm")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "cannot open" in r.stderr
+    r = subprocess.run([_exe(), "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "usage" in r.stdout
+
+
+def test_output_and_verbose_keys_are_declared():
+    p = G.Prm("subsection glhip\n  set output directory = out\n  set verbose = true\nend\n")
+    assert p.get("glhip", "output directory") == "out" and p.get("glhip", "verbose") == "true"

@@ -63,6 +63,14 @@ def build(jobs=4, verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    # the solver executable (the reference's sol/ driver) over the library
+    exe = os.path.join(PKG, "glhip_run")
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), os.path.join(SRC, "run_main.cpp"),
+           "-o", exe + ".tmp", "-L", PKG, "-lglhip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("glhip_run link failed:\n" + r.stdout + r.stderr)
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 

@@ -11,8 +11,12 @@
 // Newton step, the stuck-refine test |R - R_last| < threshold, the final
 // absolute test ||R|| <= converge accuracy, and adaptive (Kelly, fixed
 // fraction) or global refinement between cycles.
+#include <sys/stat.h>
+
 #include <cctype>
+#include <cerrno>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <sstream>
@@ -82,6 +86,16 @@ void declare(std::map<std::string, Entry> &m)
   d(X, "noise amplitude", "0.05", 'd');
   d(X, "mixed precision", "false", 'b');
   d(X, "GMRES restart", "100", 'i');
+  d(X, "output directory", "", 's');                 // output_results per iteration (empty: none)
+  d(X, "verbose", "false", 'b');                     // progress lines of run() on stdout
+}
+
+// mkdir that accepts an existing directory
+bool make_dir(const std::string &path)
+{
+  if (::mkdir(path.c_str(), 0755) == 0) return true;
+  struct stat sb;
+  return errno == EEXIST && ::stat(path.c_str(), &sb) == 0 && S_ISDIR(sb.st_mode);
 }
 
 bool check_value(const Entry &e, const std::string &v, std::string &why)
@@ -307,14 +321,33 @@ int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int 
     // the iteration loop of run.cc:202-249
     double res_last = 0.0;
     int it = 0, lin_total = 0;
+    const std::string outdir = p->s(X, "output directory");
+    const bool verbose = p->b(X, "verbose");
+    std::string cycdir;
+    if (!outdir.empty()) {  // "./refine-cycle_<cycle>/" of run.cc:222
+      cycdir = outdir + "/refine-cycle_" + std::to_string(cycle);
+      if (!make_dir(outdir) || !make_dir(cycdir)) { rc = GLH_ERR_ARG; break; }
+    }
     for (it = 0; it <= n_iter; ++it) {
       double st[2] = {0, 0}, out[4];
+      if (verbose) std::printf("Refinement cycle : %d, iteration_loop: %d\n", cycle, it);
       if ((rc = glh_assemble(ctx, GLH_ASM_J | GLH_ASM_R | (fp32 ? GLH_ASM_FP32 : 0)))) break;
       rc = glh_solve(ctx, tol[std::min(cycle, 4)], maxit, restart, st);
       if (rc) break;  // SolverControl::NoConvergence ends the run, as in the reference
       lin_total += (int)st[0];
       if ((rc = glh_newton_step(ctx, step, damped, out))) break;
       res = out[1];
+      if (verbose) {
+        std::printf(" linear solve: %d iterations (tol %g), Newton step: ||R|| %.6e\n", (int)st[0],
+                    tol[std::min(cycle, 4)], res);
+        std::fflush(stdout);
+      }
+      if (!cycdir.empty()) {  // output_results (io.cc:105-168), counter = iteration_loop
+        std::vector<double> dU(U.size());
+        if ((rc = glh_get_solution(ctx, U.data())) || (rc = glh_get_update(ctx, dU.data())) ||
+            (rc = glh_write_vtu(m, U.data(), dU.data(), cycdir.c_str(), it)))
+          break;
+      }
       // stuck: refine (the cycle test first, so no threshold beyond the four is read)
       if (cycle < n_cycles && res > conv && std::fabs(res - res_last) < thr[std::min(cycle, 3)]) break;
       if (res <= conv) break;
