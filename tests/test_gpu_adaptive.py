@@ -29,18 +29,18 @@ def ctx():
     c.close()
 
 
-def _case(ctx, steps, seed, p=22.0, t=0.8, amp=0.05, sides=SIDES, n=(3, 3, 2), lo=LO, hi=HI, degree=1):
+def _case(ctx, steps, seed, p=22.0, t=0.8, amp=0.05, sides=SIDES, n=(3, 3, 2), lo=LO, hi=HI, degree=1, bt=1.0e10):
     f = adaptive_forest(G, n, lo, hi, sides, steps, seed, degree=degree)
     m = f.mesh()
     say("adaptive case: steps %d seed %d cells %d nodes %d hanging %d" % (steps, seed, m.n_cells, m.n_nodes,
                                                                          m.constraints()["n_hanging"]))
-    ctx.set_material_pt(p, t)
+    ctx.set_material_pt(p, t, bt=bt)
     ctx.upload_mesh(m)
     om = oracle_adaptive(m, lo, hi, sides)
     U = O.initial_guess(m.n_nodes, O.matep(p, t)["gap_B"], amp=amp, seed=seed)
     Ud = O.distribute(om, U)
     ctx.set_solution(U)
-    return f, m, om, Ud, O.coef(p, t)
+    return f, m, om, Ud, O.coef(p, t, bt=bt)
 
 
 def _dense(row_ptr, col, val, n_nodes):
@@ -531,3 +531,25 @@ end
     for cyc in (0, 1):
         d = tmp_path / "out" / ("refine-cycle_%d" % cyc)
         assert (d / "solution_00.pvtu").exists() and (d / "solution_00.0.vtu").exists()
+
+
+@pytest.mark.parametrize("deg", [1, 2])
+def test_robin_walls_on_adaptive_mesh_match_oracle(ctx, deg):
+    # AdGR diffuse walls (bt < 1e10, assemble.cc:286-348) on the z faces of
+    # a refined forest: the face terms of the coarse and the fine boundary cells
+    sides = [1, 1, 1, 1, 4, 4]
+    f, m, om, Ud, co = _case(ctx, 1, 7, sides=sides, n=(2, 2, 2), degree=deg, bt=3.0)
+    ctx.assemble()
+    Jo, ro = O.assemble(om, co, Ud)
+    assert np.abs(ctx.rhs() - ro).max() <= 1e-12 * np.abs(ro).max()
+    a = m.arrays()
+    bg = _blocks(a["row_ptr"], a["col"], ctx.jacobian(), m.n_nodes)
+    bo = _blocks(om.row_ptr, om.col, Jo, om.n_nodes)
+    scale = max(np.abs(b).max() for b in bo.values())
+    z = np.zeros((18, 18))
+    err = max(np.abs(bg.get(k, z) - bo.get(k, z)).max() / scale for k in set(bg) | set(bo))
+    say("Robin adaptive J rel err", err)
+    assert err <= 1e-12
+    # the wall term is present: without it the matrices differ
+    Jn, rn = O.assemble(om, O.coef(22.0, 0.8), Ud)
+    assert np.abs(rn - ro).max() > 1e-6 * np.abs(ro).max()
