@@ -115,9 +115,12 @@ int glh_forest_create(int nx, int ny, int nz, const double lo[3], const double h
                       glh_forest **out);
 int glh_forest_destroy(glh_forest *f);
 int64_t glh_forest_n_cells(const glh_forest *f);
-/* New forest: the cells with flags[e] != 0 split into 8, then neighbours
- * (faces, edges, vertices) more than one level apart are refined
- * (execute_coarsening_and_refinement with 2:1 balance).  f is unchanged. */
+/* New forest (execute_coarsening_and_refinement with 2:1 balance,
+ * refine.cc:160-165): flags[e] = 2 marks cell e for coarsening, 1 for
+ * refinement, 0 neither.  Complete families of 8 cells all marked 2 become
+ * their parent where the 2:1 balance allows; then the cells marked 1 split
+ * into 8 and neighbours (faces, edges, vertices) more than one level apart
+ * are refined.  f is unchanged. */
 int glh_forest_refine(const glh_forest *f, const uint8_t *flags, glh_forest **out);
 /* Element degree (1 or 2, default 1) of the meshes glh_forest_mesh builds
  * from f and from the forests refined from it (FE_Q(degree) of
@@ -148,6 +151,11 @@ int glh_kelly(const glh_forest *f, const glh_mesh *m, const double *U, float *et
 /* refine_and_coarsen_fixed_number(eta, frac, 0) (refine.cc:153-157): flags
  * the floor(frac * n) cells of largest eta (ties in cell order). */
 int glh_flag_fixed_number(int64_t n, const float *eta, double frac, uint8_t *flags);
+/* refine_and_coarsen_fixed_number (refine.cc:153-157) with a coarsening
+ * fraction: flags 1 for the floor(refine_frac n) cells of largest indicator,
+ * 2 for the floor(coarsen_frac n) of smallest, 0 otherwise (the input of
+ * glh_forest_refine); refine_frac + coarsen_frac <= 1. */
+int glh_flag_refine_coarsen(int64_t n, const float *eta, double refine_frac, double coarsen_frac, uint8_t *flags);
 
 /* FemGL::run (femgl/src/run.cc:109-260) for adaptive Q1 meshes: cycle 0
  * solves on the mesh of f0 from U0 (n_nodes*18 of that mesh); each of the

@@ -430,3 +430,108 @@ def test_periodic_forest_lines_match_independent_derivation(deg, steps, seed):
         assert set(got[v]) == set(ref[v]), v
         for mst, w in got[v].items():
             assert w == pytest.approx(ref[v][mst], abs=1e-14)
+
+
+# ----------------------------------------------------------------- coarsening
+def _touching_levels_ok(mesh, tol=1e-9):
+    """2:1 balance seen from geometry: cells whose closed boxes touch differ
+    in size by at most a factor 2."""
+    xyz, cn, lo, h = _cells(mesh)
+    hi = lo + h
+    for e in range(cn.shape[0]):
+        touch = np.all((lo <= hi[e] + tol) & (hi >= lo[e] - tol), 1)
+        r = h[touch, 0] / h[e, 0]
+        if np.any(r > 2.0 + 1e-12) or np.any(r < 0.5 - 1e-12):
+            return False
+    return True
+
+
+def test_refine_coarsen_flags():
+    eta = np.array([0.5, 3.0, 1.0, 3.0, 0.1, 2.0, 0.0, 7.0, 1.5, 0.2], np.float32)
+    fl = G.flag_refine_coarsen(eta, 0.3, 0.2)
+    assert set(np.nonzero(fl == 1)[0]) == {7, 1, 3} and set(np.nonzero(fl == 2)[0]) == {6, 4}
+    with pytest.raises(G.GLHError):
+        G.flag_refine_coarsen(eta, 0.7, 0.5)
+
+
+@pytest.mark.parametrize("deg", [1, 2])
+def test_coarsening_a_uniform_refinement_restores_the_roots(deg):
+    f0 = G.Forest((3, 2, 2), LO, HI, SIDES, degree=deg)
+    m0 = f0.mesh()
+    f1 = f0.refine(np.ones(f0.n_cells, np.uint8))
+    f2 = f1.refine(np.full(f1.n_cells, 2, np.uint8))
+    m2 = f2.mesh()
+    assert f2.n_cells == f0.n_cells and m2.n_nodes == m0.n_nodes
+    assert np.array_equal(m2.arrays()["node_xyz"], m0.arrays()["node_xyz"])
+    # an incomplete family stays: one child not flagged
+    fl = np.full(f1.n_cells, 2, np.uint8)
+    fl[0] = 0
+    assert f1.refine(fl).n_cells == f0.n_cells - 1 + 8
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_coarsening_keeps_the_balance_and_transfers_by_injection(seed):
+    f = adaptive_forest(G, (3, 3, 2), LO, HI, SIDES, 3, seed, frac=0.35)
+    m = f.mesh()
+    rng = np.random.default_rng(seed)
+    xyz0, cn0, lo0, h0 = _cells(m)
+    centre = lo0 + 0.5 * h0
+    # coarsen everything in x < 0 (whole families there collapse), refine
+    # five cells in x > 0
+    fl = np.where(centre[:, 0] < 0.0, 2, 0).astype(np.uint8)
+    fl[rng.choice(np.nonzero(centre[:, 0] > 0.0)[0], 5, replace=False)] = 1
+    f2 = f.refine(fl)
+    m2 = f2.mesh()
+    assert f2.n_cells < f.n_cells  # some families did coarsen
+    assert _touching_levels_ok(m2)
+    # coarse nodes sit on nodes of the fine mesh: the transfer is an injection
+    U = _continuous(m, rng.standard_normal(m.n_nodes * 18)).reshape(-1, 18)
+    U2 = f.transfer(m, U.reshape(-1), f2, m2).reshape(-1, 18)
+    xyz, xyz2 = m.arrays()["node_xyz"].reshape(-1, 3), m2.arrays()["node_xyz"].reshape(-1, 3)
+    key = {tuple(np.round(x, 9)): i for i, x in enumerate(xyz)}
+    hang2 = np.diff(m2.constraints()["hang_ptr"]) if m2.constraints()["n_hanging"] else np.zeros(m2.n_nodes)
+    checked = 0
+    for v in range(m2.n_nodes):
+        k = key.get(tuple(np.round(xyz2[v], 9)))
+        if k is not None and not (len(hang2) and hang2[v] > 0):
+            assert np.abs(U2[v] - U[k]).max() <= 1e-12 * np.abs(U).max()
+            checked += 1
+    assert checked > 0
+
+
+@pytest.mark.parametrize("deg", [1, 2])
+def test_a_family_next_to_a_refined_cell_keeps_its_children(deg):
+    # coarsening root 0's family while its neighbour's child is refined would
+    # put a level-0 cell next to level-2 cells: the family stays, and its
+    # children keep their values
+    f0 = G.Forest((3, 2, 2), LO, HI, SIDES, degree=deg)
+    f1 = f0.refine(np.ones(f0.n_cells, np.uint8))
+    m1 = f1.mesh()
+    xyz1, cn1, lo1, h1 = _cells(m1)
+    c = lo1 + 0.5 * h1
+    w = (HI[0] - LO[0]) / 3.0
+    fam = c[:, 0] < LO[0] + w  # the children of the roots in the first column
+    fl = np.where(fam, 2, 0).astype(np.uint8)
+    # a child of the next root on the shared face, in the corner of the domain
+    # (so that it touches only one of the four families)
+    cand = np.nonzero((c[:, 0] > LO[0] + w) & (c[:, 0] < LO[0] + 1.5 * w))[0]
+    nb = cand[np.argmin(c[cand, 1] + c[cand, 2])]
+    fl[nb] = 1
+    f2 = f1.refine(fl)
+    assert f2.n_cells == f1.n_cells + 7 - 7 * (4 - 1)  # 3 of the 4 families coarsen
+    m2 = f2.mesh()
+    assert _touching_levels_ok(m2)
+    rng = np.random.default_rng(deg)
+    U = _continuous(m1, rng.standard_normal(m1.n_nodes * 18)).reshape(-1, 18)
+    U2 = f1.transfer(m1, U.reshape(-1), f2, m2).reshape(-1, 18)
+    key = {tuple(np.round(x, 9)): i for i, x in enumerate(xyz1.reshape(-1, 3))}
+    cons = m2.constraints()
+    hang2 = np.diff(cons["hang_ptr"]) if cons["n_hanging"] else np.zeros(m2.n_nodes, int)
+    xyz2 = m2.arrays()["node_xyz"].reshape(-1, 3)
+    kept = 0
+    for v in range(m2.n_nodes):
+        k = key.get(tuple(np.round(xyz2[v], 9)))
+        if k is not None and hang2[v] == 0:
+            assert np.abs(U2[v] - U[k]).max() <= 1e-12 * np.abs(U).max()
+            kept += 1
+    assert kept > 0

@@ -553,3 +553,44 @@ def test_robin_walls_on_adaptive_mesh_match_oracle(ctx, deg):
     # the wall term is present: without it the matrices differ
     Jn, rn = O.assemble(om, O.coef(22.0, 0.8), Ud)
     assert np.abs(rn - ro).max() > 1e-6 * np.abs(ro).max()
+
+
+def test_run_prm_coarsens_with_the_coarsen_ratio(ctx):
+    # refine_and_coarsen_fixed_number with a nonzero "adaptive coarsen ratio"
+    # (refine.cc:140-157): the cells of lowest Kelly indicator are coarsened
+    # where whole families are flagged and balance allows (on 8^3 cells every
+    # such family touches a refined one), so the refined 16^3 mesh has fewer
+    # cells than without coarsening, and the last cycle still converges
+    def prm(cratio):
+        return G.Prm(f"""
+subsection physical parameters
+  set pressure in bar = 22.0
+  set t_reduced = 0.8
+end
+subsection control parameters
+  set half x length of retangle = 5
+  set half y length of retangle = 5
+  set half z length of retangle = 6
+  set Number of initial global refinments = 4
+  set Number of refinements = 1
+  set Number of interations = 60
+  set converge accuracy = 1e-7
+  set adaptive refinment ratio = 0.2
+  set adaptive coarsen ratio = {cratio}
+  set Cycle 0 linear solver tol = 1e-2
+  set Cycle 1 linear solver tol = 1e-2
+end
+subsection glhip
+  set boundary ids = 1 1 1 1 0 0
+  set random seed = 1
+  set GMRES restart = 50
+end
+""")
+    plain = ctx.run_prm(prm(0.0))
+    rows = ctx.run_prm(prm(0.5))
+    for r in plain + rows:
+        say(r)
+    assert plain[0]["cells"] == rows[0]["cells"] == 4096
+    assert rows[1]["cells"] < plain[1]["cells"]
+    assert rows[1]["residual"] <= 1e-7
+    assert ctx.residual_norm() == pytest.approx(rows[1]["residual"], rel=1e-6)

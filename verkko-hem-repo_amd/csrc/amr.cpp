@@ -87,17 +87,109 @@ int forest_init(Forest &f, int nx, int ny, int nz, const double lo[3], const dou
   return 0;
 }
 
-int forest_refine(Forest &f, const std::vector<char> &flag, std::string &err)
+namespace {
+
+// 2:1 balance violations: leaves that have a neighbour (26 directions, across
+// periodic faces too) more than one level finer; returned as leaf indices
+std::vector<int> too_coarse(const Forest &f)
 {
-  if (flag.size() != f.leaves.size()) { err = "one refinement flag per cell expected"; return -1; }
-  std::vector<char> fl(flag);
+  std::vector<int> out;
+  std::vector<char> seen(f.leaves.size(), 0);
+  for (const auto &L : f.leaves) {
+    if (L.level < 2) continue;
+    const int n[3] = {f.nx << L.level, f.ny << L.level, f.nz << L.level};
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          if (!dx && !dy && !dz) continue;
+          int q[3] = {L.ix + dx, L.iy + dy, L.iz + dz};
+          bool outside = false;
+          for (int d = 0; d < 3; ++d) {
+            if (f.periodic(d)) q[d] = (q[d] + n[d]) % n[d];  // across the periodic faces
+            else if (q[d] < 0 || q[d] >= n[d]) outside = true;
+          }
+          if (outside) continue;
+          const int K = f.locate(L.level, q[0], q[1], q[2]);
+          if (K >= 0 && f.leaves[K].level < L.level - 1 && !seen[K]) {
+            seen[K] = 1;
+            out.push_back(K);
+          }
+        }
+  }
+  return out;
+}
+
+// Coarsening of complete families: the 8 children of a parent, all leaves
+// and all flagged 2, become the parent, unless that breaks the 2:1 balance
+// with the rest of the forest.  fl is carried over to the new
+// leaves (0 for the parents).  A parent that the later refinement has to
+// refine again for balance gets back the same 8 children, and the transfer
+// (from the old mesh) keeps their values.
+void coarsen_families(Forest &f, std::vector<char> &fl)
+{
+  std::unordered_map<uint64_t, int> count;
+  for (size_t e = 0; e < f.leaves.size(); ++e) {
+    const auto &L = f.leaves[e];
+    if (fl[e] == 2 && L.level > 0) ++count[Forest::key(L.level - 1, L.ix >> 1, L.iy >> 1, L.iz >> 1)];
+  }
+  std::unordered_map<uint64_t, char> parent;  // candidate parents (1 = coarsen)
+  for (auto &kv : count)
+    if (kv.second == 8) parent[kv.first] = 1;
+  if (parent.empty()) return;
+  const std::vector<Forest::Leaf> old = f.leaves;
+  const std::vector<char> oldfl = fl;
+  for (int pass = 0; pass < 64; ++pass) {
+    std::vector<Forest::Leaf> nl;
+    std::vector<char> nf;
+    std::unordered_map<uint64_t, char> added;
+    for (size_t e = 0; e < old.size(); ++e) {
+      const auto &L = old[e];
+      const uint64_t pk = L.level > 0 ? Forest::key(L.level - 1, L.ix >> 1, L.iy >> 1, L.iz >> 1) : ~0ULL;
+      auto it = parent.find(pk);
+      if (L.level > 0 && it != parent.end() && it->second) {
+        if (!added.count(pk)) {
+          added[pk] = 1;
+          nl.push_back({L.level - 1, L.ix >> 1, L.iy >> 1, L.iz >> 1});
+          nf.push_back(0);
+        }
+      } else {
+        nl.push_back(L);
+        nf.push_back(oldfl[e] == 1 ? 1 : 0);
+      }
+    }
+    // the flags follow their leaves into the Morton order of the new forest
+    std::unordered_map<uint64_t, char> flk;
+    for (size_t e = 0; e < nl.size(); ++e) flk[Forest::key(nl[e].level, nl[e].ix, nl[e].iy, nl[e].iz)] = nf[e];
+    f.leaves.swap(nl);
+    f.reindex();
+    fl.assign(f.leaves.size(), 0);
+    for (size_t e = 0; e < f.leaves.size(); ++e)
+      fl[e] = flk[Forest::key(f.leaves[e].level, f.leaves[e].ix, f.leaves[e].iy, f.leaves[e].iz)];
+    // undo the coarsenings that left a parent next to a leaf two levels finer
+    bool undone = false;
+    for (int K : too_coarse(f)) {
+      const auto &L = f.leaves[K];
+      auto it = parent.find(Forest::key(L.level, L.ix, L.iy, L.iz));
+      if (it != parent.end() && it->second) {
+        it->second = 0;
+        undone = true;
+      }
+    }
+    if (!undone) return;
+  }
+}
+
+// Refinement of the leaves flagged 1, then of every leaf more than one level
+// coarser than a neighbour, until the forest is 2:1 balanced.
+int refine_balanced(Forest &f, std::vector<char> &fl, std::string &err)
+{
   while (true) {
     bool any = false;
     std::vector<Forest::Leaf> nl;
     nl.reserve(f.leaves.size() + 8 * std::count(fl.begin(), fl.end(), 1));
     for (size_t e = 0; e < f.leaves.size(); ++e) {
       const Forest::Leaf L = f.leaves[e];
-      if (!fl[e]) { nl.push_back(L); continue; }
+      if (fl[e] != 1) { nl.push_back(L); continue; }
       if (L.level >= Forest::LMAX) { err = "refinement deeper than the lattice allows"; return -1; }
       any = true;
       for (int c = 0; c < 8; ++c)
@@ -109,26 +201,23 @@ int forest_refine(Forest &f, const std::vector<char> &flag, std::string &err)
     // 2:1 balance: a neighbour (26 directions) more than one level coarser
     // than a leaf is refined in the next pass
     fl.assign(f.leaves.size(), 0);
-    for (const auto &L : f.leaves) {
-      if (L.level < 2) continue;
-      const int n[3] = {f.nx << L.level, f.ny << L.level, f.nz << L.level};
-      for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-          for (int dx = -1; dx <= 1; ++dx) {
-            if (!dx && !dy && !dz) continue;
-            int q[3] = {L.ix + dx, L.iy + dy, L.iz + dz};
-            bool out = false;
-            for (int d = 0; d < 3; ++d) {
-              if (f.periodic(d)) q[d] = (q[d] + n[d]) % n[d];  // across the periodic faces
-              else if (q[d] < 0 || q[d] >= n[d]) out = true;
-            }
-            if (out) continue;
-            const int K = f.locate(L.level, q[0], q[1], q[2]);
-            if (K >= 0 && f.leaves[K].level < L.level - 1) fl[K] = 1;
-          }
-    }
+    for (int K : too_coarse(f)) fl[K] = 1;
   }
   return 0;
+}
+
+}  // namespace
+
+int forest_refine(Forest &f, const std::vector<char> &flag, std::string &err)
+{
+  if (flag.size() != f.leaves.size()) { err = "one refinement flag per cell expected"; return -1; }
+  for (char c : flag)
+    if (c != 0 && c != 1 && c != 2) { err = "flags are 0 (keep), 1 (refine) or 2 (coarsen)"; return -1; }
+  // refine_and_coarsen: the coarsening of complete families first, then the
+  // refinement with 2:1 balance
+  std::vector<char> fl(flag);
+  coarsen_families(f, fl);
+  return refine_balanced(f, fl, err);
 }
 
 // Lagrange basis of degree deg (1 or 2) on the equidistant nodes of [0,1]

@@ -48,9 +48,11 @@ struct Forest {
 
 int forest_init(Forest &f, int nx, int ny, int nz, const double lo[3], const double hi[3], const int side_id[6],
                 std::string &err);
-// Refine the leaves whose flag is nonzero, then refine further until
-// neighbouring leaves (across faces, edges and vertices) differ by at most
-// one level.
+// refine_and_coarsen: flag 2 marks a leaf for coarsening, 1 for refinement.
+// Complete families of 8 leaves all flagged 2 become their parent where the
+// 2:1 balance allows it; then the leaves flagged 1 are refined, and refined
+// further until neighbouring leaves (across faces, edges and vertices)
+// differ by at most one level.
 int forest_refine(Forest &f, const std::vector<char> &flag, std::string &err);
 // Q1 or Q2 (deg) mesh of the leaves (cell e = leaf e).  Hanging nodes get closed
 // constraint lines in m.hang_*, the flag HANG_BIT in node_mask and no
@@ -75,5 +77,9 @@ int forest_transfer(const Forest &fo, const Mesh &mo, const double *Uo, const Fo
 int kelly_estimate(const Forest &f, const Mesh &m, const double *U, float *eta, std::string &err);
 // Flags for the floor(frac * n) cells of largest indicator (ties in cell order).
 void flag_fixed_number(int n, const float *eta, double frac, std::vector<char> &flag);
+// refine_and_coarsen_fixed_number: 1 for the floor(refine * n) cells of
+// largest indicator, 2 for the floor(coarsen * n) of smallest (ties in cell
+// order), 0 otherwise
+void flag_refine_coarsen(int n, const float *eta, double refine, double coarsen, std::vector<char> &flag);
 
 }  // namespace glh

@@ -1203,6 +1203,17 @@ int glh_flag_fixed_number(int64_t n, const float *eta, double frac, uint8_t *fla
   return GLH_OK;
 }
 
+int glh_flag_refine_coarsen(int64_t n, const float *eta, double refine_frac, double coarsen_frac, uint8_t *flags)
+{
+  if (n < 0 || (n > 0 && (!eta || !flags)) || !(refine_frac >= 0.0 && refine_frac <= 1.0) ||
+      !(coarsen_frac >= 0.0 && coarsen_frac <= 1.0) || refine_frac + coarsen_frac > 1.0)
+    return GLH_ERR_ARG;
+  std::vector<char> fl;
+  glh::flag_refine_coarsen((int)n, eta, refine_frac, coarsen_frac, fl);
+  for (int64_t i = 0; i < n; ++i) flags[i] = (uint8_t)fl[i];
+  return GLH_OK;
+}
+
 int glh_mesh_constraint_sizes(const glh_mesh *mm, int64_t sizes[3])
 {
   if (!mm || !sizes) return GLH_ERR_ARG;

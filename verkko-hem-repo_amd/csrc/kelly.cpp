@@ -128,6 +128,19 @@ int kelly_estimate(const Forest &f, const Mesh &m, const double *U, float *eta, 
   return 0;
 }
 
+void flag_refine_coarsen(int n, const float *eta, double refine, double coarsen, std::vector<char> &flag)
+{
+  flag.assign(n, 0);
+  const int kr = std::max(0, std::min(n, (int)std::floor(refine * n)));
+  const int kc = std::max(0, std::min(n - kr, (int)std::floor(coarsen * n)));
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return eta[a] > eta[b]; });
+  for (int i = 0; i < kr; ++i) flag[ord[i]] = 1;
+  // smallest indicators: the end of the same order (ties in reverse cell order)
+  for (int i = 0; i < kc; ++i) flag[ord[n - 1 - i]] = 2;
+}
+
 void flag_fixed_number(int n, const float *eta, double frac, std::vector<char> &flag)
 {
   flag.assign(n, 0);

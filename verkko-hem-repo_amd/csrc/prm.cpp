@@ -225,6 +225,8 @@ int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int 
   const int scc = p->b(P, "trun on Strong Coupling Correction");
   const int n_cycles = (int)p->i(C, "Number of refinements"), n_iter = (int)p->i(C, "Number of interations");
   const double conv = p->d(C, "converge accuracy"), frac = p->d(C, "adaptive refinment ratio");
+  const double cfrac = p->d(C, "adaptive coarsen ratio");
+  if (frac + cfrac > 1.0) return GLH_ERR_ARG;
   const int maxit = (int)std::max(1L, p->i(C, "maximum linear iteration number"));
   const int damped = p->b(C, "Using dampped Newton iteration");
   const double step = p->d(C, "primary step length of dampped newton iteration");
@@ -236,7 +238,7 @@ int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int 
   const std::string geo = p->s(X, "geometry"), guess = p->s(X, "initial guess");
   if ((geo != "retangle" && geo != "cube") || (guess != "B" && guess != "A" && guess != "BnA" && guess != "BinA" && guess != "BnB" && guess != "PDW"))
     return GLH_ERR_ARG;
-  if (n_glob > 6 || n_cycles > 8) return GLH_ERR_ARG;
+  if (n_glob < 0 || n_glob > 6 || n_cycles < 0 || n_cycles > 8) return GLH_ERR_ARG;
   double thr[4], tol[5];
   bool glob[5] = {false, false, false, false, false};
   for (int k = 0; k < 4; ++k) thr[k] = p->d(C, ("Cycle " + std::to_string(k) + " refinement threshold").c_str());
@@ -263,10 +265,26 @@ int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int 
   const double K[3] = {0.42072, 0.42072, 0.42072};  // femgl.h:320-322
   int rc = glh_set_material(ctx, mp[0], mp + 1, K, bt);
   if (rc) return rc;
-  const int nroot = 1 << n_glob;
+  // The reference refines one coarse cell n_glob times, so every initial cell
+  // has a parent that the coarsen ratio can restore. Here the forest starts
+  // from 2 roots per axis (a periodic axis needs two) refined n_glob - 1
+  // times: the same cells, coarsenable down to the roots.
+  const int nroot = n_glob > 0 ? 2 : 1;
   glh_forest *f = nullptr;
   if ((rc = glh_forest_create(nroot, nroot, nroot, lo, hi, sid, &f)) != GLH_OK) return rc;
   if ((rc = glh_forest_set_degree(f, deg)) != GLH_OK) {
+    glh_forest_destroy(f);
+    return rc;
+  }
+  for (int g = 1; g < n_glob && rc == GLH_OK; ++g) {
+    std::vector<uint8_t> all(glh_forest_n_cells(f), 1);
+    glh_forest *fn = nullptr;
+    if ((rc = glh_forest_refine(f, all.data(), &fn)) == GLH_OK) {
+      glh_forest_destroy(f);
+      f = fn;
+    }
+  }
+  if (rc) {
     glh_forest_destroy(f);
     return rc;
   }
@@ -280,7 +298,7 @@ int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int 
       if (!glob[std::min(cycle, 4)]) {
         std::vector<float> eta(nc);
         if ((rc = glh_kelly(f, m, U.data(), eta.data())) ||
-            (rc = glh_flag_fixed_number(nc, eta.data(), frac, fl.data())))
+            (rc = glh_flag_refine_coarsen(nc, eta.data(), frac, cfrac, fl.data())))
           break;
       }
       glh_forest *fn = nullptr;

@@ -37,7 +37,8 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_mlp_predict", "glh_instance_residuals", "glh_set_forcing", "glh_set_residual_reference",
            "glh_set_schedule", "glh_autotune", "glh_prm_parse", "glh_prm_destroy", "glh_prm_get",
            "glh_prm_get_double", "glh_run_prm", "glh_write_vtu", "glh_initial_guess_field",
-           "glh_transfer_solution", "glh_forest_set_degree", "glh_bench_read"]
+           "glh_transfer_solution", "glh_forest_set_degree", "glh_bench_read",
+           "glh_flag_refine_coarsen"]
 
 _lib = None
 
@@ -131,6 +132,7 @@ def lib():
     L.glh_transfer_solution.argtypes = [vp, vp, vp, vp, vp]
     L.glh_forest_set_degree.argtypes = [vp, C.c_int]
     L.glh_bench_read.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+    L.glh_flag_refine_coarsen.argtypes = [C.c_int64, _f32p, C.c_double, C.c_double, _u8p]
     L.glh_initial_guess_field.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_uint64, _dp]
     _lib = L
     return L
@@ -431,6 +433,17 @@ def flag_fixed_number(eta, frac):
     rc = lib().glh_flag_fixed_number(eta.size, eta, float(frac), fl)
     if rc:
         raise GLHError("glh_flag_fixed_number failed (%d)" % rc)
+    return fl
+
+
+def flag_refine_coarsen(eta, refine, coarsen):
+    """1 (refine) for the top fraction, 2 (coarsen) for the bottom fraction
+    (glh_flag_refine_coarsen)."""
+    eta = np.ascontiguousarray(eta, np.float32)
+    fl = np.zeros(eta.size, np.uint8)
+    rc = lib().glh_flag_refine_coarsen(eta.size, eta, float(refine), float(coarsen), fl)
+    if rc:
+        raise GLHError("glh_flag_refine_coarsen failed (%d)" % rc)
     return fl
 
 
