@@ -339,14 +339,9 @@ def run_warm_start(G, args):
     net = G.MLP(n_freq=64, sigma=args.sigma, width=256, depth=5, n_out=18, seed=3)
     net.set_data(X, Y)
     t0 = time.perf_counter()
-    curve = []
-    done = 0
-    while done < args.train_steps:
-        k = min(2000, args.train_steps - done)
-        L = net.train(k, batch=args.train_batch, lr=1e-3, seed=3, loss_every=k)
-        done += k
-        curve.append(float(L[-1]))
-        log("training step %d loss %.3e (%.1f s)" % (done, curve[-1], time.perf_counter() - t0))
+    curve = WS.train(net, args.train_steps, args.train_batch, lr=1e-3, segment=1000, seed=3,
+                     log=lambda done, L, rate: log("training step %d loss %.3e lr %.2e (%.1f s)"
+                                                   % (done, L, rate, time.perf_counter() - t0)))
     t_train = time.perf_counter() - t0
     # 3. held-out instances
     test = WS.slab_instances(G, args.instances, 4)

@@ -395,7 +395,8 @@ int glh_timing(glh_ctx *ctx, double out[6]);
 int glh_bench_spmv(glh_ctx *ctx, int k, double *ms_avg);
 
 /* Average time of a streaming read of nbytes (multiple of 16) over k
- * launches: the device's sustained HBM read rate, the practical ceiling of
+ * launches (non-temporal 16-byte loads, 8 in flight per thread, 1024
+ * workgroups): the device's sustained HBM read rate, the practical ceiling of
  * the read-dominated SpMV. */
 int glh_bench_read(glh_ctx *ctx, size_t nbytes, int k, double *ms_avg);
 /* Run k launches of a plain 16-byte-vector device copy of nbytes (a multiple

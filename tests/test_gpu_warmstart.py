@@ -37,10 +37,17 @@ def trained():
         Ys.append(Y)
         say("training batch", b0, int(ist[:, 1].sum()))
     net = G.MLP(n_freq=64, sigma=0.5, width=256, depth=5, n_out=18, seed=3)
-    net.set_data(np.concatenate(Xs), np.concatenate(Ys))
-    L = net.train(6000, batch=32768, lr=1e-3, seed=3, loss_every=2000)
-    say("loss", L)
-    assert L[-1] < L[0]
+    X, Y = np.concatenate(Xs), np.concatenate(Ys)
+    net.set_data(X, Y)
+    # the mean squared error over a fixed sample of the training data (the
+    # sampled mini-batch losses of Adam fluctuate from step to step)
+    sub = np.random.default_rng(3).choice(X.shape[0], min(32768, X.shape[0]), replace=False)
+    mse = lambda: float(np.mean((net.predict(X[sub]) - Y[sub]) ** 2))
+    e0 = mse()
+    L = WS.train(net, 6000, 32768, lr=1e-3, segment=500, seed=3)
+    e1 = mse()
+    say("loss", L, "sample MSE", e0, "->", e1)
+    assert e1 < 0.1 * e0
     yield net
     net.close()
 

@@ -7,7 +7,8 @@
 //
 // Layout: vectors are node-major (dof = node*18 + c); the matrix is BSR with
 // row-major 18x18 blocks of 324 contiguous values.  All kernels are
-// HBM-bound; the SpMV streams each block once with 16-byte loads.
+// HBM-bound; the SpMV streams each block once, every load instruction of a
+// wavefront covering one contiguous span of the block.
 #include "glhip_dev.hpp"
 
 namespace glh {
@@ -17,15 +18,76 @@ template <typename T> struct V2t;
 template <> struct V2t<double> { using type = double2; };
 template <> struct V2t<float> { using type = float2; };
 
-// 6 consecutive values starting at a 16-byte (double) / 8-byte (float) boundary
+// Pair of values at a 16-byte (double) / 8-byte (float) boundary.  Matrix
+// values are read once per product with non-temporal loads (streamed past the
+// caches, which keep x); x is read with ordinary loads.
+template <typename T> struct NV2;
+template <> struct NV2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct NV2<float> { typedef float type __attribute__((ext_vector_type(2))); };
+
 template <typename T>
-__device__ inline void load6(const T *__restrict__ p, double *o)
+__device__ inline void pair_nt(const T *__restrict__ p, double &a, double &b)
 {
-  using V2 = typename V2t<T>::type;
-  const V2 *q = reinterpret_cast<const V2 *>(p);
-  const V2 a = q[0], b = q[1], c = q[2];
-  o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y; o[4] = c.x; o[5] = c.y;
+  const typename NV2<T>::type v = __builtin_nontemporal_load(reinterpret_cast<const typename NV2<T>::type *>(p));
+  a = v.x;
+  b = v.y;
 }
+
+template <typename T>
+__device__ inline void pair_ld(const T *__restrict__ p, double &a, double &b)
+{
+  const typename V2t<T>::type v = *reinterpret_cast<const typename V2t<T>::type *>(p);
+  a = v.x;
+  b = v.y;
+}
+
+// Block-row layout of the SpMV and the block-Jacobi apply: one wavefront per
+// block row.  The 324 row-major values of a block are read as 162 consecutive
+// pairs; lane l < 54 takes pairs k*54 + l (k < 3), i.e. row 6k + l/9 at
+// columns 2(l%9) and 2(l%9)+1.  Each load instruction of the wave so covers 54
+// consecutive pairs (864 B of FP64), and a lane needs one pair of x per block.
+// The lane's three partial row sums are combined through LDS at the end of
+// the row: row r = 6k + m is the sum over the 9 lanes 9m..9m+8 of their
+// k-th partial sum, in lane order.
+struct RowAcc {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  template <typename TJ>
+  __device__ inline void add_block(const TJ *__restrict__ blk, int lane, double w0, double w1)
+  {
+    double p[6];
+    pair_nt(blk + 2 * lane, p[0], p[1]);
+    pair_nt(blk + 2 * (54 + lane), p[2], p[3]);
+    pair_nt(blk + 2 * (108 + lane), p[4], p[5]);
+    a0 += p[0] * w0;
+    a0 += p[1] * w1;
+    a1 += p[2] * w0;
+    a1 += p[3] * w1;
+    a2 += p[4] * w0;
+    a2 += p[5] * w1;
+  }
+  // lanes < 18 return the sum of row `lane`; red holds 162 doubles of this wave
+  __device__ inline double row_sum(int lane, double *red) const
+  {
+    if (lane < 54) {
+      red[lane] = a0;
+      red[54 + lane] = a1;
+      red[108 + lane] = a2;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double t = 0.0;
+    if (lane < 18) {
+      const double *q = red + (lane / 6) * 54 + 9 * (lane % 6);
+#pragma unroll
+      for (int j = 0; j < 9; ++j) t += q[j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return t;
+  }
+};
 
 __device__ inline int work_count(const ActiveSet &a, int n_chunks) { return a.counts ? a.counts[1] : n_chunks; }
 __device__ inline int work_chunk(const ActiveSet &a, int i) { return a.counts ? a.chunk[i] : i; }
@@ -54,41 +116,32 @@ __device__ inline int xcd_block(int b, int n_blocks)
   return x * q + min(x, r) + (b >> 3);
 }
 
-// One wavefront per block row.  Lane = g*18 + c (g < 3): row c of every
-// block, columns 6g..6g+5.  Partial sums of the three column groups are
-// combined with cross-lane reads at the end of the row.
+// Sum of row `lane` (lanes < 18) of block row `row` times x; two blocks per
+// step keep six pair loads of the matrix in flight per lane.
 template <typename TJ, typename TX>
-__device__ inline void spmv_row(int row, int lane, const int *__restrict__ rp, const int *__restrict__ col,
-                                const TJ *__restrict__ val, const TX *__restrict__ x, TX *__restrict__ y)
+__device__ inline double spmv_row(int row, int lane, const int *__restrict__ rp, const int *__restrict__ col,
+                                  const TJ *__restrict__ val, const TX *__restrict__ x, double *red)
 {
-  const int g = lane / 18, c = lane % 18;
-  double acc = 0.0;
+  RowAcc acc;
   const int b0 = rp[row], b1 = rp[row + 1];
   if (lane < 54) {
+    const int cp = 2 * (lane % 9);
     int b = b0;
     for (; b + 1 < b1; b += 2) {
       const int s0 = col[b], s1 = col[b + 1];
-      double B0[6], B1[6], X0[6], X1[6];
-      load6(val + (size_t)b * 324 + c * 18 + 6 * g, B0);
-      load6(val + (size_t)(b + 1) * 324 + c * 18 + 6 * g, B1);
-      load6(x + (size_t)s0 * 18 + 6 * g, X0);
-      load6(x + (size_t)s1 * 18 + 6 * g, X1);
-#pragma unroll
-      for (int j = 0; j < 6; ++j) acc += B0[j] * X0[j];
-#pragma unroll
-      for (int j = 0; j < 6; ++j) acc += B1[j] * X1[j];
+      double w0, w1, z0, z1;
+      pair_ld(x + (size_t)s0 * 18 + cp, w0, w1);
+      pair_ld(x + (size_t)s1 * 18 + cp, z0, z1);
+      acc.add_block(val + (size_t)b * 324, lane, w0, w1);
+      acc.add_block(val + (size_t)(b + 1) * 324, lane, z0, z1);
     }
     if (b < b1) {
-      double B0[6], X0[6];
-      load6(val + (size_t)b * 324 + c * 18 + 6 * g, B0);
-      load6(x + (size_t)col[b] * 18 + 6 * g, X0);
-#pragma unroll
-      for (int j = 0; j < 6; ++j) acc += B0[j] * X0[j];
+      double w0, w1;
+      pair_ld(x + (size_t)col[b] * 18 + cp, w0, w1);
+      acc.add_block(val + (size_t)b * 324, lane, w0, w1);
     }
   }
-  const double a1 = __shfl(acc, (c + 18) & 63);
-  const double a2 = __shfl(acc, (c + 36) & 63);
-  if (lane < 18) y[(size_t)row * 18 + c] = (TX)(acc + a1 + a2);
+  return acc.row_sum(lane, red);
 }
 
 // Without an active set: one row per wave, grid = rows / 4.  With one:
@@ -100,19 +153,22 @@ __global__ __launch_bounds__(256) void k_spmv(int n_rows, const int *__restrict_
                                               TX *__restrict__ y, const int *__restrict__ row_seg,
                                               const int *__restrict__ seg_flag, ActiveSet act)
 {
-  const int lane = threadIdx.x & 63;
+  __shared__ double red[4][162];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (!act.counts) {
-    const int row = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    const int row = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
     if (row >= n_rows) return;
     if (seg_flag && !seg_flag[row_seg[row]]) return;  // instance not iterating
-    spmv_row(row, lane, rp, col, val, x, y);
+    const double t = spmv_row(row, lane, rp, col, val, x, red[wv]);
+    if (lane < 18) y[(size_t)row * 18 + lane] = (TX)t;
     return;
   }
   const int total = act.counts[2];
-  for (int gi = blockIdx.x * 4 + (threadIdx.x >> 6); gi < total; gi += gridDim.x * 4) {
+  for (int gi = blockIdx.x * 4 + wv; gi < total; gi += gridDim.x * 4) {
     const int row = active_row(act, gi);
     if (seg_flag && !seg_flag[row_seg[row]]) continue;
-    spmv_row(row, lane, rp, col, val, x, y);
+    const double t = spmv_row(row, lane, rp, col, val, x, red[wv]);
+    if (lane < 18) y[(size_t)row * 18 + lane] = (TX)t;
   }
 }
 
@@ -123,22 +179,12 @@ __global__ __launch_bounds__(256) void k_res_spmv(int n_rows, const int *__restr
                                                   const TJ *__restrict__ val, const double *__restrict__ x,
                                                   const double *__restrict__ bv, double *__restrict__ r)
 {
-  const int lane = threadIdx.x & 63;
-  const int row = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  __shared__ double red[4][162];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int row = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
   if (row >= n_rows) return;
-  const int g = lane / 18, c = lane % 18;
-  double acc = 0.0;
-  if (lane < 54)
-    for (int b = rp[row]; b < rp[row + 1]; ++b) {
-      double B0[6], X0[6];
-      load6(val + (size_t)b * 324 + c * 18 + 6 * g, B0);
-      load6(x + (size_t)col[b] * 18 + 6 * g, X0);
-#pragma unroll
-      for (int j = 0; j < 6; ++j) acc += B0[j] * X0[j];
-    }
-  const double a1 = __shfl(acc, (c + 18) & 63);
-  const double a2 = __shfl(acc, (c + 36) & 63);
-  if (lane < 18) r[(size_t)row * 18 + c] = bv[(size_t)row * 18 + c] - (acc + a1 + a2);
+  const double t = spmv_row(row, lane, rp, col, val, x, red[wv]);
+  if (lane < 18) r[(size_t)row * 18 + lane] = bv[(size_t)row * 18 + lane] - t;
 }
 
 // Gauss-Jordan inverse of one 18x18 diagonal block per workgroup (64 threads,
@@ -197,20 +243,16 @@ __global__ __launch_bounds__(64) void k_bj_invert(int n_rows, const int *__restr
 
 template <typename TJ, typename TX>
 __device__ inline void bj_row(int row, int lane, const TJ *__restrict__ Dinv, const TX *__restrict__ x,
-                              TX *__restrict__ y)
+                              TX *__restrict__ y, double *red)
 {
-  const int g = lane / 18, c = lane % 18;
-  double acc = 0.0;
+  RowAcc acc;
   if (lane < 54) {
-    double B0[6], X0[6];
-    load6(Dinv + (size_t)row * 324 + c * 18 + 6 * g, B0);
-    load6(x + (size_t)row * 18 + 6 * g, X0);
-#pragma unroll
-    for (int j = 0; j < 6; ++j) acc += B0[j] * X0[j];
+    double w0, w1;
+    pair_ld(x + (size_t)row * 18 + 2 * (lane % 9), w0, w1);
+    acc.add_block(Dinv + (size_t)row * 324, lane, w0, w1);
   }
-  const double a1 = __shfl(acc, (c + 18) & 63);
-  const double a2 = __shfl(acc, (c + 36) & 63);
-  if (lane < 18) y[(size_t)row * 18 + c] = (TX)(acc + a1 + a2);
+  const double t = acc.row_sum(lane, red);
+  if (lane < 18) y[(size_t)row * 18 + lane] = (TX)t;
 }
 
 template <typename TJ, typename TX>
@@ -218,19 +260,20 @@ __global__ __launch_bounds__(256) void k_bj_apply(int n_rows, const TJ *__restri
                                                   TX *__restrict__ y, const int *__restrict__ row_seg,
                                                   const int *__restrict__ seg_flag, ActiveSet act)
 {
-  const int lane = threadIdx.x & 63;
+  __shared__ double red[4][162];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (!act.counts) {
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int row = blockIdx.x * 4 + wv;
     if (row >= n_rows) return;
     if (seg_flag && !seg_flag[row_seg[row]]) return;
-    bj_row(row, lane, Dinv, x, y);
+    bj_row(row, lane, Dinv, x, y, red[wv]);
     return;
   }
   const int total = act.counts[2];
-  for (int gi = blockIdx.x * 4 + (threadIdx.x >> 6); gi < total; gi += gridDim.x * 4) {
+  for (int gi = blockIdx.x * 4 + wv; gi < total; gi += gridDim.x * 4) {
     const int row = active_row(act, gi);
     if (seg_flag && !seg_flag[row_seg[row]]) continue;
-    bj_row(row, lane, Dinv, x, y);
+    bj_row(row, lane, Dinv, x, y, red[wv]);
   }
 }
 
@@ -410,20 +453,24 @@ __global__ __launch_bounds__(RT) void k_copy16(size_t n16, const float4 *__restr
 }
 
 // streaming read of n16 16-byte words: every workgroup reads contiguous
-// blocks of RT * RD words, each thread RD independent loads in flight; one
-// float per thread is written so that the loads stay live
+// blocks of RT * RD words, each thread RD independent non-temporal loads in
+// flight (the fastest form measured on the device: 7.0 TB/s with 1024
+// workgroups, against 5.9 TB/s with ordinary loads); one float per thread is
+// written so that the loads stay live
 constexpr int RD = 8;
 __global__ __launch_bounds__(RT) void k_read16(size_t n16, const float4 *__restrict__ src, float *__restrict__ out)
 {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4 *in = reinterpret_cast<const f4 *>(src);
   float acc = 0.f;
   const size_t blk = (size_t)RT * RD, nblk = (n16 + blk - 1) / blk;
   for (size_t b = blockIdx.x; b < nblk; b += gridDim.x) {
     const size_t base = b * blk + threadIdx.x;
-    float4 v[RD];
+    f4 v[RD];
 #pragma unroll
     for (int r = 0; r < RD; ++r) {
       const size_t idx = base + (size_t)r * RT;
-      v[r] = idx < n16 ? src[idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+      v[r] = idx < n16 ? __builtin_nontemporal_load(in + idx) : f4{0.f, 0.f, 0.f, 0.f};
     }
 #pragma unroll
     for (int r = 0; r < RD; ++r) acc += (v[r].x + v[r].y) + (v[r].z + v[r].w);

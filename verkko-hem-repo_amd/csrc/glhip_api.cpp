@@ -1937,7 +1937,7 @@ int glh_bench_read(glh_ctx *c, size_t nbytes, int k, double *ms_avg)
 {
   if (!c || !ms_avg || k < 1 || nbytes % 16 || nbytes == 0) return GLH_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->device));
-  constexpr int grid = 4096;
+  constexpr int grid = 1024;
   DBuf<unsigned char> a;
   DBuf<float> out;
   hipError_t e = a.alloc(nbytes);

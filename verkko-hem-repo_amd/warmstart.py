@@ -147,6 +147,25 @@ def training_set(insts, Us, converged):
     return np.concatenate(Xs), np.concatenate(Ys)
 
 
+def train(net, steps, batch, lr=1e-3, lr_end=1e-5, segment=500, seed=3, log=None):
+    """Adam on the GPU with a cosine learning-rate decay from lr to lr_end,
+    held constant over segments of `segment` steps (the step counter, the
+    moments and the batch sampler carry over between segments).  A constant
+    rate leaves the final weights wherever the last fluctuation of the
+    mini-batch loss put them; the decay settles them.  Returns the last
+    mini-batch loss of every segment."""
+    done, curve = 0, []
+    while done < steps:
+        k = min(segment, steps - done)
+        rate = lr_end + 0.5 * (lr - lr_end) * (1.0 + np.cos(np.pi * (done + 0.5 * k) / steps))
+        L = net.train(k, batch=batch, lr=rate, seed=seed, loss_every=k)
+        done += k
+        curve.append(float(L[-1]))
+        if log:
+            log(done, curve[-1], rate)
+    return curve
+
+
 def predict_guesses(net, insts):
     X = np.concatenate([features(it) for it in insts])
     Y = net.predict(X)
