@@ -389,7 +389,7 @@ def run_warm_start(G, args):
         "warmup": 0, "ms_per_step": 1000.0 * t_warm32, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 MLP + f32/f64 solve", "data": "synthetic",
         "config": {"workload": "learned warm start: coordinate MLP (64 Fourier frequencies, 5 x 256 SiLU, 18 "
-                               "outputs, BF16 with FP32 accumulation, Adam lr 1e-3, %d steps, batch %d) trained on "
+                               "outputs, BF16 with FP32 accumulation, Adam with cosine lr decay 1e-3 -> 1e-5, %d steps, batch %d) trained on "
                                "%d slab solutions (seed 3); %d held-out instances (seed 4), batched FP32 "
                                "Newton-GMRES, B-phase guess as fallback" % (args.train_steps, args.train_batch,
                                                                             n_conv, len(test)),
