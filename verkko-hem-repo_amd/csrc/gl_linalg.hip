@@ -494,6 +494,22 @@ __global__ void k_set_scalar(T *p, T v)
 // workgroups).  Per-chunk partials are reduced per instance in fixed order
 // (deterministic).
 
+// Chunk bounds are even (an instance holds 18 values per node and chunks are
+// 1024 values long), so the Gram-Schmidt kernels below read pairs of values:
+// the basis vectors with non-temporal loads (a cycle's basis is far larger
+// than the caches and is streamed), w with ordinary loads (every group of
+// basis vectors reads it again).
+template <typename T>
+__device__ inline typename NV2<T>::type ld2nt(const T *__restrict__ p)
+{
+  return __builtin_nontemporal_load(reinterpret_cast<const typename NV2<T>::type *>(p));
+}
+template <typename T>
+__device__ inline typename NV2<T>::type ld2(const T *__restrict__ p)
+{
+  return *reinterpret_cast<const typename NV2<T>::type *>(p);
+}
+
 // 2D grid: x = chunks (or the active list), y = groups of KCH vectors.
 template <typename T>
 __global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__restrict__ ch, int n_chunks,
@@ -513,11 +529,15 @@ __global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__rest
     T acc[KCH];
 #pragma unroll
     for (int j = 0; j < KCH; ++j) acc[j] = 0;
-    for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
-      const T x = w[idx];
+    for (long long idx = beg + 2 * threadIdx.x; idx < end; idx += 2 * RT) {
+      const auto x = ld2(w + idx);
 #pragma unroll
       for (int j = 0; j < KCH; ++j)
-        if (j < kk) acc[j] += V[(size_t)(i0 + j) * ldv + idx] * x;
+        if (j < kk) {
+          const auto v = ld2nt(V + (size_t)(i0 + j) * ldv + idx);
+          acc[j] += v.x * x.x;
+          acc[j] += v.y * x.y;
+        }
     }
 #pragma unroll
     for (int j = 0; j < KCH; ++j) {
@@ -572,11 +592,34 @@ __global__ __launch_bounds__(RT) void k_seg_sub_lincomb_norm(const long long *__
     if (flag && !flag[sg]) continue;
     const T *hs = h + sg * ldh;
     T acc = 0;
-    for (long long idx = beg + threadIdx.x; idx < end; idx += RT) {
-      T s = w[idx];
-      for (int i = 0; i < k; ++i) s -= hs[i] * V[(size_t)i * ldv + idx];
-      w[idx] = s;
-      acc += s * s;
+    for (long long idx = beg + 2 * threadIdx.x; idx < end; idx += 2 * RT) {
+      const auto w2 = ld2(w + idx);
+      T s0 = w2.x, s1 = w2.y;
+      const T *vp = V + idx;
+      int i = 0;
+      for (; i + 4 <= k; i += 4) {  // four basis vectors in flight
+        const auto v0 = ld2nt(vp + (size_t)i * ldv), v1 = ld2nt(vp + (size_t)(i + 1) * ldv);
+        const auto v2 = ld2nt(vp + (size_t)(i + 2) * ldv), v3 = ld2nt(vp + (size_t)(i + 3) * ldv);
+        s0 -= hs[i] * v0.x;
+        s1 -= hs[i] * v0.y;
+        s0 -= hs[i + 1] * v1.x;
+        s1 -= hs[i + 1] * v1.y;
+        s0 -= hs[i + 2] * v2.x;
+        s1 -= hs[i + 2] * v2.y;
+        s0 -= hs[i + 3] * v3.x;
+        s1 -= hs[i + 3] * v3.y;
+      }
+      for (; i < k; ++i) {
+        const auto v = ld2nt(vp + (size_t)i * ldv);
+        s0 -= hs[i] * v.x;
+        s1 -= hs[i] * v.y;
+      }
+      typename NV2<T>::type o;
+      o.x = s0;
+      o.y = s1;
+      *reinterpret_cast<typename NV2<T>::type *>(w + idx) = o;
+      acc += s0 * s0;
+      acc += s1 * s1;
     }
     const T r = block_reduce(acc, sh);
     if (threadIdx.x == 0) partial[cb] = r;

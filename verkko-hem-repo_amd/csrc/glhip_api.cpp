@@ -84,6 +84,8 @@ struct HBuf {
 };
 
 constexpr long long CHUNK = 1024;  // dofs per reduction chunk (one workgroup each)
+// the Gram-Schmidt kernels read value pairs: chunk bounds must stay even
+static_assert(CHUNK % 2 == 0 && NC % 2 == 0, "reduction chunks must have even bounds");
 
 }  // namespace
 
