@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void k_bj_apply(int n_rows, const TJ *__restri
 
 // -------------------------------------------------------------- reductions
 constexpr int RT = 256;
-constexpr int KCH = 8;  // vectors per multi-dot pass
+constexpr int KCH = 8;  // vectors per multi-dot pass (4 measured the same, 16 slower)
 
 // Sum over the 64 lanes of a wave (butterfly: every lane gets the same,
 // order-fixed result).
@@ -597,17 +597,15 @@ __global__ __launch_bounds__(RT) void k_seg_sub_lincomb_norm(const long long *__
       T s0 = w2.x, s1 = w2.y;
       const T *vp = V + idx;
       int i = 0;
-      for (; i + 4 <= k; i += 4) {  // four basis vectors in flight
-        const auto v0 = ld2nt(vp + (size_t)i * ldv), v1 = ld2nt(vp + (size_t)(i + 1) * ldv);
-        const auto v2 = ld2nt(vp + (size_t)(i + 2) * ldv), v3 = ld2nt(vp + (size_t)(i + 3) * ldv);
-        s0 -= hs[i] * v0.x;
-        s1 -= hs[i] * v0.y;
-        s0 -= hs[i + 1] * v1.x;
-        s1 -= hs[i + 1] * v1.y;
-        s0 -= hs[i + 2] * v2.x;
-        s1 -= hs[i + 2] * v2.y;
-        s0 -= hs[i + 3] * v3.x;
-        s1 -= hs[i + 3] * v3.y;
+      for (; i + 8 <= k; i += 8) {  // eight basis vectors in flight
+        typename NV2<T>::type v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = ld2nt(vp + (size_t)(i + u) * ldv);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          s0 -= hs[i + u] * v[u].x;
+          s1 -= hs[i + u] * v[u].y;
+        }
       }
       for (; i < k; ++i) {
         const auto v = ld2nt(vp + (size_t)i * ldv);
