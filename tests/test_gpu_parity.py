@@ -127,6 +127,17 @@ def test_gmres_no_convergence_is_reported(ctx):
         ctx.solve(rtol=1e-14, maxit=2, restart=30)
 
 
+@pytest.mark.parametrize("bad", [dict(lin_tol=0.0), dict(lin_maxit=0), dict(max_newton=-1), dict(rtol=-1.0),
+                                 dict(ls_step=1.0), dict(ls_step=0.0)])
+def test_newton_solve_rejects_bad_controls_before_any_work(ctx, bad):
+    mesh = cfg_mesh(G, "cube_q1_small")
+    _setup(ctx, mesh, 22.0, 0.8, 0.05, 2)
+    U = ctx.solution()
+    with pytest.raises(G.GLHError, match="bad Newton controls"):
+        ctx.newton_solve(**bad)
+    assert np.array_equal(ctx.solution(), U)
+
+
 def test_newton_step_line_search_matches_oracle_semantics(ctx):
     mesh = cfg_mesh(G, "cube_q1_small")
     om, U, co = _setup(ctx, mesh, 22.0, 0.8, 0.05, 2)

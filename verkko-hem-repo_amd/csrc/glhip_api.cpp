@@ -1198,7 +1198,7 @@ int glh_kelly(const glh_forest *f, const glh_mesh *m, const double *U, float *et
 
 int glh_flag_fixed_number(int64_t n, const float *eta, double frac, uint8_t *flags)
 {
-  if (n < 0 || (n > 0 && (!eta || !flags)) || !(frac >= 0.0 && frac <= 1.0)) return GLH_ERR_ARG;
+  if (n < 0 || n > INT32_MAX || (n > 0 && (!eta || !flags)) || !(frac >= 0.0 && frac <= 1.0)) return GLH_ERR_ARG;
   std::vector<char> fl;
   glh::flag_fixed_number((int)n, eta, frac, fl);
   for (int64_t i = 0; i < n; ++i) flags[i] = (uint8_t)fl[i];
@@ -1207,7 +1207,7 @@ int glh_flag_fixed_number(int64_t n, const float *eta, double frac, uint8_t *fla
 
 int glh_flag_refine_coarsen(int64_t n, const float *eta, double refine_frac, double coarsen_frac, uint8_t *flags)
 {
-  if (n < 0 || (n > 0 && (!eta || !flags)) || !(refine_frac >= 0.0 && refine_frac <= 1.0) ||
+  if (n < 0 || n > INT32_MAX || (n > 0 && (!eta || !flags)) || !(refine_frac >= 0.0 && refine_frac <= 1.0) ||
       !(coarsen_frac >= 0.0 && coarsen_frac <= 1.0) || refine_frac + coarsen_frac > 1.0)
     return GLH_ERR_ARG;
   std::vector<char> fl;
@@ -1608,6 +1608,9 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
                      double ls_step, int damped, int fp32, double stats[10])
 {
   RCHK(check_ready(c));
+  if (!(rtol >= 0) || max_newton < 0 || !(lin_tol > 0) || lin_maxit < 1 || (damped && !(ls_step > 0 && ls_step < 1)))
+    return fail(c, GLH_ERR_ARG, "bad Newton controls (rtol >= 0, max_newton >= 0, lin_tol > 0, lin_maxit >= 1, "
+                                "line-search step in (0,1))");
   if (restart <= 0) restart = c->tuned_restart;
   // the instance filter of the assembly lives only inside this solve
   struct FilterReset {
