@@ -463,6 +463,8 @@ def main():
                     help="slab_batch: pseudo-transient continuation sigma0 for the batched and the one-at-a-time "
                          "solves (0: plain damped Newton)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--matrix-free", action="store_true",
+                    help="headline solve with the matrix-free Jacobian (glh_set_matrix_free)")
     ap.add_argument("--kernels", action="store_true",
                     help="short fixed kernel sequence for rocprofv3: 3 FP64 and 3 FP32 J+R assemblies, 3 residual "
                          "assemblies, 10 FP64 and 10 FP32 SpMVs, 5 copies of 1 GiB (counter calibration)")
@@ -510,6 +512,7 @@ def main():
         ctx.close()
         return
     kw = dict(rtol=1e-10, max_newton=100, lin_tol=args.lin_tol, lin_maxit=100000, restart=args.restart)
+    ctx.set_matrix_free(args.matrix_free)
     for w in range(args.warmup):
         ctx.restore_initial()
         st = ctx.newton_solve(**kw)

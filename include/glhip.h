@@ -43,6 +43,7 @@ typedef struct glh_forest glh_forest;
 #define GLH_ASM_J 1           /* Jacobian (assemble.cc:177-280) */
 #define GLH_ASM_R 2           /* right-hand side -R (assemble.cc:257-276) */
 #define GLH_ASM_FP32 4        /* Jacobian stored and solved in FP32 */
+#define GLH_ASM_MATRIX_FREE 8 /* Jacobian in matrix-free form (see glh_set_matrix_free) */
 
 /* ---------------------------------------------------------------- context */
 /* FemGL<dim> constructor state (femgl.cc:107-203): device + stream. */
@@ -292,11 +293,18 @@ int glh_get_rhs(glh_ctx *ctx, double *rhs_host);                /* system_rhs (=
 /* system_matrix values in BSR order [nnzb][18][18] (FP64; an FP32 Jacobian
  * is widened).  For parity dumps of the assembled system. */
 int glh_get_jacobian(glh_ctx *ctx, double *val_host);
+/* y = J x with the Jacobian of the last assembly or Newton step, assembled
+ * or matrix-free (FP64 vectors of n_dofs on the host).  A Jacobian-vector
+ * product for parity checks of either form. */
+int glh_apply_jacobian(glh_ctx *ctx, const double *x_host, double *y_host);
 
 /* ---------------------------------------------------------- Newton step */
 /* FemGL::assemble_system (assemble.cc:108-372): J and/or system_rhs = -R at
  * the current solution, with constrained rows condensed as
- * AffineConstraints::distribute_local_to_global does. flags: GLH_ASM_*. */
+ * AffineConstraints::distribute_local_to_global does. flags: GLH_ASM_*;
+ * with GLH_ASM_MATRIX_FREE the Jacobian is kept in the matrix-free form of
+ * glh_set_matrix_free (FP64, one instance, no constraints) instead of BSR
+ * values. */
 int glh_assemble(glh_ctx *ctx, int flags);
 /* ||system_rhs|| after glh_assemble with GLH_ASM_R. */
 int glh_rhs_norm(glh_ctx *ctx, double *norm);
@@ -382,6 +390,16 @@ int glh_set_residual_reference(glh_ctx *ctx, int n, const double *r_ref);
  * gradient flow of F far from a minimum and is plain Newton near it.  The
  * fixed point is unchanged.  0 (default) = off. */
 int glh_set_continuation(glh_ctx *ctx, double sigma0);
+/* Matrix-free Jacobian in glh_newton_solve (a GPU alternative to the
+ * assembled Trilinos matrix behind SolverFGMRES, solve.cc:159-174): on != 0
+ * makes each Newton step store the bulk Hessian of every cell at its
+ * quadrature points and only the 18x18 diagonal blocks, and GMRES applies
+ * J x cell by cell from them (the same operator as the assembled BSR matrix,
+ * to rounding).  It applies to FP64 solves of one instance on meshes without
+ * hanging-node or periodic constraints and without continuation; other solves
+ * assemble J as before.  glh_assemble always assembles the BSR values;
+ * glh_get_jacobian fails after a matrix-free step.  0 (default) = off. */
+int glh_set_matrix_free(glh_ctx *ctx, int on);
 int glh_set_verbose(glh_ctx *ctx, int level);
 /* Live kernel timing with HIP events on the context stream.  on != 0 resets
  * the counters and records an event pair around every SpMV launch and every

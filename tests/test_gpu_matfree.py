@@ -1,0 +1,129 @@
+"""The matrix-free Jacobian (glh_set_matrix_free, GLH_ASM_MATRIX_FREE) against
+the assembled BSR Jacobian and the oracle's: J x to 1e-12 relative (normwise,
+max|diff| / max|ref|) on Q1 and Q2 box meshes with Dirichlet (id 0), AdGR
+(ids 2-4) and Robin walls, x random over every DoF so that the Dirichlet
+columns and rows are exercised; a Newton solve reaches the assembled path's
+fixed point (free energy 1e-10, order parameter 1e-8); meshes with
+constraints keep the assembled Jacobian."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from product_util import adaptive_forest, glhip, initial_state, oracle_mesh
+
+pytestmark = pytest.mark.gpu
+G = glhip()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = G.Context(0)
+    yield c
+    c.close()
+
+
+def _bsr_mv(om, val, x):
+    x = x.reshape(-1, 18)
+    y = np.zeros_like(x)
+    for r in range(len(om.row_ptr) - 1):
+        for b in range(om.row_ptr[r], om.row_ptr[r + 1]):
+            y[r] += val[b] @ x[om.col[b]]
+    return y.reshape(-1)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+CASES = [
+    # name, degree, cells, lo, hi, boundary ids, p, T/Tc, bt
+    ("q1_dirichlet_z", 1, (3, 3, 3), (-3.0, -3.0, -3.0), (3.0, 3.0, 3.0), [1, 1, 1, 1, 0, 0], 22.0, 0.8, 1.0e10),
+    ("q2_dirichlet_z", 2, (3, 2, 2), (-6.0, -4.0, -4.0), (6.0, 4.0, 4.0), [1, 1, 1, 1, 0, 0], 26.0, 0.7, 1.0e10),
+    ("q1_adgr_robin", 1, (2, 2, 1), (-2.0, -2.0, -1.0), (2.0, 2.0, 1.0), [2, 2, 3, 3, 4, 4], 22.0, 0.8, 2.0),
+    ("q2_adgr_robin", 2, (2, 2, 2), (-3.0, -3.0, -3.0), (3.0, 3.0, 3.0), [2, 2, 3, 0, 4, 1], 20.0, 0.6, 1.5),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_matrix_free_product_equals_the_assembled_and_the_oracle(ctx, case):
+    name, deg, n, lo, hi, sides, p, t, bt = case
+    say("case", name)
+    mesh = G.Mesh(deg, n, lo, hi, sides)
+    ctx.set_material_pt(p, t, bt=bt)
+    ctx.upload_mesh(mesh)
+    U = initial_state(mesh.n_nodes, p, t, 0.05, 3)
+    ctx.set_solution(U)
+    om = oracle_mesh(mesh)
+    co = O.coef(p, t, bt=bt)
+    Jo, _ = O.assemble(om, co, O.distribute(om, U), R=False)
+    rng = np.random.default_rng(deg)
+    for k in range(2):
+        x = rng.standard_normal(mesh.n_nodes * 18)
+        yo = _bsr_mv(om, Jo, x)
+        ctx.assemble(J=True, R=False)
+        ya = ctx.apply_jacobian(x)
+        ctx.assemble(J=True, R=False, matrix_free=True)
+        ym = ctx.apply_jacobian(x)
+        say(" ", k, "assembled", _rel(ya, yo), "matrix-free", _rel(ym, yo), "mf vs assembled", _rel(ym, ya))
+        assert _rel(ya, yo) <= 1e-12
+        assert _rel(ym, yo) <= 1e-12
+    # the matrix-free form holds no BSR values
+    with pytest.raises(G.GLHError, match="matrix-free"):
+        ctx.jacobian()
+    with pytest.raises(G.GLHError, match="FP64"):
+        ctx.assemble(J=True, R=False, fp32=True, matrix_free=True)
+
+
+def test_matrix_free_newton_reaches_the_assembled_fixed_point(ctx):
+    mesh = G.Mesh(2, (4, 4, 4), (-20.0, -20.0, -20.0), (20.0, 20.0, 20.0), [1, 1, 1, 1, 0, 0])
+    p, t = 26.0, 0.7
+    ctx.set_material_pt(p, t)
+    ctx.upload_mesh(mesh)
+    U0 = initial_state(mesh.n_nodes, p, t, 0.05, 0)
+    kw = dict(rtol=1e-10, max_newton=50, lin_tol=0.1, lin_maxit=20000, restart=50)
+    ctx.set_solution(U0)
+    st_a = ctx.newton_solve(**kw)
+    Ua = ctx.solution()
+    ctx.set_solution(U0)
+    ctx.set_matrix_free(True)
+    try:
+        st_m = ctx.newton_solve(**kw)
+    finally:
+        ctx.set_matrix_free(False)
+    Um = ctx.solution()
+    say("assembled", st_a)
+    say("matrix-free", st_m)
+    assert st_a["converged"] == 1.0 and st_m["converged"] == 1.0
+    assert abs(st_m["newton_steps"] - st_a["newton_steps"]) <= 1
+    assert st_m["F"] == pytest.approx(st_a["F"], rel=1e-10)
+    assert np.abs(Um - Ua).max() <= 1e-8 * np.abs(Ua).max()
+    om = oracle_mesh(mesh)
+    assert O.free_energy(om, O.coef(p, t), Um) == pytest.approx(st_m["F"], rel=1e-10)
+    # the last Jacobian was matrix-free
+    with pytest.raises(G.GLHError, match="matrix-free"):
+        ctx.jacobian()
+
+
+def test_meshes_with_constraints_keep_the_assembled_jacobian(ctx):
+    lo, hi, sides = (-5.0, -5.0, -6.0), (5.0, 5.0, 6.0), [1, 1, 1, 1, 0, 0]
+    f = adaptive_forest(G, (2, 2, 2), lo, hi, sides, 1, 0, frac=0.3)
+    m = f.mesh()
+    assert m.constraints()["n_hanging"] > 0
+    p, t = 22.0, 0.8
+    ctx.set_material_pt(p, t)
+    ctx.upload_mesh(m)
+    ctx.set_solution(initial_state(m.n_nodes, p, t, 0.05, 1))
+    ctx.set_matrix_free(True)
+    try:
+        st = ctx.newton_solve(rtol=1e-8, max_newton=30, lin_tol=0.1, restart=50)
+    finally:
+        ctx.set_matrix_free(False)
+    assert st["converged"] == 1.0
+    assert ctx.jacobian().shape[1:] == (18, 18)  # assembled values are there

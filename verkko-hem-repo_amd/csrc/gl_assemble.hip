@@ -378,6 +378,210 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
   }
 }
 
+// ------------------------------------------------ matrix-free Jacobian
+// J x without the assembled matrix, for meshes without constraints.  The
+// Jacobian of the cell loop is
+//   J_(a,c),(b,d) = vol sum_q w_q N_a N_b H_cd(q)             (bulk, H symmetric)
+//                 + K1 delta_cd vol sum_q w_q grad N_a . grad N_b
+//                 + K23 [c, d same part and spin] vol sum_q w_q d_kc N_a d_kd N_b
+//                 + Robin face mass on the diagonal components,
+// with kc, kd the orbital indices of c, d.  Per cell this is applied as
+//   X_q = sum_b N_b(q) x_b, G_kq = sum_b d_k N_b(q) x_b / h_k,
+//   y_a += sum_q w_q vol (N_a(q) H(q) X_q + sum_k d_k N_a(q) / h_k gamma_k(q)),
+//   gamma_{c,k} = K1 G_kq[c] + K23 [k == kc] sum_kd G_kd,q[9 part + 3 mu + kd],
+// about 1.1e5 multiply-adds per Q2 cell instead of streaming the cell's
+// 729 blocks.  k_mf_setup stores H_cd(q) per cell (the only state of the
+// operator besides the diagonal blocks) and assembles the diagonal blocks
+// for the point-block Jacobi preconditioner with the arithmetic of
+// k_assemble; Dirichlet columns are applied as a zero x there, Dirichlet
+// rows keep only their diagonal (k_mf_rows).
+
+// index of the component pair (c <= d) in the order of the owner threads
+__device__ inline int upair(int c, int d)
+{
+  const int lo = c < d ? c : d, hi = c < d ? d : c;
+  return lo * 18 - lo * (lo - 1) / 2 + (hi - lo);
+}
+
+template <int DEG>
+__global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ U,
+                                                 const int *__restrict__ cells, double *__restrict__ Hq,
+                                                 double *__restrict__ Jd)
+{
+  constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS;
+  __shared__ double sU[NS * 18];
+  __shared__ double sX[NQ * 18];
+  __shared__ double sGram[NQ * GRAM];
+  const int tid = threadIdx.x;
+  const int e = cells[blockIdx.x];
+  const Material M = Mh.mats ? Mh.mats[Mh.cell_mat[e]] : M0;
+  const int *cn = Mh.cell_nodes + (size_t)e * NS;
+  double h[3], ih[3];
+  for (int k = 0; k < 3; ++k) { h[k] = Mh.cell_h[3 * e + k]; ih[k] = 1.0 / h[k]; }
+  const double vol = h[0] * h[1] * h[2];
+  const uint32_t cface = Mh.cell_face[e];
+  for (int i = tid; i < NS * 18; i += NT) sU[i] = U[(size_t)cn[i / 18] * 18 + (i % 18)];
+  __syncthreads();
+  for (int i = tid; i < NQ * 18; i += NT) {
+    const int q = i / 18, c = i % 18;
+    double xv = 0;
+    for (int a = 0; a < NS; ++a) xv += T.N[q * NS + a] * sU[a * 18 + c];
+    sX[i] = xv;
+  }
+  __syncthreads();
+  for (int i = tid; i < NQ * GRAM; i += NT) {
+    const int q = i / GRAM, g = i % GRAM;
+    sGram[i] = gram_entry(sX + q * 18, g);
+  }
+  __syncthreads();
+  if (tid >= NUPAIR) return;
+  int c = 0, d = 0;
+  {
+    int t = tid;
+    while (t >= 18 - c) { t -= 18 - c; ++c; }
+    d = c + t;
+  }
+  double Hv[NQ];
+  double *Hc = Hq + (size_t)e * NQ * NUPAIR;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    Hv[q] = hess_entry(sGram + q * GRAM, sX + q * 18, c, d, M);
+    Hc[q * NUPAIR + tid] = Hv[q];
+  }
+  // diagonal blocks (a,a), as k_assemble forms and masks them
+  const bool grad_cd = (c / 9 == d / 9) && ((c % 9) / 3 == (d % 9) / 3);
+  const int kc = c % 3, kd = d % 3;
+  const double lap0 = vol * ih[0] * ih[0], lap1 = vol * ih[1] * ih[1], lap2 = vol * ih[2] * ih[2];
+  const double k23_cd = M.K23 * vol * ih[kc] * ih[kd];
+  const int o_cd = c * 18 + d, o_dc = d * 18 + c;
+  const int *cblk = Mh.cell_blk + (size_t)e * NS * NS;
+  for (int a = 0; a < NS; ++a) {
+    const int p = a * NS - a * (a - 1) / 2;  // pair (a, a)
+    const double *wp = T.Wp + (size_t)p * NQ, *sp = T.Sp + (size_t)p * 9;
+    double sym = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) sym += wp[q] * Hv[q];
+    sym *= vol;
+    if (c == d) {
+      sym += M.K1 * (sp[0] * lap0 + sp[4] * lap1 + sp[8] * lap2);
+      if (M.robin != 0.0)
+        for (int lf = 0; lf < 6; ++lf) {
+          const int bid = (int)((cface >> (3 * lf)) & 7u) - 1;
+          if (bid >= 2 && bid <= 4 && !face_zeroed(bid, c))
+            sym += (M.robin * face_area(lf, h)) * T.Mf[(lf * NS + a) * NS + a];
+        }
+    }
+    double v1 = sym, v2 = sym;
+    if (grad_cd) {
+      v1 += k23_cd * sp[kc * 3 + kd];
+      v2 += k23_cd * sp[kd * 3 + kc];
+    }
+    const uint32_t ma = Mh.node_mask[cn[a]];
+    const bool r1 = (ma >> c) & 1u, c1 = (ma >> d) & 1u, r2 = (ma >> d) & 1u, c2 = (ma >> c) & 1u;
+    if (r1 || c1) v1 = (r1 && c == d) ? fabs(v1) : 0.0;
+    if (r2 || c2) v2 = (r2 && c == d) ? fabs(v2) : 0.0;
+    const bool first = cblk[a * NS + a] < 0;  // this cell writes the node's diagonal block first
+    double *B = Jd + (size_t)cn[a] * 324;
+    B[o_cd] = (first ? 0.0 : B[o_cd]) + v1;
+    if (c != d) B[o_dc] = (first ? 0.0 : B[o_dc]) + v2;
+  }
+}
+
+constexpr int NTA = 256;  // threads per workgroup of the matrix-free product
+
+template <int DEG>
+__global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ Hq,
+                                                  const double *__restrict__ x, const int *__restrict__ cells,
+                                                  double *__restrict__ y)
+{
+  constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS;
+  __shared__ double sx[NS * 18];   // x at the cell's nodes, Dirichlet components 0
+  __shared__ double sX[NQ * 18];   // x at the quadrature points
+  __shared__ double sG[NQ * 54];   // [q][k][c] d_k x / h_k
+  __shared__ double sZ[NQ * 18];   // w vol H X
+  __shared__ double sK[NQ * 54];   // [q][k][c] w vol gamma_{c,k} / h_k
+  const int tid = threadIdx.x;
+  const int e = cells[blockIdx.x];
+  const Material M = Mh.mats ? Mh.mats[Mh.cell_mat[e]] : M0;
+  const int *cn = Mh.cell_nodes + (size_t)e * NS;
+  double h[3], ih[3];
+  for (int k = 0; k < 3; ++k) { h[k] = Mh.cell_h[3 * e + k]; ih[k] = 1.0 / h[k]; }
+  const double vol = h[0] * h[1] * h[2];
+  const uint32_t cface = Mh.cell_face[e];
+  for (int i = tid; i < NS * 18; i += NTA) {
+    const int node = cn[i / 18], c = i % 18;
+    sx[i] = ((Mh.node_mask[node] >> c) & 1u) ? 0.0 : x[(size_t)node * 18 + c];
+  }
+  __syncthreads();
+  for (int i = tid; i < NQ * 18; i += NTA) {
+    const int q = i / 18, c = i % 18;
+    double xv = 0, g0 = 0, g1 = 0, g2 = 0;
+    for (int b = 0; b < NS; ++b) {
+      const double xb = sx[b * 18 + c];
+      const double *dn = T.dN + (q * NS + b) * 3;
+      xv += T.N[q * NS + b] * xb;
+      g0 += dn[0] * xb;
+      g1 += dn[1] * xb;
+      g2 += dn[2] * xb;
+    }
+    sX[i] = xv;
+    sG[q * 54 + c] = g0 * ih[0];
+    sG[q * 54 + 18 + c] = g1 * ih[1];
+    sG[q * 54 + 36 + c] = g2 * ih[2];
+  }
+  __syncthreads();
+  const double *Hc = Hq + (size_t)e * NQ * NUPAIR;
+  for (int i = tid; i < NQ * 18; i += NTA) {
+    const int q = i / 18, c = i % 18;
+    const double *Hrow = Hc + q * NUPAIR;
+    const double *X = sX + q * 18;
+    double z = 0.0;
+#pragma unroll
+    for (int d = 0; d < 18; ++d) z += Hrow[upair(c, d)] * X[d];
+    const double wv = T.w[q] * vol;
+    sZ[i] = wv * z;
+    const int part = c / 9, mu = (c % 9) / 3, kc = c % 3;
+    const double *g = sG + q * 54;
+    const double dv = g[9 * part + 3 * mu + 0] + g[18 + 9 * part + 3 * mu + 1] + g[36 + 9 * part + 3 * mu + 2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      sK[q * 54 + 18 * k + c] = wv * ih[k] * (M.K1 * g[18 * k + c] + (k == kc ? M.K23 * dv : 0.0));
+  }
+  __syncthreads();
+  for (int i = tid; i < NS * 18; i += NTA) {
+    const int a = i / 18, c = i % 18;
+    const int node = cn[a];
+    if ((Mh.node_mask[node] >> c) & 1u) continue;  // Dirichlet row: k_mf_rows
+    double r = 0.0;
+    for (int q = 0; q < NQ; ++q) {
+      const double *dn = T.dN + (q * NS + a) * 3;
+      const double *K = sK + q * 54;
+      r += T.N[q * NS + a] * sZ[q * 18 + c] + dn[0] * K[c] + dn[1] * K[18 + c] + dn[2] * K[36 + c];
+    }
+    if (M.robin != 0.0)
+      for (int lf = 0; lf < 6; ++lf) {
+        const int bid = (int)((cface >> (3 * lf)) & 7u) - 1;
+        if (bid < 2 || bid > 4 || face_zeroed(bid, c)) continue;
+        const double *mf = T.Mf + (lf * NS + a) * NS;
+        double sf = 0.0;
+        for (int b = 0; b < NS; ++b) sf += mf[b] * sx[b * 18 + c];
+        r += (M.robin * face_area(lf, h)) * sf;
+      }
+    y[(size_t)node * 18 + c] += r;
+  }
+}
+
+// Dirichlet rows of J x: their diagonal entry (the sum over cells of
+// |local diagonal|) times x
+__global__ void k_mf_rows(int n_nodes, const uint32_t *__restrict__ mask, const double *__restrict__ Jd,
+                          const double *__restrict__ x, double *__restrict__ y)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)n_nodes * 18) return;
+  const int node = (int)(i / 18), c = (int)(i % 18);
+  if ((mask[node] >> c) & 1u) y[i] = Jd[(size_t)node * 324 + c * 19] * x[i];
+}
+
 template <int DEG>
 __global__ __launch_bounds__(NT) void k_energy(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ U,
                                                int n_cells, double *__restrict__ Fcell)
@@ -506,6 +710,33 @@ hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefT
     if (do_r) GLH_ASM(2, double, false, true, false);
   }
 #undef GLH_ASM
+  return hipGetLastError();
+}
+
+hipError_t launch_mf_setup(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
+                           const int *cells, int n, double *Hq, double *Jd, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  if (deg == 1) hipLaunchKernelGGL(k_mf_setup<1>, dim3(n), dim3(NT), 0, s, T, M, mat, U, cells, Hq, Jd);
+  else hipLaunchKernelGGL(k_mf_setup<2>, dim3(n), dim3(NT), 0, s, T, M, mat, U, cells, Hq, Jd);
+  return hipGetLastError();
+}
+
+hipError_t launch_mf_apply(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *Hq,
+                           const double *x, const int *cells, int n, double *y, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  if (deg == 1) hipLaunchKernelGGL(k_mf_apply<1>, dim3(n), dim3(NTA), 0, s, T, M, mat, Hq, x, cells, y);
+  else hipLaunchKernelGGL(k_mf_apply<2>, dim3(n), dim3(NTA), 0, s, T, M, mat, Hq, x, cells, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_mf_rows(int n_nodes, const uint32_t *mask, const double *Jd, const double *x, double *y,
+                          hipStream_t s)
+{
+  if (n_nodes <= 0) return hipSuccess;
+  const long long n = (long long)n_nodes * 18;
+  hipLaunchKernelGGL(k_mf_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_nodes, mask, Jd, x, y);
   return hipGetLastError();
 }
 

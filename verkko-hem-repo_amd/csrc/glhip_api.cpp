@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "glhip.h"
@@ -131,6 +132,12 @@ struct glh_ctx {
   DBuf<double> J64, D64;
   DBuf<float> J32, D32;
   bool j_fp32 = false, j_valid = false, d_valid = false;
+  // matrix-free Jacobian (glh_set_matrix_free): per-cell bulk Hessian at the
+  // quadrature points and the diagonal blocks, in place of the BSR values
+  bool matfree = false, j_mf = false;
+  DBuf<double> Hq64, Jd64;
+  DBuf<int> diag_id;  // node -> its block in Jd64 (the identity)
+  int diag_id_n = 0;
   DBuf<int> status;
   // GMRES workspace
   DBuf<double> gV64, gZ64, gW64, gh64, gp64, gy64, gs64, gq64;
@@ -351,10 +358,22 @@ int norm64(glh_ctx *c, const double *v, double *out)
 }
 
 // assemble (J and/or -R) at state Ustate into (J, rhs_out)
-int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp32, double *rhs_out, bool timed)
+int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp32, double *rhs_out, bool timed,
+                bool mf = false)
 {
   void *J = nullptr;
-  if (do_j) {
+  if (do_j && mf) {  // matrix-free: H at the quadrature points and the diagonal blocks
+    if (fp32 || c->constrained || c->n_seg != 1) return fail(c, GLH_ERR_STATE, "matrix-free Jacobian: FP64, one instance, no constraints");
+    HIPCHK(c, c->Hq64.alloc((size_t)c->n_cells * c->nq * 171));
+    HIPCHK(c, c->Jd64.alloc((size_t)c->n_nodes * 324));
+    if (c->diag_id_n != c->n_nodes) {
+      std::vector<int> id(c->n_nodes);
+      for (int i = 0; i < c->n_nodes; ++i) id[i] = i;
+      HIPCHK(c, upload(c->diag_id, id.data(), id.size(), c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));  // the host vector goes out of scope
+      c->diag_id_n = c->n_nodes;
+    }
+  } else if (do_j) {
     const size_t nv = (size_t)c->nnzb * 324;
     if (fp32) { HIPCHK(c, c->J32.alloc(nv)); J = c->J32.p; }
     else { HIPCHK(c, c->J64.alloc(nv)); J = c->J64.p; }
@@ -379,8 +398,12 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
   const int ncol = (int)c->color_ptr.size() - 1;
   for (int k = 0; k < ncol; ++k) {
     const int off = c->color_ptr[k], n = c->color_ptr[k + 1] - off;
-    HIPCHK(c, glh::launch_assemble(c->deg, do_j, do_r, fp32, T, M, c->mat, Ustate, c->color_cells.p + off, n,
-                                   Jcell, rcell, c->stream));
+    if (do_j && mf)
+      HIPCHK(c, glh::launch_mf_setup(c->deg, T, M, c->mat, Ustate, c->color_cells.p + off, n, c->Hq64.p, c->Jd64.p,
+                                     c->stream));
+    if (do_r || (do_j && !mf))
+      HIPCHK(c, glh::launch_assemble(c->deg, do_j && !mf, do_r, fp32, T, M, c->mat, Ustate, c->color_cells.p + off,
+                                     n, Jcell, rcell, c->stream));
   }
   if (c->constrained) {
     if (do_j) {
@@ -403,6 +426,7 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
   }
   if (do_j) {
     c->j_fp32 = fp32;
+    c->j_mf = mf;
     c->j_valid = true;
     c->d_valid = false;
   }
@@ -429,7 +453,10 @@ int ensure_precond(glh_ctx *c)
     HIPCHK(c, glh::launch_bj_invert<float>(c->n_nodes, c->diag.p, c->J32.p, c->D32.p, c->status.p, c->stream));
   } else {
     HIPCHK(c, c->D64.alloc(nv));
-    HIPCHK(c, glh::launch_bj_invert<double>(c->n_nodes, c->diag.p, c->J64.p, c->D64.p, c->status.p, c->stream));
+    if (c->j_mf)
+      HIPCHK(c, glh::launch_bj_invert<double>(c->n_nodes, c->diag_id.p, c->Jd64.p, c->D64.p, c->status.p, c->stream));
+    else
+      HIPCHK(c, glh::launch_bj_invert<double>(c->n_nodes, c->diag.p, c->J64.p, c->D64.p, c->status.p, c->stream));
   }
   int st = 0;
   HIPCHK(c, hipMemcpyAsync(&st, c->status.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -483,6 +510,29 @@ int flush_events(glh_ctx *c)
 }
 
 template <typename TJ, typename TV>
+int apply_j(glh_ctx *c, const TJ *J, const TV *x, TV *y, const int *flag, glh::ActiveSet act)
+{
+  if constexpr (std::is_same<TJ, double>::value && std::is_same<TV, double>::value) {
+    if (c->j_mf) {  // one instance, every row iterating: flag and act are not needed
+      const glh::RefTables T = c->tables();
+      const glh::MeshDev M = c->meshdev();
+      HIPCHK(c, glh::launch_fill<double>(ndofs(c), 0.0, y, c->stream));
+      const int ncol = (int)c->color_ptr.size() - 1;
+      for (int k = 0; k < ncol; ++k) {
+        const int off = c->color_ptr[k], n = c->color_ptr[k + 1] - off;
+        HIPCHK(c, glh::launch_mf_apply(c->deg, T, M, c->mat, c->Hq64.p, x, c->color_cells.p + off, n, y,
+                                       c->stream));
+      }
+      HIPCHK(c, glh::launch_mf_rows(c->n_nodes, c->node_mask.p, c->Jd64.p, x, y, c->stream));
+      return GLH_OK;
+    }
+  }
+  HIPCHK(c, glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, x, y, c->stream, c->row_seg.p, flag,
+                                     act));
+  return GLH_OK;
+}
+
+template <typename TJ, typename TV>
 int spmv(glh_ctx *c, const TJ *J, const TV *x, TV *y, const int *flag = nullptr, glh::ActiveSet act = {})
 {
   const bool t = c->timing && !c->ev_pool.empty();
@@ -490,8 +540,7 @@ int spmv(glh_ctx *c, const TJ *J, const TV *x, TV *y, const int *flag = nullptr,
     if (2 * c->ev_used >= (int)c->ev_pool.size()) RCHK(flush_events(c));
     HIPCHK(c, hipEventRecord(c->ev_pool[2 * c->ev_used], c->stream));
   }
-  HIPCHK(c, glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, x, y, c->stream, c->row_seg.p, flag,
-                                     act));
+  RCHK((apply_j<TJ, TV>(c, J, x, y, flag, act)));
   if (t) {
     HIPCHK(c, hipEventRecord(c->ev_pool[2 * c->ev_used + 1], c->stream));
     ++c->ev_used;
@@ -535,8 +584,8 @@ int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, i
   SA.act = act;
   auto mv = [&](const TV *in, TV *out, const int *flag, glh::ActiveSet a) -> int {
     if (timed) return spmv<TJ, TV>(c, J, in, out, flag, a);
-    HIPCHK(c, (glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, in, out, s, rs, flag, a)));
-    return GLH_OK;
+    (void)rs;
+    return apply_j<TJ, TV>(c, J, in, out, flag, a);
   };
   if (part == 0) {  // head: V0 = b - J x, beta_s, cycle flags
     RCHK(mv(x, W.p, nullptr, glh::ActiveSet{}));
@@ -659,7 +708,8 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
                                          c->gmjseg.p, c->gmits.p,    c->gmro.p,   c->gmfrozen.p, c->gmjlast.p,
                                          c->row_seg.p, c->d_node_off.p, c->act_counts.p, c->act_seg.p,
                                          c->act_rpref.p, c->act_chunk.p, c->chunks.p, c->seg_chunk.p,
-                                         c->row_ptr.p, c->col.p,
+                                         c->row_ptr.p, c->col.p, c->Hq64.p, c->Jd64.p,
+                                         (const void *)(size_t)c->j_mf,
                                          (const void *)(size_t)m, (const void *)(size_t)S,
                                          (const void *)(size_t)c->n_chunks, (const void *)(size_t)c->n_nodes,
                                          (const void *)(size_t)q, (const void *)(size_t)c->row_cap,
@@ -1507,10 +1557,34 @@ int glh_get_solution(glh_ctx *c, double *Uh) { return c ? get_vec(c, c->U.p, Uh)
 int glh_get_update(glh_ctx *c, double *d) { return c ? get_vec(c, c->dU.p, d) : GLH_ERR_ARG; }
 int glh_get_rhs(glh_ctx *c, double *r) { return c ? get_vec(c, c->rhs.p, r) : GLH_ERR_ARG; }
 
+int glh_apply_jacobian(glh_ctx *c, const double *x, double *y)
+{
+  RCHK(check_ready(c));
+  if (!x || !y) return fail(c, GLH_ERR_ARG, "null vector");
+  if (!c->j_valid) return fail(c, GLH_ERR_STATE, "no Jacobian assembled");
+  const size_t n = ndofs(c);
+  HIPCHK(c, c->gW64.alloc(n));
+  HIPCHK(c, c->gZ64.alloc(n));
+  HIPCHK(c, hipMemcpyAsync(c->gZ64.p, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (c->j_fp32) {
+    HIPCHK(c, c->gW32.alloc(n));
+    HIPCHK(c, c->gZ32.alloc(n));
+    HIPCHK(c, (glh::launch_convert<double, float>(n, c->gZ64.p, c->gZ32.p, c->stream)));
+    RCHK((apply_j<float, float>(c, c->J32.p, c->gZ32.p, c->gW32.p, nullptr, glh::ActiveSet{})));
+    HIPCHK(c, (glh::launch_convert<float, double>(n, c->gW32.p, c->gW64.p, c->stream)));
+  } else {
+    RCHK((apply_j<double, double>(c, c->J64.p, c->gZ64.p, c->gW64.p, nullptr, glh::ActiveSet{})));
+  }
+  HIPCHK(c, hipMemcpyAsync(y, c->gW64.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GLH_OK;
+}
+
 int glh_get_jacobian(glh_ctx *c, double *val)
 {
   RCHK(check_ready(c, false));
   if (!c->j_valid) return fail(c, GLH_ERR_STATE, "no Jacobian assembled");
+  if (c->j_mf) return fail(c, GLH_ERR_STATE, "the last Jacobian is matrix-free: no assembled values (glh_assemble)");
   const size_t nv = (size_t)c->nnzb * 324;
   if (!c->j_fp32) {
     HIPCHK(c, hipMemcpyAsync(val, c->J64.p, nv * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1528,8 +1602,10 @@ int glh_assemble(glh_ctx *c, int flags)
 {
   RCHK(check_ready(c));
   const bool dj = flags & GLH_ASM_J, dr = flags & GLH_ASM_R, f32 = flags & GLH_ASM_FP32;
+  const bool mf = flags & GLH_ASM_MATRIX_FREE;
   if (!dj && !dr) return fail(c, GLH_ERR_ARG, "nothing to assemble");
-  RCHK(assemble_at(c, c->U.p, dj, dr, f32, c->rhs.p, true));
+  if (mf && (!dj || f32)) return fail(c, GLH_ERR_ARG, "the matrix-free form is an FP64 Jacobian");
+  RCHK(assemble_at(c, c->U.p, dj, dr, f32, c->rhs.p, true, mf));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return GLH_OK;
 }
@@ -1633,6 +1709,9 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
   std::vector<char> active(S), stalled(S, 0);
   HIPCHK(c, c->s_flag.alloc(S));
   const bool f32 = fp32 != 0;
+  // the matrix-free operator where it applies (FP64, one instance, no
+  // constraints, no continuation shift); the assembled BSR Jacobian otherwise
+  const bool mf = c->matfree && !f32 && !c->constrained && S == 1 && !(c->ptc_sigma0 > 0);
   std::vector<int> flag(S);
   // batches only: an instance whose ||R|| has fallen by less than 10% over the
   // last STALL_WINDOW steps is stopped and reported as not converged, so that
@@ -1665,7 +1744,7 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
     HIPCHK(c, hipMemcpyAsync(c->asm_flag.p, flag.data(), S * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->s_flag.p, flag.data(), S * sizeof(int), hipMemcpyHostToDevice, c->stream));
     c->asm_active = S > 1 ? c->asm_flag.p : nullptr;
-    RCHK(assemble_at(c, c->U.p, true, true, f32, c->rhs.p, false));
+    RCHK(assemble_at(c, c->U.p, true, true, f32, c->rhs.p, false, mf));
     // pseudo-transient continuation: J + sigma_k |diag J|, sigma_k =
     // sigma0 ||R_k|| / ||R_0|| per instance (switched evolution relaxation)
     if (c->ptc_sigma0 > 0) {
@@ -1873,6 +1952,13 @@ int glh_set_residual_reference(glh_ctx *c, int n, const double *r_ref)
   return GLH_OK;
 }
 
+int glh_set_matrix_free(glh_ctx *c, int on)
+{
+  if (!c) return GLH_ERR_ARG;
+  c->matfree = on != 0;
+  return GLH_OK;
+}
+
 int glh_set_continuation(glh_ctx *c, double sigma0)
 {
   if (!c || !(sigma0 >= 0.0)) return GLH_ERR_ARG;
@@ -1927,8 +2013,7 @@ int glh_bench_spmv(glh_ctx *c, int k, double *ms_avg)
       HIPCHK(c, (glh::launch_spmv<float, float>(c->n_nodes, c->row_ptr.p, c->col.p, c->J32.p, c->gZ32.p,
                                                 c->gW32.p, c->stream)));
     else
-      HIPCHK(c, (glh::launch_spmv<double, double>(c->n_nodes, c->row_ptr.p, c->col.p, c->J64.p, c->U.p, c->res.p,
-                                                  c->stream)));
+      RCHK((apply_j<double, double>(c, c->J64.p, c->U.p, c->res.p, nullptr, glh::ActiveSet{})));
   }
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));

@@ -75,6 +75,17 @@ hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefT
                            const Material &mat, const double *U, const int *cells, int n, void *J, double *rhs,
                            hipStream_t s);
 // Per-cell free energy into Fcell[cell] for cells [0, n_cells).
+// Matrix-free Jacobian (meshes without constraints): per-cell bulk Hessian
+// at the quadrature points (Hq: [n_cells][NQ][171]) and the diagonal blocks
+// (Jd: [n_nodes][324], colour-ordered like the assembly); the product adds
+// J x over one colour's cells into y (zeroed before the first colour), then
+// launch_mf_rows writes the Dirichlet rows.
+hipError_t launch_mf_setup(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
+                           const int *cells, int n, double *Hq, double *Jd, hipStream_t s);
+hipError_t launch_mf_apply(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *Hq,
+                           const double *x, const int *cells, int n, double *y, hipStream_t s);
+hipError_t launch_mf_rows(int n_nodes, const uint32_t *mask, const double *Jd, const double *x, double *y,
+                          hipStream_t s);
 hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
                          int n_cells, double *Fcell, hipStream_t s);
 

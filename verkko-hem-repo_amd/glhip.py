@@ -12,7 +12,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libglhip.so")
 
-GLH_ASM_J, GLH_ASM_R, GLH_ASM_FP32 = 1, 2, 4
+GLH_ASM_J, GLH_ASM_R, GLH_ASM_FP32, GLH_ASM_MATRIX_FREE = 1, 2, 4, 8
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "bad state", -4: "no convergence", -5: "singular block",
           -6: "pressure out of range"}
 
@@ -32,7 +32,7 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_mesh_constraint_sizes", "glh_mesh_constraints", "glh_forest_create", "glh_forest_destroy",
            "glh_forest_n_cells", "glh_forest_refine", "glh_forest_mesh", "glh_forest_transfer", "glh_kelly",
            "glh_flag_fixed_number", "glh_initial_guess_bna", "glh_run_adaptive",
-           "glh_set_continuation", "glh_mlp_create", "glh_mlp_destroy", "glh_mlp_last_error", "glh_mlp_n_params",
+           "glh_set_continuation", "glh_set_matrix_free", "glh_apply_jacobian", "glh_mlp_create", "glh_mlp_destroy", "glh_mlp_last_error", "glh_mlp_n_params",
            "glh_mlp_get_params", "glh_mlp_set_params", "glh_mlp_set_data", "glh_mlp_train", "glh_mlp_loss_grad",
            "glh_mlp_predict", "glh_instance_residuals", "glh_set_forcing", "glh_set_residual_reference",
            "glh_set_schedule", "glh_autotune", "glh_prm_parse", "glh_prm_destroy", "glh_prm_get",
@@ -89,6 +89,8 @@ def lib():
     L.glh_set_timing.argtypes = [vp, C.c_int]
     L.glh_set_verbose.argtypes = [vp, C.c_int]
     L.glh_set_continuation.argtypes = [vp, C.c_double]
+    L.glh_set_matrix_free.argtypes = [vp, C.c_int]
+    L.glh_apply_jacobian.argtypes = [vp, _dp, _dp]
     L.glh_set_forcing.argtypes = [vp, C.c_int]
     L.glh_write_vtu.argtypes = [vp, _dp, vp, C.c_char_p, C.c_int]
     L.glh_prm_parse.argtypes = [C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_int]
@@ -529,11 +531,19 @@ class Context:
     def rhs(self):
         return self._get("glh_get_rhs", self.n_dofs)
 
+    def apply_jacobian(self, x):
+        """J x with the current Jacobian, assembled or matrix-free."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros_like(x)
+        self._chk(lib().glh_apply_jacobian(self.h, x, y), "glh_apply_jacobian")
+        return y
+
     def jacobian(self):
         return self._get("glh_get_jacobian", self.nnzb * 324).reshape(-1, 18, 18)
 
-    def assemble(self, J=True, R=True, fp32=False):
-        flags = (GLH_ASM_J if J else 0) | (GLH_ASM_R if R else 0) | (GLH_ASM_FP32 if fp32 else 0)
+    def assemble(self, J=True, R=True, fp32=False, matrix_free=False):
+        flags = ((GLH_ASM_J if J else 0) | (GLH_ASM_R if R else 0) | (GLH_ASM_FP32 if fp32 else 0) |
+                 (GLH_ASM_MATRIX_FREE if matrix_free else 0))
         self._chk(lib().glh_assemble(self.h, flags), "glh_assemble")
 
     def rhs_norm(self):
@@ -634,6 +644,11 @@ class Context:
     def set_continuation(self, sigma0):
         """Pseudo-transient continuation (J + sigma0 ||R||/||R0|| |diag J|); 0: off."""
         self._chk(lib().glh_set_continuation(self.h, float(sigma0)), "glh_set_continuation")
+
+    def set_matrix_free(self, on=True):
+        """Matrix-free Jacobian in newton_solve (FP64, one instance, no
+        constraints); glh_set_matrix_free."""
+        self._chk(lib().glh_set_matrix_free(self.h, int(bool(on))), "glh_set_matrix_free")
 
     def set_timing(self, on=True):
         self._chk(lib().glh_set_timing(self.h, int(on)), "glh_set_timing")
