@@ -30,6 +30,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "converged GL solves/hr @ ||F||<=1e-10 vs plain FP64; assembly GDoF/s, HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6  # MI355X dense FP64 (matrix and vector)
 
 WORKLOADS = {
     # name: (degree, cells per side, half side, p [bar], T/Tc)
@@ -51,6 +52,24 @@ def spmv_bytes(n_nodes, nnzb, val_bytes, vec_bytes):
     once, one column index per block, the row pointer, x read once and y
     written once."""
     return nnzb * 324 * val_bytes + nnzb * 4 + (n_nodes + 1) * 4 + 2 * n_nodes * 18 * vec_bytes
+
+
+def mf_flops(deg, n_cells):
+    """Algorithmic FP64 flops of one matrix-free product J x (k_mf_apply), per
+    cell: x and its three gradients at the NQ quadrature points (4 NS NQ
+    multiply-adds per component), the 18 x 18 bulk Hessian at each point
+    (18 NQ per component), the test against N and its gradients (4 NS NQ per
+    component); Robin faces left out."""
+    ns = (deg + 1) ** 3
+    return n_cells * 2 * 18 * (8 * ns * ns + 18 * ns)
+
+
+def mf_bytes(deg, n_cells, n_dofs):
+    """Algorithmic HBM bytes of one matrix-free product: the stored bulk
+    Hessian (171 values per quadrature point and cell) read once, x read once
+    and y written once."""
+    ns = (deg + 1) ** 3
+    return n_cells * ns * 171 * 8 + 2 * n_dofs * 8
 
 
 def cpu_baseline(deg, h, mat, n_cells, n_nodes, nnzb, newton_steps, gmres_its, ls_trials, restart, budget_s=25.0):
@@ -463,8 +482,10 @@ def main():
                     help="slab_batch: pseudo-transient continuation sigma0 for the batched and the one-at-a-time "
                          "solves (0: plain damped Newton)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--matrix-free", action="store_true",
-                    help="headline solve with the matrix-free Jacobian (glh_set_matrix_free)")
+    ap.add_argument("--assembled", action="store_true",
+                    help="headline solve with the assembled BSR Jacobian instead of the matrix-free one")
+    ap.add_argument("--no-assembled", action="store_true",
+                    help="skip the assembled-Jacobian solve reported beside the matrix-free headline")
     ap.add_argument("--kernels", action="store_true",
                     help="short fixed kernel sequence for rocprofv3: 3 FP64 and 3 FP32 J+R assemblies, 3 residual "
                          "assemblies, 10 FP64 and 10 FP32 SpMVs, 5 copies of 1 GiB (counter calibration)")
@@ -505,6 +526,10 @@ def main():
             print("bench: spmv fp32=%d %.3f ms" % (fp32, ctx.bench_spmv(10)), file=sys.stderr, flush=True)
         for _ in range(3):
             ctx.assemble(J=False, R=True)
+        for _ in range(3):
+            ctx.assemble(J=True, R=False, matrix_free=True)
+            print("bench: matrix-free setup %.3f ms" % ctx.last_assembly_ms(), file=sys.stderr, flush=True)
+        print("bench: matrix-free J x %.3f ms" % ctx.bench_spmv(10), file=sys.stderr, flush=True)
         print("bench: copy 1 GiB %.3f ms" % ctx.bench_copy(1 << 30, 5), file=sys.stderr, flush=True)
         tr = ctx.bench_read(4 << 30, 5)
         print("bench: read 4 GiB %.3f ms (%.0f GB/s)" % (tr, (4 << 30) / (tr * 1e-3) / 1e9), file=sys.stderr,
@@ -512,7 +537,8 @@ def main():
         ctx.close()
         return
     kw = dict(rtol=1e-10, max_newton=100, lin_tol=args.lin_tol, lin_maxit=100000, restart=args.restart)
-    ctx.set_matrix_free(args.matrix_free)
+    mf = not args.assembled
+    ctx.set_matrix_free(mf)
     for w in range(args.warmup):
         ctx.restore_initial()
         st = ctx.newton_solve(**kw)
@@ -529,15 +555,32 @@ def main():
         print("bench: step %d: %s" % (k, st), file=sys.stderr, flush=True)
     converged = all(st["r_final"] <= 1e-10 * st["r0"] for st in runs)
     ms_step = 1000.0 * elapsed / args.steps
-    spmv_avg_ms = tm["spmv_ms"] / max(1, tm["spmv_n"])
-    traffic_alg = spmv_bytes(mesh.n_nodes, mesh.nnzb, 8, 8)
-    achieved = traffic_alg / (spmv_avg_ms * 1e-3) / 1e9
+    apply_avg_ms = tm["spmv_ms"] / max(1, tm["spmv_n"])  # one J x product (GMRES), either operator
     jasm_ms = tm["jasm_ms"] / max(1, tm["jasm_n"])
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(args.workload, {}).get("k_spmv<double,double>", {}).get(
-            "bytes_per_launch")
+    traffic_alg = spmv_bytes(mesh.n_nodes, mesh.nnzb, 8, 8)
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    tjson = json.load(open(tfile)).get(args.workload, {}) if os.path.exists(tfile) else {}
+    t_read = ctx.bench_read(4 << 30, 5)
+    stream_gbs = (4 << 30) / (t_read * 1e-3) / 1e9
+
+    def spmv_roofline(spmv_ms):
+        ach = traffic_alg / (spmv_ms * 1e-3) / 1e9
+        return {"kernel": "k_spmv<double,double> (BSR 18x18 SpMV in GMRES)", "bound": "hbm", "achieved": ach,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": tjson.get("k_spmv<double,double>", {}).get("bytes_per_launch"),
+                "algorithmic_bytes": traffic_alg, "stream_read_gb_s": stream_gbs,
+                "frac_of_stream_read": ach / stream_gbs}
+
+    if mf:
+        flops = mf_flops(deg, mesh.n_cells)
+        ach = flops / (apply_avg_ms * 1e-3) / 1e12
+        roof = {"kernel": "k_mf_apply<%d> (matrix-free J x in GMRES; one product = one launch per colour)" % deg,
+                "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                "frac": ach / FP64_PEAK_TFS, "traffic": tjson.get("k_mf_apply", {}).get("bytes_per_product"),
+                "algorithmic_flops": flops, "algorithmic_bytes": mf_bytes(deg, mesh.n_cells, n_dofs),
+                "hbm_gb_s": mf_bytes(deg, mesh.n_cells, n_dofs) / (apply_avg_ms * 1e-3) / 1e9}
+    else:
+        roof = spmv_roofline(apply_avg_ms)
     st0 = runs[0]
     line = {
         "metric": METRIC,
@@ -553,27 +596,48 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "GL Newton-GMRES solve, Q%d x 18 on [-%g,%g]^3, %d^3 cells, %d DoFs, T/Tc=%g, "
-                               "p=%g bar, FP64 J + GMRES(%d) + point-block Jacobi, eta=%g, ||R||/||R0||<=1e-10"
-                               % (deg, half, half, ncell, n_dofs, t, p, args.restart, args.lin_tol),
-                   "name": args.workload, "dofs": n_dofs, "cells": mesh.n_cells, "nnz": mesh.nnzb * 324},
+                               "p=%g bar, FP64 J (%s) + GMRES(%d) + point-block Jacobi, eta=%g, ||R||/||R0||<=1e-10"
+                               % (deg, half, half, ncell, n_dofs, t, p, "matrix-free" if mf else "assembled BSR",
+                                  args.restart, args.lin_tol),
+                   "name": args.workload, "dofs": n_dofs, "cells": mesh.n_cells, "nnz": mesh.nnzb * 324,
+                   "operator": "matrix-free" if mf else "assembled BSR"},
         "converged": converged,
         "newton_steps": st0["newton_steps"],
         "gmres_its": st0["gmres_its"],
         "free_energy": st0["F"],
-        "assembly_ms": jasm_ms,
-        "assembly_gdof_s": n_dofs / (jasm_ms * 1e-3) / 1e9 if jasm_ms > 0 else None,
-        "assembly_hbm_gb_s": (mesh.nnzb * 324 * 8 + 2 * n_dofs * 8) / (jasm_ms * 1e-3) / 1e9 if jasm_ms else None,
-        "spmv_ms": spmv_avg_ms,
-        "spmv_launches": tm["spmv_n"],
-        "roofline": {"kernel": "k_spmv<double,double> (BSR 18x18 SpMV in GMRES)", "bound": "hbm",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes": traffic_alg},
+        "jacobian_setup_ms": jasm_ms,
+        "jx_ms": apply_avg_ms,
+        "jx_products": tm["spmv_n"],
+        "roofline": roof,
     }
-    # the device's sustained streaming-read rate (4 GiB, 16-byte loads): the
-    # practical ceiling of the read-dominated SpMV, beside the published peak
-    t_read = ctx.bench_read(4 << 30, 5)
-    line["roofline"]["stream_read_gb_s"] = (4 << 30) / (t_read * 1e-3) / 1e9
-    line["roofline"]["frac_of_stream_read"] = achieved / line["roofline"]["stream_read_gb_s"]
+    if mf and not args.no_assembled:
+        # the same FP64 solve with the assembled BSR Jacobian (the reference's
+        # matrix), timed the same way, reported beside the headline
+        U_mf = ctx.solution()
+        ctx.set_matrix_free(False)
+        ctx.set_timing(False)
+        ctx.restore_initial()
+        ctx.assemble(J=True, R=False)  # allocates the BSR values outside the timed solve
+        ctx.restore_initial()
+        ctx.set_timing(True)
+        t1 = time.perf_counter()
+        sa = ctx.newton_solve(**kw)
+        ela = time.perf_counter() - t1
+        tma = ctx.timing()
+        Ua = ctx.solution()
+        ja = tma["jasm_ms"] / max(1, tma["jasm_n"])
+        sp = tma["spmv_ms"] / max(1, tma["spmv_n"])
+        line["assembled"] = {
+            "ms_per_solve": 1000.0 * ela, "solves_hr": 3600.0 / ela,
+            "converged": bool(sa["r_final"] <= 1e-10 * sa["r0"]), "newton_steps": sa["newton_steps"],
+            "gmres_its": sa["gmres_its"], "F_rel_diff_vs_matrix_free": abs(sa["F"] - st0["F"]) / abs(st0["F"]),
+            "U_rel_diff_vs_matrix_free": float(np.abs(Ua - U_mf).max() / np.abs(U_mf).max()),
+            "assembly_ms": ja, "assembly_gdof_s": n_dofs / (ja * 1e-3) / 1e9 if ja > 0 else None,
+            "assembly_hbm_gb_s": (mesh.nnzb * 324 * 8 + 2 * n_dofs * 8) / (ja * 1e-3) / 1e9 if ja else None,
+            "spmv_ms": sp, "spmv_roofline": spmv_roofline(sp)}
+        print("bench: assembled %s" % {k: v for k, v in line["assembled"].items() if k != "spmv_roofline"},
+              file=sys.stderr, flush=True)
+        ctx.set_matrix_free(True)
     if not args.no_mixed:
         # the same solve with FP32 J and GMRES inside FP64 iterative refinement
         # (reported beside the FP64 headline, not part of its value)
