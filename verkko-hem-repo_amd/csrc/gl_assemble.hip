@@ -389,8 +389,8 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
 //   X_q = sum_b N_b(q) x_b, G_kq = sum_b d_k N_b(q) x_b / h_k,
 //   y_a += sum_q w_q vol (N_a(q) H(q) X_q + sum_k d_k N_a(q) / h_k gamma_k(q)),
 //   gamma_{c,k} = K1 G_kq[c] + K23 [k == kc] sum_kd G_kd,q[9 part + 3 mu + kd],
-// about 1.1e5 multiply-adds per Q2 cell instead of streaming the cell's
-// 729 blocks.  k_mf_setup stores H_cd(q) per cell (the only state of the
+// about 3.5e4 multiply-adds per Q2 cell (sum-factorised) instead of
+// streaming the cell's 729 blocks.  k_mf_setup stores H_cd(q) per cell (the only state of the
 // operator besides the diagonal blocks) and assembles the diagonal blocks
 // for the point-block Jacobi preconditioner with the arithmetic of
 // k_assemble; Dirichlet columns are applied as a zero x there, Dirichlet
@@ -489,17 +489,20 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
 
 constexpr int NTA = 256;  // threads per workgroup of the matrix-free product
 
+// The product runs in sum-factorised form: N_b(q) = L_bx(x_qx) L_by(x_qy)
+// L_bz(x_qz), so x and its gradients at the quadrature points take three 1D
+// passes (x, y, z) and the test against N_a and its gradients three more (z,
+// y, x), 2 x 18 x 27 x 27 multiply-adds per Q2 cell instead of 8 x 18 x 27 x 27.
 template <int DEG>
 __global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ Hq,
                                                   const double *__restrict__ x, const int *__restrict__ cells,
                                                   double *__restrict__ y)
 {
-  constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS;
-  __shared__ double sx[NS * 18];   // x at the cell's nodes, Dirichlet components 0
-  __shared__ double sX[NQ * 18];   // x at the quadrature points
-  __shared__ double sG[NQ * 54];   // [q][k][c] d_k x / h_k
-  __shared__ double sZ[NQ * 18];   // w vol H X
-  __shared__ double sK[NQ * 54];   // [q][k][c] w vol gamma_{c,k} / h_k
+  constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS, V = NS * 18;
+  __shared__ double sx[V];      // x at the cell's nodes, Dirichlet components 0
+  __shared__ double R1[5 * V];  // interpolation passes; then w vol H X and w vol gamma / h; then the y pass
+  __shared__ double R2[4 * V];  // x and its gradients at the quadrature points; then the z pass
+  __shared__ double sL[N1 * N1], sdL[N1 * N1];
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
   const Material M = Mh.mats ? Mh.mats[Mh.cell_mat[e]] : M0;
@@ -508,30 +511,71 @@ __global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Mater
   for (int k = 0; k < 3; ++k) { h[k] = Mh.cell_h[3 * e + k]; ih[k] = 1.0 / h[k]; }
   const double vol = h[0] * h[1] * h[2];
   const uint32_t cface = Mh.cell_face[e];
-  for (int i = tid; i < NS * 18; i += NTA) {
+  if (tid < N1 * N1) {
+    sL[tid] = T.L1[tid];
+    sdL[tid] = T.dL1[tid];
+  }
+  for (int i = tid; i < V; i += NTA) {
     const int node = cn[i / 18], c = i % 18;
     sx[i] = ((Mh.node_mask[node] >> c) & 1u) ? 0.0 : x[(size_t)node * 18 + c];
   }
   __syncthreads();
-  for (int i = tid; i < NQ * 18; i += NTA) {
-    const int q = i / 18, c = i % 18;
-    double xv = 0, g0 = 0, g1 = 0, g2 = 0;
-    for (int b = 0; b < NS; ++b) {
-      const double xb = sx[b * 18 + c];
-      const double *dn = T.dN + (q * NS + b) * 3;
-      xv += T.N[q * NS + b] * xb;
-      g0 += dn[0] * xb;
-      g1 += dn[1] * xb;
-      g2 += dn[2] * xb;
+  // index of point (i0, i1, i2) (i0 fastest), component c
+  auto at = [](int i0, int i1, int i2, int c) { return ((i2 * N1 + i1) * N1 + i0) * 18 + c; };
+  double *A0 = R1, *A1 = R1 + V, *B0 = R1 + 2 * V, *B1 = R1 + 3 * V, *B2 = R1 + 4 * V;
+  double *sX = R2, *sG = R2 + V;  // sG: [k][q][c]
+  // x pass: (qx, by, bz)
+  for (int i = tid; i < V; i += NTA) {
+    const int c = i % 18, r = i / 18, qx = r % N1, by = (r / N1) % N1, bz = r / (N1 * N1);
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int bx = 0; bx < N1; ++bx) {
+      const double v = sx[at(bx, by, bz, c)];
+      a0 += sL[qx * N1 + bx] * v;
+      a1 += sdL[qx * N1 + bx] * v;
     }
-    sX[i] = xv;
-    sG[q * 54 + c] = g0 * ih[0];
-    sG[q * 54 + 18 + c] = g1 * ih[1];
-    sG[q * 54 + 36 + c] = g2 * ih[2];
+    A0[i] = a0;
+    A1[i] = a1;
   }
   __syncthreads();
+  // y pass: (qx, qy, bz)
+  for (int i = tid; i < V; i += NTA) {
+    const int c = i % 18, r = i / 18, qx = r % N1, qy = (r / N1) % N1, bz = r / (N1 * N1);
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int by = 0; by < N1; ++by) {
+      const int k = at(qx, by, bz, c);
+      b0 += sL[qy * N1 + by] * A0[k];
+      b1 += sL[qy * N1 + by] * A1[k];
+      b2 += sdL[qy * N1 + by] * A0[k];
+    }
+    B0[i] = b0;
+    B1[i] = b1;
+    B2[i] = b2;
+  }
+  __syncthreads();
+  // z pass: (qx, qy, qz) = q
+  for (int i = tid; i < V; i += NTA) {
+    const int c = i % 18, r = i / 18, qx = r % N1, qy = (r / N1) % N1, qz = r / (N1 * N1);
+    double xv = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+#pragma unroll
+    for (int bz = 0; bz < N1; ++bz) {
+      const int k = at(qx, qy, bz, c);
+      xv += sL[qz * N1 + bz] * B0[k];
+      g0 += sL[qz * N1 + bz] * B1[k];
+      g1 += sL[qz * N1 + bz] * B2[k];
+      g2 += sdL[qz * N1 + bz] * B0[k];
+    }
+    sX[i] = xv;
+    sG[i] = g0 * ih[0];
+    sG[V + i] = g1 * ih[1];
+    sG[2 * V + i] = g2 * ih[2];
+  }
+  __syncthreads();
+  // w vol H(q) X(q) and w vol gamma_k(q) / h_k
+  double *sZ = R1, *sK = R1 + V;  // sK: [k][q][c]
   const double *Hc = Hq + (size_t)e * NQ * NUPAIR;
-  for (int i = tid; i < NQ * 18; i += NTA) {
+  for (int i = tid; i < V; i += NTA) {
     const int q = i / 18, c = i % 18;
     const double *Hrow = Hc + q * NUPAIR;
     const double *X = sX + q * 18;
@@ -540,23 +584,55 @@ __global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Mater
     for (int d = 0; d < 18; ++d) z += Hrow[upair(c, d)] * X[d];
     const double wv = T.w[q] * vol;
     sZ[i] = wv * z;
-    const int part = c / 9, mu = (c % 9) / 3, kc = c % 3;
-    const double *g = sG + q * 54;
-    const double dv = g[9 * part + 3 * mu + 0] + g[18 + 9 * part + 3 * mu + 1] + g[36 + 9 * part + 3 * mu + 2];
+    const int part = c / 9, mu = (c % 9) / 3, kc = c % 3, base = q * 18 + 9 * part + 3 * mu;
+    const double dv = sG[base] + sG[V + base + 1] + sG[2 * V + base + 2];
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      sK[q * 54 + 18 * k + c] = wv * ih[k] * (M.K1 * g[18 * k + c] + (k == kc ? M.K23 * dv : 0.0));
+    for (int k = 0; k < 3; ++k) sK[k * V + i] = wv * ih[k] * (M.K1 * sG[k * V + i] + (k == kc ? M.K23 * dv : 0.0));
   }
   __syncthreads();
-  for (int i = tid; i < NS * 18; i += NTA) {
-    const int a = i / 18, c = i % 18;
+  // test z pass: (qx, qy, az)
+  double *C0 = R2, *C1 = R2 + V, *C2 = R2 + 2 * V;
+  for (int i = tid; i < V; i += NTA) {
+    const int c = i % 18, r = i / 18, qx = r % N1, qy = (r / N1) % N1, az = r / (N1 * N1);
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+#pragma unroll
+    for (int qz = 0; qz < N1; ++qz) {
+      const int k = at(qx, qy, qz, c);
+      const double l = sL[qz * N1 + az];
+      c0 += l * sZ[k] + sdL[qz * N1 + az] * sK[2 * V + k];
+      c1 += l * sK[k];
+      c2 += l * sK[V + k];
+    }
+    C0[i] = c0;
+    C1[i] = c1;
+    C2[i] = c2;
+  }
+  __syncthreads();
+  // test y pass: (qx, ay, az)
+  double *D0 = R1, *D1 = R1 + V;
+  for (int i = tid; i < V; i += NTA) {
+    const int c = i % 18, r = i / 18, qx = r % N1, ay = (r / N1) % N1, az = r / (N1 * N1);
+    double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+    for (int qy = 0; qy < N1; ++qy) {
+      const int k = at(qx, qy, az, c);
+      d0 += sL[qy * N1 + ay] * C0[k] + sdL[qy * N1 + ay] * C2[k];
+      d1 += sL[qy * N1 + ay] * C1[k];
+    }
+    D0[i] = d0;
+    D1[i] = d1;
+  }
+  __syncthreads();
+  // test x pass: (ax, ay, az) = a, then the Robin faces and the scatter
+  for (int i = tid; i < V; i += NTA) {
+    const int c = i % 18, a = i / 18, ax = a % N1, ay = (a / N1) % N1, az = a / (N1 * N1);
     const int node = cn[a];
     if ((Mh.node_mask[node] >> c) & 1u) continue;  // Dirichlet row: k_mf_rows
     double r = 0.0;
-    for (int q = 0; q < NQ; ++q) {
-      const double *dn = T.dN + (q * NS + a) * 3;
-      const double *K = sK + q * 54;
-      r += T.N[q * NS + a] * sZ[q * 18 + c] + dn[0] * K[c] + dn[1] * K[18 + c] + dn[2] * K[36 + c];
+#pragma unroll
+    for (int qx = 0; qx < N1; ++qx) {
+      const int k = at(qx, ay, az, c);
+      r += sL[qx * N1 + ax] * D0[k] + sdL[qx * N1 + ax] * D1[k];
     }
     if (M.robin != 0.0)
       for (int lf = 0; lf < 6; ++lf) {

@@ -106,6 +106,7 @@ struct glh_ctx {
   // reference tables
   DBuf<double> tN, tdN, tw, tWp, tSp, tMf;
   DBuf<int> tpa, tpb;
+  DBuf<double> tL1, tdL1;
   // mesh arrays
   DBuf<int> cell_nodes, row_ptr, col, diag, cell_blk, color_cells, cell_mat, seg_chunk;
   // hanging-node constraints: the cell loop assembles Jf (nnzb_a blocks over
@@ -184,7 +185,7 @@ struct glh_ctx {
 
   glh::RefTables tables() const
   {
-    return glh::RefTables{tN.p, tdN.p, tw.p, tWp.p, tSp.p, tMf.p, tpa.p, tpb.p};
+    return glh::RefTables{tN.p, tdN.p, tw.p, tWp.p, tSp.p, tMf.p, tpa.p, tpb.p, tL1.p, tdL1.p};
   }
   glh::MeshDev meshdev() const
   {
@@ -315,6 +316,11 @@ int build_tables(glh_ctx *c, int deg)
   HIPCHK(c, upload(c->tMf, Mf.data(), Mf.size(), s));
   HIPCHK(c, upload(c->tpa, pa.data(), pa.size(), s));
   HIPCHK(c, upload(c->tpb, pb.data(), pb.size(), s));
+  std::vector<double> L1(n1 * n1), dL1(n1 * n1);
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n1; ++j) { L1[i * n1 + j] = L[i][j]; dL1[i * n1 + j] = dL[i][j]; }
+  HIPCHK(c, upload(c->tL1, L1.data(), L1.size(), s));
+  HIPCHK(c, upload(c->tdL1, dL1.data(), dL1.size(), s));
   HIPCHK(c, hipStreamSynchronize(s));
   c->deg = deg; c->ns = ns; c->nq = nq; c->npair = npair;
   return GLH_OK;

@@ -19,6 +19,8 @@ struct RefTables {
   const double *Sp;   // [NPAIR*9]  sum_q w_q dN_a/dxi_k dN_b/dxi_l, index k*3+l
   const double *Mf;   // [6*NS*NS]  face mass on each unit face
   const int *pa, *pb; // [NPAIR]    pair list (a <= b)
+  const double *L1;   // [N1*N1]   1D Lagrange basis: L1[qx*N1 + bx] = L_bx(x_qx)
+  const double *dL1;  // [N1*N1]   its derivative
 };
 
 struct Material {
