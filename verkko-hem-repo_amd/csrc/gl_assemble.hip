@@ -394,7 +394,7 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
 // operator besides the diagonal blocks) and assembles the diagonal blocks
 // for the point-block Jacobi preconditioner with the arithmetic of
 // k_assemble; Dirichlet columns are applied as a zero x there, Dirichlet
-// rows keep only their diagonal (k_mf_rows).
+// rows keep only their diagonal (k_mf_gather).
 
 // index of the component pair (c <= d) in the order of the owner threads
 __device__ inline int upair(int c, int d)
@@ -496,7 +496,7 @@ constexpr int NTA = 256;  // threads per workgroup of the matrix-free product
 template <int DEG>
 __global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ Hq,
                                                   const double *__restrict__ x, const int *__restrict__ cells,
-                                                  double *__restrict__ y)
+                                                  double *__restrict__ ycell)
 {
   constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS, V = NS * 18;
   __shared__ double sx[V];      // x at the cell's nodes, Dirichlet components 0
@@ -623,11 +623,9 @@ __global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Mater
     D1[i] = d1;
   }
   __syncthreads();
-  // test x pass: (ax, ay, az) = a, then the Robin faces and the scatter
+  // test x pass: (ax, ay, az) = a, then the Robin faces; one slot per cell
   for (int i = tid; i < V; i += NTA) {
     const int c = i % 18, a = i / 18, ax = a % N1, ay = (a / N1) % N1, az = a / (N1 * N1);
-    const int node = cn[a];
-    if ((Mh.node_mask[node] >> c) & 1u) continue;  // Dirichlet row: k_mf_rows
     double r = 0.0;
 #pragma unroll
     for (int qx = 0; qx < N1; ++qx) {
@@ -643,19 +641,27 @@ __global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Mater
         for (int b = 0; b < NS; ++b) sf += mf[b] * sx[b * 18 + c];
         r += (M.robin * face_area(lf, h)) * sf;
       }
-    y[(size_t)node * 18 + c] += r;
+    ycell[(size_t)e * V + i] = r;  // Dirichlet rows are replaced in the gather
   }
 }
 
-// Dirichlet rows of J x: their diagonal entry (the sum over cells of
-// |local diagonal|) times x
-__global__ void k_mf_rows(int n_nodes, const uint32_t *__restrict__ mask, const double *__restrict__ Jd,
-                          const double *__restrict__ x, double *__restrict__ y)
+// y = sum over the cells of each node of their contributions (fixed cell
+// order), and for the Dirichlet rows their diagonal entry (the sum over cells
+// of |local diagonal|) times x
+__global__ void k_mf_gather(int n_nodes, const int *__restrict__ ptr, const int *__restrict__ idx,
+                            const double *__restrict__ ycell, const uint32_t *__restrict__ mask,
+                            const double *__restrict__ Jd, const double *__restrict__ x, double *__restrict__ y)
 {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)n_nodes * 18) return;
   const int node = (int)(i / 18), c = (int)(i % 18);
-  if ((mask[node] >> c) & 1u) y[i] = Jd[(size_t)node * 324 + c * 19] * x[i];
+  if ((mask[node] >> c) & 1u) {
+    y[i] = Jd[(size_t)node * 324 + c * 19] * x[i];
+    return;
+  }
+  double s = 0.0;
+  for (int k = ptr[node]; k < ptr[node + 1]; ++k) s += ycell[(size_t)idx[k] * 18 + c];
+  y[i] = s;
 }
 
 template <int DEG>
@@ -799,20 +805,21 @@ hipError_t launch_mf_setup(int deg, const RefTables &T, const MeshDev &M, const 
 }
 
 hipError_t launch_mf_apply(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *Hq,
-                           const double *x, const int *cells, int n, double *y, hipStream_t s)
+                           const double *x, const int *cells, int n, double *ycell, hipStream_t s)
 {
   if (n <= 0) return hipSuccess;
-  if (deg == 1) hipLaunchKernelGGL(k_mf_apply<1>, dim3(n), dim3(NTA), 0, s, T, M, mat, Hq, x, cells, y);
-  else hipLaunchKernelGGL(k_mf_apply<2>, dim3(n), dim3(NTA), 0, s, T, M, mat, Hq, x, cells, y);
+  if (deg == 1) hipLaunchKernelGGL(k_mf_apply<1>, dim3(n), dim3(NTA), 0, s, T, M, mat, Hq, x, cells, ycell);
+  else hipLaunchKernelGGL(k_mf_apply<2>, dim3(n), dim3(NTA), 0, s, T, M, mat, Hq, x, cells, ycell);
   return hipGetLastError();
 }
 
-hipError_t launch_mf_rows(int n_nodes, const uint32_t *mask, const double *Jd, const double *x, double *y,
-                          hipStream_t s)
+hipError_t launch_mf_gather(int n_nodes, const int *ptr, const int *idx, const double *ycell, const uint32_t *mask,
+                            const double *Jd, const double *x, double *y, hipStream_t s)
 {
   if (n_nodes <= 0) return hipSuccess;
   const long long n = (long long)n_nodes * 18;
-  hipLaunchKernelGGL(k_mf_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_nodes, mask, Jd, x, y);
+  hipLaunchKernelGGL(k_mf_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_nodes, ptr, idx, ycell, mask,
+                     Jd, x, y);
   return hipGetLastError();
 }
 

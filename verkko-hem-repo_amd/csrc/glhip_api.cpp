@@ -141,6 +141,8 @@ struct glh_ctx {
   bool matfree = false, j_mf = false;
   DBuf<double> Hq64, Jd64;
   DBuf<int> diag_id;  // node -> its block in Jd64 (the identity)
+  DBuf<int> n2c_ptr, n2c_idx;  // node -> its (cell * ns + local node) entries
+  DBuf<double> mf_ycell;       // per-cell contributions of a product [n_cells][ns][18]
   int diag_id_n = 0;
   DBuf<int> status;
   // GMRES workspace
@@ -374,6 +376,7 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
   if (do_j && mf) {  // matrix-free: H at the quadrature points and the diagonal blocks
     if (fp32 || c->constrained || c->n_seg != 1) return fail(c, GLH_ERR_STATE, "matrix-free Jacobian: FP64, one instance, no constraints");
     HIPCHK(c, c->Hq64.alloc((size_t)c->n_cells * c->nq * 171));
+    HIPCHK(c, c->mf_ycell.alloc((size_t)c->n_cells * c->ns * NC));
     HIPCHK(c, c->Jd64.alloc((size_t)c->n_nodes * 324));
     if (c->diag_id_n != c->n_nodes) {
       std::vector<int> id(c->n_nodes);
@@ -525,14 +528,12 @@ int apply_j(glh_ctx *c, const TJ *J, const TV *x, TV *y, const int *flag, glh::A
     if (c->j_mf) {  // one instance, every row iterating: flag and act are not needed
       const glh::RefTables T = c->tables();
       const glh::MeshDev M = c->meshdev();
-      HIPCHK(c, glh::launch_fill<double>(ndofs(c), 0.0, y, c->stream));
-      const int ncol = (int)c->color_ptr.size() - 1;
-      for (int k = 0; k < ncol; ++k) {
-        const int off = c->color_ptr[k], n = c->color_ptr[k + 1] - off;
-        HIPCHK(c, glh::launch_mf_apply(c->deg, T, M, c->mat, c->Hq64.p, x, c->color_cells.p + off, n, y,
-                                       c->stream));
-      }
-      HIPCHK(c, glh::launch_mf_rows(c->n_nodes, c->node_mask.p, c->Jd64.p, x, y, c->stream));
+      // every cell at once into its own slot, then a fixed-order gather per
+      // node (no colours: one launch, no read-modify-write of y)
+      HIPCHK(c, glh::launch_mf_apply(c->deg, T, M, c->mat, c->Hq64.p, x, c->color_cells.p, c->n_cells,
+                                     c->mf_ycell.p, c->stream));
+      HIPCHK(c, glh::launch_mf_gather(c->n_nodes, c->n2c_ptr.p, c->n2c_idx.p, c->mf_ycell.p, c->node_mask.p,
+                                      c->Jd64.p, x, y, c->stream));
       return GLH_OK;
     }
   }
@@ -1490,6 +1491,17 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
   }
   c->n_chunks = (int)(ch.size() / 3);
   HIPCHK(c, upload(c->cell_nodes, m.cell_nodes.data(), m.cell_nodes.size(), s));
+  // node -> (cell * ns + local node) lists in cell order, for the gather of
+  // the matrix-free product
+  std::vector<int> n2c_ptr(m.n_nodes + 1, 0), n2c_idx(m.cell_nodes.size());
+  for (int v : m.cell_nodes) ++n2c_ptr[v + 1];
+  for (int i = 0; i < m.n_nodes; ++i) n2c_ptr[i + 1] += n2c_ptr[i];
+  {
+    std::vector<int> fill(n2c_ptr.begin(), n2c_ptr.end() - 1);
+    for (size_t k = 0; k < m.cell_nodes.size(); ++k) n2c_idx[fill[m.cell_nodes[k]]++] = (int)k;
+  }
+  HIPCHK(c, upload(c->n2c_ptr, n2c_ptr.data(), n2c_ptr.size(), s));
+  HIPCHK(c, upload(c->n2c_idx, n2c_idx.data(), n2c_idx.size(), s));
   HIPCHK(c, upload(c->cell_h, m.cell_h.data(), m.cell_h.size(), s));
   HIPCHK(c, upload(c->cell_face, cface.data(), cface.size(), s));
   HIPCHK(c, upload(c->node_mask, m.node_mask.data(), m.node_mask.size(), s));

@@ -79,15 +79,16 @@ hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefT
 // Per-cell free energy into Fcell[cell] for cells [0, n_cells).
 // Matrix-free Jacobian (meshes without constraints): per-cell bulk Hessian
 // at the quadrature points (Hq: [n_cells][NQ][171]) and the diagonal blocks
-// (Jd: [n_nodes][324], colour-ordered like the assembly); the product adds
-// J x over one colour's cells into y (zeroed before the first colour), then
-// launch_mf_rows writes the Dirichlet rows.
+// (Jd: [n_nodes][324], colour-ordered like the assembly); the product writes
+// each listed cell's contribution to its slot of ycell ([n_cells][NS][18]),
+// and launch_mf_gather sums them per node (node -> cell*NS + a lists, fixed
+// order) and writes the Dirichlet rows.
 hipError_t launch_mf_setup(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
                            const int *cells, int n, double *Hq, double *Jd, hipStream_t s);
 hipError_t launch_mf_apply(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *Hq,
-                           const double *x, const int *cells, int n, double *y, hipStream_t s);
-hipError_t launch_mf_rows(int n_nodes, const uint32_t *mask, const double *Jd, const double *x, double *y,
-                          hipStream_t s);
+                           const double *x, const int *cells, int n, double *ycell, hipStream_t s);
+hipError_t launch_mf_gather(int n_nodes, const int *ptr, const int *idx, const double *ycell, const uint32_t *mask,
+                            const double *Jd, const double *x, double *y, hipStream_t s);
 hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
                          int n_cells, double *Fcell, hipStream_t s);
 
