@@ -577,7 +577,8 @@ def main():
         # cube) takes longer at the HBM peak than the flops at the FP64 peak
         flops, nbytes = mf_flops(deg, mesh.n_cells), mf_bytes(deg, mesh.n_cells, n_dofs)
         ach = nbytes / (apply_avg_ms * 1e-3) / 1e9
-        roof = {"kernel": "k_mf_apply<%d> (matrix-free J x in GMRES; one product = one launch per colour)" % deg,
+        roof = {"kernel": "k_mf_apply<%d> + k_mf_gather (matrix-free J x in GMRES: one launch over the cells, "
+                          "one node gather)" % deg,
                 "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                 "traffic": tjson.get("k_mf_apply", {}).get("bytes_per_product"), "algorithmic_bytes": nbytes,
                 "algorithmic_flops": flops, "tflop_s": flops / (apply_avg_ms * 1e-3) / 1e12,
