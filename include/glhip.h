@@ -43,7 +43,7 @@ typedef struct glh_forest glh_forest;
 #define GLH_ASM_J 1           /* Jacobian (assemble.cc:177-280) */
 #define GLH_ASM_R 2           /* right-hand side -R (assemble.cc:257-276) */
 #define GLH_ASM_FP32 4        /* Jacobian stored and solved in FP32 */
-#define GLH_ASM_MATRIX_FREE 8 /* Jacobian in matrix-free form (see glh_set_matrix_free) */
+#define GLH_ASM_MATRIX_FREE 8 /* Jacobian in matrix-free form (see glh_set_matrix_free); with FP32 the Hessian is FP32 */
 
 /* ---------------------------------------------------------------- context */
 /* FemGL<dim> constructor state (femgl.cc:107-203): device + stream. */
@@ -303,8 +303,7 @@ int glh_apply_jacobian(glh_ctx *ctx, const double *x_host, double *y_host);
  * the current solution, with constrained rows condensed as
  * AffineConstraints::distribute_local_to_global does. flags: GLH_ASM_*;
  * with GLH_ASM_MATRIX_FREE the Jacobian is kept in the matrix-free form of
- * glh_set_matrix_free (FP64, one instance, no constraints) instead of BSR
- * values. */
+ * glh_set_matrix_free (one instance, no constraints) instead of BSR values. */
 int glh_assemble(glh_ctx *ctx, int flags);
 /* ||system_rhs|| after glh_assemble with GLH_ASM_R. */
 int glh_rhs_norm(glh_ctx *ctx, double *norm);
@@ -395,9 +394,11 @@ int glh_set_continuation(glh_ctx *ctx, double sigma0);
  * makes each Newton step store the bulk Hessian of every cell at its
  * quadrature points and only the 18x18 diagonal blocks, and GMRES applies
  * J x cell by cell from them (the same operator as the assembled BSR matrix,
- * to rounding).  It applies to FP64 solves of one instance on meshes without
- * hanging-node or periodic constraints and without continuation; other solves
- * assemble J as before.  glh_assemble always assembles the BSR values;
+ * to rounding).  With fp32 the Hessian is stored in FP32 and the inner GMRES,
+ * its Krylov basis and the block Jacobi run in FP32, inside the FP64
+ * iterative refinement.  It applies to solves of one instance on meshes
+ * without hanging-node or periodic constraints and without continuation;
+ * other solves assemble J as before.  glh_assemble always assembles the BSR values;
  * glh_get_jacobian fails after a matrix-free step.  0 (default) = off. */
 int glh_set_matrix_free(glh_ctx *ctx, int on);
 int glh_set_verbose(glh_ctx *ctx, int level);

@@ -2,7 +2,8 @@
 the assembled BSR Jacobian and the oracle's: J x to 1e-12 relative (normwise,
 max|diff| / max|ref|) on Q1 and Q2 box meshes with Dirichlet (id 0), AdGR
 (ids 2-4) and Robin walls, x random over every DoF so that the Dirichlet
-columns and rows are exercised; a Newton solve reaches the assembled path's
+columns and rows are exercised, and the FP32 forms to 2e-6; a Newton solve,
+FP64 or with the FP32 matrix-free inner solve, reaches the assembled FP64
 fixed point (free energy 1e-10, order parameter 1e-8); meshes with
 constraints keep the assembled Jacobian."""
 import sys
@@ -77,8 +78,13 @@ def test_matrix_free_product_equals_the_assembled_and_the_oracle(ctx, case):
     # the matrix-free form holds no BSR values
     with pytest.raises(G.GLHError, match="matrix-free"):
         ctx.jacobian()
-    with pytest.raises(G.GLHError, match="FP64"):
-        ctx.assemble(J=True, R=False, fp32=True, matrix_free=True)
+    # FP32 forms (the Hessian stored in FP32): the two agree to FP32 rounding
+    ctx.assemble(J=True, R=False, fp32=True)
+    ya32 = ctx.apply_jacobian(x)
+    ctx.assemble(J=True, R=False, fp32=True, matrix_free=True)
+    ym32 = ctx.apply_jacobian(x)
+    say("  fp32: assembled", _rel(ya32, yo), "matrix-free", _rel(ym32, yo))
+    assert _rel(ya32, yo) <= 2e-6 and _rel(ym32, yo) <= 2e-6
 
 
 def test_matrix_free_newton_reaches_the_assembled_fixed_point(ctx):
@@ -109,6 +115,32 @@ def test_matrix_free_newton_reaches_the_assembled_fixed_point(ctx):
     # the last Jacobian was matrix-free
     with pytest.raises(G.GLHError, match="matrix-free"):
         ctx.jacobian()
+
+
+def test_mixed_precision_matrix_free_newton_reaches_the_fp64_fixed_point(ctx):
+    # FP32 matrix-free inner solve (FP32 Hessian, Krylov basis and block
+    # Jacobi) inside FP64 iterative refinement with an FP64 residual
+    mesh = G.Mesh(2, (4, 4, 4), (-20.0, -20.0, -20.0), (20.0, 20.0, 20.0), [1, 1, 1, 1, 0, 0])
+    p, t = 26.0, 0.7
+    ctx.set_material_pt(p, t)
+    ctx.upload_mesh(mesh)
+    U0 = initial_state(mesh.n_nodes, p, t, 0.05, 0)
+    kw = dict(rtol=1e-10, max_newton=50, lin_tol=0.1, lin_maxit=20000, restart=50)
+    ctx.set_solution(U0)
+    st64 = ctx.newton_solve(**kw)
+    U64 = ctx.solution()
+    ctx.set_solution(U0)
+    ctx.set_matrix_free(True)
+    try:
+        st32 = ctx.newton_solve(fp32=True, **kw)
+    finally:
+        ctx.set_matrix_free(False)
+    U32 = ctx.solution()
+    say("fp64", st64)
+    say("fp32 matrix-free", st32)
+    assert st32["converged"] == 1.0
+    assert st32["F"] == pytest.approx(st64["F"], rel=1e-10)
+    assert np.abs(U32 - U64).max() <= 1e-8 * np.abs(U64).max()
 
 
 def test_meshes_with_constraints_keep_the_assembled_jacobian(ctx):

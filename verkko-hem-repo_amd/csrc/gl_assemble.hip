@@ -403,9 +403,9 @@ __device__ inline int upair(int c, int d)
   return lo * 18 - lo * (lo - 1) / 2 + (hi - lo);
 }
 
-template <int DEG>
+template <int DEG, typename TH>
 __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ U,
-                                                 const int *__restrict__ cells, double *__restrict__ Hq,
+                                                 const int *__restrict__ cells, TH *__restrict__ Hq,
                                                  double *__restrict__ Jd)
 {
   constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS;
@@ -442,11 +442,11 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
     d = c + t;
   }
   double Hv[NQ];
-  double *Hc = Hq + (size_t)e * NQ * NUPAIR;
+  TH *Hc = Hq + (size_t)e * NQ * NUPAIR;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     Hv[q] = hess_entry(sGram + q * GRAM, sX + q * 18, c, d, M);
-    Hc[q * NUPAIR + tid] = Hv[q];
+    Hc[q * NUPAIR + tid] = (TH)Hv[q];  // FP32 for the FP32 inner solve; the diagonal blocks stay FP64
   }
   // diagonal blocks (a,a), as k_assemble forms and masks them
   const bool grad_cd = (c / 9 == d / 9) && ((c % 9) / 3 == (d % 9) / 3);
@@ -493,9 +493,11 @@ constexpr int NTA = 256;  // threads per workgroup of the matrix-free product
 // L_bz(x_qz), so x and its gradients at the quadrature points take three 1D
 // passes (x, y, z) and the test against N_a and its gradients three more (z,
 // y, x), 2 x 18 x 27 x 27 multiply-adds per Q2 cell instead of 8 x 18 x 27 x 27.
-template <int DEG>
-__global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ Hq,
-                                                  const double *__restrict__ x, const int *__restrict__ cells,
+// TH: stored Hessian (FP64, or FP32 for the FP32 inner solve); TX: x.  The
+// arithmetic is FP64.
+template <int DEG, typename TH, typename TX>
+__global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Material M0, const TH *__restrict__ Hq,
+                                                  const TX *__restrict__ x, const int *__restrict__ cells,
                                                   double *__restrict__ ycell)
 {
   constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS, V = NS * 18;
@@ -517,7 +519,7 @@ __global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Mater
   }
   for (int i = tid; i < V; i += NTA) {
     const int node = cn[i / 18], c = i % 18;
-    sx[i] = ((Mh.node_mask[node] >> c) & 1u) ? 0.0 : x[(size_t)node * 18 + c];
+    sx[i] = ((Mh.node_mask[node] >> c) & 1u) ? 0.0 : (double)x[(size_t)node * 18 + c];
   }
   __syncthreads();
   // index of point (i0, i1, i2) (i0 fastest), component c
@@ -574,14 +576,14 @@ __global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Mater
   __syncthreads();
   // w vol H(q) X(q) and w vol gamma_k(q) / h_k
   double *sZ = R1, *sK = R1 + V;  // sK: [k][q][c]
-  const double *Hc = Hq + (size_t)e * NQ * NUPAIR;
+  const TH *Hc = Hq + (size_t)e * NQ * NUPAIR;
   for (int i = tid; i < V; i += NTA) {
     const int q = i / 18, c = i % 18;
-    const double *Hrow = Hc + q * NUPAIR;
+    const TH *Hrow = Hc + q * NUPAIR;
     const double *X = sX + q * 18;
     double z = 0.0;
 #pragma unroll
-    for (int d = 0; d < 18; ++d) z += Hrow[upair(c, d)] * X[d];
+    for (int d = 0; d < 18; ++d) z += (double)Hrow[upair(c, d)] * X[d];
     const double wv = T.w[q] * vol;
     sZ[i] = wv * z;
     const int part = c / 9, mu = (c % 9) / 3, kc = c % 3, base = q * 18 + 9 * part + 3 * mu;
@@ -648,20 +650,23 @@ __global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Mater
 // y = sum over the cells of each node of their contributions (fixed cell
 // order), and for the Dirichlet rows their diagonal entry (the sum over cells
 // of |local diagonal|) times x
+// With b != null the gather returns the residual y = b - J x instead.
+template <typename TX>
 __global__ void k_mf_gather(int n_nodes, const int *__restrict__ ptr, const int *__restrict__ idx,
                             const double *__restrict__ ycell, const uint32_t *__restrict__ mask,
-                            const double *__restrict__ Jd, const double *__restrict__ x, double *__restrict__ y)
+                            const double *__restrict__ Jd, const TX *__restrict__ x, const TX *__restrict__ b,
+                            TX *__restrict__ y)
 {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)n_nodes * 18) return;
   const int node = (int)(i / 18), c = (int)(i % 18);
-  if ((mask[node] >> c) & 1u) {
-    y[i] = Jd[(size_t)node * 324 + c * 19] * x[i];
-    return;
-  }
   double s = 0.0;
-  for (int k = ptr[node]; k < ptr[node + 1]; ++k) s += ycell[(size_t)idx[k] * 18 + c];
-  y[i] = s;
+  if ((mask[node] >> c) & 1u) {
+    s = Jd[(size_t)node * 324 + c * 19] * (double)x[i];
+  } else {
+    for (int k = ptr[node]; k < ptr[node + 1]; ++k) s += ycell[(size_t)idx[k] * 18 + c];
+  }
+  y[i] = b ? (TX)((double)b[i] - s) : (TX)s;
 }
 
 template <int DEG>
@@ -795,33 +800,49 @@ hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefT
   return hipGetLastError();
 }
 
-hipError_t launch_mf_setup(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
-                           const int *cells, int n, double *Hq, double *Jd, hipStream_t s)
+hipError_t launch_mf_setup(int deg, bool h32, const RefTables &T, const MeshDev &M, const Material &mat,
+                           const double *U, const int *cells, int n, void *Hq, double *Jd, hipStream_t s)
 {
   if (n <= 0) return hipSuccess;
-  if (deg == 1) hipLaunchKernelGGL(k_mf_setup<1>, dim3(n), dim3(NT), 0, s, T, M, mat, U, cells, Hq, Jd);
-  else hipLaunchKernelGGL(k_mf_setup<2>, dim3(n), dim3(NT), 0, s, T, M, mat, U, cells, Hq, Jd);
+#define GLH_MFS(D, TH) hipLaunchKernelGGL((k_mf_setup<D, TH>), dim3(n), dim3(NT), 0, s, T, M, mat, U, cells, (TH *)Hq, Jd)
+  if (deg == 1) { if (h32) GLH_MFS(1, float); else GLH_MFS(1, double); }
+  else { if (h32) GLH_MFS(2, float); else GLH_MFS(2, double); }
+#undef GLH_MFS
   return hipGetLastError();
 }
 
-hipError_t launch_mf_apply(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *Hq,
-                           const double *x, const int *cells, int n, double *ycell, hipStream_t s)
+hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, const MeshDev &M, const Material &mat,
+                           const void *Hq, const void *x, const int *cells, int n, double *ycell, hipStream_t s)
 {
   if (n <= 0) return hipSuccess;
-  if (deg == 1) hipLaunchKernelGGL(k_mf_apply<1>, dim3(n), dim3(NTA), 0, s, T, M, mat, Hq, x, cells, ycell);
-  else hipLaunchKernelGGL(k_mf_apply<2>, dim3(n), dim3(NTA), 0, s, T, M, mat, Hq, x, cells, ycell);
+  if (x32 && !h32) return hipErrorInvalidValue;  // an FP64 Hessian is applied to FP64 vectors only
+#define GLH_MFA(D, TH, TX) \
+  hipLaunchKernelGGL((k_mf_apply<D, TH, TX>), dim3(n), dim3(NTA), 0, s, T, M, mat, (const TH *)Hq, (const TX *)x, cells, ycell)
+  if (deg == 1) {
+    if (h32) { if (x32) GLH_MFA(1, float, float); else GLH_MFA(1, float, double); }
+    else GLH_MFA(1, double, double);
+  } else {
+    if (h32) { if (x32) GLH_MFA(2, float, float); else GLH_MFA(2, float, double); }
+    else GLH_MFA(2, double, double);
+  }
+#undef GLH_MFA
   return hipGetLastError();
 }
 
+template <typename TX>
 hipError_t launch_mf_gather(int n_nodes, const int *ptr, const int *idx, const double *ycell, const uint32_t *mask,
-                            const double *Jd, const double *x, double *y, hipStream_t s)
+                            const double *Jd, const TX *x, const TX *b, TX *y, hipStream_t s)
 {
   if (n_nodes <= 0) return hipSuccess;
   const long long n = (long long)n_nodes * 18;
-  hipLaunchKernelGGL(k_mf_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_nodes, ptr, idx, ycell, mask,
-                     Jd, x, y);
+  hipLaunchKernelGGL((k_mf_gather<TX>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_nodes, ptr, idx, ycell,
+                     mask, Jd, x, b, y);
   return hipGetLastError();
 }
+template hipError_t launch_mf_gather<double>(int, const int *, const int *, const double *, const uint32_t *,
+                                             const double *, const double *, const double *, double *, hipStream_t);
+template hipError_t launch_mf_gather<float>(int, const int *, const int *, const double *, const uint32_t *,
+                                            const double *, const float *, const float *, float *, hipStream_t);
 
 hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
                          int n_cells, double *Fcell, hipStream_t s)

@@ -140,6 +140,7 @@ struct glh_ctx {
   // quadrature points and the diagonal blocks, in place of the BSR values
   bool matfree = false, j_mf = false;
   DBuf<double> Hq64, Jd64;
+  DBuf<float> Hq32;  // the stored Hessian of an FP32 matrix-free Jacobian
   DBuf<int> diag_id;  // node -> its block in Jd64 (the identity)
   DBuf<int> n2c_ptr, n2c_idx;  // node -> its (cell * ns + local node) entries
   DBuf<double> mf_ycell;       // per-cell contributions of a product [n_cells][ns][18]
@@ -374,8 +375,9 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
 {
   void *J = nullptr;
   if (do_j && mf) {  // matrix-free: H at the quadrature points and the diagonal blocks
-    if (fp32 || c->constrained || c->n_seg != 1) return fail(c, GLH_ERR_STATE, "matrix-free Jacobian: FP64, one instance, no constraints");
-    HIPCHK(c, c->Hq64.alloc((size_t)c->n_cells * c->nq * 171));
+    if (c->constrained || c->n_seg != 1) return fail(c, GLH_ERR_STATE, "matrix-free Jacobian: one instance, no constraints");
+    if (fp32) HIPCHK(c, c->Hq32.alloc((size_t)c->n_cells * c->nq * 171));
+    else HIPCHK(c, c->Hq64.alloc((size_t)c->n_cells * c->nq * 171));
     HIPCHK(c, c->mf_ycell.alloc((size_t)c->n_cells * c->ns * NC));
     HIPCHK(c, c->Jd64.alloc((size_t)c->n_nodes * 324));
     if (c->diag_id_n != c->n_nodes) {
@@ -411,8 +413,8 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
   for (int k = 0; k < ncol; ++k) {
     const int off = c->color_ptr[k], n = c->color_ptr[k + 1] - off;
     if (do_j && mf)
-      HIPCHK(c, glh::launch_mf_setup(c->deg, T, M, c->mat, Ustate, c->color_cells.p + off, n, c->Hq64.p, c->Jd64.p,
-                                     c->stream));
+      HIPCHK(c, glh::launch_mf_setup(c->deg, fp32, T, M, c->mat, Ustate, c->color_cells.p + off, n,
+                                     fp32 ? (void *)c->Hq32.p : (void *)c->Hq64.p, c->Jd64.p, c->stream));
     if (do_r || (do_j && !mf))
       HIPCHK(c, glh::launch_assemble(c->deg, do_j && !mf, do_r, fp32, T, M, c->mat, Ustate, c->color_cells.p + off,
                                      n, Jcell, rcell, c->stream));
@@ -460,7 +462,12 @@ int ensure_precond(glh_ctx *c)
   HIPCHK(c, c->status.alloc(1));
   HIPCHK(c, hipMemsetAsync(c->status.p, 0, sizeof(int), c->stream));
   const size_t nv = (size_t)c->n_nodes * 324;
-  if (c->j_fp32) {
+  if (c->j_fp32 && c->j_mf) {  // FP64 inverses of the FP64 diagonal blocks, rounded to FP32
+    HIPCHK(c, c->D64.alloc(nv));
+    HIPCHK(c, c->D32.alloc(nv));
+    HIPCHK(c, glh::launch_bj_invert<double>(c->n_nodes, c->diag_id.p, c->Jd64.p, c->D64.p, c->status.p, c->stream));
+    HIPCHK(c, (glh::launch_convert<double, float>(nv, c->D64.p, c->D32.p, c->stream)));
+  } else if (c->j_fp32) {
     HIPCHK(c, c->D32.alloc(nv));
     HIPCHK(c, glh::launch_bj_invert<float>(c->n_nodes, c->diag.p, c->J32.p, c->D32.p, c->status.p, c->stream));
   } else {
@@ -521,21 +528,30 @@ int flush_events(glh_ctx *c)
   return GLH_OK;
 }
 
+// y = J x (b == null) or y = b - J x with the matrix-free Jacobian: every cell
+// at once into its own slot, then a fixed-order gather per node (no colours:
+// one launch, no read-modify-write of y).  The stored Hessian is FP32 for an
+// FP32 Jacobian, whatever the vector type.
+template <typename TV>
+int mf_product(glh_ctx *c, const TV *x, const TV *b, TV *y)
+{
+  const glh::RefTables T = c->tables();
+  const glh::MeshDev M = c->meshdev();
+  const bool h32 = c->j_fp32, x32 = std::is_same<TV, float>::value;
+  HIPCHK(c, glh::launch_mf_apply(c->deg, h32, x32, T, M, c->mat, h32 ? (const void *)c->Hq32.p : (const void *)c->Hq64.p,
+                                 x, c->color_cells.p, c->n_cells, c->mf_ycell.p, c->stream));
+  HIPCHK(c, glh::launch_mf_gather<TV>(c->n_nodes, c->n2c_ptr.p, c->n2c_idx.p, c->mf_ycell.p, c->node_mask.p,
+                                      c->Jd64.p, x, b, y, c->stream));
+  return GLH_OK;
+}
+
 template <typename TJ, typename TV>
 int apply_j(glh_ctx *c, const TJ *J, const TV *x, TV *y, const int *flag, glh::ActiveSet act)
 {
-  if constexpr (std::is_same<TJ, double>::value && std::is_same<TV, double>::value) {
-    if (c->j_mf) {  // one instance, every row iterating: flag and act are not needed
-      const glh::RefTables T = c->tables();
-      const glh::MeshDev M = c->meshdev();
-      // every cell at once into its own slot, then a fixed-order gather per
-      // node (no colours: one launch, no read-modify-write of y)
-      HIPCHK(c, glh::launch_mf_apply(c->deg, T, M, c->mat, c->Hq64.p, x, c->color_cells.p, c->n_cells,
-                                     c->mf_ycell.p, c->stream));
-      HIPCHK(c, glh::launch_mf_gather(c->n_nodes, c->n2c_ptr.p, c->n2c_idx.p, c->mf_ycell.p, c->node_mask.p,
-                                      c->Jd64.p, x, y, c->stream));
-      return GLH_OK;
-    }
+  if (c->j_mf) {  // one instance, every row iterating: flag and act are not needed
+    static_assert(std::is_same<TJ, TV>::value, "the matrix-free product keeps x and y in one type");
+    RCHK(mf_product<TV>(c, x, nullptr, y));
+    return GLH_OK;
   }
   HIPCHK(c, glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, x, y, c->stream, c->row_seg.p, flag,
                                      act));
@@ -718,7 +734,7 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
                                          c->gmjseg.p, c->gmits.p,    c->gmro.p,   c->gmfrozen.p, c->gmjlast.p,
                                          c->row_seg.p, c->d_node_off.p, c->act_counts.p, c->act_seg.p,
                                          c->act_rpref.p, c->act_chunk.p, c->chunks.p, c->seg_chunk.p,
-                                         c->row_ptr.p, c->col.p, c->Hq64.p, c->Jd64.p,
+                                         c->row_ptr.p, c->col.p, c->Hq64.p, c->Hq32.p, c->Jd64.p,
                                          (const void *)(size_t)c->j_mf,
                                          (const void *)(size_t)m, (const void *)(size_t)S,
                                          (const void *)(size_t)c->n_chunks, (const void *)(size_t)c->n_nodes,
@@ -864,8 +880,10 @@ int solve_impl(glh_ctx *c, const std::vector<double> &rtolv, int maxit, int rest
       for (int i = 0; i < S; ++i) its_seg[i] += is2[i];
       if (rc && rc != GLH_ERR_NOCONV) return rc;
       HIPCHK(c, (glh::launch_axpy<float, double>(n, 1.0, c->gx32.p, c->dU.p, s)));
-      HIPCHK(c, glh::launch_residual_spmv<float>(c->n_nodes, c->row_ptr.p, c->col.p, c->J32.p, c->dU.p, c->rhs.p,
-                                                 c->ir_r.p, s));
+      if (c->j_mf) RCHK(mf_product<double>(c, c->dU.p, c->rhs.p, c->ir_r.p));
+      else
+        HIPCHK(c, glh::launch_residual_spmv<float>(c->n_nodes, c->row_ptr.p, c->col.p, c->J32.p, c->dU.p, c->rhs.p,
+                                                   c->ir_r.p, s));
       RCHK(seg_norm2(c, c->ir_r.p, rn2));
     }
     rel.assign(S, 0.0);
@@ -1626,7 +1644,7 @@ int glh_assemble(glh_ctx *c, int flags)
   const bool dj = flags & GLH_ASM_J, dr = flags & GLH_ASM_R, f32 = flags & GLH_ASM_FP32;
   const bool mf = flags & GLH_ASM_MATRIX_FREE;
   if (!dj && !dr) return fail(c, GLH_ERR_ARG, "nothing to assemble");
-  if (mf && (!dj || f32)) return fail(c, GLH_ERR_ARG, "the matrix-free form is an FP64 Jacobian");
+  if (mf && !dj) return fail(c, GLH_ERR_ARG, "the matrix-free form is a Jacobian form");
   RCHK(assemble_at(c, c->U.p, dj, dr, f32, c->rhs.p, true, mf));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return GLH_OK;
@@ -1733,7 +1751,7 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
   const bool f32 = fp32 != 0;
   // the matrix-free operator where it applies (FP64, one instance, no
   // constraints, no continuation shift); the assembled BSR Jacobian otherwise
-  const bool mf = c->matfree && !f32 && !c->constrained && S == 1 && !(c->ptc_sigma0 > 0);
+  const bool mf = c->matfree && !c->constrained && S == 1 && !(c->ptc_sigma0 > 0);
   std::vector<int> flag(S);
   // batches only: an instance whose ||R|| has fallen by less than 10% over the
   // last STALL_WINDOW steps is stopped and reported as not converged, so that
@@ -2032,8 +2050,7 @@ int glh_bench_spmv(glh_ctx *c, int k, double *ms_avg)
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   for (int i = 0; i < k; ++i) {
     if (c->j_fp32)
-      HIPCHK(c, (glh::launch_spmv<float, float>(c->n_nodes, c->row_ptr.p, c->col.p, c->J32.p, c->gZ32.p,
-                                                c->gW32.p, c->stream)));
+      RCHK((apply_j<float, float>(c, c->J32.p, c->gZ32.p, c->gW32.p, nullptr, glh::ActiveSet{})));
     else
       RCHK((apply_j<double, double>(c, c->J64.p, c->U.p, c->res.p, nullptr, glh::ActiveSet{})));
   }

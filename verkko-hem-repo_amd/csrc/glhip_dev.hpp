@@ -83,12 +83,15 @@ hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefT
 // each listed cell's contribution to its slot of ycell ([n_cells][NS][18]),
 // and launch_mf_gather sums them per node (node -> cell*NS + a lists, fixed
 // order) and writes the Dirichlet rows.
-hipError_t launch_mf_setup(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
-                           const int *cells, int n, double *Hq, double *Jd, hipStream_t s);
-hipError_t launch_mf_apply(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *Hq,
-                           const double *x, const int *cells, int n, double *ycell, hipStream_t s);
+// h32: the Hessian stored in FP32 (for the FP32 inner solve); x32: x in FP32.
+// The gather returns y = J x, or y = b - J x with b != null.
+hipError_t launch_mf_setup(int deg, bool h32, const RefTables &T, const MeshDev &M, const Material &mat,
+                           const double *U, const int *cells, int n, void *Hq, double *Jd, hipStream_t s);
+hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, const MeshDev &M, const Material &mat,
+                           const void *Hq, const void *x, const int *cells, int n, double *ycell, hipStream_t s);
+template <typename TX>
 hipError_t launch_mf_gather(int n_nodes, const int *ptr, const int *idx, const double *ycell, const uint32_t *mask,
-                            const double *Jd, const double *x, double *y, hipStream_t s);
+                            const double *Jd, const TX *x, const TX *b, TX *y, hipStream_t s);
 hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
                          int n_cells, double *Fcell, hipStream_t s);
 
