@@ -653,6 +653,7 @@ def main():
         el32 = time.perf_counter() - t1
         U32 = ctx.solution()
         line["mixed_precision"] = {
+            "operator": "matrix-free" if mf else "assembled BSR",
             "ms_per_solve": 1000.0 * el32, "converged": bool(s32["r_final"] <= 1e-10 * s32["r0"]),
             "newton_steps": s32["newton_steps"], "gmres_its": s32["gmres_its"], "free_energy": s32["F"],
             "F_rel_err_vs_fp64": abs(s32["F"] - st0["F"]) / abs(st0["F"]),
