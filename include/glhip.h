@@ -211,7 +211,8 @@ int glh_prm_get_double(const glh_prm *p, const char *subsection, const char *key
  * refinment ratio") or global refinement between cycles.  rows[max_rows][6]:
  * cycle, cells, dofs, Newton iterations, final ||R||, free energy; *n_rows
  * the number written.  GLH_ERR_NOCONV when a linear solve fails, as the
- * reference's SolverControl::NoConvergence ends its run. */
+ * reference's SolverControl::NoConvergence ends its run.  Sets the context's
+ * matrix-free switch (glh_set_matrix_free) from the glhip key "matrix free". */
 int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int *n_rows);
 
 /* ------------------------------------------- coordinate-MLP warm start */

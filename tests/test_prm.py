@@ -69,6 +69,7 @@ end
     assert p.get_double("control parameters", "Cycle 2 linear solver tol") == 0.1
     assert p.get("glhip", "initial guess") == "BnA"
     assert p.get("glhip", "boundary ids") == "1 1 1 1 0 0"
+    assert p.get("glhip", "matrix free") == "true"  # default
 
 
 @pytest.mark.parametrize("text,what", [

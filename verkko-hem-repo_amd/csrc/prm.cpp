@@ -85,6 +85,7 @@ void declare(std::map<std::string, Entry> &m)
   d(X, "random seed", "0", 'i');
   d(X, "noise amplitude", "0.05", 'd');
   d(X, "mixed precision", "false", 'b');
+  d(X, "matrix free", "true", 'b');                  // matrix-free Jacobian where it applies (glh_set_matrix_free)
   d(X, "GMRES restart", "100", 'i');
   d(X, "output directory", "", 's');                 // output_results per iteration (empty: none)
   d(X, "verbose", "false", 'b');                     // progress lines of run() on stdout
@@ -233,6 +234,7 @@ int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int 
   const int n_glob = (int)p->i(C, "Number of initial global refinments");
   const int deg = (int)p->i(X, "finite element degree");
   const int fp32 = p->b(X, "mixed precision");
+  if (const int rmf = glh_set_matrix_free(ctx, p->b(X, "matrix free"))) return rmf;
   const int restart = (int)std::max(1L, std::min(126L, p->i(X, "GMRES restart")));
   if (deg != 1 && deg != 2) return GLH_ERR_ARG;
   const std::string geo = p->s(X, "geometry"), guess = p->s(X, "initial guess");
