@@ -170,6 +170,7 @@ def run_batch(G, args):
     ctx.set_solution(np.concatenate(Us))
     ctx.store_initial()
     ctx.set_continuation(args.continuation)
+    ctx.set_matrix_free((args.operator == "matrix-free"))
     out = {}
     tuned = {}
     for prec in ("fp64", "fp32"):
@@ -199,6 +200,7 @@ def run_batch(G, args):
     # another; a seeded random sample when fewer than all are requested
     single = G.Context(0)
     single.set_continuation(args.continuation)
+    single.set_matrix_free((args.operator == "matrix-free"))
     nseq = min(len(meshes), args.sequential)
     sample = np.arange(len(meshes)) if nseq == len(meshes) else np.sort(
         np.random.default_rng(7).choice(len(meshes), nseq, replace=False))
@@ -252,7 +254,8 @@ def run_batch(G, args):
                                "%d DoFs total), T/Tc U[0.3,0.95], p U[0,34) bar, D U[6,40], seed 2; FP32 J + FP32 "
                                "GMRES in FP64 iterative refinement, FP64 residual and line search, one batched "
                                "launch per operation" % (len(meshes), n_dofs),
-                   "name": "slab_batch%d" % len(meshes), "dofs": n_dofs, "instances": len(meshes)},
+                   "name": "slab_batch%d" % len(meshes), "dofs": n_dofs, "instances": len(meshes),
+                   "operator": "matrix-free" if args.operator == "matrix-free" else "assembled BSR"},
         "converged_instances": int(conv32.sum()),
         "p50_instance_s": float(np.percentile(t_conv[t_conv >= 0], 50)) if (t_conv >= 0).any() else None,
         "p99_instance_s": float(np.percentile(t_conv[t_conv >= 0], 99)) if (t_conv >= 0).any() else None,
@@ -345,7 +348,8 @@ def run_warm_start(G, args):
     for b0 in range(0, len(train), 256):
         part = train[b0:b0 + 256]
         guesses = [WS.plain_guess(G, it, 3000 + b0 + i) for i, it in enumerate(part)]
-        st, ist, F, Us = WS.solve_batch(G, part, guesses, fp32=True, rtol=1e-8, lin_maxit=args.lin_maxit)
+        st, ist, F, Us = WS.solve_batch(G, part, guesses, fp32=True, rtol=1e-8, lin_maxit=args.lin_maxit,
+                                       matrix_free=(args.operator == "matrix-free"))
         ok = ist[:, 1] == 1
         n_conv += int(ok.sum())
         X, Y = WS.training_set(part, Us, ok)
@@ -366,7 +370,7 @@ def run_warm_start(G, args):
     # 3. held-out instances
     test = WS.slab_instances(G, args.instances, 4)
     plain = [WS.plain_guess(G, it, 4000 + i) for i, it in enumerate(test)]
-    kw = dict(rtol=1e-10, lin_maxit=args.lin_maxit)
+    kw = dict(rtol=1e-10, lin_maxit=args.lin_maxit, matrix_free=(args.operator == "matrix-free"))
     # Both batched FP32 paths use Eisenstat-Walker forcing and stop at
     # ||R|| <= 1e-10 ||R(plain guess)||, the same absolute accuracy; each
     # timed region holds the upload, the first residuals and the solve.
@@ -413,7 +417,8 @@ def run_warm_start(G, args):
                                "%d slab solutions (seed 3); %d held-out instances (seed 4), batched FP32 "
                                "Newton-GMRES, B-phase guess as fallback" % (args.train_steps, args.train_batch,
                                                                             n_conv, len(test)),
-                   "name": "warm_start", "train_points": int(X.shape[0])},
+                   "name": "warm_start", "train_points": int(X.shape[0]),
+                   "operator": "matrix-free" if args.operator == "matrix-free" else "assembled BSR"},
         "newton_steps_mean": {"learned": float(ist_w[:, 0].mean()), "plain": float(ist_p[:, 0].mean())},
         "newton_steps_lockstep": {"learned": st_w["newton_steps"], "plain": st_p["newton_steps"]},
         "gmres_its_lockstep": {"learned": st_w["gmres_its"], "plain": st_p["gmres_its"]},
@@ -483,14 +488,17 @@ def main():
                     help="slab_batch: pseudo-transient continuation sigma0 for the batched and the one-at-a-time "
                          "solves (0: plain damped Newton)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--assembled", action="store_true",
-                    help="headline solve with the assembled BSR Jacobian instead of the matrix-free one")
+    ap.add_argument("--operator", choices=["matrix-free", "assembled"], default=None,
+                    help="Jacobian form of the solves; default: the faster one measured per workload "
+                         "(matrix-free for the cube and warm_start, assembled for slab_batch)")
     ap.add_argument("--no-assembled", action="store_true",
                     help="skip the assembled-Jacobian solve reported beside the matrix-free headline")
     ap.add_argument("--kernels", action="store_true",
                     help="short fixed kernel sequence for rocprofv3: 3 FP64 and 3 FP32 J+R assemblies, 3 residual "
                          "assemblies, 10 FP64 and 10 FP32 SpMVs, 5 copies of 1 GiB (counter calibration)")
     args = ap.parse_args()
+    if args.operator is None:
+        args.operator = "assembled" if args.workload == "slab_batch" else "matrix-free"
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
               file=sys.stderr)
@@ -538,7 +546,7 @@ def main():
         ctx.close()
         return
     kw = dict(rtol=1e-10, max_newton=100, lin_tol=args.lin_tol, lin_maxit=100000, restart=args.restart)
-    mf = not args.assembled
+    mf = (args.operator == "matrix-free")
     ctx.set_matrix_free(mf)
     for w in range(args.warmup):
         ctx.restore_initial()

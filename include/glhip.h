@@ -304,7 +304,7 @@ int glh_apply_jacobian(glh_ctx *ctx, const double *x_host, double *y_host);
  * the current solution, with constrained rows condensed as
  * AffineConstraints::distribute_local_to_global does. flags: GLH_ASM_*;
  * with GLH_ASM_MATRIX_FREE the Jacobian is kept in the matrix-free form of
- * glh_set_matrix_free (one instance, no constraints) instead of BSR values. */
+ * glh_set_matrix_free (no constraints) instead of BSR values. */
 int glh_assemble(glh_ctx *ctx, int flags);
 /* ||system_rhs|| after glh_assemble with GLH_ASM_R. */
 int glh_rhs_norm(glh_ctx *ctx, double *norm);
@@ -397,9 +397,9 @@ int glh_set_continuation(glh_ctx *ctx, double sigma0);
  * J x cell by cell from them (the same operator as the assembled BSR matrix,
  * to rounding).  With fp32 the Hessian is stored in FP32 and the inner GMRES,
  * its Krylov basis and the block Jacobi run in FP32, inside the FP64
- * iterative refinement.  It applies to solves of one instance on meshes
- * without hanging-node or periodic constraints and without continuation;
- * other solves assemble J as before.  glh_assemble always assembles the BSR values;
+ * iterative refinement.  It applies to solves (one instance or a batch) on
+ * meshes without hanging-node or periodic constraints and without
+ * continuation; other solves assemble J as before.  glh_assemble always assembles the BSR values;
  * glh_get_jacobian fails after a matrix-free step.  0 (default) = off. */
 int glh_set_matrix_free(glh_ctx *ctx, int on);
 int glh_set_verbose(glh_ctx *ctx, int level);

@@ -92,6 +92,38 @@ def test_batch_newton_equals_single_instance_solves(ctx):
         single.close()
 
 
+@pytest.mark.parametrize("fp32", [False, True])
+def test_matrix_free_batch_reaches_the_assembled_batch_fixed_point(ctx, fp32):
+    # the matrix-free Jacobian on a batch: finished instances are skipped by
+    # the setup and the product, as by the assembly and the batched SpMV
+    meshes, batch, cos, Us = _batch(ctx, CASES)
+    U0 = ctx.solution()
+    kw = dict(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=40, fp32=fp32)
+    st_a = ctx.newton_solve(**kw)
+    ist_a, F_a, U_a = ctx.instance_stats(), ctx.instance_energy(), ctx.solution()
+    ctx.set_solution(U0)
+    ctx.set_matrix_free(True)
+    try:
+        st_m = ctx.newton_solve(**kw)
+    finally:
+        ctx.set_matrix_free(False)
+    ist_m, F_m, U_m = ctx.instance_stats(), ctx.instance_energy(), ctx.solution()
+    say("assembled", st_a)
+    say("matrix-free", st_m)
+    assert np.all(ist_a[:, 1] == 1) and np.all(ist_m[:, 1] == 1)
+    assert np.all(np.abs(ist_m[:, 0] - ist_a[:, 0]) <= 1)
+    assert np.all(np.abs(F_m - F_a) <= 1e-10 * np.abs(F_a))
+    if not fp32:
+        assert np.abs(U_m - U_a).max() <= 1e-8 * np.abs(U_a).max()
+    else:
+        # FP32 inner solves drift along the slabs' nearly flat directions (the
+        # rotations of the order parameter that the two walls leave free), as
+        # the FP32 batch test above allows: the energies and the FP64
+        # residuals are the fixed point's
+        R = ctx.instance_residuals()
+        assert np.all(R <= 1e-10 * ist_m[:, 2] * (1 + 1e-6))
+
+
 def test_autotuned_schedule_restores_state_and_keeps_the_fixed_point(ctx):
     meshes, batch, cos, Us = _batch(ctx, CASES)
     U0 = ctx.solution()

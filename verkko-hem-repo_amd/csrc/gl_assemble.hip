@@ -414,6 +414,9 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
   __shared__ double sGram[NQ * GRAM];
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
+  // a finished instance of a batch keeps the Hessian and blocks of its last
+  // step (all of its cells are skipped together)
+  if (Mh.inst_active && !Mh.inst_active[Mh.cell_inst[e]]) return;
   const Material M = Mh.mats ? Mh.mats[Mh.cell_mat[e]] : M0;
   const int *cn = Mh.cell_nodes + (size_t)e * NS;
   double h[3], ih[3];
@@ -498,7 +501,7 @@ constexpr int NTA = 256;  // threads per workgroup of the matrix-free product
 template <int DEG, typename TH, typename TX>
 __global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Material M0, const TH *__restrict__ Hq,
                                                   const TX *__restrict__ x, const int *__restrict__ cells,
-                                                  double *__restrict__ ycell)
+                                                  const int *__restrict__ flag, double *__restrict__ ycell)
 {
   constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS, V = NS * 18;
   __shared__ double sx[V];      // x at the cell's nodes, Dirichlet components 0
@@ -507,6 +510,7 @@ __global__ __launch_bounds__(NTA) void k_mf_apply(RefTables T, MeshDev Mh, Mater
   __shared__ double sL[N1 * N1], sdL[N1 * N1];
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
+  if (flag && !flag[Mh.cell_inst[e]]) return;  // instance not iterating: its rows are not read
   const Material M = Mh.mats ? Mh.mats[Mh.cell_mat[e]] : M0;
   const int *cn = Mh.cell_nodes + (size_t)e * NS;
   double h[3], ih[3];
@@ -655,11 +659,12 @@ template <typename TX>
 __global__ void k_mf_gather(int n_nodes, const int *__restrict__ ptr, const int *__restrict__ idx,
                             const double *__restrict__ ycell, const uint32_t *__restrict__ mask,
                             const double *__restrict__ Jd, const TX *__restrict__ x, const TX *__restrict__ b,
-                            TX *__restrict__ y)
+                            const int *__restrict__ row_seg, const int *__restrict__ flag, TX *__restrict__ y)
 {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)n_nodes * 18) return;
   const int node = (int)(i / 18), c = (int)(i % 18);
+  if (flag && !flag[row_seg[node]]) return;  // instance not iterating: its rows are left as they are
   double s = 0.0;
   if ((mask[node] >> c) & 1u) {
     s = Jd[(size_t)node * 324 + c * 19] * (double)x[i];
@@ -812,12 +817,14 @@ hipError_t launch_mf_setup(int deg, bool h32, const RefTables &T, const MeshDev 
 }
 
 hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, const MeshDev &M, const Material &mat,
-                           const void *Hq, const void *x, const int *cells, int n, double *ycell, hipStream_t s)
+                           const void *Hq, const void *x, const int *cells, int n, const int *flag, double *ycell,
+                           hipStream_t s)
 {
   if (n <= 0) return hipSuccess;
   if (x32 && !h32) return hipErrorInvalidValue;  // an FP64 Hessian is applied to FP64 vectors only
 #define GLH_MFA(D, TH, TX) \
-  hipLaunchKernelGGL((k_mf_apply<D, TH, TX>), dim3(n), dim3(NTA), 0, s, T, M, mat, (const TH *)Hq, (const TX *)x, cells, ycell)
+  hipLaunchKernelGGL((k_mf_apply<D, TH, TX>), dim3(n), dim3(NTA), 0, s, T, M, mat, (const TH *)Hq, (const TX *)x, cells, \
+                     flag, ycell)
   if (deg == 1) {
     if (h32) { if (x32) GLH_MFA(1, float, float); else GLH_MFA(1, float, double); }
     else GLH_MFA(1, double, double);
@@ -831,18 +838,21 @@ hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, cons
 
 template <typename TX>
 hipError_t launch_mf_gather(int n_nodes, const int *ptr, const int *idx, const double *ycell, const uint32_t *mask,
-                            const double *Jd, const TX *x, const TX *b, TX *y, hipStream_t s)
+                            const double *Jd, const TX *x, const TX *b, const int *row_seg, const int *flag, TX *y,
+                            hipStream_t s)
 {
   if (n_nodes <= 0) return hipSuccess;
   const long long n = (long long)n_nodes * 18;
   hipLaunchKernelGGL((k_mf_gather<TX>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_nodes, ptr, idx, ycell,
-                     mask, Jd, x, b, y);
+                     mask, Jd, x, b, row_seg, flag, y);
   return hipGetLastError();
 }
 template hipError_t launch_mf_gather<double>(int, const int *, const int *, const double *, const uint32_t *,
-                                             const double *, const double *, const double *, double *, hipStream_t);
+                                             const double *, const double *, const double *, const int *,
+                                             const int *, double *, hipStream_t);
 template hipError_t launch_mf_gather<float>(int, const int *, const int *, const double *, const uint32_t *,
-                                            const double *, const float *, const float *, float *, hipStream_t);
+                                            const double *, const float *, const float *, const int *, const int *,
+                                            float *, hipStream_t);
 
 hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
                          int n_cells, double *Fcell, hipStream_t s)

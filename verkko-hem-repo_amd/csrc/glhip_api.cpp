@@ -375,7 +375,7 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
 {
   void *J = nullptr;
   if (do_j && mf) {  // matrix-free: H at the quadrature points and the diagonal blocks
-    if (c->constrained || c->n_seg != 1) return fail(c, GLH_ERR_STATE, "matrix-free Jacobian: one instance, no constraints");
+    if (c->constrained) return fail(c, GLH_ERR_STATE, "matrix-free Jacobian: mesh without constraints");
     if (fp32) HIPCHK(c, c->Hq32.alloc((size_t)c->n_cells * c->nq * 171));
     else HIPCHK(c, c->Hq64.alloc((size_t)c->n_cells * c->nq * 171));
     HIPCHK(c, c->mf_ycell.alloc((size_t)c->n_cells * c->ns * NC));
@@ -533,24 +533,24 @@ int flush_events(glh_ctx *c)
 // one launch, no read-modify-write of y).  The stored Hessian is FP32 for an
 // FP32 Jacobian, whatever the vector type.
 template <typename TV>
-int mf_product(glh_ctx *c, const TV *x, const TV *b, TV *y)
+int mf_product(glh_ctx *c, const TV *x, const TV *b, TV *y, const int *flag = nullptr)
 {
   const glh::RefTables T = c->tables();
   const glh::MeshDev M = c->meshdev();
   const bool h32 = c->j_fp32, x32 = std::is_same<TV, float>::value;
   HIPCHK(c, glh::launch_mf_apply(c->deg, h32, x32, T, M, c->mat, h32 ? (const void *)c->Hq32.p : (const void *)c->Hq64.p,
-                                 x, c->color_cells.p, c->n_cells, c->mf_ycell.p, c->stream));
+                                 x, c->color_cells.p, c->n_cells, flag, c->mf_ycell.p, c->stream));
   HIPCHK(c, glh::launch_mf_gather<TV>(c->n_nodes, c->n2c_ptr.p, c->n2c_idx.p, c->mf_ycell.p, c->node_mask.p,
-                                      c->Jd64.p, x, b, y, c->stream));
+                                      c->Jd64.p, x, b, c->row_seg.p, flag, y, c->stream));
   return GLH_OK;
 }
 
 template <typename TJ, typename TV>
 int apply_j(glh_ctx *c, const TJ *J, const TV *x, TV *y, const int *flag, glh::ActiveSet act)
 {
-  if (c->j_mf) {  // one instance, every row iterating: flag and act are not needed
+  if (c->j_mf) {  // every cell of the iterating instances (flag); act only narrows the SpMV's grid
     static_assert(std::is_same<TJ, TV>::value, "the matrix-free product keeps x and y in one type");
-    RCHK(mf_product<TV>(c, x, nullptr, y));
+    RCHK(mf_product<TV>(c, x, nullptr, y, flag));
     return GLH_OK;
   }
   HIPCHK(c, glh::launch_spmv<TJ, TV>(c->n_nodes, c->row_ptr.p, c->col.p, J, x, y, c->stream, c->row_seg.p, flag,
@@ -1751,7 +1751,7 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
   const bool f32 = fp32 != 0;
   // the matrix-free operator where it applies (FP64, one instance, no
   // constraints, no continuation shift); the assembled BSR Jacobian otherwise
-  const bool mf = c->matfree && !c->constrained && S == 1 && !(c->ptc_sigma0 > 0);
+  const bool mf = c->matfree && !c->constrained && !(c->ptc_sigma0 > 0);
   std::vector<int> flag(S);
   // batches only: an instance whose ||R|| has fallen by less than 10% over the
   // last STALL_WINDOW steps is stopped and reported as not converged, so that

@@ -87,11 +87,15 @@ hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefT
 // The gather returns y = J x, or y = b - J x with b != null.
 hipError_t launch_mf_setup(int deg, bool h32, const RefTables &T, const MeshDev &M, const Material &mat,
                            const double *U, const int *cells, int n, void *Hq, double *Jd, hipStream_t s);
+// flag (per instance, or null): cells and rows of instances whose flag is 0
+// are skipped, as in the batched SpMV.
 hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, const MeshDev &M, const Material &mat,
-                           const void *Hq, const void *x, const int *cells, int n, double *ycell, hipStream_t s);
+                           const void *Hq, const void *x, const int *cells, int n, const int *flag, double *ycell,
+                           hipStream_t s);
 template <typename TX>
 hipError_t launch_mf_gather(int n_nodes, const int *ptr, const int *idx, const double *ycell, const uint32_t *mask,
-                            const double *Jd, const TX *x, const TX *b, TX *y, hipStream_t s);
+                            const double *Jd, const TX *x, const TX *b, const int *row_seg, const int *flag, TX *y,
+                            hipStream_t s);
 hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
                          int n_cells, double *Fcell, hipStream_t s);
 

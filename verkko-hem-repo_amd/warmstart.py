@@ -57,7 +57,7 @@ def plain_guess(G, inst, seed):
 
 
 def solve_batch(G, insts, guesses, fp32=True, rtol=1e-10, lin_tol=0.1, lin_maxit=3000, restart=100,
-                max_newton=60, verbose=0, ew=False, r_ref=None):
+                max_newton=60, verbose=0, ew=False, r_ref=None, matrix_free=False):
     """Batched Newton solve; returns (stats, per-instance stats, F, U list).
     ew: Eisenstat-Walker forcing; r_ref: per-instance stopping reference
     (||R|| <= rtol r_ref), e.g. the first residuals of the plain guesses, so
@@ -69,6 +69,7 @@ def solve_batch(G, insts, guesses, fp32=True, rtol=1e-10, lin_tol=0.1, lin_maxit
         ctx.upload_mesh(batch)
         ctx.set_materials(np.array([it.coef for it in insts]))
         ctx.set_forcing(ew)
+        ctx.set_matrix_free(matrix_free)
         if r_ref is not None:
             ctx.set_residual_reference(r_ref)
         ctx.set_solution(np.concatenate(guesses))
@@ -86,7 +87,7 @@ def solve_batch(G, insts, guesses, fp32=True, rtol=1e-10, lin_tol=0.1, lin_maxit
 
 
 def solve_from_candidates(G, insts, candidates, fp32=True, rtol=1e-10, lin_tol=0.1, lin_maxit=3000, restart=100,
-                          max_newton=60, ew=True, verbose=0):
+                          max_newton=60, ew=True, verbose=0, matrix_free=False):
     """One context for the whole batch: the first residual of every candidate
     guess set (candidates[0] is the plain guess), per instance the candidate
     with the lowest residual (the plain guess wins ties: it is the fallback),
@@ -108,6 +109,7 @@ def solve_from_candidates(G, insts, candidates, fp32=True, rtol=1e-10, lin_tol=0
         choice = np.argmin(res, axis=0)  # first minimum: the plain guess on ties
         guess = np.concatenate([candidates[c][i] for i, c in enumerate(choice)])
         ctx.set_forcing(ew)
+        ctx.set_matrix_free(matrix_free)
         ctx.set_residual_reference(res[0])
         ctx.set_solution(guess)
         st = ctx.newton_solve(rtol=rtol, max_newton=max_newton, lin_tol=lin_tol, lin_maxit=lin_maxit,
