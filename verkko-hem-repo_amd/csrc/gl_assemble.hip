@@ -490,11 +490,12 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
   }
 }
 
-// threads per workgroup of the matrix-free product: each thread takes at
-// most two of the cell's (point, component) items per pass (Q1: 144 items
-// on 128 threads, Q2: 486 on 256)
-template <int DEG>
-constexpr int mf_threads() { return DEG == 1 ? 128 : 256; }
+// threads per workgroup of the matrix-free product: at most two of the
+// cell's (point, component) items per thread and pass (Q1: 144 items on 128
+// threads; Q2: 486 on 256 with the FP64 Hessian, on 512 with the FP32 one,
+// each measured the faster of the two for its type)
+template <int DEG, typename TH>
+constexpr int mf_threads() { return DEG == 1 ? 128 : (sizeof(TH) == 4 ? 512 : 256); }
 
 // The product runs in sum-factorised form: N_b(q) = L_bx(x_qx) L_by(x_qy)
 // L_bz(x_qz), so x and its gradients at the quadrature points take three 1D
@@ -503,11 +504,11 @@ constexpr int mf_threads() { return DEG == 1 ? 128 : 256; }
 // TH: stored Hessian (FP64, or FP32 for the FP32 inner solve); TX: x.  The
 // arithmetic is FP64.
 template <int DEG, typename TH, typename TX>
-__global__ __launch_bounds__(mf_threads<DEG>()) void k_mf_apply(RefTables T, MeshDev Mh, Material M0, const TH *__restrict__ Hq,
+__global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables T, MeshDev Mh, Material M0, const TH *__restrict__ Hq,
                                                   const TX *__restrict__ x, const int *__restrict__ cells,
                                                   const int *__restrict__ flag, double *__restrict__ ycell)
 {
-  constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS, V = NS * 18, NTA = mf_threads<DEG>();
+  constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS, V = NS * 18, NTA = mf_threads<DEG, TH>();
   __shared__ double sx[V];      // x at the cell's nodes, Dirichlet components 0
   __shared__ double R1[5 * V];  // interpolation passes; then w vol H X and w vol gamma / h; then the y pass
   __shared__ double R2[4 * V];  // x and its gradients at the quadrature points; then the z pass
@@ -827,7 +828,7 @@ hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, cons
   if (n <= 0) return hipSuccess;
   if (x32 && !h32) return hipErrorInvalidValue;  // an FP64 Hessian is applied to FP64 vectors only
 #define GLH_MFA(D, TH, TX) \
-  hipLaunchKernelGGL((k_mf_apply<D, TH, TX>), dim3(n), dim3(mf_threads<D>()), 0, s, T, M, mat, (const TH *)Hq, (const TX *)x, cells, \
+  hipLaunchKernelGGL((k_mf_apply<D, TH, TX>), dim3(n), dim3((mf_threads<D, TH>())), 0, s, T, M, mat, (const TH *)Hq, (const TX *)x, cells, \
                      flag, ycell)
   if (deg == 1) {
     if (h32) { if (x32) GLH_MFA(1, float, float); else GLH_MFA(1, float, double); }
