@@ -504,9 +504,12 @@ constexpr int mf_threads() { return DEG == 1 ? 128 : (sizeof(TH) == 4 ? 512 : 25
 // TH: stored Hessian (FP64, or FP32 for the FP32 inner solve); TX: x.  The
 // arithmetic is FP64.
 template <int DEG, typename TH, typename TX>
-__global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables T, MeshDev Mh, Material M0, const TH *__restrict__ Hq,
-                                                  const TX *__restrict__ x, const int *__restrict__ cells,
-                                                  const int *__restrict__ flag, double *__restrict__ ycell)
+__global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables T, MeshDev Mh, Material M0,
+                                                                      const TH *__restrict__ Hq,
+                                                                      const TX *__restrict__ x,
+                                                                      const int *__restrict__ cells,
+                                                                      const int *__restrict__ flag,
+                                                                      double *__restrict__ ycell)
 {
   constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS, V = NS * 18, NTA = mf_threads<DEG, TH>();
   __shared__ double sx[V];      // x at the cell's nodes, Dirichlet components 0
