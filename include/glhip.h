@@ -93,6 +93,53 @@ int glh_mesh_sizes(const glh_mesh *m, int64_t sizes[8]);
 int glh_mesh_get(const glh_mesh *m, int *cell_nodes, double *cell_h, double *node_xyz, int *faces,
                  uint32_t *node_mask, int *color_ptr, int *color_cells, int *row_ptr, int *col);
 
+/* Mesh from the caller's own per-cycle arrays, for a caller that keeps the
+ * mesh, the DoF numbering and the constraints in deal.II: the data
+ * setup_system() builds (setup_uniform_B-phase.cc:125-198, the
+ * AffineConstraints of femgl.h:307-308 that assemble.cc:356-361 applies),
+ * passed as plain arrays.  Nodes are the FE_Q support points; node v owns
+ * dofs 18 v + c in the layout above, in the caller's node numbering.
+ *   degree        1 or 2; ns = (degree+1)^3 nodes per cell
+ *   local_order   GLH_ORDER_LEXICOGRAPHIC (x fastest) or GLH_ORDER_HIERARCHIC
+ *                 (FE_Q's support-point order: vertices, then the midpoints
+ *                 of the lines, faces and cell in GeometryInfo<3> numbering)
+ *   cell_nodes    [n_cells*ns] node of each local support point
+ *   cell_lo/h     [n_cells*3] lower corner and edge lengths of each box cell
+ *   node_xyz      [n_nodes*3] or NULL; when given, every cell must place its
+ *                 nodes there (checks the local order)
+ *   faces         [n_faces*3] (cell, local face 0..5 = -x +x -y +y -z +z,
+ *                 boundary id): 0 = A = 0 wall, 2/3/4 = AdGR wall (normal
+ *                 column Dirichlet + Robin term), others natural (periodic
+ *                 faces go through constraint lines)
+ *   node_mask     [n_nodes] Dirichlet component bits, or NULL to derive them
+ *                 from the faces' ids (femgl.h:294-301)
+ *   lines         n_lines node constraint lines U[line_node[l]] = sum_k
+ *                 line_w[k] U[line_master[k]], k in [line_ptr[l],
+ *                 line_ptr[l+1]), shared by the 18 components (hanging nodes,
+ *                 periodic images); closed by the library.  A constrained
+ *                 node's line takes precedence over its Dirichlet mask, as in
+ *                 the reference, which adds the hanging-node lines first.
+ * The library derives the colours and the BSR patterns (the condensed one of
+ * make_sparsity_pattern with constraints and the cell-loop one).  Upload the
+ * result with glh_upload_mesh. */
+#define GLH_ORDER_LEXICOGRAPHIC 0
+#define GLH_ORDER_HIERARCHIC 1
+typedef struct glh_mesh_desc {
+  int degree, n_cells, n_nodes, local_order;
+  const int *cell_nodes;
+  const double *cell_lo, *cell_h, *node_xyz;
+  int n_faces;
+  const int *faces;
+  const uint32_t *node_mask;
+  int n_lines;
+  const int *line_node, *line_ptr, *line_master;
+  const double *line_w;
+} glh_mesh_desc;
+int glh_mesh_from_arrays(const glh_mesh_desc *desc, glh_mesh **out);
+/* Message of the last failed glh_mesh_box / glh_mesh_from_arrays call on the
+ * calling thread (the mesh functions have no context to hold it). */
+const char *glh_mesh_last_error(void);
+
 /* node_off[n_inst+1], cell_off[n_inst+1] (either may be NULL) */
 int glh_mesh_instances(const glh_mesh *m, int *node_off, int *cell_off);
 
@@ -399,8 +446,9 @@ int glh_set_continuation(glh_ctx *ctx, double sigma0);
  * its Krylov basis and the block Jacobi run in FP32, inside the FP64
  * iterative refinement.  It applies to solves (one instance or a batch) on
  * meshes without hanging-node or periodic constraints and without
- * continuation; other solves assemble J as before.  glh_assemble always assembles the BSR values;
- * glh_get_jacobian fails after a matrix-free step.  0 (default) = off. */
+ * continuation; other solves assemble J as before.  glh_assemble builds the
+ * BSR values unless it is given GLH_ASM_MATRIX_FREE; glh_get_jacobian fails
+ * after a matrix-free step.  0 (default) = off. */
 int glh_set_matrix_free(glh_ctx *ctx, int on);
 int glh_set_verbose(glh_ctx *ctx, int level);
 /* Live kernel timing with HIP events on the context stream.  on != 0 resets

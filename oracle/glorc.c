@@ -932,6 +932,97 @@ int orc_assemble(int deg, int n_cells, const int *cell_nodes, const double *cell
   return rc;
 }
 
+/*
+ * Matrix-free check at scale: system_rhs = -R and Y = J X for nvec vectors
+ * (X, Y: nvec * n_nodes*18, vector-major) without forming the global J,
+ * cell by cell with the factorized cell form and the Robin faces, scattered
+ * as distribute_local_to_global does for homogeneous Dirichlet constraints
+ * only (node_mask bit c = dof node*18 + c constrained; meshes without
+ * hanging-node or periodic lines): constrained rows and columns drop out of
+ * the cell contributions, a constrained row receives the sum over its cells
+ * of |local diagonal| (local average when zero) times x, and a zero rhs
+ * (assemble.cc:356-361, residual.cc:286-289).  Cells of one colour share no
+ * node, so each colour runs in parallel over its cells (OpenMP) and the sums
+ * are in a fixed order.  rhs or Y may be NULL.  Returns 0, or -1 on a bad
+ * degree.
+ */
+int orc_cells_apply(int deg, int n_cells, const int *cell_nodes, const double *cell_h, int n_faces,
+                    const int *faces, int n_nodes, const uint32_t *node_mask, const double *co, const double *U,
+                    int n_colors, const int *color_ptr, const int *color_cells, int nvec, const double *X,
+                    double *Y, double *rhs)
+{
+  refel r0;
+  if (refel_init(&r0, deg)) return -1;
+  const int ns = r0.ns, dpc = ns * NC;
+  const size_t nd = (size_t)n_nodes * NC;
+  const int doJ = Y && nvec > 0;
+  if (rhs) memset(rhs, 0, sizeof(double) * nd);
+  if (doJ) memset(Y, 0, sizeof(double) * nd * nvec);
+  /* faces of each cell */
+  int *fptr = (int *)calloc((size_t)n_cells + 1, sizeof(int)), *fidx = (int *)malloc(sizeof(int) * (n_faces + 1));
+  for (int f = 0; f < n_faces; ++f) fptr[faces[3 * f] + 1]++;
+  for (int e = 0; e < n_cells; ++e) fptr[e + 1] += fptr[e];
+  {
+    int *pos = (int *)malloc(sizeof(int) * ((size_t)n_cells + 1));
+    memcpy(pos, fptr, sizeof(int) * ((size_t)n_cells + 1));
+    for (int f = 0; f < n_faces; ++f) fidx[pos[faces[3 * f]]++] = f;
+    free(pos);
+  }
+  for (int k = 0; k < n_colors; ++k) {
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+      double *Jl = doJ ? (double *)malloc(sizeof(double) * dpc * dpc) : NULL;
+      double Rl[27 * NC], Ul[27 * NC];
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 4)
+#endif
+      for (int ii = color_ptr[k]; ii < color_ptr[k + 1]; ++ii) {
+        const int e = color_cells[ii];
+        const int *cn = cell_nodes + (size_t)e * ns;
+        const double *h = cell_h + 3 * e;
+        for (int a = 0; a < ns; ++a)
+          for (int c = 0; c < NC; ++c) Ul[a * NC + c] = U[(size_t)cn[a] * NC + c];
+        orc_cell_factorized(deg, h, co, Ul, Jl, Rl);
+        for (int f = fptr[e]; f < fptr[e + 1]; ++f)
+          orc_face_faithful(deg, h, faces[3 * fidx[f] + 1], faces[3 * fidx[f] + 2], co, Ul, Jl, Rl);
+        double avg = 0;
+        if (doJ) {
+          for (int i = 0; i < dpc; ++i) avg += fabs(Jl[(size_t)i * dpc + i]);
+          avg /= dpc;
+        }
+        for (int i = 0; i < dpc; ++i) {
+          const int ni = cn[i / NC], ci = i % NC;
+          const size_t gi = (size_t)ni * NC + ci;
+          const int con_i = (node_mask[ni] >> ci) & 1u;
+          if (rhs && !con_i) rhs[gi] += Rl[i];
+          if (!doJ) continue;
+          if (con_i) {
+            double d = fabs(Jl[(size_t)i * dpc + i]);
+            if (d == 0) d = avg;
+            for (int v = 0; v < nvec; ++v) Y[v * nd + gi] += d * X[v * nd + gi];
+            continue;
+          }
+          for (int v = 0; v < nvec; ++v) {
+            double s = 0;
+            for (int j = 0; j < dpc; ++j) {
+              const int nj = cn[j / NC], cj = j % NC;
+              if ((node_mask[nj] >> cj) & 1u) continue;
+              s += Jl[(size_t)i * dpc + j] * X[v * nd + (size_t)nj * NC + cj];
+            }
+            Y[v * nd + gi] += s;
+          }
+        }
+      }
+      free(Jl);
+    }
+  }
+  free(fptr);
+  free(fidx);
+  return 0;
+}
+
 /* Free energy F = sum_cells sum_q JxW f(q) + Robin faces (K1/bt)|x~|^2. */
 double orc_free_energy(int deg, int n_cells, const int *cell_nodes, const double *cell_h, int n_faces,
                        const int *faces, const double *co, const double *U)

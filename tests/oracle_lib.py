@@ -54,6 +54,9 @@ def lib():
         L.orc_newton.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_int, _ip, C.c_int, _ip, _ip, _ip, _dp, _dp,
                                  _dp, _ip, _ip, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int,
                                  C.c_double, C.c_int, _dp]
+        _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+        L.orc_cells_apply.argtypes = [C.c_int, C.c_int, _ip, _dp, C.c_int, _ip, C.c_int, _u32p, _dp, _dp, C.c_int,
+                                      _ip, _ip, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -217,6 +220,28 @@ def assemble(mesh, co, U, J=True, R=True, faithful=False):
                             mesh.col, val, rhs)
     assert rc == 0, rc
     return (val.reshape(-1, 18, 18) if J else None), rhs
+
+
+def cells_apply(deg, arrays, n_nodes, co, U, X=(), rhs=True):
+    """system_rhs = -R and J x for each x in X cell by cell without a
+    global J (orc_cells_apply): meshes with Dirichlet masks only.  arrays:
+    the product mesh's arrays (cell_nodes, cell_h, faces, node_mask,
+    color_ptr, color_cells).  Returns (rhs or None, [J x ...])."""
+    a = arrays
+    nd = n_nodes * 18
+    Xs = np.ascontiguousarray(np.asarray(X, np.float64).reshape(-1, nd)) if len(X) else np.zeros((0, nd))
+    Y = np.zeros_like(Xs)
+    r = np.zeros(nd) if rhs else None
+    faces = np.ascontiguousarray(a["faces"], np.int32)
+    nc = len(a["cell_nodes"]) // (deg + 1) ** 3
+    ptr = lambda v: None if v is None or v.size == 0 else v.ctypes.data_as(C.c_void_p)
+    rc = lib().orc_cells_apply(deg, nc, i32(a["cell_nodes"]), f64(a["cell_h"]), len(faces) // 3,
+                               faces if len(faces) else np.zeros(3, np.int32), n_nodes,
+                               np.ascontiguousarray(a["node_mask"], np.uint32), f64(co), f64(U),
+                               len(a["color_ptr"]) - 1, i32(a["color_ptr"]), i32(a["color_cells"]), len(Xs),
+                               ptr(Xs), ptr(Y), ptr(r))
+    assert rc == 0, rc
+    return r, list(Y)
 
 
 def free_energy(mesh, co, U):

@@ -90,3 +90,128 @@ def oracle_adaptive(mesh, lo, hi, side_id):
     faces = a["faces"].reshape(-1, 3) if mesh.n_faces else np.zeros((0, 3), np.int32)
     return O.adaptive_oracle_mesh(a["cell_nodes"], xyz, cell_lo, a["cell_h"], [tuple(f) for f in faces], lo, hi,
                                   side_masks(side_id), deg=mesh.deg, period=period)
+
+
+# FE_Q(2) support points in deal.II's hierarchic dof order (vertices, lines,
+# faces, interior; GeometryInfo<3> numbering), as (x, y, z) in half cells
+HIER_Q2 = [(0, 0, 0), (2, 0, 0), (0, 2, 0), (2, 2, 0), (0, 0, 2), (2, 0, 2), (0, 2, 2), (2, 2, 2),
+           (0, 1, 0), (2, 1, 0), (1, 0, 0), (1, 2, 0), (0, 1, 2), (2, 1, 2), (1, 0, 2), (1, 2, 2),
+           (0, 0, 1), (2, 0, 1), (0, 2, 1), (2, 2, 1),
+           (0, 1, 1), (2, 1, 1), (1, 0, 1), (1, 2, 1), (1, 1, 0), (1, 1, 2), (1, 1, 1)]
+
+
+def caller_arrays(mesh, lo, hi, seed, hierarchic=False):
+    """The per-cycle arrays a deal.II caller would pass to
+    glh_mesh_from_arrays for this library mesh: node numbering randomly
+    permuted; on periodic axes (where the library identifies the nodes of the
+    two faces) the + face support points become nodes of their own with a
+    constraint line to their - face image (make_periodicity_constraints);
+    hanging-node lines renumbered; optionally FE_Q's hierarchic local order.
+    Returns (kwargs of Mesh.from_arrays, new id of every node -- the
+    library's nodes first, then the images --, and the library node each
+    image repeats)."""
+    a = mesh.arrays()
+    deg, ns = mesh.deg, mesh.ns
+    n1 = deg + 1
+    xyz = a["node_xyz"].reshape(-1, 3)
+    cn = a["cell_nodes"].reshape(-1, ns)
+    h = a["cell_h"].reshape(-1, 3)
+    clo = xyz[cn[:, 0]]
+    L = np.asarray(hi, float) - np.asarray(lo, float)
+    tol = 1e-9 * L.max()
+    # support point of every (cell, local node); + face images get new nodes
+    t = np.array([(i % n1, (i // n1) % n1, i // (n1 * n1)) for i in range(ns)], float) / deg
+    pos = clo[:, None, :] + h[:, None, :] * t[None, :, :]
+    nodes = list(range(mesh.n_nodes))
+    node_pos = [tuple(p) for p in xyz]
+    image = {}
+    img = []
+    lines = []
+    cells = cn.copy()
+    # a node that only + side cells touch (a hanging node of a coarse - side
+    # cell under a finer + side neighbour) moves to the + side instead of
+    # getting an image
+    shift = {}
+    at_home = np.zeros(mesh.n_nodes, bool)
+    for e in range(cn.shape[0]):
+        for i in range(ns):
+            v = cn[e, i]
+            d = tuple(np.round((pos[e, i] - xyz[v]) / L).astype(int))
+            if d == (0, 0, 0):
+                at_home[v] = True
+            else:
+                shift.setdefault(v, d)
+    for v, d in shift.items():
+        if not at_home[v]:
+            node_pos[v] = tuple(xyz[v] + np.asarray(d) * L)
+    for e in range(cn.shape[0]):
+        for i in range(ns):
+            v = cn[e, i]
+            if np.abs(pos[e, i] - np.asarray(node_pos[v])).max() <= tol:
+                continue
+            key = (v, tuple(np.round((pos[e, i] - np.asarray(node_pos[v])) / L).astype(int)))
+            if key not in image:
+                image[key] = len(nodes)
+                img.append(v)
+                nodes.append(len(nodes))
+                node_pos.append(tuple(pos[e, i]))
+                lines.append((image[key], [v], [1.0]))
+            cells[e, i] = image[key]
+    con = mesh.constraints()
+    hp, hm, hw = con["hang_ptr"], con["hang_m"], con["hang_w"]
+    for v in range(mesh.n_nodes):
+        if hp[v + 1] > hp[v]:
+            lines.append((v, list(hm[hp[v]:hp[v + 1]]), list(hw[hp[v]:hp[v + 1]])))
+    n = len(nodes)
+    perm = np.random.default_rng(seed).permutation(n).astype(np.int32)
+    cells = perm[cells]
+    if hierarchic and deg == 2:
+        lex = [x + 3 * (y + 3 * z) for (x, y, z) in HIER_Q2]
+        cells = cells[:, lex]
+    xyz2 = np.zeros((n, 3))
+    xyz2[perm] = np.asarray(node_pos)
+    lines = [(int(perm[v]), [int(perm[m]) for m in ms], ws) for v, ms, ws in lines]
+    faces = a["faces"].reshape(-1, 3) if mesh.n_faces else np.zeros((0, 3), np.int32)
+    kw = dict(degree=deg, cell_nodes=cells, cell_lo=clo, cell_h=h, node_xyz=xyz2, faces=faces, lines=lines,
+              order=1 if (hierarchic and deg == 2) else 0)
+    return kw, perm, np.asarray(img, np.int32)
+
+
+def oracle_caller(om, perm, images, mesh2):
+    """Oracle description of the caller-built mesh mesh2 (caller_arrays) in
+    the caller's numbering, with deal.II constraint semantics: the dof lines
+    of the library's nodes are those of om (the oracle of the library mesh,
+    derived from the geometry alone), renumbered; an image dof follows its
+    - face dof, closed as AffineConstraints::close does (a Dirichlet master
+    drops out, so the image of a Dirichlet dof is constrained to zero).  The
+    BSR container is mesh2's pattern."""
+    n1 = om.n_nodes
+    n2 = n1 + len(images)
+    assert len(perm) == n2
+    lines = {}
+    cof, cptr, cm, cw = om.con_of, om.con_ptr, om.con_master, om.con_weight
+    for g in range(n1 * 18):
+        k = cof[g]
+        if k >= 0:
+            v, c = divmod(g, 18)
+            lines[int(perm[v]) * 18 + c] = [(int(perm[cm[j] // 18]) * 18 + cm[j] % 18, cw[j])
+                                            for j in range(cptr[k], cptr[k + 1])]
+    for i, v in enumerate(images):
+        u = int(perm[n1 + i])
+        for c in range(18):
+            g = int(v) * 18 + c
+            k = cof[g]
+            lines[u * 18 + c] = [(int(perm[v]) * 18 + c, 1.0)] if k < 0 else \
+                [(int(perm[cm[j] // 18]) * 18 + cm[j] % 18, cw[j]) for j in range(cptr[k], cptr[k + 1])]
+    con_of = -np.ones(n2 * 18, np.int32)
+    ptr, ms, ws = [0], [], []
+    for i, g in enumerate(sorted(lines)):
+        con_of[g] = i
+        for mm, w in lines[g]:
+            ms.append(mm)
+            ws.append(w)
+        ptr.append(len(ms))
+    a = mesh2.arrays()
+    faces = a["faces"].reshape(-1, 3) if mesh2.n_faces else np.zeros((0, 3), np.int32)
+    return O.Mesh(mesh2.deg, a["cell_nodes"], a["cell_h"], [tuple(f) for f in faces], n2, con_of,
+                  np.array(ptr, np.int32), ms, ws, a["row_ptr"], a["col"], a["node_xyz"].reshape(-1, 3))

@@ -601,3 +601,45 @@ end
     assert rows[1]["cells"] < plain[1]["cells"]
     assert rows[1]["residual"] <= 1e-7
     assert ctx.residual_norm() == pytest.approx(rows[1]["residual"], rel=1e-6)
+
+
+CALLER_CASES = [
+    # degree, refinement steps, seed, boundary ids, Robin length bt
+    (1, 2, 1, PERIODIC, 1.0e10),
+    (2, 1, 2, PERIODIC, 1.0e10),
+    (1, 2, 3, [2, 2, 3, 3, 4, 4], 2.0),
+]
+
+
+@pytest.mark.parametrize("deg,steps,seed,sides,bt", CALLER_CASES)
+def test_caller_built_mesh_matches_oracle(ctx, deg, steps, seed, sides, bt):
+    # The per-cycle upload from the arrays a deal.II caller holds
+    # (glh_mesh_from_arrays): its own node numbering (randomly permuted),
+    # FE_Q's hierarchic local order, periodic images as constraint lines and
+    # the hanging-node lines; the library builds colours and patterns.  The
+    # oracle of that constraint set (hanging lines derived from the geometry,
+    # images from the coordinates, closed as deal.II closes them) gives R and
+    # the condensed J to 1e-12, constrained rows included.
+    from product_util import caller_arrays, oracle_caller
+    lo, hi = (-6.0, -6.0, -4.0), (6.0, 6.0, 4.0)
+    f = adaptive_forest(G, (3, 3, 2) if deg == 1 else (2, 2, 2), lo, hi, sides, steps, seed, degree=deg)
+    m = f.mesh()
+    kw, perm, images = caller_arrays(m, lo, hi, seed=11 + seed, hierarchic=True)
+    m2 = G.Mesh.from_arrays(**kw)
+    say("caller mesh: deg %d cells %d nodes %d (library %d) lines %d" % (deg, m2.n_cells, m2.n_nodes, m.n_nodes,
+                                                                         len(kw["lines"])))
+    p, t = 22.0, 0.8
+    om2 = oracle_caller(oracle_adaptive(m, lo, hi, sides), perm, images, m2)
+    co = O.coef(p, t, bt=bt)
+    U = O.initial_guess(m2.n_nodes, O.matep(p, t)["gap_B"], amp=0.05, seed=seed)
+    Ud = O.distribute(om2, U)
+    ctx.set_material_pt(p, t, bt=bt)
+    ctx.upload_mesh(m2)
+    ctx.set_solution(U)
+    assert np.abs(ctx.solution() - Ud).max() <= 1e-15 * np.abs(Ud).max()  # distribute
+    ctx.assemble()
+    Jo, ro = O.assemble(om2, co, Ud)
+    er = np.abs(ctx.rhs() - ro).max() / np.abs(ro).max()
+    ej = np.abs(ctx.jacobian() - Jo).max() / np.abs(Jo).max()
+    say("caller mesh rhs rel err", er, "J rel err", ej, "images", len(images))
+    assert er <= 1e-12 and ej <= 1e-12

@@ -182,3 +182,26 @@ def test_newton_converges_and_lowers_energy():
     assert O.free_energy(mesh, co, U) < O.free_energy(mesh, co, U0)
     _, r = O.assemble(mesh, co, U, J=False)
     assert np.linalg.norm(r) <= 1e-10 * st[2]
+
+
+@pytest.mark.parametrize("deg,sides,bt", [(1, [1, 1, 1, 1, 0, 0], 1.0e10), (2, [2, 2, 3, 0, 4, 1], 1.5),
+                                          (2, [1, 1, 1, 1, 0, 0], 1.0e10)])
+def test_cellwise_apply_equals_the_assembled_operator(deg, sides, bt):
+    # orc_cells_apply (J x and -R without a global J, colour-parallel) is the
+    # full-size check of the GPU path; here it must equal the BSR assembly of
+    # orc_assemble (distribute_local_to_global semantics) on small meshes
+    from product_util import glhip, oracle_mesh
+    G = glhip()
+    mesh = G.Mesh(deg, (3, 2, 2), (-3.0, -2.0, -2.0), (3.0, 2.0, 2.0), sides)
+    om = oracle_mesh(mesh)
+    co = O.coef(22.0, 0.8, bt=bt)
+    rng = np.random.default_rng(deg)
+    U = O.initial_guess(mesh.n_nodes, O.matep(22.0, 0.8)["gap_B"], amp=0.3, seed=5)
+    J, r = O.assemble(om, co, U)
+    X = rng.standard_normal((2, mesh.n_nodes * 18))
+    r2, Y = O.cells_apply(deg, mesh.arrays(), mesh.n_nodes, co, U, X)
+    assert np.abs(r2 - r).max() <= 1e-13 * np.abs(r).max()
+    for x, y in zip(X, Y):
+        ya = np.zeros_like(x)
+        O.lib().orc_bsr_spmv(om.n_nodes, om.row_ptr, om.col, J.reshape(-1), x, ya)
+        assert np.abs(y - ya).max() <= 1e-13 * np.abs(ya).max()

@@ -211,6 +211,20 @@ int fail(glh_ctx *c, int code, const std::string &msg)
   return code;
 }
 
+// Captured GMRES graphs hold kernel arguments by value: the buffers, and the
+// material (K1, K2 + K3 and the Robin coefficient of the matrix-free product)
+// together with the device pointer of the per-instance materials.  Whatever
+// changes those must drop the graphs; the next solve records them again.
+void drop_graphs(glh_ctx *c)
+{
+  for (auto *g : {&c->graph64, &c->graph32}) {
+    for (auto e : g->exec)
+      if (e) (void)hipGraphExecDestroy(e);
+    g->exec.clear();
+    g->key.clear();
+  }
+}
+
 #define HIPCHK(ctx, ...)                                                                         \
   do {                                                                                           \
     hipError_t e_ = (__VA_ARGS__);                                                               \
@@ -1063,7 +1077,12 @@ int glh_set_material(glh_ctx *c, double alpha, const double beta[5], const doubl
 {
   if (!c || !beta || !K) return GLH_ERR_ARG;
   const double co[10] = {alpha, beta[0], beta[1], beta[2], beta[3], beta[4], K[0], K[1], K[2], bt};
-  if (!make_material(co, c->mat)) return fail(c, GLH_ERR_ARG, "AdGR diffuse length must be positive");
+  glh::Material mt{};
+  if (!make_material(co, mt)) return fail(c, GLH_ERR_ARG, "AdGR diffuse length must be positive");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->stream) HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);  // the matrix-free product's graphs carry the old material by value
+  c->mat = mt;
   c->mats_host.clear();
   c->have_mat = true;
   c->j_valid = c->d_valid = false;
@@ -1077,6 +1096,8 @@ int glh_set_materials(glh_ctx *c, int n_inst, const double *coef)
   for (int i = 0; i < n_inst; ++i)
     if (!make_material(coef + 10 * i, ms[i])) return fail(c, GLH_ERR_ARG, "AdGR diffuse length must be positive");
   HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);  // recorded products read the old materials (and possibly the old buffer)
   HIPCHK(c, upload(c->mats, ms.data(), ms.size(), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->mats_host = ms;
@@ -1086,14 +1107,40 @@ int glh_set_materials(glh_ctx *c, int n_inst, const double *coef)
   return GLH_OK;
 }
 
+namespace {
+// message of the last failed mesh-building call on this thread (the mesh
+// functions have no context to keep it in)
+thread_local std::string mesh_err;
+}  // namespace
+
+const char *glh_mesh_last_error(void) { return mesh_err.c_str(); }
+
 int glh_mesh_box(int degree, int nx, int ny, int nz, const double lo[3], const double hi[3], const int side_id[6],
                  glh_mesh **out)
 {
   if (!out || !lo || !hi || !side_id) return GLH_ERR_ARG;
   *out = nullptr;
   glh_mesh *m = new glh_mesh();
-  std::string err;
-  if (glh::make_box(m->m, degree, nx, ny, nz, lo, hi, side_id, err)) {
+  if (glh::make_box(m->m, degree, nx, ny, nz, lo, hi, side_id, mesh_err)) {
+    delete m;
+    return GLH_ERR_ARG;
+  }
+  *out = m;
+  return GLH_OK;
+}
+
+int glh_mesh_from_arrays(const glh_mesh_desc *d, glh_mesh **out)
+{
+  if (!d || !out) return GLH_ERR_ARG;
+  *out = nullptr;
+  glh::ArrayMesh a;
+  a.degree = d->degree; a.n_cells = d->n_cells; a.n_nodes = d->n_nodes; a.order = d->local_order;
+  a.cell_nodes = d->cell_nodes; a.cell_lo = d->cell_lo; a.cell_h = d->cell_h; a.node_xyz = d->node_xyz;
+  a.n_faces = d->n_faces; a.faces = d->faces; a.node_mask = d->node_mask;
+  a.n_lines = d->n_lines; a.line_node = d->line_node; a.line_ptr = d->line_ptr; a.line_master = d->line_master;
+  a.line_w = d->line_w;
+  glh_mesh *m = new glh_mesh();
+  if (glh::from_arrays(m->m, a, mesh_err)) {
     delete m;
     return GLH_ERR_ARG;
   }
@@ -1444,13 +1491,7 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
   HIPCHK(c, hipSetDevice(c->device));
   if (c->deg != m.deg || !c->tN.p) RCHK(build_tables(c, m.deg));
   hipStream_t s = c->stream;
-  // captured GMRES graphs refer to the previous mesh's buffers
-  for (auto *g : {&c->graph64, &c->graph32}) {
-    for (auto e : g->exec)
-      if (e) (void)hipGraphExecDestroy(e);
-    g->exec.clear();
-    g->key.clear();
-  }
+  drop_graphs(c);  // captured GMRES graphs refer to the previous mesh's buffers
   c->n_cells = m.n_cells;
   c->n_nodes = m.n_nodes;
   c->nnzb = m.row_ptr.back();
@@ -1749,8 +1790,9 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
   std::vector<char> active(S), stalled(S, 0);
   HIPCHK(c, c->s_flag.alloc(S));
   const bool f32 = fp32 != 0;
-  // the matrix-free operator where it applies (FP64, one instance, no
-  // constraints, no continuation shift); the assembled BSR Jacobian otherwise
+  // the matrix-free operator where it applies (FP64 or FP32, one instance or
+  // a batch, no constraints, no continuation shift); the assembled BSR
+  // Jacobian otherwise
   const bool mf = c->matfree && !c->constrained && !(c->ptc_sigma0 > 0);
   std::vector<int> flag(S);
   // batches only: an instance whose ||R|| has fallen by less than 10% over the

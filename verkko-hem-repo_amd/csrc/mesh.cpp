@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 
 namespace glh {
 
@@ -306,6 +307,192 @@ void build_constrained_patterns(Mesh &m)
     m.col.insert(m.col.end(), v.begin(), v.end());
     std::vector<int>().swap(v);
   }
+}
+
+}  // namespace glh
+
+namespace glh {
+
+int hierarchic_to_lex(int deg, int i)
+{
+  // FE_Q support points in the order deal.II numbers their dofs: the 8
+  // vertices (x fastest), then for Q2 the midpoints of the 12 lines, of the
+  // 6 faces and of the cell, with lines and faces numbered as GeometryInfo<3>
+  // does (lines 0-3 on z = 0: x = 0, x = 1, y = 0, y = 1; lines 4-7 the same
+  // on z = 1; lines 8-11 along z at (0,0), (1,0), (0,1), (1,1); faces -x, +x,
+  // -y, +y, -z, +z).  Entries are (x, y, z) in units of half a cell.
+  static const int q2[27][3] = {
+      {0, 0, 0}, {2, 0, 0}, {0, 2, 0}, {2, 2, 0}, {0, 0, 2}, {2, 0, 2}, {0, 2, 2}, {2, 2, 2},  // vertices
+      {0, 1, 0}, {2, 1, 0}, {1, 0, 0}, {1, 2, 0},                                              // lines 0-3
+      {0, 1, 2}, {2, 1, 2}, {1, 0, 2}, {1, 2, 2},                                              // lines 4-7
+      {0, 0, 1}, {2, 0, 1}, {0, 2, 1}, {2, 2, 1},                                              // lines 8-11
+      {0, 1, 1}, {2, 1, 1}, {1, 0, 1}, {1, 2, 1}, {1, 1, 0}, {1, 1, 2},                        // faces
+      {1, 1, 1}};                                                                              // interior
+  if (deg == 1) return i;  // the vertices are already lexicographic
+  return q2[i][0] + 3 * (q2[i][1] + 3 * q2[i][2]);
+}
+
+int from_arrays(Mesh &m, const ArrayMesh &a, std::string &err)
+{
+  if (a.degree != 1 && a.degree != 2) { err = "degree must be 1 or 2"; return -1; }
+  if (a.order != 0 && a.order != 1) { err = "local order must be 0 (lexicographic) or 1 (hierarchic)"; return -1; }
+  if (a.n_cells < 1 || a.n_nodes < 1) { err = "a mesh needs cells and nodes"; return -1; }
+  if (!a.cell_nodes || !a.cell_lo || !a.cell_h) { err = "cell_nodes, cell_lo and cell_h are required"; return -1; }
+  if (a.n_faces < 0 || (a.n_faces > 0 && !a.faces)) { err = "bad face list"; return -1; }
+  if (a.n_lines < 0 || (a.n_lines > 0 && (!a.line_node || !a.line_ptr || !a.line_master || !a.line_w))) {
+    err = "bad constraint line arrays";
+    return -1;
+  }
+  const int deg = a.degree, n1 = deg + 1, ns = n1 * n1 * n1;
+  const long nn = a.n_nodes, nc = a.n_cells;
+  if (nn >= (1L << 31) / NC || nc * ns >= (1L << 31)) { err = "mesh too large for 32-bit indices"; return -1; }
+  m = Mesh();
+  m.deg = deg; m.n1 = n1; m.ns = ns;
+  m.n_cells = (int)nc; m.n_nodes = (int)nn;
+  m.cell_nodes.resize(nc * ns);
+  m.cell_h.assign(a.cell_h, a.cell_h + 3 * nc);
+  m.cell_lo.assign(a.cell_lo, a.cell_lo + 3 * nc);
+  std::vector<char> used(nn, 0);
+  for (long e = 0; e < nc; ++e) {
+    for (int k = 0; k < 3; ++k)
+      if (!(m.cell_h[3 * e + k] > 0.0) || !std::isfinite(m.cell_h[3 * e + k]) || !std::isfinite(m.cell_lo[3 * e + k])) {
+        err = "cell " + std::to_string(e) + ": edge lengths must be positive and finite";
+        return -1;
+      }
+    int *cn = &m.cell_nodes[e * ns];
+    for (int i = 0; i < ns; ++i) {
+      const int v = a.cell_nodes[e * ns + i];
+      if (v < 0 || v >= nn) { err = "cell " + std::to_string(e) + ": node index out of range"; return -1; }
+      cn[a.order == 1 ? hierarchic_to_lex(deg, i) : i] = v;
+      used[v] = 1;
+    }
+    for (int i = 0; i < ns; ++i)
+      for (int j = i + 1; j < ns; ++j)
+        if (cn[i] == cn[j]) { err = "cell " + std::to_string(e) + ": repeated node"; return -1; }
+  }
+  for (long v = 0; v < nn; ++v)
+    if (!used[v]) { err = "node " + std::to_string(v) + " belongs to no cell"; return -1; }
+  // support points: every cell must put its nodes where the caller says the
+  // nodes are (or, without coordinates, where the first cell puts them),
+  // which also catches a wrong local order
+  std::vector<char> seen(nn, 0);
+  m.node_xyz.assign(3 * nn, 0.0);
+  if (a.node_xyz) {
+    for (long k = 0; k < 3 * nn; ++k)
+      if (!std::isfinite(a.node_xyz[k])) { err = "non-finite node coordinate"; return -1; }
+    m.node_xyz.assign(a.node_xyz, a.node_xyz + 3 * nn);
+  }
+  for (long e = 0; e < nc; ++e) {
+    const double *lo = &m.cell_lo[3 * e], *h = &m.cell_h[3 * e];
+    const double tol = 1e-9 * (std::fabs(lo[0]) + std::fabs(lo[1]) + std::fabs(lo[2]) + h[0] + h[1] + h[2]);
+    for (int i = 0; i < ns; ++i) {
+      const int v = m.cell_nodes[e * ns + i];
+      const int t[3] = {i % n1, (i / n1) % n1, i / (n1 * n1)};
+      double P[3];
+      for (int k = 0; k < 3; ++k) P[k] = lo[k] + h[k] * t[k] / deg;
+      if (!a.node_xyz && !seen[v]) {
+        for (int k = 0; k < 3; ++k) m.node_xyz[3 * v + k] = P[k];
+        seen[v] = 1;
+        continue;
+      }
+      for (int k = 0; k < 3; ++k)
+        if (std::fabs(m.node_xyz[3 * v + k] - P[k]) > tol) {
+          err = "cell " + std::to_string(e) + ": node " + std::to_string(v) +
+                " is not at the cell's support point (wrong local order or coordinates?)";
+          return -1;
+        }
+    }
+  }
+  // boundary faces; natural faces (id 1) carry nothing
+  for (int f = 0; f < a.n_faces; ++f) {
+    const int e = a.faces[3 * f], lf = a.faces[3 * f + 1], bid = a.faces[3 * f + 2];
+    if (e < 0 || e >= nc || lf < 0 || lf > 5 || bid < 0) { err = "face " + std::to_string(f) + " out of range"; return -1; }
+    if (bid == 1) continue;
+    m.faces.push_back(e);
+    m.faces.push_back(lf);
+    m.faces.push_back(bid);
+  }
+  // Dirichlet component masks: the caller's, or the union of the masks of
+  // the boundary ids of the faces a node lies on (interpolate_boundary_values
+  // with the component masks of femgl.h:294-301)
+  m.node_mask.assign(nn, 0u);
+  if (a.node_mask) {
+    for (long v = 0; v < nn; ++v) {
+      if (a.node_mask[v] >> NC) { err = "node mask bits above component 17"; return -1; }
+      m.node_mask[v] = a.node_mask[v];
+    }
+  } else {
+    for (size_t f = 0; f + 2 < m.faces.size(); f += 3) {
+      const int e = m.faces[f], lf = m.faces[f + 1];
+      const uint32_t mk = dirichlet_mask(m.faces[f + 2]);
+      if (!mk) continue;
+      const int ax = lf / 2, side = (lf % 2) * deg;
+      for (int i = 0; i < ns; ++i) {
+        const int t[3] = {i % n1, (i / n1) % n1, i / (n1 * n1)};
+        if (t[ax] == side) m.node_mask[m.cell_nodes[(size_t)e * ns + i]] |= mk;
+      }
+    }
+  }
+  // constraint lines (hanging nodes, periodic images), closed by substituting
+  // constrained masters (AffineConstraints::close)
+  if (a.n_lines > 0) {
+    std::vector<std::vector<std::pair<int, double>>> line(nn);
+    std::vector<char> con(nn, 0);
+    for (int l = 0; l < a.n_lines; ++l) {
+      const int v = a.line_node[l], k0 = a.line_ptr[l], k1 = a.line_ptr[l + 1];
+      if (v < 0 || v >= nn) { err = "constraint line " + std::to_string(l) + ": node out of range"; return -1; }
+      if (con[v]) { err = "node " + std::to_string(v) + " has two constraint lines"; return -1; }
+      if ((l == 0 && k0 != 0) || k1 <= k0) { err = "line_ptr must start at 0 and increase"; return -1; }
+      con[v] = 1;
+      for (int k = k0; k < k1; ++k) {
+        const int u = a.line_master[k];
+        const double w = a.line_w[k];
+        if (u < 0 || u >= nn || u == v || !std::isfinite(w)) {
+          err = "constraint line " + std::to_string(l) + ": bad master or weight";
+          return -1;
+        }
+        line[v].push_back({u, w});
+      }
+    }
+    for (long v = 0; v < nn; ++v) {
+      if (!con[v]) continue;
+      for (int pass = 0;; ++pass) {
+        if (pass == 64) { err = "cyclic constraint lines"; return -1; }
+        bool changed = false;
+        std::map<int, double> acc;
+        for (auto &mw : line[v]) {
+          if (con[mw.first]) {
+            if (mw.first == v) { err = "cyclic constraint lines"; return -1; }
+            for (auto &mw2 : line[mw.first]) acc[mw2.first] += mw.second * mw2.second;
+            changed = true;
+          } else {
+            acc[mw.first] += mw.second;
+          }
+        }
+        line[v].assign(acc.begin(), acc.end());
+        if (!changed) break;
+      }
+    }
+    m.hang_ptr.assign(nn + 1, 0);
+    for (long v = 0; v < nn; ++v) {
+      m.hang_ptr[v + 1] = m.hang_ptr[v] + (con[v] ? (int)line[v].size() : 0);
+      if (!con[v]) continue;
+      m.node_mask[v] = HANG_BIT;  // the line wins over a Dirichlet mask, as it is added first
+      for (auto &mw : line[v]) {
+        m.hang_m.push_back(mw.first);
+        m.hang_w.push_back(mw.second);
+      }
+    }
+  }
+  greedy_colors(m);
+  if (m.has_hanging()) build_constrained_patterns(m);
+  else {
+    build_pattern(m);
+    m.hang_ptr.clear();
+  }
+  m.node_off = {0, m.n_nodes};
+  m.cell_off = {0, m.n_cells};
+  return 0;
 }
 
 }  // namespace glh

@@ -60,6 +60,29 @@ void build_constrained_patterns(Mesh &m);
 // Greedy colouring of cells so that no two cells of one colour share a node.
 void greedy_colors(Mesh &m);
 
+// Mesh from a caller's arrays (the per-cycle data a deal.II caller has after
+// make_grid()/setup_system(), setup_uniform_B-phase.cc:125-198): cells in
+// the caller's node numbering, local order lexicographic (order 0) or FE_Q's
+// hierarchic support-point order (order 1: vertices, lines, faces, interior
+// in GeometryInfo<3> numbering), box geometry, boundary faces, optional
+// Dirichlet component masks and node constraint lines (row node, masters,
+// weights; closed here).  Colours and both BSR patterns are derived.
+struct ArrayMesh {
+  int degree = 1, n_cells = 0, n_nodes = 0, order = 0;
+  const int *cell_nodes = nullptr;
+  const double *cell_lo = nullptr, *cell_h = nullptr, *node_xyz = nullptr;
+  int n_faces = 0;
+  const int *faces = nullptr;
+  const uint32_t *node_mask = nullptr;
+  int n_lines = 0;
+  const int *line_node = nullptr, *line_ptr = nullptr, *line_master = nullptr;
+  const double *line_w = nullptr;
+};
+int from_arrays(Mesh &m, const ArrayMesh &a, std::string &err);
+// local index (lexicographic, x fastest) of FE_Q(deg) support point i in
+// hierarchic order
+int hierarchic_to_lex(int deg, int i);
+
 // Concatenate meshes of one degree into a batch with disjoint node and cell
 // ranges; colour k of the batch is the union of colour k of every instance.
 int concat(Mesh &out, const std::vector<const Mesh *> &ms, std::string &err);

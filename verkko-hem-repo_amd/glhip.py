@@ -38,7 +38,18 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_set_schedule", "glh_autotune", "glh_prm_parse", "glh_prm_destroy", "glh_prm_get",
            "glh_prm_get_double", "glh_run_prm", "glh_write_vtu", "glh_initial_guess_field",
            "glh_transfer_solution", "glh_forest_set_degree", "glh_bench_read",
-           "glh_flag_refine_coarsen"]
+           "glh_flag_refine_coarsen", "glh_mesh_from_arrays", "glh_mesh_last_error"]
+
+GLH_ORDER_LEXICOGRAPHIC, GLH_ORDER_HIERARCHIC = 0, 1
+
+
+class MeshDesc(C.Structure):
+    """glh_mesh_desc of include/glhip.h."""
+    _fields_ = [("degree", C.c_int), ("n_cells", C.c_int), ("n_nodes", C.c_int), ("local_order", C.c_int),
+                ("cell_nodes", C.c_void_p), ("cell_lo", C.c_void_p), ("cell_h", C.c_void_p),
+                ("node_xyz", C.c_void_p), ("n_faces", C.c_int), ("faces", C.c_void_p), ("node_mask", C.c_void_p),
+                ("n_lines", C.c_int), ("line_node", C.c_void_p), ("line_ptr", C.c_void_p),
+                ("line_master", C.c_void_p), ("line_w", C.c_void_p)]
 
 _lib = None
 
@@ -136,6 +147,9 @@ def lib():
     L.glh_bench_read.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
     L.glh_flag_refine_coarsen.argtypes = [C.c_int64, _f32p, C.c_double, C.c_double, _u8p]
     L.glh_initial_guess_field.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_uint64, _dp]
+    L.glh_mesh_from_arrays.argtypes = [C.POINTER(MeshDesc), C.POINTER(vp)]
+    L.glh_mesh_last_error.argtypes = []
+    L.glh_mesh_last_error.restype = C.c_char_p
     _lib = L
     return L
 
@@ -201,6 +215,51 @@ class Mesh:
         self.deg, self.n_cells, self.n_nodes, self.n_faces, self.nnzb, self.n_colors = [int(v) for v in s[:6]]
         self.n_inst = int(s[6])
         self.ns = (self.deg + 1) ** 3
+
+    @staticmethod
+    def from_arrays(degree, cell_nodes, cell_lo, cell_h, node_xyz=None, faces=None, node_mask=None, lines=None,
+                    order=GLH_ORDER_LEXICOGRAPHIC):
+        """glh_mesh_from_arrays: a mesh from caller-built arrays.  lines is a
+        list of (node, masters, weights) constraint lines or None."""
+        cn = np.ascontiguousarray(cell_nodes, np.int32).reshape(-1)
+        ns = (int(degree) + 1) ** 3
+        n_cells = cn.size // ns
+        n_nodes = int(cn.max()) + 1 if node_xyz is None else len(node_xyz) // 3 if np.ndim(node_xyz) == 1 else \
+            len(node_xyz)
+        keep = []
+
+        def ptr(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dt).reshape(-1)
+            keep.append(a)
+            return a.ctypes.data_as(C.c_void_p)
+
+        d = MeshDesc()
+        d.degree, d.n_cells, d.n_nodes, d.local_order = int(degree), n_cells, n_nodes, int(order)
+        d.cell_nodes = ptr(cn, np.int32)
+        d.cell_lo, d.cell_h = ptr(cell_lo, np.float64), ptr(cell_h, np.float64)
+        d.node_xyz = ptr(node_xyz, np.float64)
+        fa = None if faces is None else np.asarray(faces, np.int32).reshape(-1, 3)
+        d.n_faces = 0 if fa is None else len(fa)
+        d.faces = ptr(fa, np.int32)
+        d.node_mask = ptr(node_mask, np.uint32)
+        if lines:
+            d.n_lines = len(lines)
+            d.line_node = ptr([ln[0] for ln in lines], np.int32)
+            d.line_ptr = ptr(np.concatenate([[0], np.cumsum([len(ln[1]) for ln in lines])]), np.int32)
+            d.line_master = ptr(np.concatenate([np.asarray(ln[1], np.int32) for ln in lines]), np.int32)
+            d.line_w = ptr(np.concatenate([np.asarray(ln[2], np.float64) for ln in lines]), np.float64)
+        h = C.c_void_p()
+        rc = lib().glh_mesh_from_arrays(C.byref(d), C.byref(h))
+        if rc:
+            raise GLHError("glh_mesh_from_arrays: " + lib().glh_mesh_last_error().decode())
+        return Mesh(handle=h)
+
+    def cell_lo(self):
+        """Lower corner of every cell (the coordinates of its local node 0)."""
+        a = self.arrays()
+        return a["node_xyz"].reshape(-1, 3)[a["cell_nodes"].reshape(-1, self.ns)[:, 0]]
 
     @staticmethod
     def concat(meshes):
