@@ -73,16 +73,39 @@ def mf_bytes(deg, n_cells, n_dofs):
     return n_cells * ns * 171 * 8 + 2 * n_dofs * 8
 
 
-def cpu_baseline(deg, h, mat, n_cells, n_nodes, nnzb, newton_steps, gmres_its, ls_trials, restart, budget_s=25.0):
-    """CPU baseline of the same algorithm (the oracle, kind 'port'), timed on
-    a bounded sample with OpenMP on the host cores, extrapolated with the
-    GPU run's Newton and GMRES counts."""
+def host_cpu():
+    """Host CPU of this run: model name (/proc/cpuinfo), nproc, and the CPUs
+    this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(deg, h, mat, gap, n_cells, n_nodes, nnzb, newton_steps, gmres_its, ls_trials, restart):
+    """CPU baseline of the same algorithm (the oracle, kind 'port') on the
+    host cores (OpenMP), each leg timed on a bounded sample: the literal
+    per-(i,j,q) cell loop (J + R) of assemble.cc:177-280 on one cell per
+    thread, the factorized J + R and the residual-only cell work on a few
+    cells per thread, and the oracle's Arnoldi step (point-block Jacobi, BSR
+    SpMV, one classical Gram-Schmidt pass) on an 8^3-cell (Q2) or 16^3-cell
+    (Q1) sample of the same operator at the GPU run's mean Krylov length.
+    Extrapolated linearly to the full size (cells; blocks for the Arnoldi
+    step) with the GPU run's own Newton, GMRES and line-search counts."""
     import oracle_lib as O
     L = O.lib()
     L.orc_time_cells.argtypes = [C.c_int, C.c_int, O._dp, O._dp, O._dp, C.c_int, C.c_int, O._dp]
     L.orc_time_cells.restype = C.c_double
-    L.orc_time_spmv.argtypes = [C.c_int, O._ip, O._ip, O._dp, O._dp, O._dp, C.c_int, C.c_int]
-    L.orc_time_spmv.restype = C.c_double
+    L.orc_time_residual_cells.argtypes = [C.c_int, C.c_int, O._dp, O._dp, O._dp, C.c_int, O._dp]
+    L.orc_time_residual_cells.restype = C.c_double
+    L.orc_time_arnoldi.argtypes = [C.c_int, O._ip, O._ip, O._dp, C.c_int, C.c_int]
+    L.orc_time_arnoldi.restype = C.c_double
     L.orc_max_threads.restype = C.c_int
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or L.orc_max_threads()
     co = O.f64(mat)
@@ -91,31 +114,27 @@ def cpu_baseline(deg, h, mat, n_cells, n_nodes, nnzb, newton_steps, gmres_its, l
     hv = O.f64([h, h, h])
     cs = np.zeros(1)
     # factorized oracle (one Hessian per quadrature point), J+R per cell
-    ns_f = max(threads, 64 if deg == 2 else 512)
+    ns_f = max(4 * threads, 64 if deg == 2 else 512)
     Uf = rng.standard_normal(ns_f * dpc) * 0.5
     t_fac = L.orc_time_cells(deg, ns_f, hv, co, Uf, 0, threads, cs) / ns_f
+    # residual only (the line-search trials of iteration.cc:170-206)
+    t_res = L.orc_time_residual_cells(deg, ns_f, hv, co, Uf, threads, cs) / ns_f
     # literal per-(i,j,q) restatement of assemble.cc:177-280, one cell per thread
     ns_l = threads
     Ul = rng.standard_normal(ns_l * dpc) * 0.5
     t_lit = L.orc_time_cells(deg, ns_l, hv, co, Ul, 1, threads, cs) / ns_l
-    # BSR SpMV on a sample matrix of the same block structure (values random)
+    # Arnoldi steps of the oracle's GMRES on the oracle's own J of a sample mesh
     G = load_glhip()
     ns_side = 8 if deg == 2 else 16
-    m = G.Mesh(deg, (ns_side,) * 3, (0, 0, 0), (1, 1, 1), [1] * 6)
+    m = G.Mesh(deg, (ns_side,) * 3, (0, 0, 0), (ns_side * h,) * 3, [1, 1, 1, 1, 0, 0])
+    from product_util import oracle_mesh
+    om = oracle_mesh(m)
+    Us = O.initial_guess(m.n_nodes, gap, amp=0.05, seed=0)
+    Js, _ = O.assemble(om, co, O.distribute(om, Us), R=False)
     a = m.arrays()
-    val = rng.standard_normal(m.nnzb * 324)
-    x = rng.standard_normal(m.n_nodes * 18)
-    y = np.zeros_like(x)
-    L.orc_time_spmv(m.n_nodes, a["row_ptr"], a["col"], val, x, y, 1, threads)
-    reps = 5
-    t_sp = L.orc_time_spmv(m.n_nodes, a["row_ptr"], a["col"], val, x, y, reps, threads) / reps
-    t_spmv_full = t_sp * (nnzb / m.nnzb)
-    # GMRES iteration = SpMV + preconditioner + CGS2 vector passes; the vector
-    # passes are counted by their bytes relative to the SpMV's
-    n = n_nodes * 18
-    vec_bytes = (4 * (restart / 2 + 1) + 6) * n * 8
-    t_iter = t_spmv_full * (1.0 + vec_bytes / spmv_bytes(n_nodes, nnzb, 8, 8))
-    t_res = 0.25 * t_fac  # residual-only cell work is about a quarter of J+R
+    n_steps = int(max(1, min(restart, round(gmres_its / max(1, newton_steps)))))
+    t_arn = L.orc_time_arnoldi(m.n_nodes, a["row_ptr"], a["col"], Js.reshape(-1), n_steps, threads) / n_steps
+    t_iter = t_arn * (nnzb / m.nnzb)
     solve_fac = newton_steps * t_fac * n_cells + (ls_trials + 1) * t_res * n_cells + gmres_its * t_iter
     solve_lit = newton_steps * t_lit * n_cells + (ls_trials + 1) * t_res * n_cells + gmres_its * t_iter
     return {
@@ -123,38 +142,23 @@ def cpu_baseline(deg, h, mat, n_cells, n_nodes, nnzb, newton_steps, gmres_its, l
         "unit": "solves/hr",
         "cores": threads,
         "kind": "port",
-        "sample": ("literal per-(i,j,q) cell loop of assemble.cc:177-280 on %d sample cells (%.2f s/cell/core), "
-                   "GMRES iterations from a BSR SpMV on a %d^3-cell sample of the same block structure "
-                   "(%.3f s per full-size SpMV), extrapolated with the GPU run's %d Newton steps, %d GMRES "
-                   "iterations and %d line-search residuals" % (ns_l, t_lit * threads, ns_side, t_spmv_full,
-                                                                newton_steps, gmres_its, ls_trials)),
+        "host": host_cpu(),
+        "sample": ("oracle legs timed on %d threads: literal per-(i,j,q) cell loop of assemble.cc:177-280 on %d "
+                   "cells (%.2f s/cell/core), factorized J+R and residual-only cell work on %d cells, %d Arnoldi "
+                   "steps (point-block Jacobi, BSR SpMV, one CGS pass) on the %d^3-cell sample (%.4f s per step, "
+                   "%.3f s at full size); extrapolated with the GPU run's %d Newton steps, %d GMRES iterations and "
+                   "%d line-search residuals" % (threads, ns_l, t_lit * threads, ns_f, n_steps, ns_side, t_arn,
+                                                 t_iter, newton_steps, gmres_its, ls_trials)),
+        "legs_s": {"literal_jr_per_cell_core": t_lit * threads, "factorized_jr_per_cell_core": t_fac * threads,
+                   "residual_per_cell_core": t_res * threads, "arnoldi_step_full_size": t_iter},
         "value_factorized": 3600.0 / solve_fac,
-        "factorized_s_per_cell_core": t_fac * threads,
         "extrapolated": True,
     }
 
 
 def slab_batch(G, n_inst=256, seed=2):
-    """The batched phase-diagram sweep: n_inst slab instances, T/Tc ~ U[0.3,0.95],
-    p ~ U[0,34) bar, thickness D ~ U[6,40] xi0, lateral extent 40 xi0, Q1 meshes
-    of nx = ny in 8..16 and nz in 2..8 cells, A = 0 on the two slab faces and
-    natural on the sides; B phase + 0.05 Delta_B noise (seed 1000 + i)."""
-    rng = np.random.default_rng(seed)
-    T = rng.uniform(0.3, 0.95, n_inst)
-    P = np.minimum(rng.uniform(0.0, 34.0, n_inst), np.nextafter(34.0, 0.0))
-    D = rng.uniform(6.0, 40.0, n_inst)
-    NX = rng.integers(8, 17, n_inst)
-    NZ = rng.integers(2, 9, n_inst)
-    meshes, coefs, U = [], [], []
-    K = 0.42072
-    for i in range(n_inst):
-        m = G.Mesh(1, (int(NX[i]), int(NX[i]), int(NZ[i])), (-20.0, -20.0, -D[i] / 2), (20.0, 20.0, D[i] / 2),
-                   [1, 1, 1, 1, 0, 0])
-        rc, mp = G.matep(P[i], T[i])
-        meshes.append(m)
-        coefs.append([mp[0], mp[1], mp[2], mp[3], mp[4], mp[5], K, K, K, 1.0e10])
-        U.append(G.initial_guess(m.n_nodes, mp[9], phase=0, amp=0.05, seed=1000 + i))
-    return meshes, np.array(coefs), U, dict(T=T, p=P, D=D, nx=NX, nz=NZ)
+    """The 256-slab phase-diagram sweep (warmstart.sweep_instances)."""
+    return load_module("warmstart").sweep_instances(G, n_inst, seed)
 
 
 def run_batch(G, args):
@@ -171,6 +175,10 @@ def run_batch(G, args):
     ctx.store_initial()
     ctx.set_continuation(args.continuation)
     ctx.set_matrix_free((args.operator == "matrix-free"))
+    if args.direct_launch:
+        # kernel timing on: every GMRES cycle is launched directly instead of
+        # being replayed from captured graphs (for kernel-trace profiles)
+        ctx.set_timing(True)
     out = {}
     tuned = {}
     for prec in ("fp64", "fp32"):
@@ -192,7 +200,13 @@ def run_batch(G, args):
         st = ctx.newton_solve(fp32=(prec == "fp32"), **kw_p)
         el = time.perf_counter() - t0
         ist = ctx.instance_stats()
-        out[prec] = dict(elapsed=el, st=st, inst=ist, F=ctx.instance_energy())
+        # convergence as reported, held to the residuals recomputed over the
+        # batch at the final state (FP64, residual.cc:109-297 per instance)
+        r_end = ctx.instance_residuals()
+        ok = r_end <= 1e-10 * ist[:, 2] * (1 + 1e-6)
+        failed = int(((ist[:, 1] == 1) & ~ok).sum())
+        ist[:, 1] = np.where(ok, ist[:, 1], 0.0)
+        out[prec] = dict(elapsed=el, st=st, inst=ist, F=ctx.instance_energy(), recheck_failed=failed)
         print("bench: batch %s %.3f s, %d/%d converged, %s" % (prec, el, int(ist[:, 1].sum()), len(meshes), st),
               file=sys.stderr, flush=True)
     ctx.close()
@@ -257,6 +271,10 @@ def run_batch(G, args):
                    "name": "slab_batch%d" % len(meshes), "dofs": n_dofs, "instances": len(meshes),
                    "operator": "matrix-free" if args.operator == "matrix-free" else "assembled BSR"},
         "converged_instances": int(conv32.sum()),
+        "convergence_rechecked": {"rule": "a converged instance's ||R||, recomputed in FP64 over the batch at the "
+                                          "final state, is <= 1e-10 of its ||R0||",
+                                  "reported_but_failed_fp32": out["fp32"]["recheck_failed"],
+                                  "reported_but_failed_fp64": out["fp64"]["recheck_failed"]},
         "p50_instance_s": float(np.percentile(t_conv[t_conv >= 0], 50)) if (t_conv >= 0).any() else None,
         "p99_instance_s": float(np.percentile(t_conv[t_conv >= 0], 99)) if (t_conv >= 0).any() else None,
         "free_energy_fp32_vs_fp64": {"compared": int(both.sum()), "within_1e-10": int(same32.sum()),
@@ -274,7 +292,7 @@ def run_batch(G, args):
         "lockstep_newton_steps": out["fp32"]["st"]["newton_steps"],
         "lockstep_gmres_its": out["fp32"]["st"]["gmres_its"],
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def run_adaptive(G, args):
@@ -319,7 +337,7 @@ def run_adaptive(G, args):
                    for r in rows],
         "wall_s_including_fp64_checks": el,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def load_module(name):
@@ -425,16 +443,21 @@ def run_warm_start(G, args):
         "learned_guess_used": int(use.sum()),
         "first_residual_median": {"learned": float(np.median(r_l)), "plain": float(np.median(r_p))},
         "solves_hr": {"learned_fp32_batched": rate(conv_w.sum(), t_warm32),
+                      "learned_fp32_batched_incl_training": rate(conv_w.sum(), t_warm32 + t_data + t_train),
                       "plain_fp32_batched": rate(conv_p.sum(), t_plain32),
                       "plain_fp64_sequential": plain_rate},
         "speedup_vs_plain_fp64": rate(conv_w.sum(), t_warm32) / plain_rate if plain_rate else None,
+        "speedup_incl_training_vs_plain_fp64": (rate(conv_w.sum(), t_warm32 + t_data + t_train) / plain_rate
+                                                if plain_rate else None),
+        "speedup_vs_plain_fp32_batched": (rate(conv_w.sum(), t_warm32) / rate(conv_p.sum(), t_plain32)
+                                          if conv_p.sum() else None),
         "free_energy_vs_plain_fp64": {"compared": int(both.sum()), "within_1e-10": int(same.sum()),
                                       "max_rel_err_within": float(rel[same].max()) if same.any() else None,
                                       "lower_energy_state": int((both & (F_w < F_64) & ~same).sum())},
         "training": {"data_s": t_data, "train_s": t_train, "loss_curve": curve},
         "wall_s": time.perf_counter() - t_all,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def run_mlp_kernels(G, args):
@@ -488,6 +511,14 @@ def main():
                     help="slab_batch: pseudo-transient continuation sigma0 for the batched and the one-at-a-time "
                          "solves (0: plain damped Newton)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--direct-launch", action="store_true",
+                    help="slab_batch: launch every GMRES cycle directly (kernel timing on) instead of replaying "
+                         "captured HIP graphs")
+    ap.add_argument("--no-north-star", action="store_true",
+                    help="cube workloads: skip the slab_batch, warm_start and adaptive_slab records that the default "
+                         "run attaches to the headline line")
+    ap.add_argument("--ns-sequential", type=int, default=32,
+                    help="north-star records: instances solved one by one (seeded sample) for the plain FP64 rate")
     ap.add_argument("--operator", choices=["matrix-free", "assembled"], default=None,
                     help="Jacobian form of the solves; default: the faster one measured per workload "
                          "(matrix-free for the cube and warm_start, assembled for slab_batch)")
@@ -505,13 +536,13 @@ def main():
         sys.exit(2)
     G = load_glhip()
     if args.workload == "slab_batch":
-        run_batch(G, args)
+        print(json.dumps(run_batch(G, args)), flush=True)
         return
     if args.workload == "adaptive_slab":
-        run_adaptive(G, args)
+        print(json.dumps(run_adaptive(G, args)), flush=True)
         return
     if args.workload == "warm_start":
-        run_warm_start(G, args)
+        print(json.dumps(run_warm_start(G, args)), flush=True)
         return
     if args.workload == "mlp_kernels":
         run_mlp_kernels(G, args)
@@ -669,11 +700,51 @@ def main():
         print("bench: mixed precision %s" % line["mixed_precision"], file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         ls = int(st0.get("ls_trials", st0["newton_steps"]))
-        line["cpu_baseline"] = cpu_baseline(deg, 2 * half / ncell, list(ctx_coef(m)), mesh.n_cells, mesh.n_nodes,
-                                            mesh.nnzb, int(st0["newton_steps"]), int(st0["gmres_its"]), ls,
-                                            args.restart)
+        line["cpu_baseline"] = cpu_baseline(deg, 2 * half / ncell, list(ctx_coef(m)), m[9], mesh.n_cells,
+                                            mesh.n_nodes, mesh.nnzb, int(st0["newton_steps"]), int(st0["gmres_its"]),
+                                            ls, args.restart)
+        print("bench: cpu baseline %s" % line["cpu_baseline"], file=sys.stderr, flush=True)
     ctx.close()
+    mesh.close()
+    if not args.no_north_star:
+        line["north_star"] = north_star(G, args, line)
     print(json.dumps(line), flush=True)
+
+
+def north_star(G, args, head):
+    """The three additions of the plain form, each at its full configuration,
+    run after the headline (outside its timed region) so that one default
+    run records them all: the batched 256-slab sweep (FP32 batched with the
+    autotuner against plain FP64 solves one at a time on a seeded sample),
+    the learned warm start (4096 training solutions, 20 000 Adam steps,
+    256 held-out instances), and the adaptive slab (4 Kelly cycles).  The
+    ratios the north star names are summarised beside them."""
+    import copy
+    recs = {}
+    a = copy.copy(args)
+    a.instances, a.sequential, a.warmup, a.operator, a.autotune = 256, args.ns_sequential, 0, "assembled", True
+    print("bench: north star: slab_batch", file=sys.stderr, flush=True)
+    recs["slab_batch"] = run_batch(G, a)
+    a = copy.copy(args)
+    a.instances, a.sequential, a.operator = 256, args.ns_sequential, "matrix-free"
+    print("bench: north star: warm_start", file=sys.stderr, flush=True)
+    recs["warm_start"] = run_warm_start(G, a)
+    a = copy.copy(args)
+    a.cycles = 4
+    print("bench: north star: adaptive_slab", file=sys.stderr, flush=True)
+    recs["adaptive_slab"] = run_adaptive(G, a)
+    for r in recs.values():
+        for k in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"):
+            r.pop(k, None)
+    ws = recs["warm_start"]
+    cpu = head.get("cpu_baseline", {}).get("value")
+    recs["ratios"] = {
+        "learned_fp32_batched_vs_plain_fp64_gpu": ws.get("speedup_vs_plain_fp64"),
+        "learned_fp32_batched_incl_training_vs_plain_fp64_gpu": ws.get("speedup_incl_training_vs_plain_fp64"),
+        "fp32_batched_vs_plain_fp64_gpu_slab_sweep": recs["slab_batch"].get("speedup_vs_plain_fp64"),
+        "plain_fp64_gpu_vs_reference_cpu_port": head["value"] / cpu if cpu else None,
+        "targets": {"learned_batched_vs_plain_fp64_gpu": 10.0, "plain_fp64_gpu_vs_cpu": 20.0}}
+    return recs
 
 
 def ctx_coef(m):

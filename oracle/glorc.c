@@ -1438,6 +1438,113 @@ double orc_time_spmv(int n_nodes, const int *row_ptr, const int *col, const doub
   return wall() - t0;
 }
 
+/* Wall seconds of n_steps Arnoldi steps of the oracle's GMRES (solve.cc:
+ * 159-174 with point-block Jacobi): z = D^-1 v_j, w = J z, one classical
+ * Gram-Schmidt pass against v_0..v_j (the dots, then the update), the norm
+ * and the scaling into v_{j+1}, from a seeded start vector; every loop over
+ * rows, nodes or entries in parallel (OpenMP, nthreads).  The block-Jacobi
+ * inversion is outside the timing.  Returns -1 on a singular block. */
+double orc_time_arnoldi(int n_nodes, const int *row_ptr, const int *col, const double *val, int n_steps,
+                        int nthreads)
+{
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  const size_t n = (size_t)n_nodes * NC;
+  double *Dinv = (double *)malloc(sizeof(double) * (size_t)n_nodes * NC * NC);
+  if (orc_block_jacobi(n_nodes, row_ptr, col, val, Dinv)) { free(Dinv); return -1.0; }
+  double *V = (double *)malloc(sizeof(double) * n * (n_steps + 1));
+  double *Z = (double *)malloc(sizeof(double) * n), *w = (double *)malloc(sizeof(double) * n);
+  double *h = (double *)malloc(sizeof(double) * (n_steps + 1));
+  uint64_t st = 88172645463325252ull;
+  for (size_t i = 0; i < n; ++i) {
+    st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+    V[i] = (double)(st >> 11) * 0x1.0p-53 - 0.5;
+  }
+  double t0 = wall();
+  for (int j = 0; j < n_steps; ++j) {
+    const double *vj = V + n * j;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int r = 0; r < n_nodes; ++r)
+      for (int c = 0; c < NC; ++c) {
+        double s = 0;
+        for (int d = 0; d < NC; ++d) s += Dinv[(size_t)r * NC * NC + c * NC + d] * vj[(size_t)r * NC + d];
+        Z[(size_t)r * NC + c] = s;
+      }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int r = 0; r < n_nodes; ++r) {
+      double acc[NC] = {0};
+      for (int b = row_ptr[r]; b < row_ptr[r + 1]; ++b) {
+        const double *B = val + (size_t)b * NC * NC, *zc = Z + (size_t)col[b] * NC;
+        for (int c = 0; c < NC; ++c) {
+          double s = 0;
+          for (int d = 0; d < NC; ++d) s += B[c * NC + d] * zc[d];
+          acc[c] += s;
+        }
+      }
+      memcpy(w + (size_t)r * NC, acc, sizeof acc);
+    }
+    for (int i = 0; i <= j; ++i) {
+      double s = 0;
+      const double *vi = V + n * i;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : s) schedule(static)
+#endif
+      for (size_t k = 0; k < n; ++k) s += vi[k] * w[k];
+      h[i] = s;
+    }
+    double nn = 0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : nn) schedule(static)
+#endif
+    for (size_t k = 0; k < n; ++k) {
+      double v = w[k];
+      for (int i = 0; i <= j; ++i) v -= h[i] * V[n * i + k];
+      w[k] = v;
+      nn += v * v;
+    }
+    const double inv = nn > 0 ? 1.0 / sqrt(nn) : 0.0;
+    double *vn = V + n * (j + 1);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (size_t k = 0; k < n; ++k) vn[k] = w[k] * inv;
+  }
+  double el = wall() - t0;
+  free(Dinv); free(V); free(Z); free(w); free(h);
+  return el;
+}
+
+/* Residual-only cell work (compute_residual, residual.cc:166-229) of n
+ * sample cells with the factorized form, in parallel; wall seconds. */
+double orc_time_residual_cells(int deg, int n, const double *h, const double *co, const double *Uloc_all,
+                               int nthreads, double *checksum)
+{
+  int ns = (deg + 1) * (deg + 1) * (deg + 1), dpc = ns * NC;
+  double t0 = wall(), sum = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel reduction(+ : sum)
+#endif
+  {
+    double *R = (double *)malloc(sizeof(double) * dpc);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 4)
+#endif
+    for (int e = 0; e < n; ++e) {
+      orc_cell_factorized(deg, h, co, Uloc_all + (size_t)e * dpc, NULL, R);
+      sum += R[e % dpc];
+    }
+    free(R);
+  }
+  if (checksum) checksum[0] = sum;
+  return wall() - t0;
+}
+
 int orc_max_threads(void)
 {
 #ifdef _OPENMP

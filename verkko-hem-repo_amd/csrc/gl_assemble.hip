@@ -490,172 +490,248 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
   }
 }
 
-// threads per workgroup of the matrix-free product: at most two of the
-// cell's (point, component) items per thread and pass (Q1: 144 items on 128
-// threads; Q2: 486 on 256 with the FP64 Hessian, on 512 with the FP32 one,
-// each measured the faster of the two for its type)
+// The product in sum-factorised form: N_b(q) = L_bx(x_qx) L_by(x_qy) L_bz(x_qz),
+// so x and its gradients at the quadrature points take three 1D passes (x, y,
+// z) and the test against N_a and its gradients three more (z, y, x),
+// 2 x 18 x 27 x 27 multiply-adds per Q2 cell instead of 8 x 18 x 27 x 27.
+// One thread per 1D line of points and component (a pencil) in each pass: it
+// reads its N1 inputs once and applies the 1D bases from registers, so the
+// passes move about 2.5x fewer LDS words than one thread per (point,
+// component) with the bases in LDS.  The thread's Hessian rows (H(q) X(q) has
+// one thread per (point, component) item) are requested before the passes
+// start, so the loads that dominate the HBM traffic are in flight while the
+// cell interpolates x.  TH: stored Hessian (FP64, or FP32 for the FP32 inner
+// solve); TX: x.  The arithmetic is FP64; 35 KB of LDS per cell.
 template <int DEG, typename TH>
-constexpr int mf_threads() { return DEG == 1 ? 128 : (sizeof(TH) == 4 ? 512 : 256); }
+constexpr int mf_threads() { return DEG == 1 ? 128 : 256; }
 
-// The product runs in sum-factorised form: N_b(q) = L_bx(x_qx) L_by(x_qy)
-// L_bz(x_qz), so x and its gradients at the quadrature points take three 1D
-// passes (x, y, z) and the test against N_a and its gradients three more (z,
-// y, x), 2 x 18 x 27 x 27 multiply-adds per Q2 cell instead of 8 x 18 x 27 x 27.
-// TH: stored Hessian (FP64, or FP32 for the FP32 inner solve); TX: x.  The
-// arithmetic is FP64.
 template <int DEG, typename TH, typename TX>
 __global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables T, MeshDev Mh, Material M0,
-                                                                      const TH *__restrict__ Hq,
-                                                                      const TX *__restrict__ x,
-                                                                      const int *__restrict__ cells,
-                                                                      const int *__restrict__ flag,
-                                                                      double *__restrict__ ycell)
+                                                                          const TH *__restrict__ Hq,
+                                                                          const TX *__restrict__ x,
+                                                                          const int *__restrict__ cells,
+                                                                          const int *__restrict__ flag,
+                                                                          double *__restrict__ ycell)
 {
   constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS, V = NS * 18, NTA = mf_threads<DEG, TH>();
-  __shared__ double sx[V];      // x at the cell's nodes, Dirichlet components 0
-  __shared__ double R1[5 * V];  // interpolation passes; then w vol H X and w vol gamma / h; then the y pass
-  __shared__ double R2[4 * V];  // x and its gradients at the quadrature points; then the z pass
-  __shared__ double sL[N1 * N1], sdL[N1 * N1];
+  constexpr int NP = N1 * N1 * 18;             // pencils per pass
+  constexpr int NIT = (V + NTA - 1) / NTA;     // (point, component) items per thread in H X
+  __shared__ double sx[V];     // x at the cell's nodes, Dirichlet components 0
+  __shared__ double R1[4 * V];  // B0..B2, then w vol H X and w vol gamma / h, then D0, D1
+  __shared__ double R2[4 * V];  // A0, A1, then X and its gradients, then C0..C2
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
   if (flag && !flag[Mh.cell_inst[e]]) return;  // instance not iterating: its rows are not read
+  // the Hessian rows of this thread's (point, component) items, requested first
+  const TH *Hc = Hq + (size_t)e * NQ * NUPAIR;
+  TH hv[NIT][18];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int i = tid + NTA * it;
+    if (i < V) {
+      const TH *Hrow = Hc + (i / 18) * NUPAIR;
+      const int c = i % 18;
+      // upair(c, d) for d = 0, 1, ...: column c of rows d < c (steps 17 - d),
+      // then row c itself (steps of 1)
+      int off = c;
+#pragma unroll
+      for (int d = 0; d < 18; ++d) {
+        hv[it][d] = Hrow[off];
+        off += d < c ? 17 - d : 1;
+      }
+    }
+  }
   const Material M = Mh.mats ? Mh.mats[Mh.cell_mat[e]] : M0;
   const int *cn = Mh.cell_nodes + (size_t)e * NS;
   double h[3], ih[3];
   for (int k = 0; k < 3; ++k) { h[k] = Mh.cell_h[3 * e + k]; ih[k] = 1.0 / h[k]; }
   const double vol = h[0] * h[1] * h[2];
   const uint32_t cface = Mh.cell_face[e];
-  if (tid < N1 * N1) {
-    sL[tid] = T.L1[tid];
-    sdL[tid] = T.dL1[tid];
-  }
+  double L[N1][N1], dL[N1][N1];  // L[q][b] = L_b(x_q): uniform, scalar loads
+#pragma unroll
+  for (int q = 0; q < N1; ++q)
+#pragma unroll
+    for (int b = 0; b < N1; ++b) {
+      L[q][b] = T.L1[q * N1 + b];
+      dL[q][b] = T.dL1[q * N1 + b];
+    }
   for (int i = tid; i < V; i += NTA) {
     const int node = cn[i / 18], c = i % 18;
     sx[i] = ((Mh.node_mask[node] >> c) & 1u) ? 0.0 : (double)x[(size_t)node * 18 + c];
   }
   __syncthreads();
-  // index of point (i0, i1, i2) (i0 fastest), component c
   auto at = [](int i0, int i1, int i2, int c) { return ((i2 * N1 + i1) * N1 + i0) * 18 + c; };
-  double *A0 = R1, *A1 = R1 + V, *B0 = R1 + 2 * V, *B1 = R1 + 3 * V, *B2 = R1 + 4 * V;
+  double *A0 = R2, *A1 = R2 + V;
+  double *B0 = R1, *B1 = R1 + V, *B2 = R1 + 2 * V;
   double *sX = R2, *sG = R2 + V;  // sG: [k][q][c]
-  // x pass: (qx, by, bz)
-  for (int i = tid; i < V; i += NTA) {
-    const int c = i % 18, r = i / 18, qx = r % N1, by = (r / N1) % N1, bz = r / (N1 * N1);
-    double a0 = 0.0, a1 = 0.0;
+  double *sZ = R1, *sK = R1 + V;  // sK: [k][q][c]
+  double *C0 = R2, *C1 = R2 + V, *C2 = R2 + 2 * V;
+  double *D0 = R1, *D1 = R1 + V;
+  const int c = tid % 18, l = tid / 18, la = l % N1, lb = l / N1;  // pencil: line (la, lb), component c
+  const bool pen = tid < NP;
+  // x pass: line (by, bz) -> (qx, by, bz)
+  if (pen) {
+    double v[N1];
 #pragma unroll
-    for (int bx = 0; bx < N1; ++bx) {
-      const double v = sx[at(bx, by, bz, c)];
-      a0 += sL[qx * N1 + bx] * v;
-      a1 += sdL[qx * N1 + bx] * v;
+    for (int b = 0; b < N1; ++b) v[b] = sx[at(b, la, lb, c)];
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+      double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+      for (int b = 0; b < N1; ++b) {
+        a0 += L[q][b] * v[b];
+        a1 += dL[q][b] * v[b];
+      }
+      A0[at(q, la, lb, c)] = a0;
+      A1[at(q, la, lb, c)] = a1;
     }
-    A0[i] = a0;
-    A1[i] = a1;
   }
   __syncthreads();
-  // y pass: (qx, qy, bz)
-  for (int i = tid; i < V; i += NTA) {
-    const int c = i % 18, r = i / 18, qx = r % N1, qy = (r / N1) % N1, bz = r / (N1 * N1);
-    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+  // y pass: line (qx, bz) -> (qx, qy, bz)
+  if (pen) {
+    double v0[N1], v1[N1];
 #pragma unroll
-    for (int by = 0; by < N1; ++by) {
-      const int k = at(qx, by, bz, c);
-      b0 += sL[qy * N1 + by] * A0[k];
-      b1 += sL[qy * N1 + by] * A1[k];
-      b2 += sdL[qy * N1 + by] * A0[k];
+    for (int b = 0; b < N1; ++b) {
+      v0[b] = A0[at(la, b, lb, c)];
+      v1[b] = A1[at(la, b, lb, c)];
     }
-    B0[i] = b0;
-    B1[i] = b1;
-    B2[i] = b2;
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+      double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+#pragma unroll
+      for (int b = 0; b < N1; ++b) {
+        b0 += L[q][b] * v0[b];
+        b1 += L[q][b] * v1[b];
+        b2 += dL[q][b] * v0[b];
+      }
+      B0[at(la, q, lb, c)] = b0;
+      B1[at(la, q, lb, c)] = b1;
+      B2[at(la, q, lb, c)] = b2;
+    }
   }
   __syncthreads();
-  // z pass: (qx, qy, qz) = q
-  for (int i = tid; i < V; i += NTA) {
-    const int c = i % 18, r = i / 18, qx = r % N1, qy = (r / N1) % N1, qz = r / (N1 * N1);
-    double xv = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  // z pass: line (qx, qy) -> (qx, qy, qz) = q
+  if (pen) {
+    double v0[N1], v1[N1], v2[N1];
 #pragma unroll
-    for (int bz = 0; bz < N1; ++bz) {
-      const int k = at(qx, qy, bz, c);
-      xv += sL[qz * N1 + bz] * B0[k];
-      g0 += sL[qz * N1 + bz] * B1[k];
-      g1 += sL[qz * N1 + bz] * B2[k];
-      g2 += sdL[qz * N1 + bz] * B0[k];
+    for (int b = 0; b < N1; ++b) {
+      v0[b] = B0[at(la, lb, b, c)];
+      v1[b] = B1[at(la, lb, b, c)];
+      v2[b] = B2[at(la, lb, b, c)];
     }
-    sX[i] = xv;
-    sG[i] = g0 * ih[0];
-    sG[V + i] = g1 * ih[1];
-    sG[2 * V + i] = g2 * ih[2];
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+      double xv = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+#pragma unroll
+      for (int b = 0; b < N1; ++b) {
+        xv += L[q][b] * v0[b];
+        g0 += L[q][b] * v1[b];
+        g1 += L[q][b] * v2[b];
+        g2 += dL[q][b] * v0[b];
+      }
+      const int o = at(la, lb, q, c);
+      sX[o] = xv;
+      sG[o] = g0 * ih[0];
+      sG[V + o] = g1 * ih[1];
+      sG[2 * V + o] = g2 * ih[2];
+    }
   }
   __syncthreads();
   // w vol H(q) X(q) and w vol gamma_k(q) / h_k
-  double *sZ = R1, *sK = R1 + V;  // sK: [k][q][c]
-  const TH *Hc = Hq + (size_t)e * NQ * NUPAIR;
-  for (int i = tid; i < V; i += NTA) {
-    const int q = i / 18, c = i % 18;
-    const TH *Hrow = Hc + q * NUPAIR;
-    const double *X = sX + q * 18;
-    double z = 0.0;
 #pragma unroll
-    for (int d = 0; d < 18; ++d) z += (double)Hrow[upair(c, d)] * X[d];
-    const double wv = T.w[q] * vol;
-    sZ[i] = wv * z;
-    const int part = c / 9, mu = (c % 9) / 3, kc = c % 3, base = q * 18 + 9 * part + 3 * mu;
-    const double dv = sG[base] + sG[V + base + 1] + sG[2 * V + base + 2];
+  for (int it = 0; it < NIT; ++it) {
+    const int i = tid + NTA * it;
+    if (i < V) {
+      const int q = i / 18, ci = i % 18;
+      const double *X = sX + q * 18;
+      double z = 0.0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) sK[k * V + i] = wv * ih[k] * (M.K1 * sG[k * V + i] + (k == kc ? M.K23 * dv : 0.0));
+      for (int d = 0; d < 18; ++d) z += (double)hv[it][d] * X[d];
+      const double wv = T.w[q] * vol;
+      sZ[i] = wv * z;
+      const int part = ci / 9, mu = (ci % 9) / 3, kc = ci % 3, base = q * 18 + 9 * part + 3 * mu;
+      const double dv = sG[base] + sG[V + base + 1] + sG[2 * V + base + 2];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        sK[k * V + i] = wv * ih[k] * (M.K1 * sG[k * V + i] + (k == kc ? M.K23 * dv : 0.0));
+    }
   }
   __syncthreads();
-  // test z pass: (qx, qy, az)
-  double *C0 = R2, *C1 = R2 + V, *C2 = R2 + 2 * V;
-  for (int i = tid; i < V; i += NTA) {
-    const int c = i % 18, r = i / 18, qx = r % N1, qy = (r / N1) % N1, az = r / (N1 * N1);
-    double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+  // test z pass: line (qx, qy) -> (qx, qy, az)
+  if (pen) {
+    double z[N1], k0[N1], k1[N1], k2[N1];
 #pragma unroll
-    for (int qz = 0; qz < N1; ++qz) {
-      const int k = at(qx, qy, qz, c);
-      const double l = sL[qz * N1 + az];
-      c0 += l * sZ[k] + sdL[qz * N1 + az] * sK[2 * V + k];
-      c1 += l * sK[k];
-      c2 += l * sK[V + k];
+    for (int q = 0; q < N1; ++q) {
+      const int o = at(la, lb, q, c);
+      z[q] = sZ[o];
+      k0[q] = sK[o];
+      k1[q] = sK[V + o];
+      k2[q] = sK[2 * V + o];
     }
-    C0[i] = c0;
-    C1[i] = c1;
-    C2[i] = c2;
-  }
-  __syncthreads();
-  // test y pass: (qx, ay, az)
-  double *D0 = R1, *D1 = R1 + V;
-  for (int i = tid; i < V; i += NTA) {
-    const int c = i % 18, r = i / 18, qx = r % N1, ay = (r / N1) % N1, az = r / (N1 * N1);
-    double d0 = 0.0, d1 = 0.0;
 #pragma unroll
-    for (int qy = 0; qy < N1; ++qy) {
-      const int k = at(qx, qy, az, c);
-      d0 += sL[qy * N1 + ay] * C0[k] + sdL[qy * N1 + ay] * C2[k];
-      d1 += sL[qy * N1 + ay] * C1[k];
-    }
-    D0[i] = d0;
-    D1[i] = d1;
-  }
-  __syncthreads();
-  // test x pass: (ax, ay, az) = a, then the Robin faces; one slot per cell
-  for (int i = tid; i < V; i += NTA) {
-    const int c = i % 18, a = i / 18, ax = a % N1, ay = (a / N1) % N1, az = a / (N1 * N1);
-    double r = 0.0;
+    for (int a = 0; a < N1; ++a) {
+      double c0 = 0.0, c1 = 0.0, c2 = 0.0;
 #pragma unroll
-    for (int qx = 0; qx < N1; ++qx) {
-      const int k = at(qx, ay, az, c);
-      r += sL[qx * N1 + ax] * D0[k] + sdL[qx * N1 + ax] * D1[k];
-    }
-    if (M.robin != 0.0)
-      for (int lf = 0; lf < 6; ++lf) {
-        const int bid = (int)((cface >> (3 * lf)) & 7u) - 1;
-        if (bid < 2 || bid > 4 || face_zeroed(bid, c)) continue;
-        const double *mf = T.Mf + (lf * NS + a) * NS;
-        double sf = 0.0;
-        for (int b = 0; b < NS; ++b) sf += mf[b] * sx[b * 18 + c];
-        r += (M.robin * face_area(lf, h)) * sf;
+      for (int q = 0; q < N1; ++q) {
+        c0 += L[q][a] * z[q] + dL[q][a] * k2[q];
+        c1 += L[q][a] * k0[q];
+        c2 += L[q][a] * k1[q];
       }
-    ycell[(size_t)e * V + i] = r;  // Dirichlet rows are replaced in the gather
+      const int o = at(la, lb, a, c);
+      C0[o] = c0;
+      C1[o] = c1;
+      C2[o] = c2;
+    }
+  }
+  __syncthreads();
+  // test y pass: line (qx, az) -> (qx, ay, az)
+  if (pen) {
+    double v0[N1], v1[N1], v2[N1];
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+      const int o = at(la, q, lb, c);
+      v0[q] = C0[o];
+      v1[q] = C1[o];
+      v2[q] = C2[o];
+    }
+#pragma unroll
+    for (int a = 0; a < N1; ++a) {
+      double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+      for (int q = 0; q < N1; ++q) {
+        d0 += L[q][a] * v0[q] + dL[q][a] * v2[q];
+        d1 += L[q][a] * v1[q];
+      }
+      D0[at(la, a, lb, c)] = d0;
+      D1[at(la, a, lb, c)] = d1;
+    }
+  }
+  __syncthreads();
+  // test x pass: line (ay, az) -> (ax, ay, az) = a, then the Robin faces
+  if (pen) {
+    double v0[N1], v1[N1];
+#pragma unroll
+    for (int q = 0; q < N1; ++q) {
+      v0[q] = D0[at(q, la, lb, c)];
+      v1[q] = D1[at(q, la, lb, c)];
+    }
+#pragma unroll
+    for (int a = 0; a < N1; ++a) {
+      double r = 0.0;
+#pragma unroll
+      for (int q = 0; q < N1; ++q) r += L[q][a] * v0[q] + dL[q][a] * v1[q];
+      const int na = (lb * N1 + la) * N1 + a, o = at(a, la, lb, c);
+      if (M.robin != 0.0)
+        for (int lf = 0; lf < 6; ++lf) {
+          const int bid = (int)((cface >> (3 * lf)) & 7u) - 1;
+          if (bid < 2 || bid > 4 || face_zeroed(bid, c)) continue;
+          const double *mf = T.Mf + (lf * NS + na) * NS;
+          double sf = 0.0;
+          for (int b = 0; b < NS; ++b) sf += mf[b] * sx[b * 18 + c];
+          r += (M.robin * face_area(lf, h)) * sf;
+        }
+      ycell[(size_t)e * V + o] = r;  // Dirichlet rows are replaced in the gather
+    }
   }
 }
 
@@ -830,9 +906,9 @@ hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, cons
 {
   if (n <= 0) return hipSuccess;
   if (x32 && !h32) return hipErrorInvalidValue;  // an FP64 Hessian is applied to FP64 vectors only
-#define GLH_MFA(D, TH, TX) \
-  hipLaunchKernelGGL((k_mf_apply<D, TH, TX>), dim3(n), dim3((mf_threads<D, TH>())), 0, s, T, M, mat, (const TH *)Hq, (const TX *)x, cells, \
-                     flag, ycell)
+#define GLH_MFA(D, TH, TX)                                                                                  \
+  hipLaunchKernelGGL((k_mf_apply<D, TH, TX>), dim3(n), dim3((mf_threads<D, TH>())), 0, s, T, M, mat, (const TH *)Hq, \
+                     (const TX *)x, cells, flag, ycell)
   if (deg == 1) {
     if (h32) { if (x32) GLH_MFA(1, float, float); else GLH_MFA(1, float, double); }
     else GLH_MFA(1, double, double);

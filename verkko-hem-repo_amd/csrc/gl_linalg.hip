@@ -511,6 +511,11 @@ __device__ inline typename NV2<T>::type ld2(const T *__restrict__ p)
 }
 
 // 2D grid: x = chunks (or the active list), y = groups of KCH vectors.
+// Without an active set the grid is numbered XCD by XCD: workgroups go to
+// the 8 XCDs in turn (linear id mod 8), so linear id L = 8 r + x runs chunk
+// 8 (r / groups) + x, group r % groups, and every group of a chunk runs on
+// the XCD of that chunk, whose L2 then serves w to all but the first
+// (gridDim.x is a multiple of 8 there).
 template <typename T>
 __global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__restrict__ ch, int n_chunks,
                                                          ActiveSet act, int k, const T *__restrict__ V, size_t ldv,
@@ -518,11 +523,17 @@ __global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__rest
                                                          const int *__restrict__ flag)
 {
   __shared__ T sh[KCH][RT / 64];
-  const int nw = work_count(act, n_chunks);
-  const int i0 = blockIdx.y * KCH;
+  int nw = work_count(act, n_chunks), first = blockIdx.x, step = gridDim.x, grp = blockIdx.y;
+  if (!act.counts) {
+    const int L = blockIdx.x + gridDim.x * blockIdx.y, r = L >> 3;
+    grp = r % gridDim.y;
+    first = 8 * (r / gridDim.y) + (L & 7);
+    step = nw;  // one chunk per workgroup
+  }
+  const int i0 = grp * KCH;
   const int kk = (k - i0) < KCH ? (k - i0) : KCH;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int wi = blockIdx.x; wi < nw; wi += gridDim.x) {
+  for (int wi = first; wi < nw; wi += step) {
     const int cb = work_chunk(act, wi);
     if (flag && !flag[ch[3 * cb]]) continue;  // skipped instance: its partials are never read
     const long long beg = ch[3 * cb + 1], end = ch[3 * cb + 2];
@@ -1036,8 +1047,9 @@ hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, cons
                            int off, bool accumulate, hipStream_t s, const int *flag)
 {
   if (k <= 0 || S.n_chunks <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_seg_mdot_partial<T>), dim3(chunk_grid(S), (k + KCH - 1) / KCH), dim3(RT), 0, s, S.chunks,
-                     S.n_chunks, S.act, k, V, ldv, w, partial, flag);
+  const int gx = S.act.counts ? chunk_grid(S) : (S.n_chunks + 7) / 8 * 8;  // XCD-numbered: a multiple of 8
+  hipLaunchKernelGGL((k_seg_mdot_partial<T>), dim3(gx, (k + KCH - 1) / KCH), dim3(RT), 0, s, S.chunks, S.n_chunks,
+                     S.act, k, V, ldv, w, partial, flag);
   hipLaunchKernelGGL((k_seg_final<T>), dim3(seg_grid(S), k), dim3(RT), 0, s, S.seg_chunk, S.n_seg, S.act, k, partial,
                      out, ldo, off, accumulate ? 1 : 0, flag);
   return hipGetLastError();

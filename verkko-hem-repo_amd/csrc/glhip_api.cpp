@@ -163,7 +163,7 @@ struct glh_ctx {
     int q = 1;
   } graph64, graph32;
   HBuf<float> pin32;
-  HBuf<int> pin_i;
+  HBuf<int> pin_i, pin_fl;  // pinned staging of the per-instance flags and counters
   DBuf<double> ir_r;
   // per-instance results of the last Newton solve: 8 values per instance
   std::vector<double> inst_stats;
@@ -704,7 +704,9 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
   for (auto *q : ib) HIPCHK(c, q->alloc(S));
   HIPCHK(c, c->pinv.alloc((size_t)4 * S));
   HIPCHK(c, c->act_counts.alloc(4));
-  HIPCHK(c, c->pin_i.alloc(4));
+  // every host <-> device copy of the loop below goes through pinned memory
+  HIPCHK(c, c->pin_i.alloc(2 * (size_t)S + 4));
+  HIPCHK(c, Work<TV>::pin(c).alloc(S));
   HIPCHK(c, c->act_seg.alloc(S));
   HIPCHK(c, c->act_rpref.alloc(S + 1));
   HIPCHK(c, c->act_chunk.alloc(std::max(1, c->n_chunks)));
@@ -714,8 +716,8 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
                       DGKS_RATIO};
   // ||b_s|| and the absolute targets
   HIPCHK(c, glh::launch_seg_mdot<TV>(SI, 1, b, n, b, pt.p, sc.p, 1, 0, false, s));
-  std::vector<TV> bn2(S);
-  HIPCHK(c, hipMemcpyAsync(bn2.data(), sc.p, S * sizeof(TV), hipMemcpyDeviceToHost, s));
+  TV *bn2 = Work<TV>::pin(c).p;
+  HIPCHK(c, hipMemcpyAsync(bn2, sc.p, S * sizeof(TV), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
   std::vector<double> bn(S);
   for (int i = 0; i < S; ++i) {
@@ -787,12 +789,11 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
   // cycle loop
   int it = 0;
   bool conv = false;
-  std::vector<int> hconv(S), hj(S);
-  int *hcnt = c->pin_i.p;
+  int *hconv = c->pin_i.p, *hj = hconv + S, *hcnt = hj + S;
   while (true) {
     RCHK(launch(0, 0, 0));
-    HIPCHK(c, hipMemcpyAsync(hconv.data(), c->gmconv.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(c, hipMemcpyAsync(hj.data(), c->gmjlast.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(hconv, c->gmconv.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(hj, c->gmjlast.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
     if (S > 1) HIPCHK(c, hipMemcpyAsync(hcnt, c->act_counts.p, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     bool all = true;
@@ -815,7 +816,7 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
     for (int blk = 0; blk < nblk; ++blk) {
       RCHK(launch(1, lev, blk));
       if (blk + 1 == nblk) break;
-      HIPCHK(c, hipMemcpyAsync(hconv.data(), c->gmcyc.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(c, hipMemcpyAsync(hconv, c->gmcyc.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
       HIPCHK(c, hipStreamSynchronize(s));
       bool any = false;
       for (int i = 0; i < S && !any; ++i) any = hconv[i] != 0;
@@ -825,12 +826,11 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
   }
   RCHK(flush_events(c));
   // the last head computed the true residual norms of the final iterate
-  std::vector<int> hits(S);
   HIPCHK(c, hipMemcpyAsync(c->pinv.p, c->gmbeta.p, S * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIPCHK(c, hipMemcpyAsync(hits.data(), c->gmits.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(hconv, c->gmits.p, S * sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
   *its_out = it;
-  its_seg = hits;
+  its_seg.assign(hconv, hconv + S);
   rel_out.assign(S, 0.0);
   for (int i = 0; i < S; ++i) {
     const double r = c->pinv.p[i];
@@ -883,7 +883,9 @@ int solve_impl(glh_ctx *c, const std::vector<double> &rtolv, int maxit, int rest
         if (flag[i]) rt[i] = std::max(inner, rtolv[i] * bn / rn);
       }
       if (!any) break;
-      HIPCHK(c, hipMemcpyAsync(c->s_flag.p, flag.data(), S * sizeof(int), hipMemcpyHostToDevice, s));
+      HIPCHK(c, c->pin_fl.alloc(S));
+      std::copy(flag.begin(), flag.end(), c->pin_fl.p);
+      HIPCHK(c, hipMemcpyAsync(c->s_flag.p, c->pin_fl.p, S * sizeof(int), hipMemcpyHostToDevice, s));
       HIPCHK(c, (glh::launch_convert<double, float>(n, c->ir_r.p, c->gb32.p, s)));
       HIPCHK(c, glh::launch_seg_zero<float>(c->seg(), c->s_flag.p, c->gb32.p, s));
       int it2 = 0;
@@ -1062,6 +1064,7 @@ int glh_destroy(glh_ctx *c)
   c->pinv.release();
   c->pin32.release();
   c->pin_i.release();
+  c->pin_fl.release();
   for (auto e : c->ev_pool)
     if (e) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1823,17 +1826,22 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
     // (a buffer of its own: solve_impl reuses s_flag for its refinement sweeps)
     for (int i = 0; i < S; ++i) flag[i] = active[i];
     HIPCHK(c, c->asm_flag.alloc(S));
-    HIPCHK(c, hipMemcpyAsync(c->asm_flag.p, flag.data(), S * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->s_flag.p, flag.data(), S * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, c->pin_fl.alloc(S));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // the pinned flags of the last step are no longer read
+    std::copy(flag.begin(), flag.end(), c->pin_fl.p);
+    HIPCHK(c, hipMemcpyAsync(c->asm_flag.p, c->pin_fl.p, S * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->s_flag.p, c->pin_fl.p, S * sizeof(int), hipMemcpyHostToDevice, c->stream));
     c->asm_active = S > 1 ? c->asm_flag.p : nullptr;
     RCHK(assemble_at(c, c->U.p, true, true, f32, c->rhs.p, false, mf));
     // pseudo-transient continuation: J + sigma_k |diag J|, sigma_k =
     // sigma0 ||R_k|| / ||R_0|| per instance (switched evolution relaxation)
     if (c->ptc_sigma0 > 0) {
-      std::vector<double> sg(S);
+      HIPCHK(c, c->pin64.alloc(S));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      double *sg = c->pin64.p;  // pinned: the copy below is asynchronous
       for (int i = 0; i < S; ++i) sg[i] = active[i] ? c->ptc_sigma0 * rcur[i] / std::max(r0[i], 1e-300) : 0.0;
       HIPCHK(c, c->s_norm.alloc(S));
-      HIPCHK(c, hipMemcpyAsync(c->s_norm.p, sg.data(), S * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->s_norm.p, sg, S * sizeof(double), hipMemcpyHostToDevice, c->stream));
       if (f32)
         HIPCHK(c, glh::launch_diag_shift<float>(c->n_nodes, c->diag.p, c->node_mask.p, c->row_seg.p, c->s_norm.p,
                                                 c->J32.p, c->stream));

@@ -38,6 +38,30 @@ def slab_instances(G, n, seed):
     return out
 
 
+def sweep_instances(G, n_inst=256, seed=2):
+    """The batched phase-diagram sweep: n_inst slab instances, T/Tc ~
+    U[0.3,0.95], p ~ U[0,34) bar, thickness D ~ U[6,40] xi0, lateral extent
+    40 xi0, Q1 meshes of nx = ny in 8..16 and nz in 2..8 cells, A = 0 on the
+    two slab faces and natural on the sides; B phase + 0.05 Delta_B noise
+    (seed 1000 + i).  Returns (meshes, coefficient rows, guesses, params)."""
+    rng = np.random.default_rng(seed)
+    T = rng.uniform(0.3, 0.95, n_inst)
+    P = np.minimum(rng.uniform(0.0, 34.0, n_inst), np.nextafter(34.0, 0.0))
+    D = rng.uniform(6.0, 40.0, n_inst)
+    NX = rng.integers(8, 17, n_inst)
+    NZ = rng.integers(2, 9, n_inst)
+    meshes, coefs, U = [], [], []
+    h = LATERAL / 2
+    for i in range(n_inst):
+        m = G.Mesh(1, (int(NX[i]), int(NX[i]), int(NZ[i])), (-h, -h, -D[i] / 2), (h, h, D[i] / 2),
+                   [1, 1, 1, 1, 0, 0])
+        rc, mp = G.matep(P[i], T[i])
+        meshes.append(m)
+        coefs.append([mp[0], mp[1], mp[2], mp[3], mp[4], mp[5], K_CONST[0], K_CONST[1], K_CONST[2], 1.0e10])
+        U.append(G.initial_guess(m.n_nodes, mp[9], phase=0, amp=0.05, seed=1000 + i))
+    return meshes, np.array(coefs), U, dict(T=T, p=P, D=D, nx=NX, nz=NZ)
+
+
 def features(inst):
     """Network inputs of the nodes of an instance, each scaled to about [-1, 1]."""
     xyz = inst.xyz
