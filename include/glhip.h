@@ -358,6 +358,20 @@ int glh_rhs_norm(glh_ctx *ctx, double *norm);
 /* FemGL::compute_residual (residual.cc:109-297) at the current solution:
  * writes ||R|| (R = residual_vector). */
 int glh_residual(glh_ctx *ctx, double *norm);
+/* A variant of glh_residual that allocates nothing.  It has no counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  U (n_dofs), R
+ * (n_dofs, receives system_rhs = -R at U) and norm2 (n_inst, receives
+ * ||R_s||^2 per instance) are device pointers; the call only enqueues
+ * kernels on the context stream and returns (synchronise the device before
+ * reading the results).  The partial sums of the norm live in a fixed
+ * scratch array of 65536 doubles per device (one per reduction chunk of 1024
+ * dofs, 4096 in instances of 2^20 dofs and more), so meshes needing more
+ * chunks are refused with GLH_ERR_ARG; every call on one device shares that
+ * array, so calls that overlap on a device race.  The mesh and material come
+ * from the context as for glh_residual. */
+int glh_residual_into(glh_ctx *ctx, const double *U, double *R, double *norm2);
+
 /* FemGL::solve (solve.cc:108-187) with GMRES(restart) + 18x18 point-block
  * Jacobi instead of FGMRES + ML AMG: J dU = system_rhs to
  * ||r|| <= rtol * ||system_rhs||.  stats[0] = iterations, stats[1] = final

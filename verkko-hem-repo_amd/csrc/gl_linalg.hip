@@ -11,7 +11,32 @@
 // wavefront covering one contiguous span of the block.
 #include "glhip_dev.hpp"
 
+#include <mutex>
+#include <vector>
+
 namespace glh {
+
+// Scratch of the no-allocation residual: the per-chunk partial sums of its
+// norm reduction (one value per reduction chunk).  Every call on one device
+// shares this array, so calls that overlap on a device race.
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+hipError_t scratch_address(int device, double **out)
+{
+  static std::mutex mu;
+  static std::vector<double *> addr;  // per device, found once
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)addr.size() <= device) addr.resize(device + 1, nullptr);
+  if (!addr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = static_cast<double *>(p);
+  }
+  *out = addr[device];
+  return hipSuccess;
+}
+
 namespace {
 
 template <typename T> struct V2t;

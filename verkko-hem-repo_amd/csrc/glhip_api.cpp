@@ -184,6 +184,7 @@ struct glh_ctx {
   DBuf<int> asm_flag;
   std::vector<double> r_ref;  // per-instance stopping reference (empty: ||R0|| of the solve)
   long reorth_count = 0;
+  double *scratch = nullptr;  // device address of the library's scratch array on this device
   std::vector<hipEvent_t> ev_pool;
   int ev_used = 0;
   double spmv_ms_total = 0.0, spmv_count = 0;
@@ -1015,7 +1016,8 @@ int glh_create(int device, glh_ctx **out)
   glh_ctx *c = new glh_ctx();
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      glh::scratch_address(device, &c->scratch) != hipSuccess) {
     glh_destroy(c);  // releases whatever was created
     return GLH_ERR_HIP;
   }
@@ -1705,6 +1707,33 @@ int glh_residual(glh_ctx *c, double *nrm)
   RCHK(check_ready(c));
   RCHK(assemble_at(c, c->U.p, false, true, false, c->res.p, false));
   return norm64(c, c->res.p, nrm);
+}
+
+int glh_residual_into(glh_ctx *c, const double *U, double *R, double *norm2)
+{
+  RCHK(check_ready(c));
+  if (!U || !R || !norm2) return fail(c, GLH_ERR_ARG, "null device pointer");
+  if ((long long)c->n_chunks > glh::SCRATCH_DOUBLES)
+    return fail(c, GLH_ERR_ARG,
+                "the mesh needs " + std::to_string(c->n_chunks) + " reduction chunks; glh_residual_into holds " +
+                    std::to_string(glh::SCRATCH_DOUBLES) + " (use glh_residual)");
+  const size_t n = ndofs(c);
+  hipStream_t s = c->stream;
+  // the cell loop of compute_residual (residual.cc:164-289) at U into R, or
+  // into the uploaded unconstrained buffer and then condensed into R
+  double *rcell = c->constrained ? c->rfull.p : R;
+  HIPCHK(c, glh::launch_fill<double>(n, 0.0, rcell, s));  // a kernel, not a memset: capturable
+  const glh::RefTables T = c->tables();
+  const glh::MeshDev M = c->meshdev();
+  for (int k = 0; k + 1 < (int)c->color_ptr.size(); ++k) {
+    const int off = c->color_ptr[k], nk = c->color_ptr[k + 1] - off;
+    HIPCHK(c, glh::launch_assemble(c->deg, false, true, false, T, M, c->mat, U, c->color_cells.p + off, nk, nullptr,
+                                   rcell, s));
+  }
+  if (c->constrained) HIPCHK(c, glh::launch_condense_rhs(c->cons, c->rfull.p, R, s));
+  // per-instance ||R_s||^2 with the chunk partials in the scratch array
+  HIPCHK(c, glh::launch_seg_mdot<double>(c->seg(), 1, R, n, R, c->scratch, norm2, 1, 0, false, s));
+  return GLH_OK;
 }
 
 int glh_solve(glh_ctx *c, double rtol, int maxit, int restart, double stats[2])

@@ -249,4 +249,12 @@ hipError_t launch_read(size_t nbytes, const void *src, float *out, int grid, hip
 // out = sum_i v[i] in fixed order (deterministic)
 hipError_t launch_sum(size_t n, const double *v, double *partial, double *out, hipStream_t s);
 
+// Device scratch of the no-allocation entry point (glh_residual_into): a
+// file-scope array of SCRATCH_DOUBLES doubles on every device the library
+// runs on, held until the process ends.  scratch_address gives its device
+// address on the current device (looked up once per device and kept); host
+// code must use that address, never the array's name.
+constexpr long long SCRATCH_DOUBLES = 1 << 16;
+hipError_t scratch_address(int device, double **out);
+
 }  // namespace glh

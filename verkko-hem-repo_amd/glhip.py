@@ -38,7 +38,7 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_set_schedule", "glh_autotune", "glh_prm_parse", "glh_prm_destroy", "glh_prm_get",
            "glh_prm_get_double", "glh_run_prm", "glh_write_vtu", "glh_initial_guess_field",
            "glh_transfer_solution", "glh_forest_set_degree", "glh_bench_read",
-           "glh_flag_refine_coarsen", "glh_mesh_from_arrays", "glh_mesh_last_error"]
+           "glh_flag_refine_coarsen", "glh_mesh_from_arrays", "glh_mesh_last_error", "glh_residual_into"]
 
 GLH_ORDER_LEXICOGRAPHIC, GLH_ORDER_HIERARCHIC = 0, 1
 
@@ -150,6 +150,7 @@ def lib():
     L.glh_mesh_from_arrays.argtypes = [C.POINTER(MeshDesc), C.POINTER(vp)]
     L.glh_mesh_last_error.argtypes = []
     L.glh_mesh_last_error.restype = C.c_char_p
+    L.glh_residual_into.argtypes = [vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -614,6 +615,13 @@ class Context:
         v = C.c_double()
         self._chk(lib().glh_residual(self.h, C.byref(v)), "glh_residual")
         return v.value
+
+    def residual_into(self, U_ptr, R_ptr, norm2_ptr):
+        """glh_residual_into on device pointers (ints): enqueues -R at U into
+        R and the per-instance ||R||^2 into norm2; allocates nothing and does
+        not synchronise."""
+        self._chk(lib().glh_residual_into(self.h, C.c_void_p(U_ptr), C.c_void_p(R_ptr), C.c_void_p(norm2_ptr)),
+                  "glh_residual_into")
 
     def solve(self, rtol=0.1, maxit=10000, restart=30):
         st = np.zeros(2)
