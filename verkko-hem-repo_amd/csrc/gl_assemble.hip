@@ -32,6 +32,8 @@
 // deal.II's AffineConstraints::distribute_local_to_global).
 #include "glhip_dev.hpp"
 
+#include <type_traits>
+
 namespace glh {
 namespace {
 
@@ -497,86 +499,108 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
 // One thread per 1D line of points and component (a pencil) in each pass: it
 // reads its N1 inputs once and applies the 1D bases from registers, so the
 // passes move about 2.5x fewer LDS words than one thread per (point,
-// component) with the bases in LDS.  The thread's Hessian rows (H(q) X(q) has
-// one thread per (point, component) item) are requested before the passes
-// start, so the loads that dominate the HBM traffic are in flight while the
-// cell interpolates x.  TH: stored Hessian (FP64, or FP32 for the FP32 inner
-// solve); TX: x.  The arithmetic is FP64; 35 KB of LDS per cell.
+// component) with the bases in LDS.  In H(q) X(q) a thread owns the
+// components c and c + 9 of one point (the same orbital entry in u and in v),
+// reads X(q) once for both, and has requested both Hessian rows before the
+// passes start, so the loads that dominate the HBM traffic are in flight
+// while the cell interpolates x.
+// TH: stored Hessian (FP64, or FP32 for the FP32 inner solve); TX: x.  TA,
+// the arithmetic, LDS and cell-slot type, is FP32 when both are FP32 (the
+// FP32 inner solve, with FP32 accumulation; 17.5 KB of LDS per Q2 cell) and
+// FP64 otherwise (35 KB).  cells[] lists the cells in the launch order, -1
+// for padding (the XCD-aware order of mf_cell_order).
 template <int DEG, typename TH>
 constexpr int mf_threads() { return DEG == 1 ? 128 : 256; }
+template <typename TH, typename TX>
+using mf_arith = typename std::conditional<std::is_same<TH, float>::value && std::is_same<TX, float>::value, float,
+                                           double>::type;
+
+// waves per SIMD the register allocation aims at: the FP64 kernel is held to
+// 4 workgroups per CU by its 35 KB of LDS anyway; the FP32 one (17.5 KB) is
+// held only by its registers
+template <typename TH, typename TX>
+constexpr int mf_waves() { return std::is_same<mf_arith<TH, TX>, float>::value ? 5 : 4; }
 
 template <int DEG, typename TH, typename TX>
-__global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables T, MeshDev Mh, Material M0,
-                                                                          const TH *__restrict__ Hq,
-                                                                          const TX *__restrict__ x,
-                                                                          const int *__restrict__ cells,
-                                                                          const int *__restrict__ flag,
-                                                                          double *__restrict__ ycell)
+__global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<TH, TX>())) void k_mf_apply(RefTables T, MeshDev Mh, Material M0,
+                                                                      const TH *__restrict__ Hq,
+                                                                      const TX *__restrict__ x,
+                                                                      const int *__restrict__ cells,
+                                                                      const int *__restrict__ flag,
+                                                                      mf_arith<TH, TX> *__restrict__ ycell)
 {
+  using TA = mf_arith<TH, TX>;
   constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS, V = NS * 18, NTA = mf_threads<DEG, TH>();
-  constexpr int NP = N1 * N1 * 18;             // pencils per pass
-  constexpr int NIT = (V + NTA - 1) / NTA;     // (point, component) items per thread in H X
-  __shared__ double sx[V];     // x at the cell's nodes, Dirichlet components 0
-  __shared__ double R1[4 * V];  // B0..B2, then w vol H X and w vol gamma / h, then D0, D1
-  __shared__ double R2[4 * V];  // A0, A1, then X and its gradients, then C0..C2
+  constexpr int NP = N1 * N1 * 18;  // pencils per pass
+  constexpr int NH = NQ * 9;        // (point, component pair) items of H X
+  static_assert(NH <= NTA, "one H X item per thread");
+  __shared__ TA sx[V];      // x at the cell's nodes, Dirichlet components 0
+  __shared__ TA R1[4 * V];  // B0..B2, then w vol H X and w vol gamma / h, then D0, D1
+  __shared__ TA R2[4 * V];  // A0, A1, then X and its gradients, then C0..C2
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
+  if (e < 0) return;                           // padding of the XCD-aware order
   if (flag && !flag[Mh.cell_inst[e]]) return;  // instance not iterating: its rows are not read
-  // the Hessian rows of this thread's (point, component) items, requested first
-  const TH *Hc = Hq + (size_t)e * NQ * NUPAIR;
-  TH hv[NIT][18];
+  // the two Hessian rows of this thread's H X item (point hq, components hc
+  // and hc + 9), requested first
+  const int hq = tid / 9, hc = tid % 9;
+  const bool hx = tid < NH;
+  TH hv[2][18];
+  if (hx) {
+    const TH *Hrow = Hq + ((size_t)e * NQ + hq) * NUPAIR;
 #pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    const int i = tid + NTA * it;
-    if (i < V) {
-      const TH *Hrow = Hc + (i / 18) * NUPAIR;
-      const int c = i % 18;
+    for (int r = 0; r < 2; ++r) {
+      const int c = hc + 9 * r;
       // upair(c, d) for d = 0, 1, ...: column c of rows d < c (steps 17 - d),
       // then row c itself (steps of 1)
       int off = c;
 #pragma unroll
       for (int d = 0; d < 18; ++d) {
-        hv[it][d] = Hrow[off];
+        hv[r][d] = Hrow[off];
         off += d < c ? 17 - d : 1;
       }
     }
   }
   const Material M = Mh.mats ? Mh.mats[Mh.cell_mat[e]] : M0;
   const int *cn = Mh.cell_nodes + (size_t)e * NS;
-  double h[3], ih[3];
-  for (int k = 0; k < 3; ++k) { h[k] = Mh.cell_h[3 * e + k]; ih[k] = 1.0 / h[k]; }
+  double h[3];
+  TA ih[3];
+  for (int k = 0; k < 3; ++k) { h[k] = Mh.cell_h[3 * e + k]; ih[k] = (TA)(1.0 / h[k]); }
   const double vol = h[0] * h[1] * h[2];
   const uint32_t cface = Mh.cell_face[e];
-  double L[N1][N1], dL[N1][N1];  // L[q][b] = L_b(x_q): uniform, scalar loads
+  TA L[N1][N1], dL[N1][N1];  // L[q][b] = L_b(x_q): uniform, scalar loads into scalar registers
+  const TA *tL, *tdL;
+  if constexpr (std::is_same<TA, float>::value) { tL = T.L1f; tdL = T.dL1f; }
+  else { tL = T.L1; tdL = T.dL1; }
 #pragma unroll
   for (int q = 0; q < N1; ++q)
 #pragma unroll
     for (int b = 0; b < N1; ++b) {
-      L[q][b] = T.L1[q * N1 + b];
-      dL[q][b] = T.dL1[q * N1 + b];
+      L[q][b] = tL[q * N1 + b];
+      dL[q][b] = tdL[q * N1 + b];
     }
   for (int i = tid; i < V; i += NTA) {
     const int node = cn[i / 18], c = i % 18;
-    sx[i] = ((Mh.node_mask[node] >> c) & 1u) ? 0.0 : (double)x[(size_t)node * 18 + c];
+    sx[i] = ((Mh.node_mask[node] >> c) & 1u) ? (TA)0 : (TA)x[(size_t)node * 18 + c];
   }
   __syncthreads();
   auto at = [](int i0, int i1, int i2, int c) { return ((i2 * N1 + i1) * N1 + i0) * 18 + c; };
-  double *A0 = R2, *A1 = R2 + V;
-  double *B0 = R1, *B1 = R1 + V, *B2 = R1 + 2 * V;
-  double *sX = R2, *sG = R2 + V;  // sG: [k][q][c]
-  double *sZ = R1, *sK = R1 + V;  // sK: [k][q][c]
-  double *C0 = R2, *C1 = R2 + V, *C2 = R2 + 2 * V;
-  double *D0 = R1, *D1 = R1 + V;
+  TA *A0 = R2, *A1 = R2 + V;
+  TA *B0 = R1, *B1 = R1 + V, *B2 = R1 + 2 * V;
+  TA *sX = R2, *sG = R2 + V;  // sG: [k][q][c]
+  TA *sZ = R1, *sK = R1 + V;  // sK: [k][q][c]
+  TA *C0 = R2, *C1 = R2 + V, *C2 = R2 + 2 * V;
+  TA *D0 = R1, *D1 = R1 + V;
   const int c = tid % 18, l = tid / 18, la = l % N1, lb = l / N1;  // pencil: line (la, lb), component c
   const bool pen = tid < NP;
   // x pass: line (by, bz) -> (qx, by, bz)
   if (pen) {
-    double v[N1];
+    TA v[N1];
 #pragma unroll
     for (int b = 0; b < N1; ++b) v[b] = sx[at(b, la, lb, c)];
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
-      double a0 = 0.0, a1 = 0.0;
+      TA a0 = 0, a1 = 0;
 #pragma unroll
       for (int b = 0; b < N1; ++b) {
         a0 += L[q][b] * v[b];
@@ -589,7 +613,7 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables 
   __syncthreads();
   // y pass: line (qx, bz) -> (qx, qy, bz)
   if (pen) {
-    double v0[N1], v1[N1];
+    TA v0[N1], v1[N1];
 #pragma unroll
     for (int b = 0; b < N1; ++b) {
       v0[b] = A0[at(la, b, lb, c)];
@@ -597,7 +621,7 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables 
     }
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
-      double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+      TA b0 = 0, b1 = 0, b2 = 0;
 #pragma unroll
       for (int b = 0; b < N1; ++b) {
         b0 += L[q][b] * v0[b];
@@ -612,7 +636,7 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables 
   __syncthreads();
   // z pass: line (qx, qy) -> (qx, qy, qz) = q
   if (pen) {
-    double v0[N1], v1[N1], v2[N1];
+    TA v0[N1], v1[N1], v2[N1];
 #pragma unroll
     for (int b = 0; b < N1; ++b) {
       v0[b] = B0[at(la, lb, b, c)];
@@ -621,7 +645,7 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables 
     }
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
-      double xv = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+      TA xv = 0, g0 = 0, g1 = 0, g2 = 0;
 #pragma unroll
       for (int b = 0; b < N1; ++b) {
         xv += L[q][b] * v0[b];
@@ -637,29 +661,31 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables 
     }
   }
   __syncthreads();
-  // w vol H(q) X(q) and w vol gamma_k(q) / h_k
+  // w vol H(q) X(q) and w vol gamma_k(q) / h_k for components hc, hc + 9 of point hq
+  if (hx) {
+    const TA *X = sX + hq * 18;
+    TA z[2] = {0, 0};
 #pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    const int i = tid + NTA * it;
-    if (i < V) {
-      const int q = i / 18, ci = i % 18;
-      const double *X = sX + q * 18;
-      double z = 0.0;
+    for (int d = 0; d < 18; ++d) {
+      const TA xd = X[d];  // one read of X(q) for both rows
+      z[0] += (TA)hv[0][d] * xd;
+      z[1] += (TA)hv[1][d] * xd;
+    }
+    const TA wv = (TA)(T.w[hq] * vol), K1 = (TA)M.K1, K23 = (TA)M.K23;
+    const int mu = hc / 3, kc = hc % 3;
 #pragma unroll
-      for (int d = 0; d < 18; ++d) z += (double)hv[it][d] * X[d];
-      const double wv = T.w[q] * vol;
-      sZ[i] = wv * z;
-      const int part = ci / 9, mu = (ci % 9) / 3, kc = ci % 3, base = q * 18 + 9 * part + 3 * mu;
-      const double dv = sG[base] + sG[V + base + 1] + sG[2 * V + base + 2];
+    for (int r = 0; r < 2; ++r) {
+      const int ci = hc + 9 * r, i = hq * 18 + ci, base = hq * 18 + 9 * r + 3 * mu;
+      sZ[i] = wv * z[r];
+      const TA dv = sG[base] + sG[V + base + 1] + sG[2 * V + base + 2];
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        sK[k * V + i] = wv * ih[k] * (M.K1 * sG[k * V + i] + (k == kc ? M.K23 * dv : 0.0));
+      for (int k = 0; k < 3; ++k) sK[k * V + i] = wv * ih[k] * (K1 * sG[k * V + i] + (k == kc ? K23 * dv : (TA)0));
     }
   }
   __syncthreads();
   // test z pass: line (qx, qy) -> (qx, qy, az)
   if (pen) {
-    double z[N1], k0[N1], k1[N1], k2[N1];
+    TA z[N1], k0[N1], k1[N1], k2[N1];
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
       const int o = at(la, lb, q, c);
@@ -670,7 +696,7 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables 
     }
 #pragma unroll
     for (int a = 0; a < N1; ++a) {
-      double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+      TA c0 = 0, c1 = 0, c2 = 0;
 #pragma unroll
       for (int q = 0; q < N1; ++q) {
         c0 += L[q][a] * z[q] + dL[q][a] * k2[q];
@@ -686,7 +712,7 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables 
   __syncthreads();
   // test y pass: line (qx, az) -> (qx, ay, az)
   if (pen) {
-    double v0[N1], v1[N1], v2[N1];
+    TA v0[N1], v1[N1], v2[N1];
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
       const int o = at(la, q, lb, c);
@@ -696,7 +722,7 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables 
     }
 #pragma unroll
     for (int a = 0; a < N1; ++a) {
-      double d0 = 0.0, d1 = 0.0;
+      TA d0 = 0, d1 = 0;
 #pragma unroll
       for (int q = 0; q < N1; ++q) {
         d0 += L[q][a] * v0[q] + dL[q][a] * v2[q];
@@ -709,7 +735,7 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables 
   __syncthreads();
   // test x pass: line (ay, az) -> (ax, ay, az) = a, then the Robin faces
   if (pen) {
-    double v0[N1], v1[N1];
+    TA v0[N1], v1[N1];
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
       v0[q] = D0[at(q, la, lb, c)];
@@ -717,7 +743,7 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables 
     }
 #pragma unroll
     for (int a = 0; a < N1; ++a) {
-      double r = 0.0;
+      TA r = 0;
 #pragma unroll
       for (int q = 0; q < N1; ++q) r += L[q][a] * v0[q] + dL[q][a] * v1[q];
       const int na = (lb * N1 + la) * N1 + a, o = at(a, la, lb, c);
@@ -726,9 +752,9 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables 
           const int bid = (int)((cface >> (3 * lf)) & 7u) - 1;
           if (bid < 2 || bid > 4 || face_zeroed(bid, c)) continue;
           const double *mf = T.Mf + (lf * NS + na) * NS;
-          double sf = 0.0;
-          for (int b = 0; b < NS; ++b) sf += mf[b] * sx[b * 18 + c];
-          r += (M.robin * face_area(lf, h)) * sf;
+          TA sf = 0;
+          for (int b = 0; b < NS; ++b) sf += (TA)mf[b] * sx[b * 18 + c];
+          r += (TA)(M.robin * face_area(lf, h)) * sf;
         }
       ycell[(size_t)e * V + o] = r;  // Dirichlet rows are replaced in the gather
     }
@@ -737,25 +763,33 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>())) void k_mf_apply(RefTables 
 
 // y = sum over the cells of each node of their contributions (fixed cell
 // order), and for the Dirichlet rows their diagonal entry (the sum over cells
-// of |local diagonal|) times x
-// With b != null the gather returns the residual y = b - J x instead.
-template <typename TX>
+// of |local diagonal|) times x; with b != null the residual y = b - J x.  One
+// thread per node and component pair (c, c + 1): two slot values per load.
+// TY, the slot and accumulation type, is that of the product (FP32 for the
+// FP32 inner solve).
+template <typename TX, typename TY>
 __global__ void k_mf_gather(int n_nodes, const int *__restrict__ ptr, const int *__restrict__ idx,
-                            const double *__restrict__ ycell, const uint32_t *__restrict__ mask,
+                            const TY *__restrict__ ycell, const uint32_t *__restrict__ mask,
                             const double *__restrict__ Jd, const TX *__restrict__ x, const TX *__restrict__ b,
                             const int *__restrict__ row_seg, const int *__restrict__ flag, TX *__restrict__ y)
 {
+  using Y2 = typename Vec2<TY>::type;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)n_nodes * 18) return;
-  const int node = (int)(i / 18), c = (int)(i % 18);
+  if (i >= (long long)n_nodes * 9) return;
+  const int node = (int)(i / 9), c = 2 * (int)(i % 9);
   if (flag && !flag[row_seg[node]]) return;  // instance not iterating: its rows are left as they are
-  double s = 0.0;
-  if ((mask[node] >> c) & 1u) {
-    s = Jd[(size_t)node * 324 + c * 19] * (double)x[i];
-  } else {
-    for (int k = ptr[node]; k < ptr[node + 1]; ++k) s += ycell[(size_t)idx[k] * 18 + c];
+  TY s0 = 0, s1 = 0;
+  for (int k = ptr[node]; k < ptr[node + 1]; ++k) {
+    const Y2 v = *reinterpret_cast<const Y2 *>(ycell + (size_t)idx[k] * 18 + c);
+    s0 += v.x;
+    s1 += v.y;
   }
-  y[i] = b ? (TX)((double)b[i] - s) : (TX)s;
+  const uint32_t mk = mask[node];
+  const size_t o = (size_t)node * 18 + c;
+  const double r0 = ((mk >> c) & 1u) ? Jd[(size_t)node * 324 + c * 19] * (double)x[o] : (double)s0;
+  const double r1 = ((mk >> (c + 1)) & 1u) ? Jd[(size_t)node * 324 + (c + 1) * 19] * (double)x[o + 1] : (double)s1;
+  y[o] = b ? (TX)((double)b[o] - r0) : (TX)r0;
+  y[o + 1] = b ? (TX)((double)b[o + 1] - r1) : (TX)r1;
 }
 
 template <int DEG>
@@ -901,14 +935,14 @@ hipError_t launch_mf_setup(int deg, bool h32, const RefTables &T, const MeshDev 
 }
 
 hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, const MeshDev &M, const Material &mat,
-                           const void *Hq, const void *x, const int *cells, int n, const int *flag, double *ycell,
+                           const void *Hq, const void *x, const int *cells, int n, const int *flag, void *ycell,
                            hipStream_t s)
 {
   if (n <= 0) return hipSuccess;
   if (x32 && !h32) return hipErrorInvalidValue;  // an FP64 Hessian is applied to FP64 vectors only
 #define GLH_MFA(D, TH, TX)                                                                                  \
   hipLaunchKernelGGL((k_mf_apply<D, TH, TX>), dim3(n), dim3((mf_threads<D, TH>())), 0, s, T, M, mat, (const TH *)Hq, \
-                     (const TX *)x, cells, flag, ycell)
+                     (const TX *)x, cells, flag, (mf_arith<TH, TX> *)ycell)
   if (deg == 1) {
     if (h32) { if (x32) GLH_MFA(1, float, float); else GLH_MFA(1, float, double); }
     else GLH_MFA(1, double, double);
@@ -921,20 +955,22 @@ hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, cons
 }
 
 template <typename TX>
-hipError_t launch_mf_gather(int n_nodes, const int *ptr, const int *idx, const double *ycell, const uint32_t *mask,
+hipError_t launch_mf_gather(int n_nodes, const int *ptr, const int *idx, const void *ycell, const uint32_t *mask,
                             const double *Jd, const TX *x, const TX *b, const int *row_seg, const int *flag, TX *y,
                             hipStream_t s)
 {
   if (n_nodes <= 0) return hipSuccess;
-  const long long n = (long long)n_nodes * 18;
-  hipLaunchKernelGGL((k_mf_gather<TX>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_nodes, ptr, idx, ycell,
-                     mask, Jd, x, b, row_seg, flag, y);
+  const long long n = (long long)n_nodes * 9;
+  // the slots are FP32 exactly when x is (the product's arithmetic type)
+  using TY = typename std::conditional<std::is_same<TX, float>::value, float, double>::type;
+  hipLaunchKernelGGL((k_mf_gather<TX, TY>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_nodes, ptr, idx,
+                     (const TY *)ycell, mask, Jd, x, b, row_seg, flag, y);
   return hipGetLastError();
 }
-template hipError_t launch_mf_gather<double>(int, const int *, const int *, const double *, const uint32_t *,
+template hipError_t launch_mf_gather<double>(int, const int *, const int *, const void *, const uint32_t *,
                                              const double *, const double *, const double *, const int *,
                                              const int *, double *, hipStream_t);
-template hipError_t launch_mf_gather<float>(int, const int *, const int *, const double *, const uint32_t *,
+template hipError_t launch_mf_gather<float>(int, const int *, const int *, const void *, const uint32_t *,
                                             const double *, const float *, const float *, const int *, const int *,
                                             float *, hipStream_t);
 

@@ -110,6 +110,7 @@ struct glh_ctx {
   DBuf<double> tN, tdN, tw, tWp, tSp, tMf;
   DBuf<int> tpa, tpb;
   DBuf<double> tL1, tdL1;
+  DBuf<float> tL1f, tdL1f;  // the same 1D bases rounded to FP32 (FP32 products)
   // mesh arrays
   DBuf<int> cell_nodes, row_ptr, col, diag, cell_blk, color_cells, cell_mat, seg_chunk;
   // hanging-node constraints: the cell loop assembles Jf (nnzb_a blocks over
@@ -143,7 +144,9 @@ struct glh_ctx {
   DBuf<float> Hq32;  // the stored Hessian of an FP32 matrix-free Jacobian
   DBuf<int> diag_id;  // node -> its block in Jd64 (the identity)
   DBuf<int> n2c_ptr, n2c_idx;  // node -> its (cell * ns + local node) entries
-  DBuf<double> mf_ycell;       // per-cell contributions of a product [n_cells][ns][18]
+  DBuf<double> mf_ycell;       // per-cell contributions of a product [n_cells][ns][18] (FP32 in an FP32 product)
+  DBuf<int> mf_order;          // launch order of the product's cells (XCD-aware, -1 padding)
+  int mf_order_n = 0;
   int diag_id_n = 0;
   DBuf<int> status;
   // GMRES workspace
@@ -192,7 +195,7 @@ struct glh_ctx {
 
   glh::RefTables tables() const
   {
-    return glh::RefTables{tN.p, tdN.p, tw.p, tWp.p, tSp.p, tMf.p, tpa.p, tpb.p, tL1.p, tdL1.p};
+    return glh::RefTables{tN.p, tdN.p, tw.p, tWp.p, tSp.p, tMf.p, tpa.p, tpb.p, tL1.p, tdL1.p, tL1f.p, tdL1f.p};
   }
   glh::MeshDev meshdev() const
   {
@@ -342,6 +345,9 @@ int build_tables(glh_ctx *c, int deg)
     for (int j = 0; j < n1; ++j) { L1[i * n1 + j] = L[i][j]; dL1[i * n1 + j] = dL[i][j]; }
   HIPCHK(c, upload(c->tL1, L1.data(), L1.size(), s));
   HIPCHK(c, upload(c->tdL1, dL1.data(), dL1.size(), s));
+  std::vector<float> L1f(L1.begin(), L1.end()), dL1f(dL1.begin(), dL1.end());
+  HIPCHK(c, upload(c->tL1f, L1f.data(), L1f.size(), s));
+  HIPCHK(c, upload(c->tdL1f, dL1f.data(), dL1f.size(), s));
   HIPCHK(c, hipStreamSynchronize(s));
   c->deg = deg; c->ns = ns; c->nq = nq; c->npair = npair;
   return GLH_OK;
@@ -554,7 +560,7 @@ int mf_product(glh_ctx *c, const TV *x, const TV *b, TV *y, const int *flag = nu
   const glh::MeshDev M = c->meshdev();
   const bool h32 = c->j_fp32, x32 = std::is_same<TV, float>::value;
   HIPCHK(c, glh::launch_mf_apply(c->deg, h32, x32, T, M, c->mat, h32 ? (const void *)c->Hq32.p : (const void *)c->Hq64.p,
-                                 x, c->color_cells.p, c->n_cells, flag, c->mf_ycell.p, c->stream));
+                                 x, c->mf_order.p, c->mf_order_n, flag, c->mf_ycell.p, c->stream));
   HIPCHK(c, glh::launch_mf_gather<TV>(c->n_nodes, c->n2c_ptr.p, c->n2c_idx.p, c->mf_ycell.p, c->node_mask.p,
                                       c->Jd64.p, x, b, c->row_seg.p, flag, y, c->stream));
   return GLH_OK;
@@ -1563,6 +1569,21 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
   {
     std::vector<int> fill(n2c_ptr.begin(), n2c_ptr.end() - 1);
     for (size_t k = 0; k < m.cell_nodes.size(); ++k) n2c_idx[fill[m.cell_nodes[k]]++] = (int)k;
+  }
+  // the product's launch order: workgroups go to the 8 XCDs in turn, so
+  // workgroup L runs cell (L % 8) * ceil(n / 8) + L / 8 and each XCD works
+  // through a contiguous range of cells, whose neighbours share nodes (x is
+  // then gathered from that XCD's L2)
+  {
+    const int n8 = (m.n_cells + 7) / 8;
+    std::vector<int> order((size_t)8 * n8);
+    for (int L = 0; L < 8 * n8; ++L) {
+      const int e = (L % 8) * n8 + L / 8;
+      order[L] = e < m.n_cells ? e : -1;
+    }
+    HIPCHK(c, upload(c->mf_order, order.data(), order.size(), s));
+    HIPCHK(c, hipStreamSynchronize(s));  // the host vector goes out of scope
+    c->mf_order_n = 8 * n8;
   }
   HIPCHK(c, upload(c->n2c_ptr, n2c_ptr.data(), n2c_ptr.size(), s));
   HIPCHK(c, upload(c->n2c_idx, n2c_idx.data(), n2c_idx.size(), s));

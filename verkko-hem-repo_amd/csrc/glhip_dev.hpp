@@ -21,6 +21,7 @@ struct RefTables {
   const int *pa, *pb; // [NPAIR]    pair list (a <= b)
   const double *L1;   // [N1*N1]   1D Lagrange basis: L1[qx*N1 + bx] = L_bx(x_qx)
   const double *dL1;  // [N1*N1]   its derivative
+  const float *L1f, *dL1f;  // the two rounded to FP32
 };
 
 struct Material {
@@ -90,10 +91,10 @@ hipError_t launch_mf_setup(int deg, bool h32, const RefTables &T, const MeshDev 
 // flag (per instance, or null): cells and rows of instances whose flag is 0
 // are skipped, as in the batched SpMV.
 hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, const MeshDev &M, const Material &mat,
-                           const void *Hq, const void *x, const int *cells, int n, const int *flag, double *ycell,
+                           const void *Hq, const void *x, const int *cells, int n, const int *flag, void *ycell,
                            hipStream_t s);
 template <typename TX>
-hipError_t launch_mf_gather(int n_nodes, const int *ptr, const int *idx, const double *ycell, const uint32_t *mask,
+hipError_t launch_mf_gather(int n_nodes, const int *ptr, const int *idx, const void *ycell, const uint32_t *mask,
                             const double *Jd, const TX *x, const TX *b, const int *row_seg, const int *flag, TX *y,
                             hipStream_t s);
 hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
