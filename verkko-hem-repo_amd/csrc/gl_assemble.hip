@@ -506,8 +506,7 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
 // while the cell interpolates x.
 // TH: stored Hessian (FP64, or FP32 for the FP32 inner solve); TX: x.  TA,
 // the arithmetic, LDS and cell-slot type, is FP32 when both are FP32 (the
-// FP32 inner solve, with FP32 accumulation; 17.5 KB of LDS per Q2 cell) and
-// FP64 otherwise (35 KB).  cells[] lists the cells in the launch order, -1
+// FP32 inner solve, with FP32 accumulation) and FP64 otherwise.  cells[] lists the cells in the launch order, -1
 // for padding (the XCD-aware order of mf_cell_order).
 template <int DEG, typename TH>
 constexpr int mf_threads() { return DEG == 1 ? 128 : 256; }
@@ -515,14 +514,13 @@ template <typename TH, typename TX>
 using mf_arith = typename std::conditional<std::is_same<TH, float>::value && std::is_same<TX, float>::value, float,
                                            double>::type;
 
-// waves per SIMD the register allocation aims at: the FP64 kernel is held to
-// 4 workgroups per CU by its 35 KB of LDS anyway; the FP32 one (17.5 KB) is
-// held only by its registers
-template <typename TH, typename TX>
-constexpr int mf_waves() { return std::is_same<mf_arith<TH, TX>, float>::value ? 5 : 4; }
+// waves per SIMD the register allocation aims at (the prefetched Hessian rows
+// alone hold 72 registers in FP64, 36 in FP32; the Q1 kernel spills at 6)
+template <int DEG, typename TH, typename TX>
+constexpr int mf_waves() { return std::is_same<mf_arith<TH, TX>, float>::value ? (DEG == 1 ? 5 : 6) : 4; }
 
 template <int DEG, typename TH, typename TX>
-__global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<TH, TX>())) void k_mf_apply(RefTables T, MeshDev Mh, Material M0,
+__global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>())) void k_mf_apply(RefTables T, MeshDev Mh, Material M0,
                                                                       const TH *__restrict__ Hq,
                                                                       const TX *__restrict__ x,
                                                                       const int *__restrict__ cells,
@@ -534,30 +532,35 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<TH, TX>())) void
   constexpr int NP = N1 * N1 * 18;  // pencils per pass
   constexpr int NH = NQ * 9;        // (point, component pair) items of H X
   static_assert(NH <= NTA, "one H X item per thread");
-  __shared__ TA sx[V];      // x at the cell's nodes, Dirichlet components 0
-  __shared__ TA R1[4 * V];  // B0..B2, then w vol H X and w vol gamma / h, then D0, D1
-  __shared__ TA R2[4 * V];  // A0, A1, then X and its gradients, then C0..C2
+  // Regions of one value per (point or node, component), in the order
+  //   x -> A0 A1 -> B0 B1 B2 -> X G0 G1 G2 -> Z K0 K1 K2 -> C0 C1 C2 -> D0 D1.
+  // A pencil reads exactly the points it writes, so the FP32 kernel lets each
+  // pass overwrite its inputs in place in one set of four regions (7.8 KB for
+  // Q2: register-limited at 6 waves per SIMD).  The FP64 kernel, limited to 4
+  // waves by its registers anyway, alternates between two sets and keeps x
+  // (35 KB), which saves a barrier and the Robin faces' second gather of x.
+  constexpr bool IP = std::is_same<TA, float>::value;
+  __shared__ TA S[(IP ? 4 : 9) * V];
+  TA *const P0 = S, *const P1 = IP ? S : S + 4 * V, *const SX = IP ? S : S + 8 * V;
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
   if (e < 0) return;                           // padding of the XCD-aware order
   if (flag && !flag[Mh.cell_inst[e]]) return;  // instance not iterating: its rows are not read
   // the two Hessian rows of this thread's H X item (point hq, components hc
-  // and hc + 9), requested first
+  // and hc + 9), requested first: they are in flight while the cell gathers x
+  // and interpolates it
   const int hq = tid / 9, hc = tid % 9;
-  const bool hx = tid < NH;
   TH hv[2][18];
-  if (hx) {
+  if (tid < NH) {
     const TH *Hrow = Hq + ((size_t)e * NQ + hq) * NUPAIR;
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-      const int c = hc + 9 * r;
-      // upair(c, d) for d = 0, 1, ...: column c of rows d < c (steps 17 - d),
-      // then row c itself (steps of 1)
-      int off = c;
+      const int cr = hc + 9 * r;
+      int off = cr;  // upair(cr, d): steps 17 - d while d < cr, then 1
 #pragma unroll
       for (int d = 0; d < 18; ++d) {
         hv[r][d] = Hrow[off];
-        off += d < c ? 17 - d : 1;
+        off += d < cr ? 17 - d : 1;
       }
     }
   }
@@ -579,25 +582,21 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<TH, TX>())) void
       L[q][b] = tL[q * N1 + b];
       dL[q][b] = tdL[q * N1 + b];
     }
-  for (int i = tid; i < V; i += NTA) {
+  // x at the cell's nodes, Dirichlet components 0 (the masked columns)
+  auto xval = [&](int i) -> TA {
     const int node = cn[i / 18], c = i % 18;
-    sx[i] = ((Mh.node_mask[node] >> c) & 1u) ? (TA)0 : (TA)x[(size_t)node * 18 + c];
-  }
+    return ((Mh.node_mask[node] >> c) & 1u) ? (TA)0 : (TA)x[(size_t)node * 18 + c];
+  };
+  for (int i = tid; i < V; i += NTA) SX[i] = xval(i);
   __syncthreads();
   auto at = [](int i0, int i1, int i2, int c) { return ((i2 * N1 + i1) * N1 + i0) * 18 + c; };
-  TA *A0 = R2, *A1 = R2 + V;
-  TA *B0 = R1, *B1 = R1 + V, *B2 = R1 + 2 * V;
-  TA *sX = R2, *sG = R2 + V;  // sG: [k][q][c]
-  TA *sZ = R1, *sK = R1 + V;  // sK: [k][q][c]
-  TA *C0 = R2, *C1 = R2 + V, *C2 = R2 + 2 * V;
-  TA *D0 = R1, *D1 = R1 + V;
   const int c = tid % 18, l = tid / 18, la = l % N1, lb = l / N1;  // pencil: line (la, lb), component c
   const bool pen = tid < NP;
-  // x pass: line (by, bz) -> (qx, by, bz)
+  // x pass: line (by, bz) -> (qx, by, bz): x -> A0 A1
   if (pen) {
     TA v[N1];
 #pragma unroll
-    for (int b = 0; b < N1; ++b) v[b] = sx[at(b, la, lb, c)];
+    for (int b = 0; b < N1; ++b) v[b] = SX[at(b, la, lb, c)];
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
       TA a0 = 0, a1 = 0;
@@ -606,18 +605,18 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<TH, TX>())) void
         a0 += L[q][b] * v[b];
         a1 += dL[q][b] * v[b];
       }
-      A0[at(q, la, lb, c)] = a0;
-      A1[at(q, la, lb, c)] = a1;
+      P1[at(q, la, lb, c)] = a0;
+      P1[V + at(q, la, lb, c)] = a1;
     }
   }
   __syncthreads();
-  // y pass: line (qx, bz) -> (qx, qy, bz)
+  // y pass: line (qx, bz) -> (qx, qy, bz): A0 A1 -> B0 B1 B2
   if (pen) {
     TA v0[N1], v1[N1];
 #pragma unroll
     for (int b = 0; b < N1; ++b) {
-      v0[b] = A0[at(la, b, lb, c)];
-      v1[b] = A1[at(la, b, lb, c)];
+      v0[b] = P1[at(la, b, lb, c)];
+      v1[b] = P1[V + at(la, b, lb, c)];
     }
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
@@ -628,20 +627,20 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<TH, TX>())) void
         b1 += L[q][b] * v1[b];
         b2 += dL[q][b] * v0[b];
       }
-      B0[at(la, q, lb, c)] = b0;
-      B1[at(la, q, lb, c)] = b1;
-      B2[at(la, q, lb, c)] = b2;
+      P0[at(la, q, lb, c)] = b0;
+      P0[V + at(la, q, lb, c)] = b1;
+      P0[2 * V + at(la, q, lb, c)] = b2;
     }
   }
   __syncthreads();
-  // z pass: line (qx, qy) -> (qx, qy, qz) = q
+  // z pass: line (qx, qy) -> (qx, qy, qz) = q: B0 B1 B2 -> X G0 G1 G2
   if (pen) {
     TA v0[N1], v1[N1], v2[N1];
 #pragma unroll
     for (int b = 0; b < N1; ++b) {
-      v0[b] = B0[at(la, lb, b, c)];
-      v1[b] = B1[at(la, lb, b, c)];
-      v2[b] = B2[at(la, lb, b, c)];
+      v0[b] = P0[at(la, lb, b, c)];
+      v1[b] = P0[V + at(la, lb, b, c)];
+      v2[b] = P0[2 * V + at(la, lb, b, c)];
     }
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
@@ -654,16 +653,18 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<TH, TX>())) void
         g2 += dL[q][b] * v0[b];
       }
       const int o = at(la, lb, q, c);
-      sX[o] = xv;
-      sG[o] = g0 * ih[0];
-      sG[V + o] = g1 * ih[1];
-      sG[2 * V + o] = g2 * ih[2];
+      P1[o] = xv;
+      P1[V + o] = g0 * ih[0];
+      P1[2 * V + o] = g1 * ih[1];
+      P1[3 * V + o] = g2 * ih[2];
     }
   }
   __syncthreads();
-  // w vol H(q) X(q) and w vol gamma_k(q) / h_k for components hc, hc + 9 of point hq
-  if (hx) {
-    const TA *X = sX + hq * 18;
+  // w vol H(q) X(q) and w vol gamma_k(q) / h_k for the components hc and hc + 9
+  // of point hq: read everything, then write Z and K (over X and G in place)
+  TA zo[2], ko[2][3];
+  if (tid < NH) {
+    const TA *X = &P1[hq * 18];
     TA z[2] = {0, 0};
 #pragma unroll
     for (int d = 0; d < 18; ++d) {
@@ -675,24 +676,34 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<TH, TX>())) void
     const int mu = hc / 3, kc = hc % 3;
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-      const int ci = hc + 9 * r, i = hq * 18 + ci, base = hq * 18 + 9 * r + 3 * mu;
-      sZ[i] = wv * z[r];
-      const TA dv = sG[base] + sG[V + base + 1] + sG[2 * V + base + 2];
+      const int i = hq * 18 + hc + 9 * r, base = hq * 18 + 9 * r + 3 * mu;
+      zo[r] = wv * z[r];
+      const TA dv = P1[V + base] + P1[2 * V + base + 1] + P1[3 * V + base + 2];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) sK[k * V + i] = wv * ih[k] * (K1 * sG[k * V + i] + (k == kc ? K23 * dv : (TA)0));
+      for (int k = 0; k < 3; ++k) ko[r][k] = wv * ih[k] * (K1 * P1[(1 + k) * V + i] + (k == kc ? K23 * dv : (TA)0));
+    }
+  }
+  if constexpr (IP) __syncthreads();
+  if (tid < NH) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int i = hq * 18 + hc + 9 * r;
+      P0[i] = zo[r];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) P0[(1 + k) * V + i] = ko[r][k];
     }
   }
   __syncthreads();
-  // test z pass: line (qx, qy) -> (qx, qy, az)
+  // test z pass: line (qx, qy) -> (qx, qy, az): Z K0 K1 K2 -> C0 C1 C2
   if (pen) {
     TA z[N1], k0[N1], k1[N1], k2[N1];
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
       const int o = at(la, lb, q, c);
-      z[q] = sZ[o];
-      k0[q] = sK[o];
-      k1[q] = sK[V + o];
-      k2[q] = sK[2 * V + o];
+      z[q] = P0[o];
+      k0[q] = P0[V + o];
+      k1[q] = P0[2 * V + o];
+      k2[q] = P0[3 * V + o];
     }
 #pragma unroll
     for (int a = 0; a < N1; ++a) {
@@ -704,21 +715,21 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<TH, TX>())) void
         c2 += L[q][a] * k1[q];
       }
       const int o = at(la, lb, a, c);
-      C0[o] = c0;
-      C1[o] = c1;
-      C2[o] = c2;
+      P1[o] = c0;
+      P1[V + o] = c1;
+      P1[2 * V + o] = c2;
     }
   }
   __syncthreads();
-  // test y pass: line (qx, az) -> (qx, ay, az)
+  // test y pass: line (qx, az) -> (qx, ay, az): C0 C1 C2 -> D0 D1
   if (pen) {
     TA v0[N1], v1[N1], v2[N1];
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
       const int o = at(la, q, lb, c);
-      v0[q] = C0[o];
-      v1[q] = C1[o];
-      v2[q] = C2[o];
+      v0[q] = P1[o];
+      v1[q] = P1[V + o];
+      v2[q] = P1[2 * V + o];
     }
 #pragma unroll
     for (int a = 0; a < N1; ++a) {
@@ -728,8 +739,8 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<TH, TX>())) void
         d0 += L[q][a] * v0[q] + dL[q][a] * v2[q];
         d1 += L[q][a] * v1[q];
       }
-      D0[at(la, a, lb, c)] = d0;
-      D1[at(la, a, lb, c)] = d1;
+      P0[at(la, a, lb, c)] = d0;
+      P0[V + at(la, a, lb, c)] = d1;
     }
   }
   __syncthreads();
@@ -738,8 +749,8 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<TH, TX>())) void
     TA v0[N1], v1[N1];
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
-      v0[q] = D0[at(q, la, lb, c)];
-      v1[q] = D1[at(q, la, lb, c)];
+      v0[q] = P0[at(q, la, lb, c)];
+      v1[q] = P0[V + at(q, la, lb, c)];
     }
 #pragma unroll
     for (int a = 0; a < N1; ++a) {
@@ -753,7 +764,7 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<TH, TX>())) void
           if (bid < 2 || bid > 4 || face_zeroed(bid, c)) continue;
           const double *mf = T.Mf + (lf * NS + na) * NS;
           TA sf = 0;
-          for (int b = 0; b < NS; ++b) sf += (TA)mf[b] * sx[b * 18 + c];
+          for (int b = 0; b < NS; ++b) sf += (TA)mf[b] * (IP ? xval(b * 18 + c) : SX[b * 18 + c]);
           r += (TA)(M.robin * face_area(lf, h)) * sf;
         }
       ycell[(size_t)e * V + o] = r;  // Dirichlet rows are replaced in the gather

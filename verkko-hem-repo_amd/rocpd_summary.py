@@ -7,6 +7,8 @@ kernel.  FETCH_DB / WRITE_DB: --pmc FETCH_SIZE and --pmc WRITE_SIZE runs;
 prints the average KiB per dispatch of each counter and, when the run holds
 the calibration copy kernel (k_copy16 of --calib-bytes bytes), the factor
 that maps each counter to true bytes (known traffic / counter reading).
+Further --pmc databases (SQ_ event counters and the like) print the average
+per dispatch of each counter as read.
 """
 import argparse
 import collections
@@ -27,9 +29,15 @@ def kernel_times(db):
 
 
 def counters(db):
+    """(kernel, counter) -> value per dispatch.  An SQ counter comes as one row
+    per shader engine; the rows of a dispatch are summed."""
+    per = collections.defaultdict(float)
+    for n, cn, di, v in sqlite3.connect(db).execute(
+            "select name, counter_name, dispatch_id, counter_value from pmc_events"):
+        per[(short(n), cn, di)] += v
     d = collections.defaultdict(list)
-    for n, cn, v in sqlite3.connect(db).execute("select name, counter_name, counter_value from pmc_events"):
-        d[(short(n), cn)].append(v)
+    for (n, cn, _), v in per.items():
+        d[(n, cn)].append(v)
     return d
 
 
@@ -49,14 +57,17 @@ def main():
         c = counters(db)
         factor = {}
         for (n, cn), v in c.items():
-            if n.endswith("k_copy16"):
+            if n.endswith("k_copy16") and cn in ("FETCH_SIZE", "WRITE_SIZE"):
                 kib = sum(v) / len(v)
                 factor[cn] = a.calib_bytes / (kib * 1024.0)
         for (n, cn), v in sorted(c.items()):
-            kib = sum(v) / len(v)
+            avg = sum(v) / len(v)
+            if cn not in ("FETCH_SIZE", "WRITE_SIZE"):  # event counts: the average per dispatch as read
+                print("  %-24s %14.4g per dispatch  n=%3d  %s" % (cn, avg, len(v), n))
+                continue
             f = factor.get(cn)
             print("  %-10s %7.1f MiB/dispatch  x%.3f -> %8.3f GB  n=%3d  %s" % (
-                cn, kib / 1024, f or 1.0, kib * 1024 * (f or 1.0) / 1e9, len(v), n))
+                cn, avg / 1024, f or 1.0, avg * 1024 * (f or 1.0) / 1e9, len(v), n))
 
 
 if __name__ == "__main__":
