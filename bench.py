@@ -312,6 +312,9 @@ def run_adaptive(G, args):
     # the A/B wall is not stationary at 0.8 Tc, 22 bar: pseudo-transient
     # continuation (sigma0 = 1) carries Newton to the minimum
     ctx.set_continuation(1.0)
+    # matrix-free J under the hanging-node constraints (condensed product and
+    # diagonal blocks), or the assembled condensed matrix
+    ctx.set_matrix_free(args.operator == "matrix-free")
     U0 = G.initial_guess_bna(m.arrays()["node_xyz"], 0.0, mp[8], mp[9], amp=0.05, seed=1)
     print("bench: adaptive slab, %d cells at cycle 0" % m.n_cells, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -331,7 +334,8 @@ def run_adaptive(G, args):
         "config": {"workload": "adaptive slab: 40 x 40 x 12 xi0, 16x16x4 Q1 root cells, %d Kelly cycles of 30%%, "
                                "T/Tc 0.8, p 22 bar, BnA guess seed 1; one solve = one converged cycle "
                                "(FP32 mixed precision), FP64 check per cycle" % args.cycles,
-                   "name": "adaptive_slab", "final_dofs": int(rows[-1]["dofs"]), "final_cells": int(rows[-1]["cells"])},
+                   "name": "adaptive_slab", "final_dofs": int(rows[-1]["dofs"]), "final_cells": int(rows[-1]["cells"]),
+                   "operator": args.operator},
         "cycles": [{k: r[k] for k in ("cycle", "cells", "dofs", "hanging", "newton_steps", "gmres_its", "F",
                                       "converged", "t_solve", "F64", "F_rel_err_vs_fp64", "U_rel_err_vs_fp64")}
                    for r in rows],

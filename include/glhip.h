@@ -345,13 +345,21 @@ int glh_get_jacobian(glh_ctx *ctx, double *val_host);
  * or matrix-free (FP64 vectors of n_dofs on the host).  A Jacobian-vector
  * product for parity checks of either form. */
 int glh_apply_jacobian(glh_ctx *ctx, const double *x_host, double *y_host);
+/* The 18x18 diagonal block of every node (n_nodes * 324 FP64 values on the
+ * host, row-major) of the current Jacobian, assembled or matrix-free: the
+ * blocks the point-block Jacobi preconditioner inverts (with constraints, the
+ * condensed ones; a hanging node's is its diagonal row).  The assembled form's
+ * are the diagonal blocks of the BSR values.  No counterpart in the
+ * reference, which preconditions with Trilinos AMG (solve.cc:130-152); for
+ * parity checks of the preconditioner of either form. */
+int glh_diagonal_blocks(glh_ctx *ctx, double *out);
 
 /* ---------------------------------------------------------- Newton step */
 /* FemGL::assemble_system (assemble.cc:108-372): J and/or system_rhs = -R at
  * the current solution, with constrained rows condensed as
  * AffineConstraints::distribute_local_to_global does. flags: GLH_ASM_*;
  * with GLH_ASM_MATRIX_FREE the Jacobian is kept in the matrix-free form of
- * glh_set_matrix_free (no constraints) instead of BSR values. */
+ * glh_set_matrix_free instead of BSR values. */
 int glh_assemble(glh_ctx *ctx, int flags);
 /* ||system_rhs|| after glh_assemble with GLH_ASM_R. */
 int glh_rhs_norm(glh_ctx *ctx, double *norm);
@@ -458,11 +466,15 @@ int glh_set_continuation(glh_ctx *ctx, double sigma0);
  * J x cell by cell from them (the same operator as the assembled BSR matrix,
  * to rounding).  With fp32 the Hessian is stored in FP32 and the inner GMRES,
  * its Krylov basis and the block Jacobi run in FP32, inside the FP64
- * iterative refinement.  It applies to solves (one instance or a batch) on
- * meshes without hanging-node or periodic constraints and without
- * continuation; other solves assemble J as before.  glh_assemble builds the
- * BSR values unless it is given GLH_ASM_MATRIX_FREE; glh_get_jacobian fails
- * after a matrix-free step.  0 (default) = off. */
+ * iterative refinement.  On a mesh with constraint lines (hanging nodes,
+ * periodic images) the product is C^T (J_cells (C x)) with the hanging rows
+ * diagonal, the condensed matrix of the assembled path, and the block Jacobi
+ * uses the condensed diagonal blocks; the continuation shift is added to the
+ * diagonal blocks and to the product.  It applies to solves of one instance
+ * or, on meshes without constraint lines, of a batch; other solves assemble
+ * J as before.  glh_assemble builds the BSR values unless it is given
+ * GLH_ASM_MATRIX_FREE; glh_get_jacobian fails after a matrix-free step.
+ * 0 (default) = off. */
 int glh_set_matrix_free(glh_ctx *ctx, int on);
 int glh_set_verbose(glh_ctx *ctx, int level);
 /* Live kernel timing with HIP events on the context stream.  on != 0 resets

@@ -5,7 +5,8 @@ max|diff| / max|ref|) on Q1 and Q2 box meshes with Dirichlet (id 0), AdGR
 columns and rows are exercised, and the FP32 forms to 2e-6; a Newton solve,
 FP64 or with the FP32 matrix-free inner solve, reaches the assembled FP64
 fixed point (free energy 1e-10, order parameter 1e-8); meshes with
-constraints keep the assembled Jacobian."""
+constraint lines solve matrix-free too (their parity:
+test_gpu_matfree_constrained.py)."""
 import sys
 
 import numpy as np
@@ -143,7 +144,7 @@ def test_mixed_precision_matrix_free_newton_reaches_the_fp64_fixed_point(ctx):
     assert np.abs(U32 - U64).max() <= 1e-8 * np.abs(U64).max()
 
 
-def test_meshes_with_constraints_keep_the_assembled_jacobian(ctx):
+def test_meshes_with_constraints_solve_matrix_free(ctx):
     lo, hi, sides = (-5.0, -5.0, -6.0), (5.0, 5.0, 6.0), [1, 1, 1, 1, 0, 0]
     f = adaptive_forest(G, (2, 2, 2), lo, hi, sides, 1, 0, frac=0.3)
     m = f.mesh()
@@ -158,7 +159,11 @@ def test_meshes_with_constraints_keep_the_assembled_jacobian(ctx):
     finally:
         ctx.set_matrix_free(False)
     assert st["converged"] == 1.0
-    assert ctx.jacobian().shape[1:] == (18, 18)  # assembled values are there
+    # the Newton steps kept the matrix-free form (condensed under the hanging
+    # lines; its parity is in test_gpu_matfree_constrained): no BSR values
+    with pytest.raises(G.GLHError):
+        ctx.jacobian()
+    assert ctx.diagonal_blocks().shape == (m.n_nodes, 18, 18)
 
 
 def test_material_change_on_one_mesh_matches_a_fresh_context(ctx):

@@ -408,7 +408,7 @@ __device__ inline int upair(int c, int d)
 template <int DEG, typename TH>
 __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ U,
                                                  const int *__restrict__ cells, TH *__restrict__ Hq,
-                                                 double *__restrict__ Jd)
+                                                 double *__restrict__ Jd, MfCondDev X)
 {
   constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS;
   __shared__ double sU[NS * 18];
@@ -460,8 +460,9 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
   const double k23_cd = M.K23 * vol * ih[kc] * ih[kd];
   const int o_cd = c * 18 + d, o_dc = d * 18 + c;
   const int *cblk = Mh.cell_blk + (size_t)e * NS * NS;
-  for (int a = 0; a < NS; ++a) {
-    const int p = a * NS - a * (a - 1) / 2;  // pair (a, a)
+  // entries [c][d] (v1) and [d][c] (v2) of the local block of node pair p =
+  // (a <= b), before the Dirichlet rows and columns are applied
+  auto pair_block = [&](int p, int a, int b, double &v1, double &v2) {
     const double *wp = T.Wp + (size_t)p * NQ, *sp = T.Sp + (size_t)p * 9;
     double sym = 0.0;
 #pragma unroll
@@ -473,15 +474,22 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
         for (int lf = 0; lf < 6; ++lf) {
           const int bid = (int)((cface >> (3 * lf)) & 7u) - 1;
           if (bid >= 2 && bid <= 4 && !face_zeroed(bid, c))
-            sym += (M.robin * face_area(lf, h)) * T.Mf[(lf * NS + a) * NS + a];
+            sym += (M.robin * face_area(lf, h)) * T.Mf[(lf * NS + a) * NS + b];
         }
     }
-    double v1 = sym, v2 = sym;
+    v1 = sym;
+    v2 = sym;
     if (grad_cd) {
       v1 += k23_cd * sp[kc * 3 + kd];
       v2 += k23_cd * sp[kd * 3 + kc];
     }
+  };
+  for (int a = 0; a < NS; ++a) {
+    double v1, v2;
+    pair_block(a * NS - a * (a - 1) / 2, a, a, v1, v2);  // pair (a, a)
     const uint32_t ma = Mh.node_mask[cn[a]];
+    // a hanging node's condensed row keeps the sum over cells of |local diagonal|
+    if (Mh.hdiag && c == d && (ma & HANG_BIT)) Mh.hdiag[(size_t)cn[a] * 18 + c] += fabs(v1);
     const bool r1 = (ma >> c) & 1u, c1 = (ma >> d) & 1u, r2 = (ma >> d) & 1u, c2 = (ma >> c) & 1u;
     if (r1 || c1) v1 = (r1 && c == d) ? fabs(v1) : 0.0;
     if (r2 || c2) v2 = (r2 && c == d) ? fabs(v2) : 0.0;
@@ -490,6 +498,17 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
     B[o_cd] = (first ? 0.0 : B[o_cd]) + v1;
     if (c != d) B[o_dc] = (first ? 0.0 : B[o_dc]) + v2;
   }
+  // constrained meshes: the raw local blocks that constraint lines fold into
+  // the condensed diagonal blocks of masters (few cells have any)
+  if (X.xptr)
+    for (int k = X.xptr[e]; k < X.xptr[e + 1]; ++k) {
+      const int p = X.xpair[k];
+      double v1, v2;
+      pair_block(p, T.pa[p], T.pb[p], v1, v2);
+      double *B = X.LB + (size_t)k * 324;
+      B[o_cd] = v1;
+      if (c != d) B[o_dc] = v2;
+    }
 }
 
 // The product in sum-factorised form: N_b(q) = L_bx(x_qx) L_by(x_qy) L_bz(x_qz),
@@ -781,8 +800,9 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
 template <typename TX, typename TY>
 __global__ void k_mf_gather(int n_nodes, const int *__restrict__ ptr, const int *__restrict__ idx,
                             const TY *__restrict__ ycell, const uint32_t *__restrict__ mask,
-                            const double *__restrict__ Jd, const TX *__restrict__ x, const TX *__restrict__ b,
-                            const int *__restrict__ row_seg, const int *__restrict__ flag, TX *__restrict__ y)
+                            const double *__restrict__ Jd, const double *__restrict__ sh, const TX *__restrict__ x,
+                            const TX *__restrict__ b, const int *__restrict__ row_seg, const int *__restrict__ flag,
+                            TX *__restrict__ y)
 {
   using Y2 = typename Vec2<TY>::type;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -797,8 +817,12 @@ __global__ void k_mf_gather(int n_nodes, const int *__restrict__ ptr, const int 
   }
   const uint32_t mk = mask[node];
   const size_t o = (size_t)node * 18 + c;
-  const double r0 = ((mk >> c) & 1u) ? Jd[(size_t)node * 324 + c * 19] * (double)x[o] : (double)s0;
-  const double r1 = ((mk >> (c + 1)) & 1u) ? Jd[(size_t)node * 324 + (c + 1) * 19] * (double)x[o + 1] : (double)s1;
+  double r0 = ((mk >> c) & 1u) ? Jd[(size_t)node * 324 + c * 19] * (double)x[o] : (double)s0;
+  double r1 = ((mk >> (c + 1)) & 1u) ? Jd[(size_t)node * 324 + (c + 1) * 19] * (double)x[o + 1] : (double)s1;
+  if (sh) {  // zero on the Dirichlet and hanging rows
+    r0 += sh[o] * (double)x[o];
+    r1 += sh[o + 1] * (double)x[o + 1];
+  }
   y[o] = b ? (TX)((double)b[o] - r0) : (TX)r0;
   y[o + 1] = b ? (TX)((double)b[o + 1] - r1) : (TX)r1;
 }
@@ -935,10 +959,12 @@ hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefT
 }
 
 hipError_t launch_mf_setup(int deg, bool h32, const RefTables &T, const MeshDev &M, const Material &mat,
-                           const double *U, const int *cells, int n, void *Hq, double *Jd, hipStream_t s)
+                           const double *U, const int *cells, int n, void *Hq, double *Jd, const MfCondDev &X,
+                           hipStream_t s)
 {
   if (n <= 0) return hipSuccess;
-#define GLH_MFS(D, TH) hipLaunchKernelGGL((k_mf_setup<D, TH>), dim3(n), dim3(NT), 0, s, T, M, mat, U, cells, (TH *)Hq, Jd)
+#define GLH_MFS(D, TH) \
+  hipLaunchKernelGGL((k_mf_setup<D, TH>), dim3(n), dim3(NT), 0, s, T, M, mat, U, cells, (TH *)Hq, Jd, X)
   if (deg == 1) { if (h32) GLH_MFS(1, float); else GLH_MFS(1, double); }
   else { if (h32) GLH_MFS(2, float); else GLH_MFS(2, double); }
 #undef GLH_MFS
@@ -967,23 +993,23 @@ hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, cons
 
 template <typename TX>
 hipError_t launch_mf_gather(int n_nodes, const int *ptr, const int *idx, const void *ycell, const uint32_t *mask,
-                            const double *Jd, const TX *x, const TX *b, const int *row_seg, const int *flag, TX *y,
-                            hipStream_t s)
+                            const double *Jd, const double *sh, const TX *x, const TX *b, const int *row_seg,
+                            const int *flag, TX *y, hipStream_t s)
 {
   if (n_nodes <= 0) return hipSuccess;
   const long long n = (long long)n_nodes * 9;
   // the slots are FP32 exactly when x is (the product's arithmetic type)
   using TY = typename std::conditional<std::is_same<TX, float>::value, float, double>::type;
   hipLaunchKernelGGL((k_mf_gather<TX, TY>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_nodes, ptr, idx,
-                     (const TY *)ycell, mask, Jd, x, b, row_seg, flag, y);
+                     (const TY *)ycell, mask, Jd, sh, x, b, row_seg, flag, y);
   return hipGetLastError();
 }
 template hipError_t launch_mf_gather<double>(int, const int *, const int *, const void *, const uint32_t *,
-                                             const double *, const double *, const double *, const int *,
-                                             const int *, double *, hipStream_t);
+                                             const double *, const double *, const double *, const double *,
+                                             const int *, const int *, double *, hipStream_t);
 template hipError_t launch_mf_gather<float>(int, const int *, const int *, const void *, const uint32_t *,
-                                            const double *, const float *, const float *, const int *, const int *,
-                                            float *, hipStream_t);
+                                            const double *, const double *, const float *, const float *,
+                                            const int *, const int *, float *, hipStream_t);
 
 hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
                          int n_cells, double *Fcell, hipStream_t s)

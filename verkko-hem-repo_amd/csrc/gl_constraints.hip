@@ -92,6 +92,128 @@ __global__ __launch_bounds__(CT) void k_distribute(int n_hang, const int *__rest
   v[(size_t)hnode[j] * 18 + c] = acc;
 }
 
+// ------------------------------------------------ matrix-free Jacobian
+// Condensed diagonal blocks, in place: thread (t, c, d) adds to block
+// tnode[t] the weighted terms of its list (in list order), each the [c][d]
+// entry of a raw local block with the cell loop's Dirichlet rule applied
+// (row of A or column of B constrained: 0, except |v| on the diagonal of a
+// node's own (a, a) block) and the contributions through a line dropped on
+// the Dirichlet components of the master (as k_condense_j drops them).
+__global__ __launch_bounds__(CT) void k_mf_cond_diag(int n_targets, const int *__restrict__ tnode,
+                                                     const int *__restrict__ tptr, const int *__restrict__ tslot,
+                                                     const int *__restrict__ tA, const int *__restrict__ tB,
+                                                     const double *__restrict__ tw,
+                                                     const unsigned char *__restrict__ tfl,
+                                                     const uint32_t *__restrict__ mask, const double *__restrict__ LB,
+                                                     double *__restrict__ Jd)
+{
+  const long long i = (long long)blockIdx.x * CT + threadIdx.x;
+  if (i >= (long long)n_targets * 324) return;
+  const int t = (int)(i / 324), e = (int)(i % 324), c = e / 18, d = e % 18;
+  const int n = tnode[t];
+  const uint32_t mn = mask[n];
+  double acc = 0.0;
+  for (int k = tptr[t]; k < tptr[t + 1]; ++k) {
+    const unsigned f = tfl[k];
+    if (((f & 1u) && ((mn >> c) & 1u)) || ((f & 2u) && ((mn >> d) & 1u))) continue;
+    double v = LB[(size_t)tslot[k] * 324 + ((f & 4u) ? d * 18 + c : e)];
+    const bool rA = (mask[tA[k]] >> c) & 1u, cB = (mask[tB[k]] >> d) & 1u;
+    if (rA || cB) v = ((f & 8u) && rA && c == d) ? fabs(v) : 0.0;
+    acc += tw[k] * v;
+  }
+  Jd[(size_t)n * 324 + e] += acc;
+}
+
+// Block n_nodes + j of the hanging node hnode[j]: its condensed row, the
+// diagonal hdiag and nothing else.
+__global__ __launch_bounds__(CT) void k_mf_hang_blocks(int n_nodes, int n_hang, const int *__restrict__ hnode,
+                                                       const double *__restrict__ hdiag, double *__restrict__ Jd)
+{
+  const long long i = (long long)blockIdx.x * CT + threadIdx.x;
+  if (i >= (long long)n_hang * 324) return;
+  const int j = (int)(i / 324), e = (int)(i % 324), c = e / 18, d = e % 18;
+  Jd[((size_t)n_nodes + j) * 324 + e] = c == d ? hdiag[(size_t)hnode[j] * 18 + c] : 0.0;
+}
+
+// xf = C x, one thread per value.  A hanging node's value is the weighted sum
+// of its masters' (the closed line), leaving out the masters' Dirichlet
+// components; every other value is copied.
+template <typename TV>
+__global__ __launch_bounds__(CT) void k_mf_distribute(int n_nodes, const int *__restrict__ hidx,
+                                                      const int *__restrict__ hptr, const int *__restrict__ hm,
+                                                      const double *__restrict__ hw, const uint32_t *__restrict__ mask,
+                                                      const TV *__restrict__ x, const int *__restrict__ row_seg,
+                                                      const int *__restrict__ flag, TV *__restrict__ xf)
+{
+  const long long i = (long long)blockIdx.x * CT + threadIdx.x;
+  if (i >= (long long)n_nodes * 18) return;
+  const int r = (int)(i / 18), c = (int)(i % 18);
+  if (flag && !flag[row_seg[r]]) return;
+  const int j = hidx[r];
+  if (j < 0) {
+    xf[i] = x[i];
+    return;
+  }
+  double acc = 0.0;
+  for (int k = hptr[j]; k < hptr[j + 1]; ++k) {
+    const int mnode = hm[k];
+    if (!((mask[mnode] >> c) & 1u)) acc += hw[k] * (double)x[(size_t)mnode * 18 + c];
+  }
+  xf[i] = (TV)acc;
+}
+
+// y = C^T yf: an unconstrained row gathers itself and the rows of the hanging
+// nodes whose lines hold it (lists of k_condense_r, the latter dropped on the
+// row's Dirichlet components); a hanging row is hdiag x.  With b: y = b - that.
+template <typename TV>
+__global__ __launch_bounds__(CT) void k_mf_condense_y(int n_nodes, const int *__restrict__ hidx,
+                                                      const int *__restrict__ rptr, const int *__restrict__ src,
+                                                      const double *__restrict__ w,
+                                                      const unsigned char *__restrict__ fl,
+                                                      const uint32_t *__restrict__ mask,
+                                                      const double *__restrict__ hdiag, const TV *__restrict__ yf,
+                                                      const TV *__restrict__ x, const TV *__restrict__ b,
+                                                      const int *__restrict__ row_seg, const int *__restrict__ flag,
+                                                      TV *__restrict__ y)
+{
+  const long long i = (long long)blockIdx.x * CT + threadIdx.x;
+  if (i >= (long long)n_nodes * 18) return;
+  const int r = (int)(i / 18), c = (int)(i % 18);
+  if (flag && !flag[row_seg[r]]) return;
+  double acc;
+  if (hidx[r] >= 0) {
+    acc = hdiag[i] * (double)x[i];
+  } else {
+    const bool mr = (mask[r] >> c) & 1u;
+    acc = 0.0;
+    for (int k = rptr[r]; k < rptr[r + 1]; ++k) {
+      if ((fl[k] & 1u) && mr) continue;
+      acc += w[k] * (double)yf[(size_t)src[k] * 18 + c];
+    }
+  }
+  y[i] = b ? (TV)((double)b[i] - acc) : (TV)acc;
+}
+
+// Continuation shift of the matrix-free Jacobian (k_diag_shift's rule on the
+// diagonal blocks): sh = sigma |v| on the free components, v + sh in place.
+__global__ __launch_bounds__(CT) void k_mf_shift(int n_nodes, const uint32_t *__restrict__ mask,
+                                                 const int *__restrict__ row_seg, const double *__restrict__ sigma,
+                                                 double *__restrict__ Jd, double *__restrict__ sh)
+{
+  const long long i = (long long)blockIdx.x * CT + threadIdx.x;
+  if (i >= (long long)n_nodes * 18) return;
+  const int r = (int)(i / 18), c = (int)(i % 18);
+  const uint32_t mk = mask[r];
+  if ((mk & HANG_BIT) || ((mk >> c) & 1u)) {
+    sh[i] = 0.0;
+    return;
+  }
+  double *p = Jd + (size_t)r * 324 + c * 19;
+  const double v = *p, t = sigma[row_seg[r]] * fabs(v);
+  sh[i] = t;
+  *p = v + t;
+}
+
 // Solution transfer to a refined mesh (SolutionTransfer of FE_Q(deg),
 // femgl/src/refine.cc:163-175): value (v, c) is the Q_deg interpolant of the
 // old cell cell[v] at t[v], with every product and sum rounded on its own in
@@ -171,6 +293,64 @@ hipError_t launch_distribute(const ConstraintDev &C, double *v, hipStream_t s)
   return hipGetLastError();
 }
 
+hipError_t launch_mf_cond_diag(const ConstraintDev &C, const MfCondDev &X, int n_nodes, const double *hdiag,
+                               double *Jd, hipStream_t s)
+{
+  if (X.n_targets > 0) {
+    const long long nv = (long long)X.n_targets * 324;
+    hipLaunchKernelGGL(k_mf_cond_diag, dim3((unsigned)((nv + CT - 1) / CT)), dim3(CT), 0, s, X.n_targets, X.tnode,
+                       X.tptr, X.tslot, X.tA, X.tB, X.tw, X.tfl, C.mask, X.LB, Jd);
+  }
+  if (C.n_hang > 0) {
+    const long long nv = (long long)C.n_hang * 324;
+    hipLaunchKernelGGL(k_mf_hang_blocks, dim3((unsigned)((nv + CT - 1) / CT)), dim3(CT), 0, s, n_nodes, C.n_hang,
+                       C.hnode, hdiag, Jd);
+  }
+  return hipGetLastError();
+}
+
+template <typename TV>
+hipError_t launch_mf_distribute(const ConstraintDev &C, int n_nodes, const TV *x, TV *xf, const int *row_seg,
+                                const int *flag, hipStream_t s)
+{
+  const long long n = (long long)n_nodes * 18;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_mf_distribute<TV>), dim3((unsigned)((n + CT - 1) / CT)), dim3(CT), 0, s, n_nodes, C.hidx,
+                     C.hptr, C.hm, C.hw, C.mask, x, row_seg, flag, xf);
+  return hipGetLastError();
+}
+
+template <typename TV>
+hipError_t launch_mf_condense(const ConstraintDev &C, int n_nodes, const double *hdiag, const TV *yf, const TV *x,
+                              const TV *b, TV *y, const int *row_seg, const int *flag, hipStream_t s)
+{
+  const long long n = (long long)n_nodes * 18;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_mf_condense_y<TV>), dim3((unsigned)((n + CT - 1) / CT)), dim3(CT), 0, s, n_nodes, C.hidx,
+                     C.rptr, C.rsrc, C.rw, C.rfl, C.mask, hdiag, yf, x, b, row_seg, flag, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_mf_shift(int n_nodes, const uint32_t *mask, const int *row_seg, const double *sigma, double *Jd,
+                           double *sh, hipStream_t s)
+{
+  const long long n = (long long)n_nodes * 18;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mf_shift, dim3((unsigned)((n + CT - 1) / CT)), dim3(CT), 0, s, n_nodes, mask, row_seg, sigma,
+                     Jd, sh);
+  return hipGetLastError();
+}
+
+template hipError_t launch_mf_distribute<double>(const ConstraintDev &, int, const double *, double *, const int *,
+                                                 const int *, hipStream_t);
+template hipError_t launch_mf_distribute<float>(const ConstraintDev &, int, const float *, float *, const int *,
+                                                const int *, hipStream_t);
+template hipError_t launch_mf_condense<double>(const ConstraintDev &, int, const double *, const double *,
+                                               const double *, const double *, double *, const int *, const int *,
+                                               hipStream_t);
+template hipError_t launch_mf_condense<float>(const ConstraintDev &, int, const double *, const float *,
+                                              const float *, const float *, float *, const int *, const int *,
+                                              hipStream_t);
 template hipError_t launch_condense<double>(const ConstraintDev &, const double *, double *, const double *,
                                             hipStream_t);
 template hipError_t launch_condense<float>(const ConstraintDev &, const float *, float *, const double *,

@@ -436,6 +436,16 @@ __global__ __launch_bounds__(RT) void k_convert(size_t n, const TS *__restrict__
   for (size_t idx = (size_t)blockIdx.x * RT + threadIdx.x; idx < n; idx += stride) d[idx] = (TD)s[idx];
 }
 
+template <typename TJ>
+__global__ __launch_bounds__(RT) void k_gather_blocks(int n_rows, const int *__restrict__ blk,
+                                                      const TJ *__restrict__ val, double *__restrict__ dst)
+{
+  const long long i = (long long)blockIdx.x * RT + threadIdx.x;
+  if (i >= (long long)n_rows * 324) return;
+  const long long r = i / 324;
+  dst[i] = (double)val[(size_t)blk[r] * 324 + (i - r * 324)];
+}
+
 __global__ __launch_bounds__(RT) void k_sub(size_t n, const double *__restrict__ b, const double *__restrict__ w,
                                            double *__restrict__ d)
 {
@@ -1006,6 +1016,17 @@ hipError_t launch_convert(size_t n, const TS *src, TD *dst, hipStream_t s)
   hipLaunchKernelGGL((k_convert<TS, TD>), dim3(grid_for(n)), dim3(RT), 0, s, n, src, dst);
   return hipGetLastError();
 }
+
+template <typename TJ>
+hipError_t launch_gather_blocks(int n_rows, const int *blk, const TJ *val, double *dst, hipStream_t s)
+{
+  const long long n = (long long)n_rows * 324;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_gather_blocks<TJ>), dim3((unsigned)((n + RT - 1) / RT)), dim3(RT), 0, s, n_rows, blk, val, dst);
+  return hipGetLastError();
+}
+template hipError_t launch_gather_blocks<double>(int, const int *, const double *, double *, hipStream_t);
+template hipError_t launch_gather_blocks<float>(int, const int *, const float *, double *, hipStream_t);
 
 hipError_t launch_sub(size_t n, const double *b, const double *w, double *dst, hipStream_t s)
 {

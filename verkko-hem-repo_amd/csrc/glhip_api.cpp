@@ -142,12 +142,23 @@ struct glh_ctx {
   bool matfree = false, j_mf = false;
   DBuf<double> Hq64, Jd64;
   DBuf<float> Hq32;  // the stored Hessian of an FP32 matrix-free Jacobian
-  DBuf<int> diag_id;  // node -> its block in Jd64 (the identity)
+  DBuf<int> diag_id;  // node -> its block in Jd64 (the identity, hanging nodes aside)
   DBuf<int> n2c_ptr, n2c_idx;  // node -> its (cell * ns + local node) entries
   DBuf<double> mf_ycell;       // per-cell contributions of a product [n_cells][ns][18] (FP32 in an FP32 product)
   DBuf<int> mf_order;          // launch order of the product's cells (XCD-aware, -1 padding)
   int mf_order_n = 0;
-  int diag_id_n = 0;
+  // constrained meshes: the lists of glh::MfCondDev (raw local blocks folded
+  // into the masters' diagonal blocks), node -> hanging index, and the C x and
+  // J C x of a product
+  DBuf<int> mfx_ptr, mfx_pair, mft_node, mft_ptr, mft_slot, mft_A, mft_B, c_hidx;
+  DBuf<double> mfx_LB, mft_w, mf_xf, mf_yf;
+  DBuf<unsigned char> mft_fl;
+  glh::MfCondDev mfc;
+  size_t mfx_slots = 0;
+  // continuation shift of a matrix-free Jacobian (sigma |diag| per value),
+  // used by the product while mf_sh_on
+  DBuf<double> mf_sh;
+  bool mf_sh_on = false;
   DBuf<int> status;
   // GMRES workspace
   DBuf<double> gV64, gZ64, gW64, gh64, gp64, gy64, gs64, gq64;
@@ -396,18 +407,19 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
 {
   void *J = nullptr;
   if (do_j && mf) {  // matrix-free: H at the quadrature points and the diagonal blocks
-    if (c->constrained) return fail(c, GLH_ERR_STATE, "matrix-free Jacobian: mesh without constraints");
     if (fp32) HIPCHK(c, c->Hq32.alloc((size_t)c->n_cells * c->nq * 171));
     else HIPCHK(c, c->Hq64.alloc((size_t)c->n_cells * c->nq * 171));
     HIPCHK(c, c->mf_ycell.alloc((size_t)c->n_cells * c->ns * NC));
-    HIPCHK(c, c->Jd64.alloc((size_t)c->n_nodes * 324));
-    if (c->diag_id_n != c->n_nodes) {
-      std::vector<int> id(c->n_nodes);
-      for (int i = 0; i < c->n_nodes; ++i) id[i] = i;
-      HIPCHK(c, upload(c->diag_id, id.data(), id.size(), c->stream));
-      HIPCHK(c, hipStreamSynchronize(c->stream));  // the host vector goes out of scope
-      c->diag_id_n = c->n_nodes;
+    // one block per node, and on a constrained mesh one more per hanging node
+    // (its condensed row for the point-block Jacobi, diag_id points there)
+    HIPCHK(c, c->Jd64.alloc(((size_t)c->n_nodes + (c->constrained ? c->n_hang : 0)) * 324));
+    if (c->constrained) {
+      HIPCHK(c, c->mfx_LB.alloc(c->mfx_slots * 324));
+      c->mfc.LB = c->mfx_LB.p;
+      HIPCHK(c, c->mf_xf.alloc(ndofs(c)));
+      HIPCHK(c, c->mf_yf.alloc(ndofs(c)));
     }
+    c->mf_sh_on = false;
   } else if (do_j) {
     const size_t nv = (size_t)c->nnzb * 324;
     if (fp32) { HIPCHK(c, c->J32.alloc(nv)); J = c->J32.p; }
@@ -417,12 +429,12 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
   void *Jcell = J;
   double *rcell = rhs_out;
   if (c->constrained) {
-    if (do_j) {
+    if (do_j && !mf) {
       const size_t nv = (size_t)c->nnzb_a * 324;
       if (fp32) { HIPCHK(c, c->Jf32.alloc(nv)); Jcell = c->Jf32.p; }
       else { HIPCHK(c, c->Jf64.alloc(nv)); Jcell = c->Jf64.p; }
-      HIPCHK(c, glh::launch_fill<double>(ndofs(c), 0.0, c->hdiag.p, c->stream));
     }
+    if (do_j) HIPCHK(c, glh::launch_fill<double>(ndofs(c), 0.0, c->hdiag.p, c->stream));
     if (do_r) rcell = c->rfull.p;
   }
   if (do_r) HIPCHK(c, hipMemsetAsync(rcell, 0, ndofs(c) * sizeof(double), c->stream));
@@ -435,13 +447,15 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
     const int off = c->color_ptr[k], n = c->color_ptr[k + 1] - off;
     if (do_j && mf)
       HIPCHK(c, glh::launch_mf_setup(c->deg, fp32, T, M, c->mat, Ustate, c->color_cells.p + off, n,
-                                     fp32 ? (void *)c->Hq32.p : (void *)c->Hq64.p, c->Jd64.p, c->stream));
+                                     fp32 ? (void *)c->Hq32.p : (void *)c->Hq64.p, c->Jd64.p,
+                                     c->constrained ? c->mfc : glh::MfCondDev{}, c->stream));
     if (do_r || (do_j && !mf))
       HIPCHK(c, glh::launch_assemble(c->deg, do_j && !mf, do_r, fp32, T, M, c->mat, Ustate, c->color_cells.p + off,
                                      n, Jcell, rcell, c->stream));
   }
   if (c->constrained) {
-    if (do_j) {
+    if (do_j && mf) HIPCHK(c, glh::launch_mf_cond_diag(c->cons, c->mfc, c->n_nodes, c->hdiag.p, c->Jd64.p, c->stream));
+    else if (do_j) {
       if (fp32) HIPCHK(c, glh::launch_condense<float>(c->cons, c->Jf32.p, c->J32.p, c->hdiag.p, c->stream));
       else HIPCHK(c, glh::launch_condense<double>(c->cons, c->Jf64.p, c->J64.p, c->hdiag.p, c->stream));
     }
@@ -559,10 +573,19 @@ int mf_product(glh_ctx *c, const TV *x, const TV *b, TV *y, const int *flag = nu
   const glh::RefTables T = c->tables();
   const glh::MeshDev M = c->meshdev();
   const bool h32 = c->j_fp32, x32 = std::is_same<TV, float>::value;
+  const double *sh = c->mf_sh_on ? c->mf_sh.p : nullptr;
+  // constrained meshes: y = C^T Jf (C x), hanging rows hdiag x
+  const bool con = c->constrained;
+  TV *xf = con ? reinterpret_cast<TV *>(c->mf_xf.p) : nullptr, *yf = con ? reinterpret_cast<TV *>(c->mf_yf.p) : y;
+  const TV *xin = con ? xf : x;
+  if (con) HIPCHK(c, glh::launch_mf_distribute<TV>(c->cons, c->n_nodes, x, xf, c->row_seg.p, flag, c->stream));
   HIPCHK(c, glh::launch_mf_apply(c->deg, h32, x32, T, M, c->mat, h32 ? (const void *)c->Hq32.p : (const void *)c->Hq64.p,
-                                 x, c->mf_order.p, c->mf_order_n, flag, c->mf_ycell.p, c->stream));
+                                 xin, c->mf_order.p, c->mf_order_n, flag, c->mf_ycell.p, c->stream));
   HIPCHK(c, glh::launch_mf_gather<TV>(c->n_nodes, c->n2c_ptr.p, c->n2c_idx.p, c->mf_ycell.p, c->node_mask.p,
-                                      c->Jd64.p, x, b, c->row_seg.p, flag, y, c->stream));
+                                      c->Jd64.p, sh, xin, con ? nullptr : b, c->row_seg.p, flag, yf, c->stream));
+  if (con)
+    HIPCHK(c, glh::launch_mf_condense<TV>(c->cons, c->n_nodes, c->hdiag.p, yf, x, b, y, c->row_seg.p, flag,
+                                          c->stream));
   return GLH_OK;
 }
 
@@ -758,6 +781,7 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
                                          c->row_seg.p, c->d_node_off.p, c->act_counts.p, c->act_seg.p,
                                          c->act_rpref.p, c->act_chunk.p, c->chunks.p, c->seg_chunk.p,
                                          c->row_ptr.p, c->col.p, c->Hq64.p, c->Hq32.p, c->Jd64.p,
+                                         c->mf_xf.p, c->mf_yf.p, c->mf_sh_on ? c->mf_sh.p : nullptr,
                                          (const void *)(size_t)c->j_mf,
                                          (const void *)(size_t)m, (const void *)(size_t)S,
                                          (const void *)(size_t)c->n_chunks, (const void *)(size_t)c->n_nodes,
@@ -1056,10 +1080,14 @@ int glh_destroy(glh_ctx *c)
                      &c->gmjlast, &c->row_seg,    &c->d_node_off, &c->act_counts, &c->act_seg, &c->act_rpref,
                      &c->act_chunk};
   for (auto *b : gi) b->release();
-  DBuf<int> *ci[] = {&c->c_gptr, &c->c_gsrc, &c->c_brow, &c->c_rptr, &c->c_rsrc, &c->c_hnode, &c->c_hptr, &c->c_hm};
+  DBuf<int> *ci[] = {&c->c_gptr,  &c->c_gsrc,   &c->c_brow,   &c->c_rptr,  &c->c_rsrc,  &c->c_hnode, &c->c_hptr,
+                     &c->c_hm,    &c->c_hidx,   &c->mfx_ptr,  &c->mfx_pair, &c->mft_node, &c->mft_ptr, &c->mft_slot,
+                     &c->mft_A,   &c->mft_B};
   for (auto *b : ci) b->release();
-  DBuf<double> *cd[] = {&c->c_gw, &c->c_rw, &c->c_hw, &c->hdiag, &c->rfull, &c->Jf64};
+  DBuf<double> *cd[] = {&c->c_gw, &c->c_rw, &c->c_hw, &c->hdiag, &c->rfull, &c->Jf64, &c->mfx_LB, &c->mft_w,
+                        &c->mf_xf, &c->mf_yf, &c->mf_sh};
   for (auto *b : cd) b->release();
+  c->mft_fl.release();
   c->c_gfl.release();
   c->c_rfl.release();
   c->asm_flag.release();
@@ -1487,8 +1515,84 @@ int upload_constraints(glh_ctx *c, const glh::Mesh &m, const std::vector<int> &d
   HIPCHK(c, upload(c->c_hptr, hptr.data(), hptr.size(), s));
   HIPCHK(c, upload(c->c_hm, hm.data(), hm.size(), s));
   HIPCHK(c, upload(c->c_hw, hw.data(), hw.size(), s));
+  std::vector<int> hidx(nn, -1);
+  for (size_t j = 0; j < hnode.size(); ++j) hidx[hnode[j]] = (int)j;
+  HIPCHK(c, upload(c->c_hidx, hidx.data(), hidx.size(), s));
   HIPCHK(c, c->hdiag.alloc((size_t)nn * NC));
   HIPCHK(c, c->rfull.alloc((size_t)nn * NC));
+  // Matrix-free Jacobian: the condensed diagonal block of a master n holds,
+  // besides the (n, n) blocks of its cells, w_A(n) w_B(n) J_K(a, b) for every
+  // cell K and local pair (a, b) with A = cn[a], B = cn[b] both reaching n
+  // through T(A) = {A} or the line of a hanging A (the gather of Jc's
+  // diagonal blocks above, cell by cell).  Slots: local pairs (a <= b) whose
+  // raw block some term uses, per cell in pair order.
+  {
+    const int ns = m.ns;
+    struct Term {
+      int n, slot, A, B;
+      double w;
+      unsigned char fl;
+    };
+    std::vector<Term> terms;
+    std::vector<int> xptr(m.n_cells + 1, 0), xpair;
+    for (int e = 0; e < m.n_cells; ++e) {
+      const int *cn = &m.cell_nodes[(size_t)e * ns];
+      int p = 0;
+      for (int a = 0; a < ns; ++a)
+        for (int b = a; b < ns; ++b, ++p) {
+          const int A = cn[a], B = cn[b];
+          if (!hanging(A) && !hanging(B)) continue;
+          const int slot = (int)xpair.size();
+          bool used = false;
+          for (int o = 0; o < (a == b ? 1 : 2); ++o) {  // (a, b), then (b, a): the transposed block
+            const int P = o ? B : A, Q = o ? A : B;
+            const bool hp = hanging(P), hq = hanging(Q);
+            const int kp0 = hp ? m.hang_ptr[P] : 0, kp1 = hp ? m.hang_ptr[P + 1] : 1;
+            const int kq0 = hq ? m.hang_ptr[Q] : 0, kq1 = hq ? m.hang_ptr[Q + 1] : 1;
+            for (int kp = kp0; kp < kp1; ++kp)
+              for (int kq = kq0; kq < kq1; ++kq) {
+                const int np = hp ? m.hang_m[kp] : P, nq = hq ? m.hang_m[kq] : Q;
+                if (np != nq) continue;
+                const double w = (hp ? m.hang_w[kp] : 1.0) * (hq ? m.hang_w[kq] : 1.0);
+                terms.push_back(Term{np, slot, P, Q, w,
+                                     (unsigned char)((hp ? 1 : 0) | (hq ? 2 : 0) | (o ? 4 : 0) | (a == b ? 8 : 0))});
+                used = true;
+              }
+          }
+          if (used) xpair.push_back(p);
+        }
+      xptr[e + 1] = (int)xpair.size();
+    }
+    std::stable_sort(terms.begin(), terms.end(), [](const Term &x, const Term &y) { return x.n < y.n; });
+    std::vector<int> tnode, tptr(1, 0), tslot, tA, tB;
+    std::vector<double> tw;
+    std::vector<unsigned char> tfl;
+    for (size_t k = 0; k < terms.size(); ++k) {
+      if (k == 0 || terms[k].n != terms[k - 1].n) {
+        if (k) tptr.push_back((int)k);
+        tnode.push_back(terms[k].n);
+      }
+      tslot.push_back(terms[k].slot);
+      tA.push_back(terms[k].A);
+      tB.push_back(terms[k].B);
+      tw.push_back(terms[k].w);
+      tfl.push_back(terms[k].fl);
+    }
+    if (!terms.empty()) tptr.push_back((int)terms.size());
+    HIPCHK(c, upload(c->mfx_ptr, xptr.data(), xptr.size(), s));
+    HIPCHK(c, upload(c->mfx_pair, xpair.data(), xpair.size(), s));
+    HIPCHK(c, upload(c->mft_node, tnode.data(), tnode.size(), s));
+    HIPCHK(c, upload(c->mft_ptr, tptr.data(), tptr.size(), s));
+    HIPCHK(c, upload(c->mft_slot, tslot.data(), tslot.size(), s));
+    HIPCHK(c, upload(c->mft_A, tA.data(), tA.size(), s));
+    HIPCHK(c, upload(c->mft_B, tB.data(), tB.size(), s));
+    HIPCHK(c, upload(c->mft_w, tw.data(), tw.size(), s));
+    HIPCHK(c, upload(c->mft_fl, tfl.data(), tfl.size(), s));
+    c->mfx_slots = xpair.size();
+    c->mfc = glh::MfCondDev{c->mfx_ptr.p, c->mfx_pair.p, nullptr, (int)tnode.size(), c->mft_node.p, c->mft_ptr.p,
+                            c->mft_slot.p, c->mft_A.p, c->mft_B.p, c->mft_w.p, c->mft_fl.p};
+    HIPCHK(c, hipStreamSynchronize(s));  // the host vectors go out of scope
+  }
   HIPCHK(c, hipStreamSynchronize(s));
   return GLH_OK;
 }
@@ -1525,7 +1629,14 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
   }
   c->constrained = m.has_hanging();
   c->nnzb_a = c->constrained ? m.arow_ptr.back() : c->nnzb;
+  c->n_hang = 0;
   if (c->constrained) RCHK(upload_constraints(c, m, diag));
+  // the matrix-free Jacobian's diagonal block of each node: its own, or for a
+  // hanging node block n_nodes + j (hanging nodes in node order)
+  std::vector<int> did(m.n_nodes);
+  for (int i = 0, j = 0; i < m.n_nodes; ++i)
+    did[i] = (c->constrained && m.hang_ptr[i + 1] > m.hang_ptr[i]) ? m.n_nodes + j++ : i;
+  HIPCHK(c, upload(c->diag_id, did.data(), did.size(), s));
   // cell -> block map (cell-loop pattern) with first-writer marks in colour order
   const int ns = m.ns;
   std::vector<int> cblk((size_t)m.n_cells * ns * ns);
@@ -1603,6 +1714,7 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
     C.brow = c->c_brow.p; C.col = c->col.p; C.mask = c->node_mask.p; C.diag = c->diag.p;
     C.rptr = c->c_rptr.p; C.rsrc = c->c_rsrc.p; C.rw = c->c_rw.p; C.rfl = c->c_rfl.p;
     C.hnode = c->c_hnode.p; C.hptr = c->c_hptr.p; C.hm = c->c_hm.p; C.hw = c->c_hw.p;
+    C.hidx = c->c_hidx.p;
   }
   HIPCHK(c, upload(c->cell_blk, cblk.data(), cblk.size(), s));
   HIPCHK(c, upload(c->color_cells, m.color_cells.data(), m.color_cells.size(), s));
@@ -1683,6 +1795,22 @@ int glh_apply_jacobian(glh_ctx *c, const double *x, double *y)
     RCHK((apply_j<double, double>(c, c->J64.p, c->gZ64.p, c->gW64.p, nullptr, glh::ActiveSet{})));
   }
   HIPCHK(c, hipMemcpyAsync(y, c->gW64.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GLH_OK;
+}
+
+int glh_diagonal_blocks(glh_ctx *c, double *out)
+{
+  RCHK(check_ready(c, false));
+  if (!out) return fail(c, GLH_ERR_ARG, "null output");
+  if (!c->j_valid) return fail(c, GLH_ERR_STATE, "no Jacobian assembled");
+  const size_t nv = (size_t)c->n_nodes * 324;
+  DBuf<double> tmp;
+  HIPCHK(c, tmp.alloc(nv));
+  if (c->j_mf) HIPCHK(c, glh::launch_gather_blocks<double>(c->n_nodes, c->diag_id.p, c->Jd64.p, tmp.p, c->stream));
+  else if (c->j_fp32) HIPCHK(c, glh::launch_gather_blocks<float>(c->n_nodes, c->diag.p, c->J32.p, tmp.p, c->stream));
+  else HIPCHK(c, glh::launch_gather_blocks<double>(c->n_nodes, c->diag.p, c->J64.p, tmp.p, c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, tmp.p, nv * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return GLH_OK;
 }
@@ -1843,10 +1971,10 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
   std::vector<char> active(S), stalled(S, 0);
   HIPCHK(c, c->s_flag.alloc(S));
   const bool f32 = fp32 != 0;
-  // the matrix-free operator where it applies (FP64 or FP32, one instance or
-  // a batch, no constraints, no continuation shift); the assembled BSR
-  // Jacobian otherwise
-  const bool mf = c->matfree && !c->constrained && !(c->ptc_sigma0 > 0);
+  // the matrix-free operator when switched on (FP64 or FP32, with or without
+  // the continuation shift; a batch, or one instance on a constrained mesh);
+  // the assembled BSR Jacobian otherwise
+  const bool mf = c->matfree && !(c->constrained && S > 1);
   std::vector<int> flag(S);
   // batches only: an instance whose ||R|| has fallen by less than 10% over the
   // last STALL_WINDOW steps is stopped and reported as not converged, so that
@@ -1892,7 +2020,12 @@ int glh_newton_solve(glh_ctx *c, double rtol, int max_newton, double lin_tol, in
       for (int i = 0; i < S; ++i) sg[i] = active[i] ? c->ptc_sigma0 * rcur[i] / std::max(r0[i], 1e-300) : 0.0;
       HIPCHK(c, c->s_norm.alloc(S));
       HIPCHK(c, hipMemcpyAsync(c->s_norm.p, sg, S * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      if (f32)
+      if (mf) {  // on the diagonal blocks, and as a diagonal term of the product
+        HIPCHK(c, c->mf_sh.alloc(ndofs(c)));
+        HIPCHK(c, glh::launch_mf_shift(c->n_nodes, c->node_mask.p, c->row_seg.p, c->s_norm.p, c->Jd64.p, c->mf_sh.p,
+                                       c->stream));
+        c->mf_sh_on = true;
+      } else if (f32)
         HIPCHK(c, glh::launch_diag_shift<float>(c->n_nodes, c->diag.p, c->node_mask.p, c->row_seg.p, c->s_norm.p,
                                                 c->J32.p, c->stream));
       else

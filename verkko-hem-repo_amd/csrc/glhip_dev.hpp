@@ -57,6 +57,25 @@ struct ConstraintDev {
   const unsigned char *rfl = nullptr;
   const int *hnode = nullptr, *hptr = nullptr, *hm = nullptr;  // hanging nodes and their closed lines
   const double *hw = nullptr;
+  const int *hidx = nullptr;                  // node -> its index in hnode, -1 if not hanging
+};
+// The matrix-free Jacobian under constraints.  The condensed diagonal block of
+// a master node n is the sum over cells of its own local block (n, n), which
+// the setup forms as on an unconstrained mesh, plus the local blocks (a, b) of
+// every cell whose nodes A, B both reach n through constraint lines (A or B
+// hanging), weighted wA wB.  Per cell e, the setup stores the raw local blocks
+// of the pairs xpair[xptr[e] .. xptr[e+1]) (pair index of (a <= b)) into LB
+// slots numbered like the list; per target node tnode[t], terms
+// tptr[t] .. tptr[t+1] name a slot, the nodes A (row) and B (column) and the
+// weight, and tfl: bit 0 A hanging, bit 1 B hanging, bit 2 the slot's block
+// transposed (orientation (b, a)), bit 3 a == b.
+struct MfCondDev {
+  const int *xptr = nullptr, *xpair = nullptr;
+  double *LB = nullptr;
+  int n_targets = 0;
+  const int *tnode = nullptr, *tptr = nullptr, *tslot = nullptr, *tA = nullptr, *tB = nullptr;
+  const double *tw = nullptr;
+  const unsigned char *tfl = nullptr;
 };
 // Jc = C^T Jf C over the condensed pattern, hanging rows diagonal (hdiag)
 template <typename TJ>
@@ -65,6 +84,27 @@ hipError_t launch_condense(const ConstraintDev &C, const TJ *Jf, TJ *Jc, const d
 hipError_t launch_condense_rhs(const ConstraintDev &C, const double *rf, double *rc, hipStream_t s);
 // hanging values of v from their masters
 hipError_t launch_distribute(const ConstraintDev &C, double *v, hipStream_t s);
+// Matrix-free Jacobian under constraints (with MfCondDev):
+// the condensed diagonal blocks in place in Jd ([n_nodes + n_hang][324]):
+// the listed terms added to the masters' blocks, and block n_nodes + j of
+// hanging node hnode[j] set to diag(hdiag) (its condensed row)
+hipError_t launch_mf_cond_diag(const ConstraintDev &C, const MfCondDev &X, int n_nodes, const double *hdiag,
+                               double *Jd, hipStream_t s);
+// xf = C x: x, with every hanging node's values taken from its masters
+// (Dirichlet components of a master contribute 0)
+template <typename TV>
+hipError_t launch_mf_distribute(const ConstraintDev &C, int n_nodes, const TV *x, TV *xf, const int *row_seg,
+                                const int *flag, hipStream_t s);
+// y = C^T yf on the unconstrained rows and hdiag x on the hanging ones, or
+// b minus that with b != null
+template <typename TV>
+hipError_t launch_mf_condense(const ConstraintDev &C, int n_nodes, const double *hdiag, const TV *yf, const TV *x,
+                              const TV *b, TV *y, const int *row_seg, const int *flag, hipStream_t s);
+// continuation shift of the matrix-free Jacobian: sh = sigma |diag| on every
+// free component (0 on Dirichlet and hanging rows), added to the diagonal of
+// the blocks in Jd
+hipError_t launch_mf_shift(int n_nodes, const uint32_t *mask, const int *row_seg, const double *sigma, double *Jd,
+                           double *sh, hipStream_t s);
 // Un[v*18+c] = Q_deg interpolant of the old cell cell[v] (nodes
 // old_cn[cell*(deg+1)^3..], lexicographic) at t[3v..3v+2], from Uo
 hipError_t launch_transfer(int n_nodes, int deg, const int *cell, const float *t, const int *old_cn,
@@ -77,26 +117,31 @@ hipError_t launch_transfer(int n_nodes, int deg, const int *cell, const float *t
 hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefTables &T, const MeshDev &M,
                            const Material &mat, const double *U, const int *cells, int n, void *J, double *rhs,
                            hipStream_t s);
-// Per-cell free energy into Fcell[cell] for cells [0, n_cells).
-// Matrix-free Jacobian (meshes without constraints): per-cell bulk Hessian
-// at the quadrature points (Hq: [n_cells][NQ][171]) and the diagonal blocks
-// (Jd: [n_nodes][324], colour-ordered like the assembly); the product writes
+// Matrix-free Jacobian: per-cell bulk Hessian at the quadrature points (Hq:
+// [n_cells][NQ][171]) and the diagonal blocks (Jd: [n_nodes][324], one more
+// per hanging node on a constrained mesh, colour-ordered like the assembly);
+// the product writes
 // each listed cell's contribution to its slot of ycell ([n_cells][NS][18]),
 // and launch_mf_gather sums them per node (node -> cell*NS + a lists, fixed
 // order) and writes the Dirichlet rows.
 // h32: the Hessian stored in FP32 (for the FP32 inner solve); x32: x in FP32.
 // The gather returns y = J x, or y = b - J x with b != null.
+// With X.xptr set (constrained meshes) the setup also stores the raw local
+// blocks X lists and adds the |local diagonal| of hanging nodes to M.hdiag.
 hipError_t launch_mf_setup(int deg, bool h32, const RefTables &T, const MeshDev &M, const Material &mat,
-                           const double *U, const int *cells, int n, void *Hq, double *Jd, hipStream_t s);
+                           const double *U, const int *cells, int n, void *Hq, double *Jd, const MfCondDev &X,
+                           hipStream_t s);
 // flag (per instance, or null): cells and rows of instances whose flag is 0
 // are skipped, as in the batched SpMV.
 hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, const MeshDev &M, const Material &mat,
                            const void *Hq, const void *x, const int *cells, int n, const int *flag, void *ycell,
                            hipStream_t s);
+// sh (or null): a diagonal term added to every row (the continuation shift).
 template <typename TX>
 hipError_t launch_mf_gather(int n_nodes, const int *ptr, const int *idx, const void *ycell, const uint32_t *mask,
-                            const double *Jd, const TX *x, const TX *b, const int *row_seg, const int *flag, TX *y,
-                            hipStream_t s);
+                            const double *Jd, const double *sh, const TX *x, const TX *b, const int *row_seg,
+                            const int *flag, TX *y, hipStream_t s);
+// Per-cell free energy into Fcell[cell] for cells [0, n_cells).
 hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
                          int n_cells, double *Fcell, hipStream_t s);
 
@@ -164,6 +209,9 @@ hipError_t launch_axpy(size_t n, double a, const TX *x, TY *y, hipStream_t s);
 // dst = src (type conversion)
 template <typename TS, typename TD>
 hipError_t launch_convert(size_t n, const TS *src, TD *dst, hipStream_t s);
+// dst[r] (324 values) = the block blk[r] of val, converted to FP64
+template <typename TJ>
+hipError_t launch_gather_blocks(int n_rows, const int *blk, const TJ *val, double *dst, hipStream_t s);
 // dst = b - w  (double)
 hipError_t launch_sub(size_t n, const double *b, const double *w, double *dst, hipStream_t s);
 // X = U + a dU, then Dirichlet components of X set to zero (constraints distribute)
