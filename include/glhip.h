@@ -272,7 +272,7 @@ int glh_run_prm(glh_ctx *ctx, const glh_prm *p, int max_rows, double *rows, int 
  * outputs.  Training and inference run in hand-written BF16 kernels with
  * FP32 accumulation (matrix cores); master weights and Adam moments are FP32.
  * Constraints: width a multiple of 64, 2*n_freq a multiple of 64,
- * n_out <= 64.  Inputs X are n x 6, targets and predictions n x n_out,
+ * 1 <= depth <= 7, n_out <= 64.  Inputs X are n x 6, targets and predictions n x n_out,
  * row-major FP32 on the host. */
 typedef struct glh_mlp glh_mlp;
 int glh_mlp_create(int device, int n_freq, double sigma, int width, int depth, int n_out, uint64_t seed,

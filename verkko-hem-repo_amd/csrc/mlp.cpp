@@ -58,9 +58,8 @@ struct Buf {
   }
 };
 
-constexpr int OUTP = 64;     // padded width of the output layer
-constexpr int DW_SPLITS = 64;    // K splits of the weight-gradient products
-constexpr int COL_PARTS = 256;   // row slices of the bias-gradient column sums
+constexpr int OUTP = 64;       // padded width of the output layer
+constexpr int DW_SPLITS = 64;  // K splits of the weight-gradient products
 
 }  // namespace
 
@@ -78,7 +77,7 @@ struct glh_mlp {
   long long n_data = 0;
   // batch workspace
   int nb = 0;
-  Buf<float> Xb, Yb, Yo, dH, part, gtmp, lpart, loss;
+  Buf<float> Xb, Yb, Yo, part, cpart, lpart, loss;
   std::vector<Buf<bf16>> H;
   std::vector<Buf<float>> Z;
   Buf<bf16> dY, dZa, dZb;
@@ -132,11 +131,15 @@ int ensure_batch(glh_mlp *m, int nb)
   MCHK(m, m->dY.alloc((size_t)nb * OUTP));
   MCHK(m, m->dZa.alloc((size_t)nb * m->width));
   MCHK(m, m->dZb.alloc((size_t)nb * m->width));
-  MCHK(m, m->dH.alloc((size_t)nb * m->width));
-  size_t pmax = 0;
-  for (int l = 0; l <= m->depth; ++l) pmax = std::max(pmax, (size_t)m->K[l] * m->Np[l]);
-  MCHK(m, m->part.alloc(std::max(pmax * DW_SPLITS, (size_t)COL_PARTS * std::max(m->width, OUTP))));
-  MCHK(m, m->gtmp.alloc(pmax));
+  // split-K partials of every layer's weight and bias gradients at once (the
+  // gradient is summed from them in one pass after the backward sweep)
+  size_t ptot = 0, ctot = 0;
+  for (int l = 0; l <= m->depth; ++l) {
+    ptot += (size_t)m->K[l] * m->Np[l];
+    ctot += (size_t)m->Np[l];
+  }
+  MCHK(m, m->part.alloc(ptot * DW_SPLITS));
+  MCHK(m, m->cpart.alloc(ctot * DW_SPLITS));
   MCHK(m, m->lpart.alloc((size_t)glh::mlp::mse_blocks(nb, OUTP)));
   MCHK(m, m->loss.alloc(1));
   m->nb = nb;
@@ -145,53 +148,92 @@ int ensure_batch(glh_mlp *m, int nb)
   return GLH_OK;
 }
 
+// weight-gradient splits of a batch of nb rows (fixed slices of the batch)
+int dw_splits(int nb)
+{
+  int sp = 1;
+  while (sp < DW_SPLITS && (nb / 32) % (2 * sp) == 0) sp *= 2;
+  return sp;
+}
+
+glh::mlp::GradLayout grad_layout(const glh_mlp *m, int sp)
+{
+  glh::mlp::GradLayout L;
+  L.n_layers = m->depth + 1;
+  L.splits = sp;
+  long long po = 0, co = 0;
+  for (int l = 0; l <= m->depth; ++l) {
+    L.K[l] = m->K[l]; L.N[l] = m->N[l]; L.Np[l] = m->Np[l];
+    L.woff[l] = m->woff[l]; L.boff[l] = m->boff[l];
+    L.poff[l] = po; L.coff[l] = co;
+    po += (long long)sp * m->K[l] * m->Np[l];
+    co += (long long)sp * m->Np[l];
+  }
+  return L;
+}
+
 // forward over the nb rows in Xb (the first n_real are real points); with
-// grad, also the MSE against Yb and the full gradient into G
+// grad, also the MSE against Yb and the full gradient into G.  The bias and
+// SiLU of every hidden layer run in its GEMM's epilogue, the SiLU derivative
+// in the epilogue of the input-gradient GEMM, and the bias gradients beside
+// the weight gradients in their GEMMs (column sums of dZ).
 int forward_backward(glh_mlp *m, int n_real, bool grad)
 {
+  using glh::mlp::GemmEpi;
   const int nb = m->nb, W = m->width, D = m->depth;
   hipStream_t s = m->stream;
   MCHK(m, glh::mlp::fourier(nb, n_real, m->Xb.p, m->Bf.p, m->F, m->H[0].p, s));
   for (int l = 0; l < D; ++l) {
+    GemmEpi E;
+    E.mode = glh::mlp::GEMM_BIAS_SILU;
+    E.bias = m->P.p + m->boff[l];
+    E.Z = m->Z[l].p;
+    E.ldz = W;
+    E.H = m->H[l + 1].p;
+    E.ldh = W;
     MCHK(m, glh::mlp::gemm(nb, W, m->K[l], m->H[l].p, true, m->K[l], m->Wb[l].p, false, m->Np[l], m->Z[l].p, W,
-                           false, 1, 0, s));
-    MCHK(m, glh::mlp::bias_silu((long long)nb * W, W, m->Z[l].p, m->P.p + m->boff[l], m->H[l + 1].p, s));
+                           false, 1, 0, s, E));
   }
-  MCHK(m, glh::mlp::gemm(nb, OUTP, m->K[D], m->H[D].p, true, m->K[D], m->Wb[D].p, false, OUTP, m->Yo.p, OUTP, false,
-                         1, 0, s));
-  MCHK(m, glh::mlp::bias_out(nb, m->n_out, OUTP, m->Yo.p, m->P.p + m->boff[D], s));
+  {
+    GemmEpi E;
+    E.mode = glh::mlp::GEMM_BIAS;
+    E.bias = m->P.p + m->boff[D];
+    E.nbias = m->n_out;
+    MCHK(m, glh::mlp::gemm(nb, OUTP, m->K[D], m->H[D].p, true, m->K[D], m->Wb[D].p, false, OUTP, m->Yo.p, OUTP,
+                           false, 1, 0, s, E));
+  }
   if (!grad) return GLH_OK;
   const int nlp = glh::mlp::mse_blocks(nb, OUTP);
   MCHK(m, glh::mlp::mse(nb, n_real, m->n_out, OUTP, m->Yo.p, m->Yb.p, m->dY.p, m->lpart.p, s));
   MCHK(m, glh::mlp::reduce1(nlp, m->lpart.p, m->loss.p, s));
   // backward: dZ of layer l is dY (l = D) or a BF16 buffer alternating A/B.
-  // The weight gradients split the batch into `sp` fixed slices.
-  int sp = 1;
-  while (sp < DW_SPLITS && (nb / 32) % (2 * sp) == 0) sp *= 2;
+  // The weight gradients split the batch into sp fixed slices.
+  const int sp = dw_splits(nb);
+  const glh::mlp::GradLayout L = grad_layout(m, sp);
   const bf16 *dZ = m->dY.p;
   int ldz = OUTP;
   for (int l = D; l >= 0; --l) {
-    const int Kl = m->K[l], Npl = m->Np[l], Nl = m->N[l];
-    // dW_l = H_l^T dZ_l (split over the batch, fixed-order sum)
-    MCHK(m, glh::mlp::gemm(Kl, Npl, nb, m->H[l].p, false, Kl, dZ, false, ldz, m->part.p, Npl, false, sp,
-                           (long long)Kl * Npl, s));
-    if (Npl == Nl) {
-      MCHK(m, glh::mlp::sum_parts(sp, (long long)Kl * Npl, m->part.p, m->G.p + m->woff[l], s));
-    } else {
-      MCHK(m, glh::mlp::sum_parts(sp, (long long)Kl * Npl, m->part.p, m->gtmp.p, s));
-      MCHK(m, glh::mlp::unpad(Kl, Nl, Npl, m->gtmp.p, m->G.p + m->woff[l], s));
-    }
-    // db_l = column sums of dZ_l
-    MCHK(m, glh::mlp::colsum(nb, Npl, dZ, m->part.p, std::min(COL_PARTS, nb), m->gtmp.p, s));
-    MCHK(m, glh::mlp::unpad(1, Nl, Npl, m->gtmp.p, m->G.p + m->boff[l], s));
+    const int Kl = m->K[l], Npl = m->Np[l];
+    // dW_l = H_l^T dZ_l per slice, and the slice's column sums of dZ_l (db_l)
+    GemmEpi Ew;
+    Ew.colsum = m->cpart.p + L.coff[l];
+    MCHK(m, glh::mlp::gemm(Kl, Npl, nb, m->H[l].p, false, Kl, dZ, false, ldz, m->part.p + L.poff[l], Npl, false, sp,
+                           (long long)Kl * Npl, s, Ew));
     if (l == 0) break;
-    // dH_{l-1} = dZ_l W_l^T, dZ_{l-1} = dH * SiLU'(Z_{l-1})
-    MCHK(m, glh::mlp::gemm(nb, Kl, Npl, dZ, true, ldz, m->Wb[l].p, true, Npl, m->dH.p, Kl, false, 1, 0, s));
+    // dZ_{l-1} = (dZ_l W_l^T) * SiLU'(Z_{l-1})
     bf16 *dZn = (dZ == m->dZa.p) ? m->dZb.p : m->dZa.p;
-    MCHK(m, glh::mlp::silu_back((long long)nb * Kl, m->dH.p, m->Z[l - 1].p, dZn, s));
+    GemmEpi Eb;
+    Eb.mode = glh::mlp::GEMM_SILU_BACK;
+    Eb.Z = m->Z[l - 1].p;
+    Eb.ldz = Kl;
+    Eb.H = dZn;
+    Eb.ldh = Kl;
+    MCHK(m, glh::mlp::gemm(nb, Kl, Npl, dZ, true, ldz, m->Wb[l].p, true, Npl, nullptr, Kl, false, 1, 0, s, Eb));
     dZ = dZn;
     ldz = Kl;
   }
+  // the flat gradient: every parameter's slices summed in order
+  MCHK(m, glh::mlp::grad_reduce(m->n_params, L, m->part.p, m->cpart.p, m->G.p, s));
   return GLH_OK;
 }
 
@@ -201,8 +243,11 @@ int train_step(glh_mlp *m, uint64_t seed, float lr)
   MCHK(m, glh::mlp::step_inc(m->step.p, s));
   MCHK(m, glh::mlp::gather(m->nb, m->n_data, seed, m->step.p, m->Xall.p, m->Yall.p, m->n_out, m->Xb.p, m->Yb.p, s));
   RCHK(forward_backward(m, m->nb, true));
-  MCHK(m, glh::mlp::adam(m->n_params, m->P.p, m->G.p, m->Mo.p, m->Vo.p, lr, 0.9f, 0.999f, 1e-8f, m->step.p, s));
-  RCHK(pack_all(m));
+  // Adam, and the BF16 copies of the updated weights for the next step
+  std::vector<bf16 *> wb(m->depth + 1);
+  for (int l = 0; l <= m->depth; ++l) wb[l] = m->Wb[l].p;
+  MCHK(m, glh::mlp::adam_pack(m->n_params, m->P.p, m->G.p, m->Mo.p, m->Vo.p, lr, 0.9f, 0.999f, 1e-8f, m->step.p,
+                              grad_layout(m, 1), wb.data(), s));
   return GLH_OK;
 }
 
@@ -215,7 +260,8 @@ int glh_mlp_create(int device, int n_freq, double sigma, int width, int depth, i
 {
   if (!out) return GLH_ERR_ARG;
   *out = nullptr;
-  if (n_freq < 1 || width < 64 || width % 64 || depth < 1 || n_out < 1 || n_out > OUTP || (2 * n_freq) % 64 ||
+  if (n_freq < 1 || width < 64 || width % 64 || depth < 1 || depth + 1 > glh::mlp::MLP_MAX_LAYERS || n_out < 1 ||
+      n_out > OUTP || (2 * n_freq) % 64 ||
       !(sigma > 0))
     return GLH_ERR_ARG;
   auto *m = new glh_mlp();
@@ -286,7 +332,7 @@ int glh_mlp_destroy(glh_mlp *m)
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   if (m->graph) (void)hipGraphExecDestroy(m->graph);
   Buf<float> *fb[] = {&m->P, &m->G, &m->Mo, &m->Vo, &m->Bf, &m->Xall, &m->Yall, &m->Xb, &m->Yb,
-                      &m->Yo, &m->dH, &m->part, &m->gtmp, &m->lpart, &m->loss};
+                      &m->Yo, &m->part, &m->cpart, &m->lpart, &m->loss};
   for (auto *b : fb) b->release();
   for (auto &b : m->Wb) b.release();
   for (auto &b : m->H) b.release();

@@ -37,20 +37,28 @@ __device__ inline float from_bf16(bf16 h) { return __uint_as_float((uint32_t)h.b
 
 // A(m,k) = A[m*sam + k*sak] with A_K (sak == 1) or A_M (sam == 1);
 // B(k,n) = B[k*sbk + n*sbn] with B_K (sbk == 1) or B_N (sbn == 1).
-// C[m*ldc + n] = (accumulate ? C : 0) + sum_k A(m,k) B(k,n), over the k
-// range [blockIdx.z * kchunk, +kchunk), written to C + blockIdx.z * c_split.
-template <bool A_K, bool B_K>
+// Sum over the k range [blockIdx.z * kchunk, +kchunk) of A(m,k) B(k,n), then
+// the epilogue EPI (GemmEpi modes): C[m*ldc + n] (+)= it at C + blockIdx.z *
+// c_split, with the column sums of B for CS; the bias and SiLU of a hidden
+// layer; the output layer's bias; the SiLU derivative of the backward pass.
+template <bool A_K, bool B_K, int EPI, bool CS>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int kchunk, const bf16 *__restrict__ A, long long sa,
                                               const bf16 *__restrict__ B, long long sb, float *__restrict__ C,
-                                              int ldc, long long c_split, int accumulate)
+                                              int ldc, long long c_split, int accumulate, GemmEpi E)
 {
   __shared__ __attribute__((aligned(16))) uint16_t As[BM][LDK];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][LDK];
+  __shared__ float red[CS ? 32 : 1][BN];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int kbeg = blockIdx.z * kchunk;
   const uint16_t *Ag = reinterpret_cast<const uint16_t *>(A);
   const uint16_t *Bg = reinterpret_cast<const uint16_t *>(B);
+  // column sums of B (row block 0 only): thread t sums the 8 columns it stages
+  const bool cs = CS && blockIdx.y == 0;
+  float csum[CS ? 8 : 1];
+#pragma unroll
+  for (int i = 0; i < (CS ? 8 : 1); ++i) csum[i] = 0.f;
   f32x4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -80,6 +88,10 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int kchunk, const bf
       const uint16_t *e = reinterpret_cast<const uint16_t *>(&v);
 #pragma unroll
       for (int i = 0; i < 8; ++i) Bs[n + i][kk] = e[i];
+      if (CS && cs) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) csum[i] += __uint_as_float((uint32_t)e[i] << 16);
+      }
     }
     __syncthreads();
     // lane l: row/col l % 16 of the 16 x 16 block, k = 8 (l / 16) .. + 7
@@ -98,6 +110,17 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int kchunk, const bf
       for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     __syncthreads();
   }
+  if (CS && cs) {  // the 32 threads staging one group of 8 columns, summed in row order
+    const int kk = t >> 3, n = (t & 7) * 8;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) red[kk][n + i] = csum[i];
+    __syncthreads();
+    if (t < BN) {
+      float s = 0.f;
+      for (int q = 0; q < 32; ++q) s += red[q][t];
+      E.colsum[(size_t)blockIdx.z * N + n0 + t] = s;
+    }
+  }
   // D layout: lane l holds rows 4 (l / 16) + q, column l % 16
   float *Cz = C + (size_t)blockIdx.z * c_split;
   const int cc = lane & 15, rq = (lane >> 4) * 4;
@@ -107,8 +130,21 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int kchunk, const bf
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        float *p = Cz + (size_t)(m0 + wm + i * 16 + rq + q) * ldc + (n0 + wn + j * 16 + cc);
-        *p = accumulate ? *p + acc[i][j][q] : acc[i][j][q];
+        const int m = m0 + wm + i * 16 + rq + q, n = n0 + wn + j * 16 + cc;
+        const float v = acc[i][j][q];
+        if (EPI == GEMM_BIAS_SILU) {
+          const float z = v + E.bias[n];
+          E.Z[(size_t)m * E.ldz + n] = z;
+          E.H[(size_t)m * E.ldh + n] = to_bf16(z / (1.f + expf(-z)));
+        } else if (EPI == GEMM_BIAS) {
+          C[(size_t)m * ldc + n] = n < E.nbias ? v + E.bias[n] : v;
+        } else if (EPI == GEMM_SILU_BACK) {
+          const float z = E.Z[(size_t)m * E.ldz + n], sg = 1.f / (1.f + expf(-z));
+          E.H[(size_t)m * E.ldh + n] = to_bf16(v * sg * (1.f + z * (1.f - sg)));
+        } else {
+          float *p = Cz + (size_t)m * ldc + n;
+          *p = accumulate ? *p + v : v;
+        }
       }
 }
 
@@ -126,27 +162,6 @@ __global__ __launch_bounds__(ET) void k_fourier(int n, int n_real, const float *
   const float ang = 6.283185307179586f * a;
   H[(size_t)s * 2 * F + j] = to_bf16(s < n_real ? sinf(ang) : 0.f);
   H[(size_t)s * 2 * F + F + j] = to_bf16(s < n_real ? cosf(ang) : 0.f);
-}
-
-// Z += bias; H = bf16(SiLU(Z)) (Z kept for the backward pass)
-__global__ __launch_bounds__(ET) void k_bias_silu(long long total, int N, float *__restrict__ Z,
-                                                  const float *__restrict__ bias, bf16 *__restrict__ H)
-{
-  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
-  if (i >= total) return;
-  const float z = Z[i] + bias[i % N];
-  Z[i] = z;
-  H[i] = to_bf16(z / (1.f + expf(-z)));
-}
-
-// output layer: Y[s, o] += bias[o] (o < n_out; padded columns untouched)
-__global__ __launch_bounds__(ET) void k_bias_out(int n, int n_out, int ldy, float *__restrict__ Y,
-                                                 const float *__restrict__ bias)
-{
-  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
-  if (i >= (long long)n * n_out) return;
-  const int s = (int)(i / n_out), o = (int)(i % n_out);
-  Y[(size_t)s * ldy + o] += bias[o];
 }
 
 // mean-squared error over the n_real x n_out real outputs: dY = bf16(2 (Y - T)
@@ -177,28 +192,6 @@ __global__ __launch_bounds__(ET) void k_mse(int n, int n_real, int n_out, int ld
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
-// dZ = bf16(dH * SiLU'(Z))
-__global__ __launch_bounds__(ET) void k_silu_back(long long total, const float *__restrict__ dH,
-                                                  const float *__restrict__ Z, bf16 *__restrict__ dZ)
-{
-  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
-  if (i >= total) return;
-  const float z = Z[i], sg = 1.f / (1.f + expf(-z));
-  dZ[i] = to_bf16(dH[i] * sg * (1.f + z * (1.f - sg)));
-}
-
-// column sums of a BF16 n x N matrix, FP32, in fixed order: partial[blk][col]
-__global__ __launch_bounds__(ET) void k_colsum_partial(int n, int N, int rows_per_blk, const bf16 *__restrict__ D,
-                                                       float *__restrict__ partial)
-{
-  const int col = threadIdx.x + blockIdx.y * ET;
-  if (col >= N) return;
-  const int r0 = blockIdx.x * rows_per_blk, r1 = min(n, r0 + rows_per_blk);
-  float s = 0.f;
-  for (int r = r0; r < r1; ++r) s += from_bf16(D[(size_t)r * N + col]);
-  partial[(size_t)blockIdx.x * N + col] = s;
-}
-
 // out[0] = sum of n values, one workgroup, fixed order (strided sums, then a tree)
 __global__ __launch_bounds__(ET) void k_reduce1(int n, const float *__restrict__ in, float *__restrict__ out)
 {
@@ -214,21 +207,49 @@ __global__ __launch_bounds__(ET) void k_reduce1(int n, const float *__restrict__
   if (threadIdx.x == 0) out[0] = sh[0];
 }
 
-// out[i] = sum_p partial[p*len + i] (fixed order)
-__global__ __launch_bounds__(ET) void k_sum_parts(int parts, long long len, const float *__restrict__ partial,
-                                                  float *__restrict__ out)
+struct WbPtrs {
+  bf16 *p[MLP_MAX_LAYERS];
+};
+
+// layer of flat parameter i (layers in order, each its weights then its biases)
+__device__ inline int param_layer(const GradLayout &L, long long i)
 {
-  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
-  if (i >= len) return;
-  float s = 0.f;
-  for (int p = 0; p < parts; ++p) s += partial[(size_t)p * len + i];
-  out[i] = s;
+  int l = 0;
+  while (l + 1 < L.n_layers && i >= L.woff[l + 1]) ++l;
+  return l;
 }
 
-// Adam on the FP32 master parameters (bias-corrected, step t >= 1)
-__global__ __launch_bounds__(ET) void k_adam(long long n, float *__restrict__ p, const float *__restrict__ g,
-                                             float *__restrict__ m, float *__restrict__ v, float lr, float b1,
-                                             float b2, float eps, const int *__restrict__ step)
+// G[i] = sum over the splits z (in order) of the partials of parameter i: the
+// weight-gradient GEMM's part[poff + z K Np + k Np + n], or for a bias the
+// column sums cpart[coff + z Np + n]
+__global__ __launch_bounds__(ET) void k_grad_reduce(long long n, GradLayout L, const float *__restrict__ part,
+                                                    const float *__restrict__ cpart, float *__restrict__ G)
+{
+  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
+  if (i >= n) return;
+  const int l = param_layer(L, i);
+  const float *src;
+  long long stride;
+  if (i < L.boff[l]) {
+    const long long r = i - L.woff[l];
+    const int k = (int)(r / L.N[l]), c = (int)(r % L.N[l]);
+    src = part + L.poff[l] + (long long)k * L.Np[l] + c;
+    stride = (long long)L.K[l] * L.Np[l];
+  } else {
+    src = cpart + L.coff[l] + (i - L.boff[l]);
+    stride = L.Np[l];
+  }
+  float s = 0.f;
+  for (int z = 0; z < L.splits; ++z) s += src[(size_t)z * stride];
+  G[i] = s;
+}
+
+// Adam on the FP32 master parameters (bias-corrected, step t >= 1); a weight
+// also refreshes its BF16 copy Wb[l][k Np + n] (the padding stays 0)
+__global__ __launch_bounds__(ET) void k_adam_pack(long long n, float *__restrict__ p, const float *__restrict__ g,
+                                                  float *__restrict__ m, float *__restrict__ v, float lr, float b1,
+                                                  float b2, float eps, const int *__restrict__ step, GradLayout L,
+                                                  WbPtrs Wb)
 {
   const long long i = (long long)blockIdx.x * ET + threadIdx.x;
   if (i >= n) return;
@@ -238,7 +259,13 @@ __global__ __launch_bounds__(ET) void k_adam(long long n, float *__restrict__ p,
   m[i] = mi;
   v[i] = vi;
   const float mh = mi / (1.f - powf(b1, (float)t)), vh = vi / (1.f - powf(b2, (float)t));
-  p[i] -= lr * mh / (sqrtf(vh) + eps);
+  const float pn = p[i] - lr * mh / (sqrtf(vh) + eps);
+  p[i] = pn;
+  const int l = param_layer(L, i);
+  if (i < L.boff[l]) {
+    const long long r = i - L.woff[l];
+    Wb.p[l][(r / L.N[l]) * L.Np[l] + r % L.N[l]] = to_bf16(pn);
+  }
 }
 
 // BF16 copy of a K x N FP32 weight into a K x Np padded matrix (zeros beyond N)
@@ -271,16 +298,6 @@ __global__ __launch_bounds__(ET) void k_gather(int nb, long long n_data, uint64_
   for (int o = 0; o < n_out; ++o) Yb[(size_t)i * n_out + o] = Y[(size_t)j * n_out + o];
 }
 
-// dst (K x N) = the first N columns of src (K x Np)
-__global__ __launch_bounds__(ET) void k_unpad(int K, int N, int Np, const float *__restrict__ src,
-                                              float *__restrict__ dst)
-{
-  const long long i = (long long)blockIdx.x * ET + threadIdx.x;
-  if (i >= (long long)K * N) return;
-  const int k = (int)(i / N), c = (int)(i % N);
-  dst[i] = src[(size_t)k * Np + c];
-}
-
 // Y[s, o] = Yp[s*ldy + o] for s < n, o < n_out
 __global__ __launch_bounds__(ET) void k_take_out(int n, int n_out, int ldy, const float *__restrict__ Yp,
                                                  float *__restrict__ Y)
@@ -300,37 +317,53 @@ inline unsigned grid1(long long n) { return (unsigned)((n + ET - 1) / ET); }
 }  // namespace
 
 hipError_t gemm(int M, int N, int K, const bf16 *A, bool a_kcontig, long long sa, const bf16 *B, bool b_kcontig,
-                long long sb, float *C, int ldc, bool accumulate, int splits, long long c_split, hipStream_t s)
+                long long sb, float *C, int ldc, bool accumulate, int splits, long long c_split, hipStream_t s,
+                const GemmEpi &E)
 {
   if (M % BM || N % BN || K % (BK * splits)) return hipErrorInvalidValue;
   const dim3 grid(N / BN, M / BM, splits);
   const int kchunk = K / splits;
   const int acc = accumulate ? 1 : 0;
-  if (a_kcontig && b_kcontig)
-    hipLaunchKernelGGL((k_gemm<true, true>), grid, dim3(256), 0, s, M, N, kchunk, A, sa, B, sb, C, ldc, c_split, acc);
-  else if (a_kcontig)
-    hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(256), 0, s, M, N, kchunk, A, sa, B, sb, C, ldc, c_split, acc);
-  else if (b_kcontig)
-    hipLaunchKernelGGL((k_gemm<false, true>), grid, dim3(256), 0, s, M, N, kchunk, A, sa, B, sb, C, ldc, c_split, acc);
-  else
-    hipLaunchKernelGGL((k_gemm<false, false>), grid, dim3(256), 0, s, M, N, kchunk, A, sa, B, sb, C, ldc, c_split,
-                       acc);
+#define GLH_GEMM(AK, BK_, EPI, CS) \
+  hipLaunchKernelGGL((k_gemm<AK, BK_, EPI, CS>), grid, dim3(256), 0, s, M, N, kchunk, A, sa, B, sb, C, ldc, c_split, acc, E)
+  // the combinations the network uses
+  if (E.mode == GEMM_BIAS_SILU && a_kcontig && !b_kcontig) GLH_GEMM(true, false, GEMM_BIAS_SILU, false);
+  else if (E.mode == GEMM_BIAS && a_kcontig && !b_kcontig) GLH_GEMM(true, false, GEMM_BIAS, false);
+  else if (E.mode == GEMM_SILU_BACK && a_kcontig && b_kcontig) GLH_GEMM(true, true, GEMM_SILU_BACK, false);
+  else if (E.mode != GEMM_STORE) return hipErrorInvalidValue;
+  else if (E.colsum) {
+    if (b_kcontig) return hipErrorInvalidValue;
+    if (a_kcontig) GLH_GEMM(true, false, GEMM_STORE, true);
+    else GLH_GEMM(false, false, GEMM_STORE, true);
+  } else if (a_kcontig && b_kcontig) GLH_GEMM(true, true, GEMM_STORE, false);
+  else if (a_kcontig) GLH_GEMM(true, false, GEMM_STORE, false);
+  else if (b_kcontig) GLH_GEMM(false, true, GEMM_STORE, false);
+  else GLH_GEMM(false, false, GEMM_STORE, false);
+#undef GLH_GEMM
+  return hipGetLastError();
+}
+
+hipError_t grad_reduce(long long n_params, const GradLayout &L, const float *part, const float *cpart, float *G,
+                       hipStream_t s)
+{
+  if (L.n_layers < 1 || L.n_layers > MLP_MAX_LAYERS) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_grad_reduce, dim3(grid1(n_params)), dim3(ET), 0, s, n_params, L, part, cpart, G);
+  return hipGetLastError();
+}
+
+hipError_t adam_pack(long long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
+                     const int *step, const GradLayout &L, bf16 *const *Wb, hipStream_t s)
+{
+  if (L.n_layers < 1 || L.n_layers > MLP_MAX_LAYERS) return hipErrorInvalidValue;
+  WbPtrs W{};
+  for (int l = 0; l < L.n_layers; ++l) W.p[l] = Wb[l];
+  hipLaunchKernelGGL(k_adam_pack, dim3(grid1(n)), dim3(ET), 0, s, n, p, g, m, v, lr, b1, b2, eps, step, L, W);
   return hipGetLastError();
 }
 
 hipError_t fourier(int n, int n_real, const float *X, const float *Bf, int F, bf16 *H, hipStream_t s)
 {
   hipLaunchKernelGGL(k_fourier, dim3(grid1((long long)n * F)), dim3(ET), 0, s, n, n_real, X, Bf, F, H);
-  return hipGetLastError();
-}
-hipError_t bias_silu(long long total, int N, float *Z, const float *bias, bf16 *H, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_bias_silu, dim3(grid1(total)), dim3(ET), 0, s, total, N, Z, bias, H);
-  return hipGetLastError();
-}
-hipError_t bias_out(int n, int n_out, int ldy, float *Y, const float *bias, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_bias_out, dim3(grid1((long long)n * n_out)), dim3(ET), 0, s, n, n_out, ldy, Y, bias);
   return hipGetLastError();
 }
 hipError_t mse(int n, int n_real, int n_out, int ldy, const float *Y, const float *T, bf16 *dY, float *partial,
@@ -341,32 +374,9 @@ hipError_t mse(int n, int n_real, int n_out, int ldy, const float *Y, const floa
   return hipGetLastError();
 }
 int mse_blocks(int n, int ldy) { return (int)grid1((long long)n * ldy); }
-hipError_t silu_back(long long total, const float *dH, const float *Z, bf16 *dZ, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_silu_back, dim3(grid1(total)), dim3(ET), 0, s, total, dH, Z, dZ);
-  return hipGetLastError();
-}
-hipError_t colsum(int n, int N, const bf16 *D, float *partial, int parts, float *out, hipStream_t s)
-{
-  const int rows = (n + parts - 1) / parts;
-  hipLaunchKernelGGL(k_colsum_partial, dim3(parts, (N + ET - 1) / ET), dim3(ET), 0, s, n, N, rows, D, partial);
-  hipLaunchKernelGGL(k_sum_parts, dim3(grid1(N)), dim3(ET), 0, s, parts, (long long)N, partial, out);
-  return hipGetLastError();
-}
 hipError_t reduce1(int n, const float *in, float *out, hipStream_t s)
 {
   hipLaunchKernelGGL(k_reduce1, dim3(1), dim3(ET), 0, s, n, in, out);
-  return hipGetLastError();
-}
-hipError_t sum_parts(int parts, long long len, const float *partial, float *out, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_sum_parts, dim3(grid1(len)), dim3(ET), 0, s, parts, len, partial, out);
-  return hipGetLastError();
-}
-hipError_t adam(long long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
-                const int *step, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_adam, dim3(grid1(n)), dim3(ET), 0, s, n, p, g, m, v, lr, b1, b2, eps, step);
   return hipGetLastError();
 }
 hipError_t pack_w(int K, int N, int Np, const float *W, bf16 *Wb, hipStream_t s)
@@ -378,11 +388,6 @@ hipError_t gather(int nb, long long n_data, uint64_t seed, const int *step, cons
                   float *Xb, float *Yb, hipStream_t s)
 {
   hipLaunchKernelGGL(k_gather, dim3(grid1(nb)), dim3(ET), 0, s, nb, n_data, seed, step, X, Y, n_out, Xb, Yb);
-  return hipGetLastError();
-}
-hipError_t unpad(int K, int N, int Np, const float *src, float *dst, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_unpad, dim3(grid1((long long)K * N)), dim3(ET), 0, s, K, N, Np, src, dst);
   return hipGetLastError();
 }
 hipError_t take_out(int n, int n_out, int ldy, const float *Yp, float *Y, hipStream_t s)
