@@ -628,7 +628,13 @@ def main():
                 "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                 "traffic": tjson.get("k_mf_apply", {}).get("bytes_per_product"), "algorithmic_bytes": nbytes,
                 "algorithmic_flops": flops, "tflop_s": flops / (apply_avg_ms * 1e-3) / 1e12,
-                "frac_fp64_peak": flops / (apply_avg_ms * 1e-3) / 1e12 / FP64_PEAK_TFS}
+                "frac_fp64_peak": flops / (apply_avg_ms * 1e-3) / 1e12 / FP64_PEAK_TFS,
+                "stream_read_gb_s": stream_gbs, "frac_of_stream_read": ach / stream_gbs,
+                # the algorithm's bound is HBM; the kernel itself waits on the
+                # latency of its loads and barriers at full occupancy (SQ
+                # counters in profiles/matfree_cube_q2_ref5.md)
+                "limiter": "load and barrier latency at the register-limited occupancy (waves wait 57% of "
+                           "their cycles, VALU 27% busy)"}
     else:
         roof = spmv_roofline(apply_avg_ms)
     st0 = runs[0]

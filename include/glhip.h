@@ -427,21 +427,23 @@ int glh_instance_stats(glh_ctx *ctx, double *out);
  * (HIP events): ms[0] = J+R assembly ms, ms[1] = number of launches. */
 int glh_last_assembly_ms(glh_ctx *ctx, double ms[2]);
 /* level > 0: one progress line per Newton step on stderr (glh_newton_solve). */
-/* Schedule of the batched solve: GMRES restart used when glh_newton_solve
- * gets restart <= 0 (1..126), Arnoldi steps per replayed graph block
+/* Schedule of the solve: GMRES restart used when glh_newton_solve gets
+ * restart <= 0 (1..126), Arnoldi steps per replayed graph block of a batch
  * (0: 8), grid caps of the active-set launches in block rows / 4 and in
- * chunks (0: 4096 / 1024). */
-int glh_set_schedule(glh_ctx *ctx, int restart, int q_block, int row_cap, int chunk_cap);
-/* Autotuner of the batched launch schedule: times `trial_newton` Newton
- * steps (after one untimed step that records the graphs) from the current
- * solution for blocks of 4 / 8 / 16 Arnoldi steps per replayed graph and
- * grid caps 2048 / 8192 block rows, keeps the fastest in the context and
- * restores the solution.  These knobs change launch shapes only, not the
- * arithmetic.  The restart is the context's (glh_set_schedule, default
- * 100): it changes the iteration, which short trials do not predict.
- * out: [0] restart [1] block [2] row cap [3] chunk cap [4] best trial
- * seconds [5] trials run. */
-int glh_autotune(glh_ctx *ctx, int fp32, int trial_newton, double lin_tol, int lin_maxit, double out[6]);
+ * chunks (0: 4096 / 1024), Gram-Schmidt tile (4, 8 or 16 basis vectors per
+ * multi-dot workgroup and in flight per thread of the update; 0: 8). */
+int glh_set_schedule(glh_ctx *ctx, int restart, int q_block, int row_cap, int chunk_cap, int gs_tile);
+/* Autotuner of the launch schedule and kernel tiles: times `trial_newton`
+ * Newton steps (after one untimed step that records the graphs) from the
+ * current solution, for a batch over blocks of 4 / 8 / 16 Arnoldi steps per
+ * replayed graph and grid caps 2048 / 8192 block rows, then with the best of
+ * those over Gram-Schmidt tiles 4 / 8 / 16; keeps the fastest in the context
+ * and restores the solution.  These knobs change launch and tile shapes
+ * only: the iterates are the same bits.  The restart is the context's
+ * (glh_set_schedule, default 100): it changes the iteration, which short
+ * trials do not predict.  out: [0] restart [1] block [2] row cap [3] chunk
+ * cap [4] best trial seconds [5] trials run [6] Gram-Schmidt tile. */
+int glh_autotune(glh_ctx *ctx, int fp32, int trial_newton, double lin_tol, int lin_maxit, double out[7]);
 /* Forcing terms of glh_newton_solve: 0 (default) the fixed lin_tol of the
  * reference (solve.cc:119-128); 1 Eisenstat-Walker (choice 2: eta_k =
  * 0.9 (||R_k|| / ||R_k-1||)^2 with the usual safeguard), capped at lin_tol,

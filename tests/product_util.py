@@ -215,3 +215,39 @@ def oracle_caller(om, perm, images, mesh2):
     faces = a["faces"].reshape(-1, 3) if mesh2.n_faces else np.zeros((0, 3), np.int32)
     return O.Mesh(mesh2.deg, a["cell_nodes"], a["cell_h"], [tuple(f) for f in faces], n2, con_of,
                   np.array(ptr, np.int32), ms, ws, a["row_ptr"], a["col"], a["node_xyz"].reshape(-1, 3))
+
+
+def oracle_from_library(mesh):
+    """oracle_lib.Mesh of a library mesh under the library's own constraint
+    lines, in deal.II order: the closed hanging-node lines first (a master's
+    Dirichlet components drop out), then homogeneous Dirichlet lines for the
+    remaining masked dofs; the library's condensed pattern.  The lines
+    themselves are checked against an independent derivation on small forests
+    (oracle_adaptive); this form is vectorised for meshes of millions of
+    dofs, where that derivation is too slow."""
+    a = mesh.arrays()
+    con = mesh.constraints()
+    n = mesh.n_nodes
+    mask = a["node_mask"].astype(np.int64) & ((1 << 18) - 1)
+    bits = ((mask[:, None] >> np.arange(18)) & 1).astype(bool)
+    hp, hm, hw = (np.asarray(con["hang_ptr"], np.int64), np.asarray(con["hang_m"], np.int64),
+                  np.asarray(con["hang_w"], np.float64))
+    hanging = hp[1:] > hp[:-1]
+    constrained = (bits | hanging[:, None]).reshape(-1)
+    cdofs = np.nonzero(constrained)[0]
+    con_of = -np.ones(n * 18, np.int32)
+    con_of[cdofs] = np.arange(cdofs.size, dtype=np.int32)
+    v, c = cdofs // 18, cdofs % 18
+    lens = np.where(hanging[v], hp[v + 1] - hp[v], 0)
+    line = np.repeat(np.arange(cdofs.size), lens)
+    start = np.repeat(hp[v] - np.concatenate(([0], np.cumsum(lens)[:-1])), lens)
+    ent = start + np.arange(lens.sum())
+    m, comp = hm[ent], np.repeat(c, lens)
+    keep = ~bits[m, comp]
+    line, m, comp, w = line[keep], m[keep], comp[keep], hw[ent][keep]
+    con_ptr = np.zeros(cdofs.size + 1, np.int32)
+    np.add.at(con_ptr, line + 1, 1)
+    con_ptr = np.cumsum(con_ptr).astype(np.int32)
+    faces = a["faces"].reshape(-1, 3) if mesh.n_faces else np.zeros((0, 3), np.int32)
+    return O.Mesh(mesh.deg, a["cell_nodes"], a["cell_h"], [tuple(f) for f in faces], n, con_of, con_ptr,
+                  (m * 18 + comp).astype(np.int32), w, a["row_ptr"], a["col"], a["node_xyz"].reshape(-1, 3))

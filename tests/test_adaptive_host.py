@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from product_util import adaptive_forest, glhip, oracle_adaptive, side_masks
+from product_util import adaptive_forest, glhip, oracle_adaptive, oracle_from_library, side_masks
 
 G = glhip()
 LO, HI = (-20.0, -20.0, -6.0), (20.0, 20.0, 6.0)
@@ -535,3 +535,23 @@ def test_a_family_next_to_a_refined_cell_keeps_its_children(deg):
             assert np.abs(U2[v] - U[k]).max() <= 1e-12 * np.abs(U).max()
             kept += 1
     assert kept > 0
+
+
+@pytest.mark.parametrize("deg,sides,steps,seed", [(1, [1, 1, 1, 1, 0, 0], 2, 1), (2, [1, 1, 1, 1, 4, 4], 1, 2),
+                                                  (1, [0, 0, 1, 1, 0, 0], 2, 3)])
+def test_vectorised_oracle_lines_equal_the_independent_derivation(deg, sides, steps, seed):
+    # the oracle mesh built from the library's own lines (used at full
+    # size, where the independent derivation is too slow) distributes and
+    # condenses exactly as the independent one on small forests
+    lo, hi = (-6.0, -6.0, -4.0), (6.0, 6.0, 4.0)
+    f = adaptive_forest(G, (3, 3, 2) if deg == 1 else (2, 2, 2), lo, hi, sides, steps, seed, degree=deg)
+    m = f.mesh()
+    om, ol = oracle_adaptive(m, lo, hi, sides), oracle_from_library(m)
+    assert np.array_equal(om.con_of >= 0, ol.con_of >= 0)
+    co = O.coef(22.0, 0.8)
+    U = O.initial_guess(m.n_nodes, O.matep(22.0, 0.8)["gap_B"], amp=0.05, seed=seed)
+    Ua, Ub = O.distribute(om, U), O.distribute(ol, U)
+    assert np.array_equal(Ua, Ub)
+    _, ra = O.assemble(om, co, Ua, J=False)
+    _, rb = O.assemble(ol, co, Ub, J=False)
+    assert np.abs(ra - rb).max() <= 1e-14 * np.abs(ra).max()

@@ -131,7 +131,7 @@ def test_autotuned_schedule_restores_state_and_keeps_the_fixed_point(ctx):
     say("autotune", tuned)
     assert tuned["restart"] == 100 and tuned["q_block"] in (4, 8, 16)
     assert tuned["row_cap"] in (2048, 8192) and tuned["chunk_cap"] == tuned["row_cap"] // 4
-    assert tuned["candidates"] == 6
+    assert tuned["candidates"] == 8 and tuned["gs_tile"] in (4, 8, 16)
     assert np.array_equal(ctx.solution(), U0)  # the trials leave the solution as it was
     # restart <= 0: the tuned restart; the schedule changes only the launch
     # shapes, so the fixed point is the one of a fixed schedule
@@ -144,6 +144,16 @@ def test_autotuned_schedule_restores_state_and_keeps_the_fixed_point(ctx):
     say(st, st_fixed)
     assert st["converged"] == 1.0 and st_fixed["converged"] == 1.0
     assert np.allclose(F_tuned, F_fixed, rtol=1e-10, atol=0)
+    # the Gram-Schmidt tiles sum in one order: the same iterates to the bit
+    res = []
+    for gs in (4, 16):
+        ctx.set_schedule(restart=tuned["restart"], q_block=8, row_cap=4096, chunk_cap=1024, gs_tile=gs)
+        ctx.set_solution(U0)
+        st_gs = ctx.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=tuned["restart"], fp32=True)
+        res.append((st_gs["gmres_its"], ctx.instance_energy(), ctx.solution()))
+    assert res[0][0] == res[1][0] == st_fixed["gmres_its"]
+    assert np.array_equal(res[0][1], res[1][1]) and np.array_equal(res[0][2], res[1][2])
+    assert np.array_equal(res[0][1], F_fixed)
     ctx.set_schedule(restart=100, q_block=0, row_cap=0, chunk_cap=0)
 
 

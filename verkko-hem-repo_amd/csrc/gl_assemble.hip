@@ -405,7 +405,9 @@ __device__ inline int upair(int c, int d)
   return lo * 18 - lo * (lo - 1) / 2 + (hi - lo);
 }
 
-template <int DEG, typename TH>
+// CON: a mesh with constraint lines (the hanging nodes' |local diagonal| and
+// the raw local blocks of X as well)
+template <int DEG, typename TH, bool CON>
 __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ U,
                                                  const int *__restrict__ cells, TH *__restrict__ Hq,
                                                  double *__restrict__ Jd, MfCondDev X)
@@ -489,7 +491,7 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
     pair_block(a * NS - a * (a - 1) / 2, a, a, v1, v2);  // pair (a, a)
     const uint32_t ma = Mh.node_mask[cn[a]];
     // a hanging node's condensed row keeps the sum over cells of |local diagonal|
-    if (Mh.hdiag && c == d && (ma & HANG_BIT)) Mh.hdiag[(size_t)cn[a] * 18 + c] += fabs(v1);
+    if (CON && c == d && (ma & HANG_BIT)) Mh.hdiag[(size_t)cn[a] * 18 + c] += fabs(v1);
     const bool r1 = (ma >> c) & 1u, c1 = (ma >> d) & 1u, r2 = (ma >> d) & 1u, c2 = (ma >> c) & 1u;
     if (r1 || c1) v1 = (r1 && c == d) ? fabs(v1) : 0.0;
     if (r2 || c2) v2 = (r2 && c == d) ? fabs(v2) : 0.0;
@@ -500,7 +502,7 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
   }
   // constrained meshes: the raw local blocks that constraint lines fold into
   // the condensed diagonal blocks of masters (few cells have any)
-  if (X.xptr)
+  if (CON)
     for (int k = X.xptr[e]; k < X.xptr[e + 1]; ++k) {
       const int p = X.xpair[k];
       double v1, v2;
@@ -963,10 +965,14 @@ hipError_t launch_mf_setup(int deg, bool h32, const RefTables &T, const MeshDev 
                            hipStream_t s)
 {
   if (n <= 0) return hipSuccess;
-#define GLH_MFS(D, TH) \
-  hipLaunchKernelGGL((k_mf_setup<D, TH>), dim3(n), dim3(NT), 0, s, T, M, mat, U, cells, (TH *)Hq, Jd, X)
-  if (deg == 1) { if (h32) GLH_MFS(1, float); else GLH_MFS(1, double); }
-  else { if (h32) GLH_MFS(2, float); else GLH_MFS(2, double); }
+  if (X.xptr && !M.hdiag) return hipErrorInvalidValue;  // the constrained setup writes hdiag
+#define GLH_MFS(D, TH, CON) \
+  hipLaunchKernelGGL((k_mf_setup<D, TH, CON>), dim3(n), dim3(NT), 0, s, T, M, mat, U, cells, (TH *)Hq, Jd, X)
+#define GLH_MFS2(D, TH) \
+  if (X.xptr) GLH_MFS(D, TH, true); else GLH_MFS(D, TH, false)
+  if (deg == 1) { if (h32) GLH_MFS2(1, float); else GLH_MFS2(1, double); }
+  else { if (h32) GLH_MFS2(2, float); else GLH_MFS2(2, double); }
+#undef GLH_MFS2
 #undef GLH_MFS
   return hipGetLastError();
 }

@@ -545,19 +545,19 @@ __device__ inline typename NV2<T>::type ld2(const T *__restrict__ p)
   return *reinterpret_cast<const typename NV2<T>::type *>(p);
 }
 
-// 2D grid: x = chunks (or the active list), y = groups of KCH vectors.
+// 2D grid: x = chunks (or the active list), y = groups of G vectors.
 // Without an active set the grid is numbered XCD by XCD: workgroups go to
 // the 8 XCDs in turn (linear id mod 8), so linear id L = 8 r + x runs chunk
 // 8 (r / groups) + x, group r % groups, and every group of a chunk runs on
 // the XCD of that chunk, whose L2 then serves w to all but the first
 // (gridDim.x is a multiple of 8 there).
-template <typename T>
+template <typename T, int G>
 __global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__restrict__ ch, int n_chunks,
                                                          ActiveSet act, int k, const T *__restrict__ V, size_t ldv,
                                                          const T *__restrict__ w, T *__restrict__ partial,
                                                          const int *__restrict__ flag)
 {
-  __shared__ T sh[KCH][RT / 64];
+  __shared__ T sh[G][RT / 64];
   int nw = work_count(act, n_chunks), first = blockIdx.x, step = gridDim.x, grp = blockIdx.y;
   if (!act.counts) {
     const int L = blockIdx.x + gridDim.x * blockIdx.y, r = L >> 3;
@@ -565,20 +565,20 @@ __global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__rest
     first = 8 * (r / gridDim.y) + (L & 7);
     step = nw;  // one chunk per workgroup
   }
-  const int i0 = grp * KCH;
-  const int kk = (k - i0) < KCH ? (k - i0) : KCH;
+  const int i0 = grp * G;
+  const int kk = (k - i0) < G ? (k - i0) : G;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int wi = first; wi < nw; wi += step) {
     const int cb = work_chunk(act, wi);
     if (flag && !flag[ch[3 * cb]]) continue;  // skipped instance: its partials are never read
     const long long beg = ch[3 * cb + 1], end = ch[3 * cb + 2];
-    T acc[KCH];
+    T acc[G];
 #pragma unroll
-    for (int j = 0; j < KCH; ++j) acc[j] = 0;
+    for (int j = 0; j < G; ++j) acc[j] = 0;
     for (long long idx = beg + 2 * threadIdx.x; idx < end; idx += 2 * RT) {
       const auto x = ld2(w + idx);
 #pragma unroll
-      for (int j = 0; j < KCH; ++j)
+      for (int j = 0; j < G; ++j)
         if (j < kk) {
           const auto v = ld2nt(V + (size_t)(i0 + j) * ldv + idx);
           acc[j] += v.x * x.x;
@@ -586,7 +586,7 @@ __global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__rest
         }
     }
 #pragma unroll
-    for (int j = 0; j < KCH; ++j) {
+    for (int j = 0; j < G; ++j) {
       const T r = wave_sum(acc[j]);
       if (lane == 0) sh[j][wv] = r;
     }
@@ -623,7 +623,7 @@ __global__ __launch_bounds__(RT) void k_seg_final(const int *__restrict__ seg_ch
   }
 }
 
-template <typename T>
+template <typename T, int UF>
 __global__ __launch_bounds__(RT) void k_seg_sub_lincomb_norm(const long long *__restrict__ ch, int n_chunks,
                                                              ActiveSet act, int k, const T *__restrict__ V,
                                                              size_t ldv, const T *__restrict__ h, int ldh,
@@ -643,12 +643,12 @@ __global__ __launch_bounds__(RT) void k_seg_sub_lincomb_norm(const long long *__
       T s0 = w2.x, s1 = w2.y;
       const T *vp = V + idx;
       int i = 0;
-      for (; i + 8 <= k; i += 8) {  // eight basis vectors in flight
-        typename NV2<T>::type v[8];
+      for (; i + UF <= k; i += UF) {  // UF basis vectors in flight
+        typename NV2<T>::type v[UF];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = ld2nt(vp + (size_t)(i + u) * ldv);
+        for (int u = 0; u < UF; ++u) v[u] = ld2nt(vp + (size_t)(i + u) * ldv);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < UF; ++u) {
           s0 -= hs[i + u] * v[u].x;
           s1 -= hs[i + u] * v[u].y;
         }
@@ -1094,8 +1094,14 @@ hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, cons
 {
   if (k <= 0 || S.n_chunks <= 0) return hipSuccess;
   const int gx = S.act.counts ? chunk_grid(S) : (S.n_chunks + 7) / 8 * 8;  // XCD-numbered: a multiple of 8
-  hipLaunchKernelGGL((k_seg_mdot_partial<T>), dim3(gx, (k + KCH - 1) / KCH), dim3(RT), 0, s, S.chunks, S.n_chunks,
-                     S.act, k, V, ldv, w, partial, flag);
+#define GLH_MDOT(G)                                                                                              \
+  hipLaunchKernelGGL((k_seg_mdot_partial<T, G>), dim3(gx, (k + G - 1) / G), dim3(RT), 0, s, S.chunks, S.n_chunks, \
+                     S.act, k, V, ldv, w, partial, flag)
+  if (S.gs == 4) GLH_MDOT(4);
+  else if (S.gs == 8) GLH_MDOT(8);
+  else if (S.gs == 16) GLH_MDOT(16);
+  else return hipErrorInvalidValue;
+#undef GLH_MDOT
   hipLaunchKernelGGL((k_seg_final<T>), dim3(seg_grid(S), k), dim3(RT), 0, s, S.seg_chunk, S.n_seg, S.act, k, partial,
                      out, ldo, off, accumulate ? 1 : 0, flag);
   return hipGetLastError();
@@ -1106,8 +1112,14 @@ hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size
                                        T *partial, T *norm2, int ldo, int off, hipStream_t s, const int *flag)
 {
   if (S.n_chunks <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_seg_sub_lincomb_norm<T>), dim3(chunk_grid(S)), dim3(RT), 0, s, S.chunks, S.n_chunks, S.act, k,
-                     V, ldv, h, ldh, w, partial, flag);
+#define GLH_SLN(UF)                                                                                              \
+  hipLaunchKernelGGL((k_seg_sub_lincomb_norm<T, UF>), dim3(chunk_grid(S)), dim3(RT), 0, s, S.chunks, S.n_chunks, \
+                     S.act, k, V, ldv, h, ldh, w, partial, flag)
+  if (S.gs == 4) GLH_SLN(4);
+  else if (S.gs == 8) GLH_SLN(8);
+  else if (S.gs == 16) GLH_SLN(16);
+  else return hipErrorInvalidValue;
+#undef GLH_SLN
   hipLaunchKernelGGL((k_seg_final<T>), dim3(seg_grid(S), 1), dim3(RT), 0, s, S.seg_chunk, S.n_seg, S.act, 1, partial,
                      norm2, ldo, off, 0, flag);
   return hipGetLastError();

@@ -194,6 +194,7 @@ struct glh_ctx {
   // Arnoldi steps per replayed block, grid caps of the active-set launches,
   // restart used when glh_newton_solve is called with restart <= 0
   int q_block = 0, row_cap = 0, chunk_cap = 0, tuned_restart = 100;
+  int gs_tile = 8;  // Gram-Schmidt tile (glh::SegInfo::gs)
   const int *asm_active = nullptr;  // device flags: assemble only these instances (inside glh_newton_solve)
   DBuf<int> asm_flag;
   std::vector<double> r_ref;  // per-instance stopping reference (empty: ||R0|| of the solve)
@@ -215,7 +216,12 @@ struct glh_ctx {
                         cell_face.p,          node_mask.p,                  cell_blk.p,
                         constrained ? hdiag.p : nullptr, cell_mat.p, asm_active};
   }
-  glh::SegInfo seg() const { return glh::SegInfo{chunks.p, seg_chunk.p, n_chunks, n_seg}; }
+  glh::SegInfo seg() const
+  {
+    glh::SegInfo S{chunks.p, seg_chunk.p, n_chunks, n_seg};
+    S.gs = gs_tile;
+    return S;
+  }
 };
 
 namespace {
@@ -2132,23 +2138,25 @@ int glh_last_assembly_ms(glh_ctx *c, double ms[2])
   return GLH_OK;
 }
 
-int glh_set_schedule(glh_ctx *c, int restart, int q_block, int row_cap, int chunk_cap)
+int glh_set_schedule(glh_ctx *c, int restart, int q_block, int row_cap, int chunk_cap, int gs_tile)
 {
   if (!c || restart < 1 || restart > 126 || q_block < 0 || row_cap < 0 || chunk_cap < 0) return GLH_ERR_ARG;
+  if (gs_tile != 0 && gs_tile != 4 && gs_tile != 8 && gs_tile != 16) return GLH_ERR_ARG;
   c->tuned_restart = restart;
   c->q_block = q_block;
   c->row_cap = row_cap;
   c->chunk_cap = chunk_cap;
+  c->gs_tile = gs_tile ? gs_tile : 8;
   return GLH_OK;
 }
 
-int glh_autotune(glh_ctx *c, int fp32, int trial_newton, double lin_tol, int lin_maxit, double out[6])
+int glh_autotune(glh_ctx *c, int fp32, int trial_newton, double lin_tol, int lin_maxit, double out[7])
 {
   RCHK(check_ready(c));
   if (trial_newton < 1 || trial_newton > 8 || !(lin_tol > 0) || lin_maxit < 1 || !out)
     return fail(c, GLH_ERR_ARG, "bad autotune parameters");
   struct Cand {
-    int m, q, rc, cc;
+    int m, q, rc, cc, gs;
   };
   RCHK(glh_store_initial(c));
   const int old_verbose = c->verbose;
@@ -2159,6 +2167,7 @@ int glh_autotune(glh_ctx *c, int fp32, int trial_newton, double lin_tol, int lin
     c->q_block = k.q;
     c->row_cap = k.rc;
     c->chunk_cap = k.cc;
+    c->gs_tile = k.gs;
     double st[10];
     ++tried;
     if ((rc = glh_restore_initial(c)) != GLH_OK) return;
@@ -2172,37 +2181,51 @@ int glh_autotune(glh_ctx *c, int fp32, int trial_newton, double lin_tol, int lin
   // a trial of a few Newton steps does not predict its effect on the later,
   // harder linear systems of a solve (measured on 256 slabs: short trials
   // picked restart 30 or 60, and the full solve then took up to 2.7x as long
-  // as with 100).  The candidates are the
-  // launch schedule of a batch, which leaves the arithmetic as it is:
-  // Arnoldi steps per replayed graph block and grid caps.
+  // as with 100).  The candidates leave the arithmetic as it is: the launch
+  // schedule of a batch (Arnoldi steps per replayed graph block, grid caps),
+  // then, with the best schedule, the Gram-Schmidt tile (basis vectors per
+  // multi-dot workgroup and in flight per update thread; every tile sums in
+  // the same order, so the iterates are the same bits).
   const bool batch = c->n_seg > 1;
   const int m = c->tuned_restart;
-  Cand best{m, batch ? 8 : 0, batch ? 8192 : 0, batch ? 2048 : 0};
+  Cand best{m, batch ? 8 : 0, batch ? 8192 : 0, batch ? 2048 : 0, 8};
   double tbest = 1e300;
+  auto consider = [&](const Cand &k) {
+    double t = 1e300;
+    trial(k, lin_tol, t);
+    if (!rc && t < tbest) {
+      tbest = t;
+      best = k;
+    }
+  };
   if (batch) {
-    for (int q : {4, 8, 16})
+    for (int q : {4, 8, 16}) {
       for (int cap : {2048, 8192}) {
-        const Cand k{m, q, cap, cap / 4};
-        double t = 1e300;
-        trial(k, lin_tol, t);
+        consider(Cand{m, q, cap, cap / 4, 8});
         if (rc) break;
-        if (t < tbest) {
-          tbest = t;
-          best = k;
-        }
       }
+      if (rc) break;
+    }
   } else {
-    trial(best, lin_tol, tbest);
+    consider(best);
+  }
+  for (int gs : {4, 16}) {
+    if (rc) break;
+    Cand k = best;
+    k.gs = gs;
+    consider(k);
   }
   c->verbose = old_verbose;
   c->tuned_restart = best.m;
   c->q_block = best.q;
   c->row_cap = best.rc;
   c->chunk_cap = best.cc;
+  c->gs_tile = best.gs;
   RCHK(rc);
   RCHK(glh_restore_initial(c));
   out[0] = best.m; out[1] = best.q; out[2] = best.rc; out[3] = best.cc; out[4] = tbest;
   out[5] = (double)tried;
+  out[6] = best.gs;
   return GLH_OK;
 }
 

@@ -228,6 +228,10 @@ struct SegInfo {
   const int *seg_chunk;
   int n_chunks, n_seg;
   ActiveSet act;  // optional: only the listed instances' chunks are visited
+  // Gram-Schmidt tile (4, 8 or 16): basis vectors per multi-dot workgroup,
+  // and basis vectors in flight per thread of the update.  The sums run in
+  // the same order for every tile, so the results do not depend on it.
+  int gs = 8;
 };
 // flag (optional): instances whose flag is 0 are skipped (outputs untouched).
 template <typename T>

@@ -110,7 +110,7 @@ def lib():
     L.glh_prm_get.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
     L.glh_prm_get_double.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(C.c_double)]
     L.glh_run_prm.argtypes = [vp, vp, C.c_int, _dp, C.POINTER(C.c_int)]
-    L.glh_set_schedule.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.glh_set_schedule.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     L.glh_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, _dp]
     L.glh_set_residual_reference.argtypes = [vp, C.c_int, vp]
     L.glh_mlp_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]
@@ -692,18 +692,18 @@ class Context:
     def set_verbose(self, level=1):
         self._chk(lib().glh_set_verbose(self.h, int(level)), "glh_set_verbose")
 
-    def set_schedule(self, restart=100, q_block=0, row_cap=0, chunk_cap=0):
-        self._chk(lib().glh_set_schedule(self.h, int(restart), int(q_block), int(row_cap), int(chunk_cap)),
-                  "glh_set_schedule")
+    def set_schedule(self, restart=100, q_block=0, row_cap=0, chunk_cap=0, gs_tile=0):
+        self._chk(lib().glh_set_schedule(self.h, int(restart), int(q_block), int(row_cap), int(chunk_cap),
+                                         int(gs_tile)), "glh_set_schedule")
 
     def autotune(self, fp32=True, trial_newton=2, lin_tol=0.1, lin_maxit=3000):
-        """Time candidate schedules on the current problem and keep the fastest
-        (see glh_autotune); returns the choice as a dict."""
-        out = np.zeros(6)
+        """Time candidate schedules and kernel tiles on the current problem and
+        keep the fastest (see glh_autotune); returns the choice as a dict."""
+        out = np.zeros(7)
         self._chk(lib().glh_autotune(self.h, int(bool(fp32)), int(trial_newton), float(lin_tol), int(lin_maxit), out),
                   "glh_autotune")
         return dict(restart=int(out[0]), q_block=int(out[1]), row_cap=int(out[2]), chunk_cap=int(out[3]),
-                    trial_s=float(out[4]), candidates=int(out[5]))
+                    trial_s=float(out[4]), candidates=int(out[5]), gs_tile=int(out[6]))
 
     def set_forcing(self, eisenstat_walker):
         self._chk(lib().glh_set_forcing(self.h, int(bool(eisenstat_walker))), "glh_set_forcing")
