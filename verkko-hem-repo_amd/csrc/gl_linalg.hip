@@ -262,22 +262,48 @@ __global__ __launch_bounds__(64) void k_bj_invert(int n_rows, const int *__restr
     if (t == 0) status[0] = 1;
     return;
   }
-  TJ *D = Dinv + (size_t)r * 324;
-  for (int i = t; i < 324; i += 64) D[i] = (TJ)M[i / 18][18 + i % 18];
+  // the inverse of a symmetric block is symmetric: its upper triangle,
+  // row-major over c <= d, the two rounded halves averaged
+  TJ *D = Dinv + (size_t)r * 171;
+  for (int i = t; i < 171; i += 64) {
+    int c = 0, q = i;
+    while (q >= 18 - c) { q -= 18 - c; ++c; }
+    const int d = c + q;
+    D[i] = (TJ)(0.5 * (M[c][18 + d] + M[d][18 + c]));
+  }
 }
 
+// Point-block Jacobi y = D^-1 x with the symmetric inverses stored as their
+// upper triangles (171 values per node, row-major over c <= d): 53% of the
+// bytes of the full blocks.  One wavefront takes three nodes; lane l < 54
+// owns row c = l % 18 of node l / 18 and forms y_c = sum_d D[min][max] x_d,
+// walking the packed index (steps 17 - d while d < c, then 1) with ordinary
+// loads, so that the entries two lanes share come from the cache the second
+// time; x of the three nodes is shared through LDS.
 template <typename TJ, typename TX>
-__device__ inline void bj_row(int row, int lane, const TJ *__restrict__ Dinv, const TX *__restrict__ x,
-                              TX *__restrict__ y, double *red)
+__device__ inline void bj_rows(int row, bool valid, int lane, const TJ *__restrict__ Dinv, const TX *__restrict__ x,
+                               TX *__restrict__ y, double *sx)
 {
-  RowAcc acc;
-  if (lane < 54) {
-    double w0, w1;
-    pair_ld(x + (size_t)row * 18 + 2 * (lane % 9), w0, w1);
-    acc.add_block(Dinv + (size_t)row * 324, lane, w0, w1);
+  const int sub = lane / 18, c = lane % 18;
+  if (lane < 54) sx[lane] = valid ? (double)x[(size_t)row * 18 + c] : 0.0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < 54 && valid) {
+    const TJ *P = Dinv + (size_t)row * 171;
+    const double *xs = sx + sub * 18;
+    double acc = 0.0;
+    int off = c;
+#pragma unroll
+    for (int d = 0; d < 18; ++d) {
+      acc += (double)P[off] * xs[d];
+      off += d < c ? 17 - d : 1;
+    }
+    y[(size_t)row * 18 + c] = (TX)acc;
   }
-  const double t = acc.row_sum(lane, red);
-  if (lane < 18) y[(size_t)row * 18 + lane] = (TX)t;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <typename TJ, typename TX>
@@ -285,20 +311,21 @@ __global__ __launch_bounds__(256) void k_bj_apply(int n_rows, const TJ *__restri
                                                   TX *__restrict__ y, const int *__restrict__ row_seg,
                                                   const int *__restrict__ seg_flag, ActiveSet act)
 {
-  __shared__ double red[4][162];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ double sx[4][54];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, sub = lane < 54 ? lane / 18 : 0;
+  auto live = [&](int row) { return !seg_flag || seg_flag[row_seg[row]]; };
   if (!act.counts) {
-    const int row = blockIdx.x * 4 + wv;
-    if (row >= n_rows) return;
-    if (seg_flag && !seg_flag[row_seg[row]]) return;
-    bj_row(row, lane, Dinv, x, y, red[wv]);
+    const int row = (blockIdx.x * 4 + wv) * 3 + sub;
+    const bool valid = lane < 54 && row < n_rows && live(row);
+    bj_rows(row, valid, lane, Dinv, x, y, sx[wv]);
     return;
   }
   const int total = act.counts[2];
-  for (int gi = blockIdx.x * 4 + wv; gi < total; gi += gridDim.x * 4) {
-    const int row = active_row(act, gi);
-    if (seg_flag && !seg_flag[row_seg[row]]) continue;
-    bj_row(row, lane, Dinv, x, y, red[wv]);
+  for (int g = blockIdx.x * 4 + wv; 3 * g < total; g += gridDim.x * 4) {
+    const int gi = 3 * g + sub;
+    const bool in = lane < 54 && gi < total;
+    const int row = in ? active_row(act, gi) : 0;
+    bj_rows(row, in && live(row), lane, Dinv, x, y, sx[wv]);
   }
 }
 
@@ -957,7 +984,8 @@ hipError_t launch_bj_apply(int n_rows, const TJ *Dinv, const TX *x, TX *y, hipSt
 {
   if (n_rows <= 0) return hipSuccess;
   const int cap = act.row_cap > 0 ? act.row_cap : ACT_ROW_GRID;
-  const int grid = act.counts ? std::max(1, std::min((n_rows + 3) / 4, cap) >> act.shift) : (n_rows + 3) / 4;
+  const int g12 = (n_rows + 11) / 12;  // 4 wavefronts of 3 rows
+  const int grid = act.counts ? std::max(1, std::min(g12, cap) >> act.shift) : g12;
   hipLaunchKernelGGL((k_bj_apply<TJ, TX>), dim3(grid), dim3(256), 0, s, n_rows, Dinv, x, y, row_seg, seg_flag, act);
   return hipGetLastError();
 }

@@ -502,7 +502,7 @@ int ensure_precond(glh_ctx *c)
   if (c->d_valid) return GLH_OK;
   HIPCHK(c, c->status.alloc(1));
   HIPCHK(c, hipMemsetAsync(c->status.p, 0, sizeof(int), c->stream));
-  const size_t nv = (size_t)c->n_nodes * 324;
+  const size_t nv = (size_t)c->n_nodes * 171;  // the inverses' upper triangles
   if (c->j_fp32 && c->j_mf) {  // FP64 inverses of the FP64 diagonal blocks, rounded to FP32
     HIPCHK(c, c->D64.alloc(nv));
     HIPCHK(c, c->D32.alloc(nv));

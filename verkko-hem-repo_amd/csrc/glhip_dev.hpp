@@ -173,12 +173,13 @@ hipError_t launch_spmv(int n_rows, const int *row_ptr, const int *col, const TJ 
 template <typename TJ>
 hipError_t launch_residual_spmv(int n_rows, const int *row_ptr, const int *col, const TJ *val, const double *x,
                                 const double *b, double *r, hipStream_t s);
-// Dinv[r] = inverse of the diagonal block of row r (Gauss-Jordan with
-// partial pivoting in FP64); status[0] set to 1 if a block is singular.
+// Dinv[r] = inverse of the (symmetric) diagonal block of row r (Gauss-Jordan
+// with partial pivoting in FP64), stored as its upper triangle: 171 values
+// row-major over c <= d; status[0] set to 1 if a block is singular.
 template <typename TJ>
 hipError_t launch_bj_invert(int n_rows, const int *diag_blk, const TJ *val, TJ *Dinv, int *status,
                             hipStream_t s);
-// y = Dinv x, block-diagonal apply.
+// y = Dinv x, block-diagonal apply (Dinv as launch_bj_invert stores it).
 template <typename TJ, typename TX>
 hipError_t launch_bj_apply(int n_rows, const TJ *Dinv, const TX *x, TX *y, hipStream_t s,
                            const int *row_seg = nullptr, const int *seg_flag = nullptr,
