@@ -498,7 +498,8 @@ def main():
                     choices=sorted(WORKLOADS) + ["slab_batch", "adaptive_slab", "warm_start", "mlp_kernels"])
     ap.add_argument("--cycles", type=int, default=4, help="refinement cycles (adaptive_slab)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="slab_batch: fixed schedule (restart --restart, blocks of 8) instead of the autotuner")
+                    help="fixed schedule (restart --restart, blocks of 8) and Gram-Schmidt tile (8) instead of the "
+                         "autotuner's (slab_batch and the cube workloads)")
     ap.add_argument("--train-instances", type=int, default=4096, help="training solutions (warm_start)")
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--train-batch", type=int, default=65536)
@@ -586,6 +587,17 @@ def main():
     kw = dict(rtol=1e-10, max_newton=100, lin_tol=args.lin_tol, lin_maxit=100000, restart=args.restart)
     mf = (args.operator == "matrix-free")
     ctx.set_matrix_free(mf)
+    tuned = None
+    if args.autotune:
+        # Gram-Schmidt tile picked by the library's autotuner (untimed; every
+        # tile gives the same iterates to the bit)
+        t0 = time.perf_counter()
+        ctx.restore_initial()
+        # (six Newton steps per trial: the first steps' short Krylov bases
+        # alone would not tell the tiles apart)
+        tuned = ctx.autotune(fp32=False, trial_newton=6, lin_tol=args.lin_tol, lin_maxit=100000)
+        tuned["tuning_s"] = time.perf_counter() - t0
+        print("bench: autotune %s" % tuned, file=sys.stderr, flush=True)
     for w in range(args.warmup):
         ctx.restore_initial()
         st = ctx.newton_solve(**kw)
@@ -664,6 +676,7 @@ def main():
         "jacobian_setup_ms": jasm_ms,
         "jx_ms": apply_avg_ms,
         "jx_products": tm["spmv_n"],
+        "autotune": tuned,
         "roofline": roof,
     }
     if mf and not args.no_assembled:

@@ -438,7 +438,8 @@ int glh_set_schedule(glh_ctx *ctx, int restart, int q_block, int row_cap, int ch
  * current solution, for a batch over blocks of 4 / 8 / 16 Arnoldi steps per
  * replayed graph and grid caps 2048 / 8192 block rows, then with the best of
  * those over Gram-Schmidt tiles 4 / 8 / 16; keeps the fastest in the context
- * and restores the solution.  These knobs change launch and tile shapes
+ * (a later candidate has to be faster by more than 2%) and restores the
+ * solution.  These knobs change launch and tile shapes
  * only: the iterates are the same bits.  The restart is the context's
  * (glh_set_schedule, default 100): it changes the iteration, which short
  * trials do not predict.  out: [0] restart [1] block [2] row cap [3] chunk

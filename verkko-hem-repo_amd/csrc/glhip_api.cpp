@@ -2190,10 +2190,12 @@ int glh_autotune(glh_ctx *c, int fp32, int trial_newton, double lin_tol, int lin
   const int m = c->tuned_restart;
   Cand best{m, batch ? 8 : 0, batch ? 8192 : 0, batch ? 2048 : 0, 8};
   double tbest = 1e300;
+  // a candidate replaces the best so far only when it is faster by more than
+  // 2% (shorter differences are within the run-to-run spread of a trial)
   auto consider = [&](const Cand &k) {
     double t = 1e300;
     trial(k, lin_tol, t);
-    if (!rc && t < tbest) {
+    if (!rc && t < 0.98 * tbest) {
       tbest = t;
       best = k;
     }
