@@ -497,6 +497,8 @@ def main():
     ap.add_argument("--workload", default="cube_q2_ref5",
                     choices=sorted(WORKLOADS) + ["slab_batch", "adaptive_slab", "warm_start", "mlp_kernels"])
     ap.add_argument("--cycles", type=int, default=4, help="refinement cycles (adaptive_slab)")
+    ap.add_argument("--gs-tile", type=int, default=0, choices=[0, 4, 8, 16],
+                    help="cube workloads with --no-autotune: Gram-Schmidt tile (0: the library's default, 8)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="fixed schedule (restart --restart, blocks of 8) and Gram-Schmidt tile (8) instead of the "
                          "autotuner's (slab_batch and the cube workloads)")
@@ -588,6 +590,8 @@ def main():
     mf = (args.operator == "matrix-free")
     ctx.set_matrix_free(mf)
     tuned = None
+    if not args.autotune and args.gs_tile:
+        ctx.set_schedule(restart=args.restart, gs_tile=args.gs_tile)
     if args.autotune:
         # Gram-Schmidt tile picked by the library's autotuner (untimed; every
         # tile gives the same iterates to the bit)

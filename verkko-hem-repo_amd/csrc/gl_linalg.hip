@@ -332,6 +332,10 @@ __global__ __launch_bounds__(256) void k_bj_apply(int n_rows, const TJ *__restri
 // -------------------------------------------------------------- reductions
 constexpr int RT = 256;
 constexpr int KCH = 8;  // vectors per multi-dot pass (4 measured the same, 16 slower)
+// pairs of values per thread of the Gram-Schmidt update: a chunk holds at
+// most GS_MAX_CHUNK values
+constexpr int GS_PAIRS = GS_MAX_CHUNK / (2 * RT);
+static_assert(GS_PAIRS * 2 * RT == GS_MAX_CHUNK, "chunk bound in whole pairs per thread");
 
 // Sum over the 64 lanes of a wave (butterfly: every lane gets the same,
 // order-fixed result).
@@ -665,7 +669,14 @@ __global__ __launch_bounds__(RT) void k_seg_sub_lincomb_norm(const long long *__
     if (flag && !flag[sg]) continue;
     const T *hs = h + sg * ldh;
     T acc = 0;
-    for (long long idx = beg + 2 * threadIdx.x; idx < end; idx += 2 * RT) {
+    // the updated pairs are stored once the whole chunk is done: a store
+    // between the loads of the next pair holds them back (loads and stores
+    // share one counter), and the stores then go out in one burst
+    typename NV2<T>::type out[GS_PAIRS];
+#pragma unroll
+    for (int it = 0; it < GS_PAIRS; ++it) {
+      const long long idx = beg + 2 * threadIdx.x + (long long)it * 2 * RT;
+      if (idx >= end) break;
       const auto w2 = ld2(w + idx);
       T s0 = w2.x, s1 = w2.y;
       const T *vp = V + idx;
@@ -685,12 +696,15 @@ __global__ __launch_bounds__(RT) void k_seg_sub_lincomb_norm(const long long *__
         s0 -= hs[i] * v.x;
         s1 -= hs[i] * v.y;
       }
-      typename NV2<T>::type o;
-      o.x = s0;
-      o.y = s1;
-      *reinterpret_cast<typename NV2<T>::type *>(w + idx) = o;
+      out[it].x = s0;
+      out[it].y = s1;
       acc += s0 * s0;
       acc += s1 * s1;
+    }
+#pragma unroll
+    for (int it = 0; it < GS_PAIRS; ++it) {
+      const long long idx = beg + 2 * threadIdx.x + (long long)it * 2 * RT;
+      if (idx < end) *reinterpret_cast<typename NV2<T>::type *>(w + idx) = out[it];
     }
     const T r = block_reduce(acc, sh);
     if (threadIdx.x == 0) partial[cb] = r;

@@ -90,6 +90,7 @@ struct HBuf {
 constexpr long long CHUNK = 1024, CHUNK_LARGE = 4096, LARGE_INSTANCE = 1 << 20;
 // the Gram-Schmidt kernels read value pairs: chunk bounds must stay even
 static_assert(CHUNK % 2 == 0 && CHUNK_LARGE % 2 == 0 && NC % 2 == 0, "reduction chunks must have even bounds");
+static_assert(CHUNK <= glh::GS_MAX_CHUNK && CHUNK_LARGE <= glh::GS_MAX_CHUNK, "chunks the segmented kernels accept");
 
 }  // namespace
 

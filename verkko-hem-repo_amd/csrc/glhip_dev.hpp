@@ -224,6 +224,8 @@ hipError_t launch_mask_zero(int n_nodes, const uint32_t *mask, double *x, hipStr
 // Instances own contiguous dof ranges.  chunks: [3*n_chunks] (instance, first
 // dof, end dof), no chunk straddles an instance; seg_chunk: [n_seg+1] chunk
 // ranges per instance.  Outputs are indexed out[s*ldo + off + i].
+// longest reduction chunk (values) the segmented kernels accept
+constexpr long long GS_MAX_CHUNK = 4096;
 struct SegInfo {
   const long long *chunks;
   const int *seg_chunk;
