@@ -561,30 +561,13 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
   // waves by its registers anyway, alternates between two sets and keeps x
   // (35 KB), which saves a barrier and the Robin faces' second gather of x.
   constexpr bool IP = std::is_same<TA, float>::value;
-  __shared__ TA S[(IP ? 4 : 9) * V];
+  constexpr int NXR = (V + NTA - 1) / NTA;         // x values per thread
+  __shared__ TA S[(IP ? 4 : 9) * V + NXR * NTA - V];  // room for the unconditional x stores
   TA *const P0 = S, *const P1 = IP ? S : S + 4 * V, *const SX = IP ? S : S + 8 * V;
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
   if (e < 0) return;                           // padding of the XCD-aware order
   if (flag && !flag[Mh.cell_inst[e]]) return;  // instance not iterating: its rows are not read
-  // the two Hessian rows of this thread's H X item (point hq, components hc
-  // and hc + 9), requested first: they are in flight while the cell gathers x
-  // and interpolates it
-  const int hq = tid / 9, hc = tid % 9;
-  TH hv[2][18];
-  if (tid < NH) {
-    const TH *Hrow = Hq + ((size_t)e * NQ + hq) * NUPAIR;
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const int cr = hc + 9 * r;
-      int off = cr;  // upair(cr, d): steps 17 - d while d < cr, then 1
-#pragma unroll
-      for (int d = 0; d < 18; ++d) {
-        hv[r][d] = Hrow[off];
-        off += d < cr ? 17 - d : 1;
-      }
-    }
-  }
   const Material M = Mh.mats ? Mh.mats[Mh.cell_mat[e]] : M0;
   const int *cn = Mh.cell_nodes + (size_t)e * NS;
   double h[3];
@@ -603,12 +586,43 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
       L[q][b] = tL[q * N1 + b];
       dL[q][b] = tdL[q * N1 + b];
     }
-  // x at the cell's nodes, Dirichlet components 0 (the masked columns)
+  // A wave's vector loads may return out of order, so a wait for one is a
+  // wait for all outstanding.  x at the cell's nodes is therefore gathered
+  // and stored to LDS first, and the Hessian rows are requested after that
+  // wait: they stay in flight through the interpolation passes, which wait
+  // only on LDS, until H X.  Both without branches (indices clamped into
+  // range, unused values dropped by selects or stored to padding).
+  // x, Dirichlet components 0 (the masked columns):
   auto xval = [&](int i) -> TA {
     const int node = cn[i / 18], c = i % 18;
-    return ((Mh.node_mask[node] >> c) & 1u) ? (TA)0 : (TA)x[(size_t)node * 18 + c];
+    const TA xv = (TA)x[(size_t)node * 18 + c];
+    return ((Mh.node_mask[node] >> c) & 1u) ? (TA)0 : xv;
   };
-  for (int i = tid; i < V; i += NTA) SX[i] = xval(i);
+#pragma unroll
+  for (int r = 0; r < NXR; ++r) {
+    const int i = tid + r * NTA;
+    const TA v = xval(i < V ? i : V - 1);
+    SX[i] = i < V ? v : (TA)0;  // beyond V: the padding, never read
+  }
+  // keep the Hessian loads below the x stores (and so below their wait)
+  __builtin_amdgcn_sched_barrier(0);
+  // the two Hessian rows of this thread's H X item (point hq, components hc
+  // and hc + 9; threads beyond the items read the last point's, unused)
+  const int hq = tid / 9 < NQ ? tid / 9 : NQ - 1, hc = tid % 9;
+  TH hv[2][18];
+  {
+    const TH *Hrow = Hq + ((size_t)e * NQ + hq) * NUPAIR;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int cr = hc + 9 * r;
+      int off = cr;  // upair(cr, d): steps 17 - d while d < cr, then 1
+#pragma unroll
+      for (int d = 0; d < 18; ++d) {
+        hv[r][d] = Hrow[off];
+        off += d < cr ? 17 - d : 1;
+      }
+    }
+  }
   __syncthreads();
   auto at = [](int i0, int i1, int i2, int c) { return ((i2 * N1 + i1) * N1 + i0) * 18 + c; };
   const int c = tid % 18, l = tid / 18, la = l % N1, lb = l / N1;  // pencil: line (la, lb), component c
