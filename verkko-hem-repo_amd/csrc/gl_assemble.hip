@@ -1,5 +1,6 @@
-// Cell-loop assembly of the Ginzburg-Landau Jacobian J and system_rhs = -R
-// for FE_Q(k) x 18 on axis-aligned hexes, hand-written for gfx950.
+// Cell-loop assembly of the Ginzburg-Landau Jacobian J, its matrix-free form,
+// and system_rhs = -R for FE_Q(k) x 18 on axis-aligned hexes, hand-written
+// for gfx950.
 //
 // Reference: femgl/src/assemble.cc:177-280 (cell loop), residual.cc:152-232,
 // the term functions of femgl/src/cell_mat_vec/, and the Robin face loops
@@ -101,30 +102,50 @@ __device__ inline double hess_entry(const double *__restrict__ G, const double *
          2.0 * M.b5 * (dmn * Avu_lk + vml * unk + dmn * Auv_lk - uml * vnk + W * dkl);
 }
 
-// rho_c = alpha x_c + 2 sum_n beta_n r_n(E_c): bulk residual density
-__device__ inline double rho_entry(const double *__restrict__ G, const double *__restrict__ x, int c,
-                                   const Material &M)
+// rho_c = alpha x_c + 2 sum_n beta_n r_n(E_c), the bulk residual density, for
+// the components (mu, k) of u (ru) and of v (rv) at once, from the 18 values
+// x of one point.  The Gram entries it needs (|u|^2, |v|^2, u:v and row mu of
+// u u^T, v v^T, u v^T, v u^T) are summed in the order of gram_entry; x sits in
+// registers, and its entries at run-time indices are read again from memory
+// (LDS) so that no register array is indexed at run time.
+template <typename TA>
+__device__ inline void rho_pair(const TA *x, int mu, int k, const Material &M, double &ru, double &rv)
 {
-  const double *u = x, *v = x + 9;
-  const int part = c / 9, a = c % 9, mu = a / 3, k = a % 3;
-  const double U2 = G[54], V2 = G[55], UV = G[56], D = U2 - V2, N = U2 + V2;
-  const double *uut = G + 27, *vvt = G + 36, *uvt = G + 45;
+  double u[9], v[9];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) { u[a] = (double)x[a]; v[a] = (double)x[9 + a]; }
+  double U2 = 0.0, V2 = 0.0, UV = 0.0;
+#pragma unroll
+  for (int a = 0; a < 9; ++a) U2 += u[a] * u[a];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) V2 += v[a] * v[a];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) UV += u[a] * v[a];
+  const double D = U2 - V2, N = U2 + V2;
+  double um[3], vm[3];  // row mu of u and of v
+#pragma unroll
+  for (int j = 0; j < 3; ++j) { um[j] = (double)x[3 * mu + j]; vm[j] = (double)x[9 + 3 * mu + j]; }
   double Pu = 0, Pv = 0, Qu = 0, Qv = 0, Su = 0, Sv = 0, Wu = 0, Wv = 0;
+#pragma unroll
   for (int nu = 0; nu < 3; ++nu) {
-    const double P = uut[3 * mu + nu] - vvt[3 * mu + nu];
-    const double S = uut[3 * mu + nu] + vvt[3 * mu + nu];
-    const double Q = uvt[3 * mu + nu] + uvt[3 * nu + mu];
-    const double W = uvt[3 * nu + mu] - uvt[3 * mu + nu];
-    const double un = u[3 * nu + k], vn = v[3 * nu + k];
+    double uu = 0.0, vv = 0.0, uvmn = 0.0, uvnm = 0.0;  // uut, vvt, uvt at (mu, nu); uvt at (nu, mu)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      uu += um[j] * u[3 * nu + j];
+      vv += vm[j] * v[3 * nu + j];
+      uvmn += um[j] * v[3 * nu + j];
+      uvnm += u[3 * nu + j] * vm[j];
+    }
+    const double P = uu - vv, S = uu + vv, Q = uvmn + uvnm, W = uvnm - uvmn;
+    const double un = (double)x[3 * nu + k], vn = (double)x[9 + 3 * nu + k];
     Pu += P * un; Pv += P * vn; Qu += Q * un; Qv += Q * vn;
     Su += S * un; Sv += S * vn; Wu += W * un; Wv += W * vn;
   }
-  if (part == 0) {
-    const double g1 = D * u[a] + 2.0 * UV * v[a], g2 = N * u[a], g3 = Pu + Qv, g4 = Su - Wv, g5 = Su + Wv;
-    return M.al * u[a] + 2.0 * (M.b1 * g1 + M.b2 * g2 + M.b3 * g3 + M.b4 * g4 + M.b5 * g5);
-  }
-  const double g1 = -D * v[a] + 2.0 * UV * u[a], g2 = N * v[a], g3 = Qu - Pv, g4 = Sv + Wu, g5 = Sv - Wu;
-  return M.al * v[a] + 2.0 * (M.b1 * g1 + M.b2 * g2 + M.b3 * g3 + M.b4 * g4 + M.b5 * g5);
+  const double ua = (double)x[3 * mu + k], va = (double)x[9 + 3 * mu + k];
+  ru = M.al * ua + 2.0 * (M.b1 * (D * ua + 2.0 * UV * va) + M.b2 * (N * ua) + M.b3 * (Pu + Qv) + M.b4 * (Su - Wv) +
+                          M.b5 * (Su + Wv));
+  rv = M.al * va + 2.0 * (M.b1 * (-D * va + 2.0 * UV * ua) + M.b2 * (N * va) + M.b3 * (Qu - Pv) + M.b4 * (Sv + Wu) +
+                          M.b5 * (Sv - Wu));
 }
 
 __device__ inline bool face_zeroed(int bid, int c) { return (c % 9) % 3 == bid - 2; }
@@ -138,26 +159,22 @@ template <typename T> struct Vec2;
 template <> struct Vec2<double> { using type = double2; };
 template <> struct Vec2<float> { using type = float2; };
 
-template <int DEG, typename TJ, bool DO_J, bool DO_R, bool HANG = false>
+template <int DEG, typename TJ, bool HANG = false>
 __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ U,
-                                                 const int *__restrict__ cells, TJ *__restrict__ J,
-                                                 double *__restrict__ rhs)
+                                                 const int *__restrict__ cells, TJ *__restrict__ J)
 {
   constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS, NPAIR = NS * (NS + 1) / 2;
-  constexpr int NBUF = DO_R ? NQ * 72 : 1;
   __shared__ double sU[NS * 18];
   __shared__ double sX[NQ * 18];
-  __shared__ double sG[DO_R ? NQ * 54 : 1];
   __shared__ double sGram[NQ * GRAM];
-  __shared__ double sBuf[NBUF];
   // per-cell scatter data staged once: block codes and node constraint masks
-  __shared__ int sBlk[DO_J ? NS * NS : 1];
+  __shared__ int sBlk[NS * NS];
   // node-pair weights of the current chunk of pairs (the whole table does not
   // fit the scalar cache)
   constexpr int PCH = 32;
-  __shared__ double sWp[DO_J ? PCH * NQ : 1];
-  __shared__ double sSp[DO_J ? PCH * 9 : 1];
-  __shared__ uint32_t sMask[DO_J ? NS : 1];
+  __shared__ double sWp[PCH * NQ];
+  __shared__ double sSp[PCH * 9];
+  __shared__ uint32_t sMask[NS];
 
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
@@ -171,31 +188,18 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
 
   // gather the cell's 18 components at its NS nodes
   for (int i = tid; i < NS * 18; i += NT) sU[i] = U[(size_t)cn[i / 18] * 18 + (i % 18)];
-  if (DO_J) {
+  {
     const int *cb = Mh.cell_blk + (size_t)e * NS * NS;
     for (int i = tid; i < NS * NS; i += NT) sBlk[i] = cb[i];
     for (int i = tid; i < NS; i += NT) sMask[i] = Mh.node_mask[cn[i]];
   }
   __syncthreads();
-  // iterate and gradient at the quadrature points (s_vector2matrix.cc:116-217)
+  // the iterate at the quadrature points (s_vector2matrix.cc:116-217)
   for (int i = tid; i < NQ * 18; i += NT) {
     const int q = i / 18, c = i % 18;
-    double xv = 0, g0 = 0, g1 = 0, g2 = 0;
-    for (int a = 0; a < NS; ++a) {
-      const double Ua = sU[a * 18 + c];
-      xv += T.N[q * NS + a] * Ua;
-      if (DO_R) {
-        g0 += T.dN[(q * NS + a) * 3 + 0] * Ua;
-        g1 += T.dN[(q * NS + a) * 3 + 1] * Ua;
-        g2 += T.dN[(q * NS + a) * 3 + 2] * Ua;
-      }
-    }
+    double xv = 0;
+    for (int a = 0; a < NS; ++a) xv += T.N[q * NS + a] * sU[a * 18 + c];
     sX[i] = xv;
-    if (DO_R) {
-      sG[q * 54 + c] = g0 * ih[0];
-      sG[q * 54 + 18 + c] = g1 * ih[1];
-      sG[q * 54 + 36 + c] = g2 * ih[2];
-    }
   }
   __syncthreads();
   for (int i = tid; i < NQ * GRAM; i += NT) {
@@ -204,44 +208,7 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
   }
   __syncthreads();
 
-  if (DO_R) {
-    // rho_c(q) and gamma_{c,k}(q) = K1 d_k x_c + (K2+K3) [k == orbital(c)] div_part(mu)
-    for (int i = tid; i < NQ * 18; i += NT) {
-      const int q = i / 18, c = i % 18;
-      sBuf[q * 72 + c] = rho_entry(sGram + q * GRAM, sX + q * 18, c, M);
-      const int part = c / 9, mu = (c % 9) / 3, kc = c % 3;
-      const double *g = sG + q * 54;
-      const double dv = g[9 * part + 3 * mu + 0] + g[18 + 9 * part + 3 * mu + 1] + g[36 + 9 * part + 3 * mu + 2];
-      for (int k = 0; k < 3; ++k)
-        sBuf[q * 72 + 18 + k * 18 + c] = M.K1 * g[k * 18 + c] + (k == kc ? M.K23 * dv : 0.0);
-    }
-    __syncthreads();
-    for (int i = tid; i < NS * 18; i += NT) {
-      const int a = i / 18, c = i % 18;
-      double r = 0.0;
-      for (int q = 0; q < NQ; ++q) {
-        const double *b = sBuf + q * 72;
-        const double *d = T.dN + (q * NS + a) * 3;
-        r += T.w[q] * (T.N[q * NS + a] * b[c] + d[0] * ih[0] * b[18 + c] + d[1] * ih[1] * b[36 + c] +
-                       d[2] * ih[2] * b[54 + c]);
-      }
-      r *= vol;
-      if (M.robin != 0.0) {
-        for (int lf = 0; lf < 6; ++lf) {
-          const int bid = (int)((cface >> (3 * lf)) & 7u) - 1;
-          if (bid < 2 || bid > 4 || face_zeroed(bid, c)) continue;
-          const double *mf = T.Mf + (lf * NS + a) * NS;
-          double s = 0.0;
-          for (int b = 0; b < NS; ++b) s += mf[b] * sU[b * 18 + c];
-          r += M.robin * face_area(lf, h) * s;
-        }
-      }
-      const int node = cn[a];
-      if (!((Mh.node_mask[node] >> c) & 1u)) rhs[(size_t)node * 18 + c] -= r;
-    }
-  }
-
-  if (DO_J) {
+  {
     // Thread t < 171 owns the unique component pair (c <= d).  For every node
     // pair (a <= b) it forms block(a,b)[c][d], block(a,b)[d][c] and writes them,
     // with the transposed block(b,a), straight to the BSR values (no staging
@@ -540,7 +507,11 @@ using mf_arith = typename std::conditional<std::is_same<TH, float>::value && std
 template <int DEG, typename TH, typename TX>
 constexpr int mf_waves() { return std::is_same<mf_arith<TH, TX>, float>::value ? (DEG == 1 ? 5 : 6) : 4; }
 
-template <int DEG, typename TH, typename TX>
+// RES: the residual R(x) of the cell loop (residual.cc:164-232, the Robin
+// faces :235-278) through the same passes: x is the iterate (no column
+// masking), and rho(X(q)), the bulk residual density, takes the place of
+// H(q) X(q).  Hq is not read then; flag is the batch's instance_active.
+template <int DEG, typename TH, typename TX, bool RES = false>
 __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>())) void k_mf_apply(RefTables T, MeshDev Mh, Material M0,
                                                                       const TH *__restrict__ Hq,
                                                                       const TX *__restrict__ x,
@@ -548,6 +519,7 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
                                                                       const int *__restrict__ flag,
                                                                       mf_arith<TH, TX> *__restrict__ ycell)
 {
+  static_assert(!RES || (std::is_same<TH, double>::value && std::is_same<TX, double>::value), "FP64 residual");
   using TA = mf_arith<TH, TX>;
   constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS, V = NS * 18, NTA = mf_threads<DEG, TH>();
   constexpr int NP = N1 * N1 * 18;  // pencils per pass
@@ -592,10 +564,11 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
   // wait: they stay in flight through the interpolation passes, which wait
   // only on LDS, until H X.  Both without branches (indices clamped into
   // range, unused values dropped by selects or stored to padding).
-  // x, Dirichlet components 0 (the masked columns):
+  // x, Dirichlet components 0 (the masked columns; the residual's iterate as it is):
   auto xval = [&](int i) -> TA {
     const int node = cn[i / 18], c = i % 18;
     const TA xv = (TA)x[(size_t)node * 18 + c];
+    if constexpr (RES) return xv;
     return ((Mh.node_mask[node] >> c) & 1u) ? (TA)0 : xv;
   };
 #pragma unroll
@@ -610,7 +583,7 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
   // and hc + 9; threads beyond the items read the last point's, unused)
   const int hq = tid / 9 < NQ ? tid / 9 : NQ - 1, hc = tid % 9;
   TH hv[2][18];
-  {
+  if constexpr (!RES) {
     const TH *Hrow = Hq + ((size_t)e * NQ + hq) * NUPAIR;
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
@@ -701,11 +674,18 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
   if (tid < NH) {
     const TA *X = &P1[hq * 18];
     TA z[2] = {0, 0};
+    if constexpr (RES) {
+      double ru, rv;
+      rho_pair(X, hc / 3, hc % 3, M, ru, rv);
+      z[0] = ru;
+      z[1] = rv;
+    } else {
 #pragma unroll
-    for (int d = 0; d < 18; ++d) {
-      const TA xd = X[d];  // one read of X(q) for both rows
-      z[0] += (TA)hv[0][d] * xd;
-      z[1] += (TA)hv[1][d] * xd;
+      for (int d = 0; d < 18; ++d) {
+        const TA xd = X[d];  // one read of X(q) for both rows
+        z[0] += (TA)hv[0][d] * xd;
+        z[1] += (TA)hv[1][d] * xd;
+      }
     }
     const TA wv = (TA)(T.w[hq] * vol), K1 = (TA)M.K1, K23 = (TA)M.K23;
     const int mu = hc / 3, kc = hc % 3;
@@ -843,6 +823,32 @@ __global__ void k_mf_gather(int n_nodes, const int *__restrict__ ptr, const int 
   y[o + 1] = b ? (TX)((double)b[o + 1] - r1) : (TX)r1;
 }
 
+// system_rhs = -R from the residual's cell contributions: each node's sum in
+// the fixed cell order, negated; 0 on the Dirichlet components and on the
+// rows of a batch instance that is not active (whose cells were skipped).
+// Every row is written, so the output needs no fill.
+__global__ void k_res_gather(int n_nodes, const int *__restrict__ ptr, const int *__restrict__ idx,
+                             const double *__restrict__ ycell, const uint32_t *__restrict__ mask,
+                             const int *__restrict__ row_seg, const int *__restrict__ active,
+                             double *__restrict__ rhs)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)n_nodes * 9) return;
+  const int node = (int)(i / 9), c = 2 * (int)(i % 9);
+  const bool on = !active || active[row_seg[node]];
+  double s0 = 0.0, s1 = 0.0;
+  if (on)
+    for (int k = ptr[node]; k < ptr[node + 1]; ++k) {
+      const double2 v = *reinterpret_cast<const double2 *>(ycell + (size_t)idx[k] * 18 + c);
+      s0 += v.x;
+      s1 += v.y;
+    }
+  const uint32_t mk = mask[node];
+  const size_t o = (size_t)node * 18 + c;
+  rhs[o] = (on && !((mk >> c) & 1u)) ? -s0 : 0.0;
+  rhs[o + 1] = (on && !((mk >> (c + 1)) & 1u)) ? -s1 : 0.0;
+}
+
 template <int DEG>
 __global__ __launch_bounds__(NT) void k_energy(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ U,
                                                int n_cells, double *__restrict__ Fcell)
@@ -944,31 +950,20 @@ __global__ __launch_bounds__(NT) void k_energy(RefTables T, MeshDev Mh, Material
 
 }  // namespace
 
-hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefTables &T, const MeshDev &M,
-                           const Material &mat, const double *U, const int *cells, int n, void *J, double *rhs,
-                           hipStream_t s)
+hipError_t launch_assemble(int deg, bool jfp32, const RefTables &T, const MeshDev &M, const Material &mat,
+                           const double *U, const int *cells, int n, void *J, hipStream_t s)
 {
   if (n <= 0) return hipSuccess;
   dim3 g(n), b(NT);
-#define GLH_ASM(D, TJ, DJ, DR, HG) \
-  hipLaunchKernelGGL((k_assemble<D, TJ, DJ, DR, HG>), g, b, 0, s, T, M, mat, U, cells, (TJ *)J, rhs)
-  // J and R are launched as two kernels: the Jacobian-only kernel needs fewer
-  // registers (3 waves/SIMD instead of 2 for Q2), which hides more of the
-  // latency of its scattered read-modify-write stores.  The hanging-node
-  // variant (adaptive meshes) also gathers the constrained diagonals.
+#define GLH_ASM(D, TJ, HG) hipLaunchKernelGGL((k_assemble<D, TJ, HG>), g, b, 0, s, T, M, mat, U, cells, (TJ *)J)
+  // the hanging-node variant (adaptive meshes) also gathers the constrained diagonals
   const bool hang = M.hdiag != nullptr;
   if (deg == 1) {
-    if (do_j) {
-      if (hang) { if (jfp32) GLH_ASM(1, float, true, false, true); else GLH_ASM(1, double, true, false, true); }
-      else { if (jfp32) GLH_ASM(1, float, true, false, false); else GLH_ASM(1, double, true, false, false); }
-    }
-    if (do_r) GLH_ASM(1, double, false, true, false);
+    if (hang) { if (jfp32) GLH_ASM(1, float, true); else GLH_ASM(1, double, true); }
+    else { if (jfp32) GLH_ASM(1, float, false); else GLH_ASM(1, double, false); }
   } else {
-    if (do_j) {
-      if (hang) { if (jfp32) GLH_ASM(2, float, true, false, true); else GLH_ASM(2, double, true, false, true); }
-      else { if (jfp32) GLH_ASM(2, float, true, false, false); else GLH_ASM(2, double, true, false, false); }
-    }
-    if (do_r) GLH_ASM(2, double, false, true, false);
+    if (hang) { if (jfp32) GLH_ASM(2, float, true); else GLH_ASM(2, double, true); }
+    else { if (jfp32) GLH_ASM(2, float, false); else GLH_ASM(2, double, false); }
   }
 #undef GLH_ASM
   return hipGetLastError();
@@ -1008,6 +1003,29 @@ hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, cons
     else GLH_MFA(2, double, double);
   }
 #undef GLH_MFA
+  return hipGetLastError();
+}
+
+hipError_t launch_res_apply(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
+                            const int *cells, int n, double *ycell, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+#define GLH_RES(D)                                                                                               \
+  hipLaunchKernelGGL((k_mf_apply<D, double, double, true>), dim3(n), dim3((mf_threads<D, double>())), 0, s, T, M, \
+                     mat, (const double *)nullptr, U, cells, M.inst_active, ycell)
+  if (deg == 1) GLH_RES(1);
+  else GLH_RES(2);
+#undef GLH_RES
+  return hipGetLastError();
+}
+
+hipError_t launch_res_gather(int n_nodes, const int *ptr, const int *idx, const double *ycell, const uint32_t *mask,
+                             const int *row_seg, const int *active, double *rhs, hipStream_t s)
+{
+  if (n_nodes <= 0) return hipSuccess;
+  const long long n = (long long)n_nodes * 9;
+  hipLaunchKernelGGL(k_res_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_nodes, ptr, idx, ycell, mask,
+                     row_seg, active, rhs);
   return hipGetLastError();
 }
 

@@ -408,6 +408,21 @@ int norm64(glh_ctx *c, const double *v, double *out)
   return GLH_OK;
 }
 
+// system_rhs = -R at U into out, condensed on a constrained mesh: the cell
+// loop of compute_residual (residual.cc:164-289) in the cell slots, summed
+// per node in a fixed order (no colours, no fill: every row is written)
+int residual_cells(glh_ctx *c, const double *U, double *out)
+{
+  const glh::MeshDev M = c->meshdev();
+  double *rcell = c->constrained ? c->rfull.p : out;
+  HIPCHK(c, glh::launch_res_apply(c->deg, c->tables(), M, c->mat, U, c->mf_order.p, c->mf_order_n, c->mf_ycell.p,
+                                  c->stream));
+  HIPCHK(c, glh::launch_res_gather(c->n_nodes, c->n2c_ptr.p, c->n2c_idx.p, c->mf_ycell.p, c->node_mask.p,
+                                   c->row_seg.p, M.inst_active, rcell, c->stream));
+  if (c->constrained) HIPCHK(c, glh::launch_condense_rhs(c->cons, c->rfull.p, out, c->stream));
+  return GLH_OK;
+}
+
 // assemble (J and/or -R) at state Ustate into (J, rhs_out)
 int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp32, double *rhs_out, bool timed,
                 bool mf = false)
@@ -432,9 +447,8 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
     if (fp32) { HIPCHK(c, c->J32.alloc(nv)); J = c->J32.p; }
     else { HIPCHK(c, c->J64.alloc(nv)); J = c->J64.p; }
   }
-  // with hanging nodes the cell loop writes Jf / rf, condensed afterwards
+  // with hanging nodes the cell loop writes Jf, condensed afterwards
   void *Jcell = J;
-  double *rcell = rhs_out;
   if (c->constrained) {
     if (do_j && !mf) {
       const size_t nv = (size_t)c->nnzb_a * 324;
@@ -442,23 +456,21 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
       else { HIPCHK(c, c->Jf64.alloc(nv)); Jcell = c->Jf64.p; }
     }
     if (do_j) HIPCHK(c, glh::launch_fill<double>(ndofs(c), 0.0, c->hdiag.p, c->stream));
-    if (do_r) rcell = c->rfull.p;
   }
-  if (do_r) HIPCHK(c, hipMemsetAsync(rcell, 0, ndofs(c) * sizeof(double), c->stream));
   const glh::RefTables T = c->tables();
   const glh::MeshDev M = c->meshdev();
   timed = timed || c->timing;
   if (timed) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  const int ncol = (int)c->color_ptr.size() - 1;
+  const int ncol = do_j ? (int)c->color_ptr.size() - 1 : 0;
   for (int k = 0; k < ncol; ++k) {
     const int off = c->color_ptr[k], n = c->color_ptr[k + 1] - off;
-    if (do_j && mf)
+    if (mf)
       HIPCHK(c, glh::launch_mf_setup(c->deg, fp32, T, M, c->mat, Ustate, c->color_cells.p + off, n,
                                      fp32 ? (void *)c->Hq32.p : (void *)c->Hq64.p, c->Jd64.p,
                                      c->constrained ? c->mfc : glh::MfCondDev{}, c->stream));
-    if (do_r || (do_j && !mf))
-      HIPCHK(c, glh::launch_assemble(c->deg, do_j && !mf, do_r, fp32, T, M, c->mat, Ustate, c->color_cells.p + off,
-                                     n, Jcell, rcell, c->stream));
+    else
+      HIPCHK(c, glh::launch_assemble(c->deg, fp32, T, M, c->mat, Ustate, c->color_cells.p + off, n, Jcell,
+                                     c->stream));
   }
   if (c->constrained) {
     if (do_j && mf) HIPCHK(c, glh::launch_mf_cond_diag(c->cons, c->mfc, c->n_nodes, c->hdiag.p, c->Jd64.p, c->stream));
@@ -466,15 +478,15 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
       if (fp32) HIPCHK(c, glh::launch_condense<float>(c->cons, c->Jf32.p, c->J32.p, c->hdiag.p, c->stream));
       else HIPCHK(c, glh::launch_condense<double>(c->cons, c->Jf64.p, c->J64.p, c->hdiag.p, c->stream));
     }
-    if (do_r) HIPCHK(c, glh::launch_condense_rhs(c->cons, c->rfull.p, rhs_out, c->stream));
   }
+  if (do_r) RCHK(residual_cells(c, Ustate, rhs_out));
   if (timed) {
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->asm_ms = ms;
-    c->asm_launches = ncol;
+    c->asm_launches = ncol + (do_r ? 2 : 0);
     if (c->timing) {
       if (do_j) { c->jasm_ms_total += ms; c->jasm_count += 1; }
       else { c->rasm_ms_total += ms; c->rasm_count += 1; }
@@ -1740,6 +1752,8 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
   HIPCHK(c, c->rhs.alloc(n));
   HIPCHK(c, c->res.alloc(n));
   HIPCHK(c, c->Fcell.alloc(m.n_cells));
+  // the cell slots of the residual (and of the matrix-free product)
+  HIPCHK(c, c->mf_ycell.alloc((size_t)m.n_cells * c->ns * NC));
   HIPCHK(c, hipMemsetAsync(c->U.p, 0, n * sizeof(double), s));
   HIPCHK(c, hipMemsetAsync(c->dU.p, 0, n * sizeof(double), s));
   HIPCHK(c, hipMemsetAsync(c->rhs.p, 0, n * sizeof(double), s));
@@ -1875,18 +1889,9 @@ int glh_residual_into(glh_ctx *c, const double *U, double *R, double *norm2)
                     std::to_string(glh::SCRATCH_DOUBLES) + " (use glh_residual)");
   const size_t n = ndofs(c);
   hipStream_t s = c->stream;
-  // the cell loop of compute_residual (residual.cc:164-289) at U into R, or
-  // into the uploaded unconstrained buffer and then condensed into R
-  double *rcell = c->constrained ? c->rfull.p : R;
-  HIPCHK(c, glh::launch_fill<double>(n, 0.0, rcell, s));  // a kernel, not a memset: capturable
-  const glh::RefTables T = c->tables();
-  const glh::MeshDev M = c->meshdev();
-  for (int k = 0; k + 1 < (int)c->color_ptr.size(); ++k) {
-    const int off = c->color_ptr[k], nk = c->color_ptr[k + 1] - off;
-    HIPCHK(c, glh::launch_assemble(c->deg, false, true, false, T, M, c->mat, U, c->color_cells.p + off, nk, nullptr,
-                                   rcell, s));
-  }
-  if (c->constrained) HIPCHK(c, glh::launch_condense_rhs(c->cons, c->rfull.p, R, s));
+  // the residual at U into R (through the context's cell slots, allocated
+  // with the mesh)
+  RCHK(residual_cells(c, U, R));
   // per-instance ||R_s||^2 with the chunk partials in the scratch array
   HIPCHK(c, glh::launch_seg_mdot<double>(c->seg(), 1, R, n, R, c->scratch, norm2, 1, 0, false, s));
   return GLH_OK;

@@ -111,12 +111,10 @@ hipError_t launch_transfer(int n_nodes, int deg, const int *cell, const float *t
                            const double *Uo, double *Un, hipStream_t s);
 
 // ---------------------------------------------------------------- assembly
-// One launch per colour; cells[] lists the cells of that colour.
-// J: double* or float* (jfp32), may be null when !do_j.  rhs (system_rhs,
-// = -R) is accumulated and must be zeroed before the first colour.
-hipError_t launch_assemble(int deg, bool do_j, bool do_r, bool jfp32, const RefTables &T, const MeshDev &M,
-                           const Material &mat, const double *U, const int *cells, int n, void *J, double *rhs,
-                           hipStream_t s);
+// The Jacobian, one launch per colour; cells[] lists the cells of that
+// colour.  J: double* or float* (jfp32).
+hipError_t launch_assemble(int deg, bool jfp32, const RefTables &T, const MeshDev &M, const Material &mat,
+                           const double *U, const int *cells, int n, void *J, hipStream_t s);
 // Matrix-free Jacobian: per-cell bulk Hessian at the quadrature points (Hq:
 // [n_cells][NQ][171]) and the diagonal blocks (Jd: [n_nodes][324], one more
 // per hanging node on a constrained mesh, colour-ordered like the assembly);
@@ -141,6 +139,14 @@ template <typename TX>
 hipError_t launch_mf_gather(int n_nodes, const int *ptr, const int *idx, const void *ycell, const uint32_t *mask,
                             const double *Jd, const double *sh, const TX *x, const TX *b, const int *row_seg,
                             const int *flag, TX *y, hipStream_t s);
+// The residual through the product's sum-factorised passes: the listed cells'
+// contributions to R at U in their ycell slots (cells of batch instances that
+// are not active skipped), then rhs = -R, all rows written: 0 on the
+// Dirichlet components and on the rows of instances that are not active.
+hipError_t launch_res_apply(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
+                            const int *cells, int n, double *ycell, hipStream_t s);
+hipError_t launch_res_gather(int n_nodes, const int *ptr, const int *idx, const double *ycell, const uint32_t *mask,
+                             const int *row_seg, const int *active, double *rhs, hipStream_t s);
 // Per-cell free energy into Fcell[cell] for cells [0, n_cells).
 hipError_t launch_energy(int deg, const RefTables &T, const MeshDev &M, const Material &mat, const double *U,
                          int n_cells, double *Fcell, hipStream_t s);
