@@ -54,23 +54,32 @@ def spmv_bytes(n_nodes, nnzb, val_bytes, vec_bytes):
     return nnzb * 324 * val_bytes + nnzb * 4 + (n_nodes + 1) * 4 + 2 * n_nodes * 18 * vec_bytes
 
 
+# flops per quadrature point of H(q) X(q) formed as the derivative of the
+# residual density at the stored iterate: 6 dot products of 9 values (108),
+# the spin-space products u u^T, v v^T, u v^T and their derivatives along X
+# (378), their sums and differences (72), their products with the iterate and
+# X for the 18 components (864), the final combination (216); and the
+# gradient terms gamma with the quadrature weight (234)
+MF_POINT_FLOPS = 108 + 378 + 72 + 864 + 216 + 234
+
+
 def mf_flops(deg, n_cells):
     """Algorithmic FP64 flops of one sum-factorised matrix-free product J x
-    (k_mf_apply), per cell and component: the three 1D passes for x and its
+    (k_mf_apply): per cell and component the three 1D passes for x and its
     gradients at the quadrature points (2 + 3 + 4 outputs per point, N1
-    multiply-adds each), the 18 x 18 bulk Hessian at each point (18 NQ), the
-    three 1D test passes (4 + 3 + 2 outputs per point); Robin faces left out."""
+    multiply-adds each) and the three 1D test passes (4 + 3 + 2 outputs per
+    point), and per point MF_POINT_FLOPS; Robin faces left out."""
     n1 = deg + 1
     ns = n1 ** 3
-    return n_cells * 2 * 18 * (18 * n1 * ns + 18 * ns)
+    return n_cells * (2 * 18 * 18 * n1 * ns + ns * MF_POINT_FLOPS)
 
 
 def mf_bytes(deg, n_cells, n_dofs):
-    """Algorithmic HBM bytes of one matrix-free product: the stored bulk
-    Hessian (171 values per quadrature point and cell) read once, x read once
-    and y written once."""
+    """Algorithmic HBM bytes of one matrix-free product: the stored iterate
+    (18 values per quadrature point and cell) read once, x read once and y
+    written once."""
     ns = (deg + 1) ** 3
-    return n_cells * ns * 171 * 8 + 2 * n_dofs * 8
+    return n_cells * ns * 18 * 8 + 2 * n_dofs * 8
 
 
 def host_cpu():
@@ -635,22 +644,25 @@ def main():
                 "frac_of_stream_read": ach / stream_gbs}
 
     if mf:
-        # algorithmically HBM-bound: the stored Hessian (1.2 GB at the 32^3 Q2
-        # cube) takes longer at the HBM peak than the flops at the FP64 peak
+        # algorithmically bound by the FP64 arithmetic: with H(q) X(q) formed
+        # from the stored iterate (0.13 GB at the 32^3 Q2 cube) the flops at
+        # the FP64 peak take longer than the bytes at the HBM peak.  MI355X runs
+        # FP64 at 78.6 TFLOP/s on the vector and the matrix units alike.
         flops, nbytes = mf_flops(deg, mesh.n_cells), mf_bytes(deg, mesh.n_cells, n_dofs)
-        ach = nbytes / (apply_avg_ms * 1e-3) / 1e9
+        tfs = flops / (apply_avg_ms * 1e-3) / 1e12
+        gbs = nbytes / (apply_avg_ms * 1e-3) / 1e9
         roof = {"kernel": "k_mf_apply<%d> + k_mf_gather (matrix-free J x in GMRES: one launch over the cells, "
                           "one node gather)" % deg,
-                "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "bound": "mfma", "achieved": tfs, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                "frac": tfs / FP64_PEAK_TFS,
                 "traffic": tjson.get("k_mf_apply", {}).get("bytes_per_product"), "algorithmic_bytes": nbytes,
-                "algorithmic_flops": flops, "tflop_s": flops / (apply_avg_ms * 1e-3) / 1e12,
-                "frac_fp64_peak": flops / (apply_avg_ms * 1e-3) / 1e12 / FP64_PEAK_TFS,
-                "stream_read_gb_s": stream_gbs, "frac_of_stream_read": ach / stream_gbs,
-                # the algorithm's bound is HBM; the kernel itself waits on the
-                # latency of its loads and barriers at full occupancy (SQ
-                # counters in profiles/matfree_cube_q2_ref5.md)
-                "limiter": "load and barrier latency at the register-limited occupancy (waves wait 57% of "
-                           "their cycles, VALU 27% busy)"}
+                "algorithmic_flops": flops, "hbm_gb_s": gbs, "frac_hbm_peak": gbs / HBM_PEAK_GBS,
+                "stream_read_gb_s": stream_gbs,
+                # FP64 on the vector unit (the same peak as the matrix unit);
+                # the kernel waits on its LDS passes and barriers (SQ counters
+                # in profiles/matfree_cube_q2_ref5.md)
+                "peak_note": "FP64 dense peak, vector and matrix units alike",
+                "limiter": "LDS passes and barriers between them at 4 waves per SIMD"}
     else:
         roof = spmv_roofline(apply_avg_ms)
     st0 = runs[0]

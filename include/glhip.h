@@ -43,7 +43,7 @@ typedef struct glh_forest glh_forest;
 #define GLH_ASM_J 1           /* Jacobian (assemble.cc:177-280) */
 #define GLH_ASM_R 2           /* right-hand side -R (assemble.cc:257-276) */
 #define GLH_ASM_FP32 4        /* Jacobian stored and solved in FP32 */
-#define GLH_ASM_MATRIX_FREE 8 /* Jacobian in matrix-free form (see glh_set_matrix_free); with FP32 the Hessian is FP32 */
+#define GLH_ASM_MATRIX_FREE 8 /* Jacobian in matrix-free form (see glh_set_matrix_free); with FP32 its state is FP32 */
 
 /* ---------------------------------------------------------------- context */
 /* FemGL<dim> constructor state (femgl.cc:107-203): device + stream. */
@@ -464,10 +464,11 @@ int glh_set_residual_reference(glh_ctx *ctx, int n, const double *r_ref);
 int glh_set_continuation(glh_ctx *ctx, double sigma0);
 /* Matrix-free Jacobian in glh_newton_solve (a GPU alternative to the
  * assembled Trilinos matrix behind SolverFGMRES, solve.cc:159-174): on != 0
- * makes each Newton step store the bulk Hessian of every cell at its
- * quadrature points and only the 18x18 diagonal blocks, and GMRES applies
- * J x cell by cell from them (the same operator as the assembled BSR matrix,
- * to rounding).  With fp32 the Hessian is stored in FP32 and the inner GMRES,
+ * makes each Newton step store the iterate of every cell at its quadrature
+ * points and only the 18x18 diagonal blocks, and GMRES applies J x cell by
+ * cell from them, the bulk term as the derivative of the residual density at
+ * the stored iterate (the same operator as the assembled BSR matrix, to
+ * rounding).  With fp32 the iterate is stored in FP32 and the inner GMRES,
  * its Krylov basis and the block Jacobi run in FP32, inside the FP64
  * iterative refinement.  On a mesh with constraint lines (hanging nodes,
  * periodic images) the product is C^T (J_cells (C x)) with the hanging rows

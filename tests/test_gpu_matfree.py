@@ -79,7 +79,7 @@ def test_matrix_free_product_equals_the_assembled_and_the_oracle(ctx, case):
     # the matrix-free form holds no BSR values
     with pytest.raises(G.GLHError, match="matrix-free"):
         ctx.jacobian()
-    # FP32 forms (the Hessian stored in FP32): the two agree to FP32 rounding
+    # FP32 forms (FP32 values and state): the two agree to FP32 rounding
     ctx.assemble(J=True, R=False, fp32=True)
     ya32 = ctx.apply_jacobian(x)
     ctx.assemble(J=True, R=False, fp32=True, matrix_free=True)
@@ -119,7 +119,7 @@ def test_matrix_free_newton_reaches_the_assembled_fixed_point(ctx):
 
 
 def test_mixed_precision_matrix_free_newton_reaches_the_fp64_fixed_point(ctx):
-    # FP32 matrix-free inner solve (FP32 Hessian, Krylov basis and block
+    # FP32 matrix-free inner solve (FP32 iterate state, Krylov basis and block
     # Jacobi) inside FP64 iterative refinement with an FP64 residual
     mesh = G.Mesh(2, (4, 4, 4), (-20.0, -20.0, -20.0), (20.0, 20.0, 20.0), [1, 1, 1, 1, 0, 0])
     p, t = 26.0, 0.7

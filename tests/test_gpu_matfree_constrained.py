@@ -98,7 +98,7 @@ def test_products_and_diagonal_blocks_match_the_assembled_condensed_jacobian(ctx
         assert _rel(b, a) <= 1e-12 and _rel(b, yo) <= 1e-12
     say("diagonal blocks rel err %.3e" % _rel(Dm, Da))
     assert _rel(Dm, Da) <= 1e-12
-    # the FP32 form: FP32 Hessian and arithmetic, FP64 diagonal blocks
+    # the FP32 form: FP32 state and arithmetic, FP64 diagonal blocks
     ctx.assemble(J=True, R=False, matrix_free=True, fp32=True)
     y32 = ctx.apply_jacobian(X[0])
     say("FP32 matrix-free J x rel err %.3e" % _rel(y32, ya[0]))

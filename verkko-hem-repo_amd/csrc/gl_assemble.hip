@@ -148,6 +148,68 @@ __device__ inline void rho_pair(const TA *x, int mu, int k, const Material &M, d
                           M.b5 * (Sv - Wu));
 }
 
+// H(s) x for the components (mu, k) of u (zu) and of v (zv) without forming
+// H: the derivative of rho at the state s in the direction x (18 values
+// each, in LDS), term by term the linearisation of rho_pair:
+//   d|u|^2 = 2 u:a, d|v|^2 = 2 v:b, d(u:v) = u:b + a:v,
+//   d(u u^T) = a u^T + u a^T, d(v v^T) = b v^T + v b^T, d(u v^T) = a v^T + u b^T,
+//   d(P u) = dP u + P a, ...  with (a, b) the u and v parts of x.
+template <typename TA>
+__device__ inline void hx_pair(const TA *s, const TA *x, int mu, int k, const Material &M, TA &zu, TA &zv)
+{
+  // s and x are read from LDS where they are used (the 36 values held at
+  // once would spill at 4 waves per SIMD)
+  TA U2 = 0, V2 = 0, UV = 0, ua = 0, vb = 0, dUV = 0;
+#pragma unroll 1
+  for (int i = 0; i < 9; ++i) {
+    const TA u = s[i], v = s[9 + i], a = x[i], b = x[9 + i];
+    U2 += u * u;
+    V2 += v * v;
+    UV += u * v;
+    ua += u * a;
+    vb += v * b;
+    dUV += u * b;
+    dUV += a * v;
+  }
+  const TA D = U2 - V2, N = U2 + V2, dD = (TA)2 * (ua - vb), dN = (TA)2 * (ua + vb);
+  TA um[3], vm[3], am[3], bm[3];  // row mu of u, v, a, b
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    um[j] = s[3 * mu + j]; vm[j] = s[9 + 3 * mu + j];
+    am[j] = x[3 * mu + j]; bm[j] = x[9 + 3 * mu + j];
+  }
+  TA dPu = 0, dPv = 0, dQu = 0, dQv = 0, dSu = 0, dSv = 0, dWu = 0, dWv = 0;
+#pragma unroll 1
+  for (int nu = 0; nu < 3; ++nu) {  // not unrolled: its LDS reads would all be hoisted and spill
+    TA uu = 0, vv = 0, uvmn = 0, uvnm = 0, duu = 0, dvv = 0, duvmn = 0, duvnm = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const TA un = s[3 * nu + j], vn = s[9 + 3 * nu + j], an = x[3 * nu + j], bn = x[9 + 3 * nu + j];
+      uu += um[j] * un;
+      vv += vm[j] * vn;
+      uvmn += um[j] * vn;
+      uvnm += un * vm[j];
+      duu += am[j] * un + um[j] * an;
+      dvv += bm[j] * vn + vm[j] * bn;
+      duvmn += am[j] * vn + um[j] * bn;
+      duvnm += an * vm[j] + un * bm[j];
+    }
+    const TA P = uu - vv, S = uu + vv, Q = uvmn + uvnm, W = uvnm - uvmn;
+    const TA dP = duu - dvv, dS = duu + dvv, dQ = duvmn + duvnm, dW = duvnm - duvmn;
+    const TA un = s[3 * nu + k], vn = s[9 + 3 * nu + k], an = x[3 * nu + k], bn = x[9 + 3 * nu + k];
+    dPu += dP * un + P * an; dPv += dP * vn + P * bn;
+    dQu += dQ * un + Q * an; dQv += dQ * vn + Q * bn;
+    dSu += dS * un + S * an; dSv += dS * vn + S * bn;
+    dWu += dW * un + W * an; dWv += dW * vn + W * bn;
+  }
+  const TA u0 = s[3 * mu + k], v0 = s[9 + 3 * mu + k], a0 = x[3 * mu + k], b0 = x[9 + 3 * mu + k];
+  const TA al = (TA)M.al, b1 = (TA)M.b1, b2 = (TA)M.b2, b3 = (TA)M.b3, b4 = (TA)M.b4, b5 = (TA)M.b5;
+  zu = al * a0 + (TA)2 * (b1 * (dD * u0 + D * a0 + (TA)2 * (dUV * v0 + UV * b0)) + b2 * (dN * u0 + N * a0) +
+                          b3 * (dPu + dQv) + b4 * (dSu - dWv) + b5 * (dSu + dWv));
+  zv = al * b0 + (TA)2 * (b1 * ((TA)2 * (dUV * u0 + UV * a0) - (dD * v0 + D * b0)) + b2 * (dN * v0 + N * b0) +
+                          b3 * (dQu - dPv) + b4 * (dSv + dWu) + b5 * (dSv - dWu));
+}
+
 __device__ inline bool face_zeroed(int bid, int c) { return (c % 9) % 3 == bid - 2; }
 
 __device__ inline double face_area(int lf, const double *h)
@@ -348,8 +410,8 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
 }
 
 // ------------------------------------------------ matrix-free Jacobian
-// J x without the assembled matrix, for meshes without constraints.  The
-// Jacobian of the cell loop is
+// J x without the assembled matrix (on a mesh with constraint lines applied
+// to C x and condensed, gl_constraints.hip).  The Jacobian of the cell loop is
 //   J_(a,c),(b,d) = vol sum_q w_q N_a N_b H_cd(q)             (bulk, H symmetric)
 //                 + K1 delta_cd vol sum_q w_q grad N_a . grad N_b
 //                 + K23 [c, d same part and spin] vol sum_q w_q d_kc N_a d_kd N_b
@@ -358,25 +420,19 @@ __global__ __launch_bounds__(NT) void k_assemble(RefTables T, MeshDev Mh, Materi
 //   X_q = sum_b N_b(q) x_b, G_kq = sum_b d_k N_b(q) x_b / h_k,
 //   y_a += sum_q w_q vol (N_a(q) H(q) X_q + sum_k d_k N_a(q) / h_k gamma_k(q)),
 //   gamma_{c,k} = K1 G_kq[c] + K23 [k == kc] sum_kd G_kd,q[9 part + 3 mu + kd],
-// about 3.5e4 multiply-adds per Q2 cell (sum-factorised) instead of
-// streaming the cell's 729 blocks.  k_mf_setup stores H_cd(q) per cell (the only state of the
-// operator besides the diagonal blocks) and assembles the diagonal blocks
-// for the point-block Jacobi preconditioner with the arithmetic of
-// k_assemble; Dirichlet columns are applied as a zero x there, Dirichlet
-// rows keep only their diagonal (k_mf_gather).
-
-// index of the component pair (c <= d) in the order of the owner threads
-__device__ inline int upair(int c, int d)
-{
-  const int lo = c < d ? c : d, hi = c < d ? d : c;
-  return lo * 18 - lo * (lo - 1) / 2 + (hi - lo);
-}
+// sum-factorised, instead of streaming the cell's 729 blocks.  k_mf_setup
+// stores the iterate at the quadrature points per cell (the only state of
+// the operator besides the diagonal blocks; H(q) X(q) is formed from it in
+// the product) and assembles the diagonal blocks for the point-block Jacobi
+// preconditioner with the arithmetic of k_assemble; Dirichlet columns are
+// applied as a zero x there, Dirichlet rows keep only their diagonal
+// (k_mf_gather).
 
 // CON: a mesh with constraint lines (the hanging nodes' |local diagonal| and
 // the raw local blocks of X as well)
 template <int DEG, typename TH, bool CON>
 __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Material M0, const double *__restrict__ U,
-                                                 const int *__restrict__ cells, TH *__restrict__ Hq,
+                                                 const int *__restrict__ cells, TH *__restrict__ Uq,
                                                  double *__restrict__ Jd, MfCondDev X)
 {
   constexpr int N1 = DEG + 1, NS = N1 * N1 * N1, NQ = NS;
@@ -385,7 +441,7 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
   __shared__ double sGram[NQ * GRAM];
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
-  // a finished instance of a batch keeps the Hessian and blocks of its last
+  // a finished instance of a batch keeps the iterate and blocks of its last
   // step (all of its cells are skipped together)
   if (Mh.inst_active && !Mh.inst_active[Mh.cell_inst[e]]) return;
   const Material M = Mh.mats ? Mh.mats[Mh.cell_mat[e]] : M0;
@@ -401,6 +457,7 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
     double xv = 0;
     for (int a = 0; a < NS; ++a) xv += T.N[q * NS + a] * sU[a * 18 + c];
     sX[i] = xv;
+    Uq[(size_t)e * NQ * 18 + i] = (TH)xv;  // the iterate at the points, the product's state
   }
   __syncthreads();
   for (int i = tid; i < NQ * GRAM; i += NT) {
@@ -416,12 +473,8 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
     d = c + t;
   }
   double Hv[NQ];
-  TH *Hc = Hq + (size_t)e * NQ * NUPAIR;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    Hv[q] = hess_entry(sGram + q * GRAM, sX + q * 18, c, d, M);
-    Hc[q * NUPAIR + tid] = (TH)Hv[q];  // FP32 for the FP32 inner solve; the diagonal blocks stay FP64
-  }
+  for (int q = 0; q < NQ; ++q) Hv[q] = hess_entry(sGram + q * GRAM, sX + q * 18, c, d, M);
   // diagonal blocks (a,a), as k_assemble forms and masks them
   const bool grad_cd = (c / 9 == d / 9) && ((c % 9) / 3 == (d % 9) / 3);
   const int kc = c % 3, kd = d % 3;
@@ -487,33 +540,36 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
 // One thread per 1D line of points and component (a pencil) in each pass: it
 // reads its N1 inputs once and applies the 1D bases from registers, so the
 // passes move about 2.5x fewer LDS words than one thread per (point,
-// component) with the bases in LDS.  In H(q) X(q) a thread owns the
-// components c and c + 9 of one point (the same orbital entry in u and in v),
-// reads X(q) once for both, and has requested both Hessian rows before the
-// passes start, so the loads that dominate the HBM traffic are in flight
-// while the cell interpolates x.
-// TH: stored Hessian (FP64, or FP32 for the FP32 inner solve); TX: x.  TA,
-// the arithmetic, LDS and cell-slot type, is FP32 when both are FP32 (the
-// FP32 inner solve, with FP32 accumulation) and FP64 otherwise.  cells[] lists the cells in the launch order, -1
-// for padding (the XCD-aware order of mf_cell_order).
+// component) with the bases in LDS.  At the points, H(q) X(q) is not read
+// from a stored Hessian (171 values per point) but formed as the derivative
+// of rho at the stored iterate U(q) (18 values per point) in the direction
+// X(q) (hx_pair): the operator's state shrinks from 1.21 GB to 0.13 GB on the
+// 32^3 Q2 cube, and the kernel is bound by its arithmetic and LDS passes
+// instead of the Hessian stream.  A thread owns the components c and c + 9
+// of one point (the same orbital entry in u and in v).
+// TH: the stored iterate (FP64, or FP32 for the FP32 inner solve); TX: x.
+// TA, the arithmetic, LDS and cell-slot type, is FP32 when both are FP32 (the
+// FP32 inner solve, with FP32 accumulation) and FP64 otherwise.  cells[]
+// lists the cells in the launch order, -1 for padding (the XCD-aware order of
+// mf_cell_order).
+// RES: the residual R(x) of the cell loop (residual.cc:164-232, the Robin
+// faces :235-278) through the same passes: x is the iterate (no column
+// masking), and rho(X(q)), the bulk residual density, takes the place of
+// H(q) X(q); Uq is not read, and flag is the batch's instance_active.
 template <int DEG, typename TH>
 constexpr int mf_threads() { return DEG == 1 ? 128 : 256; }
 template <typename TH, typename TX>
 using mf_arith = typename std::conditional<std::is_same<TH, float>::value && std::is_same<TX, float>::value, float,
                                            double>::type;
 
-// waves per SIMD the register allocation aims at (the prefetched Hessian rows
-// alone hold 72 registers in FP64, 36 in FP32; the Q1 kernel spills at 6)
+// waves per SIMD the register allocation aims at (FP64: also the LDS limit of
+// 4 workgroups per CU)
 template <int DEG, typename TH, typename TX>
 constexpr int mf_waves() { return std::is_same<mf_arith<TH, TX>, float>::value ? (DEG == 1 ? 5 : 6) : 4; }
 
-// RES: the residual R(x) of the cell loop (residual.cc:164-232, the Robin
-// faces :235-278) through the same passes: x is the iterate (no column
-// masking), and rho(X(q)), the bulk residual density, takes the place of
-// H(q) X(q).  Hq is not read then; flag is the batch's instance_active.
 template <int DEG, typename TH, typename TX, bool RES = false>
 __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>())) void k_mf_apply(RefTables T, MeshDev Mh, Material M0,
-                                                                      const TH *__restrict__ Hq,
+                                                                      const TH *__restrict__ Uq,
                                                                       const TX *__restrict__ x,
                                                                       const int *__restrict__ cells,
                                                                       const int *__restrict__ flag,
@@ -526,16 +582,19 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
   constexpr int NH = NQ * 9;        // (point, component pair) items of H X
   static_assert(NH <= NTA, "one H X item per thread");
   // Regions of one value per (point or node, component), in the order
-  //   x -> A0 A1 -> B0 B1 B2 -> X G0 G1 G2 -> Z K0 K1 K2 -> C0 C1 C2 -> D0 D1.
+  //   x -> A0 A1 -> B0 B1 B2 -> X G0 G1 G2 -> Z K0 K1 K2 -> C0 C1 C2 -> D0 D1,
+  // and the iterate U(q) of the product.
   // A pencil reads exactly the points it writes, so the FP32 kernel lets each
-  // pass overwrite its inputs in place in one set of four regions (7.8 KB for
-  // Q2: register-limited at 6 waves per SIMD).  The FP64 kernel, limited to 4
-  // waves by its registers anyway, alternates between two sets and keeps x
-  // (35 KB), which saves a barrier and the Robin faces' second gather of x.
+  // pass overwrite its inputs in place in one set of four regions (9.9 KB for
+  // Q2 with U(q): register-limited at 6 waves per SIMD).  The FP64 kernel
+  // alternates between two sets and keeps x (39 KB, 4 workgroups per CU),
+  // which saves a barrier and the Robin faces' second gather of x.
   constexpr bool IP = std::is_same<TA, float>::value;
-  constexpr int NXR = (V + NTA - 1) / NTA;         // x values per thread
-  __shared__ TA S[(IP ? 4 : 9) * V + NXR * NTA - V];  // room for the unconditional x stores
-  TA *const P0 = S, *const P1 = IP ? S : S + 4 * V, *const SX = IP ? S : S + 8 * V;
+  constexpr int NXR = (V + NTA - 1) / NTA;  // x values per thread
+  // room for the unconditional x and U(q) stores, U(q) after x
+  constexpr int SU_OFF = (IP ? 3 * V : 8 * V) + NXR * NTA;
+  __shared__ TA S[SU_OFF + (RES ? 0 : NXR * NTA)];
+  TA *const P0 = S, *const P1 = IP ? S : S + 4 * V, *const SX = IP ? S : S + 8 * V, *const SU = S + SU_OFF;
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
   if (e < 0) return;                           // padding of the XCD-aware order
@@ -560,10 +619,10 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
     }
   // A wave's vector loads may return out of order, so a wait for one is a
   // wait for all outstanding.  x at the cell's nodes is therefore gathered
-  // and stored to LDS first, and the Hessian rows are requested after that
-  // wait: they stay in flight through the interpolation passes, which wait
-  // only on LDS, until H X.  Both without branches (indices clamped into
-  // range, unused values dropped by selects or stored to padding).
+  // and stored to LDS first, and U(q) is requested after that wait: it stays
+  // in flight through the interpolation passes, which wait only on LDS.
+  // Both without branches (indices clamped into range, unused values stored
+  // to padding).
   // x, Dirichlet components 0 (the masked columns; the residual's iterate as it is):
   auto xval = [&](int i) -> TA {
     const int node = cn[i / 18], c = i % 18;
@@ -577,25 +636,18 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
     const TA v = xval(i < V ? i : V - 1);
     SX[i] = i < V ? v : (TA)0;  // beyond V: the padding, never read
   }
-  // keep the Hessian loads below the x stores (and so below their wait)
+  // keep the U(q) loads below the x stores (and so below their wait)
   __builtin_amdgcn_sched_barrier(0);
-  // the two Hessian rows of this thread's H X item (point hq, components hc
-  // and hc + 9; threads beyond the items read the last point's, unused)
-  const int hq = tid / 9 < NQ ? tid / 9 : NQ - 1, hc = tid % 9;
-  TH hv[2][18];
+  TH us[NXR];  // this thread's share of the cell's U(q), stored to LDS after the z pass
   if constexpr (!RES) {
-    const TH *Hrow = Hq + ((size_t)e * NQ + hq) * NUPAIR;
+    const TH *Uc = Uq + (size_t)e * V;
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const int cr = hc + 9 * r;
-      int off = cr;  // upair(cr, d): steps 17 - d while d < cr, then 1
-#pragma unroll
-      for (int d = 0; d < 18; ++d) {
-        hv[r][d] = Hrow[off];
-        off += d < cr ? 17 - d : 1;
-      }
+    for (int r = 0; r < NXR; ++r) {
+      const int i = tid + r * NTA;
+      us[r] = Uc[i < V ? i : V - 1];
     }
   }
+  const int hq = tid / 9 < NQ ? tid / 9 : NQ - 1, hc = tid % 9;  // H X item: point hq, components hc, hc + 9
   __syncthreads();
   auto at = [](int i0, int i1, int i2, int c) { return ((i2 * N1 + i1) * N1 + i0) * 18 + c; };
   const int c = tid % 18, l = tid / 18, la = l % N1, lb = l / N1;  // pencil: line (la, lb), component c
@@ -667,6 +719,10 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
       P1[3 * V + o] = g2 * ih[2];
     }
   }
+  if constexpr (!RES) {
+#pragma unroll
+    for (int r = 0; r < NXR; ++r) SU[tid + r * NTA] = (TA)us[r];  // beyond V: padding, never read
+  }
   __syncthreads();
   // w vol H(q) X(q) and w vol gamma_k(q) / h_k for the components hc and hc + 9
   // of point hq: read everything, then write Z and K (over X and G in place)
@@ -680,12 +736,7 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
       z[0] = ru;
       z[1] = rv;
     } else {
-#pragma unroll
-      for (int d = 0; d < 18; ++d) {
-        const TA xd = X[d];  // one read of X(q) for both rows
-        z[0] += (TA)hv[0][d] * xd;
-        z[1] += (TA)hv[1][d] * xd;
-      }
+      hx_pair(SU + hq * 18, X, hc / 3, hc % 3, M, z[0], z[1]);
     }
     const TA wv = (TA)(T.w[hq] * vol), K1 = (TA)M.K1, K23 = (TA)M.K23;
     const int mu = hc / 3, kc = hc % 3;
@@ -969,37 +1020,37 @@ hipError_t launch_assemble(int deg, bool jfp32, const RefTables &T, const MeshDe
   return hipGetLastError();
 }
 
-hipError_t launch_mf_setup(int deg, bool h32, const RefTables &T, const MeshDev &M, const Material &mat,
-                           const double *U, const int *cells, int n, void *Hq, double *Jd, const MfCondDev &X,
+hipError_t launch_mf_setup(int deg, bool u32, const RefTables &T, const MeshDev &M, const Material &mat,
+                           const double *U, const int *cells, int n, void *Uq, double *Jd, const MfCondDev &X,
                            hipStream_t s)
 {
   if (n <= 0) return hipSuccess;
   if (X.xptr && !M.hdiag) return hipErrorInvalidValue;  // the constrained setup writes hdiag
 #define GLH_MFS(D, TH, CON) \
-  hipLaunchKernelGGL((k_mf_setup<D, TH, CON>), dim3(n), dim3(NT), 0, s, T, M, mat, U, cells, (TH *)Hq, Jd, X)
+  hipLaunchKernelGGL((k_mf_setup<D, TH, CON>), dim3(n), dim3(NT), 0, s, T, M, mat, U, cells, (TH *)Uq, Jd, X)
 #define GLH_MFS2(D, TH) \
   if (X.xptr) GLH_MFS(D, TH, true); else GLH_MFS(D, TH, false)
-  if (deg == 1) { if (h32) GLH_MFS2(1, float); else GLH_MFS2(1, double); }
-  else { if (h32) GLH_MFS2(2, float); else GLH_MFS2(2, double); }
+  if (deg == 1) { if (u32) GLH_MFS2(1, float); else GLH_MFS2(1, double); }
+  else { if (u32) GLH_MFS2(2, float); else GLH_MFS2(2, double); }
 #undef GLH_MFS2
 #undef GLH_MFS
   return hipGetLastError();
 }
 
-hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, const MeshDev &M, const Material &mat,
-                           const void *Hq, const void *x, const int *cells, int n, const int *flag, void *ycell,
+hipError_t launch_mf_apply(int deg, bool u32, bool x32, const RefTables &T, const MeshDev &M, const Material &mat,
+                           const void *Uq, const void *x, const int *cells, int n, const int *flag, void *ycell,
                            hipStream_t s)
 {
   if (n <= 0) return hipSuccess;
-  if (x32 && !h32) return hipErrorInvalidValue;  // an FP64 Hessian is applied to FP64 vectors only
+  if (x32 && !u32) return hipErrorInvalidValue;  // an FP64 operator is applied to FP64 vectors only
 #define GLH_MFA(D, TH, TX)                                                                                  \
-  hipLaunchKernelGGL((k_mf_apply<D, TH, TX>), dim3(n), dim3((mf_threads<D, TH>())), 0, s, T, M, mat, (const TH *)Hq, \
+  hipLaunchKernelGGL((k_mf_apply<D, TH, TX>), dim3(n), dim3((mf_threads<D, TH>())), 0, s, T, M, mat, (const TH *)Uq, \
                      (const TX *)x, cells, flag, (mf_arith<TH, TX> *)ycell)
   if (deg == 1) {
-    if (h32) { if (x32) GLH_MFA(1, float, float); else GLH_MFA(1, float, double); }
+    if (u32) { if (x32) GLH_MFA(1, float, float); else GLH_MFA(1, float, double); }
     else GLH_MFA(1, double, double);
   } else {
-    if (h32) { if (x32) GLH_MFA(2, float, float); else GLH_MFA(2, float, double); }
+    if (u32) { if (x32) GLH_MFA(2, float, float); else GLH_MFA(2, float, double); }
     else GLH_MFA(2, double, double);
   }
 #undef GLH_MFA

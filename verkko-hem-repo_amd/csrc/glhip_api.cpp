@@ -138,11 +138,11 @@ struct glh_ctx {
   DBuf<double> J64, D64;
   DBuf<float> J32, D32;
   bool j_fp32 = false, j_valid = false, d_valid = false;
-  // matrix-free Jacobian (glh_set_matrix_free): per-cell bulk Hessian at the
-  // quadrature points and the diagonal blocks, in place of the BSR values
+  // matrix-free Jacobian (glh_set_matrix_free): per cell the iterate at the
+  // quadrature points, and the diagonal blocks, in place of the BSR values
   bool matfree = false, j_mf = false;
-  DBuf<double> Hq64, Jd64;
-  DBuf<float> Hq32;  // the stored Hessian of an FP32 matrix-free Jacobian
+  DBuf<double> Uq64, Jd64;
+  DBuf<float> Uq32;  // the stored iterate of an FP32 matrix-free Jacobian
   DBuf<int> diag_id;  // node -> its block in Jd64 (the identity, hanging nodes aside)
   DBuf<int> n2c_ptr, n2c_idx;  // node -> its (cell * ns + local node) entries
   DBuf<double> mf_ycell;       // per-cell contributions of a product [n_cells][ns][18] (FP32 in an FP32 product)
@@ -429,8 +429,9 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
 {
   void *J = nullptr;
   if (do_j && mf) {  // matrix-free: H at the quadrature points and the diagonal blocks
-    if (fp32) HIPCHK(c, c->Hq32.alloc((size_t)c->n_cells * c->nq * 171));
-    else HIPCHK(c, c->Hq64.alloc((size_t)c->n_cells * c->nq * 171));
+    const size_t un = (size_t)c->n_cells * c->nq * NC;
+    if (fp32) HIPCHK(c, c->Uq32.alloc(un));
+    else HIPCHK(c, c->Uq64.alloc(un));
     HIPCHK(c, c->mf_ycell.alloc((size_t)c->n_cells * c->ns * NC));
     // one block per node, and on a constrained mesh one more per hanging node
     // (its condensed row for the point-block Jacobi, diag_id points there)
@@ -466,7 +467,7 @@ int assemble_at(glh_ctx *c, const double *Ustate, bool do_j, bool do_r, bool fp3
     const int off = c->color_ptr[k], n = c->color_ptr[k + 1] - off;
     if (mf)
       HIPCHK(c, glh::launch_mf_setup(c->deg, fp32, T, M, c->mat, Ustate, c->color_cells.p + off, n,
-                                     fp32 ? (void *)c->Hq32.p : (void *)c->Hq64.p, c->Jd64.p,
+                                     fp32 ? (void *)c->Uq32.p : (void *)c->Uq64.p, c->Jd64.p,
                                      c->constrained ? c->mfc : glh::MfCondDev{}, c->stream));
     else
       HIPCHK(c, glh::launch_assemble(c->deg, fp32, T, M, c->mat, Ustate, c->color_cells.p + off, n, Jcell,
@@ -584,21 +585,21 @@ int flush_events(glh_ctx *c)
 
 // y = J x (b == null) or y = b - J x with the matrix-free Jacobian: every cell
 // at once into its own slot, then a fixed-order gather per node (no colours:
-// one launch, no read-modify-write of y).  The stored Hessian is FP32 for an
+// one launch, no read-modify-write of y).  The stored iterate is FP32 for an
 // FP32 Jacobian, whatever the vector type.
 template <typename TV>
 int mf_product(glh_ctx *c, const TV *x, const TV *b, TV *y, const int *flag = nullptr)
 {
   const glh::RefTables T = c->tables();
   const glh::MeshDev M = c->meshdev();
-  const bool h32 = c->j_fp32, x32 = std::is_same<TV, float>::value;
+  const bool u32 = c->j_fp32, x32 = std::is_same<TV, float>::value;
   const double *sh = c->mf_sh_on ? c->mf_sh.p : nullptr;
   // constrained meshes: y = C^T Jf (C x), hanging rows hdiag x
   const bool con = c->constrained;
   TV *xf = con ? reinterpret_cast<TV *>(c->mf_xf.p) : nullptr, *yf = con ? reinterpret_cast<TV *>(c->mf_yf.p) : y;
   const TV *xin = con ? xf : x;
   if (con) HIPCHK(c, glh::launch_mf_distribute<TV>(c->cons, c->n_nodes, x, xf, c->row_seg.p, flag, c->stream));
-  HIPCHK(c, glh::launch_mf_apply(c->deg, h32, x32, T, M, c->mat, h32 ? (const void *)c->Hq32.p : (const void *)c->Hq64.p,
+  HIPCHK(c, glh::launch_mf_apply(c->deg, u32, x32, T, M, c->mat, u32 ? (const void *)c->Uq32.p : (const void *)c->Uq64.p,
                                  xin, c->mf_order.p, c->mf_order_n, flag, c->mf_ycell.p, c->stream));
   HIPCHK(c, glh::launch_mf_gather<TV>(c->n_nodes, c->n2c_ptr.p, c->n2c_idx.p, c->mf_ycell.p, c->node_mask.p,
                                       c->Jd64.p, sh, xin, con ? nullptr : b, c->row_seg.p, flag, yf, c->stream));
@@ -799,7 +800,7 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
                                          c->gmjseg.p, c->gmits.p,    c->gmro.p,   c->gmfrozen.p, c->gmjlast.p,
                                          c->row_seg.p, c->d_node_off.p, c->act_counts.p, c->act_seg.p,
                                          c->act_rpref.p, c->act_chunk.p, c->chunks.p, c->seg_chunk.p,
-                                         c->row_ptr.p, c->col.p, c->Hq64.p, c->Hq32.p, c->Jd64.p,
+                                         c->row_ptr.p, c->col.p, c->Uq64.p, c->Uq32.p, c->Jd64.p,
                                          c->mf_xf.p, c->mf_yf.p, c->mf_sh_on ? c->mf_sh.p : nullptr,
                                          (const void *)(size_t)c->j_mf,
                                          (const void *)(size_t)m, (const void *)(size_t)S,

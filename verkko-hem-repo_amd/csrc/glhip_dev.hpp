@@ -115,24 +115,24 @@ hipError_t launch_transfer(int n_nodes, int deg, const int *cell, const float *t
 // colour.  J: double* or float* (jfp32).
 hipError_t launch_assemble(int deg, bool jfp32, const RefTables &T, const MeshDev &M, const Material &mat,
                            const double *U, const int *cells, int n, void *J, hipStream_t s);
-// Matrix-free Jacobian: per-cell bulk Hessian at the quadrature points (Hq:
-// [n_cells][NQ][171]) and the diagonal blocks (Jd: [n_nodes][324], one more
-// per hanging node on a constrained mesh, colour-ordered like the assembly);
-// the product writes
+// Matrix-free Jacobian: per cell the iterate at the quadrature points (Uq:
+// [n_cells][NQ][18]; the product forms H(q) X(q) from it) and the diagonal
+// blocks (Jd: [n_nodes][324], one more per hanging node on a constrained
+// mesh), the setup colour-ordered like the assembly; the product writes
 // each listed cell's contribution to its slot of ycell ([n_cells][NS][18]),
 // and launch_mf_gather sums them per node (node -> cell*NS + a lists, fixed
 // order) and writes the Dirichlet rows.
-// h32: the Hessian stored in FP32 (for the FP32 inner solve); x32: x in FP32.
+// u32: the iterate stored in FP32 (for the FP32 inner solve); x32: x in FP32.
 // The gather returns y = J x, or y = b - J x with b != null.
 // With X.xptr set (constrained meshes) the setup also stores the raw local
 // blocks X lists and adds the |local diagonal| of hanging nodes to M.hdiag.
-hipError_t launch_mf_setup(int deg, bool h32, const RefTables &T, const MeshDev &M, const Material &mat,
-                           const double *U, const int *cells, int n, void *Hq, double *Jd, const MfCondDev &X,
+hipError_t launch_mf_setup(int deg, bool u32, const RefTables &T, const MeshDev &M, const Material &mat,
+                           const double *U, const int *cells, int n, void *Uq, double *Jd, const MfCondDev &X,
                            hipStream_t s);
 // flag (per instance, or null): cells and rows of instances whose flag is 0
 // are skipped, as in the batched SpMV.
-hipError_t launch_mf_apply(int deg, bool h32, bool x32, const RefTables &T, const MeshDev &M, const Material &mat,
-                           const void *Hq, const void *x, const int *cells, int n, const int *flag, void *ycell,
+hipError_t launch_mf_apply(int deg, bool u32, bool x32, const RefTables &T, const MeshDev &M, const Material &mat,
+                           const void *Uq, const void *x, const int *cells, int n, const int *flag, void *ycell,
                            hipStream_t s);
 // sh (or null): a diagonal term added to every row (the continuation shift).
 template <typename TX>
