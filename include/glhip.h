@@ -353,6 +353,14 @@ int glh_apply_jacobian(glh_ctx *ctx, const double *x_host, double *y_host);
  * reference, which preconditions with Trilinos AMG (solve.cc:130-152); for
  * parity checks of the preconditioner of either form. */
 int glh_diagonal_blocks(glh_ctx *ctx, double *out);
+/* y = D^-1 x with the point-block Jacobi preconditioner of the current
+ * Jacobian (the inverses of the blocks glh_diagonal_blocks returns, as the
+ * GMRES of glh_solve applies them: FP32 for an FP32 Jacobian), FP64 vectors
+ * of n_dofs on the host.  Inverts the blocks first if the Jacobian changed
+ * (GLH_ERR_SINGULAR for a singular block).  No counterpart in the
+ * reference (Trilinos AMG, solve.cc:130-152); for parity checks of the
+ * preconditioner. */
+int glh_apply_block_jacobi(glh_ctx *ctx, const double *x_host, double *y_host);
 
 /* ---------------------------------------------------------- Newton step */
 /* FemGL::assemble_system (assemble.cc:108-372): J and/or system_rhs = -R at

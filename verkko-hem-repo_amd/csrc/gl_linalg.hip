@@ -212,64 +212,100 @@ __global__ __launch_bounds__(256) void k_res_spmv(int n_rows, const int *__restr
   if (lane < 18) r[(size_t)row * 18 + lane] = bv[(size_t)row * 18 + lane] - t;
 }
 
-// Gauss-Jordan inverse of one 18x18 diagonal block per workgroup (64 threads,
-// thread j owns column j of the augmented [A | I] matrix).
-template <typename TJ>
-__global__ __launch_bounds__(64) void k_bj_invert(int n_rows, const int *__restrict__ diag,
-                                                  const TJ *__restrict__ val, TJ *__restrict__ Dinv,
-                                                  int *__restrict__ status)
+// Order the LDS accesses of one wavefront around a point where its lanes
+// exchange values (the wavefront runs in step; no workgroup barrier).
+__device__ inline void wave_lds_sync()
 {
-  __shared__ double M[18][37];
-  __shared__ double colc[18];
-  __shared__ int piv;
-  const int r = blockIdx.x, t = threadIdx.x;
-  if (r >= n_rows) return;
-  const TJ *A = val + (size_t)diag[r] * 324;
-  for (int i = t; i < 18 * 36; i += 64) {
-    const int row = i / 36, cl = i % 36;
-    M[row][cl] = cl < 18 ? (double)A[row * 18 + cl] : (cl - 18 == row ? 1.0 : 0.0);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Gauss-Jordan inverse of the 18x18 diagonal blocks with partial pivoting,
+// in place: three nodes per wavefront, lane j of a node's 18 holding column j
+// of its block in registers.  A pivot step is one division and 17 FMAs per
+// lane; the pivot row and value come from the pivot column's lane by
+// shuffles, that column's multipliers through LDS, and no workgroup barrier
+// is needed.  The row interchanges are undone as column interchanges at the
+// end.  The in-place form keeps exactly the right half of the augmented
+// [A | I] elimination (same pivots, same divisions and FMAs in the same
+// order), so the inverse is that elimination's to the bit.
+template <typename TJ>
+__global__ __launch_bounds__(256) void k_bj_invert(int n_rows, const int *__restrict__ diag,
+                                                   const TJ *__restrict__ val, TJ *__restrict__ Dinv,
+                                                   int *__restrict__ status)
+{
+  __shared__ double sC[4][4][18];     // the multipliers of the current step, per node (slot 3: idle lanes)
+  __shared__ double sM[4][3][18 * 18]; // the unscrambled inverse, for the symmetric packed store
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sub = lane / 18, j = lane - 18 * sub;  // sub 3: lanes 54..63 hold no node
+  const long long node = ((long long)blockIdx.x * 4 + wv) * 3 + sub;
+  const bool valid = sub < 3 && node < n_rows;
+  const int base = 18 * sub;
+  double a[18];
+  {
+    const TJ *A = val + (valid ? (size_t)diag[node] * 324 : 0);
+#pragma unroll
+    for (int i = 0; i < 18; ++i) a[i] = valid ? (double)A[i * 18 + j] : (i == j ? 1.0 : 0.0);
   }
-  __syncthreads();
+  int perm[18];
   bool singular = false;
+#pragma unroll
   for (int c = 0; c < 18; ++c) {
-    if (t == 0) {
-      int p = c;
-      double best = fabs(M[c][c]);
-      for (int i = c + 1; i < 18; ++i)
-        if (fabs(M[i][c]) > best) { best = fabs(M[i][c]); p = i; }
-      piv = p;
+    // pivot: the first row r >= c of largest |A[r][c]| (column c is lane base + c)
+    int pl = c;
+    double best = fabs(a[c]);
+#pragma unroll
+    for (int r = c + 1; r < 18; ++r)
+      if (fabs(a[r]) > best) { best = fabs(a[r]); pl = r; }
+    const int p = __shfl(pl, base + c);
+    perm[c] = p;
+#pragma unroll
+    for (int r = c + 1; r < 18; ++r) {  // interchange rows c and p of this column
+      const bool sw = p == r;
+      const double t = a[c];
+      a[c] = sw ? a[r] : t;
+      a[r] = sw ? t : a[r];
     }
-    __syncthreads();
-    const int p = piv;
-    if (p != c && t < 36) {
-      const double tmp = M[c][t];
-      M[c][t] = M[p][t];
-      M[p][t] = tmp;
-    }
-    __syncthreads();
-    const double pv = M[c][c];
-    if (pv == 0.0) { singular = true; break; }
-    __syncthreads();
-    if (t < 36) M[c][t] /= pv;
-    if (t < 18) colc[t] = M[t][c];
-    __syncthreads();
-    if (t < 36)
-      for (int i = 0; i < 18; ++i)
-        if (i != c) M[i][t] -= colc[i] * M[c][t];
-    __syncthreads();
+    const double piv = __shfl(a[c], base + c);
+    singular = singular || piv == 0.0;
+    if (j == c)
+#pragma unroll
+      for (int i = 0; i < 18; ++i) sC[wv][sub][i] = a[i];  // the multipliers A[i][c], i != c
+    a[c] = (j == c ? 1.0 : a[c]) / piv;                     // row c / pivot (column c: 1 / pivot)
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < 18; ++i)
+      if (i != c) a[i] = (j == c ? 0.0 : a[i]) - sC[wv][sub][i] * a[c];
+    wave_lds_sync();  // this step's reads before the next step's stores
   }
+  // undo the row interchanges as column interchanges, last step first
+  int pos = j;
+#pragma unroll
+  for (int c = 17; c >= 0; --c) {
+    if (pos == c) pos = perm[c];
+    else if (pos == perm[c]) pos = c;
+  }
+  if (sub < 3) {
+    double *M = sM[wv][sub];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) M[i * 18 + pos] = a[i];
+  }
+  wave_lds_sync();
+  if (!valid) return;
   if (singular) {
-    if (t == 0) status[0] = 1;
+    if (j == 0) status[0] = 1;
     return;
   }
   // the inverse of a symmetric block is symmetric: its upper triangle,
   // row-major over c <= d, the two rounded halves averaged
-  TJ *D = Dinv + (size_t)r * 171;
-  for (int i = t; i < 171; i += 64) {
-    int c = 0, q = i;
-    while (q >= 18 - c) { q -= 18 - c; ++c; }
-    const int d = c + q;
-    D[i] = (TJ)(0.5 * (M[c][18 + d] + M[d][18 + c]));
+  const double *M = sM[wv][sub];
+  TJ *D = Dinv + (size_t)node * 171;
+  for (int q = j; q < 171; q += 18) {
+    int c = 0, r = q;
+    while (r >= 18 - c) { r -= 18 - c; ++c; }
+    const int d = c + r;
+    D[q] = (TJ)(0.5 * (M[c * 18 + d] + M[d * 18 + c]));
   }
 }
 
@@ -988,7 +1024,7 @@ template <typename TJ>
 hipError_t launch_bj_invert(int n_rows, const int *diag_blk, const TJ *val, TJ *Dinv, int *status, hipStream_t s)
 {
   if (n_rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_bj_invert<TJ>), dim3(n_rows), dim3(64), 0, s, n_rows, diag_blk, val, Dinv, status);
+  hipLaunchKernelGGL((k_bj_invert<TJ>), dim3((n_rows + 11) / 12), dim3(256), 0, s, n_rows, diag_blk, val, Dinv, status);
   return hipGetLastError();
 }
 

@@ -1837,6 +1837,32 @@ int glh_diagonal_blocks(glh_ctx *c, double *out)
   return GLH_OK;
 }
 
+int glh_apply_block_jacobi(glh_ctx *c, const double *x, double *y)
+{
+  RCHK(check_ready(c));
+  if (!x || !y) return fail(c, GLH_ERR_ARG, "null vector");
+  if (!c->j_valid) return fail(c, GLH_ERR_STATE, "no Jacobian assembled");
+  RCHK(ensure_precond(c));
+  const size_t n = ndofs(c);
+  HIPCHK(c, c->gW64.alloc(n));
+  HIPCHK(c, c->gZ64.alloc(n));
+  HIPCHK(c, hipMemcpyAsync(c->gZ64.p, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (c->j_fp32) {  // the FP32 inverses the FP32 inner solve applies, on FP32 vectors
+    HIPCHK(c, c->gW32.alloc(n));
+    HIPCHK(c, c->gZ32.alloc(n));
+    HIPCHK(c, (glh::launch_convert<double, float>(n, c->gZ64.p, c->gZ32.p, c->stream)));
+    HIPCHK(c, (glh::launch_bj_apply<float, float>(c->n_nodes, c->D32.p, c->gZ32.p, c->gW32.p, c->stream, nullptr,
+                                                  nullptr, glh::ActiveSet{})));
+    HIPCHK(c, (glh::launch_convert<float, double>(n, c->gW32.p, c->gW64.p, c->stream)));
+  } else {
+    HIPCHK(c, (glh::launch_bj_apply<double, double>(c->n_nodes, c->D64.p, c->gZ64.p, c->gW64.p, c->stream, nullptr,
+                                                    nullptr, glh::ActiveSet{})));
+  }
+  HIPCHK(c, hipMemcpyAsync(y, c->gW64.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GLH_OK;
+}
+
 int glh_get_jacobian(glh_ctx *c, double *val)
 {
   RCHK(check_ready(c, false));

@@ -32,7 +32,7 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_mesh_constraint_sizes", "glh_mesh_constraints", "glh_forest_create", "glh_forest_destroy",
            "glh_forest_n_cells", "glh_forest_refine", "glh_forest_mesh", "glh_forest_transfer", "glh_kelly",
            "glh_flag_fixed_number", "glh_initial_guess_bna", "glh_run_adaptive",
-           "glh_set_continuation", "glh_set_matrix_free", "glh_apply_jacobian", "glh_diagonal_blocks", "glh_mlp_create", "glh_mlp_destroy", "glh_mlp_last_error", "glh_mlp_n_params",
+           "glh_set_continuation", "glh_set_matrix_free", "glh_apply_jacobian", "glh_diagonal_blocks", "glh_apply_block_jacobi", "glh_mlp_create", "glh_mlp_destroy", "glh_mlp_last_error", "glh_mlp_n_params",
            "glh_mlp_get_params", "glh_mlp_set_params", "glh_mlp_set_data", "glh_mlp_train", "glh_mlp_loss_grad",
            "glh_mlp_predict", "glh_instance_residuals", "glh_set_forcing", "glh_set_residual_reference",
            "glh_set_schedule", "glh_autotune", "glh_prm_parse", "glh_prm_destroy", "glh_prm_get",
@@ -103,6 +103,7 @@ def lib():
     L.glh_set_matrix_free.argtypes = [vp, C.c_int]
     L.glh_apply_jacobian.argtypes = [vp, _dp, _dp]
     L.glh_diagonal_blocks.argtypes = [vp, _dp]
+    L.glh_apply_block_jacobi.argtypes = [vp, _dp, _dp]
     L.glh_set_forcing.argtypes = [vp, C.c_int]
     L.glh_write_vtu.argtypes = [vp, _dp, vp, C.c_char_p, C.c_int]
     L.glh_prm_parse.argtypes = [C.c_char_p, C.POINTER(vp), C.c_char_p, C.c_int]
@@ -605,6 +606,14 @@ class Context:
         out = np.zeros(self.n_dofs * 18)
         self._chk(lib().glh_diagonal_blocks(self.h, out), "glh_diagonal_blocks")
         return out.reshape(-1, 18, 18)
+
+    def apply_block_jacobi(self, x):
+        """D^-1 x with the point-block Jacobi preconditioner of the current
+        Jacobian, as GMRES applies it; glh_apply_block_jacobi."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros_like(x)
+        self._chk(lib().glh_apply_block_jacobi(self.h, x, y), "glh_apply_block_jacobi")
+        return y
 
     def jacobian(self):
         return self._get("glh_get_jacobian", self.nnzb * 324).reshape(-1, 18, 18)
