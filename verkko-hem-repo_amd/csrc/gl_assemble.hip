@@ -439,6 +439,11 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
   __shared__ double sU[NS * 18];
   __shared__ double sX[NQ * 18];
   __shared__ double sGram[NQ * GRAM];
+  // per node of the cell: global node, constraint mask, and whether this
+  // cell writes the node's diagonal block first
+  __shared__ int sNode[NS];
+  __shared__ uint32_t sMask[NS];
+  __shared__ int sFirst[NS];
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
   // a finished instance of a batch keeps the iterate and blocks of its last
@@ -450,7 +455,14 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
   for (int k = 0; k < 3; ++k) { h[k] = Mh.cell_h[3 * e + k]; ih[k] = 1.0 / h[k]; }
   const double vol = h[0] * h[1] * h[2];
   const uint32_t cface = Mh.cell_face[e];
+  const int *cblk = Mh.cell_blk + (size_t)e * NS * NS;
   for (int i = tid; i < NS * 18; i += NT) sU[i] = U[(size_t)cn[i / 18] * 18 + (i % 18)];
+  if (tid < NS) {
+    const int node = cn[tid];
+    sNode[tid] = node;
+    sMask[tid] = Mh.node_mask[node];
+    sFirst[tid] = cblk[tid * NS + tid] < 0;
+  }
   __syncthreads();
   for (int i = tid; i < NQ * 18; i += NT) {
     const int q = i / 18, c = i % 18;
@@ -481,7 +493,6 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
   const double lap0 = vol * ih[0] * ih[0], lap1 = vol * ih[1] * ih[1], lap2 = vol * ih[2] * ih[2];
   const double k23_cd = M.K23 * vol * ih[kc] * ih[kd];
   const int o_cd = c * 18 + d, o_dc = d * 18 + c;
-  const int *cblk = Mh.cell_blk + (size_t)e * NS * NS;
   // entries [c][d] (v1) and [d][c] (v2) of the local block of node pair p =
   // (a <= b), before the Dirichlet rows and columns are applied
   auto pair_block = [&](int p, int a, int b, double &v1, double &v2) {
@@ -506,19 +517,38 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
       v2 += k23_cd * sp[kd * 3 + kc];
     }
   };
-  for (int a = 0; a < NS; ++a) {
-    double v1, v2;
-    pair_block(a * NS - a * (a - 1) / 2, a, a, v1, v2);  // pair (a, a)
-    const uint32_t ma = Mh.node_mask[cn[a]];
-    // a hanging node's condensed row keeps the sum over cells of |local diagonal|
-    if (CON && c == d && (ma & HANG_BIT)) Mh.hdiag[(size_t)cn[a] * 18 + c] += fabs(v1);
-    const bool r1 = (ma >> c) & 1u, c1 = (ma >> d) & 1u, r2 = (ma >> d) & 1u, c2 = (ma >> c) & 1u;
-    if (r1 || c1) v1 = (r1 && c == d) ? fabs(v1) : 0.0;
-    if (r2 || c2) v2 = (r2 && c == d) ? fabs(v2) : 0.0;
-    const bool first = cblk[a * NS + a] < 0;  // this cell writes the node's diagonal block first
-    double *B = Jd + (size_t)cn[a] * 324;
-    B[o_cd] = (first ? 0.0 : B[o_cd]) + v1;
-    if (c != d) B[o_dc] = (first ? 0.0 : B[o_dc]) + v2;
+  // The nodes' blocks are updated a chunk of nodes at a time: the chunk's
+  // contributions, then all of its old values requested at once, then the
+  // stores, so that one load latency covers the chunk (the nodes of a cell
+  // are distinct, so the chunk's blocks do not overlap).
+  constexpr int AC = DEG == 1 ? 4 : 3;
+  static_assert(NS % AC == 0, "whole chunks of nodes");
+  for (int a0 = 0; a0 < NS; a0 += AC) {
+    double v1[AC], v2[AC], o1[AC], o2[AC];
+#pragma unroll
+    for (int k = 0; k < AC; ++k) {
+      const int a = a0 + k;
+      pair_block(a * NS - a * (a - 1) / 2, a, a, v1[k], v2[k]);  // pair (a, a)
+      const uint32_t ma = sMask[a];
+      // a hanging node's condensed row keeps the sum over cells of |local diagonal|
+      if (CON && c == d && (ma & HANG_BIT)) Mh.hdiag[(size_t)sNode[a] * 18 + c] += fabs(v1[k]);
+      const bool r1 = (ma >> c) & 1u, c1 = (ma >> d) & 1u, r2 = (ma >> d) & 1u, c2 = (ma >> c) & 1u;
+      if (r1 || c1) v1[k] = (r1 && c == d) ? fabs(v1[k]) : 0.0;
+      if (r2 || c2) v2[k] = (r2 && c == d) ? fabs(v2[k]) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < AC; ++k) {
+      const double *B = Jd + (size_t)sNode[a0 + k] * 324;
+      const bool first = sFirst[a0 + k];
+      o1[k] = first ? 0.0 : B[o_cd];
+      o2[k] = (first || c == d) ? 0.0 : B[o_dc];
+    }
+#pragma unroll
+    for (int k = 0; k < AC; ++k) {
+      double *B = Jd + (size_t)sNode[a0 + k] * 324;
+      B[o_cd] = o1[k] + v1[k];
+      if (c != d) B[o_dc] = o2[k] + v2[k];
+    }
   }
   // constrained meshes: the raw local blocks that constraint lines fold into
   // the condensed diagonal blocks of masters (few cells have any)
