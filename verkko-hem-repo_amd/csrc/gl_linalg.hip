@@ -626,7 +626,7 @@ __global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__rest
 {
   __shared__ T sh[G][RT / 64];
   int nw = work_count(act, n_chunks), first = blockIdx.x, step = gridDim.x, grp = blockIdx.y;
-  if (!act.counts) {
+  if (!act.counts && (int)gridDim.x >= nw) {  // (a capped grid strides over the chunks instead)
     const int L = blockIdx.x + gridDim.x * blockIdx.y, r = L >> 3;
     grp = r % gridDim.y;
     first = 8 * (r / gridDim.y) + (L & 7);
@@ -1159,7 +1159,8 @@ hipError_t launch_copy(size_t nbytes, const void *src, void *dst, hipStream_t s)
 inline int chunk_grid(const SegInfo &S)
 {
   const int cap = S.act.chunk_cap > 0 ? S.act.chunk_cap : ACT_CHUNK_GRID;
-  return S.act.counts ? std::max(1, std::min(S.n_chunks, cap) >> S.act.shift) : S.n_chunks;
+  if (S.act.counts) return std::max(1, std::min(S.n_chunks, cap) >> S.act.shift);
+  return S.cap > 0 ? std::min(S.n_chunks, S.cap) : S.n_chunks;
 }
 inline int seg_grid(const SegInfo &S)
 {
@@ -1171,7 +1172,8 @@ hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, cons
                            int off, bool accumulate, hipStream_t s, const int *flag)
 {
   if (k <= 0 || S.n_chunks <= 0) return hipSuccess;
-  const int gx = S.act.counts ? chunk_grid(S) : (S.n_chunks + 7) / 8 * 8;  // XCD-numbered: a multiple of 8
+  int gx = S.act.counts ? chunk_grid(S) : (S.n_chunks + 7) / 8 * 8;  // XCD-numbered: a multiple of 8
+  if (!S.act.counts && S.cap > 0 && S.cap < S.n_chunks) gx = S.cap;  // grid-stride over the chunks
 #define GLH_MDOT(G)                                                                                              \
   hipLaunchKernelGGL((k_seg_mdot_partial<T, G>), dim3(gx, (k + G - 1) / G), dim3(RT), 0, s, S.chunks, S.n_chunks, \
                      S.act, k, V, ldv, w, partial, flag)

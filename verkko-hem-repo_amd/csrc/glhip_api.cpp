@@ -672,6 +672,10 @@ int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, i
                          c->row_cap, c->chunk_cap};
   glh::SegInfo SA = SI;
   SA.act = act;
+  // the second Gram-Schmidt pass rarely runs (its flag is usually off): a
+  // capped grid keeps the skipped multi-dot launch cheap
+  glh::SegInfo SR = SA;
+  SR.cap = 256;
   auto mv = [&](const TV *in, TV *out, const int *flag, glh::ActiveSet a) -> int {
     if (timed) return spmv<TJ, TV>(c, J, in, out, flag, a);
     (void)rs;
@@ -703,7 +707,7 @@ int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, i
     HIPCHK(c, glh::launch_seg_mdot<TV>(SA, j + 2, V.p, n, wv, pt.p, hs, ldh, 0, false, s, cyc));
     HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hs, ldh, wv, pt.p, nrm2, 1, 0, s, cyc));
     HIPCHK(c, glh::launch_gmres_ctl<TV>(C, 1, j, s));
-    HIPCHK(c, glh::launch_seg_mdot<TV>(SA, j + 1, V.p, n, wv, pt.p, hq, ldh, 0, false, s, ro));
+    HIPCHK(c, glh::launch_seg_mdot<TV>(SR, j + 1, V.p, n, wv, pt.p, hq, ldh, 0, false, s, ro));
     HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hq, ldh, wv, pt.p, nrm2, 1, 0, s, ro));
     HIPCHK(c, glh::launch_seg_scale<TV>(SA, wv, nrm2, 1, wv, s, cyc, true));
     HIPCHK(c, glh::launch_gmres_ctl<TV>(C, 2, j, s));

@@ -241,6 +241,11 @@ struct SegInfo {
   // and basis vectors in flight per thread of the update.  The sums run in
   // the same order for every tile, so the results do not depend on it.
   int gs = 8;
+  // grid cap of the chunk kernels without an active set (0: one workgroup
+  // per chunk): for launches that usually find their flag off (the second
+  // Gram-Schmidt pass), where dispatching a workgroup per chunk only to skip
+  // it is most of the cost
+  int cap = 0;
 };
 // flag (optional): instances whose flag is 0 are skipped (outputs untouched).
 template <typename T>
