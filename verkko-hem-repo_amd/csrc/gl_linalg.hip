@@ -901,10 +901,8 @@ __global__ __launch_bounds__(RT) void k_diag_shift(int n_rows, const int *__rest
 // flags); op 1: DGKS flag for column j; op 2: Givens update of column j;
 // op 3: back-substitution for y.
 template <typename T>
-__global__ void k_gmres_ctl(GmresCtl<T> C, int op, int j)
+__device__ inline void gmres_ctl_op(const GmresCtl<T> &C, int op, int j, int s)
 {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= C.S) return;
   const int m = C.m, ldh = C.ldh;
   double *g = C.g + (size_t)s * (m + 1);
   double *H = C.H + (size_t)s * (m + 1) * m;
@@ -961,6 +959,38 @@ __global__ void k_gmres_ctl(GmresCtl<T> C, int op, int j)
     for (int k = r + 1; k < ji; ++k) sum -= H[(size_t)r * m + k] * yd[k];
     yd[r] = sum / H[(size_t)r * m + r];
     y[r] = (T)yd[r];
+  }
+}
+
+template <typename T>
+__global__ void k_gmres_ctl(GmresCtl<T> C, int op, int j)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < C.S) gmres_ctl_op(C, op, j, s);
+}
+
+// The norm reduction of k_seg_final (one value per instance, in its order)
+// and then, on the same workgroup, control op 1 or 2 of that instance: one
+// launch instead of two per Gram-Schmidt pass.  The instances visited are the
+// listed ones (act) or all; the op of an instance runs whether or not its
+// reduction is flagged (op 1 clears the second-pass flag of an instance that
+// stopped iterating, op 2 returns at once for it).
+template <typename T>
+__global__ __launch_bounds__(RT) void k_seg_final_ctl(const int *__restrict__ seg_chunk, int n_seg, ActiveSet act,
+                                                      const T *__restrict__ partial, T *norm2,
+                                                      const int *__restrict__ flag, GmresCtl<T> C, int op, int j)
+{
+  __shared__ T sh[RT];
+  const int na = act.counts ? act.counts[0] : n_seg;
+  for (int a = blockIdx.x; a < na; a += gridDim.x) {
+    const int sg = act.counts ? act.seg[a] : a;
+    if (!flag || flag[sg]) {
+      T acc = 0;
+      for (int b = seg_chunk[sg] + threadIdx.x; b < seg_chunk[sg + 1]; b += RT) acc += partial[b];
+      const T r = block_reduce(acc, sh);
+      if (threadIdx.x == 0) norm2[sg] = r;
+    }
+    if (threadIdx.x == 0) gmres_ctl_op(C, op, j, sg);  // reads the norm this thread wrote
   }
 }
 
@@ -1189,7 +1219,8 @@ hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, cons
 
 template <typename T>
 hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size_t ldv, const T *h, int ldh, T *w,
-                                       T *partial, T *norm2, int ldo, int off, hipStream_t s, const int *flag)
+                                       T *partial, T *norm2, int ldo, int off, hipStream_t s, const int *flag,
+                                       const GmresCtl<T> *ctl, int op, int j)
 {
   if (S.n_chunks <= 0) return hipSuccess;
 #define GLH_SLN(UF)                                                                                              \
@@ -1200,8 +1231,14 @@ hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size
   else if (S.gs == 16) GLH_SLN(16);
   else return hipErrorInvalidValue;
 #undef GLH_SLN
-  hipLaunchKernelGGL((k_seg_final<T>), dim3(seg_grid(S), 1), dim3(RT), 0, s, S.seg_chunk, S.n_seg, S.act, 1, partial,
-                     norm2, ldo, off, 0, flag);
+  if (ctl) {
+    if (ldo != 1 || off != 0 || norm2 != ctl->nrm2) return hipErrorInvalidValue;  // the op reads the norms it gets
+    hipLaunchKernelGGL((k_seg_final_ctl<T>), dim3(seg_grid(S)), dim3(RT), 0, s, S.seg_chunk, S.n_seg, S.act, partial,
+                       norm2, flag, *ctl, op, j);
+  } else {
+    hipLaunchKernelGGL((k_seg_final<T>), dim3(seg_grid(S), 1), dim3(RT), 0, s, S.seg_chunk, S.n_seg, S.act, 1,
+                       partial, norm2, ldo, off, 0, flag);
+  }
   return hipGetLastError();
 }
 
@@ -1284,7 +1321,8 @@ hipError_t launch_seg_residual(const SegInfo &S, const T *b, const T *w, T *r, T
   template hipError_t launch_seg_mdot<T>(const SegInfo &, int, const T *, size_t, const T *, T *, T *, int, int,   \
                                          bool, hipStream_t, const int *);                                          \
   template hipError_t launch_seg_sub_lincomb_norm<T>(const SegInfo &, int, const T *, size_t, const T *, int, T *, \
-                                                     T *, T *, int, int, hipStream_t, const int *);                \
+                                                     T *, T *, int, int, hipStream_t, const int *,                 \
+                                                     const GmresCtl<T> *, int, int);                               \
   template hipError_t launch_seg_lincomb<T>(const SegInfo &, int, const T *, size_t, const T *, int, T *,          \
                                             hipStream_t, const int *);                                             \
   template hipError_t launch_seg_scale<T>(const SegInfo &, const T *, const T *, int, T *, hipStream_t,            \

@@ -704,13 +704,14 @@ int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, i
     TV *vj = V.p + n * j, *wv = V.p + n * (j + 1);
     HIPCHK(c, (glh::launch_bj_apply<TJ, TV>(c->n_nodes, Dinv, vj, Z.p, s, rs, cyc, act)));
     RCHK(mv(Z.p, wv, cyc, act));
+    // each pass's norm reduction runs the control op of its instance
+    // (op 1: second-pass flag; op 2: Givens update and stopping test), so an
+    // instance that stops here keeps V_{j+1} unnormalised: it is not used
     HIPCHK(c, glh::launch_seg_mdot<TV>(SA, j + 2, V.p, n, wv, pt.p, hs, ldh, 0, false, s, cyc));
-    HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hs, ldh, wv, pt.p, nrm2, 1, 0, s, cyc));
-    HIPCHK(c, glh::launch_gmres_ctl<TV>(C, 1, j, s));
+    HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hs, ldh, wv, pt.p, nrm2, 1, 0, s, cyc, &C, 1, j));
     HIPCHK(c, glh::launch_seg_mdot<TV>(SR, j + 1, V.p, n, wv, pt.p, hq, ldh, 0, false, s, ro));
-    HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hq, ldh, wv, pt.p, nrm2, 1, 0, s, ro));
+    HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hq, ldh, wv, pt.p, nrm2, 1, 0, s, ro, &C, 2, j));
     HIPCHK(c, glh::launch_seg_scale<TV>(SA, wv, nrm2, 1, wv, s, cyc, true));
-    HIPCHK(c, glh::launch_gmres_ctl<TV>(C, 2, j, s));
   }
   return GLH_OK;
 }
