@@ -384,8 +384,10 @@ int glh_residual(glh_ctx *ctx, double *norm);
  * scratch array of 65536 doubles per device (one per reduction chunk of 1024
  * dofs, 4096 in instances of 2^20 dofs and more), so meshes needing more
  * chunks are refused with GLH_ERR_ARG; every call on one device shares that
- * array, so calls that overlap on a device race.  The mesh and material come
- * from the context as for glh_residual. */
+ * array, so calls that overlap on a device race.  The cells' contributions
+ * go through the context's cell-slot buffer, allocated with the mesh by
+ * glh_upload_mesh (work on the same context is ordered on its stream).  The
+ * mesh and material come from the context as for glh_residual. */
 int glh_residual_into(glh_ctx *ctx, const double *U, double *R, double *norm2);
 
 /* FemGL::solve (solve.cc:108-187) with GMRES(restart) + 18x18 point-block
