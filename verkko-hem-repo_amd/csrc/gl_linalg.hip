@@ -693,17 +693,33 @@ __global__ __launch_bounds__(RT) void k_seg_final(const int *__restrict__ seg_ch
 template <typename T, int UF>
 __global__ __launch_bounds__(RT) void k_seg_sub_lincomb_norm(const long long *__restrict__ ch, int n_chunks,
                                                              ActiveSet act, int k, const T *__restrict__ V,
-                                                             size_t ldv, const T *__restrict__ h, int ldh,
-                                                             T *__restrict__ w, T *__restrict__ partial,
-                                                             const int *__restrict__ flag)
+                                                             size_t ldv, T *h, int ldh, T *__restrict__ w,
+                                                             T *__restrict__ partial, const int *__restrict__ flag,
+                                                             const int *__restrict__ seg_chunk,
+                                                             const T *__restrict__ hpart, int hk)
 {
   __shared__ T sh[RT];
+  __shared__ T shh[128];
   const int nw = work_count(act, n_chunks);
   for (int wi = blockIdx.x; wi < nw; wi += gridDim.x) {
     const int cb = work_chunk(act, wi);
     const long long sg = ch[3 * cb], beg = ch[3 * cb + 1], end = ch[3 * cb + 2];
     if (flag && !flag[sg]) continue;
     const T *hs = h + sg * ldh;
+    if (hpart) {
+      // the instance's hk multi-dot values from its chunk partials, in chunk
+      // order (every workgroup of the instance forms the same sums); the
+      // instance's first chunk stores them for the control ops
+      const int c0 = seg_chunk[sg], c1 = seg_chunk[sg + 1];
+      for (int i = threadIdx.x; i < hk; i += RT) {
+        T acc = 0;
+        for (int b = c0; b < c1; ++b) acc += hpart[(size_t)b * hk + i];
+        shh[i] = acc;
+        if (cb == c0) h[sg * ldh + i] = acc;
+      }
+      __syncthreads();
+      hs = shh;
+    }
     T acc = 0;
     // the updated pairs are stored once the whole chunk is done: a store
     // between the loads of the next pair holds them back (loads and stores
@@ -1199,7 +1215,7 @@ inline int seg_grid(const SegInfo &S)
 
 template <typename T>
 hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, const T *w, T *partial, T *out, int ldo,
-                           int off, bool accumulate, hipStream_t s, const int *flag)
+                           int off, bool accumulate, hipStream_t s, const int *flag, bool reduce)
 {
   if (k <= 0 || S.n_chunks <= 0) return hipSuccess;
   int gx = S.act.counts ? chunk_grid(S) : (S.n_chunks + 7) / 8 * 8;  // XCD-numbered: a multiple of 8
@@ -1212,20 +1228,22 @@ hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, cons
   else if (S.gs == 16) GLH_MDOT(16);
   else return hipErrorInvalidValue;
 #undef GLH_MDOT
+  if (!reduce) return hipGetLastError();  // the consumer sums the chunk partials itself
   hipLaunchKernelGGL((k_seg_final<T>), dim3(seg_grid(S), k), dim3(RT), 0, s, S.seg_chunk, S.n_seg, S.act, k, partial,
                      out, ldo, off, accumulate ? 1 : 0, flag);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size_t ldv, const T *h, int ldh, T *w,
+hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size_t ldv, T *h, int ldh, T *w,
                                        T *partial, T *norm2, int ldo, int off, hipStream_t s, const int *flag,
-                                       const GmresCtl<T> *ctl, int op, int j)
+                                       const GmresCtl<T> *ctl, int op, int j, const T *hpart, int hk)
 {
   if (S.n_chunks <= 0) return hipSuccess;
+  if (hpart && (hk < k || hk > 128 || hpart == partial)) return hipErrorInvalidValue;
 #define GLH_SLN(UF)                                                                                              \
   hipLaunchKernelGGL((k_seg_sub_lincomb_norm<T, UF>), dim3(chunk_grid(S)), dim3(RT), 0, s, S.chunks, S.n_chunks, \
-                     S.act, k, V, ldv, h, ldh, w, partial, flag)
+                     S.act, k, V, ldv, h, ldh, w, partial, flag, S.seg_chunk, hpart, hk)
   if (S.gs == 4) GLH_SLN(4);
   else if (S.gs == 8) GLH_SLN(8);
   else if (S.gs == 16) GLH_SLN(16);
@@ -1319,10 +1337,10 @@ hipError_t launch_seg_residual(const SegInfo &S, const T *b, const T *w, T *r, T
 
 #define GLH_SEG_INST(T)                                                                                         \
   template hipError_t launch_seg_mdot<T>(const SegInfo &, int, const T *, size_t, const T *, T *, T *, int, int,   \
-                                         bool, hipStream_t, const int *);                                          \
-  template hipError_t launch_seg_sub_lincomb_norm<T>(const SegInfo &, int, const T *, size_t, const T *, int, T *, \
+                                         bool, hipStream_t, const int *, bool);                                    \
+  template hipError_t launch_seg_sub_lincomb_norm<T>(const SegInfo &, int, const T *, size_t, T *, int, T *,       \
                                                      T *, T *, int, int, hipStream_t, const int *,                 \
-                                                     const GmresCtl<T> *, int, int);                               \
+                                                     const GmresCtl<T> *, int, int, const T *, int);               \
   template hipError_t launch_seg_lincomb<T>(const SegInfo &, int, const T *, size_t, const T *, int, T *,          \
                                             hipStream_t, const int *);                                             \
   template hipError_t launch_seg_scale<T>(const SegInfo &, const T *, const T *, int, T *, hipStream_t,            \

@@ -107,6 +107,7 @@ struct glh_ctx {
   std::vector<int> color_ptr;
   std::vector<int> node_off, cell_off;  // instance ranges
   int n_seg = 1, n_chunks = 0;
+  int max_seg_chunks = 0;  // reduction chunks of the largest instance
   // reference tables
   DBuf<double> tN, tdN, tw, tWp, tSp, tMf;
   DBuf<int> tpa, tpb;
@@ -162,8 +163,8 @@ struct glh_ctx {
   bool mf_sh_on = false;
   DBuf<int> status;
   // GMRES workspace
-  DBuf<double> gV64, gZ64, gW64, gh64, gp64, gy64, gs64, gq64;
-  DBuf<float> gV32, gZ32, gW32, gh32, gp32, gy32, gs32, gq32, gb32, gx32;
+  DBuf<double> gV64, gZ64, gW64, gh64, gp64, gy64, gs64, gq64, gn64;
+  DBuf<float> gV32, gZ32, gW32, gh32, gp32, gy32, gs32, gq32, gb32, gx32, gn32;
   HBuf<double> pin64, pinv;
   // device-side GMRES control (per instance)
   DBuf<double> gmH, gmcs, gmsn, gmg, gmbeta, gmtarget;
@@ -550,6 +551,7 @@ struct Work<double> {
   static DBuf<double> &h(glh_ctx *c) { return c->gh64; }
   static DBuf<double> &hp(glh_ctx *c) { return c->gq64; }
   static DBuf<double> &part(glh_ctx *c) { return c->gp64; }
+  static DBuf<double> &npart(glh_ctx *c) { return c->gn64; }
   static DBuf<double> &y(glh_ctx *c) { return c->gy64; }
   static DBuf<double> &sc(glh_ctx *c) { return c->gs64; }
   static HBuf<double> &pin(glh_ctx *c) { return c->pin64; }
@@ -563,6 +565,7 @@ struct Work<float> {
   static DBuf<float> &h(glh_ctx *c) { return c->gh32; }
   static DBuf<float> &hp(glh_ctx *c) { return c->gq32; }
   static DBuf<float> &part(glh_ctx *c) { return c->gp32; }
+  static DBuf<float> &npart(glh_ctx *c) { return c->gn32; }
   static DBuf<float> &y(glh_ctx *c) { return c->gy32; }
   static DBuf<float> &sc(glh_ctx *c) { return c->gs32; }
   static HBuf<float> &pin(glh_ctx *c) { return c->pin32; }
@@ -638,6 +641,10 @@ int spmv(glh_ctx *c, const TJ *J, const TV *x, TV *y, const int *flag = nullptr,
   return GLH_OK;
 }
 
+// batches whose instances have at most this many reduction chunks let each
+// Gram-Schmidt update workgroup sum its instance's multi-dot partials
+constexpr int FOLD_MAX_CHUNKS = 64;
+
 // Right-preconditioned restarted GMRES(m) with 18x18 point-block Jacobi and
 // classical Gram-Schmidt with selective (DGKS) reorthogonalisation, batched
 // over the instances of the context: every instance has its own Krylov
@@ -676,6 +683,8 @@ int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, i
   // capped grid keeps the skipped multi-dot launch cheap
   glh::SegInfo SR = SA;
   SR.cap = 256;
+  const bool fold = c->n_seg > 1 && c->max_seg_chunks <= FOLD_MAX_CHUNKS;
+  TV *np = Work<TV>::npart(c).p;  // the norm partials beside the multi-dot partials (fold)
   auto mv = [&](const TV *in, TV *out, const int *flag, glh::ActiveSet a) -> int {
     if (timed) return spmv<TJ, TV>(c, J, in, out, flag, a);
     (void)rs;
@@ -707,10 +716,21 @@ int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, i
     // each pass's norm reduction runs the control op of its instance
     // (op 1: second-pass flag; op 2: Givens update and stopping test), so an
     // instance that stops here keeps V_{j+1} unnormalised: it is not used
-    HIPCHK(c, glh::launch_seg_mdot<TV>(SA, j + 2, V.p, n, wv, pt.p, hs, ldh, 0, false, s, cyc));
-    HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hs, ldh, wv, pt.p, nrm2, 1, 0, s, cyc, &C, 1, j));
-    HIPCHK(c, glh::launch_seg_mdot<TV>(SR, j + 1, V.p, n, wv, pt.p, hq, ldh, 0, false, s, ro));
-    HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hq, ldh, wv, pt.p, nrm2, 1, 0, s, ro, &C, 2, j));
+    // (batches of small instances: each update workgroup sums its instance's
+    // multi-dot partials itself, instead of a reduction launch per pass)
+    if (fold) {
+      HIPCHK(c, glh::launch_seg_mdot<TV>(SA, j + 2, V.p, n, wv, pt.p, hs, ldh, 0, false, s, cyc, false));
+      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hs, ldh, wv, np, nrm2, 1, 0, s, cyc, &C, 1, j,
+                                                     pt.p, j + 2));
+      HIPCHK(c, glh::launch_seg_mdot<TV>(SR, j + 1, V.p, n, wv, pt.p, hq, ldh, 0, false, s, ro, false));
+      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hq, ldh, wv, np, nrm2, 1, 0, s, ro, &C, 2, j,
+                                                     pt.p, j + 1));
+    } else {
+      HIPCHK(c, glh::launch_seg_mdot<TV>(SA, j + 2, V.p, n, wv, pt.p, hs, ldh, 0, false, s, cyc));
+      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hs, ldh, wv, pt.p, nrm2, 1, 0, s, cyc, &C, 1, j));
+      HIPCHK(c, glh::launch_seg_mdot<TV>(SR, j + 1, V.p, n, wv, pt.p, hq, ldh, 0, false, s, ro));
+      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hq, ldh, wv, pt.p, nrm2, 1, 0, s, ro, &C, 2, j));
+    }
     HIPCHK(c, glh::launch_seg_scale<TV>(SA, wv, nrm2, 1, wv, s, cyc, true));
   }
   return GLH_OK;
@@ -749,6 +769,7 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
   HIPCHK(c, yb.alloc((size_t)S * ldh));
   HIPCHK(c, sc.alloc((size_t)S * 2));
   HIPCHK(c, pt.alloc((size_t)std::max(1, c->n_chunks) * (m + 2)));
+  HIPCHK(c, Work<TV>::npart(c).alloc((size_t)std::max(1, c->n_chunks)));
   HIPCHK(c, c->gmH.alloc((size_t)S * (m + 1) * m));
   HIPCHK(c, c->gmcs.alloc((size_t)S * m));
   HIPCHK(c, c->gmsn.alloc((size_t)S * m));
@@ -799,7 +820,7 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
   auto &gc = Work<TV>::graph(c);
   // every buffer and size the captured launches use
   const std::vector<const void *> key = {J,           Dinv,          b,           x,            V.p,
-                                         Z.p,         W.p,           pt.p,        hs.p,         hq.p,
+                                         Z.p,         W.p,           pt.p,        hs.p,         hq.p, Work<TV>::npart(c).p,
                                          yb.p,        sc.p,          c->gmH.p,    c->gmcs.p,    c->gmsn.p,
                                          c->gmg.p,    c->gmbeta.p,   c->gmtarget.p, c->gmcyc.p, c->gmconv.p,
                                          c->gmjseg.p, c->gmits.p,    c->gmro.p,   c->gmfrozen.p, c->gmjlast.p,
@@ -1088,10 +1109,10 @@ int glh_destroy(glh_ctx *c)
   DBuf<double> *dbs[] = {&c->tN,     &c->tdN,  &c->tw,   &c->tWp,  &c->tSp,   &c->tMf,   &c->cell_h,
                          &c->U,      &c->U0,   &c->dU,   &c->X,    &c->rhs,   &c->res,   &c->Fcell,
                          &c->s_norm, &c->s_alpha, &c->s_part, &c->J64, &c->D64, &c->gV64, &c->gZ64,
-                         &c->gW64,   &c->gh64, &c->gp64, &c->gy64, &c->gs64,  &c->gq64,  &c->ir_r};
+                         &c->gW64,   &c->gh64, &c->gp64, &c->gy64, &c->gs64,  &c->gq64,  &c->ir_r, &c->gn64};
   for (auto *b : dbs) b->release();
   DBuf<float> *fbs[] = {&c->J32,  &c->D32,  &c->gV32, &c->gZ32, &c->gW32, &c->gh32,
-                        &c->gp32, &c->gy32, &c->gs32, &c->gq32, &c->gb32, &c->gx32};
+                        &c->gp32, &c->gy32, &c->gs32, &c->gq32, &c->gb32, &c->gx32, &c->gn32};
   for (auto *b : fbs) b->release();
   DBuf<int> *ibs[] = {&c->tpa,      &c->tpb,         &c->cell_nodes, &c->row_ptr,  &c->col,       &c->diag,
                       &c->cell_blk, &c->color_cells, &c->status,     &c->cell_mat, &c->seg_chunk, &c->s_flag};
@@ -1696,6 +1717,8 @@ int glh_upload_mesh(glh_ctx *c, const glh_mesh *mm)
     segch.push_back((int)(ch.size() / 3));
   }
   c->n_chunks = (int)(ch.size() / 3);
+  c->max_seg_chunks = 0;
+  for (int sg = 0; sg < c->n_seg; ++sg) c->max_seg_chunks = std::max(c->max_seg_chunks, segch[sg + 1] - segch[sg]);
   HIPCHK(c, upload(c->cell_nodes, m.cell_nodes.data(), m.cell_nodes.size(), s));
   // node -> (cell * ns + local node) lists in cell order, for the gather of
   // the matrix-free product

@@ -250,18 +250,23 @@ struct SegInfo {
 // flag (optional): instances whose flag is 0 are skipped (outputs untouched).
 template <typename T>
 hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, const T *w, T *partial, T *out, int ldo,
-                           int off, bool accumulate, hipStream_t s, const int *flag = nullptr);
+                           int off, bool accumulate, hipStream_t s, const int *flag = nullptr, bool reduce = true);
 template <typename T>
 struct GmresCtl;
 // w -= sum_i h[s*ldh + i] V_i per instance; norm2[s*ldo + off] = ||w_s||^2 after.
 // With ctl (norm2 == ctl->nrm2, ldo 1, off 0), the workgroup that reduces an
 // instance's norm then runs GMRES control op `op` (1 or 2) for column j of
-// that instance, in place of a launch_gmres_ctl.
+// that instance, in place of a launch_gmres_ctl.  With hpart, the chunk
+// partials of a launch_seg_mdot of hk values (reduce = false) and not the
+// norm partials' buffer, each workgroup forms its instance's h from them
+// (chunk order) and the instance's first chunk stores h[s*ldh + i], i < hk:
+// for batches of small instances, where a separate reduction launch costs
+// more than the sums.
 template <typename T>
-hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size_t ldv, const T *h, int ldh, T *w,
+hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size_t ldv, T *h, int ldh, T *w,
                                        T *partial, T *norm2, int ldo, int off, hipStream_t s,
                                        const int *flag = nullptr, const GmresCtl<T> *ctl = nullptr, int op = 0,
-                                       int j = 0);
+                                       int j = 0, const T *hpart = nullptr, int hk = 0);
 // w = sum_{i < min(k, kseg[s])} y[s*ldy + i] V_i
 template <typename T>
 hipError_t launch_seg_lincomb(const SegInfo &S, int k, const T *V, size_t ldv, const T *y, int ldy, T *w,
