@@ -92,6 +92,32 @@ def test_batch_newton_equals_single_instance_solves(ctx):
         single.close()
 
 
+def test_batch_with_a_large_instance_equals_single_instance_solves(ctx):
+    # an instance of more than 64 reduction chunks (88k dofs) turns off the
+    # update workgroups' own sums of the multi-dot partials for the whole
+    # batch (separate reduction launches instead): same answers as alone
+    cases = [((16, 16, 16), 30.0, 20.0, 0.5), ((4, 4, 2), 6.0, 26.0, 0.7)]
+    meshes, batch, cos, Us = _batch(ctx, cases, seed0=40)
+    assert meshes[0].n_nodes * 18 > 64 * 1024
+    st = ctx.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=40)
+    ist = ctx.instance_stats()
+    say("batch", st)
+    assert st["converged"] == 1.0 and np.all(ist[:, 1] == 1)
+    Fb = ctx.instance_energy()
+    single = G.Context(0)
+    try:
+        for i, m in enumerate(meshes):
+            single.upload_mesh(m)
+            single.set_material(cos[i][0], cos[i][1:6], cos[i][6:9], cos[i][9])
+            single.set_solution(Us[i])
+            s1 = single.newton_solve(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=40)
+            say("single", i, s1)
+            assert s1["F"] == pytest.approx(Fb[i], rel=1e-10)
+            assert ist[i, 0] == s1["newton_steps"]
+    finally:
+        single.close()
+
+
 @pytest.mark.parametrize("fp32", [False, True])
 def test_matrix_free_batch_reaches_the_assembled_batch_fixed_point(ctx, fp32):
     # the matrix-free Jacobian on a batch: finished instances are skipped by
