@@ -929,7 +929,7 @@ __global__ __launch_bounds__(RT) void k_diag_shift(int n_rows, const int *__rest
 // ----------------------------------------- device-side GMRES control (batched)
 // One thread per instance.  op 0: start of a restart cycle (beta, g, cycle
 // flags); op 1: DGKS flag for column j; op 2: Givens update of column j;
-// op 3: back-substitution for y.
+// op 3, the back-substitution for y, has a kernel of its own below.
 template <typename T>
 __device__ inline void gmres_ctl_op(const GmresCtl<T> &C, int op, int j, int s)
 {
@@ -979,17 +979,7 @@ __device__ inline void gmres_ctl_op(const GmresCtl<T> &C, int op, int j, int s)
     if (fabs(g[j + 1]) <= C.target[s] || hn == 0.0) C.cyc[s] = 0;
     return;
   }
-  // op 3: y = R^-1 g on the first jseg columns, zero beyond
-  const int ji = C.jseg[s];
-  T *y = C.y + (size_t)s * ldh;
-  for (int r = ji; r < ldh; ++r) y[r] = (T)0;
-  double yd[128];
-  for (int r = ji - 1; r >= 0; --r) {
-    double sum = g[r];
-    for (int k = r + 1; k < ji; ++k) sum -= H[(size_t)r * m + k] * yd[k];
-    yd[r] = sum / H[(size_t)r * m + r];
-    y[r] = (T)yd[r];
-  }
+  // op 3 (the back-substitution) is k_gmres_backsub
 }
 
 template <typename T>
@@ -997,6 +987,36 @@ __global__ void k_gmres_ctl(GmresCtl<T> C, int op, int j)
 {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < C.S) gmres_ctl_op(C, op, j, s);
+}
+
+// Control op 3 (y = R^-1 g on the first jseg columns, zero beyond) with one
+// wavefront per instance: each row's sum over the columns already solved is
+// formed by the lanes from a coalesced read of the row of R and a wavefront
+// sum (one thread per instance, reading its rows serially, took about 0.2 ms
+// for a batch of 256 instances at 100 columns).
+template <typename T>
+__global__ __launch_bounds__(64) void k_gmres_backsub(GmresCtl<T> C)
+{
+  __shared__ double ys[128];
+  const int s = blockIdx.x, lane = threadIdx.x;
+  if (s >= C.S) return;
+  const int m = C.m, ldh = C.ldh, ji = C.jseg[s];
+  const double *g = C.g + (size_t)s * (m + 1);
+  const double *H = C.H + (size_t)s * (m + 1) * m;
+  T *y = C.y + (size_t)s * ldh;
+  for (int r = ji + lane; r < ldh; r += 64) y[r] = (T)0;
+  for (int r = ji - 1; r >= 0; --r) {
+    double part = 0.0;
+    for (int k = r + 1 + lane; k < ji; k += 64) part += H[(size_t)r * m + k] * ys[k];
+    part = wave_sum(part);
+    if (lane == 0) {
+      const double yr = (g[r] - part) / H[(size_t)r * m + r];
+      ys[r] = yr;
+      y[r] = (T)yr;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // ys[r] before the next row reads it
+    __syncthreads();
+  }
 }
 
 // The norm reduction of k_seg_final (one value per instance, in its order)
@@ -1329,7 +1349,8 @@ hipError_t launch_seg_zero(const SegInfo &S, const int *flag, T *v, hipStream_t 
 template <typename T>
 hipError_t launch_gmres_ctl(const GmresCtl<T> &C, int op, int j, hipStream_t s)
 {
-  hipLaunchKernelGGL((k_gmres_ctl<T>), dim3((C.S + 63) / 64), dim3(64), 0, s, C, op, j);
+  if (op == 3) hipLaunchKernelGGL((k_gmres_backsub<T>), dim3(C.S), dim3(64), 0, s, C);
+  else hipLaunchKernelGGL((k_gmres_ctl<T>), dim3((C.S + 63) / 64), dim3(64), 0, s, C, op, j);
   return hipGetLastError();
 }
 
