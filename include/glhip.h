@@ -492,10 +492,13 @@ int glh_set_continuation(glh_ctx *ctx, double sigma0);
 int glh_set_matrix_free(glh_ctx *ctx, int on);
 int glh_set_verbose(glh_ctx *ctx, int level);
 /* Live kernel timing with HIP events on the context stream.  on != 0 resets
- * the counters and records an event pair around every SpMV launch and every
- * assembly sweep from then on.  out: [0] total SpMV ms, [1] SpMV launches,
- * [2] total Jacobian(+rhs) assembly ms, [3] such sweeps, [4] total
- * residual-only assembly ms, [5] such sweeps. */
+ * the counters and records an event pair around every Jacobian product
+ * (BSR SpMV, or the matrix-free product and its gather) and every assembly
+ * from then on; GMRES cycles are then launched directly instead of replayed
+ * from captured graphs.  out: [0] total product ms, [1] products, [2] total
+ * Jacobian (+ residual) assembly ms, [3] such assemblies, [4] total
+ * residual-only ms (the residual's cell pass and node gather, and on a
+ * constrained mesh its condensation), [5] such residuals. */
 int glh_set_timing(glh_ctx *ctx, int on);
 int glh_timing(glh_ctx *ctx, double out[6]);
 /* Run k back-to-back SpMV launches y = J x with x = current solution and
