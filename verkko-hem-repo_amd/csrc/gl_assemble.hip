@@ -154,48 +154,70 @@ __device__ inline void rho_pair(const TA *x, int mu, int k, const Material &M, d
 //   d|u|^2 = 2 u:a, d|v|^2 = 2 v:b, d(u:v) = u:b + a:v,
 //   d(u u^T) = a u^T + u a^T, d(v v^T) = b v^T + v b^T, d(u v^T) = a v^T + u b^T,
 //   d(P u) = dP u + P a, ...  with (a, b) the u and v parts of x.
+// In two stages, so that a point's 9 items share the sums through LDS
+// instead of each forming all of them.  The row-pair sums are symmetric in
+// (mu, nu) but for W and dW, which change sign:
+//   uu, vv symmetric; uvmn(nu, mu) = uvnm(mu, nu), so Q is symmetric and W
+//   antisymmetric; likewise duu, dvv, dQ and dW.
+// The six pairs mu <= nu, numbered (0,0) (0,1) (0,2) (1,1) (1,2) (2,2), hold
+// 8 values each, P S Q W dP dS dQ dW; the pairs (mu, mu) also give the 9-term
+// sums: D = sum_mu P, N = sum_mu S, u:v = sum_mu Q / 2, dD = sum_mu dP,
+// dN = sum_mu dS, d(u:v) = sum_mu dQ / 2.
+// Stage 1: the 8 sums of pair p at the state s in the direction x, to r.
 template <typename TA>
-__device__ inline void hx_pair(const TA *s, const TA *x, int mu, int k, const Material &M, TA &zu, TA &zv)
+__device__ inline void hx_row_sums(const TA *s, const TA *x, int p, TA *r)
 {
-  // s and x are read from LDS where they are used (the 36 values held at
-  // once would spill at 4 waves per SIMD)
-  TA U2 = 0, V2 = 0, UV = 0, ua = 0, vb = 0, dUV = 0;
-#pragma unroll 1
-  for (int i = 0; i < 9; ++i) {
-    const TA u = s[i], v = s[9 + i], a = x[i], b = x[9 + i];
-    U2 += u * u;
-    V2 += v * v;
-    UV += u * v;
-    ua += u * a;
-    vb += v * b;
-    dUV += u * b;
-    dUV += a * v;
-  }
-  const TA D = U2 - V2, N = U2 + V2, dD = (TA)2 * (ua - vb), dN = (TA)2 * (ua + vb);
-  TA um[3], vm[3], am[3], bm[3];  // row mu of u, v, a, b
+  const int mu = p < 3 ? 0 : (p < 5 ? 1 : 2), nu = p < 3 ? p : (p < 5 ? p - 2 : 2);
+  TA uu = 0, vv = 0, uvmn = 0, uvnm = 0, duu = 0, dvv = 0, duvmn = 0, duvnm = 0;
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    um[j] = s[3 * mu + j]; vm[j] = s[9 + 3 * mu + j];
-    am[j] = x[3 * mu + j]; bm[j] = x[9 + 3 * mu + j];
+    const TA um = s[3 * mu + j], vm = s[9 + 3 * mu + j], am = x[3 * mu + j], bm = x[9 + 3 * mu + j];
+    const TA un = s[3 * nu + j], vn = s[9 + 3 * nu + j], an = x[3 * nu + j], bn = x[9 + 3 * nu + j];
+    uu += um * un;
+    vv += vm * vn;
+    uvmn += um * vn;
+    uvnm += un * vm;
+    duu += am * un + um * an;
+    dvv += bm * vn + vm * bn;
+    duvmn += am * vn + um * bn;
+    duvnm += an * vm + un * bm;
   }
-  TA dPu = 0, dPv = 0, dQu = 0, dQv = 0, dSu = 0, dSv = 0, dWu = 0, dWv = 0;
-#pragma unroll 1
-  for (int nu = 0; nu < 3; ++nu) {  // not unrolled: its LDS reads would all be hoisted and spill
-    TA uu = 0, vv = 0, uvmn = 0, uvnm = 0, duu = 0, dvv = 0, duvmn = 0, duvnm = 0;
+  r[0] = uu - vv;
+  r[1] = uu + vv;
+  r[2] = uvmn + uvnm;
+  r[3] = uvnm - uvmn;
+  r[4] = duu - dvv;
+  r[5] = duu + dvv;
+  r[6] = duvmn + duvnm;
+  r[7] = duvnm - duvmn;
+}
+
+// Stage 2: H(s) x for the components (mu, k) of u (zu) and of v (zv) from the
+// point's 48 pair sums R (stage 1) and its s and x.
+template <typename TA>
+__device__ inline void hx_from_sums(const TA *R, const TA *s, const TA *x, int mu, int k, const Material &M, TA &zu,
+                                    TA &zv)
+{
+  // the 9-term sums from the pairs (0,0), (1,1), (2,2)
+  TA D = 0, N = 0, Q2 = 0, dD = 0, dN = 0, dQ2 = 0;
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const TA un = s[3 * nu + j], vn = s[9 + 3 * nu + j], an = x[3 * nu + j], bn = x[9 + 3 * nu + j];
-      uu += um[j] * un;
-      vv += vm[j] * vn;
-      uvmn += um[j] * vn;
-      uvnm += un * vm[j];
-      duu += am[j] * un + um[j] * an;
-      dvv += bm[j] * vn + vm[j] * bn;
-      duvmn += am[j] * vn + um[j] * bn;
-      duvnm += an * vm[j] + un * bm[j];
-    }
-    const TA P = uu - vv, S = uu + vv, Q = uvmn + uvnm, W = uvnm - uvmn;
-    const TA dP = duu - dvv, dS = duu + dvv, dQ = duvmn + duvnm, dW = duvnm - duvmn;
+  for (int m = 0; m < 3; ++m) {
+    const TA *r = R + 8 * (m == 0 ? 0 : (m == 1 ? 3 : 5));
+    D += r[0];
+    N += r[1];
+    Q2 += r[2];
+    dD += r[4];
+    dN += r[5];
+    dQ2 += r[6];
+  }
+  const TA UV = (TA)0.5 * Q2, dUV = (TA)0.5 * dQ2;
+  TA dPu = 0, dPv = 0, dQu = 0, dQv = 0, dSu = 0, dSv = 0, dWu = 0, dWv = 0;
+#pragma unroll
+  for (int nu = 0; nu < 3; ++nu) {
+    const int lo = mu < nu ? mu : nu, hi = mu < nu ? nu : mu;
+    const TA *r = R + 8 * (lo == 0 ? hi : (lo == 1 ? hi + 2 : 5));
+    const TA sg = mu > nu ? (TA)-1 : (TA)1;  // W and dW of the pair (mu, nu)
+    const TA P = r[0], S = r[1], Q = r[2], W = sg * r[3], dP = r[4], dS = r[5], dQ = r[6], dW = sg * r[7];
     const TA un = s[3 * nu + k], vn = s[9 + 3 * nu + k], an = x[3 * nu + k], bn = x[9 + 3 * nu + k];
     dPu += dP * un + P * an; dPv += dP * vn + P * bn;
     dQu += dQ * un + Q * an; dQv += dQ * vn + Q * bn;
@@ -573,10 +595,12 @@ __global__ __launch_bounds__(NT) void k_mf_setup(RefTables T, MeshDev Mh, Materi
 // component) with the bases in LDS.  At the points, H(q) X(q) is not read
 // from a stored Hessian (171 values per point) but formed as the derivative
 // of rho at the stored iterate U(q) (18 values per point) in the direction
-// X(q) (hx_pair): the operator's state shrinks from 1.21 GB to 0.13 GB on the
-// 32^3 Q2 cube, and the kernel is bound by its arithmetic and LDS passes
-// instead of the Hessian stream.  A thread owns the components c and c + 9
-// of one point (the same orbital entry in u and in v).
+// X(q) (hx_row_sums, hx_from_sums): the operator's state shrinks from
+// 1.21 GB to 0.13 GB on the 32^3 Q2 cube, and the kernel is bound by its
+// arithmetic and LDS passes instead of the Hessian stream.  At the points a
+// thread owns the components c and c + 9 of one point (the same orbital
+// entry in u and in v), after six threads per point have formed the point's
+// row-pair sums.
 // TH: the stored iterate (FP64, or FP32 for the FP32 inner solve); TX: x.
 // TA, the arithmetic, LDS and cell-slot type, is FP32 when both are FP32 (the
 // FP32 inner solve, with FP32 accumulation) and FP64 otherwise.  cells[]
@@ -615,15 +639,17 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
   //   x -> A0 A1 -> B0 B1 B2 -> X G0 G1 G2 -> Z K0 K1 K2 -> C0 C1 C2 -> D0 D1,
   // and the iterate U(q) of the product.
   // A pencil reads exactly the points it writes, so the FP32 kernel lets each
-  // pass overwrite its inputs in place in one set of four regions (9.9 KB for
-  // Q2 with U(q): register-limited at 6 waves per SIMD).  The FP64 kernel
-  // alternates between two sets and keeps x (39 KB, 4 workgroups per CU),
-  // which saves a barrier and the Robin faces' second gather of x.
+  // pass overwrite its inputs in place in one set of four regions, and keeps
+  // the point stage's pair sums and Z in four more (17.7 KB for Q2 with U(q):
+  // register-limited at 6 waves per SIMD).  The FP64 kernel alternates
+  // between two sets and keeps x (39 KB, 4 workgroups per CU), which saves a
+  // barrier and the Robin faces' second gather of x.
   constexpr bool IP = std::is_same<TA, float>::value;
   constexpr int NXR = (V + NTA - 1) / NTA;  // x values per thread
   // room for the unconditional x and U(q) stores, U(q) after x
   constexpr int SU_OFF = (IP ? 3 * V : 8 * V) + NXR * NTA;
-  __shared__ TA S[SU_OFF + (RES ? 0 : NXR * NTA)];
+  constexpr int RS_OFF = SU_OFF + (RES ? 0 : NXR * NTA);
+  __shared__ TA S[RS_OFF + (IP && !RES ? 4 * V : 0)];
   TA *const P0 = S, *const P1 = IP ? S : S + 4 * V, *const SX = IP ? S : S + 8 * V, *const SU = S + SU_OFF;
   const int tid = threadIdx.x;
   const int e = cells[blockIdx.x];
@@ -755,38 +781,61 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
   }
   __syncthreads();
   // w vol H(q) X(q) and w vol gamma_k(q) / h_k for the components hc and hc + 9
-  // of point hq: read everything, then write Z and K (over X and G in place)
+  // of point hq: read everything, then write Z and K.
+  // SPLIT (the product): H(q) X(q) from the point's shared pair sums
+  // (hx_row_sums, hx_from_sums), one more barrier.  Regions: the z pass left
+  // X G0 G1 G2 in P1; the pair sums (54 per point, 3V in all) go to RS, then
+  // Z to Zs = RS + 3V and K over G (Ks = P1 + V); the test passes write C to
+  // P0 and D to P1.  FP64: RS is P0, dead after the z pass; FP32: RS is a
+  // region of its own, and P0 = P1.  The residual (not SPLIT) writes Z and K
+  // over the regions of P0.
+  constexpr bool SPLIT = !RES;
+  TA *const RS = IP ? S + RS_OFF : P0;
+  TA *const Zs = SPLIT ? RS + 3 * V : P0, *const Ks = SPLIT ? P1 + V : P0 + V;
+  TA *const Cd = SPLIT ? P0 : P1, *const Dd = SPLIT ? P1 : P0;
   TA zo[2], ko[2][3];
   if (tid < NH) {
-    const TA *X = &P1[hq * 18];
-    TA z[2] = {0, 0};
-    if constexpr (RES) {
-      double ru, rv;
-      rho_pair(X, hc / 3, hc % 3, M, ru, rv);
-      z[0] = ru;
-      z[1] = rv;
-    } else {
-      hx_pair(SU + hq * 18, X, hc / 3, hc % 3, M, z[0], z[1]);
-    }
     const TA wv = (TA)(T.w[hq] * vol), K1 = (TA)M.K1, K23 = (TA)M.K23;
     const int mu = hc / 3, kc = hc % 3;
+    if constexpr (RES) {  // rho first: fewer registers live through it
+      double ru, rv;
+      rho_pair(&P1[hq * 18], mu, kc, M, ru, rv);
+      zo[0] = wv * ru;
+      zo[1] = wv * rv;
+    }
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const int i = hq * 18 + hc + 9 * r, base = hq * 18 + 9 * r + 3 * mu;
-      zo[r] = wv * z[r];
       const TA dv = P1[V + base] + P1[2 * V + base + 1] + P1[3 * V + base + 2];
 #pragma unroll
       for (int k = 0; k < 3; ++k) ko[r][k] = wv * ih[k] * (K1 * P1[(1 + k) * V + i] + (k == kc ? K23 * dv : (TA)0));
     }
   }
-  if constexpr (IP) __syncthreads();
+  if constexpr (SPLIT) {
+    // pair p of point tid / 6 (162 items for Q2), then the 9 items per point;
+    // 54 values per point (48 used): an odd multiple of 2 words, so the
+    // points of a wave read distinct banks
+    if (tid < NQ * 6) {
+      const int sq = tid / 6, p = tid % 6;
+      hx_row_sums(SU + sq * 18, P1 + sq * 18, p, RS + sq * 54 + p * 8);
+    }
+    __syncthreads();
+    if (tid < NH) {
+      TA z[2];
+      hx_from_sums(RS + hq * 54, SU + hq * 18, P1 + hq * 18, hc / 3, hc % 3, M, z[0], z[1]);
+      const TA wv = (TA)(T.w[hq] * vol);
+      zo[0] = wv * z[0];
+      zo[1] = wv * z[1];
+    }
+  }
+  if constexpr (IP && !SPLIT) __syncthreads();
   if (tid < NH) {
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const int i = hq * 18 + hc + 9 * r;
-      P0[i] = zo[r];
+      Zs[i] = zo[r];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) P0[(1 + k) * V + i] = ko[r][k];
+      for (int k = 0; k < 3; ++k) Ks[k * V + i] = ko[r][k];
     }
   }
   __syncthreads();
@@ -796,10 +845,10 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
       const int o = at(la, lb, q, c);
-      z[q] = P0[o];
-      k0[q] = P0[V + o];
-      k1[q] = P0[2 * V + o];
-      k2[q] = P0[3 * V + o];
+      z[q] = Zs[o];
+      k0[q] = Ks[o];
+      k1[q] = Ks[V + o];
+      k2[q] = Ks[2 * V + o];
     }
 #pragma unroll
     for (int a = 0; a < N1; ++a) {
@@ -811,9 +860,9 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
         c2 += L[q][a] * k1[q];
       }
       const int o = at(la, lb, a, c);
-      P1[o] = c0;
-      P1[V + o] = c1;
-      P1[2 * V + o] = c2;
+      Cd[o] = c0;
+      Cd[V + o] = c1;
+      Cd[2 * V + o] = c2;
     }
   }
   __syncthreads();
@@ -823,9 +872,9 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
       const int o = at(la, q, lb, c);
-      v0[q] = P1[o];
-      v1[q] = P1[V + o];
-      v2[q] = P1[2 * V + o];
+      v0[q] = Cd[o];
+      v1[q] = Cd[V + o];
+      v2[q] = Cd[2 * V + o];
     }
 #pragma unroll
     for (int a = 0; a < N1; ++a) {
@@ -835,8 +884,8 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
         d0 += L[q][a] * v0[q] + dL[q][a] * v2[q];
         d1 += L[q][a] * v1[q];
       }
-      P0[at(la, a, lb, c)] = d0;
-      P0[V + at(la, a, lb, c)] = d1;
+      Dd[at(la, a, lb, c)] = d0;
+      Dd[V + at(la, a, lb, c)] = d1;
     }
   }
   __syncthreads();
@@ -845,8 +894,8 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
     TA v0[N1], v1[N1];
 #pragma unroll
     for (int q = 0; q < N1; ++q) {
-      v0[q] = P0[at(q, la, lb, c)];
-      v1[q] = P0[V + at(q, la, lb, c)];
+      v0[q] = Dd[at(q, la, lb, c)];
+      v1[q] = Dd[V + at(q, la, lb, c)];
     }
 #pragma unroll
     for (int a = 0; a < N1; ++a) {
