@@ -813,8 +813,9 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
   }
   if constexpr (SPLIT) {
     // pair p of point tid / 6 (162 items for Q2), then the 9 items per point;
-    // 54 values per point (48 used): an odd multiple of 2 words, so the
-    // points of a wave read distinct banks
+    // 54 values per point (48 used) keep each pair's 8 values 16-byte aligned
+    // for 128-bit LDS accesses; of the strides 48-54 it ran fastest (an odd
+    // stride, 49, cost 10% of the kernel in bank conflicts)
     if (tid < NQ * 6) {
       const int sq = tid / 6, p = tid % 6;
       hx_row_sums(SU + sq * 18, P1 + sq * 18, p, RS + sq * 54 + p * 8);
