@@ -72,7 +72,7 @@ struct glh_mlp {
   std::vector<long long> woff, boff;     // offsets in the flat parameters
   long long n_params = 0;
   Buf<float> P, G, Mo, Vo, Bf;
-  std::vector<Buf<bf16>> Wb;
+  std::vector<Buf<bf16>> Wb, WbT;  // BF16 weights, K x Np and transposed Np x K
   Buf<float> Xall, Yall;
   long long n_data = 0;
   // batch workspace
@@ -110,7 +110,8 @@ int fail(glh_mlp *m, int code, const std::string &msg)
 int pack_all(glh_mlp *m)
 {
   for (int l = 0; l <= m->depth; ++l)
-    MCHK(m, glh::mlp::pack_w(m->K[l], m->N[l], m->Np[l], m->P.p + m->woff[l], m->Wb[l].p, m->stream));
+    MCHK(m, glh::mlp::pack_w(m->K[l], m->N[l], m->Np[l], m->P.p + m->woff[l], m->Wb[l].p, m->WbT[l].p,
+                             m->stream));
   return GLH_OK;
 }
 
@@ -191,7 +192,7 @@ int forward_backward(glh_mlp *m, int n_real, bool grad)
     E.ldz = W;
     E.H = m->H[l + 1].p;
     E.ldh = W;
-    MCHK(m, glh::mlp::gemm(nb, W, m->K[l], m->H[l].p, true, m->K[l], m->Wb[l].p, false, m->Np[l], m->Z[l].p, W,
+    MCHK(m, glh::mlp::gemm(nb, W, m->K[l], m->H[l].p, true, m->K[l], m->WbT[l].p, true, m->K[l], m->Z[l].p, W,
                            false, 1, 0, s, E));
   }
   {
@@ -199,7 +200,7 @@ int forward_backward(glh_mlp *m, int n_real, bool grad)
     E.mode = glh::mlp::GEMM_BIAS;
     E.bias = m->P.p + m->boff[D];
     E.nbias = m->n_out;
-    MCHK(m, glh::mlp::gemm(nb, OUTP, m->K[D], m->H[D].p, true, m->K[D], m->Wb[D].p, false, OUTP, m->Yo.p, OUTP,
+    MCHK(m, glh::mlp::gemm(nb, OUTP, m->K[D], m->H[D].p, true, m->K[D], m->WbT[D].p, true, m->K[D], m->Yo.p, OUTP,
                            false, 1, 0, s, E));
   }
   if (!grad) return GLH_OK;
@@ -244,10 +245,13 @@ int train_step(glh_mlp *m, uint64_t seed, float lr)
   MCHK(m, glh::mlp::gather(m->nb, m->n_data, seed, m->step.p, m->Xall.p, m->Yall.p, m->n_out, m->Xb.p, m->Yb.p, s));
   RCHK(forward_backward(m, m->nb, true));
   // Adam, and the BF16 copies of the updated weights for the next step
-  std::vector<bf16 *> wb(m->depth + 1);
-  for (int l = 0; l <= m->depth; ++l) wb[l] = m->Wb[l].p;
+  std::vector<bf16 *> wb(m->depth + 1), wbt(m->depth + 1);
+  for (int l = 0; l <= m->depth; ++l) {
+    wb[l] = m->Wb[l].p;
+    wbt[l] = m->WbT[l].p;
+  }
   MCHK(m, glh::mlp::adam_pack(m->n_params, m->P.p, m->G.p, m->Mo.p, m->Vo.p, lr, 0.9f, 0.999f, 1e-8f, m->step.p,
-                              grad_layout(m, 1), wb.data(), s));
+                              grad_layout(m, 1), wb.data(), wbt.data(), s));
   return GLH_OK;
 }
 
@@ -306,7 +310,11 @@ int glh_mlp_create(int device, int n_freq, double sigma, int width, int depth, i
   chk(m->Bf.alloc(B.size()));
   chk(m->step.alloc(1));
   m->Wb.resize(depth + 1);
-  for (int l = 0; l <= depth; ++l) chk(m->Wb[l].alloc((size_t)m->K[l] * m->Np[l]));
+  m->WbT.resize(depth + 1);
+  for (int l = 0; l <= depth; ++l) {
+    chk(m->Wb[l].alloc((size_t)m->K[l] * m->Np[l]));
+    chk(m->WbT[l].alloc((size_t)m->K[l] * m->Np[l]));
+  }
   if (rc == GLH_OK) {
     chk(hipMemcpyAsync(m->P.p, P.data(), P.size() * sizeof(float), hipMemcpyHostToDevice, m->stream));
     chk(hipMemcpyAsync(m->Bf.p, B.data(), B.size() * sizeof(float), hipMemcpyHostToDevice, m->stream));
@@ -335,6 +343,7 @@ int glh_mlp_destroy(glh_mlp *m)
                       &m->Yo, &m->part, &m->cpart, &m->lpart, &m->loss};
   for (auto *b : fb) b->release();
   for (auto &b : m->Wb) b.release();
+  for (auto &b : m->WbT) b.release();
   for (auto &b : m->H) b.release();
   for (auto &b : m->Z) b.release();
   m->dY.release();

@@ -245,11 +245,12 @@ __global__ __launch_bounds__(ET) void k_grad_reduce(long long n, GradLayout L, c
 }
 
 // Adam on the FP32 master parameters (bias-corrected, step t >= 1); a weight
-// also refreshes its BF16 copy Wb[l][k Np + n] (the padding stays 0)
+// also refreshes its BF16 copies Wb[l][k Np + n] and WbT[l][n K + k] (the
+// padding stays 0)
 __global__ __launch_bounds__(ET) void k_adam_pack(long long n, float *__restrict__ p, const float *__restrict__ g,
                                                   float *__restrict__ m, float *__restrict__ v, float lr, float b1,
                                                   float b2, float eps, const int *__restrict__ step, GradLayout L,
-                                                  WbPtrs Wb)
+                                                  WbPtrs Wb, WbPtrs WbT)
 {
   const long long i = (long long)blockIdx.x * ET + threadIdx.x;
   if (i >= n) return;
@@ -263,19 +264,24 @@ __global__ __launch_bounds__(ET) void k_adam_pack(long long n, float *__restrict
   p[i] = pn;
   const int l = param_layer(L, i);
   if (i < L.boff[l]) {
-    const long long r = i - L.woff[l];
-    Wb.p[l][(r / L.N[l]) * L.Np[l] + r % L.N[l]] = to_bf16(pn);
+    const long long r = i - L.woff[l], k = r / L.N[l], c = r % L.N[l];
+    const bf16 h = to_bf16(pn);
+    Wb.p[l][k * L.Np[l] + c] = h;
+    WbT.p[l][c * L.K[l] + k] = h;
   }
 }
 
-// BF16 copy of a K x N FP32 weight into a K x Np padded matrix (zeros beyond N)
+// BF16 copies of a K x N FP32 weight: K x Np and its transpose Np x K
+// (zeros beyond N)
 __global__ __launch_bounds__(ET) void k_pack_w(int K, int N, int Np, const float *__restrict__ W,
-                                               bf16 *__restrict__ Wb)
+                                               bf16 *__restrict__ Wb, bf16 *__restrict__ WbT)
 {
   const long long i = (long long)blockIdx.x * ET + threadIdx.x;
   if (i >= (long long)K * Np) return;
   const int k = (int)(i / Np), c = (int)(i % Np);
-  Wb[i] = to_bf16(c < N ? W[(size_t)k * N + c] : 0.f);
+  const bf16 h = to_bf16(c < N ? W[(size_t)k * N + c] : 0.f);
+  Wb[i] = h;
+  WbT[(size_t)c * K + k] = h;
 }
 
 // random training batch: sample i of step t is point hash(seed, t, i) mod n_data
@@ -327,8 +333,8 @@ hipError_t gemm(int M, int N, int K, const bf16 *A, bool a_kcontig, long long sa
 #define GLH_GEMM(AK, BK_, EPI, CS) \
   hipLaunchKernelGGL((k_gemm<AK, BK_, EPI, CS>), grid, dim3(256), 0, s, M, N, kchunk, A, sa, B, sb, C, ldc, c_split, acc, E)
   // the combinations the network uses
-  if (E.mode == GEMM_BIAS_SILU && a_kcontig && !b_kcontig) GLH_GEMM(true, false, GEMM_BIAS_SILU, false);
-  else if (E.mode == GEMM_BIAS && a_kcontig && !b_kcontig) GLH_GEMM(true, false, GEMM_BIAS, false);
+  if (E.mode == GEMM_BIAS_SILU && a_kcontig && b_kcontig) GLH_GEMM(true, true, GEMM_BIAS_SILU, false);
+  else if (E.mode == GEMM_BIAS && a_kcontig && b_kcontig) GLH_GEMM(true, true, GEMM_BIAS, false);
   else if (E.mode == GEMM_SILU_BACK && a_kcontig && b_kcontig) GLH_GEMM(true, true, GEMM_SILU_BACK, false);
   else if (E.mode != GEMM_STORE) return hipErrorInvalidValue;
   else if (E.colsum) {
@@ -352,12 +358,15 @@ hipError_t grad_reduce(long long n_params, const GradLayout &L, const float *par
 }
 
 hipError_t adam_pack(long long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
-                     const int *step, const GradLayout &L, bf16 *const *Wb, hipStream_t s)
+                     const int *step, const GradLayout &L, bf16 *const *Wb, bf16 *const *WbT, hipStream_t s)
 {
   if (L.n_layers < 1 || L.n_layers > MLP_MAX_LAYERS) return hipErrorInvalidValue;
-  WbPtrs W{};
-  for (int l = 0; l < L.n_layers; ++l) W.p[l] = Wb[l];
-  hipLaunchKernelGGL(k_adam_pack, dim3(grid1(n)), dim3(ET), 0, s, n, p, g, m, v, lr, b1, b2, eps, step, L, W);
+  WbPtrs W{}, WT{};
+  for (int l = 0; l < L.n_layers; ++l) {
+    W.p[l] = Wb[l];
+    WT.p[l] = WbT[l];
+  }
+  hipLaunchKernelGGL(k_adam_pack, dim3(grid1(n)), dim3(ET), 0, s, n, p, g, m, v, lr, b1, b2, eps, step, L, W, WT);
   return hipGetLastError();
 }
 
@@ -379,9 +388,9 @@ hipError_t reduce1(int n, const float *in, float *out, hipStream_t s)
   hipLaunchKernelGGL(k_reduce1, dim3(1), dim3(ET), 0, s, n, in, out);
   return hipGetLastError();
 }
-hipError_t pack_w(int K, int N, int Np, const float *W, bf16 *Wb, hipStream_t s)
+hipError_t pack_w(int K, int N, int Np, const float *W, bf16 *Wb, bf16 *WbT, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_pack_w, dim3(grid1((long long)K * Np)), dim3(ET), 0, s, K, N, Np, W, Wb);
+  hipLaunchKernelGGL(k_pack_w, dim3(grid1((long long)K * Np)), dim3(ET), 0, s, K, N, Np, W, Wb, WbT);
   return hipGetLastError();
 }
 hipError_t gather(int nb, long long n_data, uint64_t seed, const int *step, const float *X, const float *Y, int n_out,

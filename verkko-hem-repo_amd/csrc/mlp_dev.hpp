@@ -53,17 +53,18 @@ struct GradLayout {
 // G[i] = the sum over the splits (in order) of parameter i's partials
 hipError_t grad_reduce(long long n_params, const GradLayout &L, const float *part, const float *cpart, float *G,
                        hipStream_t s);
-// Adam on the FP32 master parameters, then the BF16 weight copies (Wb[l],
-// K[l] x Np[l]) of the updated weights
+// Adam on the FP32 master parameters, then the BF16 weight copies of the
+// updated weights: Wb[l] (K[l] x Np[l]) and WbT[l] (Np[l] x K[l])
 hipError_t adam_pack(long long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
-                     const int *step, const GradLayout &L, bf16 *const *Wb, hipStream_t s);
+                     const int *step, const GradLayout &L, bf16 *const *Wb, bf16 *const *WbT, hipStream_t s);
 // H[s, j] = sin(2 pi X[s].Bf[:, j]), H[s, F + j] = cos(...); rows >= n_real are 0
 hipError_t fourier(int n, int n_real, const float *X, const float *Bf, int F, bf16 *H, hipStream_t s);
 hipError_t mse(int n, int n_real, int n_out, int ldy, const float *Y, const float *T, bf16 *dY, float *partial,
                hipStream_t s);
 int mse_blocks(int n, int ldy);
 hipError_t reduce1(int n, const float *in, float *out, hipStream_t s);
-hipError_t pack_w(int K, int N, int Np, const float *W, bf16 *Wb, hipStream_t s);
+// Wb (K x Np) and WbT (Np x K) = BF16(W) (K x N), zeros in the padding
+hipError_t pack_w(int K, int N, int Np, const float *W, bf16 *Wb, bf16 *WbT, hipStream_t s);
 hipError_t gather(int nb, long long n_data, uint64_t seed, const int *step, const float *X, const float *Y, int n_out,
                   float *Xb, float *Yb, hipStream_t s);
 hipError_t take_out(int n, int n_out, int ldy, const float *Yp, float *Y, hipStream_t s);
