@@ -148,6 +148,107 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int kchunk, const bf
       }
 }
 
+// The weight gradient C = A B with A(m, k) = A[k*sa + m] and B(k, n) =
+// B[k*sb + n] (both operands contiguous along their non-reduced index: the
+// activations and dZ, k the batch), GEMM_STORE per split: 64 x 64 tiles as
+// k_gemm, staged 128 values of k at a time.  Thread t < 128 stages the 8 x 8
+// block (k block t / 8, m block t % 8) of A and thread 128 + t the same of
+// B: eight 16-byte loads of 8 values each along m, transposed in registers,
+// eight 16-byte LDS stores of 8 values each along k (where k_gemm's
+// transposing path stores single 16-bit values).  With CS, row block 0 also
+// writes the column sums of B over its k range to colsum[z*N + n].
+constexpr int WK = 128, WKP = WK + 8;
+template <bool CS>
+__global__ __launch_bounds__(256) void k_gemm_wgrad(int N, int kchunk, const bf16 *__restrict__ A, long long sa,
+                                                    const bf16 *__restrict__ B, long long sb, float *__restrict__ C,
+                                                    int ldc, long long c_split, float *__restrict__ colsum)
+{
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM][WKP];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][WKP];
+  __shared__ float red[CS ? WK / 8 : 1][BN];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int kbeg = blockIdx.z * kchunk;
+  const bool isB = t >= 128;
+  const int tb = t & 127, kb = tb >> 3, cb = (tb & 7) * 8;  // k block, column (m or n) block
+  const uint16_t *src = reinterpret_cast<const uint16_t *>(isB ? B : A);
+  const long long ss = isB ? sb : sa;
+  const int c0 = isB ? n0 : m0;
+  uint16_t(*dst)[WKP] = isB ? Bs : As;
+  const bool cs = CS && isB && blockIdx.y == 0;
+  float csum[CS ? 8 : 1];
+#pragma unroll
+  for (int i = 0; i < (CS ? 8 : 1); ++i) csum[i] = 0.f;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int r = lane & 15, kq = (lane >> 4) * 8;
+  for (int k0 = kbeg; k0 < kbeg + kchunk; k0 += WK) {
+    const int kw = kbeg + kchunk - k0 < WK ? kbeg + kchunk - k0 : WK;
+    if (kb * 8 < kw) {
+      uint4 x[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        x[e] = *reinterpret_cast<const uint4 *>(src + (size_t)(k0 + kb * 8 + e) * ss + c0 + cb);
+      // 8 x 8 transpose: column i of the block (8 values along k) from the
+      // 16-bit halves of word i / 2 of the eight rows
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        uint32_t wd[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const uint32_t v = (i >> 1) == 0 ? x[e].x : (i >> 1) == 1 ? x[e].y : (i >> 1) == 2 ? x[e].z : x[e].w;
+          wd[e] = (i & 1) ? v >> 16 : v & 0xffffu;
+        }
+        const uint4 col = {wd[0] | (wd[1] << 16), wd[2] | (wd[3] << 16), wd[4] | (wd[5] << 16), wd[6] | (wd[7] << 16)};
+        *reinterpret_cast<uint4 *>(&dst[cb + i][kb * 8]) = col;
+        if (CS && cs) {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) csum[i] += __uint_as_float(wd[e] << 16);
+        }
+      }
+    }
+    __syncthreads();
+    for (int ks = 0; ks < kw; ks += BK) {
+      bf16x8 a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        a[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&As[wm + i * 16 + r][ks + kq]));
+        b[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&Bs[wn + i * 16 + r][ks + kq]));
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  if (CS && blockIdx.y == 0) {  // the 16 k blocks of each column, summed in k order
+    if (cs) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) red[kb][cb + i] = csum[i];
+    }
+    __syncthreads();
+    if (t < BN) {
+      float s = 0.f;
+      for (int q = 0; q < WK / 8; ++q) s += red[q][t];
+      colsum[(size_t)blockIdx.z * N + n0 + t] = s;
+    }
+  }
+  float *Cz = C + (size_t)blockIdx.z * c_split;
+  const int cc = lane & 15, rq = (lane >> 4) * 4;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        Cz[(size_t)(m0 + wm + i * 16 + rq + q) * ldc + n0 + wn + j * 16 + cc] = acc[i][j][q];
+}
+
 constexpr int ET = 256;
 
 __global__ __launch_bounds__(ET) void k_fourier(int n, int n_real, const float *__restrict__ X,
@@ -197,7 +298,15 @@ __global__ __launch_bounds__(ET) void k_reduce1(int n, const float *__restrict__
 {
   __shared__ float sh[ET];
   float s = 0.f;
-  for (int i = threadIdx.x; i < n; i += ET) s += in[i];
+  int i = threadIdx.x;
+  for (; i + 7 * ET < n; i += 8 * ET) {  // 8 loads in flight, summed in the same order
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = in[i + u * ET];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; i < n; i += ET) s += in[i];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int st = ET / 2; st > 0; st >>= 1) {
@@ -239,8 +348,17 @@ __global__ __launch_bounds__(ET) void k_grad_reduce(long long n, GradLayout L, c
     src = cpart + L.coff[l] + (i - L.boff[l]);
     stride = L.Np[l];
   }
+  // the splits in order; 8 loads in flight at a time
   float s = 0.f;
-  for (int z = 0; z < L.splits; ++z) s += src[(size_t)z * stride];
+  int z = 0;
+  for (; z + 8 <= L.splits; z += 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = src[(size_t)(z + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; z < L.splits; ++z) s += src[(size_t)z * stride];
   G[i] = s;
 }
 
@@ -337,7 +455,14 @@ hipError_t gemm(int M, int N, int K, const bf16 *A, bool a_kcontig, long long sa
   else if (E.mode == GEMM_BIAS && a_kcontig && b_kcontig) GLH_GEMM(true, true, GEMM_BIAS, false);
   else if (E.mode == GEMM_SILU_BACK && a_kcontig && b_kcontig) GLH_GEMM(true, true, GEMM_SILU_BACK, false);
   else if (E.mode != GEMM_STORE) return hipErrorInvalidValue;
-  else if (E.colsum) {
+  else if (!a_kcontig && !b_kcontig && !accumulate) {  // the weight gradients
+    if (E.colsum)
+      hipLaunchKernelGGL((k_gemm_wgrad<true>), grid, dim3(256), 0, s, N, kchunk, A, sa, B, sb, C, ldc, c_split,
+                         E.colsum);
+    else
+      hipLaunchKernelGGL((k_gemm_wgrad<false>), grid, dim3(256), 0, s, N, kchunk, A, sa, B, sb, C, ldc, c_split,
+                         nullptr);
+  } else if (E.colsum) {
     if (b_kcontig) return hipErrorInvalidValue;
     if (a_kcontig) GLH_GEMM(true, false, GEMM_STORE, true);
     else GLH_GEMM(false, false, GEMM_STORE, true);
