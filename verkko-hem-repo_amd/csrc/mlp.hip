@@ -249,6 +249,93 @@ __global__ __launch_bounds__(256) void k_gemm_wgrad(int N, int kchunk, const bf1
         Cz[(size_t)(m0 + wm + i * 16 + rq + q) * ldc + n0 + wn + j * 16 + cc] = acc[i][j][q];
 }
 
+// The forward, output-layer and input-gradient products, C = A B with both
+// operands contiguous along k (A(m, k) = A[m*sa + k], B(k, n) = B[n*sb + k])
+// and the epilogues of k_gemm (GEMM_BIAS_SILU, GEMM_BIAS, GEMM_SILU_BACK):
+// 64 x 64 tiles staged 128 values of k at a time, two barriers per 128
+// values.  A full stage issues its eight 16-byte loads per thread without
+// conditions, so they are all in flight at once (with a per-load condition
+// the stage took twice as long); a last, partial stage loads what it holds.
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm_k128(int K, const bf16 *__restrict__ A, long long sa,
+                                                   const bf16 *__restrict__ B, long long sb, float *__restrict__ C,
+                                                   int ldc, GemmEpi E)
+{
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM][WKP];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][WKP];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const uint16_t *Ag = reinterpret_cast<const uint16_t *>(A);
+  const uint16_t *Bg = reinterpret_cast<const uint16_t *>(B);
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int r = lane & 15, kq = (lane >> 4) * 8;
+  for (int k0 = 0; k0 < K; k0 += WK) {
+    const int kw = K - k0 < WK ? K - k0 : WK;
+    uint4 va[4], vb[4];
+    if (kw == WK) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int p = t + 256 * u, row = p >> 4, c = (p & 15) * 8;
+        va[u] = *reinterpret_cast<const uint4 *>(Ag + (size_t)(m0 + row) * sa + k0 + c);
+        vb[u] = *reinterpret_cast<const uint4 *>(Bg + (size_t)(n0 + row) * sb + k0 + c);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int p = t + 256 * u, row = p >> 4, c = (p & 15) * 8;
+        *reinterpret_cast<uint4 *>(&As[row][c]) = va[u];
+        *reinterpret_cast<uint4 *>(&Bs[row][c]) = vb[u];
+      }
+    } else {
+      for (int u = 0; u < 4; ++u) {
+        const int p = t + 256 * u, row = p >> 4, c = (p & 15) * 8;
+        if (c < kw) {
+          *reinterpret_cast<uint4 *>(&As[row][c]) = *reinterpret_cast<const uint4 *>(Ag + (size_t)(m0 + row) * sa + k0 + c);
+          *reinterpret_cast<uint4 *>(&Bs[row][c]) = *reinterpret_cast<const uint4 *>(Bg + (size_t)(n0 + row) * sb + k0 + c);
+        }
+      }
+    }
+    __syncthreads();
+    for (int ks = 0; ks < kw; ks += BK) {
+      bf16x8 a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        a[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&As[wm + i * 16 + r][ks + kq]));
+        b[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&Bs[wn + i * 16 + r][ks + kq]));
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int cc = lane & 15, rq = (lane >> 4) * 4;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = m0 + wm + i * 16 + rq + q, n = n0 + wn + j * 16 + cc;
+        const float v = acc[i][j][q];
+        if (EPI == GEMM_BIAS_SILU) {
+          const float z = v + E.bias[n];
+          E.Z[(size_t)m * E.ldz + n] = z;
+          E.H[(size_t)m * E.ldh + n] = to_bf16(z / (1.f + expf(-z)));
+        } else if (EPI == GEMM_BIAS) {
+          C[(size_t)m * ldc + n] = n < E.nbias ? v + E.bias[n] : v;
+        } else {
+          const float z = E.Z[(size_t)m * E.ldz + n], sg = 1.f / (1.f + expf(-z));
+          E.H[(size_t)m * E.ldh + n] = to_bf16(v * sg * (1.f + z * (1.f - sg)));
+        }
+      }
+}
+
 constexpr int ET = 256;
 
 __global__ __launch_bounds__(ET) void k_fourier(int n, int n_real, const float *__restrict__ X,
@@ -451,9 +538,14 @@ hipError_t gemm(int M, int N, int K, const bf16 *A, bool a_kcontig, long long sa
 #define GLH_GEMM(AK, BK_, EPI, CS) \
   hipLaunchKernelGGL((k_gemm<AK, BK_, EPI, CS>), grid, dim3(256), 0, s, M, N, kchunk, A, sa, B, sb, C, ldc, c_split, acc, E)
   // the combinations the network uses
-  if (E.mode == GEMM_BIAS_SILU && a_kcontig && b_kcontig) GLH_GEMM(true, true, GEMM_BIAS_SILU, false);
-  else if (E.mode == GEMM_BIAS && a_kcontig && b_kcontig) GLH_GEMM(true, true, GEMM_BIAS, false);
-  else if (E.mode == GEMM_SILU_BACK && a_kcontig && b_kcontig) GLH_GEMM(true, true, GEMM_SILU_BACK, false);
+#define GLH_GEMM128(EPI) \
+  hipLaunchKernelGGL((k_gemm_k128<EPI>), grid, dim3(256), 0, s, K, A, sa, B, sb, C, ldc, E)
+  // the epilogue products take both operands contiguous along k, one split
+  if (E.mode != GEMM_STORE && (!a_kcontig || !b_kcontig || splits != 1 || accumulate)) return hipErrorInvalidValue;
+  if (E.mode == GEMM_BIAS_SILU) GLH_GEMM128(GEMM_BIAS_SILU);
+  else if (E.mode == GEMM_BIAS) GLH_GEMM128(GEMM_BIAS);
+  else if (E.mode == GEMM_SILU_BACK) GLH_GEMM128(GEMM_SILU_BACK);
+#undef GLH_GEMM128
   else if (E.mode != GEMM_STORE) return hipErrorInvalidValue;
   else if (!a_kcontig && !b_kcontig && !accumulate) {  // the weight gradients
     if (E.colsum)
