@@ -256,13 +256,20 @@ __global__ __launch_bounds__(256) void k_gemm_wgrad(int N, int kchunk, const bf1
 // values.  A full stage issues its eight 16-byte loads per thread without
 // conditions, so they are all in flight at once (with a per-load condition
 // the stage took twice as long); a last, partial stage loads what it holds.
+// The epilogue goes through a 64 x 64 FP32 tile in the staging LDS, so that
+// Z, C and H are read and written by rows, 4 columns per thread.
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm_k128(int K, const bf16 *__restrict__ A, long long sa,
                                                    const bf16 *__restrict__ B, long long sb, float *__restrict__ C,
                                                    int ldc, GemmEpi E)
 {
-  __shared__ __attribute__((aligned(16))) uint16_t As[BM][WKP];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][WKP];
+  // the two stages of A and B, then the 64 x 64 FP32 result tile
+  constexpr int TPC = BN + 4;
+  static_assert(BM * TPC * 4 <= 2 * BM * WKP * 2, "result tile within the staging buffers");
+  __shared__ __attribute__((aligned(16))) uint16_t SM[2 * BM * WKP];
+  uint16_t(*As)[WKP] = reinterpret_cast<uint16_t(*)[WKP]>(SM);
+  uint16_t(*Bs)[WKP] = reinterpret_cast<uint16_t(*)[WKP]>(SM + BM * WKP);
+  float(*Tt)[TPC] = reinterpret_cast<float(*)[TPC]>(SM);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const uint16_t *Ag = reinterpret_cast<const uint16_t *>(A);
@@ -314,26 +321,50 @@ __global__ __launch_bounds__(256) void k_gemm_k128(int K, const bf16 *__restrict
     }
     __syncthreads();
   }
+  // epilogue by rows: the accumulators (lane l: rows 4 (l / 16) + q of a
+  // block, column l % 16) go to the tile, then each thread takes 4
+  // consecutive columns of a row: 16-byte FP32 and 8-byte BF16 accesses
   const int cc = lane & 15, rq = (lane >> 4) * 4;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int m = m0 + wm + i * 16 + rq + q, n = n0 + wn + j * 16 + cc;
-        const float v = acc[i][j][q];
-        if (EPI == GEMM_BIAS_SILU) {
-          const float z = v + E.bias[n];
-          E.Z[(size_t)m * E.ldz + n] = z;
-          E.H[(size_t)m * E.ldh + n] = to_bf16(z / (1.f + expf(-z)));
-        } else if (EPI == GEMM_BIAS) {
-          C[(size_t)m * ldc + n] = n < E.nbias ? v + E.bias[n] : v;
-        } else {
-          const float z = E.Z[(size_t)m * E.ldz + n], sg = 1.f / (1.f + expf(-z));
-          E.H[(size_t)m * E.ldh + n] = to_bf16(v * sg * (1.f + z * (1.f - sg)));
+      for (int q = 0; q < 4; ++q) Tt[wm + i * 16 + rq + q][wn + j * 16 + cc] = acc[i][j][q];
+  __syncthreads();
+  const int c4 = (t & 15) * 4, n = n0 + c4;
+#pragma unroll
+  for (int pass = 0; pass < 4; ++pass) {
+    const int row = (t >> 4) + 16 * pass, m = m0 + row;
+    const float4 v4 = *reinterpret_cast<const float4 *>(&Tt[row][c4]);
+    float o[4] = {v4.x, v4.y, v4.z, v4.w};
+    if (EPI == GEMM_BIAS) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (n + c < E.nbias) o[c] += E.bias[n + c];
+      *reinterpret_cast<float4 *>(C + (size_t)m * ldc + n) = float4{o[0], o[1], o[2], o[3]};
+    } else {
+      if (EPI == GEMM_BIAS_SILU) {
+        float z[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) z[c] = o[c] + E.bias[n + c];
+        *reinterpret_cast<float4 *>(E.Z + (size_t)m * E.ldz + n) = float4{z[0], z[1], z[2], z[3]};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) o[c] = z[c] / (1.f + expf(-z[c]));
+      } else {
+        const float4 z4 = *reinterpret_cast<const float4 *>(E.Z + (size_t)m * E.ldz + n);
+        const float z[4] = {z4.x, z4.y, z4.z, z4.w};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const float sg = 1.f / (1.f + expf(-z[c]));
+          o[c] = o[c] * sg * (1.f + z[c] * (1.f - sg));
         }
       }
+      const uint2 pk = {(uint32_t)to_bf16(o[0]).bits | ((uint32_t)to_bf16(o[1]).bits << 16),
+                        (uint32_t)to_bf16(o[2]).bits | ((uint32_t)to_bf16(o[3]).bits << 16)};
+      *reinterpret_cast<uint2 *>(reinterpret_cast<uint16_t *>(E.H) + (size_t)m * E.ldh + n) = pk;
+    }
+  }
 }
 
 constexpr int ET = 256;
@@ -542,6 +573,9 @@ hipError_t gemm(int M, int N, int K, const bf16 *A, bool a_kcontig, long long sa
   hipLaunchKernelGGL((k_gemm_k128<EPI>), grid, dim3(256), 0, s, K, A, sa, B, sb, C, ldc, E)
   // the epilogue products take both operands contiguous along k, one split
   if (E.mode != GEMM_STORE && (!a_kcontig || !b_kcontig || splits != 1 || accumulate)) return hipErrorInvalidValue;
+  // the epilogue's 16-byte FP32 and 8-byte BF16 row accesses
+  if ((E.mode == GEMM_BIAS && ldc % 4) || ((E.mode == GEMM_BIAS_SILU || E.mode == GEMM_SILU_BACK) && (E.ldz % 4 || E.ldh % 4)))
+    return hipErrorInvalidValue;
   if (E.mode == GEMM_BIAS_SILU) GLH_GEMM128(GEMM_BIAS_SILU);
   else if (E.mode == GEMM_BIAS) GLH_GEMM128(GEMM_BIAS);
   else if (E.mode == GEMM_SILU_BACK) GLH_GEMM128(GEMM_SILU_BACK);
