@@ -7,10 +7,14 @@
 // product accumulates in FP32 (v_mfma_f32_16x16x32_bf16); master weights,
 // gradients and Adam moments are FP32.
 //
-// GEMM: C (FP32) = A B with A (M x K) and B (K x N) in BF16, either operand
-// contiguous along k or along its other index (template flags), 64 x 64
-// tiles, 4 wavefronts of 2 x 2 MFMA 16 x 16 blocks, 32-wide k steps staged
-// in LDS.  M, N multiples of 64 and K a multiple of 32 (the host pads).
+// GEMM: C (FP32) = A B with A (M x K) and B (K x N) in BF16, 64 x 64 tiles,
+// 4 wavefronts of 2 x 2 MFMA 16 x 16 blocks, M, N multiples of 64 and K a
+// multiple of 32 (the host pads).  The network's products run in two
+// kernels: k_gemm_k128 (both operands contiguous along k: the forward,
+// output-layer and input-gradient products with their epilogues) and
+// k_gemm_wgrad (both contiguous along their other index: the weight
+// gradients, split along the batch); k_gemm serves the other layouts of
+// the plain product, 32-wide k steps with transposing LDS stores.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -38,10 +42,9 @@ __device__ inline float from_bf16(bf16 h) { return __uint_as_float((uint32_t)h.b
 // A(m,k) = A[m*sam + k*sak] with A_K (sak == 1) or A_M (sam == 1);
 // B(k,n) = B[k*sbk + n*sbn] with B_K (sbk == 1) or B_N (sbn == 1).
 // Sum over the k range [blockIdx.z * kchunk, +kchunk) of A(m,k) B(k,n), then
-// the epilogue EPI (GemmEpi modes): C[m*ldc + n] (+)= it at C + blockIdx.z *
-// c_split, with the column sums of B for CS; the bias and SiLU of a hidden
-// layer; the output layer's bias; the SiLU derivative of the backward pass.
-template <bool A_K, bool B_K, int EPI, bool CS>
+// C[m*ldc + n] (+)= it at C + blockIdx.z * c_split (GEMM_STORE), with the
+// column sums of B for CS.
+template <bool A_K, bool B_K, bool CS>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int kchunk, const bf16 *__restrict__ A, long long sa,
                                               const bf16 *__restrict__ B, long long sb, float *__restrict__ C,
                                               int ldc, long long c_split, int accumulate, GemmEpi E)
@@ -132,19 +135,8 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int kchunk, const bf
       for (int q = 0; q < 4; ++q) {
         const int m = m0 + wm + i * 16 + rq + q, n = n0 + wn + j * 16 + cc;
         const float v = acc[i][j][q];
-        if (EPI == GEMM_BIAS_SILU) {
-          const float z = v + E.bias[n];
-          E.Z[(size_t)m * E.ldz + n] = z;
-          E.H[(size_t)m * E.ldh + n] = to_bf16(z / (1.f + expf(-z)));
-        } else if (EPI == GEMM_BIAS) {
-          C[(size_t)m * ldc + n] = n < E.nbias ? v + E.bias[n] : v;
-        } else if (EPI == GEMM_SILU_BACK) {
-          const float z = E.Z[(size_t)m * E.ldz + n], sg = 1.f / (1.f + expf(-z));
-          E.H[(size_t)m * E.ldh + n] = to_bf16(v * sg * (1.f + z * (1.f - sg)));
-        } else {
-          float *p = Cz + (size_t)m * ldc + n;
-          *p = accumulate ? *p + v : v;
-        }
+        float *p = Cz + (size_t)m * ldc + n;
+        *p = accumulate ? *p + v : v;
       }
 }
 
@@ -251,7 +243,7 @@ __global__ __launch_bounds__(256) void k_gemm_wgrad(int N, int kchunk, const bf1
 
 // The forward, output-layer and input-gradient products, C = A B with both
 // operands contiguous along k (A(m, k) = A[m*sa + k], B(k, n) = B[n*sb + k])
-// and the epilogues of k_gemm (GEMM_BIAS_SILU, GEMM_BIAS, GEMM_SILU_BACK):
+// and the epilogues GEMM_BIAS_SILU, GEMM_BIAS and GEMM_SILU_BACK (GemmEpi):
 // 64 x 64 tiles staged 128 values of k at a time, two barriers per 128
 // values.  A full stage issues its eight 16-byte loads per thread without
 // conditions, so they are all in flight at once (with a per-load condition
@@ -566,8 +558,8 @@ hipError_t gemm(int M, int N, int K, const bf16 *A, bool a_kcontig, long long sa
   const dim3 grid(N / BN, M / BM, splits);
   const int kchunk = K / splits;
   const int acc = accumulate ? 1 : 0;
-#define GLH_GEMM(AK, BK_, EPI, CS) \
-  hipLaunchKernelGGL((k_gemm<AK, BK_, EPI, CS>), grid, dim3(256), 0, s, M, N, kchunk, A, sa, B, sb, C, ldc, c_split, acc, E)
+#define GLH_GEMM(AK, BK_, CS) \
+  hipLaunchKernelGGL((k_gemm<AK, BK_, CS>), grid, dim3(256), 0, s, M, N, kchunk, A, sa, B, sb, C, ldc, c_split, acc, E)
   // the combinations the network uses
 #define GLH_GEMM128(EPI) \
   hipLaunchKernelGGL((k_gemm_k128<EPI>), grid, dim3(256), 0, s, K, A, sa, B, sb, C, ldc, E)
@@ -590,12 +582,12 @@ hipError_t gemm(int M, int N, int K, const bf16 *A, bool a_kcontig, long long sa
                          nullptr);
   } else if (E.colsum) {
     if (b_kcontig) return hipErrorInvalidValue;
-    if (a_kcontig) GLH_GEMM(true, false, GEMM_STORE, true);
-    else GLH_GEMM(false, false, GEMM_STORE, true);
-  } else if (a_kcontig && b_kcontig) GLH_GEMM(true, true, GEMM_STORE, false);
-  else if (a_kcontig) GLH_GEMM(true, false, GEMM_STORE, false);
-  else if (b_kcontig) GLH_GEMM(false, true, GEMM_STORE, false);
-  else GLH_GEMM(false, false, GEMM_STORE, false);
+    if (a_kcontig) GLH_GEMM(true, false, true);
+    else GLH_GEMM(false, false, true);
+  } else if (a_kcontig && b_kcontig) GLH_GEMM(true, true, false);
+  else if (a_kcontig) GLH_GEMM(true, false, false);
+  else if (b_kcontig) GLH_GEMM(false, true, false);
+  else GLH_GEMM(false, false, false);
 #undef GLH_GEMM
   return hipGetLastError();
 }
