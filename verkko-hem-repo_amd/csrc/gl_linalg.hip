@@ -1044,6 +1044,53 @@ __global__ __launch_bounds__(RT) void k_seg_final_ctl(const int *__restrict__ se
   }
 }
 
+// Batches whose instances fold their reductions: the second Gram-Schmidt
+// pass's norm reduction, control op 2 and the normalisation of V_{j+1} in
+// one launch over the chunks.  Every workgroup of an iterating instance forms
+// the norm as k_seg_final_ctl does (the chunk partials of the second pass
+// where it ran, else the first pass's norm), with the same arithmetic, so all
+// of them and the op agree; the workgroup of the instance's first chunk runs
+// op 2 (Givens update and stopping test), and every workgroup scales its
+// chunk of w.  An instance that the op stops keeps V_{j+1} partly scaled:
+// it is not used, as with the separate launches.
+template <typename T>
+__global__ __launch_bounds__(RT) void k_seg_ctl2_scale(const long long *__restrict__ ch, int n_chunks, ActiveSet act,
+                                                       const int *__restrict__ seg_chunk,
+                                                       const T *__restrict__ partial, T *norm2, GmresCtl<T> C,
+                                                       int j, T *__restrict__ w)
+{
+  __shared__ T sh[RT];
+  __shared__ int s_it;
+  const int nw = work_count(act, n_chunks);
+  for (int wi = blockIdx.x; wi < nw; wi += gridDim.x) {
+    const int cb = work_chunk(act, wi);
+    const long long sg = ch[3 * cb], beg = ch[3 * cb + 1], end = ch[3 * cb + 2];
+    // one read of the flag per workgroup: the op of this instance may clear
+    // it meanwhile, and the branch below holds a block reduction
+    if (threadIdx.x == 0) s_it = C.cyc[sg];
+    __syncthreads();
+    const bool it = s_it != 0;
+    __syncthreads();
+    if (!it) continue;
+    T nrm;
+    if (C.ro[sg]) {
+      T acc = 0;
+      for (int b = seg_chunk[sg] + threadIdx.x; b < seg_chunk[sg + 1]; b += RT) acc += partial[b];
+      nrm = block_reduce(acc, sh);
+    } else {
+      nrm = norm2[sg];
+    }
+    if (cb == seg_chunk[sg] && threadIdx.x == 0) {
+      if (C.ro[sg]) norm2[sg] = nrm;
+      gmres_ctl_op(C, 2, j, (int)sg);  // reads the norm this thread wrote
+    }
+    const T a = nrm > (T)0 ? sqrt(nrm) : (T)0;
+    const T inv = a != (T)0 ? (T)1 / a : (T)1;
+    for (long long idx = beg + threadIdx.x; idx < end; idx += RT) w[idx] = w[idx] * inv;
+  }
+}
+
+
 template <typename T>
 __global__ __launch_bounds__(RT) void k_fill(size_t n, T v, T *__restrict__ d)
 {
@@ -1271,7 +1318,7 @@ hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, cons
 template <typename T>
 hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size_t ldv, T *h, int ldh, T *w,
                                        T *partial, T *norm2, int ldo, int off, hipStream_t s, const int *flag,
-                                       const GmresCtl<T> *ctl, int op, int j, const T *hpart, int hk)
+                                       const GmresCtl<T> *ctl, int op, int j, const T *hpart, int hk, bool reduce)
 {
   if (S.n_chunks <= 0) return hipSuccess;
   if (hpart && (hk < k || hk > 128 || hpart == partial)) return hipErrorInvalidValue;
@@ -1283,6 +1330,7 @@ hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size
   else if (S.gs == 16) GLH_SLN(16);
   else return hipErrorInvalidValue;
 #undef GLH_SLN
+  if (!reduce) return hipGetLastError();  // the norm partials go to a later launch
   if (ctl) {
     if (ldo != 1 || off != 0 || norm2 != ctl->nrm2) return hipErrorInvalidValue;  // the op reads the norms it gets
     hipLaunchKernelGGL((k_seg_final_ctl<T>), dim3(seg_grid(S)), dim3(RT), 0, s, S.seg_chunk, S.n_seg, S.act, partial,
@@ -1291,6 +1339,17 @@ hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size
     hipLaunchKernelGGL((k_seg_final<T>), dim3(seg_grid(S), 1), dim3(RT), 0, s, S.seg_chunk, S.n_seg, S.act, 1,
                        partial, norm2, ldo, off, 0, flag);
   }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_seg_ctl2_scale(const SegInfo &S, const T *partial, T *norm2, const GmresCtl<T> &C, int j, T *w,
+                                 hipStream_t s)
+{
+  if (S.n_chunks <= 0) return hipSuccess;
+  if (norm2 != C.nrm2) return hipErrorInvalidValue;  // the op reads the norms it gets
+  hipLaunchKernelGGL((k_seg_ctl2_scale<T>), dim3(chunk_grid(S)), dim3(RT), 0, s, S.chunks, S.n_chunks, S.act,
+                     S.seg_chunk, partial, norm2, C, j, w);
   return hipGetLastError();
 }
 
@@ -1375,7 +1434,9 @@ hipError_t launch_seg_residual(const SegInfo &S, const T *b, const T *w, T *r, T
                                          bool, hipStream_t, const int *, bool);                                    \
   template hipError_t launch_seg_sub_lincomb_norm<T>(const SegInfo &, int, const T *, size_t, T *, int, T *,       \
                                                      T *, T *, int, int, hipStream_t, const int *,                 \
-                                                     const GmresCtl<T> *, int, int, const T *, int);               \
+                                                     const GmresCtl<T> *, int, int, const T *, int, bool);         \
+  template hipError_t launch_seg_ctl2_scale<T>(const SegInfo &, const T *, T *, const GmresCtl<T> &, int, T *,     \
+                                               hipStream_t);                                                       \
   template hipError_t launch_seg_lincomb<T>(const SegInfo &, int, const T *, size_t, const T *, int, T *,          \
                                             hipStream_t, const int *);                                             \
   template hipError_t launch_seg_scale<T>(const SegInfo &, const T *, const T *, int, T *, hipStream_t,            \

@@ -723,8 +723,11 @@ int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, i
       HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hs, ldh, wv, np, nrm2, 1, 0, s, cyc, &C, 1, j,
                                                      pt.p, j + 2));
       HIPCHK(c, glh::launch_seg_mdot<TV>(SR, j + 1, V.p, n, wv, pt.p, hq, ldh, 0, false, s, ro, false));
-      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hq, ldh, wv, np, nrm2, 1, 0, s, ro, &C, 2, j,
-                                                     pt.p, j + 1));
+      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hq, ldh, wv, np, nrm2, 1, 0, s, ro, nullptr, 0,
+                                                     j, pt.p, j + 1, false));
+      // the pass's norm reduction, op 2 and the normalisation in one launch
+      HIPCHK(c, glh::launch_seg_ctl2_scale<TV>(SA, np, nrm2, C, j, wv, s));
+      continue;
     } else {
       HIPCHK(c, glh::launch_seg_mdot<TV>(SA, j + 2, V.p, n, wv, pt.p, hs, ldh, 0, false, s, cyc));
       HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hs, ldh, wv, pt.p, nrm2, 1, 0, s, cyc, &C, 1, j));

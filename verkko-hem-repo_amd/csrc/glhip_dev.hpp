@@ -266,7 +266,14 @@ template <typename T>
 hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size_t ldv, T *h, int ldh, T *w,
                                        T *partial, T *norm2, int ldo, int off, hipStream_t s,
                                        const int *flag = nullptr, const GmresCtl<T> *ctl = nullptr, int op = 0,
-                                       int j = 0, const T *hpart = nullptr, int hk = 0);
+                                       int j = 0, const T *hpart = nullptr, int hk = 0, bool reduce = true);
+// reduce = false: no norm reduction launch (k_seg_ctl2_scale takes the partials)
+// Folded batches, after the second Gram-Schmidt pass: norm2 of each iterating
+// instance from the pass's chunk partials (where it ran; else the first
+// pass's), control op 2 and w /= sqrt(norm2), in one launch
+template <typename T>
+hipError_t launch_seg_ctl2_scale(const SegInfo &S, const T *partial, T *norm2, const GmresCtl<T> &C, int j, T *w,
+                                 hipStream_t s);
 // w = sum_{i < min(k, kseg[s])} y[s*ldy + i] V_i
 template <typename T>
 hipError_t launch_seg_lincomb(const SegInfo &S, int k, const T *V, size_t ldv, const T *y, int ldy, T *w,
