@@ -1044,9 +1044,9 @@ __global__ __launch_bounds__(RT) void k_seg_final_ctl(const int *__restrict__ se
   }
 }
 
-// Batches whose instances fold their reductions: the second Gram-Schmidt
-// pass's norm reduction, control op 2 and the normalisation of V_{j+1} in
-// one launch over the chunks.  Every workgroup of an iterating instance forms
+// After the second Gram-Schmidt pass: its norm reduction, control op 2 and
+// the normalisation of V_{j+1} in one launch over the chunks (the second
+// pass rarely runs; where it did not, the first pass's norm is read).  Every workgroup of an iterating instance forms
 // the norm as k_seg_final_ctl does (the chunk partials of the second pass
 // where it ran, else the first pass's norm), with the same arithmetic, so all
 // of them and the op agree; the workgroup of the instance's first chunk runs

@@ -727,14 +727,16 @@ int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, i
                                                      j, pt.p, j + 1, false));
       // the pass's norm reduction, op 2 and the normalisation in one launch
       HIPCHK(c, glh::launch_seg_ctl2_scale<TV>(SA, np, nrm2, C, j, wv, s));
-      continue;
     } else {
       HIPCHK(c, glh::launch_seg_mdot<TV>(SA, j + 2, V.p, n, wv, pt.p, hs, ldh, 0, false, s, cyc));
       HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hs, ldh, wv, pt.p, nrm2, 1, 0, s, cyc, &C, 1, j));
       HIPCHK(c, glh::launch_seg_mdot<TV>(SR, j + 1, V.p, n, wv, pt.p, hq, ldh, 0, false, s, ro));
-      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hq, ldh, wv, pt.p, nrm2, 1, 0, s, ro, &C, 2, j));
+      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hq, ldh, wv, pt.p, nrm2, 1, 0, s, ro, nullptr,
+                                                     0, j, nullptr, 0, false));
+      // as above: the pass's norm reduction (its partials where it ran), op 2
+      // and the normalisation in one launch
+      HIPCHK(c, glh::launch_seg_ctl2_scale<TV>(SA, pt.p, nrm2, C, j, wv, s));
     }
-    HIPCHK(c, glh::launch_seg_scale<TV>(SA, wv, nrm2, 1, wv, s, cyc, true));
   }
   return GLH_OK;
 }

@@ -268,7 +268,7 @@ hipError_t launch_seg_sub_lincomb_norm(const SegInfo &S, int k, const T *V, size
                                        const int *flag = nullptr, const GmresCtl<T> *ctl = nullptr, int op = 0,
                                        int j = 0, const T *hpart = nullptr, int hk = 0, bool reduce = true);
 // reduce = false: no norm reduction launch (k_seg_ctl2_scale takes the partials)
-// Folded batches, after the second Gram-Schmidt pass: norm2 of each iterating
+// After the second Gram-Schmidt pass: norm2 of each iterating
 // instance from the pass's chunk partials (where it ran; else the first
 // pass's), control op 2 and w /= sqrt(norm2), in one launch
 template <typename T>
