@@ -1060,29 +1060,69 @@ __global__ __launch_bounds__(RT) void k_seg_ctl2_scale(const long long *__restri
                                                        int j, T *__restrict__ w)
 {
   __shared__ T sh[RT];
-  __shared__ int s_it;
+  __shared__ int s_it, s_ro;
+  __shared__ T s_n2;
+  __shared__ double s_col[128], s_cs[128], s_sn[128];  // op 2: j < m <= 126
   const int nw = work_count(act, n_chunks);
   for (int wi = blockIdx.x; wi < nw; wi += gridDim.x) {
     const int cb = work_chunk(act, wi);
     const long long sg = ch[3 * cb], beg = ch[3 * cb + 1], end = ch[3 * cb + 2];
-    // one read of the flag per workgroup: the op of this instance may clear
-    // it meanwhile, and the branch below holds a block reduction
-    if (threadIdx.x == 0) s_it = C.cyc[sg];
+    // the flags and the first pass's norm, read once per workgroup with the
+    // three loads in flight together: the op of this instance may clear the
+    // flag meanwhile, and the branch below holds a block reduction
+    if (threadIdx.x == 0) {
+      const int it0 = C.cyc[sg], ro0 = C.ro[sg];
+      const T n20 = norm2[sg];
+      s_it = it0;
+      s_ro = ro0;
+      s_n2 = n20;
+    }
     __syncthreads();
-    const bool it = s_it != 0;
+    const bool it = s_it != 0, ro = s_ro != 0;
+    T nrm = s_n2;
     __syncthreads();
     if (!it) continue;
-    T nrm;
-    if (C.ro[sg]) {
+    if (ro) {
       T acc = 0;
       for (int b = seg_chunk[sg] + threadIdx.x; b < seg_chunk[sg + 1]; b += RT) acc += partial[b];
       nrm = block_reduce(acc, sh);
-    } else {
-      nrm = norm2[sg];
     }
-    if (cb == seg_chunk[sg] && threadIdx.x == 0) {
-      if (C.ro[sg]) norm2[sg] = nrm;
-      gmres_ctl_op(C, 2, j, (int)sg);  // reads the norm this thread wrote
+    if (cb == seg_chunk[sg]) {
+      // op 2 with the new column (both passes summed) and the previous
+      // rotations staged in LDS by the workgroup: the rotation loop of one
+      // thread then reads LDS, not a chain of global loads (same arithmetic
+      // and order as gmres_ctl_op)
+      const size_t so = (size_t)sg * C.ldh, sr = (size_t)sg * C.m;
+      for (int r = threadIdx.x; r <= j; r += RT)
+        s_col[r] = (double)C.hs[so + r] + (ro ? (double)C.hq[so + r] : 0.0);
+      for (int r = threadIdx.x; r < j; r += RT) {
+        s_cs[r] = C.cs[sr + r];
+        s_sn[r] = C.sn[sr + r];
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        if (ro) norm2[sg] = nrm;
+        const int m = C.m;
+        double *g = C.g + (size_t)sg * (m + 1);
+        double *H = C.H + (size_t)sg * (m + 1) * m;
+        double *cs = C.cs + sr, *sn = C.sn + sr;
+        const double hn = nrm > (T)0 ? sqrt((double)nrm) : 0.0;
+        double cur = s_col[0];
+        for (int r = 0; r < j; ++r) {
+          const double nxt = s_col[r + 1], c_ = s_cs[r], s_ = s_sn[r];
+          H[(size_t)r * m + j] = c_ * cur + s_ * nxt;
+          cur = -s_ * cur + c_ * nxt;
+        }
+        const double rr = hypot(cur, hn);
+        cs[j] = rr > 0 ? cur / rr : 1.0;
+        sn[j] = rr > 0 ? hn / rr : 0.0;
+        H[(size_t)j * m + j] = rr;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        C.jseg[sg] = j + 1;
+        C.its[sg] += 1;
+        if (fabs(g[j + 1]) <= C.target[sg] || hn == 0.0) C.cyc[sg] = 0;
+      }
     }
     const T a = nrm > (T)0 ? sqrt(nrm) : (T)0;
     const T inv = a != (T)0 ? (T)1 / a : (T)1;
