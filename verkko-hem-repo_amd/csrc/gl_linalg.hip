@@ -636,9 +636,13 @@ template <typename T, int G>
 __global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__restrict__ ch, int n_chunks,
                                                          ActiveSet act, int k, const T *__restrict__ V, size_t ldv,
                                                          const T *__restrict__ w, T *__restrict__ partial,
-                                                         const int *__restrict__ flag)
+                                                         const int *__restrict__ flag, Op1In<T> O)
 {
   __shared__ T sh[G][RT / 64];
+  __shared__ T shr[RT];
+  __shared__ int s_cy;
+  int o_sg = -1;  // O.npart: the instance whose flag this workgroup formed last
+  bool o_on = false;
   int nw = work_count(act, n_chunks), first = blockIdx.x, step = gridDim.x, grp = blockIdx.y;
   if (!act.counts && (int)gridDim.x >= nw) {  // (a capped grid strides over the chunks instead)
     const int L = blockIdx.x + gridDim.x * blockIdx.y, r = L >> 3;
@@ -651,7 +655,34 @@ __global__ __launch_bounds__(RT) void k_seg_mdot_partial(const long long *__rest
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int wi = first; wi < nw; wi += step) {
     const int cb = work_chunk(act, wi);
-    if (flag && !flag[ch[3 * cb]]) continue;  // skipped instance: its partials are never read
+    if (O.npart) {
+      // control op 1 of the instance here (folded batches, the second pass):
+      // its norm after the first pass from that pass's chunk partials, with
+      // k_seg_final_ctl's arithmetic, then the pass's flag; the instance's
+      // first chunk (vector group 0) records both for the later kernels
+      const int sg = (int)ch[3 * cb];
+      if (sg != o_sg) {
+        if (threadIdx.x == 0) s_cy = O.C.cyc[sg];
+        __syncthreads();
+        const bool cy = s_cy != 0;
+        __syncthreads();
+        T nrm = 0;
+        if (cy) {
+          T a = 0;
+          for (int b = O.seg_chunk[sg] + threadIdx.x; b < O.seg_chunk[sg + 1]; b += RT) a += O.npart[b];
+          nrm = block_reduce(a, shr);
+        }
+        o_on = cy && (double)nrm < O.C.dgks * (double)O.C.hs[(size_t)sg * O.C.ldh + O.j + 1];
+        o_sg = sg;
+        if (cb == O.seg_chunk[sg] && grp == 0 && threadIdx.x == 0) {
+          if (cy) O.norm2[sg] = nrm;
+          O.C.ro[sg] = o_on ? 1 : 0;
+        }
+      }
+      if (!o_on) continue;
+    } else if (flag && !flag[ch[3 * cb]]) {
+      continue;  // skipped instance: its partials are never read
+    }
     const long long beg = ch[3 * cb + 1], end = ch[3 * cb + 2];
     T acc[G];
 #pragma unroll
@@ -1336,14 +1367,15 @@ inline int seg_grid(const SegInfo &S)
 
 template <typename T>
 hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, const T *w, T *partial, T *out, int ldo,
-                           int off, bool accumulate, hipStream_t s, const int *flag, bool reduce)
+                           int off, bool accumulate, hipStream_t s, const int *flag, bool reduce, const Op1In<T> &O)
 {
   if (k <= 0 || S.n_chunks <= 0) return hipSuccess;
+  if (O.npart && (!O.norm2 || O.norm2 != O.C.nrm2 || O.npart == partial || reduce)) return hipErrorInvalidValue;
   int gx = S.act.counts ? chunk_grid(S) : (S.n_chunks + 7) / 8 * 8;  // XCD-numbered: a multiple of 8
   if (!S.act.counts && S.cap > 0 && S.cap < S.n_chunks) gx = S.cap;  // grid-stride over the chunks
 #define GLH_MDOT(G)                                                                                              \
   hipLaunchKernelGGL((k_seg_mdot_partial<T, G>), dim3(gx, (k + G - 1) / G), dim3(RT), 0, s, S.chunks, S.n_chunks, \
-                     S.act, k, V, ldv, w, partial, flag)
+                     S.act, k, V, ldv, w, partial, flag, O)
   if (S.gs == 4) GLH_MDOT(4);
   else if (S.gs == 8) GLH_MDOT(8);
   else if (S.gs == 16) GLH_MDOT(16);
@@ -1471,7 +1503,7 @@ hipError_t launch_seg_residual(const SegInfo &S, const T *b, const T *w, T *r, T
 
 #define GLH_SEG_INST(T)                                                                                         \
   template hipError_t launch_seg_mdot<T>(const SegInfo &, int, const T *, size_t, const T *, T *, T *, int, int,   \
-                                         bool, hipStream_t, const int *, bool);                                    \
+                                         bool, hipStream_t, const int *, bool, const Op1In<T> &);                  \
   template hipError_t launch_seg_sub_lincomb_norm<T>(const SegInfo &, int, const T *, size_t, T *, int, T *,       \
                                                      T *, T *, int, int, hipStream_t, const int *,                 \
                                                      const GmresCtl<T> *, int, int, const T *, int, bool);         \

@@ -720,9 +720,17 @@ int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, i
     // multi-dot partials itself, instead of a reduction launch per pass)
     if (fold) {
       HIPCHK(c, glh::launch_seg_mdot<TV>(SA, j + 2, V.p, n, wv, pt.p, hs, ldh, 0, false, s, cyc, false));
-      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hs, ldh, wv, np, nrm2, 1, 0, s, cyc, &C, 1, j,
-                                                     pt.p, j + 2));
-      HIPCHK(c, glh::launch_seg_mdot<TV>(SR, j + 1, V.p, n, wv, pt.p, hq, ldh, 0, false, s, ro, false));
+      HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hs, ldh, wv, np, nrm2, 1, 0, s, cyc, nullptr,
+                                                     0, j, pt.p, j + 2, false));
+      // op 1 (the second pass's flag) formed by the second pass's multi-dot
+      // workgroups from the first pass's norm partials: no launch of its own
+      glh::Op1In<TV> o1;
+      o1.npart = np;
+      o1.norm2 = nrm2;
+      o1.seg_chunk = SR.seg_chunk;
+      o1.C = C;
+      o1.j = j;
+      HIPCHK(c, glh::launch_seg_mdot<TV>(SR, j + 1, V.p, n, wv, pt.p, hq, ldh, 0, false, s, ro, false, o1));
       HIPCHK(c, glh::launch_seg_sub_lincomb_norm<TV>(SA, j + 1, V.p, n, hq, ldh, wv, np, nrm2, 1, 0, s, ro, nullptr, 0,
                                                      j, pt.p, j + 1, false));
       // the pass's norm reduction, op 2 and the normalisation in one launch

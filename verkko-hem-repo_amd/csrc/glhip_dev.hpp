@@ -247,12 +247,9 @@ struct SegInfo {
   // it is most of the cost
   int cap = 0;
 };
-// flag (optional): instances whose flag is 0 are skipped (outputs untouched).
-template <typename T>
-hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, const T *w, T *partial, T *out, int ldo,
-                           int off, bool accumulate, hipStream_t s, const int *flag = nullptr, bool reduce = true);
 template <typename T>
 struct GmresCtl;
+
 // w -= sum_i h[s*ldh + i] V_i per instance; norm2[s*ldo + off] = ||w_s||^2 after.
 // With ctl (norm2 == ctl->nrm2, ldo 1, off 0), the workgroup that reduces an
 // instance's norm then runs GMRES control op `op` (1 or 2) for column j of
@@ -316,6 +313,24 @@ struct GmresCtl {
   int *jlast;  // [S] basis size of the previous cycle (written by op 0)
   double dgks;  // second Gram-Schmidt pass when ||w'||^2 < dgks ||w||^2
 };
+// Control op 1 taken into a launch_seg_mdot (folded batches, the second
+// Gram-Schmidt pass): with npart set, each workgroup forms its instance's
+// norm from npart (the first pass's chunk partials) and the pass's flag
+// itself, in place of `flag`, and the instance's first chunk writes norm2
+// (== C.nrm2) and C.ro (reduce must be false).
+template <typename T>
+struct Op1In {
+  const T *npart = nullptr;
+  T *norm2 = nullptr;
+  const int *seg_chunk = nullptr;
+  GmresCtl<T> C{};
+  int j = 0;
+};
+// flag (optional): instances whose flag is 0 are skipped (outputs untouched).
+template <typename T>
+hipError_t launch_seg_mdot(const SegInfo &S, int k, const T *V, size_t ldv, const T *w, T *partial, T *out, int ldo,
+                           int off, bool accumulate, hipStream_t s, const int *flag = nullptr, bool reduce = true,
+                           const Op1In<T> &op1 = Op1In<T>{});
 template <typename T>
 hipError_t launch_gmres_ctl(const GmresCtl<T> &C, int op, int j, hipStream_t s);
 hipError_t launch_seg_trial(const SegInfo &S, const double *U, const double *dU, const double *alpha,
