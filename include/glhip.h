@@ -443,6 +443,13 @@ int glh_last_assembly_ms(glh_ctx *ctx, double ms[2]);
  * chunks (0: 4096 / 1024), Gram-Schmidt tile (4, 8 or 16 basis vectors per
  * multi-dot workgroup and in flight per thread of the update; 0: 8). */
 int glh_set_schedule(glh_ctx *ctx, int restart, int q_block, int row_cap, int chunk_cap, int gs_tile);
+/* Second classical Gram-Schmidt pass of GMRES for a column whose squared norm
+ * after the projection falls below `ratio` times its squared norm before
+ * (default 0.01; 0.5 is the classic 1/sqrt(2) criterion, >= 1 always, 0
+ * never).  No counterpart in the reference (deal.II's GMRES orthogonalises
+ * inside SolverFGMRES, solve.cc:159-174): a knob of this solver's classical
+ * Gram-Schmidt, used by the tests to run the second pass on every column. */
+int glh_set_reorth(glh_ctx *ctx, double ratio);
 /* Autotuner of the launch schedule and kernel tiles: times `trial_newton`
  * Newton steps (after one untimed step that records the graphs) from the
  * current solution, for a batch over blocks of 4 / 8 / 16 Arnoldi steps per

@@ -267,3 +267,32 @@ def test_slab_sweep_of_256_instances_at_full_size():
     finally:
         single.close()
     assert compared >= 7 and same >= compared - 1
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("layout", ["batch", "single"])
+def test_second_gram_schmidt_pass_on_every_column_keeps_the_fixed_point(ctx, layout, fp32):
+    # ratio 1 runs the second classical Gram-Schmidt pass on every column,
+    # 0 on none: the batch takes the folded path (norms and control op 1
+    # inside the multi-dot launch), a single instance the separate reductions
+    cases = CASES if layout == "batch" else [((16, 16, 16), 30.0, 20.0, 0.5)]
+    meshes, batch, cos, Us = _batch(ctx, cases, seed0=60)
+    U0 = ctx.solution()
+    kw = dict(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=40, fp32=fp32)
+    out = {}
+    try:
+        for ratio in (0.01, 1.0, 0.0):
+            ctx.set_reorth(ratio)
+            ctx.set_solution(U0)
+            st = ctx.newton_solve(**kw)
+            ist = ctx.instance_stats()
+            say(layout, "fp32" if fp32 else "fp64", "ratio", ratio, st)
+            assert st["converged"] == 1.0 and np.all(ist[:, 1] == 1)
+            out[ratio] = (st["gmres_its"], ctx.instance_energy(), ctx.instance_residuals(), ist[:, 2])
+    finally:
+        ctx.set_reorth(0.01)
+    for ratio, (its, F, R, R0) in out.items():
+        assert np.all(np.abs(F - out[0.01][1]) <= 1e-10 * np.abs(out[0.01][1])), ratio
+        assert np.all(R <= 1e-10 * R0 * (1 + 1e-6)), ratio
+    # the threshold rarely fires: the iteration counts stay close
+    assert abs(out[1.0][0] - out[0.01][0]) <= 0.1 * out[0.01][0] + 5

@@ -197,6 +197,14 @@ struct glh_ctx {
   // restart used when glh_newton_solve is called with restart <= 0
   int q_block = 0, row_cap = 0, chunk_cap = 0, tuned_restart = 100;
   int gs_tile = 8;  // Gram-Schmidt tile (glh::SegInfo::gs)
+  // The second classical Gram-Schmidt pass runs for a column whose norm falls
+  // below 10% of its norm before the projection (||w'||^2 < 0.01 ||w||^2),
+  // that is, when cancellation has cost about a digit.  Measured on the 32^3
+  // Q2 cube and the 256-slab batch (FP64 and FP32): the iteration counts and
+  // free energies are those of the classic 1/sqrt(2) threshold, which
+  // re-projected nearly every column and cost 6% of the solve
+  // (glh_set_reorth sets another ratio).
+  double dgks = 0.01;
   const int *asm_active = nullptr;  // device flags: assemble only these instances (inside glh_newton_solve)
   DBuf<int> asm_flag;
   std::vector<double> r_ref;  // per-instance stopping reference (empty: ||R0|| of the solve)
@@ -749,14 +757,6 @@ int enqueue_cycle(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, i
   return GLH_OK;
 }
 
-// The second classical Gram-Schmidt pass runs for a column whose norm falls
-// below 10% of its norm before the projection (||w'||^2 < 0.01 ||w||^2), that
-// is, when cancellation has cost about a digit.  Measured on the 32^3 Q2 cube
-// and the 256-slab batch (FP64 and FP32): the iteration counts and free
-// energies are those of the classic 1/sqrt(2) threshold, which re-projected
-// nearly every column and cost 6% of the solve.
-constexpr double DGKS_RATIO = 0.01;
-
 template <typename TJ, typename TV>
 int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std::vector<double> &rtol, int maxit,
           int m, int *its_out, std::vector<double> &rel_out, std::vector<int> &its_seg)
@@ -802,7 +802,7 @@ int gmres(glh_ctx *c, const TJ *J, const TJ *Dinv, const TV *b, TV *x, const std
   glh::GmresCtl<TV> C{S,        m,          ldh,        hs.p,         hq.p,         sc.p,        yb.p,
                       c->gmH.p, c->gmcs.p,  c->gmsn.p,  c->gmg.p,     c->gmbeta.p,  c->gmtarget.p, c->gmcyc.p,
                       c->gmconv.p, c->gmjseg.p, c->gmits.p, c->gmro.p, c->gmfrozen.p, c->gmjlast.p,
-                      DGKS_RATIO};
+                      c->dgks};
   // ||b_s|| and the absolute targets
   HIPCHK(c, glh::launch_seg_mdot<TV>(SI, 1, b, n, b, pt.p, sc.p, 1, 0, false, s));
   TV *bn2 = Work<TV>::pin(c).p;
@@ -2221,6 +2221,14 @@ int glh_set_schedule(glh_ctx *c, int restart, int q_block, int row_cap, int chun
   c->row_cap = row_cap;
   c->chunk_cap = chunk_cap;
   c->gs_tile = gs_tile ? gs_tile : 8;
+  return GLH_OK;
+}
+
+int glh_set_reorth(glh_ctx *c, double ratio)
+{
+  if (!c || !(ratio >= 0.0) || !std::isfinite(ratio)) return GLH_ERR_ARG;
+  c->dgks = ratio;
+  drop_graphs(c);  // recorded control ops carry the ratio by value
   return GLH_OK;
 }
 

@@ -35,7 +35,7 @@ EXPORTS = ["glh_create", "glh_destroy", "glh_last_error", "glh_set_material", "g
            "glh_set_continuation", "glh_set_matrix_free", "glh_apply_jacobian", "glh_diagonal_blocks", "glh_apply_block_jacobi", "glh_mlp_create", "glh_mlp_destroy", "glh_mlp_last_error", "glh_mlp_n_params",
            "glh_mlp_get_params", "glh_mlp_set_params", "glh_mlp_set_data", "glh_mlp_train", "glh_mlp_loss_grad",
            "glh_mlp_predict", "glh_instance_residuals", "glh_set_forcing", "glh_set_residual_reference",
-           "glh_set_schedule", "glh_autotune", "glh_prm_parse", "glh_prm_destroy", "glh_prm_get",
+           "glh_set_schedule", "glh_set_reorth", "glh_autotune", "glh_prm_parse", "glh_prm_destroy", "glh_prm_get",
            "glh_prm_get_double", "glh_run_prm", "glh_write_vtu", "glh_initial_guess_field",
            "glh_transfer_solution", "glh_forest_set_degree", "glh_bench_read",
            "glh_flag_refine_coarsen", "glh_mesh_from_arrays", "glh_mesh_last_error", "glh_residual_into"]
@@ -112,6 +112,7 @@ def lib():
     L.glh_prm_get_double.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(C.c_double)]
     L.glh_run_prm.argtypes = [vp, vp, C.c_int, _dp, C.POINTER(C.c_int)]
     L.glh_set_schedule.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.glh_set_reorth.argtypes = [vp, C.c_double]
     L.glh_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, _dp]
     L.glh_set_residual_reference.argtypes = [vp, C.c_int, vp]
     L.glh_mlp_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]
@@ -700,6 +701,10 @@ class Context:
 
     def set_verbose(self, level=1):
         self._chk(lib().glh_set_verbose(self.h, int(level)), "glh_set_verbose")
+
+    def set_reorth(self, ratio):
+        """Second Gram-Schmidt pass when ||w'||^2 < ratio ||w||^2 (glh_set_reorth)."""
+        self._chk(lib().glh_set_reorth(self.h, float(ratio)), "glh_set_reorth")
 
     def set_schedule(self, restart=100, q_block=0, row_cap=0, chunk_cap=0, gs_tile=0):
         self._chk(lib().glh_set_schedule(self.h, int(restart), int(q_block), int(row_cap), int(chunk_cap),
