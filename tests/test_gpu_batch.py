@@ -279,6 +279,9 @@ def test_second_gram_schmidt_pass_on_every_column_keeps_the_fixed_point(ctx, lay
     meshes, batch, cos, Us = _batch(ctx, cases, seed0=60)
     U0 = ctx.solution()
     kw = dict(rtol=1e-10, max_newton=60, lin_tol=1e-2, restart=40, fp32=fp32)
+    for bad in (-0.1, float("nan"), float("inf")):
+        with pytest.raises(G.GLHError):
+            ctx.set_reorth(bad)
     out = {}
     try:
         for ratio in (0.01, 1.0, 0.0):

@@ -91,6 +91,15 @@ def test_mesh_argument_errors():
         G.Mesh(1, (0, 2, 2), (0, 0, 0), (1, 1, 1), [1] * 6)
 
 
+def test_solver_option_argument_errors():
+    # without a context the setters refuse before any device work (the
+    # ratio's own range is checked on a live context in the GPU tests)
+    L = G.lib()
+    for r in (0.5, -0.1, float("nan"), float("inf")):
+        assert L.glh_set_reorth(None, r) == -1
+    assert L.glh_set_schedule(None, 100, 0, 0, 0, 8) == -1
+
+
 @pytest.mark.parametrize("phase", [0, 1])
 def test_initial_guess_bitwise_equal_to_oracle(phase):
     gap = O.matep(20.0, 0.5)["gap_B" if phase == 0 else "gap_A"]
