@@ -918,6 +918,35 @@ __global__ __launch_bounds__((mf_threads<DEG, TH>()), (mf_waves<DEG, TH, TX>()))
   }
 }
 
+// The sums over a node's cell slots of components c and c + 1, in the
+// node's cell order: four slot indices, then their four loads in flight.
+template <typename TY>
+__device__ __forceinline__ void slot_pair_sum(const int *__restrict__ ptr, const int *__restrict__ idx,
+                                              const TY *__restrict__ ycell, int node, int c, TY &s0, TY &s1)
+{
+  using Y2 = typename Vec2<TY>::type;
+  const int k1 = ptr[node + 1];
+  int k = ptr[node];
+  for (; k + 4 <= k1; k += 4) {
+    int j[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) j[u] = idx[k + u];
+    Y2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const Y2 *>(ycell + (size_t)j[u] * 18 + c);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      s0 += v[u].x;
+      s1 += v[u].y;
+    }
+  }
+  for (; k < k1; ++k) {
+    const Y2 v = *reinterpret_cast<const Y2 *>(ycell + (size_t)idx[k] * 18 + c);
+    s0 += v.x;
+    s1 += v.y;
+  }
+}
+
 // y = sum over the cells of each node of their contributions (fixed cell
 // order), and for the Dirichlet rows their diagonal entry (the sum over cells
 // of |local diagonal|) times x; with b != null the residual y = b - J x.  One
@@ -931,17 +960,12 @@ __global__ void k_mf_gather(int n_nodes, const int *__restrict__ ptr, const int 
                             const TX *__restrict__ b, const int *__restrict__ row_seg, const int *__restrict__ flag,
                             TX *__restrict__ y)
 {
-  using Y2 = typename Vec2<TY>::type;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)n_nodes * 9) return;
   const int node = (int)(i / 9), c = 2 * (int)(i % 9);
   if (flag && !flag[row_seg[node]]) return;  // instance not iterating: its rows are left as they are
   TY s0 = 0, s1 = 0;
-  for (int k = ptr[node]; k < ptr[node + 1]; ++k) {
-    const Y2 v = *reinterpret_cast<const Y2 *>(ycell + (size_t)idx[k] * 18 + c);
-    s0 += v.x;
-    s1 += v.y;
-  }
+  slot_pair_sum(ptr, idx, ycell, node, c, s0, s1);
   const uint32_t mk = mask[node];
   const size_t o = (size_t)node * 18 + c;
   double r0 = ((mk >> c) & 1u) ? Jd[(size_t)node * 324 + c * 19] * (double)x[o] : (double)s0;
@@ -968,12 +992,7 @@ __global__ void k_res_gather(int n_nodes, const int *__restrict__ ptr, const int
   const int node = (int)(i / 9), c = 2 * (int)(i % 9);
   const bool on = !active || active[row_seg[node]];
   double s0 = 0.0, s1 = 0.0;
-  if (on)
-    for (int k = ptr[node]; k < ptr[node + 1]; ++k) {
-      const double2 v = *reinterpret_cast<const double2 *>(ycell + (size_t)idx[k] * 18 + c);
-      s0 += v.x;
-      s1 += v.y;
-    }
+  if (on) slot_pair_sum(ptr, idx, ycell, node, c, s0, s1);
   const uint32_t mk = mask[node];
   const size_t o = (size_t)node * 18 + c;
   rhs[o] = (on && !((mk >> c) & 1u)) ? -s0 : 0.0;
